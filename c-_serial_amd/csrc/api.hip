@@ -1,0 +1,738 @@
+// api.hip -- the C ABI of lskum.h: device context, host fpi_solver loop.
+//
+// The host loop keeps the reference's structure (fpi_solver_mod.h:14-150,
+// q_lskum_mod.h:28-37): each stage is one stream-ordered kernel launch on the
+// context's stream; only the per-iteration residual sums come back, once,
+// at the end of lskum_iterate.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "host.hpp"
+#include "kernels.hpp"
+#include "lskum.h"
+
+using namespace lskum;
+
+struct lskum_ctx {
+  int n = 0;
+  lskum_params prm{};
+  int block = 256;
+  hipStream_t stream = nullptr;
+  std::string err;
+  // caller <-> device order
+  std::vector<int32_t> perm, inv;  // perm[dev] = caller, inv[caller] = dev
+  // host copy of the caller-order cloud (for sub-stencil queries)
+  std::vector<int32_t> conn_off_h, conn_h;
+  std::vector<uint8_t> mask_h;
+  std::vector<double> nx_h, ny_h;
+  // device state
+  double *rec = nullptr, *alt = nullptr, *prim = nullptr, *prim_old = nullptr;
+  double *flux_res = nullptr, *delta = nullptr, *nx = nullptr, *ny = nullptr;
+  uint8_t *flag = nullptr;
+  int32_t *conn_off = nullptr, *perm_d = nullptr;
+  uint32_t *conn = nullptr;
+  Partials part{};
+  double *red = nullptr;      // [3] sum, max, argmax of the last state update
+  double *hist = nullptr;     // per-iteration sum_res_sqr of lskum_iterate
+  int hist_cap = 0;
+  double *red_h = nullptr;    // pinned host mirror of red / hist
+  int32_t *f_m = nullptr, *f_l = nullptr, *f_r = nullptr, *f_s = nullptr;
+  int n_shape = 0;
+  double *forces = nullptr;   // [3] H, V, pitch moment
+  // case
+  double theta = 0.0, q_inf[4] = {0, 0, 0, 0};
+  double rho_inf = 1.0, pr_inf = 1.0 / 1.4;
+  // iteration bookkeeping (q_lskum_mod.h, fpi_solver_mod.h:138-147)
+  int it = 0;
+  double res_old = 0.0;
+  int restart = 0;
+  int32_t max_res_point = 0;
+  // timing
+  bool timing = false;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  double k_ms[6] = {0, 0, 0, 0, 0, 0};
+  int k_cnt[6] = {0, 0, 0, 0, 0, 0};
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<std::pair<int, int>> ev_pending;  // (kernel id, event index)
+
+  DevCloud dc() const {
+    DevCloud c;
+    c.n = n;
+    c.conn_off = conn_off;
+    c.conn = conn;
+    c.nx = nx;
+    c.ny = ny;
+    c.flag = flag;
+    c.perm = perm_d;
+    return c;
+  }
+  FlowConsts fc() const {
+    FlowConsts f;
+    f.cfl = prm.cfl;
+    f.power = prm.power;
+    f.u1_inf = q_inf[1];
+    f.u2_inf = q_inf[2];
+    f.rho_inf = rho_inf;
+    f.pr_inf = pr_inf;
+    return f;
+  }
+};
+
+namespace {
+
+int set_err(lskum_ctx *c, int code, const std::string &m) {
+  if (c) c->err = m;
+  return code;
+}
+
+#define HIP_TRY(ctx, call)                                                       \
+  do {                                                                           \
+    hipError_t e_ = (call);                                                      \
+    if (e_ != hipSuccess)                                                        \
+      return set_err(ctx, LSKUM_EHIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <typename T>
+hipError_t dalloc(T **p, size_t count) {
+  return hipMalloc((void **)p, sizeof(T) * (count ? count : 1));
+}
+
+void release(lskum_ctx *c) {
+  if (!c) return;
+  void *ptrs[] = {c->rec, c->alt, c->prim, c->prim_old, c->flux_res, c->delta, c->nx,
+                  c->ny, c->flag, c->conn_off, c->perm_d, c->conn, c->part.sum,
+                  c->part.max, c->part.arg, c->red, c->hist, c->f_m, c->f_l, c->f_r,
+                  c->f_s, c->forces};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  if (c->red_h) (void)hipHostFree(c->red_h);
+  for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+}
+
+// Kernel timing: an event pair around a launch when enabled.
+struct KTimer {
+  lskum_ctx *c;
+  int id;
+  int idx = -1;
+  KTimer(lskum_ctx *ctx, int kid) : c(ctx), id(kid) {
+    if (!c->timing) return;
+    idx = (int)c->ev_pending.size() * 2;
+    while ((int)c->ev_pool.size() < idx + 2) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) { idx = -1; return; }
+      c->ev_pool.push_back(e);
+    }
+    (void)hipEventRecord(c->ev_pool[idx], c->stream);
+  }
+  ~KTimer() {
+    if (idx < 0) return;
+    (void)hipEventRecord(c->ev_pool[idx + 1], c->stream);
+    c->ev_pending.push_back({id, idx});
+  }
+};
+
+int collect_timing(lskum_ctx *c) {
+  if (c->ev_pending.empty()) return LSKUM_OK;
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  for (auto &p : c->ev_pending) {
+    float ms = 0.f;
+    HIP_TRY(c, hipEventElapsedTime(&ms, c->ev_pool[p.second], c->ev_pool[p.second + 1]));
+    c->k_ms[p.first] += ms;
+    c->k_cnt[p.first] += 1;
+  }
+  c->ev_pending.clear();
+  return LSKUM_OK;
+}
+
+int check_launch(lskum_ctx *c, const char *what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return set_err(c, LSKUM_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+  return LSKUM_OK;
+}
+
+// prim_old <- prim
+int do_snapshot(lskum_ctx *c) {
+  HIP_TRY(c, hipMemcpyAsync(c->prim_old, c->prim, sizeof(double) * 4 * c->n,
+                            hipMemcpyDeviceToDevice, c->stream));
+  return LSKUM_OK;
+}
+
+int do_func_delta(lskum_ctx *c) {
+  {
+    KTimer t(c, LSKUM_K_DELTA);
+    launch_func_delta(c->dc(), c->rec, c->prim, c->prm.cfl, c->delta, c->block, c->stream);
+  }
+  return check_launch(c, "func_delta");
+}
+
+int do_q_variables(lskum_ctx *c) {
+  {
+    KTimer t(c, LSKUM_K_QVAR);
+    launch_q_variables(c->dc(), c->prim, c->rec, c->block, c->stream);
+  }
+  return check_launch(c, "q_variables");
+}
+
+// derivatives then n_inner Jacobi passes, ping-ponging between the record
+// and the alternate buffer so that the last pass lands in the record
+int do_q_derivatives_and_inner(lskum_ctx *c, int n_inner, bool deriv) {
+  double *rec_dq = c->rec + R_DQX;
+  bool in_rec = (n_inner % 2) == 0;
+  if (deriv) {
+    KTimer t(c, LSKUM_K_QDERIV);
+    launch_q_derivatives(c->dc(), c->rec, in_rec ? rec_dq : c->alt, in_rec ? R_STRIDE : ALT_STRIDE,
+                         c->prm.power, c->block, c->stream);
+  } else if (!in_rec) {
+    // the current derivatives sit in the record: start the chain from there
+    in_rec = true;
+  }
+  int rc = check_launch(c, "q_derivatives");
+  if (rc) return rc;
+  for (int p = 0; p < n_inner; p++) {
+    KTimer t(c, LSKUM_K_QINNER);
+    if (in_rec)
+      launch_q_inner(c->dc(), c->rec, rec_dq, R_STRIDE, c->alt, ALT_STRIDE, c->prm.power,
+                     c->block, c->stream);
+    else
+      launch_q_inner(c->dc(), c->rec, c->alt, ALT_STRIDE, rec_dq, R_STRIDE, c->prm.power,
+                     c->block, c->stream);
+    in_rec = !in_rec;
+  }
+  if (!in_rec) {
+    // odd pass count from the record: copy back (only for lskum_q_inner calls)
+    HIP_TRY(c, hipMemcpy2DAsync(rec_dq, sizeof(double) * R_STRIDE, c->alt,
+                                sizeof(double) * ALT_STRIDE, sizeof(double) * 8, c->n,
+                                hipMemcpyDeviceToDevice, c->stream));
+  }
+  return check_launch(c, "q_inner");
+}
+
+int do_flux(lskum_ctx *c) {
+  {
+    KTimer t(c, LSKUM_K_FLUX);
+    launch_flux(c->dc(), c->rec, c->delta, c->prm.power, c->flux_res, c->block, c->stream);
+  }
+  return check_launch(c, "flux_residual");
+}
+
+int do_state_update(lskum_ctx *c, int rk, double *hist_slot) {
+  {
+    KTimer t(c, LSKUM_K_STATE);
+    launch_state_update(c->dc(), rk, c->prim, c->prim_old, c->flux_res, c->fc(), c->part,
+                        c->block, c->stream);
+    launch_reduce(c->part, c->red, hist_slot, c->stream);
+  }
+  return check_launch(c, "state_update");
+}
+
+void permute_in(const lskum_ctx *c, const double *src, double *dst, int ncomp) {
+  for (int r = 0; r < ncomp; r++)
+    for (int d = 0; d < c->n; d++) dst[(size_t)r * c->n + d] = src[(size_t)r * c->n + c->perm[d]];
+}
+
+void permute_out(const lskum_ctx *c, const double *src, double *dst, int ncomp) {
+  for (int r = 0; r < ncomp; r++)
+    for (int d = 0; d < c->n; d++) dst[(size_t)r * c->n + c->perm[d]] = src[(size_t)r * c->n + d];
+}
+
+// fetch ncomp strided slots of the record: out[r][dev]
+int fetch_record(lskum_ctx *c, int slot, int ncomp, std::vector<double> &out) {
+  out.assign((size_t)ncomp * c->n, 0.0);
+  std::vector<double> tmp((size_t)ncomp * c->n);
+  HIP_TRY(c, hipMemcpy2DAsync(tmp.data(), sizeof(double) * ncomp, c->rec + slot,
+                              sizeof(double) * R_STRIDE, sizeof(double) * ncomp, c->n,
+                              hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  for (int d = 0; d < c->n; d++)
+    for (int r = 0; r < ncomp; r++) out[(size_t)r * c->n + d] = tmp[(size_t)d * ncomp + r];
+  return LSKUM_OK;
+}
+
+int fetch_plain(lskum_ctx *c, const double *dev, int ncomp, std::vector<double> &out) {
+  out.resize((size_t)ncomp * c->n);
+  HIP_TRY(c, hipMemcpyAsync(out.data(), dev, sizeof(double) * ncomp * c->n,
+                            hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return LSKUM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *lskum_last_error(const lskum_ctx *ctx) {
+  return ctx ? ctx->err.c_str() : "null context";
+}
+
+int lskum_create(const lskum_cloud *cloud, const lskum_params *params, lskum_ctx **out) {
+  if (!out) return LSKUM_EINVAL;
+  *out = nullptr;
+  if (!cloud || !params) return LSKUM_EINVAL;
+  lskum_ctx *c = new (std::nothrow) lskum_ctx;
+  if (!c) return LSKUM_ENOMEM;
+  std::string err;
+  int rc = check_cloud(*cloud, &err);
+  const lskum_params &p = *params;
+  if (rc == LSKUM_OK && !(p.mach > 0.0 && p.cfl > 0.0 && p.vl_const >= 0.0 &&
+                          p.inner_iterations >= 0 &&
+                          (p.precision == LSKUM_FP64 || p.precision == LSKUM_MIXED) &&
+                          (p.reorder == LSKUM_REORDER_NONE || p.reorder == LSKUM_REORDER_RCM) &&
+                          (p.block == 0 || p.block == 64 || p.block == 128 || p.block == 256))) {
+    rc = LSKUM_EINVAL;
+    err = "invalid parameters";
+  }
+  if (rc == LSKUM_OK && p.precision == LSKUM_MIXED) {
+    rc = LSKUM_EINVAL;
+    err = "mixed precision is not built into this library";
+  }
+  if (rc != LSKUM_OK) {
+    // keep the message reachable: return the context in its failed state
+    c->err = err;
+    *out = c;
+    return rc;
+  }
+  c->prm = p;
+  c->block = p.block ? p.block : 256;
+  const int32_t n = cloud->n_points;
+  c->n = n;
+  // set-up on the host: normals and sub-stencils in the caller's order
+  compute_normals(*cloud, c->nx_h, c->ny_h);
+  generate_connectivity(*cloud, c->nx_h, c->ny_h, c->mask_h);
+  for (int32_t i = 0; i < n; i++) {
+    double r = substencil_min_rel_det(*cloud, c->nx_h, c->ny_h, c->mask_h, i);
+    if (!(r > 0.0)) {
+      char m[160];
+      std::snprintf(m, sizeof m,
+                    "degenerate sub-stencil at point %d (flag_1 %d): empty or collinear", i,
+                    cloud->flag_1[i]);
+      c->err = m;
+      *out = c;
+      return LSKUM_ESTENCIL;
+    }
+  }
+  c->conn_off_h.assign(cloud->conn_off, cloud->conn_off + n + 1);
+  c->conn_h.assign(cloud->conn, cloud->conn + cloud->conn_off[n]);
+  // device order: optional RCM, then grouped by point type (wall, interior,
+  // outer) so that waves rarely mix flux variants
+  std::vector<int32_t> base;
+  if (p.reorder == LSKUM_REORDER_RCM) {
+    rcm_order(*cloud, base);
+  } else {
+    base.resize(n);
+    for (int32_t i = 0; i < n; i++) base[i] = i;
+  }
+  c->perm.clear();
+  c->perm.reserve(n);
+  for (int t = 0; t < 3; t++)
+    for (int32_t v : base)
+      if (cloud->flag_1[v] == t) c->perm.push_back(v);
+  c->inv.assign(n, 0);
+  for (int32_t d = 0; d < n; d++) c->inv[c->perm[d]] = d;
+  // host images of the device arrays
+  std::vector<double> rec_h((size_t)R_STRIDE * n, 0.0), nxd(n), nyd(n);
+  std::vector<uint8_t> flag_h(n);
+  std::vector<int32_t> off_h(n + 1, 0);
+  std::vector<uint32_t> conn_d(cloud->conn_off[n]);
+  for (int32_t d = 0; d < n; d++) {
+    const int32_t v = c->perm[d];
+    rec_h[(size_t)R_STRIDE * d + R_X] = cloud->x[v];
+    rec_h[(size_t)R_STRIDE * d + R_Y] = cloud->y[v];
+    // limiters_mod.h:82-83: epsi = pow(VL_CONST*min_dist, 3), used squared
+    const double epsi = std::pow(p.vl_const * cloud->min_dist[v], 3);
+    rec_h[(size_t)R_STRIDE * d + R_EPS2] = epsi * epsi;
+    nxd[d] = c->nx_h[v];
+    nyd[d] = c->ny_h[v];
+    flag_h[d] = (uint8_t)cloud->flag_1[v];
+    const int32_t b = cloud->conn_off[v], e = cloud->conn_off[v + 1];
+    off_h[d + 1] = off_h[d] + (e - b);
+    for (int32_t t = b; t < e; t++)
+      conn_d[off_h[d] + (t - b)] =
+          (uint32_t)c->inv[cloud->conn[t]] | ((uint32_t)c->mask_h[t] << kMaskShift);
+  }
+  // shape points for the force reduction: the first shape_points points
+  // (compute_force_coeffs_mod.h:25-27 takes m = j)
+  int shape_points = 0;
+  for (int32_t i = 0; i < n; i++)
+    if (cloud->flag_2[i] > 0) shape_points++;
+  std::vector<int32_t> fm, fl, fr, fs;
+  for (int j = 0; j < shape_points; j++) {
+    if (cloud->flag_1[j] == 1) continue;  // no left/right: cannot contribute a panel
+    fm.push_back(c->inv[j]);
+    fl.push_back(c->inv[cloud->left[j]]);
+    fr.push_back(c->inv[cloud->right[j]]);
+    fs.push_back(cloud->flag_2[j]);
+  }
+  c->n_shape = (int)fm.size();
+  // device allocation and upload
+  auto fail = [&](int code) {
+    *out = c;
+    return code;
+  };
+  hipError_t e;
+#define ALLOC(ptr, count)                                                   \
+  if ((e = dalloc(&(ptr), (count))) != hipSuccess) {                        \
+    c->err = std::string("hipMalloc " #ptr ": ") + hipGetErrorString(e);   \
+    return fail(LSKUM_ENOMEM);                                              \
+  }
+  if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess) {
+    c->err = std::string("hipStreamCreate: ") + hipGetErrorString(e);
+    return fail(LSKUM_EHIP);
+  }
+  c->part.nblocks = blocks_for(n, 256);
+  ALLOC(c->rec, (size_t)R_STRIDE * n);
+  ALLOC(c->alt, (size_t)ALT_STRIDE * n);
+  ALLOC(c->prim, (size_t)4 * n);
+  ALLOC(c->prim_old, (size_t)4 * n);
+  ALLOC(c->flux_res, (size_t)4 * n);
+  ALLOC(c->delta, (size_t)n);
+  ALLOC(c->nx, (size_t)n);
+  ALLOC(c->ny, (size_t)n);
+  ALLOC(c->flag, (size_t)n);
+  ALLOC(c->conn_off, (size_t)n + 1);
+  ALLOC(c->perm_d, (size_t)n);
+  ALLOC(c->conn, conn_d.size());
+  ALLOC(c->part.sum, (size_t)c->part.nblocks);
+  ALLOC(c->part.max, (size_t)c->part.nblocks);
+  ALLOC(c->part.arg, (size_t)c->part.nblocks);
+  ALLOC(c->red, 3);
+  ALLOC(c->forces, 3);
+  ALLOC(c->f_m, fm.size());
+  ALLOC(c->f_l, fm.size());
+  ALLOC(c->f_r, fm.size());
+  ALLOC(c->f_s, fm.size());
+#undef ALLOC
+  if ((e = hipHostMalloc((void **)&c->red_h, sizeof(double) * 8)) != hipSuccess) {
+    c->err = std::string("hipHostMalloc: ") + hipGetErrorString(e);
+    return fail(LSKUM_ENOMEM);
+  }
+  struct Up { void *dst; const void *src; size_t bytes; };
+  Up ups[] = {{c->rec, rec_h.data(), rec_h.size() * sizeof(double)},
+              {c->nx, nxd.data(), n * sizeof(double)},
+              {c->ny, nyd.data(), n * sizeof(double)},
+              {c->flag, flag_h.data(), (size_t)n},
+              {c->conn_off, off_h.data(), (n + 1) * sizeof(int32_t)},
+              {c->perm_d, c->perm.data(), n * sizeof(int32_t)},
+              {c->conn, conn_d.data(), conn_d.size() * sizeof(uint32_t)},
+              {c->f_m, fm.data(), fm.size() * sizeof(int32_t)},
+              {c->f_l, fl.data(), fl.size() * sizeof(int32_t)},
+              {c->f_r, fr.data(), fr.size() * sizeof(int32_t)},
+              {c->f_s, fs.data(), fs.size() * sizeof(int32_t)}};
+  for (const Up &u : ups) {
+    if (!u.bytes) continue;
+    if ((e = hipMemcpy(u.dst, u.src, u.bytes, hipMemcpyHostToDevice)) != hipSuccess) {
+      c->err = std::string("hipMemcpy: ") + hipGetErrorString(e);
+      return fail(LSKUM_EHIP);
+    }
+  }
+  if ((e = hipMemset(c->alt, 0, sizeof(double) * ALT_STRIDE * n)) != hipSuccess ||
+      (e = hipMemset(c->flux_res, 0, sizeof(double) * 4 * n)) != hipSuccess ||
+      (e = hipMemset(c->delta, 0, sizeof(double) * n)) != hipSuccess) {
+    c->err = std::string("hipMemset: ") + hipGetErrorString(e);
+    return fail(LSKUM_EHIP);
+  }
+  // initial_conditions_mod.h:23-37 with parameter_mod.h:13-30
+  const double pi = 4 * std::atan(1.0);
+  c->theta = p.aoa_deg * pi / 180;
+  c->q_inf[0] = c->rho_inf;
+  c->q_inf[1] = p.mach * std::cos(c->theta);
+  c->q_inf[2] = p.mach * std::sin(c->theta);
+  c->q_inf[3] = c->pr_inf;
+  std::vector<double> prim0((size_t)4 * n);
+  for (int r = 0; r < 4; r++)
+    for (int32_t i = 0; i < n; i++) prim0[(size_t)r * n + i] = c->q_inf[r];
+  if ((e = hipMemcpy(c->prim, prim0.data(), prim0.size() * sizeof(double),
+                     hipMemcpyHostToDevice)) != hipSuccess ||
+      (e = hipMemcpy(c->prim_old, prim0.data(), prim0.size() * sizeof(double),
+                     hipMemcpyHostToDevice)) != hipSuccess) {
+    c->err = std::string("hipMemcpy prim: ") + hipGetErrorString(e);
+    return fail(LSKUM_EHIP);
+  }
+  *out = c;
+  return LSKUM_OK;
+}
+
+void lskum_destroy(lskum_ctx *ctx) {
+  if (!ctx) return;
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  release(ctx);
+  delete ctx;
+}
+
+#define CTX_OK(ctx)                                        \
+  do {                                                     \
+    if (!(ctx)) return LSKUM_EINVAL;                       \
+    if (!(ctx)->stream) return set_err(ctx, LSKUM_EINVAL, "context failed to initialise"); \
+  } while (0)
+
+int lskum_set_state(lskum_ctx *ctx, const double *prim) {
+  CTX_OK(ctx);
+  if (!prim) return set_err(ctx, LSKUM_EINVAL, "null prim");
+  std::vector<double> tmp((size_t)4 * ctx->n);
+  permute_in(ctx, prim, tmp.data(), 4);
+  HIP_TRY(ctx, hipMemcpyAsync(ctx->prim, tmp.data(), tmp.size() * sizeof(double),
+                              hipMemcpyHostToDevice, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return LSKUM_OK;
+}
+
+int lskum_get_state(lskum_ctx *ctx, double *prim) {
+  return lskum_get_array(ctx, LSKUM_A_PRIM, prim);
+}
+
+int lskum_snapshot(lskum_ctx *ctx) {
+  CTX_OK(ctx);
+  return do_snapshot(ctx);
+}
+
+int lskum_func_delta(lskum_ctx *ctx) {
+  CTX_OK(ctx);
+  return do_func_delta(ctx);
+}
+
+int lskum_q_variables(lskum_ctx *ctx) {
+  CTX_OK(ctx);
+  return do_q_variables(ctx);
+}
+
+int lskum_q_derivatives(lskum_ctx *ctx) {
+  CTX_OK(ctx);
+  return do_q_derivatives_and_inner(ctx, 0, true);
+}
+
+int lskum_q_inner(lskum_ctx *ctx, int32_t n_inner) {
+  CTX_OK(ctx);
+  if (n_inner < 0) return set_err(ctx, LSKUM_EINVAL, "n_inner < 0");
+  return do_q_derivatives_and_inner(ctx, n_inner, false);
+}
+
+int lskum_flux_residual(lskum_ctx *ctx) {
+  CTX_OK(ctx);
+  return do_flux(ctx);
+}
+
+int lskum_state_update(lskum_ctx *ctx, int32_t rk, double *sum_res_sqr, double *max_res,
+                       int32_t *max_res_point) {
+  CTX_OK(ctx);
+  if (rk < 1 || rk > 4) return set_err(ctx, LSKUM_EINVAL, "rk outside 1..4");
+  int rc = do_state_update(ctx, rk, nullptr);
+  if (rc) return rc;
+  if (!sum_res_sqr && !max_res && !max_res_point) return LSKUM_OK;
+  HIP_TRY(ctx, hipMemcpyAsync(ctx->red_h, ctx->red, sizeof(double) * 3, hipMemcpyDeviceToHost,
+                              ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  // state_update_mod.h:106-107,159-163: max_res starts at 0 and the point
+  // only moves on a strictly larger value
+  const double mx = ctx->red_h[1] > 0.0 ? ctx->red_h[1] : 0.0;
+  if (ctx->red_h[1] > 0.0) ctx->max_res_point = (int32_t)ctx->red_h[2];
+  if (sum_res_sqr) *sum_res_sqr = ctx->red_h[0];
+  if (max_res) *max_res = mx;
+  if (max_res_point) *max_res_point = ctx->max_res_point;
+  if (!std::isfinite(ctx->red_h[0]))
+    return set_err(ctx, LSKUM_ENONFINITE, "residual sum is not finite");
+  return LSKUM_OK;
+}
+
+int lskum_forces(lskum_ctx *ctx, double *cl, double *cd, double *cm) {
+  CTX_OK(ctx);
+  const double qdyn = 0.5 * ctx->rho_inf * ctx->prm.mach * ctx->prm.mach;
+  launch_forces(ctx->n_shape, ctx->f_m, ctx->f_l, ctx->f_r, ctx->f_s, ctx->rec, ctx->nx, ctx->ny,
+                ctx->prim, ctx->n, ctx->pr_inf, qdyn, ctx->forces, ctx->stream);
+  int rc = check_launch(ctx, "forces");
+  if (rc) return rc;
+  HIP_TRY(ctx, hipMemcpyAsync(ctx->red_h + 3, ctx->forces, sizeof(double) * 3,
+                              hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  const double H = ctx->red_h[3], V = ctx->red_h[4], P = ctx->red_h[5];
+  // compute_force_coeffs_mod.h:62-67
+  const double l = V * std::cos(ctx->theta) - H * std::sin(ctx->theta);
+  const double d = H * std::cos(ctx->theta) + V * std::sin(ctx->theta);
+  if (cl) *cl = l;
+  if (cd) *cd = d;
+  if (cm) *cm = P;
+  if (!std::isfinite(l) || !std::isfinite(d) || !std::isfinite(P))
+    return set_err(ctx, LSKUM_ENONFINITE, "force coefficients are not finite");
+  return LSKUM_OK;
+}
+
+int lskum_iterate(lskum_ctx *ctx, int32_t n_iter, double *res_new, double *residue) {
+  CTX_OK(ctx);
+  if (n_iter < 0) return set_err(ctx, LSKUM_EINVAL, "n_iter < 0");
+  if (n_iter == 0) return LSKUM_OK;
+  if (ctx->hist_cap < n_iter) {
+    if (ctx->hist) (void)hipFree(ctx->hist);
+    ctx->hist = nullptr;
+    ctx->hist_cap = 0;
+    HIP_TRY(ctx, dalloc(&ctx->hist, (size_t)n_iter));
+    ctx->hist_cap = n_iter;
+  }
+  const int inner = ctx->prm.inner_iterations;
+  for (int t = 0; t < n_iter; t++) {
+    int rc = do_snapshot(ctx);
+    if (!rc) rc = do_func_delta(ctx);
+    for (int rk = 1; rk <= 4 && !rc; rk++) {
+      rc = do_q_variables(ctx);
+      if (!rc) rc = do_q_derivatives_and_inner(ctx, inner, true);
+      if (!rc) rc = do_flux(ctx);
+      if (!rc) rc = do_state_update(ctx, rk, rk == 4 ? ctx->hist + t : nullptr);
+    }
+    if (rc) return rc;
+  }
+  std::vector<double> sums(n_iter);
+  HIP_TRY(ctx, hipMemcpyAsync(sums.data(), ctx->hist, sizeof(double) * n_iter,
+                              hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  int rc = collect_timing(ctx);
+  if (rc) return rc;
+  bool finite = true;
+  for (int t = 0; t < n_iter; t++) {
+    // fpi_solver_mod.h:138-147
+    const int it = ctx->it + 1;
+    const double rn = std::sqrt(sums[t]) / ctx->n;
+    double rs;
+    if (it <= 2 && ctx->restart == 0) {
+      ctx->res_old = rn;
+      rs = 0;
+    } else {
+      rs = std::log10(rn / ctx->res_old);
+    }
+    ctx->it = it;
+    if (res_new) res_new[t] = rn;
+    if (residue) residue[t] = rs;
+    if (!std::isfinite(sums[t])) finite = false;
+  }
+  if (!finite) return set_err(ctx, LSKUM_ENONFINITE, "residual became non-finite");
+  return LSKUM_OK;
+}
+
+int lskum_get_iteration(lskum_ctx *ctx, int32_t *it, double *res_old) {
+  CTX_OK(ctx);
+  if (it) *it = ctx->it;
+  if (res_old) *res_old = ctx->res_old;
+  return LSKUM_OK;
+}
+
+int lskum_set_iteration(lskum_ctx *ctx, int32_t it, double res_old, int32_t restart) {
+  CTX_OK(ctx);
+  if (it < 0) return set_err(ctx, LSKUM_EINVAL, "it < 0");
+  ctx->it = it;
+  ctx->res_old = res_old;
+  ctx->restart = restart ? 1 : 0;
+  return LSKUM_OK;
+}
+
+int lskum_get_array(lskum_ctx *ctx, int32_t which, double *dst) {
+  CTX_OK(ctx);
+  if (!dst) return set_err(ctx, LSKUM_EINVAL, "null destination");
+  std::vector<double> dev;
+  int rc = LSKUM_OK, ncomp = 0;
+  switch (which) {
+    case LSKUM_A_PRIM: rc = fetch_plain(ctx, ctx->prim, 4, dev); ncomp = 4; break;
+    case LSKUM_A_PRIM_OLD: rc = fetch_plain(ctx, ctx->prim_old, 4, dev); ncomp = 4; break;
+    case LSKUM_A_Q: rc = fetch_record(ctx, R_Q, 4, dev); ncomp = 4; break;
+    case LSKUM_A_DQ: rc = fetch_record(ctx, R_DQX, 8, dev); ncomp = 8; break;
+    case LSKUM_A_QM: rc = fetch_record(ctx, R_QMAX, 8, dev); ncomp = 8; break;
+    case LSKUM_A_DELTA: rc = fetch_plain(ctx, ctx->delta, 1, dev); ncomp = 1; break;
+    case LSKUM_A_FLUX_RES: rc = fetch_plain(ctx, ctx->flux_res, 4, dev); ncomp = 4; break;
+    case LSKUM_A_NORMALS:
+      std::memcpy(dst, ctx->nx_h.data(), sizeof(double) * ctx->n);
+      std::memcpy(dst + ctx->n, ctx->ny_h.data(), sizeof(double) * ctx->n);
+      return LSKUM_OK;
+    default: return set_err(ctx, LSKUM_EINVAL, "unknown array id");
+  }
+  if (rc) return rc;
+  permute_out(ctx, dev.data(), dst, ncomp);
+  return LSKUM_OK;
+}
+
+int lskum_get_substencil(lskum_ctx *ctx, int32_t which, int32_t *counts, int32_t *members) {
+  CTX_OK(ctx);
+  if (which < 0 || which > 3 || !counts) return set_err(ctx, LSKUM_EINVAL, "bad arguments");
+  const uint8_t bit = (uint8_t)(1u << which);
+  int64_t w = 0;
+  for (int32_t i = 0; i < ctx->n; i++) {
+    int32_t cnt = 0;
+    for (int32_t e = ctx->conn_off_h[i]; e < ctx->conn_off_h[i + 1]; e++) {
+      if (!(ctx->mask_h[e] & bit)) continue;
+      if (members) members[w] = ctx->conn_h[e];
+      w++;
+      cnt++;
+    }
+    counts[i] = cnt;
+  }
+  return LSKUM_OK;
+}
+
+void *lskum_stream(lskum_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+int lskum_set_timing(lskum_ctx *ctx, int32_t enable) {
+  CTX_OK(ctx);
+  int rc = collect_timing(ctx);
+  ctx->timing = enable != 0;
+  for (int k = 0; k < 6; k++) {
+    ctx->k_ms[k] = 0.0;
+    ctx->k_cnt[k] = 0;
+  }
+  return rc;
+}
+
+int lskum_kernel_time(lskum_ctx *ctx, int32_t kernel, double *avg_ms, int32_t *launches) {
+  CTX_OK(ctx);
+  if (kernel < 0 || kernel > 5) return set_err(ctx, LSKUM_EINVAL, "bad kernel id");
+  int rc = collect_timing(ctx);
+  if (rc) return rc;
+  if (avg_ms) *avg_ms = ctx->k_cnt[kernel] ? ctx->k_ms[kernel] / ctx->k_cnt[kernel] : 0.0;
+  if (launches) *launches = ctx->k_cnt[kernel];
+  return LSKUM_OK;
+}
+
+int lskum_kat_fluxes(int32_t n, const double *in6, double *out, int32_t mixed) {
+  if (n <= 0 || !in6 || !out || mixed) return LSKUM_EINVAL;
+  double *din = nullptr, *dout = nullptr;
+  if (dalloc(&din, (size_t)6 * n) != hipSuccess) return LSKUM_ENOMEM;
+  if (dalloc(&dout, (size_t)32 * n) != hipSuccess) {
+    (void)hipFree(din);
+    return LSKUM_ENOMEM;
+  }
+  int rc = LSKUM_OK;
+  if (hipMemcpy(din, in6, sizeof(double) * 6 * n, hipMemcpyHostToDevice) != hipSuccess) rc = LSKUM_EHIP;
+  if (!rc) {
+    launch_kat_fluxes(n, din, dout, nullptr);
+    if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) rc = LSKUM_EHIP;
+  }
+  if (!rc && hipMemcpy(out, dout, sizeof(double) * 32 * n, hipMemcpyDeviceToHost) != hipSuccess)
+    rc = LSKUM_EHIP;
+  (void)hipFree(din);
+  (void)hipFree(dout);
+  return rc;
+}
+
+int lskum_kat_qtilde(int32_t n, const double *qt4, double *prim_out) {
+  if (n <= 0 || !qt4 || !prim_out) return LSKUM_EINVAL;
+  double *din = nullptr, *dout = nullptr;
+  if (dalloc(&din, (size_t)4 * n) != hipSuccess) return LSKUM_ENOMEM;
+  if (dalloc(&dout, (size_t)4 * n) != hipSuccess) {
+    (void)hipFree(din);
+    return LSKUM_ENOMEM;
+  }
+  int rc = LSKUM_OK;
+  if (hipMemcpy(din, qt4, sizeof(double) * 4 * n, hipMemcpyHostToDevice) != hipSuccess) rc = LSKUM_EHIP;
+  if (!rc) {
+    launch_kat_qtilde(n, din, dout, nullptr);
+    if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) rc = LSKUM_EHIP;
+  }
+  if (!rc && hipMemcpy(prim_out, dout, sizeof(double) * 4 * n, hipMemcpyDeviceToHost) != hipSuccess)
+    rc = LSKUM_EHIP;
+  (void)hipFree(din);
+  (void)hipFree(dout);
+  return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,47 @@
+// host.hpp -- host-side set-up shared by the cloud tools and the context.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "lskum.h"
+
+struct lskum_cloud_h {
+  int32_t n = 0;
+  std::vector<double> x, y, min_dist;
+  std::vector<int32_t> left, right, flag1, flag2, conn_off, conn;
+  lskum_cloud view() const;
+};
+
+namespace lskum {
+
+// Sub-stencil membership bits of one neighbour, as generate_connectivity
+// assigns them (generate_connectivity_mod.h:37-71, :122-143, :188-209).
+enum : uint32_t { XPOS = 1u, XNEG = 2u, YPOS = 4u, YNEG = 8u };
+
+// Validated copy of a caller's cloud.
+int check_cloud(const lskum_cloud &c, std::string *err);
+
+// point_normals_mod.h:17-55
+void compute_normals(const lskum_cloud &c, std::vector<double> &nx,
+                     std::vector<double> &ny);
+
+// generate_connectivity_mod.h:221-249: one mask per conn entry.
+void generate_connectivity(const lskum_cloud &c, const std::vector<double> &nx,
+                           const std::vector<double> &ny,
+                           std::vector<uint8_t> &mask);
+
+// Smallest relative determinant det/(sxx*syy) of the sub-stencil least-squares
+// matrices (the det of interior_fluxes_mod.h:104) over the point's
+// sub-stencils; 0 for an empty or collinear sub-stencil.
+double substencil_min_rel_det(const lskum_cloud &c,
+                              const std::vector<double> &nx,
+                              const std::vector<double> &ny,
+                              const std::vector<uint8_t> &mask, int32_t i);
+
+// Reverse Cuthill-McKee order of the symmetrised neighbour graph.
+// perm[new] = old.
+void rcm_order(const lskum_cloud &c, std::vector<int32_t> &perm);
+
+}  // namespace lskum

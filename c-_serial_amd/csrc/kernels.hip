@@ -1,0 +1,695 @@
+// kernels.hip -- q-LSKUM stage kernels for gfx950 (FP64 form).
+//
+// One thread per point.  Stencil kernels walk the point's CSR neighbour list
+// in the reference's conn order and gather each neighbour's packed 192-byte
+// record (kernels.hpp), so every per-point sum is accumulated in the same
+// order as the reference's serial loops.  Built with -ffp-contract=off.
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+#include "physics.hpp"
+
+namespace lskum {
+
+namespace {
+
+constexpr uint32_t XPOS = 1u, XNEG = 2u, YPOS = 4u, YNEG = 8u;
+
+// blockIdx -> block of points such that each of the 8 XCDs (which receive
+// workgroups round-robin) works on one contiguous range of points and keeps
+// its neighbours' records in its own L2.
+__device__ __forceinline__ int xcd_block() {
+  const int nb = gridDim.x, b = blockIdx.x;
+  const int q = nb >> 3, r = nb & 7, xcd = b & 7, local = b >> 3;
+  return xcd < r ? xcd * (q + 1) + local : r * (q + 1) + (xcd - r) * q + local;
+}
+
+__device__ __forceinline__ int point_index() {
+  return xcd_block() * blockDim.x + threadIdx.x;
+}
+
+__device__ __forceinline__ void load4(const double *p, double v[4]) {
+  const double2 a = reinterpret_cast<const double2 *>(p)[0];
+  const double2 b = reinterpret_cast<const double2 *>(p)[1];
+  v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
+}
+
+__device__ __forceinline__ void store4(double *p, const double v[4]) {
+  reinterpret_cast<double2 *>(p)[0] = make_double2(v[0], v[1]);
+  reinterpret_cast<double2 *>(p)[1] = make_double2(v[2], v[3]);
+}
+
+// ------------------------------------------------------------ q variables
+// q_variables_mod.h:9-34
+__global__ __launch_bounds__(256) void k_q_variables(int n, const double *__restrict__ prim,
+                                                     double *__restrict__ rec) {
+  const int k = point_index();
+  if (k >= n) return;
+  const double rho = prim[k], u1 = prim[n + k], u2 = prim[2 * n + k], pr = prim[3 * n + k];
+  const double beta = 0.5 * rho / pr;
+  double q[4];
+  q[0] = log(rho) + (log(beta) * 2.5) - beta * (u1 * u1 + u2 * u2);
+  const double two_times_beta = 2.0 * beta;
+  q[1] = two_times_beta * u1;
+  q[2] = two_times_beta * u2;
+  q[3] = -two_times_beta;
+  store4(rec + (size_t)R_STRIDE * k + R_Q, q);
+}
+
+// ------------------------------------------------------------ q derivatives
+// q_variables_mod.h:36-118: unweighted (power = 0) or pow(dist, power)
+// weighted least squares over the full stencil, and qmax/qmin over the
+// point and its neighbours.
+template <bool POW>
+__global__ __launch_bounds__(256) void k_q_derivatives(int n, const int32_t *__restrict__ off,
+                                                       const uint32_t *__restrict__ conn,
+                                                       double *__restrict__ rec,
+                                                       double *__restrict__ dq_dst,
+                                                       int dq_stride, double power) {
+  const int i = point_index();
+  if (i >= n) return;
+  const double *ri = rec + (size_t)R_STRIDE * i;
+  const double2 xy = reinterpret_cast<const double2 *>(ri)[0];
+  const double x_i = xy.x, y_i = xy.y;
+  double qi[4], qmax[4], qmin[4];
+  load4(ri + R_Q, qi);
+  double sum_delx_sqr = 0., sum_dely_sqr = 0., sum_delx_dely = 0.;
+  double sdx[4] = {0.0, 0.0, 0.0, 0.0}, sdy[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int r = 0; r < 4; r++) qmax[r] = qmin[r] = qi[r];
+  const int e1 = off[i + 1];
+  for (int e = off[i]; e < e1; e++) {
+    const int k = (int)(conn[e] & kIndexMask);
+    const double *rk = rec + (size_t)R_STRIDE * k;
+    const double2 xyk = reinterpret_cast<const double2 *>(rk)[0];
+    double qk[4];
+    load4(rk + R_Q, qk);
+    for (int r = 0; r < 4; r++) {
+      if (qk[r] > qmax[r]) qmax[r] = qk[r];
+      if (qk[r] < qmin[r]) qmin[r] = qk[r];
+    }
+    const double delx = xyk.x - x_i, dely = xyk.y - y_i;
+    if (POW) {
+      const double weights = pow(sqrt(delx * delx + dely * dely), power);
+      sum_delx_sqr = sum_delx_sqr + delx * delx * weights;
+      sum_dely_sqr = sum_dely_sqr + dely * dely * weights;
+      sum_delx_dely = sum_delx_dely + delx * dely * weights;
+      for (int r = 0; r < 4; r++) {
+        sdx[r] = sdx[r] + weights * delx * (qk[r] - qi[r]);
+        sdy[r] = sdy[r] + weights * dely * (qk[r] - qi[r]);
+      }
+    } else {  // weights == 1.0 exactly: x*1.0 == x
+      sum_delx_sqr = sum_delx_sqr + delx * delx;
+      sum_dely_sqr = sum_dely_sqr + dely * dely;
+      sum_delx_dely = sum_delx_dely + delx * dely;
+      for (int r = 0; r < 4; r++) {
+        sdx[r] = sdx[r] + delx * (qk[r] - qi[r]);
+        sdy[r] = sdy[r] + dely * (qk[r] - qi[r]);
+      }
+    }
+  }
+  const double det = sum_delx_sqr * sum_dely_sqr - sum_delx_dely * sum_delx_dely;
+  const double one_by_det = 1.0 / det;
+  double dqx[4], dqy[4];
+  for (int r = 0; r < 4; r++) {
+    dqx[r] = (sdx[r] * sum_dely_sqr - sdy[r] * sum_delx_dely) * one_by_det;
+    dqy[r] = (sdy[r] * sum_delx_sqr - sdx[r] * sum_delx_dely) * one_by_det;
+  }
+  double *wi = rec + (size_t)R_STRIDE * i;
+  store4(wi + R_QMAX, qmax);
+  store4(wi + R_QMIN, qmin);
+  double *d = dq_dst + (size_t)dq_stride * i;
+  store4(d, dqx);
+  store4(d + 4, dqy);
+}
+
+// ------------------------------------------------------------ inner passes
+// q_variables_mod.h:120-228: one defect-corrected LS pass, Jacobi (reads
+// dq_src, writes dq_dst; the reference's temp array + copy).
+template <bool POW>
+__global__ __launch_bounds__(256) void k_q_inner(int n, const int32_t *__restrict__ off,
+                                                 const uint32_t *__restrict__ conn,
+                                                 const double *__restrict__ rec,
+                                                 const double *__restrict__ dq_src, int src_stride,
+                                                 double *__restrict__ dq_dst, int dst_stride,
+                                                 double power) {
+  const int i = point_index();
+  if (i >= n) return;
+  const double *ri = rec + (size_t)R_STRIDE * i;
+  const double2 xy = reinterpret_cast<const double2 *>(ri)[0];
+  const double x_i = xy.x, y_i = xy.y;
+  double qi[4], dxi[4], dyi[4];
+  load4(ri + R_Q, qi);
+  load4(dq_src + (size_t)src_stride * i, dxi);
+  load4(dq_src + (size_t)src_stride * i + 4, dyi);
+  double sum_delx_sqr = 0., sum_dely_sqr = 0., sum_delx_dely = 0.;
+  double sx[4] = {0.0, 0.0, 0.0, 0.0}, sy[4] = {0.0, 0.0, 0.0, 0.0};
+  const int e1 = off[i + 1];
+  for (int e = off[i]; e < e1; e++) {
+    const int k = (int)(conn[e] & kIndexMask);
+    const double *rk = rec + (size_t)R_STRIDE * k;
+    const double2 xyk = reinterpret_cast<const double2 *>(rk)[0];
+    double qk[4], dxk[4], dyk[4];
+    load4(rk + R_Q, qk);
+    load4(dq_src + (size_t)src_stride * k, dxk);
+    load4(dq_src + (size_t)src_stride * k + 4, dyk);
+    const double delx = xyk.x - x_i, dely = xyk.y - y_i;
+    const double weights = POW ? pow(sqrt(delx * delx + dely * dely), power) : 1.0;
+    if (POW) {
+      sum_delx_sqr = sum_delx_sqr + delx * delx * weights;
+      sum_dely_sqr = sum_dely_sqr + dely * dely * weights;
+      sum_delx_dely = sum_delx_dely + delx * dely * weights;
+    } else {
+      sum_delx_sqr = sum_delx_sqr + delx * delx;
+      sum_dely_sqr = sum_dely_sqr + dely * dely;
+      sum_delx_dely = sum_delx_dely + delx * dely;
+    }
+    for (int r = 0; r < 4; r++) {
+      const double temp1 = qi[r] - 0.5 * (delx * dxi[r] + dely * dyi[r]);
+      const double temp2 = qk[r] - 0.5 * (delx * dxk[r] + dely * dyk[r]);
+      if (POW) {
+        sx[r] = sx[r] + (weights * delx * (temp2 - temp1));
+        sy[r] = sy[r] + (weights * dely * (temp2 - temp1));
+      } else {
+        sx[r] = sx[r] + (delx * (temp2 - temp1));
+        sy[r] = sy[r] + (dely * (temp2 - temp1));
+      }
+    }
+  }
+  const double det = sum_delx_sqr * sum_dely_sqr - sum_delx_dely * sum_delx_dely;
+  const double one_by_det = 1.0 / det;
+  double ox[4], oy[4];
+  for (int r = 0; r < 4; r++) {
+    ox[r] = one_by_det * (sx[r] * sum_dely_sqr - sy[r] * sum_delx_dely);
+    oy[r] = one_by_det * (sy[r] * sum_delx_sqr - sx[r] * sum_delx_dely);
+  }
+  double *d = dq_dst + (size_t)dst_stride * i;
+  store4(d, ox);
+  store4(d + 4, oy);
+}
+
+// ------------------------------------------------------------ flux residual
+struct Acc {
+  double sxx, syy, sxy, fx[4], fy[4];
+};
+
+__device__ __forceinline__ void acc_zero(Acc &a) {
+  a.sxx = a.syy = a.sxy = 0.0;
+  for (int r = 0; r < 4; r++) a.fx[r] = a.fy[r] = 0.0;
+}
+
+// Adds one neighbour's terms to sub-stencil sums when `on`; adding +0.0 to a
+// sum that started at +0.0 leaves it bit-identical, so masked lanes see the
+// same sequence of sums as the reference's per-sub-stencil loops.
+__device__ __forceinline__ void acc_add(Acc &a, bool on, double ga, double gb, double gc,
+                                        const double Gi[4], const double Gk[4], double dsw,
+                                        double dnw) {
+  a.sxx = a.sxx + (on ? ga : 0.0);
+  a.syy = a.syy + (on ? gb : 0.0);
+  a.sxy = a.sxy + (on ? gc : 0.0);
+  for (int r = 0; r < 4; r++) {
+    const double dG = Gk[r] - Gi[r];
+    a.fx[r] = a.fx[r] + (on ? dG * dsw : 0.0);
+    a.fy[r] = a.fy[r] + (on ? dG * dnw : 0.0);
+  }
+}
+
+// x-form solve (interior_fluxes_mod.h:104-113) or y-form (:324-328)
+__device__ __forceinline__ void acc_solve(const Acc &a, bool yform, double G[4]) {
+  const double det = a.sxx * a.syy - a.sxy * a.sxy;
+  const double one_by_det = 1.0 / det;
+  for (int r = 0; r < 4; r++) {
+    G[r] = yform ? (a.fy[r] * a.sxx - a.fx[r] * a.sxy) * one_by_det
+                 : (a.fx[r] * a.syy - a.fy[r] * a.sxy) * one_by_det;
+  }
+}
+
+// flux_residual_mod.h:10-89 with the ten *_dG* sub-stencil routines of
+// interior_fluxes_mod.h, wall_fluxes_mod.h and outer_fluxes_mod.h fused into
+// one pass over the neighbour list: the limited q~ at both ends and their
+// primitive states depend only on the (point, neighbour) pair, so they are
+// computed once per neighbour and feed the x- and y-direction fluxes of the
+// (usually two) sub-stencils the neighbour belongs to.
+template <bool POW>
+__global__ __launch_bounds__(256) void k_flux(int n, const int32_t *__restrict__ off,
+                                              const uint32_t *__restrict__ conn,
+                                              const double *__restrict__ rec,
+                                              const double *__restrict__ nxa,
+                                              const double *__restrict__ nya,
+                                              const uint8_t *__restrict__ flag,
+                                              const double *__restrict__ delta, double power,
+                                              double *__restrict__ fres) {
+  const int i = point_index();
+  if (i >= n) return;
+  const double *ri = rec + (size_t)R_STRIDE * i;
+  const double2 xy = reinterpret_cast<const double2 *>(ri)[0];
+  const double x_i = xy.x, y_i = xy.y;
+  double qi[4], dxi[4], dyi[4], qmxi[4], qmni[4];
+  load4(ri + R_Q, qi);
+  load4(ri + R_DQX, dxi);
+  load4(ri + R_DQY, dyi);
+  load4(ri + R_QMAX, qmxi);
+  load4(ri + R_QMIN, qmni);
+  const double eps2_i = ri[R_EPS2];
+  const double nx = nxa[i], ny = nya[i];
+  const double tx = ny, ty = -nx;
+  const int type = flag[i];
+  // wall: quadrant fluxes with A2neg and Gyn; outer: A2pos and Gyp
+  const double s2b = type == 0 ? -1.0 : 1.0;
+  Acc axp, axn, ayp, ayn;
+  acc_zero(axp);
+  acc_zero(axn);
+  acc_zero(ayp);
+  acc_zero(ayn);
+  const int e1 = off[i + 1];
+  for (int e = off[i]; e < e1; e++) {
+    const uint32_t w = conn[e];
+    const int k = (int)(w & kIndexMask);
+    const uint32_t m = w >> kMaskShift;
+    const double *rk = rec + (size_t)R_STRIDE * k;
+    const double2 xyk = reinterpret_cast<const double2 *>(rk)[0];
+    double qk[4], dxk[4], dyk[4], qmxk[4], qmnk[4];
+    load4(rk + R_Q, qk);
+    load4(rk + R_DQX, dxk);
+    load4(rk + R_DQY, dyk);
+    load4(rk + R_QMAX, qmxk);
+    load4(rk + R_QMIN, qmnk);
+    const double eps2_k = rk[R_EPS2];
+    const double delx = xyk.x - x_i, dely = xyk.y - y_i;
+    const double dels = delx * tx + dely * ty;
+    const double deln = delx * nx + dely * ny;
+    double dsw = dels, dnw = deln;
+    if (POW) {
+      const double weights = pow(sqrt(dels * dels + deln * deln), power);
+      dsw = dels * weights;
+      dnw = deln * weights;
+    }
+    const double ga = dels * dsw, gb = deln * dnw, gc = dels * dnw;
+    // limited q~ at both ends (interior_fluxes_mod.h:72-83)
+    double qti[4], qtk[4];
+    for (int r = 0; r < 4; r++) {
+      const double di = delx * dxi[r] + dely * dyi[r];
+      const double dk = delx * dxk[r] + dely * dyk[r];
+      const double phi_i = venkat_phi(qi[r] - 0.5 * di, qi[r], qmxi[r], qmni[r], eps2_i);
+      const double phi_k = venkat_phi(qk[r] - 0.5 * dk, qk[r], qmxk[r], qmnk[r], eps2_k);
+      qti[r] = qi[r] - 0.5 * phi_i * di;
+      qtk[r] = qk[r] - 0.5 * phi_k * dk;
+    }
+    double u1, u2, rho, pr;
+    qtilde_to_primitive(qti, u1, u2, rho, pr);
+    const EndTerms<double> ei = end_terms(nx, ny, u1, u2, rho, pr);
+    qtilde_to_primitive(qtk, u1, u2, rho, pr);
+    const EndTerms<double> ek = end_terms(nx, ny, u1, u2, rho, pr);
+    double Gi[4], Gk[4];
+    // x direction: x+ (dels <= 0) or x- (dels >= 0)
+    const bool xp = (m & XPOS) != 0;
+    const double sx = xp ? 1.0 : -1.0;
+    if (type == 1) {
+      split_x(ei, sx, Gi);
+      split_x(ek, sx, Gk);
+    } else {
+      quad_x(ei, sx, s2b, Gi);
+      quad_x(ek, sx, s2b, Gk);
+    }
+    acc_add(axp, xp, ga, gb, gc, Gi, Gk, dsw, dnw);
+    acc_add(axn, !xp, ga, gb, gc, Gi, Gk, dsw, dnw);
+    if (xp && (m & XNEG)) {  // dels == 0: the neighbour is in both lists
+      if (type == 1) {
+        split_x(ei, -1.0, Gi);
+        split_x(ek, -1.0, Gk);
+      } else {
+        quad_x(ei, -1.0, s2b, Gi);
+        quad_x(ek, -1.0, s2b, Gk);
+      }
+      acc_add(axn, true, ga, gb, gc, Gi, Gk, dsw, dnw);
+    }
+    // y direction: interior y+ / y-, wall y- (all), outer y+ (all)
+    const bool yp = type == 1 ? (m & YPOS) != 0 : type == 2;
+    const double sy = yp ? 1.0 : -1.0;
+    split_y(ei, sy, Gi);
+    split_y(ek, sy, Gk);
+    acc_add(ayp, yp, ga, gb, gc, Gi, Gk, dsw, dnw);
+    acc_add(ayn, !yp, ga, gb, gc, Gi, Gk, dsw, dnw);
+    if (type == 1 && yp && (m & YNEG)) {  // deln == 0
+      split_y(ei, -1.0, Gi);
+      split_y(ek, -1.0, Gk);
+      acc_add(ayn, true, ga, gb, gc, Gi, Gk, dsw, dnw);
+    }
+  }
+  double Gxp[4], Gxn[4], Gyp[4], Gyn[4], out[4];
+  acc_solve(axp, false, Gxp);
+  acc_solve(axn, false, Gxn);
+  const double d = delta[i];
+  if (type == 0) {
+    acc_solve(ayn, true, Gyn);
+    for (int j = 0; j < 4; j++) out[j] = (2.0 * d) * (Gxp[j] + Gxn[j] + Gyn[j]);
+  } else if (type == 1) {
+    acc_solve(ayp, true, Gyp);
+    acc_solve(ayn, true, Gyn);
+    for (int j = 0; j < 4; j++) out[j] = d * (Gxp[j] + Gxn[j] + Gyp[j] + Gyn[j]);
+  } else {
+    acc_solve(ayp, true, Gyp);
+    for (int j = 0; j < 4; j++) out[j] = d * (Gxp[j] + Gxn[j] + Gyp[j]);
+  }
+  for (int j = 0; j < 4; j++) fres[(size_t)j * n + i] = out[j];
+}
+
+// ------------------------------------------------------------ local dt
+// state_update_mod.h:448-496: min over the neighbours' states only.
+__global__ __launch_bounds__(256) void k_func_delta(int n, const int32_t *__restrict__ off,
+                                                    const uint32_t *__restrict__ conn,
+                                                    const double *__restrict__ rec,
+                                                    const double *__restrict__ prim, double cfl,
+                                                    double *__restrict__ delta) {
+  const int i = point_index();
+  if (i >= n) return;
+  const double2 xy = reinterpret_cast<const double2 *>(rec + (size_t)R_STRIDE * i)[0];
+  double min_delt = 1.0;
+  const int e1 = off[i + 1];
+  for (int e = off[i]; e < e1; e++) {
+    const int k = (int)(conn[e] & kIndexMask);
+    const double rho = prim[k], u1 = prim[n + k], u2 = prim[2 * n + k], pr = prim[3 * n + k];
+    const double2 xyk = reinterpret_cast<const double2 *>(rec + (size_t)R_STRIDE * k)[0];
+    double dist = (xyk.x - xy.x) * (xyk.x - xy.x) + (xyk.y - xy.y) * (xyk.y - xy.y);
+    dist = sqrt(dist);
+    const double mod_u = sqrt(u1 * u1 + u2 * u2);
+    double delta_t = dist / (mod_u + 3.0 * sqrt(pr / rho));
+    delta_t = cfl * delta_t;
+    if (min_delt > delta_t) min_delt = delta_t;
+  }
+  delta[i] = min_delt;
+}
+
+// ------------------------------------------------------------ state update
+// state_update_mod.h:18-33
+__device__ __forceinline__ void primitive_to_conserved(const double p[4], double nx, double ny,
+                                                       double U[4]) {
+  const double rho = p[0];
+  U[0] = rho;
+  const double temp1 = rho * p[1];
+  const double temp2 = rho * p[2];
+  U[3] = 2.5 * p[3] + 0.5 * (temp1 * temp1 + temp2 * temp2) / rho;
+  U[1] = temp1 * ny - temp2 * nx;
+  U[2] = temp1 * nx + temp2 * ny;
+}
+
+// state_update_mod.h:35-90
+__device__ __forceinline__ void conserved_vector_Ubar(const double p[4], double nx, double ny,
+                                                      const FlowConsts &fc, double Ubar[4]) {
+  const double rho_inf = fc.rho_inf, pr_inf = fc.pr_inf, gamma_new = 1.4;
+  const double tx = ny, ty = -nx;
+  const double u1_inf_rot = fc.u1_inf * tx + fc.u2_inf * ty;
+  const double u2_inf_rot = fc.u1_inf * nx + fc.u2_inf * ny;
+  double temp1 = (u1_inf_rot * u1_inf_rot + u2_inf_rot * u2_inf_rot);
+  const double e_inf = pr_inf / (rho_inf * (gamma_new - 1.0)) + 0.5 * (temp1);
+  double beta = (0.5 * rho_inf) / pr_inf;
+  double S2 = u2_inf_rot * sqrt(beta);
+  const double B2_inf = exp(-S2 * S2) / (2.0 * sqrt(kPi * beta));
+  const double A2n_inf = 0.5 * (1.0 - erf(S2));
+  const double rho = p[0], u1 = p[1], u2 = p[2], pr = p[3];
+  const double u1_rot = u1 * tx + u2 * ty;
+  const double u2_rot = u1 * nx + u2 * ny;
+  temp1 = (u1_rot * u1_rot + u2_rot * u2_rot);
+  const double e = pr / (rho * (gamma_new - 1.0)) + 0.5 * (temp1);
+  beta = (rho) / (2.0 * pr);
+  S2 = u2_rot * sqrt(beta);
+  const double B2 = exp(-S2 * S2) / (2.0 * sqrt(kPi * beta));
+  const double A2p = 0.5 * (1.0 + erf(S2));
+  Ubar[0] = (rho_inf * A2n_inf) + (rho * A2p);
+  Ubar[1] = (rho_inf * u1_inf_rot * A2n_inf) + (rho * u1_rot * A2p);
+  temp1 = rho_inf * (u2_inf_rot * A2n_inf - B2_inf);
+  double temp2 = rho * (u2_rot * A2p + B2);
+  Ubar[2] = temp1 + temp2;
+  temp1 = (rho_inf * A2n_inf * e_inf - 0.5 * rho_inf * u2_inf_rot * B2_inf);
+  temp2 = (rho * A2p * e + 0.5 * rho * u2_rot * B2);
+  Ubar[3] = temp1 + temp2;
+}
+
+// (sum, max, argmax) of the block, fixed tree order; argmax ties go to the
+// smaller caller index, as the reference's strict '>' over points in order.
+__device__ void block_reduce3(double &s, double &mx, int &arg) {
+  __shared__ double ss[256], sm[256];
+  __shared__ int sa[256];
+  const int t = threadIdx.x;
+  ss[t] = s;
+  sm[t] = mx;
+  sa[t] = arg;
+  __syncthreads();
+  for (int w = blockDim.x >> 1; w > 0; w >>= 1) {
+    if (t < w) {
+      ss[t] = ss[t] + ss[t + w];
+      const double a = sm[t], b = sm[t + w];
+      const int ia = sa[t], ib = sa[t + w];
+      if (b > a || (b == a && ib < ia)) {
+        sm[t] = b;
+        sa[t] = ib;
+      }
+    }
+    __syncthreads();
+  }
+  s = ss[0];
+  mx = sm[0];
+  arg = sa[0];
+}
+
+// state_update_mod.h:92-283
+__global__ __launch_bounds__(256) void k_state_update(int n, int rk, double *__restrict__ prim,
+                                                      const double *__restrict__ prim_old,
+                                                      const double *__restrict__ fres,
+                                                      const double *__restrict__ nxa,
+                                                      const double *__restrict__ nya,
+                                                      const uint8_t *__restrict__ flag,
+                                                      const int32_t *__restrict__ perm,
+                                                      FlowConsts fc, double *__restrict__ psum,
+                                                      double *__restrict__ pmax,
+                                                      int32_t *__restrict__ parg) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double res_sqr = 0.0;
+  int arg = 0x7fffffff;
+  if (i < n) {
+    const double obt = 1.0 / 3.0, tbt = 2.0 / 3.0;
+    const double nx = nxa[i], ny = nya[i];
+    const int type = flag[i];
+    double p1[4], p2[4], U[4], U_old[4];
+    for (int r = 0; r < 4; r++) {
+      p1[r] = prim[(size_t)r * n + i];
+      p2[r] = prim_old[(size_t)r * n + i];
+    }
+    if (type != 2) {
+      primitive_to_conserved(p1, nx, ny, U);
+      primitive_to_conserved(p2, nx, ny, U_old);
+    } else {
+      conserved_vector_Ubar(p1, nx, ny, fc, U);
+      conserved_vector_Ubar(p2, nx, ny, fc, U_old);
+    }
+    const double temp0 = U[0];
+    if (rk != 3) {
+      for (int j = 0; j < 4; j++) U[j] = U[j] - (0.5 * fres[(size_t)j * n + i]);
+    } else {
+      for (int j = 0; j < 4; j++)
+        U[j] = tbt * U_old[j] + obt * (U[j] - 0.5 * fres[(size_t)j * n + i]);
+    }
+    if (type == 0) U[2] = 0.;
+    const double U2_rot = U[1], U3_rot = U[2];
+    U[1] = U2_rot * ny + U3_rot * nx;
+    U[2] = U3_rot * ny - U2_rot * nx;
+    if (type != 2) {
+      res_sqr = (U[0] - temp0) * (U[0] - temp0);
+      arg = perm[i];
+    }
+    prim[i] = U[0];
+    const double temp = 1.0 / U[0];
+    prim[(size_t)n + i] = U[1] * temp;
+    prim[(size_t)2 * n + i] = U[2] * temp;
+    prim[(size_t)3 * n + i] = 0.4 * U[3] - (0.2 * temp) * (U[1] * U[1] + U[2] * U[2]);
+  }
+  double s = res_sqr, mx = res_sqr;
+  if (arg == 0x7fffffff) mx = -1.0;
+  block_reduce3(s, mx, arg);
+  if (threadIdx.x == 0) {
+    psum[blockIdx.x] = s;
+    pmax[blockIdx.x] = mx;
+    parg[blockIdx.x] = arg;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_reduce(int nb, const double *__restrict__ psum,
+                                                const double *__restrict__ pmax,
+                                                const int32_t *__restrict__ parg,
+                                                double *__restrict__ out, double *__restrict__ hist) {
+  double s = 0.0, mx = -1.0;
+  int arg = 0x7fffffff;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+    s = s + psum[b];
+    const double v = pmax[b];
+    const int a = parg[b];
+    if (v > mx || (v == mx && a < arg)) {
+      mx = v;
+      arg = a;
+    }
+  }
+  block_reduce3(s, mx, arg);
+  if (threadIdx.x == 0) {
+    out[0] = s;
+    out[1] = mx;
+    out[2] = (double)arg;
+    if (hist) hist[0] = s;
+  }
+}
+
+// ------------------------------------------------------------ forces
+// compute_force_coeffs_mod.h:25-60 for shape 1: H, V and the pitching
+// moment about (0.25, 0), reduced in a fixed order by one block.
+__global__ __launch_bounds__(256) void k_forces(int ns, const int32_t *__restrict__ mi,
+                                                const int32_t *__restrict__ li,
+                                                const int32_t *__restrict__ ri,
+                                                const int32_t *__restrict__ shape,
+                                                const double *__restrict__ rec,
+                                                const double *__restrict__ nxa,
+                                                const double *__restrict__ nya,
+                                                const double *__restrict__ prim, int n,
+                                                double pr_inf, double qdyn,
+                                                double *__restrict__ out) {
+  __shared__ double sh[3][256];
+  double H = 0.0, V = 0.0, P = 0.0;
+  for (int j = threadIdx.x; j < ns; j += blockDim.x) {
+    if (shape[j] != 1) continue;
+    const int m = mi[j], l = li[j], r = ri[j];
+    const double2 L = reinterpret_cast<const double2 *>(rec + (size_t)R_STRIDE * l)[0];
+    const double2 Mp = reinterpret_cast<const double2 *>(rec + (size_t)R_STRIDE * m)[0];
+    const double2 Rp = reinterpret_cast<const double2 *>(rec + (size_t)R_STRIDE * r)[0];
+    double ds1 = (Mp.x - L.x) * (Mp.x - L.x) + (Mp.y - L.y) * (Mp.y - L.y);
+    ds1 = sqrt(ds1);
+    double ds2 = (Rp.x - Mp.x) * (Rp.x - Mp.x) + (Rp.y - Mp.y) * (Rp.y - Mp.y);
+    ds2 = sqrt(ds2);
+    const double ds = 0.5 * (ds1 + ds2);
+    const double nx = nxa[m], ny = nya[m];
+    double cp = prim[(size_t)3 * n + m] - pr_inf;
+    cp = -cp / qdyn;
+    H = H + cp * nx * ds;
+    V = V + cp * ny * ds;
+    P = P + (-cp * ny * ds * (Mp.x - 0.25) + cp * nx * ds * (Mp.y));
+  }
+  const int t = threadIdx.x;
+  sh[0][t] = H;
+  sh[1][t] = V;
+  sh[2][t] = P;
+  __syncthreads();
+  for (int w = blockDim.x >> 1; w > 0; w >>= 1) {
+    if (t < w)
+      for (int c = 0; c < 3; c++) sh[c][t] = sh[c][t] + sh[c][t + w];
+    __syncthreads();
+  }
+  if (t < 3) out[t] = sh[t][0];
+}
+
+// ------------------------------------------------------------ KAT
+__global__ void k_kat_fluxes(int n, const double *__restrict__ in, double *__restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double *v = in + 6 * (size_t)i;
+  const EndTerms<double> e = end_terms(v[0], v[1], v[2], v[3], v[4], v[5]);
+  double G[8][4];
+  split_x(e, 1.0, G[0]);
+  split_x(e, -1.0, G[1]);
+  split_y(e, 1.0, G[2]);
+  split_y(e, -1.0, G[3]);
+  quad_x(e, -1.0, -1.0, G[4]);
+  quad_x(e, 1.0, -1.0, G[5]);
+  quad_x(e, 1.0, 1.0, G[6]);
+  quad_x(e, -1.0, 1.0, G[7]);
+  for (int f = 0; f < 8; f++)
+    for (int r = 0; r < 4; r++) out[32 * (size_t)i + 4 * f + r] = G[f][r];
+}
+
+__global__ void k_kat_qtilde(int n, const double *__restrict__ in, double *__restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double qt[4];
+  for (int r = 0; r < 4; r++) qt[r] = in[4 * (size_t)i + r];
+  double u1, u2, rho, pr;
+  qtilde_to_primitive(qt, u1, u2, rho, pr);
+  out[4 * (size_t)i + 0] = u1;
+  out[4 * (size_t)i + 1] = u2;
+  out[4 * (size_t)i + 2] = rho;
+  out[4 * (size_t)i + 3] = pr;
+}
+
+}  // namespace
+
+int blocks_for(int n, int block) { return (n + block - 1) / block; }
+
+void launch_q_variables(const DevCloud &c, const double *prim, double *rec, int block,
+                        hipStream_t s) {
+  hipLaunchKernelGGL(k_q_variables, dim3(blocks_for(c.n, block)), dim3(block), 0, s, c.n, prim,
+                     rec);
+}
+
+void launch_q_derivatives(const DevCloud &c, double *rec, double *dq_dst, int dq_stride,
+                          double power, int block, hipStream_t s) {
+  dim3 g(blocks_for(c.n, block)), b(block);
+  if (power == 0.0)
+    hipLaunchKernelGGL(k_q_derivatives<false>, g, b, 0, s, c.n, c.conn_off, c.conn, rec, dq_dst,
+                       dq_stride, power);
+  else
+    hipLaunchKernelGGL(k_q_derivatives<true>, g, b, 0, s, c.n, c.conn_off, c.conn, rec, dq_dst,
+                       dq_stride, power);
+}
+
+void launch_q_inner(const DevCloud &c, const double *rec, const double *dq_src, int src_stride,
+                    double *dq_dst, int dst_stride, double power, int block, hipStream_t s) {
+  dim3 g(blocks_for(c.n, block)), b(block);
+  if (power == 0.0)
+    hipLaunchKernelGGL(k_q_inner<false>, g, b, 0, s, c.n, c.conn_off, c.conn, rec, dq_src,
+                       src_stride, dq_dst, dst_stride, power);
+  else
+    hipLaunchKernelGGL(k_q_inner<true>, g, b, 0, s, c.n, c.conn_off, c.conn, rec, dq_src,
+                       src_stride, dq_dst, dst_stride, power);
+}
+
+void launch_flux(const DevCloud &c, const double *rec, const double *delta, double power,
+                 double *flux_res, int block, hipStream_t s) {
+  dim3 g(blocks_for(c.n, block)), b(block);
+  if (power == 0.0)
+    hipLaunchKernelGGL(k_flux<false>, g, b, 0, s, c.n, c.conn_off, c.conn, rec, c.nx, c.ny,
+                       c.flag, delta, power, flux_res);
+  else
+    hipLaunchKernelGGL(k_flux<true>, g, b, 0, s, c.n, c.conn_off, c.conn, rec, c.nx, c.ny,
+                       c.flag, delta, power, flux_res);
+}
+
+void launch_func_delta(const DevCloud &c, const double *rec, const double *prim, double cfl,
+                       double *delta, int block, hipStream_t s) {
+  hipLaunchKernelGGL(k_func_delta, dim3(blocks_for(c.n, block)), dim3(block), 0, s, c.n,
+                     c.conn_off, c.conn, rec, prim, cfl, delta);
+}
+
+void launch_state_update(const DevCloud &c, int rk, double *prim, const double *prim_old,
+                         const double *flux_res, const FlowConsts &fc, const Partials &p,
+                         int /*block*/, hipStream_t s) {
+  // fixed 256-thread blocks: the partial layout must not depend on tuning
+  hipLaunchKernelGGL(k_state_update, dim3(p.nblocks), dim3(256), 0, s, c.n, rk, prim, prim_old,
+                     flux_res, c.nx, c.ny, c.flag, c.perm, fc, p.sum, p.max, p.arg);
+}
+
+void launch_reduce(const Partials &p, double *out, double *hist_slot, hipStream_t s) {
+  hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, s, p.nblocks, p.sum, p.max, p.arg, out,
+                     hist_slot);
+}
+
+void launch_forces(int n_shape, const int32_t *m, const int32_t *l, const int32_t *r,
+                   const int32_t *shape, const double *rec, const double *nx, const double *ny,
+                   const double *prim, int n, double pr_inf, double qdyn, double *out3,
+                   hipStream_t s) {
+  hipLaunchKernelGGL(k_forces, dim3(1), dim3(256), 0, s, n_shape, m, l, r, shape, rec, nx, ny,
+                     prim, n, pr_inf, qdyn, out3);
+}
+
+void launch_kat_fluxes(int n, const double *in6, double *out, hipStream_t s) {
+  hipLaunchKernelGGL(k_kat_fluxes, dim3(blocks_for(n, 128)), dim3(128), 0, s, n, in6, out);
+}
+
+void launch_kat_qtilde(int n, const double *qt4, double *out, hipStream_t s) {
+  hipLaunchKernelGGL(k_kat_qtilde, dim3(blocks_for(n, 128)), dim3(128), 0, s, n, qt4, out);
+}
+
+}  // namespace lskum

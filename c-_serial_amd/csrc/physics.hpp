@@ -1,0 +1,136 @@
+// physics.hpp -- q-LSKUM leaf physics as __device__ functions for gfx950.
+//
+// Each function keeps the reference's operation order (compiled with
+// -ffp-contract=off), so the FP64 form differs from the reference only where
+// the ROCm device libm (exp, log, erf) rounds differently from glibc.
+//
+// The split/quadrant fluxes are restructured around one set of per-end
+// terms (EndTerms): every flux the reference evaluates for a (point,
+// neighbour) pair -- two sub-stencils per neighbour, so two to four fluxes
+// per end -- shares beta, sqrt(beta), sqrt(pi*beta), exp(-S^2) and erf(S),
+// which are computed once.  A flux's +/- variant is selected by a sign s =
+// +-1.0: 0.5*(1 + s*erf) and x + s*y are bit-identical to the reference's
+// 0.5*(1 - erf) and x - y because multiplying by -1.0 is exact.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace lskum {
+
+constexpr double kPi = 3.14159265358979323846;  // == 4*atan(1.0) in double
+constexpr double kGammaM1 = 1.4 - 1.0;          // gamma_new - 1 as evaluated
+
+// q_variables_mod.h:230-253 (output order u1, u2, rho, pr as there)
+template <typename T>
+__device__ __forceinline__ void qtilde_to_primitive(const T qt[4], T &u1, T &u2,
+                                                    T &rho, T &pr) {
+  const T beta = -qt[3] * T(0.5);
+  const T temp = T(0.5) / beta;
+  u1 = qt[1] * temp;
+  u2 = qt[2] * temp;
+  const T temp1 = qt[0] + beta * (u1 * u1 + u2 * u2);
+  const T temp2 = temp1 - (log(beta) / T(kGammaM1));
+  rho = exp(temp2);
+  pr = rho * temp;
+}
+
+// Terms shared by all split and quadrant fluxes of one state in the frame
+// t = (ny, -nx), n = (nx, ny): split_fluxes_mod.h:14-26 and
+// quadrant_fluxes_mod.h:17-32.
+template <typename T>
+struct EndTerms {
+  T rho, ut, un, pr_by_rho, u_sqr, B1, B2, e1, e2;
+};
+
+template <typename T>
+__device__ __forceinline__ EndTerms<T> end_terms(T nx, T ny, T u1, T u2, T rho,
+                                                 T pr) {
+  EndTerms<T> e;
+  const T tx = ny, ty = -nx;
+  e.rho = rho;
+  e.ut = u1 * tx + u2 * ty;
+  e.un = u1 * nx + u2 * ny;
+  const T beta = T(0.5) * rho / pr;
+  const T sb = sqrt(beta);
+  const T S1 = e.ut * sb;
+  const T S2 = e.un * sb;
+  const T spb = sqrt(T(kPi) * beta);
+  e.B1 = T(0.5) * exp(-S1 * S1) / spb;
+  e.B2 = T(0.5) * exp(-S2 * S2) / spb;
+  e.e1 = erf(S1);
+  e.e2 = erf(S2);
+  e.pr_by_rho = pr / rho;
+  e.u_sqr = e.ut * e.ut + e.un * e.un;
+  return e;
+}
+
+// split_fluxes_mod.h:6-43 (s = +1, Gxp) and :45-84 (s = -1, Gxn)
+template <typename T>
+__device__ __forceinline__ void split_x(const EndTerms<T> &e, T s, T G[4]) {
+  const T A1 = T(0.5) * (T(1.0) + s * e.e1);
+  G[0] = e.rho * (e.ut * A1 + s * e.B1);
+  T temp1 = e.pr_by_rho + e.ut * e.ut;
+  T temp2 = temp1 * A1 + s * (e.ut * e.B1);
+  G[1] = e.rho * temp2;
+  temp1 = e.ut * e.un * A1 + s * (e.un * e.B1);
+  G[2] = e.rho * temp1;
+  temp1 = (T(7.0) * e.pr_by_rho) + e.u_sqr;
+  temp2 = T(0.5) * e.ut * temp1 * A1;
+  temp1 = (T(6.0) * e.pr_by_rho) + e.u_sqr;
+  G[3] = e.rho * (temp2 + s * (T(0.5) * temp1 * e.B1));
+}
+
+// split_fluxes_mod.h:86-125 (s = +1, Gyp) and :127-164 (s = -1, Gyn)
+template <typename T>
+__device__ __forceinline__ void split_y(const EndTerms<T> &e, T s, T G[4]) {
+  const T A2 = T(0.5) * (T(1.0) + s * e.e2);
+  G[0] = e.rho * (e.un * A2 + s * e.B2);
+  T temp1 = e.pr_by_rho + e.un * e.un;
+  T temp2 = temp1 * A2 + s * (e.un * e.B2);
+  G[2] = e.rho * temp2;
+  temp1 = e.ut * e.un * A2 + s * (e.ut * e.B2);
+  G[1] = e.rho * temp1;
+  temp1 = (T(7.0) * e.pr_by_rho) + e.u_sqr;
+  temp2 = T(0.5) * e.un * temp1 * A2;
+  temp1 = (T(6.0) * e.pr_by_rho) + e.u_sqr;
+  G[3] = e.rho * (temp2 + s * (T(0.5) * temp1 * e.B2));
+}
+
+// quadrant_fluxes_mod.h: GxI :6 (s1 -, s2 -), GxII :57 (+, -),
+// GxIII :108 (+, +), GxIV :161 (-, +)
+template <typename T>
+__device__ __forceinline__ void quad_x(const EndTerms<T> &e, T s1, T s2, T G[4]) {
+  const T A1 = T(0.5) * (T(1.0) + s1 * e.e1);
+  const T A2 = T(0.5) * (T(1.0) + s2 * e.e2);
+  const T a1b1 = e.ut * A1 + s1 * e.B1;
+  G[0] = e.rho * A2 * a1b1;
+  T temp1 = e.pr_by_rho + e.ut * e.ut;
+  T temp2 = temp1 * A1 + s1 * (e.ut * e.B1);
+  G[1] = e.rho * A2 * temp2;
+  temp2 = e.un * A2 + s2 * e.B2;
+  G[2] = e.rho * a1b1 * temp2;
+  temp1 = (T(7.0) * e.pr_by_rho) + e.u_sqr;
+  temp2 = T(0.5) * e.ut * temp1 * A1;
+  temp1 = (T(6.0) * e.pr_by_rho) + e.u_sqr;
+  const T temp3 = T(0.5) * e.B1 * temp1;
+  const T temp4 = T(0.5) * e.rho * e.un * e.B2 * a1b1;
+  G[3] = e.rho * A2 * (temp2 + s1 * temp3) + s2 * temp4;
+}
+
+// limiters_mod.h:55-104 for one component; eps2 = (pow(K*min_dist,3))^2 is
+// precomputed per point on the host with the reference's pow call.
+template <typename T>
+__device__ __forceinline__ T venkat_phi(T qtilde, T q, T qmax, T qmin, T eps2) {
+  const T del_neg = qtilde - q;
+  if (fabs(del_neg) <= T(10e-6)) return T(1.0);
+  const T del_pos = del_neg > T(0) ? qmax - q : qmin - q;
+  T num = (del_pos * del_pos) + eps2;
+  num = num * del_neg + T(2.0) * del_neg * del_neg * del_pos;
+  T den = del_pos * del_pos + T(2.0) * del_neg * del_neg;
+  den = den + del_neg * del_pos + eps2;
+  den = den * del_neg;
+  const T temp = num / den;
+  return temp < T(1) ? temp : T(1);
+}
+
+}  // namespace lskum

@@ -1,0 +1,231 @@
+/*
+ * lskum.h -- C ABI of the MI355X q-LSKUM iteration.
+ *
+ * The reference solver (Pathlessbark8/C-_serial) has no plugin or FFI layer: its
+ * hot path is a set of argument-less void functions over one global
+ * `points point` struct, called in a fixed order by fpi_solver()
+ * (fpi_solver_mod.h:32-61).  This header replaces that set one for one with
+ * explicit, stream-ordered entry points on an opaque context that owns the
+ * device state.  Every entry point cites the reference function it replaces.
+ *
+ * Conventions
+ *  - Status is int: 0 = ok, negative = error (see LSKUM_E*).  No exception
+ *    crosses the ABI.  lskum_last_error() returns a message for the last
+ *    failing call on a context.
+ *  - Arrays are 0-based and component-major ("SoA"), as prim[var*n + point].
+ *    Host arrays given to lskum_create() are copied; outputs go to caller-owned
+ *    buffers.  Point order at the ABI is always the caller's order, even when
+ *    the context reorders points internally (LSKUM_REORDER_RCM).
+ *  - Calls on one context are ordered on its HIP stream.  Only calls that
+ *    return host scalars or host arrays synchronise.
+ *  - One context per host thread; contexts are independent.
+ */
+#ifndef LSKUM_H
+#define LSKUM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+#define LSKUM_OK 0
+#define LSKUM_EINVAL (-1)     /* bad argument or inconsistent input        */
+#define LSKUM_EHIP (-2)       /* a HIP runtime call failed                 */
+#define LSKUM_ENOMEM (-3)     /* host or device allocation failed          */
+#define LSKUM_EIO (-4)        /* file missing, unreadable or truncated     */
+#define LSKUM_ENONFINITE (-5) /* residual or force came out NaN/Inf       */
+#define LSKUM_ESTENCIL (-6)   /* degenerate sub-stencil (empty or det<=0)  */
+
+/* --------------------------------------------------------------- clouds */
+/*
+ * Point cloud in the reference's "partGrid" content (main.cpp:30-58), 0-based.
+ * flag_1: 0 wall, 1 interior, 2 outer (main.cpp:37-51).  flag_2 > 0: shape
+ * index (main.cpp:53-56).  left/right: previous/next boundary point, -1 for
+ * interior points.  conn is CSR: neighbours of point i are
+ * conn[conn_off[i] .. conn_off[i+1]-1].
+ */
+typedef struct lskum_cloud {
+  int32_t n_points;
+  const double *x, *y;
+  const int32_t *left, *right;
+  const int32_t *flag_1, *flag_2;
+  const double *min_dist;
+  const int32_t *conn_off; /* n_points + 1 entries */
+  const int32_t *conn;     /* conn_off[n_points] entries */
+} lskum_cloud;
+
+/* Host-owned cloud made by the generator or the partGrid reader. */
+typedef struct lskum_cloud_h lskum_cloud_h;
+
+/*
+ * Synthetic NACA 4-digit meshfree cloud (no reference counterpart: the
+ * reference reads a fixed file, main.cpp:17).  Wall points first (cosine
+ * chord spacing, clockwise from the trailing edge), then n_radial-1 jittered
+ * O-grid rings of interior points, then the far-field ring (flag_1 = 2).
+ * Neighbours: the k nearest visible points within radius_factor x local
+ * spacing, ties broken by index; a pair is not visible when its segment
+ * crosses the body outline or when both are wall points on opposite
+ * surfaces.  Deterministic from seed.
+ */
+typedef struct lskum_gen_params {
+  int32_t naca_m;   /* max camber, % chord (NACA 4-digit first digit)   */
+  int32_t naca_p;   /* camber position, tenths of chord (second digit)  */
+  int32_t naca_t;   /* thickness, % chord (last two digits)             */
+  int32_t n_wall;   /* wall points                                      */
+  int32_t n_radial; /* rings, the last one being the far field          */
+  int32_t n_circ;   /* points per ring                                  */
+  double r_far;     /* far-field radius, chords                         */
+  double stretch;   /* radial growth ratio; <= 0: first ring spacing    */
+                    /* matched to the first ring's circumferential one  */
+  double jitter;    /* uniform jitter, fraction of local spacing        */
+  int32_t k;        /* neighbours per point                             */
+  double radius_factor; /* search radius in local spacings              */
+  uint64_t seed;
+} lskum_gen_params;
+
+int lskum_cloud_generate(const lskum_gen_params *p, lskum_cloud_h **out);
+/* Reader for the legacy partGrid text file (main.cpp:16-58). */
+int lskum_cloud_read_partgrid(const char *path, lskum_cloud_h **out);
+/* Writer in the same layout, doubles at 17 significant digits. */
+int lskum_cloud_write_partgrid(const lskum_cloud_h *c, const char *path);
+/* Copy caller arrays into a new host cloud. */
+int lskum_cloud_from_arrays(const lskum_cloud *src, lskum_cloud_h **out);
+/* Borrowed view of the arrays of c (valid until lskum_cloud_free). */
+int lskum_cloud_view(const lskum_cloud_h *c, lskum_cloud *view);
+/* Generator statistics: min/max neighbour count and min_rel_det[3]: per
+ * point type (flag_1 0..2), the smallest sub-stencil least-squares
+ * determinant relative to its scale; -1 if no point of the type. */
+int lskum_cloud_stats(const lskum_cloud_h *c, int32_t *min_nbhs,
+                      int32_t *max_nbhs, double *min_rel_det);
+void lskum_cloud_free(lskum_cloud_h *c);
+
+/* ------------------------------------------------------------ parameters */
+#define LSKUM_FP64 0  /* every operation in FP64, in the reference's order  */
+#define LSKUM_MIXED 1 /* FP32 gathers/fluxes/limiter, FP64 solves/updates  */
+
+#define LSKUM_REORDER_NONE 0
+#define LSKUM_REORDER_RCM 1 /* reverse Cuthill-McKee on the neighbour graph */
+
+/*
+ * Case and numerics parameters.  The reference hard-codes these as globals
+ * (parameter_mod.h:4-10, data_structure_mod.h:57-121); here they are runtime
+ * fields.  gamma = 1.4 is fixed, as the reference's 2.5 / 0.4 / 0.2 / 7 / 6
+ * factors assume it (q_variables_mod.h:25, state_update_mod.h:171).
+ */
+typedef struct lskum_params {
+  double mach;      /* free-stream Mach number (parameter_mod.h:4)          */
+  double aoa_deg;   /* angle of attack, degrees (parameter_mod.h:5)         */
+  double cfl;       /* CFL (data_structure_mod.h:72)                        */
+  double vl_const;  /* Venkatakrishnan K (data_structure_mod.h:96)          */
+  double power;     /* LS weight exponent (data_structure_mod.h:90)         */
+  int32_t inner_iterations; /* (data_structure_mod.h:118)                  */
+  int32_t precision;        /* LSKUM_FP64 or LSKUM_MIXED                    */
+  int32_t reorder;          /* LSKUM_REORDER_*                              */
+  int32_t block;            /* threads per block, 0 = default               */
+} lskum_params;
+
+/* --------------------------------------------------------------- context */
+typedef struct lskum_ctx lskum_ctx;
+
+/*
+ * Copies the cloud to the device, computes normals (point_normals_mod.h:17-55)
+ * and the x+/x-/y+/y- sub-stencils (generate_connectivity_mod.h:5-249) on the
+ * host, validates every sub-stencil, and sets the free-stream state
+ * (initial_conditions_mod.h:23-37, parameter_mod.h:13-30).
+ * Replaces main.cpp:59-65.
+ */
+int lskum_create(const lskum_cloud *cloud, const lskum_params *params,
+                 lskum_ctx **out);
+void lskum_destroy(lskum_ctx *ctx);
+const char *lskum_last_error(const lskum_ctx *ctx);
+
+/* prim is [4][n]: rho, u1, u2, p (point.prim, data_structure_mod.h:26). */
+int lskum_set_state(lskum_ctx *ctx, const double *prim);
+int lskum_get_state(lskum_ctx *ctx, double *prim);
+
+/* prim_old <- prim (fpi_solver_mod.h:19-25). */
+int lskum_snapshot(lskum_ctx *ctx);
+/* Local time step delta (state_update_mod.h:448 func_delta). */
+int lskum_func_delta(lskum_ctx *ctx);
+/* q from prim (q_variables_mod.h:9 eval_q_variables). */
+int lskum_q_variables(lskum_ctx *ctx);
+/* dq, qmax, qmin (q_variables_mod.h:36 eval_q_derivatives). */
+int lskum_q_derivatives(lskum_ctx *ctx);
+/* n_inner passes of eval_q_inner_loop + eval_update_innerloop
+ * (q_variables_mod.h:120, :215). */
+int lskum_q_inner(lskum_ctx *ctx, int32_t n_inner);
+/* flux_res (flux_residual_mod.h:10 cal_flux_residual). */
+int lskum_flux_residual(lskum_ctx *ctx);
+/* SSPRK stage rk in 1..4 (state_update_mod.h:92 state_update); returns the
+ * globals sum_res_sqr, max_res, max_res_point (0-based) it sets.  Any
+ * pointer may be NULL (then the call does not synchronise). */
+int lskum_state_update(lskum_ctx *ctx, int32_t rk, double *sum_res_sqr,
+                       double *max_res, int32_t *max_res_point);
+/* Cl, Cd, Cm of shape 1 (compute_force_coeffs_mod.h:7 compute_cl_cd_cm). */
+int lskum_forces(lskum_ctx *ctx, double *cl, double *cd, double *cm);
+
+/*
+ * n_iter whole iterations, each one fpi_solver() (fpi_solver_mod.h:14-150):
+ * snapshot, func_delta, then per rk = 1..4 q_variables, q_derivatives,
+ * inner_iterations inner passes, flux_residual, state_update.  Iterations are
+ * numbered it0+1 .. it0+n_iter (q_lskum_mod.h:33).  res_new[i] =
+ * sqrt(sum_res_sqr)/n and residue[i] = log10(res_new/res_old) with res_old
+ * latched at it <= 2 (fpi_solver_mod.h:138-146), unless restart is set.
+ * Either output may be NULL.
+ */
+int lskum_iterate(lskum_ctx *ctx, int32_t n_iter, double *res_new,
+                  double *residue);
+/* Iteration counter and res_old, for restart (q_lskum_mod.h:29-37,
+ * fpi_solver_mod.h:139). restart != 0 keeps res_old across it <= 2. */
+int lskum_get_iteration(lskum_ctx *ctx, int32_t *it, double *res_old);
+int lskum_set_iteration(lskum_ctx *ctx, int32_t it, double res_old,
+                        int32_t restart);
+
+/* Arrays for tests and post-processing; sizes in doubles. */
+#define LSKUM_A_PRIM 0     /* [4][n]                      */
+#define LSKUM_A_PRIM_OLD 1 /* [4][n]                      */
+#define LSKUM_A_Q 2        /* [4][n]                      */
+#define LSKUM_A_DQ 3       /* [2][4][n]: d/dx, d/dy       */
+#define LSKUM_A_QM 4       /* [2][4][n]: max, min         */
+#define LSKUM_A_DELTA 5    /* [n]                         */
+#define LSKUM_A_FLUX_RES 6 /* [4][n]                      */
+#define LSKUM_A_NORMALS 7  /* [2][n]: nx, ny              */
+int lskum_get_array(lskum_ctx *ctx, int32_t which, double *host_dst);
+/* Sub-stencil lists as made by generate_connectivity, caller order.
+ * which: 0 xpos, 1 xneg, 2 ypos, 3 yneg.  counts[n]; members are written
+ * to conn_out in point order, conn order within a point. */
+int lskum_get_substencil(lskum_ctx *ctx, int32_t which, int32_t *counts,
+                         int32_t *members);
+
+/* Stream the context's kernels run on (a hipStream_t). */
+void *lskum_stream(lskum_ctx *ctx);
+
+/* Average duration in ms of the named kernel over the last lskum_iterate
+ * call with timing enabled (0 disables).  Kernel ids: LSKUM_K_*. */
+#define LSKUM_K_FLUX 0
+#define LSKUM_K_QDERIV 1
+#define LSKUM_K_QINNER 2
+#define LSKUM_K_STATE 3
+#define LSKUM_K_DELTA 4
+#define LSKUM_K_QVAR 5
+int lskum_set_timing(lskum_ctx *ctx, int32_t enable);
+int lskum_kernel_time(lskum_ctx *ctx, int32_t kernel, double *avg_ms,
+                      int32_t *launches);
+
+/*
+ * Device copies of the leaf physics, for known-answer tests: evaluates the
+ * eight flux functions (split_fluxes_mod.h:6,45,86,127 and
+ * quadrant_fluxes_mod.h:6,57,108,161) on n input rows
+ * (nx, ny, u1, u2, rho, pr) -> out[n][8][4], order Gxp, Gxn, Gyp, Gyn, GxI,
+ * GxII, GxIII, GxIV; and qtilde_to_primitive (q_variables_mod.h:230) on
+ * n rows of q~[4] -> prim_out[n][4] as (u1, u2, rho, pr).
+ */
+int lskum_kat_fluxes(int32_t n, const double *in6, double *out, int32_t mixed);
+int lskum_kat_qtilde(int32_t n, const double *qt4, double *prim_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LSKUM_H */
