@@ -1,0 +1,177 @@
+#!/usr/bin/env python3
+"""Benchmark: q-LSKUM FP64 iterations on one MI355X.
+
+A step is one fpi_solver iteration (snapshot, local time step, four SSPRK
+stages of q-variables, q-derivatives, 2 inner passes, flux residual and state
+update) over the 2,502,000-point NACA0012 cloud (wall 2,000, 500 x 5,000
+O-grid, k = 24, RCM order), M 0.85, AoA 1.25 deg, CFL 0.2, K 5.  Inputs are
+resident in HBM before the timed region.  Prints one JSON line.
+
+  python bench.py [--gpus 1] [--steps 50] [--warmup 5] [--no-cpu-baseline]
+"""
+import argparse
+import json
+import os
+import platform
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "c-_serial_amd"))
+sys.path.insert(0, REPO)
+
+import lskum  # noqa: E402
+from lskum import _lib as L  # noqa: E402
+from lskum.solver import Solver  # noqa: E402
+
+METRIC = "point-iterations/sec + HBM GB/s, FP64 vs mixed at 2.5M pts; adjoint grads/sec"
+HBM_PEAK_GBPS = 8000.0
+CASE = dict(mach=0.85, aoa=1.25, cfl=0.2, vl_const=5.0, power=0.0, inner_iterations=2)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def bytes_per_point_iteration(k, n_inner):
+    """Compulsory HBM bytes of one iteration per point (FP64 SoA, int32
+    indices, each array once per kernel, neighbour gathers counted once):
+    snapshot 64, func_delta 60 + 4k, and per stage q_variables 64,
+    q_derivatives 180 + 4k, n_inner x (180 + 4k), flux 260 + 8k, state 148."""
+    return 64 + (60 + 4 * k) + 4 * (64 + (180 + 4 * k) + n_inner * (180 + 4 * k) + (260 + 8 * k) + 148)
+
+
+def flux_bytes_per_point(k):
+    """Compulsory bytes of one flux-residual launch per point: q, dq, qm, x,
+    y, nx, ny, min_dist-derived eps, delta, flag in; flux_res out; plus the
+    2k sub-stencil index entries."""
+    return 260 + 8 * k
+
+
+def cpu_baseline(args):
+    """The reference's own serial fpi_solver (oracle/_ref/ref_driver, built
+    from the reference sources) on a bounded sample of the workload: a
+    100,400-point cloud of the same generator family (wall 400, 100 x 1,000
+    O-grid, k = 24) with the same numerics, 3 iterations, 1 core."""
+    sample = dict(naca="0012", n_wall=400, n_radial=100, n_circ=1000, stretch=0.0, k=24, seed=2)
+    cloud = lskum.Cloud.generate(**sample)
+    iters = 3
+    driver = os.path.join(REPO, "oracle", "_ref", "ref_driver")
+    desc = (f"{cloud.n}-point NACA0012 cloud of the same generator family (wall 400, "
+            f"100 x 1,000, k 24), same numerics, {iters} iterations, 1 thread")
+    with tempfile.TemporaryDirectory() as d:
+        if os.path.isfile(driver):
+            grid = os.path.join(d, "grid")
+            cloud.write_partgrid(grid)
+            cmd = [driver, "run", grid, d, "--iters", str(iters), "--cfl", "0.2", "--vl", "5",
+                   "--inner", "2", "--mach", "0.85", "--aoa", "1.25", "--dump", "0"]
+            subprocess.run(cmd, check=True, cwd=d, stdout=subprocess.DEVNULL,
+                           env=dict(os.environ, OMP_NUM_THREADS="1"))
+            secs = float(np.fromfile(os.path.join(d, "loop_seconds.bin"))[0])
+            kind = "reference"
+        else:
+            from oracle.oracle import Oracle
+            o = Oracle(cloud, mach=0.85, aoa=1.25, cfl=0.2, vl_const=5.0, inner_iterations=2,
+                       threads=1)
+            t = time.perf_counter()
+            o.q_lskum(iters)
+            secs = time.perf_counter() - t
+            kind = "port"
+    return {"value": cloud.n * iters / secs, "unit": "point-iterations/s", "cores": 1,
+            "kind": kind, "sample": desc, "seconds": round(secs, 3),
+            "cpu": platform.processor() or platform.machine()}
+
+
+def load_traffic(name):
+    """HBM bytes per flux launch measured by rocprofv3 --pmc (FETCH_SIZE and
+    WRITE_SIZE runs, copy-kernel calibrated), if committed under profiles/."""
+    p = os.path.join(REPO, "profiles", name)
+    if os.path.isfile(p):
+        with open(p) as f:
+            return json.load(f).get("traffic_bytes_per_launch")
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="naca0012_2p5m_k24")
+    ap.add_argument("--reorder", default="rcm", choices=["rcm", "none"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only; the q-LSKUM iteration would shard "
+              "by point partition with a one-ring halo, which is not built. Use --gpus 1.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    t = time.perf_counter()
+    cloud = lskum.workload(args.workload)
+    log(f"cloud {args.workload}: {cloud.n} points, {cloud.conn.size} neighbour entries, "
+        f"{time.perf_counter() - t:.1f}s")
+    k_mean = cloud.conn.size / cloud.n
+    reorder = L.REORDER_RCM if args.reorder == "rcm" else L.REORDER_NONE
+    t = time.perf_counter()
+    solver = Solver(cloud, reorder=reorder, **CASE)
+    log(f"context created in {time.perf_counter() - t:.1f}s")
+    if args.warmup:
+        solver.q_lskum(args.warmup)
+    solver.set_timing(True)
+    t0 = time.perf_counter()
+    rn, rs = solver.q_lskum(args.steps)  # synchronises its stream at the end
+    elapsed = time.perf_counter() - t0
+    flux_ms, flux_n = solver.kernel_time(L.K_FLUX)
+    kt = {name: solver.kernel_time(kid)[0] for name, kid in
+          [("flux", L.K_FLUX), ("q_derivatives", L.K_QDERIV), ("q_inner", L.K_QINNER),
+           ("state_update", L.K_STATE), ("func_delta", L.K_DELTA), ("q_variables", L.K_QVAR)]}
+    solver.set_timing(False)
+    cl, cd, cm = solver.compute_cl_cd_cm()
+    if not np.all(np.isfinite(rn)):
+        raise SystemExit("non-finite residual during the benchmark")
+
+    n = cloud.n
+    ms_per_step = 1e3 * elapsed / args.steps
+    value = n * args.steps / elapsed
+    flux_bytes = n * flux_bytes_per_point(k_mean)
+    achieved = flux_bytes / (flux_ms * 1e-3) / 1e9
+    it_bytes = n * bytes_per_point_iteration(k_mean, CASE["inner_iterations"])
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "point-iterations/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "points": n, "k": 24, "k_mean": round(k_mean, 3),
+                   "inner_iterations": 2, "mach": 0.85, "aoa_deg": 1.25, "cfl": 0.2,
+                   "vl_const": 5.0, "precision": "fp64", "order": args.reorder},
+        "roofline": {"bound": "hbm", "kernel": "k_flux", "achieved": achieved,
+                     "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS,
+                     "traffic": load_traffic("flux_naca0012_2p5m_k24_traffic.json"),
+                     "avg_ms": flux_ms, "launches": flux_n,
+                     "algorithmic_bytes_per_launch": flux_bytes},
+        "iteration_hbm_GBps": it_bytes * args.steps / elapsed / 1e9,
+        "kernel_avg_ms": kt,
+        "final": {"res_new": float(rn[-1]), "residue": float(rs[-1]), "cl": cl, "cd": cd, "cm": cm},
+    }
+    solver.close()
+    if not args.no_cpu_baseline:
+        log("cpu baseline: reference serial fpi_solver on a bounded sample ...")
+        out["cpu_baseline"] = cpu_baseline(args)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -283,6 +283,10 @@ struct Body {
   }
 };
 
+// sub-stencil balance of the neighbour selection
+constexpr int kMinSub = 3;
+constexpr double kMinRelDet = 0.05;
+
 double derive_stretch(int L, double h_first, double depth) {
   // solve h_first * (q^L - 1)/(q - 1) = depth for q > 1
   double target = depth / h_first;
@@ -393,7 +397,23 @@ int lskum_cloud_generate(const lskum_gen_params *p, lskum_cloud_h **out) {
       }
     }
   }
-  // neighbours: k nearest within radius_factor * h, ordered by (distance, index)
+  // neighbours: k nearest visible points within radius_factor * h, ordered
+  // by (distance, index), then balanced so that every sub-stencil the flux
+  // needs (x+/x- along the point's tangent; y+/y- for interior points) has
+  // at least kMinSub members and a least-squares matrix that is not nearly
+  // singular: nearest qualifying candidates within twice the radius replace
+  // the farthest members no sub-stencil depends on.
+  std::vector<double> nxv, nyv;
+  {
+    lskum_cloud v{};
+    v.n_points = n;
+    v.x = c->x.data();
+    v.y = c->y.data();
+    v.left = c->left.data();
+    v.right = c->right.data();
+    v.flag_1 = c->flag1.data();
+    lskum::compute_normals(v, nxv, nyv);
+  }
   KdTree tree(c->x, c->y);
   Body body(c->x, c->y, nw);
   auto visible = [&](int32_t a, int32_t b) { return body.visible(a, b); };
@@ -402,15 +422,89 @@ int lskum_cloud_generate(const lskum_gen_params *p, lskum_cloud_h **out) {
   const int k = p->k;
 #pragma omp parallel
   {
-    std::vector<std::pair<double, int32_t>> best;
-    best.reserve(k + 1);
+    std::vector<std::pair<double, int32_t>> cand;
+    cand.reserve(4 * k + 1);
+    std::vector<char> used;
 #pragma omp for schedule(dynamic, 4096)
     for (int32_t i = 0; i < n; i++) {
-      double rr = p->radius_factor * h[i];
-      tree.query(i, k, rr * rr, best, visible);
-      cnt[i] = (int32_t)best.size();
-      for (size_t t = 0; t < best.size(); t++) nb[(size_t)i * k + t] = best[t].second;
-      c->min_dist[i] = best.empty() ? 0.0 : std::sqrt(best[0].first);
+      const double rr = p->radius_factor * h[i];
+      tree.query(i, 4 * k, 4.0 * rr * rr, cand, visible);
+      const int nc = (int)cand.size();
+      used.assign(nc, 0);
+      // sub-stencil membership of each candidate in i's (t, n) frame
+      const double tx = nyv[i], ty = -nxv[i], nxi = nxv[i], nyi = nyv[i];
+      const int f = c->flag1[i];
+      auto bits = [&](int a) {
+        const int32_t q = cand[a].second;
+        const double dx = c->x[q] - c->x[i], dy = c->y[q] - c->y[i];
+        const double ds = dx * tx + dy * ty, dn = dx * nxi + dy * nyi;
+        unsigned b = 0;
+        if (ds <= 0.0) b |= 1u;
+        if (ds >= 0.0) b |= 2u;
+        if (f == 1) {
+          if (dn <= 0.0) b |= 4u;
+          if (dn >= 0.0) b |= 8u;
+        } else {
+          b |= (f == 0) ? 8u : 4u;
+        }
+        return b;
+      };
+      auto rel_det = [&](unsigned sb) {
+        double sxx = 0, syy = 0, sxy = 0;
+        int m = 0;
+        for (int a = 0; a < nc; a++) {
+          if (!used[a] || !(bits(a) & sb)) continue;
+          const int32_t q = cand[a].second;
+          const double dx = c->x[q] - c->x[i], dy = c->y[q] - c->y[i];
+          const double ds = dx * tx + dy * ty, dn = dx * nxi + dy * nyi;
+          sxx += ds * ds;
+          syy += dn * dn;
+          sxy += ds * dn;
+          m++;
+        }
+        if (m < kMinSub || !(sxx > 0.0 && syy > 0.0)) return -1.0;
+        return (sxx * syy - sxy * sxy) / (sxx * syy);
+      };
+      int taken = 0;
+      for (int a = 0; a < nc && taken < k; a++)
+        if (cand[a].first <= rr * rr) {
+          used[a] = 1;
+          taken++;
+        }
+      const unsigned need = (f == 1) ? 15u : (f == 0 ? 11u : 7u);
+      for (unsigned sb = 1; sb <= 8; sb <<= 1) {
+        if (!(need & sb)) continue;
+        for (int guard = 0; guard < nc && rel_det(sb) < kMinRelDet; guard++) {
+          int add = -1;
+          for (int a = 0; a < nc; a++)
+            if (!used[a] && (bits(a) & sb)) { add = a; break; }
+          if (add < 0) break;
+          used[add] = 1;
+          if (++taken <= k) continue;
+          // drop the farthest member whose removal keeps every needed
+          // sub-stencil at kMinSub members
+          for (int a = nc - 1; a >= 0; a--) {
+            if (!used[a] || a == add) continue;
+            used[a] = 0;
+            bool ok = true;
+            for (unsigned t = 1; t <= 8 && ok; t <<= 1) {
+              if (!(need & t) || !(bits(a) & t)) continue;
+              int m = 0;
+              for (int b2 = 0; b2 < nc; b2++) m += used[b2] && (bits(b2) & t);
+              ok = m >= kMinSub;
+            }
+            if (ok) { taken--; break; }
+            used[a] = 1;
+          }
+        }
+      }
+      int t = 0;
+      for (int a = 0; a < nc; a++)
+        if (used[a] && t < k) nb[(size_t)i * k + t++] = cand[a].second;
+      cnt[i] = t;
+      c->min_dist[i] = t ? std::sqrt(cand[0].first) : 0.0;
+      for (int a = 0; a < nc; a++)
+        if (used[a]) { c->min_dist[i] = std::sqrt(cand[a].first); break; }
     }
   }
   c->conn_off.assign(n + 1, 0);
@@ -536,6 +630,27 @@ int lskum_cloud_stats(const lskum_cloud_h *c, int32_t *min_nbhs,
       if (min_rel_det[t] < 0.0 || r < min_rel_det[t]) min_rel_det[t] = r;
     }
   }
+  return LSKUM_OK;
+}
+
+int lskum_cloud_connectivity(const lskum_cloud_h *c, double *nx, double *ny, uint8_t *mask) {
+  if (!c || !nx || !ny || !mask) return LSKUM_EINVAL;
+  lskum_cloud v = c->view();
+  std::vector<double> a, b;
+  std::vector<uint8_t> m;
+  lskum::compute_normals(v, a, b);
+  lskum::generate_connectivity(v, a, b, m);
+  std::copy(a.begin(), a.end(), nx);
+  std::copy(b.begin(), b.end(), ny);
+  std::copy(m.begin(), m.end(), mask);
+  return LSKUM_OK;
+}
+
+int lskum_cloud_rcm(const lskum_cloud_h *c, int32_t *perm) {
+  if (!c || !perm) return LSKUM_EINVAL;
+  std::vector<int32_t> p;
+  lskum::rcm_order(c->view(), p);
+  std::copy(p.begin(), p.end(), perm);
   return LSKUM_OK;
 }
 

@@ -68,6 +68,8 @@ _SIGS = [
     ("lskum_cloud_from_arrays", C.c_int, [C.POINTER(CloudView), C.POINTER(C.c_void_p)]),
     ("lskum_cloud_view", C.c_int, [C.c_void_p, C.POINTER(CloudView)]),
     ("lskum_cloud_stats", C.c_int, [C.c_void_p, _ip, _ip, _dp]),
+    ("lskum_cloud_connectivity", C.c_int, [C.c_void_p, _dp, _dp, C.POINTER(C.c_uint8)]),
+    ("lskum_cloud_rcm", C.c_int, [C.c_void_p, _ip]),
     ("lskum_cloud_free", None, [C.c_void_p]),
     ("lskum_create", C.c_int, [C.POINTER(CloudView), C.POINTER(Params), C.POINTER(C.c_void_p)]),
     ("lskum_destroy", None, [C.c_void_p]),

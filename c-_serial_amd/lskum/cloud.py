@@ -86,6 +86,30 @@ class Cloud:
         finally:
             L.lib().lskum_cloud_free(h)
 
+    def connectivity(self):
+        """(nx, ny, mask): host set-up as lskum_create does it; mask bits per
+        conn entry 1 xpos, 2 xneg, 4 ypos, 8 yneg."""
+        h = self._handle()
+        try:
+            nx = np.zeros(self.n)
+            ny = np.zeros(self.n)
+            mask = np.zeros(int(self.conn_off[-1]), np.uint8)
+            L.check(L.lib().lskum_cloud_connectivity(
+                h, L.dptr(nx), L.dptr(ny), mask.ctypes.data_as(C.POINTER(C.c_uint8))))
+            return nx, ny, mask
+        finally:
+            L.lib().lskum_cloud_free(h)
+
+    def rcm(self):
+        """Reverse Cuthill-McKee order, perm[new] = old."""
+        h = self._handle()
+        try:
+            perm = np.zeros(self.n, np.int32)
+            L.check(L.lib().lskum_cloud_rcm(h, L.iptr(perm)))
+            return perm
+        finally:
+            L.lib().lskum_cloud_free(h)
+
     def stats(self):
         """(min nbhs, max nbhs, min relative sub-stencil det per point type)."""
         h = self._handle()
@@ -128,7 +152,7 @@ WORKLOADS = {
     # small clouds for parity tests
     "naca0012_1480_k20": dict(naca="0012", n_wall=40, n_radial=24, n_circ=60,
                               stretch=1.2, k=20, seed=11),
-    "naca0012_3640_k24": dict(naca="0012", n_wall=80, n_radial=30, n_circ=118,
+    "naca0012_3620_k24": dict(naca="0012", n_wall=80, n_radial=30, n_circ=118,
                               stretch=0.0, k=24, seed=12),
 }
 

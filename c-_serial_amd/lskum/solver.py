@@ -1,0 +1,214 @@
+"""Python face of the C ABI, with the reference's stage names.
+
+``Solver`` owns one lskum_ctx.  Its methods are the reference's hot-path
+functions (fpi_solver_mod.h:32-61 call order), each one C-ABI call that runs
+HIP kernels on the context's stream:
+
+    func_delta            state_update_mod.h:448
+    eval_q_variables      q_variables_mod.h:9
+    eval_q_derivatives    q_variables_mod.h:36
+    eval_q_inner_loop     q_variables_mod.h:120 (+ eval_update_innerloop :215)
+    cal_flux_residual     flux_residual_mod.h:10
+    state_update(rk)      state_update_mod.h:92
+    compute_cl_cd_cm      compute_force_coeffs_mod.h:7
+    fpi_solver(it)        fpi_solver_mod.h:14
+    q_lskum(n)            q_lskum_mod.h:13
+
+Arrays come back in the caller's point order, 0-based, component-major.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+from .cloud import Cloud
+
+_SHAPES = {L.A_PRIM: (4,), L.A_PRIM_OLD: (4,), L.A_Q: (4,), L.A_DQ: (2, 4),
+           L.A_QM: (2, 4), L.A_DELTA: (), L.A_FLUX_RES: (4,), L.A_NORMALS: (2,)}
+
+
+class Solver:
+    def __init__(self, cloud: Cloud, mach=0.63, aoa=2.0, cfl=0.1, vl_const=150.0, power=0.0,
+                 inner_iterations=3, precision=L.FP64, reorder=L.REORDER_NONE, block=0):
+        self.cloud = cloud
+        self.n = cloud.n
+        self.params = L.Params(mach, aoa, cfl, vl_const, power, inner_iterations, precision,
+                               reorder, block)
+        self.inner_iterations = inner_iterations
+        view = cloud.view()
+        ctx = C.c_void_p()
+        rc = L.lib().lskum_create(C.byref(view), C.byref(self.params), C.byref(ctx))
+        self._ctx = ctx
+        if rc != L.LSKUM_OK:
+            msg = L.lib().lskum_last_error(ctx) if ctx else b""
+            if ctx:
+                L.lib().lskum_destroy(ctx)
+                self._ctx = None
+            raise L.LskumError(rc, (msg or b"").decode())
+        self.sum_res_sqr = 0.0
+        self.max_res = 0.0
+        self.max_res_point = 0
+        self.res_new = 0.0
+        self.res_old = 0.0
+        self.residue = 0.0
+        self.Cl = self.Cd = self.Cm = 0.0
+
+    def close(self):
+        if getattr(self, "_ctx", None):
+            L.lib().lskum_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _chk(self, rc):
+        L.check(rc, self._ctx)
+
+    # ---------------------------------------------------------- state I/O
+    def set_state(self, prim):
+        prim = np.ascontiguousarray(prim, dtype=np.float64).reshape(4, self.n)
+        self._chk(L.lib().lskum_set_state(self._ctx, L.dptr(prim)))
+
+    def get(self, which):
+        shape = _SHAPES[which] + (self.n,)
+        out = np.empty(shape, dtype=np.float64)
+        self._chk(L.lib().lskum_get_array(self._ctx, which, L.dptr(out)))
+        return out
+
+    @property
+    def prim(self):
+        return self.get(L.A_PRIM)
+
+    @prim.setter
+    def prim(self, value):
+        self.set_state(value)
+
+    @property
+    def q(self):
+        return self.get(L.A_Q)
+
+    @property
+    def dq(self):
+        return self.get(L.A_DQ)
+
+    @property
+    def qm(self):
+        return self.get(L.A_QM)
+
+    @property
+    def delta(self):
+        return self.get(L.A_DELTA)
+
+    @property
+    def flux_res(self):
+        return self.get(L.A_FLUX_RES)
+
+    @property
+    def normals(self):
+        return self.get(L.A_NORMALS)
+
+    def substencil(self, which):
+        """(counts, members) of sub-stencil 0 xpos, 1 xneg, 2 ypos, 3 yneg."""
+        counts = np.zeros(self.n, np.int32)
+        self._chk(L.lib().lskum_get_substencil(self._ctx, which, L.iptr(counts), None))
+        members = np.zeros(int(counts.sum()), np.int32)
+        if members.size:
+            self._chk(L.lib().lskum_get_substencil(self._ctx, which, L.iptr(counts),
+                                                   L.iptr(members)))
+        return counts, members
+
+    # ------------------------------------------------- reference stages
+    def snapshot(self):
+        self._chk(L.lib().lskum_snapshot(self._ctx))
+
+    def func_delta(self):
+        self._chk(L.lib().lskum_func_delta(self._ctx))
+
+    def eval_q_variables(self):
+        self._chk(L.lib().lskum_q_variables(self._ctx))
+
+    def eval_q_derivatives(self):
+        self._chk(L.lib().lskum_q_derivatives(self._ctx))
+
+    def eval_q_inner_loop(self, passes=1):
+        """One (or more) Jacobi passes; the reference's temp-then-copy
+        (eval_update_innerloop) is the kernels' buffer swap."""
+        self._chk(L.lib().lskum_q_inner(self._ctx, passes))
+
+    def eval_update_innerloop(self):
+        """No-op: eval_q_inner_loop already published the new derivatives."""
+
+    def cal_flux_residual(self):
+        self._chk(L.lib().lskum_flux_residual(self._ctx))
+
+    def state_update(self, rk):
+        s, m, p = C.c_double(), C.c_double(), C.c_int32()
+        self._chk(L.lib().lskum_state_update(self._ctx, rk, C.byref(s), C.byref(m), C.byref(p)))
+        self.sum_res_sqr, self.max_res, self.max_res_point = s.value, m.value, p.value
+        return self.sum_res_sqr, self.max_res, self.max_res_point
+
+    def compute_cl_cd_cm(self):
+        cl, cd, cm = C.c_double(), C.c_double(), C.c_double()
+        self._chk(L.lib().lskum_forces(self._ctx, C.byref(cl), C.byref(cd), C.byref(cm)))
+        self.Cl, self.Cd, self.Cm = cl.value, cd.value, cm.value
+        return self.Cl, self.Cd, self.Cm
+
+    def q_lskum(self, n_iter):
+        """n_iter fpi_solver iterations in one call; returns (res_new, residue)
+        per iteration."""
+        rn = np.zeros(max(n_iter, 1))
+        rs = np.zeros(max(n_iter, 1))
+        self._chk(L.lib().lskum_iterate(self._ctx, n_iter, L.dptr(rn), L.dptr(rs)))
+        if n_iter:
+            self.res_new, self.residue = rn[n_iter - 1], rs[n_iter - 1]
+            self.res_old = self.iteration()[1]
+        return rn[:n_iter], rs[:n_iter]
+
+    def fpi_solver(self, it=None):
+        """One iteration (fpi_solver_mod.h:14-150); `it` must be the next
+        iteration number when given."""
+        if it is not None and it != self.iteration()[0] + 1:
+            raise ValueError("fpi_solver(it) must be called with the next iteration number")
+        rn, rs = self.q_lskum(1)
+        return rn[0], rs[0]
+
+    def iteration(self):
+        it, ro = C.c_int32(), C.c_double()
+        self._chk(L.lib().lskum_get_iteration(self._ctx, C.byref(it), C.byref(ro)))
+        return it.value, ro.value
+
+    def set_iteration(self, it, res_old, restart=True):
+        self._chk(L.lib().lskum_set_iteration(self._ctx, it, res_old, 1 if restart else 0))
+
+    # ------------------------------------------------------------ timing
+    def set_timing(self, enable):
+        self._chk(L.lib().lskum_set_timing(self._ctx, 1 if enable else 0))
+
+    def kernel_time(self, kernel):
+        ms, cnt = C.c_double(), C.c_int32()
+        self._chk(L.lib().lskum_kernel_time(self._ctx, kernel, C.byref(ms), C.byref(cnt)))
+        return ms.value, cnt.value
+
+
+def kat_fluxes(rows6):
+    """Device flux functions on rows (nx, ny, u1, u2, rho, pr) -> [n, 8, 4]."""
+    rows6 = np.ascontiguousarray(rows6, dtype=np.float64)
+    n = rows6.shape[0]
+    out = np.zeros((n, 8, 4))
+    L.check(L.lib().lskum_kat_fluxes(n, L.dptr(rows6), L.dptr(out), 0))
+    return out
+
+
+def kat_qtilde(qt):
+    """Device qtilde_to_primitive on rows q~[4] -> [n, 4] (u1, u2, rho, pr)."""
+    qt = np.ascontiguousarray(qt, dtype=np.float64)
+    n = qt.shape[0]
+    out = np.zeros((n, 4))
+    L.check(L.lib().lskum_kat_qtilde(n, L.dptr(qt), L.dptr(out)))
+    return out

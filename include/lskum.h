@@ -67,7 +67,12 @@ typedef struct lskum_cloud_h lskum_cloud_h;
  * Neighbours: the k nearest visible points within radius_factor x local
  * spacing, ties broken by index; a pair is not visible when its segment
  * crosses the body outline or when both are wall points on opposite
- * surfaces.  Deterministic from seed.
+ * surfaces.  The selection is then balanced: every sub-stencil the flux
+ * uses gets at least 3 members and a relative least-squares determinant of
+ * at least 0.05, by swapping in the nearest qualifying points within twice
+ * the radius for the farthest members no sub-stencil needs.  Neighbours are
+ * listed by increasing distance; min_dist is the nearest one's distance.
+ * Deterministic from seed.
  */
 typedef struct lskum_gen_params {
   int32_t naca_m;   /* max camber, % chord (NACA 4-digit first digit)   */
@@ -99,6 +104,15 @@ int lskum_cloud_view(const lskum_cloud_h *c, lskum_cloud *view);
  * determinant relative to its scale; -1 if no point of the type. */
 int lskum_cloud_stats(const lskum_cloud_h *c, int32_t *min_nbhs,
                       int32_t *max_nbhs, double *min_rel_det);
+/* Host-only set-up of a cloud, as lskum_create does it: normals
+ * (point_normals_mod.h:17-55) and, per conn entry, the sub-stencil bits
+ * 1 xpos, 2 xneg, 4 ypos, 8 yneg (generate_connectivity_mod.h:5-249).
+ * nx, ny: n_points doubles; mask: conn_off[n_points] bytes. */
+int lskum_cloud_connectivity(const lskum_cloud_h *c, double *nx, double *ny,
+                             uint8_t *mask);
+/* Reverse Cuthill-McKee order of the symmetrised neighbour graph:
+ * perm[new] = old (n_points entries). */
+int lskum_cloud_rcm(const lskum_cloud_h *c, int32_t *perm);
 void lskum_cloud_free(lskum_cloud_h *c);
 
 /* ------------------------------------------------------------ parameters */

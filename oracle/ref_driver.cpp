@@ -30,6 +30,7 @@
 #include "q_lskum_mod.h"
 #include "compute_force_coeffs_mod.h"
 
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -208,6 +209,7 @@ static int cmd_run(int argc, char **argv) {
   std::streambuf *saved = std::cout.rdbuf();
   std::ofstream sink("/dev/null");
   std::cout.rdbuf(sink.rdbuf());
+  const auto t0 = std::chrono::steady_clock::now();
   for (it = 1; it <= iters; it++) {
     if (stages.count(it)) {
       char tag[64];
@@ -260,8 +262,11 @@ static int cmd_run(int argc, char **argv) {
       dump_state(tag, true);
     }
   }
+  const double secs =
+      std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   std::cout.rdbuf(saved);
   dump_raw("history", hist);
+  dump_raw("loop_seconds", {secs});
   return 0;
 }
 

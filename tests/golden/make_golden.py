@@ -1,0 +1,133 @@
+"""Writes the golden fixtures of tests/golden/ from the compiled reference.
+
+Run in the build container only (it needs the reference sources to build
+oracle/_ref/ref_driver):  python tests/golden/make_golden.py [--c1]
+
+Every fixture is data: the clouds come from the project generator (their
+partGrid SHA-256 is stored so a generator change is caught), the outputs
+come from the reference's own functions run by oracle/ref_driver.cpp.
+  kat_data.npz       first 1000 rows of the reference's data.dat and, per
+                     row, the eight flux functions and qtilde_to_primitive
+  small_c1.npz       naca0012_1480_k20, config-1 numerics (CFL 0.2, K 5, no
+                     inner passes), 40 iterations: history, final arrays,
+                     stage-by-stage flux_res/prim at iteration 30
+  k24_m085.npz       naca0012_3620_k24, M 0.85, AoA 1.25, 2 inner passes,
+                     20 iterations: history, final prim and flux_res
+  c1_1000.npz        naca0012_9760_k20, config 1, 1000 iterations: history,
+                     final prim and SHA-256 digests of every final array
+"""
+import argparse
+import hashlib
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(REPO, "c-_serial_amd"))
+import lskum  # noqa: E402
+
+REF = os.environ.get("LSKUM_REFERENCE", "/root/reference")
+DRIVER = os.path.join(REPO, "oracle", "_ref", "ref_driver")
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def grid_sha(cloud, path):
+    cloud.write_partgrid(path)
+    with open(path, "rb") as f:
+        return hashlib.sha256(f.read()).hexdigest()
+
+
+def run_ref(cloud, iters, cfl, vl, inner, extra=()):
+    d = tempfile.mkdtemp(prefix="lskum_ref_")
+    g = os.path.join(d, "grid.txt")
+    gsha = grid_sha(cloud, g)
+    cmd = [DRIVER, "run", g, d, "--iters", str(iters), "--cfl", repr(cfl), "--vl", repr(vl),
+           "--inner", str(inner), "--dump", str(iters), *extra]
+    subprocess.run(cmd, check=True, cwd=d)
+    load = lambda name: np.fromfile(os.path.join(d, name + ".bin"))
+    return d, gsha, load
+
+
+def kat():
+    rows = np.loadtxt(os.path.join(REF, "data.dat"), max_rows=1000)
+    d = tempfile.mkdtemp(prefix="lskum_kat_")
+    out = os.path.join(d, "kat.bin")
+    subprocess.run([DRIVER, "kat", os.path.join(REF, "data.dat"), "1000", out], check=True, cwd=d)
+    res = np.fromfile(out).reshape(1000, 36)
+    np.savez_compressed(os.path.join(HERE, "kat_data.npz"), rows=rows,
+                        fluxes=res[:, :32].reshape(1000, 8, 4), qtilde_prim=res[:, 32:])
+
+
+def small_c1():
+    cloud = lskum.workload("naca0012_1480_k20")
+    n = cloud.n
+    d, gsha, load = run_ref(cloud, 40, 0.2, 5.0, 0, ("--stages", "30"))
+    stage = {}
+    for rk in range(1, 5):
+        stage[f"rk{rk}_flux_res"] = load(f"it30_rk{rk}_flux_res").reshape(4, n)
+        stage[f"rk{rk}_prim"] = load(f"it30_rk{rk}_prim").reshape(4, n)
+        stage[f"rk{rk}_red"] = load(f"it30_rk{rk}_red")
+    stage["rk1_prim_in"] = load("it30_rk1_prim_in").reshape(4, n)
+    stage["rk1_q"] = load("it30_rk1_q").reshape(4, n)
+    stage["rk1_dq"] = load("it30_rk1_dq").reshape(2, 4, n)
+    stage["rk1_qm"] = load("it30_rk1_qm").reshape(2, 4, n)
+    stage["delta30"] = load("it30_delta")
+    np.savez_compressed(
+        os.path.join(HERE, "small_c1.npz"), grid_sha=gsha,
+        params=np.array([0.63, 2.0, 0.2, 5.0, 0.0, 0]),
+        history=load("history").reshape(-1, 5),
+        prim=load("it40_prim").reshape(4, n), q=load("it40_q").reshape(4, n),
+        dq=load("it40_dq").reshape(2, 4, n), qm=load("it40_qm").reshape(2, 4, n),
+        flux_res=load("it40_flux_res").reshape(4, n), delta=load("it40_delta"),
+        normals=load("normals").reshape(2, n),
+        sub_counts=load("substencil_counts").astype(np.int32).reshape(n, 4),
+        sub_members_sha=sha(load("substencil_members").astype(np.int32)),
+        **stage)
+
+
+def k24():
+    cloud = lskum.workload("naca0012_3620_k24")
+    n = cloud.n
+    d, gsha, load = run_ref(cloud, 20, 0.2, 5.0, 2, ("--mach", "0.85", "--aoa", "1.25"))
+    np.savez_compressed(
+        os.path.join(HERE, "k24_m085.npz"), grid_sha=gsha,
+        params=np.array([0.85, 1.25, 0.2, 5.0, 0.0, 2]),
+        history=load("history").reshape(-1, 5),
+        prim=load("it20_prim").reshape(4, n), flux_res=load("it20_flux_res").reshape(4, n),
+        sub_counts_sha=sha(load("substencil_counts").astype(np.int32)),
+        sub_members_sha=sha(load("substencil_members").astype(np.int32)))
+
+
+def c1():
+    cloud = lskum.workload("naca0012_9760_k20")
+    n = cloud.n
+    d, gsha, load = run_ref(cloud, 1000, 0.2, 5.0, 0)
+    final = {nm: load("it1000_" + nm) for nm in ["prim", "q", "dq", "qm", "flux_res", "delta"]}
+    np.savez_compressed(
+        os.path.join(HERE, "c1_1000.npz"), grid_sha=gsha,
+        params=np.array([0.63, 2.0, 0.2, 5.0, 0.0, 0]),
+        history=load("history").reshape(-1, 5), prim=final["prim"].reshape(4, n),
+        **{nm + "_sha": sha(v) for nm, v in final.items()},
+        sub_counts_sha=sha(load("substencil_counts").astype(np.int32)),
+        sub_members_sha=sha(load("substencil_members").astype(np.int32)))
+
+
+if __name__ == "__main__":
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--c1", action="store_true", help="also the 1000-iteration config-1 run")
+    ap.add_argument("--only", default="")
+    a = ap.parse_args()
+    if not os.path.isfile(DRIVER):
+        sys.exit("build the reference harness first: make -C oracle ref")
+    jobs = {"kat": kat, "small": small_c1, "k24": k24, "c1": c1}
+    todo = a.only.split(",") if a.only else ["kat", "small", "k24"] + (["c1"] if a.c1 else [])
+    for j in todo:
+        jobs[j]()
+        print("wrote", j, flush=True)

@@ -1,0 +1,215 @@
+"""Host side of the product on the CPU: ABI exports, clouds, partGrid I/O,
+set-up (normals, sub-stencils, RCM) and the create-time validation paths.
+No GPU call is made here."""
+import ctypes as C
+import hashlib
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import lskum
+from lskum import _lib as L
+from conftest import REPO, golden
+from oracle import oracle as O
+
+
+def test_library_exports_every_declared_symbol():
+    header = open(os.path.join(REPO, "include", "lskum.h")).read()
+    declared = set(re.findall(r"^\s*(?:int|void|const char|void \*)\s*\*?\s*(lskum_\w+)\(",
+                              header, re.M))
+    assert len(declared) >= 30
+    handle = C.CDLL(L.LIB_PATH)
+    for name in sorted(declared):
+        assert hasattr(handle, name), name
+    out = subprocess.run(["nm", "-D", "--defined-only", L.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    exported = set(re.findall(r" T (lskum_\w+)$", out, re.M))
+    assert declared <= exported
+    # and the Python binding covers the whole ABI
+    assert declared == set(L.exported_symbols())
+
+
+def test_generator_is_deterministic_and_seeded():
+    a = lskum.workload("naca0012_1480_k20")
+    b = lskum.workload("naca0012_1480_k20")
+    for f in a.__dataclass_fields__:
+        assert np.array_equal(getattr(a, f), getattr(b, f)), f
+    c = lskum.Cloud.generate(**dict(lskum.WORKLOADS["naca0012_1480_k20"], seed=99))
+    assert not np.array_equal(a.x, c.x)
+
+
+def test_generated_cloud_layout():
+    c = lskum.workload("naca0012_9760_k20")
+    assert c.n == 160 + 64 * 150
+    assert (c.flag_1[:160] == 0).all() and (c.flag_2[:160] == 1).all()
+    assert (c.flag_1[160:160 + 63 * 150] == 1).all() and (c.flag_1[-150:] == 2).all()
+    nb = c.nbhs()
+    assert nb.max() == 20 and nb.min() >= 15
+    # neighbours sorted by distance, no self, no duplicates, min_dist = nearest
+    for i in range(0, c.n, 101):
+        k = c.conn[c.conn_off[i]:c.conn_off[i + 1]]
+        dx, dy = c.x[k] - c.x[i], c.y[k] - c.y[i]
+        d = np.sqrt(dx * dx + dy * dy)
+        assert i not in k and len(set(k)) == len(k)
+        assert np.all(np.diff(d) >= 0)
+        assert c.min_dist[i] == d[0]
+    # outer ring on the far-field circle, clockwise
+    r = np.hypot(c.x[-150:] - 0.5, c.y[-150:])
+    np.testing.assert_allclose(r, 20.0, rtol=1e-12)
+    lo, hi, dets = c.stats()
+    assert min(dets) > 0.01
+
+
+def test_no_neighbour_sees_through_the_body():
+    c = lskum.workload("naca0012_9760_k20")
+    nw = 160
+    side = np.where(np.arange(nw) == 0, 0, np.where(np.arange(nw) < nw // 2, -1, 1))
+    side[nw // 2] = 0
+    for i in range(nw):
+        k = c.conn[c.conn_off[i]:c.conn_off[i + 1]]
+        kw = k[k < nw]
+        assert np.all(side[i] * side[kw] >= 0), i
+
+
+def test_partgrid_round_trip(tmp_path):
+    c = lskum.workload("naca0012_1480_k20")
+    p = tmp_path / "grid"
+    c.write_partgrid(p)
+    d = lskum.Cloud.read_partgrid(p)
+    for f in c.__dataclass_fields__:
+        assert np.array_equal(getattr(c, f), getattr(d, f)), f
+    # 1-based in the file, as main.cpp:30-58 reads it
+    first = p.read_text().splitlines()[1].split()
+    assert int(first[2]) == c.left[0] + 1 and int(first[7]) == c.nbhs()[0]
+
+
+def test_partgrid_reader_errors(tmp_path):
+    with pytest.raises(L.LskumError) as e:
+        lskum.Cloud.read_partgrid(tmp_path / "missing")
+    assert e.value.code == -4
+    c = lskum.workload("naca0012_1480_k20")
+    p = tmp_path / "grid"
+    c.write_partgrid(p)
+    text = p.read_text()
+    (tmp_path / "trunc").write_text(text[: len(text) // 2])
+    with pytest.raises(L.LskumError) as e:
+        lskum.Cloud.read_partgrid(tmp_path / "trunc")
+    assert e.value.code == -4
+    lines = text.splitlines()
+    f = lines[5].split()
+    f[4] = "7"  # flag_1 outside 0..2
+    lines[5] = " ".join(f)
+    (tmp_path / "badflag").write_text("\n".join(lines) + "\n")
+    with pytest.raises(L.LskumError):
+        lskum.Cloud.read_partgrid(tmp_path / "badflag")
+
+
+def _sub_lists(mask, cloud, which):
+    bit = 1 << which
+    counts = np.zeros(cloud.n, np.int32)
+    mem = []
+    for i in range(cloud.n):
+        s, e = cloud.conn_off[i], cloud.conn_off[i + 1]
+        sel = cloud.conn[s:e][(mask[s:e] & bit) != 0]
+        counts[i] = len(sel)
+        mem.append(sel)
+    return counts, np.concatenate(mem)
+
+
+@pytest.mark.parametrize("name,fixture", [("naca0012_1480_k20", "small_c1.npz"),
+                                          ("naca0012_3620_k24", "k24_m085.npz")])
+def test_host_substencils_equal_oracle_and_reference(name, fixture):
+    cloud = lskum.workload(name)
+    nx, ny, mask = cloud.connectivity()
+    o = O.Oracle(cloud)
+    assert np.array_equal(nx, o.get("nx")) and np.array_equal(ny, o.get("ny"))
+    per_point = [[] for _ in range(cloud.n)]
+    all_counts = []
+    for s in range(4):
+        counts, mem = _sub_lists(mask, cloud, s)
+        oc, om = o.substencil(s)
+        assert np.array_equal(counts, oc) and np.array_equal(mem, om), s
+        all_counts.append(counts)
+        off = np.concatenate([[0], np.cumsum(counts)])
+        for i in range(cloud.n):
+            per_point[i].append(mem[off[i]:off[i + 1]])
+    g = golden(fixture)
+    members = np.concatenate([m for p in per_point for m in p]).astype(np.int32)
+    assert hashlib.sha256(members.tobytes()).hexdigest() == str(g["sub_members_sha"])
+
+
+def test_rcm_is_a_permutation_and_narrows_a_shuffled_cloud():
+    c = lskum.workload("naca0012_1480_k20")
+    rng = np.random.default_rng(0)
+    shuf = rng.permutation(c.n).astype(np.int32)  # new -> old
+    inv = np.empty_like(shuf)
+    inv[shuf] = np.arange(c.n, dtype=np.int32)
+    # shuffled copy of the cloud (left/right and conn relabelled)
+    conn_lists = [inv[c.conn[c.conn_off[o]:c.conn_off[o + 1]]] for o in shuf]
+    off = np.concatenate([[0], np.cumsum([len(x) for x in conn_lists])]).astype(np.int32)
+    relabel = lambda a: np.where(a >= 0, inv[np.maximum(a, 0)], -1).astype(np.int32)
+    s = lskum.Cloud(c.x[shuf], c.y[shuf], relabel(c.left[shuf]), relabel(c.right[shuf]),
+                    c.flag_1[shuf], c.flag_2[shuf], c.min_dist[shuf], off,
+                    np.concatenate(conn_lists).astype(np.int32))
+    perm = s.rcm()
+    assert np.array_equal(np.sort(perm), np.arange(c.n))
+
+    def bandwidth(cloud, order):
+        pos = np.empty(cloud.n, np.int64)
+        pos[order] = np.arange(cloud.n)
+        rows = np.repeat(np.arange(cloud.n), np.diff(cloud.conn_off))
+        return np.abs(pos[rows] - pos[cloud.conn]).max()
+
+    assert bandwidth(s, perm) < bandwidth(s, np.arange(c.n)) / 5
+
+
+def _params(**kw):
+    p = dict(mach=0.63, aoa=2.0, cfl=0.2, vl_const=5.0, power=0.0, inner_iterations=0,
+             precision=0, reorder=0, block=0)
+    p.update(kw)
+    return L.Params(p["mach"], p["aoa"], p["cfl"], p["vl_const"], p["power"],
+                    p["inner_iterations"], p["precision"], p["reorder"], p["block"])
+
+
+def _create(cloud, params):
+    v = cloud.view()
+    ctx = C.c_void_p()
+    rc = L.lib().lskum_create(C.byref(v), C.byref(params), C.byref(ctx))
+    msg = L.lib().lskum_last_error(ctx).decode() if ctx else ""
+    if ctx:
+        L.lib().lskum_destroy(ctx)
+    return rc, msg
+
+
+def test_create_rejects_bad_parameters_before_touching_the_device():
+    c = lskum.workload("naca0012_1480_k20")
+    for bad in [dict(mach=-1.0), dict(cfl=0.0), dict(inner_iterations=-1), dict(precision=7),
+                dict(reorder=5), dict(block=100)]:
+        rc, msg = _create(c, _params(**bad))
+        assert rc == -1, bad
+        assert msg
+
+
+def test_create_rejects_degenerate_substencils():
+    c = lskum.workload("naca0012_1480_k20")
+    # make interior point 500 see only neighbours to its left (x+ only, x- empty)
+    i = 500
+    s, e = c.conn_off[i], c.conn_off[i + 1]
+    k = c.conn[s:e]
+    left = k[c.x[k] < c.x[i]]
+    conn = np.concatenate([c.conn[:s], np.resize(left, e - s), c.conn[e:]]).astype(np.int32)
+    bad = lskum.Cloud(c.x, c.y, c.left, c.right, c.flag_1, c.flag_2, c.min_dist, c.conn_off, conn)
+    rc, msg = _create(bad, _params())
+    assert rc == -6 and "point 500" in msg
+
+
+def test_create_rejects_out_of_range_neighbours():
+    c = lskum.workload("naca0012_1480_k20")
+    conn = c.conn.copy()
+    conn[10] = c.n + 5
+    bad = lskum.Cloud(c.x, c.y, c.left, c.right, c.flag_1, c.flag_2, c.min_dist, c.conn_off, conn)
+    rc, _ = _create(bad, _params())
+    assert rc == -1
