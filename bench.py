@@ -4,7 +4,7 @@
 A step is one fpi_solver iteration (snapshot, local time step, four SSPRK
 stages of q-variables, q-derivatives, 2 inner passes, flux residual and state
 update) over the 2,502,000-point NACA0012 cloud (wall 2,000, 500 x 5,000
-O-grid, k = 24, RCM order), M 0.85, AoA 1.25 deg, CFL 0.2, K 5.  Inputs are
+O-grid, k = 24), M 0.85, AoA 1.25 deg, CFL 0.2, K 5.  Inputs are
 resident in HBM before the timed region.  Prints one JSON line.
 
   python bench.py [--gpus 1] [--steps 50] [--warmup 5] [--no-cpu-baseline]
@@ -102,7 +102,7 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="naca0012_2p5m_k24")
-    ap.add_argument("--reorder", default="rcm", choices=["rcm", "none"])
+    ap.add_argument("--reorder", default="hilbert", choices=["hilbert", "rcm", "none"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -116,10 +116,13 @@ def main():
     log(f"cloud {args.workload}: {cloud.n} points, {cloud.conn.size} neighbour entries, "
         f"{time.perf_counter() - t:.1f}s")
     k_mean = cloud.conn.size / cloud.n
-    reorder = L.REORDER_RCM if args.reorder == "rcm" else L.REORDER_NONE
+    reorder = {"hilbert": L.REORDER_HILBERT, "rcm": L.REORDER_RCM,
+               "none": L.REORDER_NONE}[args.reorder]
     t = time.perf_counter()
     solver = Solver(cloud, reorder=reorder, **CASE)
-    log(f"context created in {time.perf_counter() - t:.1f}s")
+    tiled, tile_max, _ = solver.layout()
+    log(f"context created in {time.perf_counter() - t:.1f}s; LDS tiles {tiled}, "
+        f"largest tile {tile_max} points")
     if args.warmup:
         solver.q_lskum(args.warmup)
     solver.set_timing(True)
@@ -156,7 +159,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": args.workload, "points": n, "k": 24, "k_mean": round(k_mean, 3),
                    "inner_iterations": 2, "mach": 0.85, "aoa_deg": 1.25, "cfl": 0.2,
-                   "vl_const": 5.0, "precision": "fp64", "order": args.reorder},
+                   "vl_const": 5.0, "precision": "fp64", "order": args.reorder,
+                   "lds_tiles": tiled, "tile_max": tile_max},
         "roofline": {"bound": "hbm", "kernel": "k_flux", "achieved": achieved,
                      "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS,
                      "traffic": load_traffic("flux_naca0012_2p5m_k24_traffic.json"),

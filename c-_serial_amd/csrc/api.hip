@@ -32,12 +32,19 @@ struct lskum_ctx {
   std::vector<int32_t> conn_off_h, conn_h;
   std::vector<uint8_t> mask_h;
   std::vector<double> nx_h, ny_h;
-  // device state
-  double *rec = nullptr, *alt = nullptr, *prim = nullptr, *prim_old = nullptr;
+  // device state: hot lines A/B (which one holds the current derivatives
+  // is `cur`), qmax/qmin, primitive state
+  double *hot[2] = {nullptr, nullptr}, *qm = nullptr;
+  int cur = 0;
+  double *prim = nullptr, *prim_old = nullptr;
   double *flux_res = nullptr, *delta = nullptr, *nx = nullptr, *ny = nullptr;
   uint8_t *flag = nullptr;
   int32_t *conn_off = nullptr, *perm_d = nullptr;
   uint32_t *conn = nullptr;
+  uint16_t *conn16 = nullptr;
+  int32_t *tile_off = nullptr, *tile_idx = nullptr;
+  int tile_max = 0;
+  bool tiled = false;
   Partials part{};
   double *red = nullptr;      // [3] sum, max, argmax of the last state update
   double *hist = nullptr;     // per-iteration sum_res_sqr of lskum_iterate
@@ -67,6 +74,11 @@ struct lskum_ctx {
     c.n = n;
     c.conn_off = conn_off;
     c.conn = conn;
+    c.conn16 = conn16;
+    c.tile_off = tile_off;
+    c.tile_idx = tile_idx;
+    c.tile_max = tile_max;
+    c.tiled = tiled;
     c.nx = nx;
     c.ny = ny;
     c.flag = flag;
@@ -106,7 +118,7 @@ hipError_t dalloc(T **p, size_t count) {
 
 void release(lskum_ctx *c) {
   if (!c) return;
-  void *ptrs[] = {c->rec, c->alt, c->prim, c->prim_old, c->flux_res, c->delta, c->nx,
+  void *ptrs[] = {c->hot[0], c->hot[1], c->qm, c->conn16, c->tile_off, c->tile_idx, c->prim, c->prim_old, c->flux_res, c->delta, c->nx,
                   c->ny, c->flag, c->conn_off, c->perm_d, c->conn, c->part.sum,
                   c->part.max, c->part.arg, c->red, c->hist, c->f_m, c->f_l, c->f_r,
                   c->f_s, c->forces};
@@ -168,7 +180,8 @@ int do_snapshot(lskum_ctx *c) {
 int do_func_delta(lskum_ctx *c) {
   {
     KTimer t(c, LSKUM_K_DELTA);
-    launch_func_delta(c->dc(), c->rec, c->prim, c->prm.cfl, c->delta, c->block, c->stream);
+    launch_speed(c->dc(), c->prim, c->hot[0], c->stream);
+    launch_func_delta(c->dc(), c->hot[0], c->prm.cfl, c->delta, c->stream);
   }
   return check_launch(c, "func_delta");
 }
@@ -176,41 +189,27 @@ int do_func_delta(lskum_ctx *c) {
 int do_q_variables(lskum_ctx *c) {
   {
     KTimer t(c, LSKUM_K_QVAR);
-    launch_q_variables(c->dc(), c->prim, c->rec, c->block, c->stream);
+    launch_q_variables(c->dc(), c->prim, c->hot[0], c->hot[1], c->stream);
   }
   return check_launch(c, "q_variables");
 }
 
-// derivatives then n_inner Jacobi passes, ping-ponging between the record
-// and the alternate buffer so that the last pass lands in the record
-int do_q_derivatives_and_inner(lskum_ctx *c, int n_inner, bool deriv) {
-  double *rec_dq = c->rec + R_DQX;
-  bool in_rec = (n_inner % 2) == 0;
-  if (deriv) {
+// derivatives into the buffer from which n_later Jacobi passes end in A
+int do_q_derivatives(lskum_ctx *c, int n_later) {
+  c->cur = n_later % 2;
+  {
     KTimer t(c, LSKUM_K_QDERIV);
-    launch_q_derivatives(c->dc(), c->rec, in_rec ? rec_dq : c->alt, in_rec ? R_STRIDE : ALT_STRIDE,
-                         c->prm.power, c->block, c->stream);
-  } else if (!in_rec) {
-    // the current derivatives sit in the record: start the chain from there
-    in_rec = true;
+    launch_q_derivatives(c->dc(), c->hot[c->cur], c->qm, c->prm.power, c->stream);
   }
-  int rc = check_launch(c, "q_derivatives");
-  if (rc) return rc;
+  return check_launch(c, "q_derivatives");
+}
+
+// n Jacobi passes, each reading hot[cur] and writing the other buffer
+int do_q_inner(lskum_ctx *c, int n_inner) {
   for (int p = 0; p < n_inner; p++) {
     KTimer t(c, LSKUM_K_QINNER);
-    if (in_rec)
-      launch_q_inner(c->dc(), c->rec, rec_dq, R_STRIDE, c->alt, ALT_STRIDE, c->prm.power,
-                     c->block, c->stream);
-    else
-      launch_q_inner(c->dc(), c->rec, c->alt, ALT_STRIDE, rec_dq, R_STRIDE, c->prm.power,
-                     c->block, c->stream);
-    in_rec = !in_rec;
-  }
-  if (!in_rec) {
-    // odd pass count from the record: copy back (only for lskum_q_inner calls)
-    HIP_TRY(c, hipMemcpy2DAsync(rec_dq, sizeof(double) * R_STRIDE, c->alt,
-                                sizeof(double) * ALT_STRIDE, sizeof(double) * 8, c->n,
-                                hipMemcpyDeviceToDevice, c->stream));
+    launch_q_inner(c->dc(), c->hot[c->cur], c->hot[1 - c->cur], c->prm.power, c->stream);
+    c->cur = 1 - c->cur;
   }
   return check_launch(c, "q_inner");
 }
@@ -218,7 +217,7 @@ int do_q_derivatives_and_inner(lskum_ctx *c, int n_inner, bool deriv) {
 int do_flux(lskum_ctx *c) {
   {
     KTimer t(c, LSKUM_K_FLUX);
-    launch_flux(c->dc(), c->rec, c->delta, c->prm.power, c->flux_res, c->block, c->stream);
+    launch_flux(c->dc(), c->hot[c->cur], c->qm, c->delta, c->prm.power, c->flux_res, c->stream);
   }
   return check_launch(c, "flux_residual");
 }
@@ -227,7 +226,7 @@ int do_state_update(lskum_ctx *c, int rk, double *hist_slot) {
   {
     KTimer t(c, LSKUM_K_STATE);
     launch_state_update(c->dc(), rk, c->prim, c->prim_old, c->flux_res, c->fc(), c->part,
-                        c->block, c->stream);
+                        c->stream);
     launch_reduce(c->part, c->red, hist_slot, c->stream);
   }
   return check_launch(c, "state_update");
@@ -243,12 +242,13 @@ void permute_out(const lskum_ctx *c, const double *src, double *dst, int ncomp) 
     for (int d = 0; d < c->n; d++) dst[(size_t)r * c->n + c->perm[d]] = src[(size_t)r * c->n + d];
 }
 
-// fetch ncomp strided slots of the record: out[r][dev]
-int fetch_record(lskum_ctx *c, int slot, int ncomp, std::vector<double> &out) {
+// fetch ncomp consecutive slots of a per-point record array: out[r][dev]
+int fetch_record(lskum_ctx *c, const double *base, int stride, int slot, int ncomp,
+                 std::vector<double> &out) {
   out.assign((size_t)ncomp * c->n, 0.0);
   std::vector<double> tmp((size_t)ncomp * c->n);
-  HIP_TRY(c, hipMemcpy2DAsync(tmp.data(), sizeof(double) * ncomp, c->rec + slot,
-                              sizeof(double) * R_STRIDE, sizeof(double) * ncomp, c->n,
+  HIP_TRY(c, hipMemcpy2DAsync(tmp.data(), sizeof(double) * ncomp, base + slot,
+                              sizeof(double) * stride, sizeof(double) * ncomp, c->n,
                               hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   for (int d = 0; d < c->n; d++)
@@ -284,7 +284,7 @@ int lskum_create(const lskum_cloud *cloud, const lskum_params *params, lskum_ctx
   if (rc == LSKUM_OK && !(p.mach > 0.0 && p.cfl > 0.0 && p.vl_const >= 0.0 &&
                           p.inner_iterations >= 0 &&
                           (p.precision == LSKUM_FP64 || p.precision == LSKUM_MIXED) &&
-                          (p.reorder == LSKUM_REORDER_NONE || p.reorder == LSKUM_REORDER_RCM) &&
+                          (p.reorder >= LSKUM_REORDER_NONE && p.reorder <= LSKUM_REORDER_HILBERT) &&
                           (p.block == 0 || p.block == 64 || p.block == 128 || p.block == 256))) {
     rc = LSKUM_EINVAL;
     err = "invalid parameters";
@@ -320,34 +320,38 @@ int lskum_create(const lskum_cloud *cloud, const lskum_params *params, lskum_ctx
   }
   c->conn_off_h.assign(cloud->conn_off, cloud->conn_off + n + 1);
   c->conn_h.assign(cloud->conn, cloud->conn + cloud->conn_off[n]);
-  // device order: optional RCM, then grouped by point type (wall, interior,
-  // outer) so that waves rarely mix flux variants
+  // device order: Hilbert (compact 256-point blocks whose neighbour sets fit
+  // an LDS tile), or RCM / the caller's order grouped by point type
   std::vector<int32_t> base;
-  if (p.reorder == LSKUM_REORDER_RCM) {
-    rcm_order(*cloud, base);
+  if (p.reorder == LSKUM_REORDER_HILBERT) {
+    hilbert_order(*cloud, c->perm);
   } else {
-    base.resize(n);
-    for (int32_t i = 0; i < n; i++) base[i] = i;
+    if (p.reorder == LSKUM_REORDER_RCM) {
+      rcm_order(*cloud, base);
+    } else {
+      base.resize(n);
+      for (int32_t i = 0; i < n; i++) base[i] = i;
+    }
+    c->perm.clear();
+    c->perm.reserve(n);
+    for (int t = 0; t < 3; t++)
+      for (int32_t v : base)
+        if (cloud->flag_1[v] == t) c->perm.push_back(v);
   }
-  c->perm.clear();
-  c->perm.reserve(n);
-  for (int t = 0; t < 3; t++)
-    for (int32_t v : base)
-      if (cloud->flag_1[v] == t) c->perm.push_back(v);
   c->inv.assign(n, 0);
   for (int32_t d = 0; d < n; d++) c->inv[c->perm[d]] = d;
   // host images of the device arrays
-  std::vector<double> rec_h((size_t)R_STRIDE * n, 0.0), nxd(n), nyd(n);
+  std::vector<double> hot_h((size_t)H_STRIDE * n, 0.0), nxd(n), nyd(n);
   std::vector<uint8_t> flag_h(n);
   std::vector<int32_t> off_h(n + 1, 0);
   std::vector<uint32_t> conn_d(cloud->conn_off[n]);
   for (int32_t d = 0; d < n; d++) {
     const int32_t v = c->perm[d];
-    rec_h[(size_t)R_STRIDE * d + R_X] = cloud->x[v];
-    rec_h[(size_t)R_STRIDE * d + R_Y] = cloud->y[v];
+    hot_h[(size_t)H_STRIDE * d + H_X] = cloud->x[v];
+    hot_h[(size_t)H_STRIDE * d + H_Y] = cloud->y[v];
     // limiters_mod.h:82-83: epsi = pow(VL_CONST*min_dist, 3), used squared
     const double epsi = std::pow(p.vl_const * cloud->min_dist[v], 3);
-    rec_h[(size_t)R_STRIDE * d + R_EPS2] = epsi * epsi;
+    hot_h[(size_t)H_STRIDE * d + H_EPS2] = epsi * epsi;
     nxd[d] = c->nx_h[v];
     nyd[d] = c->ny_h[v];
     flag_h[d] = (uint8_t)cloud->flag_1[v];
@@ -356,6 +360,56 @@ int lskum_create(const lskum_cloud *cloud, const lskum_params *params, lskum_ctx
     for (int32_t t = b; t < e; t++)
       conn_d[off_h[d] + (t - b)] =
           (uint32_t)c->inv[cloud->conn[t]] | ((uint32_t)c->mask_h[t] << kMaskShift);
+  }
+  // LDS tiles: per block of kTileBlock points, its own points then the
+  // sorted other points its lists reach; conn16 holds local indices
+  const int nblk = blocks_for(n, kTileBlock);
+  std::vector<int32_t> toff(nblk + 1, 0), tidx;
+  std::vector<uint16_t> conn16_h(conn_d.size());
+  bool tiles_ok = true;
+  {
+    std::vector<int32_t> loc(n, -1), extra;
+    for (int blk = 0; blk < nblk && tiles_ok; blk++) {
+      const int32_t p0 = blk * kTileBlock, p1 = std::min(n, p0 + kTileBlock);
+      extra.clear();
+      for (int32_t d = p0; d < p1; d++)
+        for (int32_t e = off_h[d]; e < off_h[d + 1]; e++) {
+          const int32_t k = (int32_t)(conn_d[e] & kIndexMask);
+          if (k < p0 || k >= p1) extra.push_back(k);
+        }
+      std::sort(extra.begin(), extra.end());
+      extra.erase(std::unique(extra.begin(), extra.end()), extra.end());
+      const int32_t size = (p1 - p0) + (int32_t)extra.size();
+      if (size > (int32_t)kLocalMask + 1) {
+        tiles_ok = false;
+        break;
+      }
+      for (int32_t d = p0; d < p1; d++) {
+        loc[d] = d - p0;
+        tidx.push_back(d);
+      }
+      for (size_t j = 0; j < extra.size(); j++) {
+        loc[extra[j]] = (p1 - p0) + (int32_t)j;
+        tidx.push_back(extra[j]);
+      }
+      for (int32_t d = p0; d < p1; d++)
+        for (int32_t e = off_h[d]; e < off_h[d + 1]; e++) {
+          const uint32_t w = conn_d[e];
+          conn16_h[e] = (uint16_t)(loc[w & kIndexMask] |
+                                   ((w >> kMaskShift) << kLocalShift));
+        }
+      toff[blk + 1] = (int32_t)tidx.size();
+      c->tile_max = std::max(c->tile_max, size);
+      for (int32_t d = p0; d < p1; d++) loc[d] = -1;
+      for (int32_t k : extra) loc[k] = -1;
+    }
+  }
+  if (tiles_ok) {
+    int dev = 0, lds_limit = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&lds_limit, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) ==
+            hipSuccess)
+      c->tiled = prepare_tiled(c->tile_max, (size_t)lds_limit);
   }
   // shape points for the force reduction: the first shape_points points
   // (compute_force_coeffs_mod.h:25-27 takes m = j)
@@ -387,8 +441,9 @@ int lskum_create(const lskum_cloud *cloud, const lskum_params *params, lskum_ctx
     return fail(LSKUM_EHIP);
   }
   c->part.nblocks = blocks_for(n, 256);
-  ALLOC(c->rec, (size_t)R_STRIDE * n);
-  ALLOC(c->alt, (size_t)ALT_STRIDE * n);
+  ALLOC(c->hot[0], (size_t)H_STRIDE * n);
+  ALLOC(c->hot[1], (size_t)H_STRIDE * n);
+  ALLOC(c->qm, (size_t)QM_STRIDE * n);
   ALLOC(c->prim, (size_t)4 * n);
   ALLOC(c->prim_old, (size_t)4 * n);
   ALLOC(c->flux_res, (size_t)4 * n);
@@ -399,6 +454,11 @@ int lskum_create(const lskum_cloud *cloud, const lskum_params *params, lskum_ctx
   ALLOC(c->conn_off, (size_t)n + 1);
   ALLOC(c->perm_d, (size_t)n);
   ALLOC(c->conn, conn_d.size());
+  if (c->tiled) {
+    ALLOC(c->conn16, conn16_h.size());
+    ALLOC(c->tile_off, toff.size());
+    ALLOC(c->tile_idx, tidx.size());
+  }
   ALLOC(c->part.sum, (size_t)c->part.nblocks);
   ALLOC(c->part.max, (size_t)c->part.nblocks);
   ALLOC(c->part.arg, (size_t)c->part.nblocks);
@@ -414,17 +474,23 @@ int lskum_create(const lskum_cloud *cloud, const lskum_params *params, lskum_ctx
     return fail(LSKUM_ENOMEM);
   }
   struct Up { void *dst; const void *src; size_t bytes; };
-  Up ups[] = {{c->rec, rec_h.data(), rec_h.size() * sizeof(double)},
-              {c->nx, nxd.data(), n * sizeof(double)},
-              {c->ny, nyd.data(), n * sizeof(double)},
-              {c->flag, flag_h.data(), (size_t)n},
-              {c->conn_off, off_h.data(), (n + 1) * sizeof(int32_t)},
-              {c->perm_d, c->perm.data(), n * sizeof(int32_t)},
-              {c->conn, conn_d.data(), conn_d.size() * sizeof(uint32_t)},
-              {c->f_m, fm.data(), fm.size() * sizeof(int32_t)},
-              {c->f_l, fl.data(), fl.size() * sizeof(int32_t)},
-              {c->f_r, fr.data(), fr.size() * sizeof(int32_t)},
-              {c->f_s, fs.data(), fs.size() * sizeof(int32_t)}};
+  std::vector<Up> ups = {{c->hot[0], hot_h.data(), hot_h.size() * sizeof(double)},
+                         {c->hot[1], hot_h.data(), hot_h.size() * sizeof(double)},
+                         {c->nx, nxd.data(), n * sizeof(double)},
+                         {c->ny, nyd.data(), n * sizeof(double)},
+                         {c->flag, flag_h.data(), (size_t)n},
+                         {c->conn_off, off_h.data(), (n + 1) * sizeof(int32_t)},
+                         {c->perm_d, c->perm.data(), n * sizeof(int32_t)},
+                         {c->conn, conn_d.data(), conn_d.size() * sizeof(uint32_t)},
+                         {c->f_m, fm.data(), fm.size() * sizeof(int32_t)},
+                         {c->f_l, fl.data(), fl.size() * sizeof(int32_t)},
+                         {c->f_r, fr.data(), fr.size() * sizeof(int32_t)},
+                         {c->f_s, fs.data(), fs.size() * sizeof(int32_t)}};
+  if (c->tiled) {
+    ups.push_back({c->conn16, conn16_h.data(), conn16_h.size() * sizeof(uint16_t)});
+    ups.push_back({c->tile_off, toff.data(), toff.size() * sizeof(int32_t)});
+    ups.push_back({c->tile_idx, tidx.data(), tidx.size() * sizeof(int32_t)});
+  }
   for (const Up &u : ups) {
     if (!u.bytes) continue;
     if ((e = hipMemcpy(u.dst, u.src, u.bytes, hipMemcpyHostToDevice)) != hipSuccess) {
@@ -432,7 +498,7 @@ int lskum_create(const lskum_cloud *cloud, const lskum_params *params, lskum_ctx
       return fail(LSKUM_EHIP);
     }
   }
-  if ((e = hipMemset(c->alt, 0, sizeof(double) * ALT_STRIDE * n)) != hipSuccess ||
+  if ((e = hipMemset(c->qm, 0, sizeof(double) * QM_STRIDE * n)) != hipSuccess ||
       (e = hipMemset(c->flux_res, 0, sizeof(double) * 4 * n)) != hipSuccess ||
       (e = hipMemset(c->delta, 0, sizeof(double) * n)) != hipSuccess) {
     c->err = std::string("hipMemset: ") + hipGetErrorString(e);
@@ -504,13 +570,13 @@ int lskum_q_variables(lskum_ctx *ctx) {
 
 int lskum_q_derivatives(lskum_ctx *ctx) {
   CTX_OK(ctx);
-  return do_q_derivatives_and_inner(ctx, 0, true);
+  return do_q_derivatives(ctx, 0);
 }
 
 int lskum_q_inner(lskum_ctx *ctx, int32_t n_inner) {
   CTX_OK(ctx);
   if (n_inner < 0) return set_err(ctx, LSKUM_EINVAL, "n_inner < 0");
-  return do_q_derivatives_and_inner(ctx, n_inner, false);
+  return do_q_inner(ctx, n_inner);
 }
 
 int lskum_flux_residual(lskum_ctx *ctx) {
@@ -543,7 +609,7 @@ int lskum_state_update(lskum_ctx *ctx, int32_t rk, double *sum_res_sqr, double *
 int lskum_forces(lskum_ctx *ctx, double *cl, double *cd, double *cm) {
   CTX_OK(ctx);
   const double qdyn = 0.5 * ctx->rho_inf * ctx->prm.mach * ctx->prm.mach;
-  launch_forces(ctx->n_shape, ctx->f_m, ctx->f_l, ctx->f_r, ctx->f_s, ctx->rec, ctx->nx, ctx->ny,
+  launch_forces(ctx->n_shape, ctx->f_m, ctx->f_l, ctx->f_r, ctx->f_s, ctx->hot[0], ctx->nx, ctx->ny,
                 ctx->prim, ctx->n, ctx->pr_inf, qdyn, ctx->forces, ctx->stream);
   int rc = check_launch(ctx, "forces");
   if (rc) return rc;
@@ -579,7 +645,8 @@ int lskum_iterate(lskum_ctx *ctx, int32_t n_iter, double *res_new, double *resid
     if (!rc) rc = do_func_delta(ctx);
     for (int rk = 1; rk <= 4 && !rc; rk++) {
       rc = do_q_variables(ctx);
-      if (!rc) rc = do_q_derivatives_and_inner(ctx, inner, true);
+      if (!rc) rc = do_q_derivatives(ctx, inner);
+      if (!rc) rc = do_q_inner(ctx, inner);
       if (!rc) rc = do_flux(ctx);
       if (!rc) rc = do_state_update(ctx, rk, rk == 4 ? ctx->hist + t : nullptr);
     }
@@ -636,9 +703,12 @@ int lskum_get_array(lskum_ctx *ctx, int32_t which, double *dst) {
   switch (which) {
     case LSKUM_A_PRIM: rc = fetch_plain(ctx, ctx->prim, 4, dev); ncomp = 4; break;
     case LSKUM_A_PRIM_OLD: rc = fetch_plain(ctx, ctx->prim_old, 4, dev); ncomp = 4; break;
-    case LSKUM_A_Q: rc = fetch_record(ctx, R_Q, 4, dev); ncomp = 4; break;
-    case LSKUM_A_DQ: rc = fetch_record(ctx, R_DQX, 8, dev); ncomp = 8; break;
-    case LSKUM_A_QM: rc = fetch_record(ctx, R_QMAX, 8, dev); ncomp = 8; break;
+    case LSKUM_A_Q: rc = fetch_record(ctx, ctx->hot[0], H_STRIDE, H_Q, 4, dev); ncomp = 4; break;
+    case LSKUM_A_DQ:
+      rc = fetch_record(ctx, ctx->hot[ctx->cur], H_STRIDE, H_DQX, 8, dev);
+      ncomp = 8;
+      break;
+    case LSKUM_A_QM: rc = fetch_record(ctx, ctx->qm, QM_STRIDE, 0, 8, dev); ncomp = 8; break;
     case LSKUM_A_DELTA: rc = fetch_plain(ctx, ctx->delta, 1, dev); ncomp = 1; break;
     case LSKUM_A_FLUX_RES: rc = fetch_plain(ctx, ctx->flux_res, 4, dev); ncomp = 4; break;
     case LSKUM_A_NORMALS:
@@ -671,6 +741,14 @@ int lskum_get_substencil(lskum_ctx *ctx, int32_t which, int32_t *counts, int32_t
 }
 
 void *lskum_stream(lskum_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+int lskum_layout(lskum_ctx *ctx, int32_t *tiled, int32_t *tile_max, int32_t *order) {
+  CTX_OK(ctx);
+  if (tiled) *tiled = ctx->tiled ? 1 : 0;
+  if (tile_max) *tile_max = ctx->tile_max;
+  if (order) *order = ctx->prm.reorder;
+  return LSKUM_OK;
+}
 
 int lskum_set_timing(lskum_ctx *ctx, int32_t enable) {
   CTX_OK(ctx);

@@ -44,4 +44,9 @@ double substencil_min_rel_det(const lskum_cloud &c,
 // perm[new] = old.
 void rcm_order(const lskum_cloud &c, std::vector<int32_t> &perm);
 
+// Hilbert-curve order of the point coordinates (21 bits per axis over the
+// bounding box; ties by index): consecutive points are close in space, so a
+// block of consecutive points reaches a small set of neighbours.
+void hilbert_order(const lskum_cloud &c, std::vector<int32_t> &perm);
+
 }  // namespace lskum

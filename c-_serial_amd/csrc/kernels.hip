@@ -1,9 +1,12 @@
 // kernels.hip -- q-LSKUM stage kernels for gfx950 (FP64 form).
 //
-// One thread per point.  Stencil kernels walk the point's CSR neighbour list
-// in the reference's conn order and gather each neighbour's packed 192-byte
-// record (kernels.hpp), so every per-point sum is accumulated in the same
-// order as the reference's serial loops.  Built with -ffp-contract=off.
+// One thread per point, 256 points per block.  Stencil kernels walk the
+// point's CSR neighbour list in the reference's conn order, so every
+// per-point sum is accumulated in the same order as the reference's serial
+// loops (built with -ffp-contract=off).  Neighbour data come from the
+// block's LDS tile (its own points and every point its lists reach, loaded
+// once per block) when the tiles fit, else straight from the 128-byte hot
+// lines in HBM.
 #include <hip/hip_runtime.h>
 
 #include "kernels.hpp"
@@ -39,11 +42,43 @@ __device__ __forceinline__ void store4(double *p, const double v[4]) {
   reinterpret_cast<double2 *>(p)[1] = make_double2(v[2], v[3]);
 }
 
+// ------------------------------------------------------------ tiles
+// A block's LDS tile holds its own 256 points first (local index =
+// threadIdx.x), then the other points its neighbour lists reach; tiled conn
+// entries carry the local index.  The cooperative load happens before any
+// thread leaves, so the barrier is reached by the whole block.
+template <int STRIDE>
+__device__ __forceinline__ void load_tile(double *lds, const int32_t *__restrict__ tile_off,
+                                          const int32_t *__restrict__ tile_idx, int b,
+                                          const double *__restrict__ src, int src_stride,
+                                          int first_chunk, int nchunks, int dst_offset) {
+  const int t0 = tile_off[b], tn = tile_off[b + 1] - t0;
+  for (int t = threadIdx.x; t < tn * nchunks; t += blockDim.x) {
+    const int r = t / nchunks, ch = t - r * nchunks;
+    const double2 v = reinterpret_cast<const double2 *>(
+        src + (size_t)src_stride * tile_idx[t0 + r])[first_chunk + ch];
+    reinterpret_cast<double2 *>(lds + (size_t)STRIDE * r + dst_offset)[ch] = v;
+  }
+}
+
+__device__ __forceinline__ int nbr_index(const uint32_t *conn, const uint16_t *conn16, int e,
+                                         bool tiled, uint32_t &mask) {
+  if (tiled) {
+    const uint16_t w = conn16[e];
+    mask = w >> kLocalShift;
+    return w & kLocalMask;
+  }
+  const uint32_t w = conn[e];
+  mask = w >> kMaskShift;
+  return (int)(w & kIndexMask);
+}
+
 // ------------------------------------------------------------ q variables
-// q_variables_mod.h:9-34
+// q_variables_mod.h:9-34, written into both hot-line buffers
 __global__ __launch_bounds__(256) void k_q_variables(int n, const double *__restrict__ prim,
-                                                     double *__restrict__ rec) {
-  const int k = point_index();
+                                                     double *__restrict__ hot_a,
+                                                     double *__restrict__ hot_b) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
   const double rho = prim[k], u1 = prim[n + k], u2 = prim[2 * n + k], pr = prim[3 * n + k];
   const double beta = 0.5 * rho / pr;
@@ -53,36 +88,100 @@ __global__ __launch_bounds__(256) void k_q_variables(int n, const double *__rest
   q[1] = two_times_beta * u1;
   q[2] = two_times_beta * u2;
   q[3] = -two_times_beta;
-  store4(rec + (size_t)R_STRIDE * k + R_Q, q);
+  store4(hot_a + (size_t)H_STRIDE * k + H_Q, q);
+  store4(hot_b + (size_t)H_STRIDE * k + H_Q, q);
+}
+
+// The denominator of state_update_mod.h:485 for the point's own state:
+// func_delta divides each neighbour distance by its neighbour's value.
+__global__ __launch_bounds__(256) void k_speed(int n, const double *__restrict__ prim,
+                                               double *__restrict__ hot) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const double rho = prim[k], u1 = prim[n + k], u2 = prim[2 * n + k], pr = prim[3 * n + k];
+  const double mod_u = sqrt(u1 * u1 + u2 * u2);
+  hot[(size_t)H_STRIDE * k + H_SPEED] = mod_u + 3.0 * sqrt(pr / rho);
+}
+
+// ------------------------------------------------------------ local dt
+// state_update_mod.h:448-496: min over the neighbours' states only.
+template <bool TILED>
+__global__ __launch_bounds__(256) void k_func_delta(int n, const int32_t *__restrict__ off,
+                                                    const uint32_t *__restrict__ conn,
+                                                    const uint16_t *__restrict__ conn16,
+                                                    const int32_t *__restrict__ tile_off,
+                                                    const int32_t *__restrict__ tile_idx,
+                                                    const double *__restrict__ hot, double cfl,
+                                                    double *__restrict__ delta) {
+  extern __shared__ double lds[];
+  constexpr int S = 4;  // x, y, speed, pad
+  const int b = xcd_block();
+  if (TILED) {
+    const int t0 = tile_off[b], tn = tile_off[b + 1] - t0;
+    for (int t = threadIdx.x; t < tn; t += blockDim.x) {
+      const double *h = hot + (size_t)H_STRIDE * tile_idx[t0 + t];
+      reinterpret_cast<double2 *>(lds + S * t)[0] = reinterpret_cast<const double2 *>(h)[0];
+      lds[S * t + 2] = h[H_SPEED];
+    }
+    __syncthreads();
+  }
+  const int i = b * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double *hi = TILED ? lds + S * threadIdx.x : hot + (size_t)H_STRIDE * i;
+  const double2 xy = reinterpret_cast<const double2 *>(hi)[0];
+  double min_delt = 1.0;
+  const int e1 = off[i + 1];
+  for (int e = off[i]; e < e1; e++) {
+    uint32_t m;
+    const int k = nbr_index(conn, conn16, e, TILED, m);
+    const double *hk = TILED ? lds + S * k : hot + (size_t)H_STRIDE * k;
+    const double2 xyk = reinterpret_cast<const double2 *>(hk)[0];
+    const double speed = TILED ? hk[2] : hk[H_SPEED];
+    double dist = (xyk.x - xy.x) * (xyk.x - xy.x) + (xyk.y - xy.y) * (xyk.y - xy.y);
+    dist = sqrt(dist);
+    double delta_t = dist / speed;
+    delta_t = cfl * delta_t;
+    if (min_delt > delta_t) min_delt = delta_t;
+  }
+  delta[i] = min_delt;
 }
 
 // ------------------------------------------------------------ q derivatives
 // q_variables_mod.h:36-118: unweighted (power = 0) or pow(dist, power)
 // weighted least squares over the full stencil, and qmax/qmin over the
-// point and its neighbours.
-template <bool POW>
-__global__ __launch_bounds__(256) void k_q_derivatives(int n, const int32_t *__restrict__ off,
-                                                       const uint32_t *__restrict__ conn,
-                                                       double *__restrict__ rec,
-                                                       double *__restrict__ dq_dst,
-                                                       int dq_stride, double power) {
-  const int i = point_index();
+// point and its neighbours.  Reads x, y, q of the hot line and writes its
+// dq slots (disjoint from what other threads read) and qm.
+template <bool TILED, bool POW>
+__global__ __launch_bounds__(256) void k_q_derivatives(
+    int n, const int32_t *__restrict__ off, const uint32_t *__restrict__ conn,
+    const uint16_t *__restrict__ conn16, const int32_t *__restrict__ tile_off,
+    const int32_t *__restrict__ tile_idx, double *hot, double *__restrict__ qm_out,
+    double power) {
+  extern __shared__ double lds[];
+  constexpr int S = 6;  // x, y, q[4]
+  const int b = xcd_block();
+  if (TILED) {
+    load_tile<S>(lds, tile_off, tile_idx, b, hot, H_STRIDE, 0, 3, 0);
+    __syncthreads();
+  }
+  const int i = b * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const double *ri = rec + (size_t)R_STRIDE * i;
-  const double2 xy = reinterpret_cast<const double2 *>(ri)[0];
+  const double *hi = TILED ? lds + S * threadIdx.x : hot + (size_t)H_STRIDE * i;
+  const double2 xy = reinterpret_cast<const double2 *>(hi)[0];
   const double x_i = xy.x, y_i = xy.y;
   double qi[4], qmax[4], qmin[4];
-  load4(ri + R_Q, qi);
+  load4(hi + H_Q, qi);
   double sum_delx_sqr = 0., sum_dely_sqr = 0., sum_delx_dely = 0.;
   double sdx[4] = {0.0, 0.0, 0.0, 0.0}, sdy[4] = {0.0, 0.0, 0.0, 0.0};
   for (int r = 0; r < 4; r++) qmax[r] = qmin[r] = qi[r];
   const int e1 = off[i + 1];
   for (int e = off[i]; e < e1; e++) {
-    const int k = (int)(conn[e] & kIndexMask);
-    const double *rk = rec + (size_t)R_STRIDE * k;
-    const double2 xyk = reinterpret_cast<const double2 *>(rk)[0];
+    uint32_t m;
+    const int k = nbr_index(conn, conn16, e, TILED, m);
+    const double *hk = TILED ? lds + S * k : hot + (size_t)H_STRIDE * k;
+    const double2 xyk = reinterpret_cast<const double2 *>(hk)[0];
     double qk[4];
-    load4(rk + R_Q, qk);
+    load4(hk + H_Q, qk);
     for (int r = 0; r < 4; r++) {
       if (qk[r] > qmax[r]) qmax[r] = qk[r];
       if (qk[r] < qmin[r]) qmin[r] = qk[r];
@@ -114,44 +213,51 @@ __global__ __launch_bounds__(256) void k_q_derivatives(int n, const int32_t *__r
     dqx[r] = (sdx[r] * sum_dely_sqr - sdy[r] * sum_delx_dely) * one_by_det;
     dqy[r] = (sdy[r] * sum_delx_sqr - sdx[r] * sum_delx_dely) * one_by_det;
   }
-  double *wi = rec + (size_t)R_STRIDE * i;
-  store4(wi + R_QMAX, qmax);
-  store4(wi + R_QMIN, qmin);
-  double *d = dq_dst + (size_t)dq_stride * i;
-  store4(d, dqx);
-  store4(d + 4, dqy);
+  double *w = hot + (size_t)H_STRIDE * i;
+  store4(w + H_DQX, dqx);
+  store4(w + H_DQY, dqy);
+  double *qmw = qm_out + (size_t)QM_STRIDE * i;
+  store4(qmw, qmax);
+  store4(qmw + 4, qmin);
 }
 
 // ------------------------------------------------------------ inner passes
 // q_variables_mod.h:120-228: one defect-corrected LS pass, Jacobi (reads
-// dq_src, writes dq_dst; the reference's temp array + copy).
-template <bool POW>
-__global__ __launch_bounds__(256) void k_q_inner(int n, const int32_t *__restrict__ off,
-                                                 const uint32_t *__restrict__ conn,
-                                                 const double *__restrict__ rec,
-                                                 const double *__restrict__ dq_src, int src_stride,
-                                                 double *__restrict__ dq_dst, int dst_stride,
-                                                 double power) {
-  const int i = point_index();
+// src, writes the dq slots of dst: the reference's temp array + copy).
+template <bool TILED, bool POW>
+__global__ __launch_bounds__(256) void k_q_inner(
+    int n, const int32_t *__restrict__ off, const uint32_t *__restrict__ conn,
+    const uint16_t *__restrict__ conn16, const int32_t *__restrict__ tile_off,
+    const int32_t *__restrict__ tile_idx, const double *__restrict__ src,
+    double *__restrict__ dst, double power) {
+  extern __shared__ double lds[];
+  constexpr int S = 14;  // x, y, q[4], dqx[4], dqy[4]
+  const int b = xcd_block();
+  if (TILED) {
+    load_tile<S>(lds, tile_off, tile_idx, b, src, H_STRIDE, 0, 7, 0);
+    __syncthreads();
+  }
+  const int i = b * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const double *ri = rec + (size_t)R_STRIDE * i;
-  const double2 xy = reinterpret_cast<const double2 *>(ri)[0];
+  const double *hi = TILED ? lds + S * threadIdx.x : src + (size_t)H_STRIDE * i;
+  const double2 xy = reinterpret_cast<const double2 *>(hi)[0];
   const double x_i = xy.x, y_i = xy.y;
   double qi[4], dxi[4], dyi[4];
-  load4(ri + R_Q, qi);
-  load4(dq_src + (size_t)src_stride * i, dxi);
-  load4(dq_src + (size_t)src_stride * i + 4, dyi);
+  load4(hi + H_Q, qi);
+  load4(hi + H_DQX, dxi);
+  load4(hi + H_DQY, dyi);
   double sum_delx_sqr = 0., sum_dely_sqr = 0., sum_delx_dely = 0.;
   double sx[4] = {0.0, 0.0, 0.0, 0.0}, sy[4] = {0.0, 0.0, 0.0, 0.0};
   const int e1 = off[i + 1];
   for (int e = off[i]; e < e1; e++) {
-    const int k = (int)(conn[e] & kIndexMask);
-    const double *rk = rec + (size_t)R_STRIDE * k;
-    const double2 xyk = reinterpret_cast<const double2 *>(rk)[0];
+    uint32_t m;
+    const int k = nbr_index(conn, conn16, e, TILED, m);
+    const double *hk = TILED ? lds + S * k : src + (size_t)H_STRIDE * k;
+    const double2 xyk = reinterpret_cast<const double2 *>(hk)[0];
     double qk[4], dxk[4], dyk[4];
-    load4(rk + R_Q, qk);
-    load4(dq_src + (size_t)src_stride * k, dxk);
-    load4(dq_src + (size_t)src_stride * k + 4, dyk);
+    load4(hk + H_Q, qk);
+    load4(hk + H_DQX, dxk);
+    load4(hk + H_DQY, dyk);
     const double delx = xyk.x - x_i, dely = xyk.y - y_i;
     const double weights = POW ? pow(sqrt(delx * delx + dely * dely), power) : 1.0;
     if (POW) {
@@ -182,9 +288,9 @@ __global__ __launch_bounds__(256) void k_q_inner(int n, const int32_t *__restric
     ox[r] = one_by_det * (sx[r] * sum_dely_sqr - sy[r] * sum_delx_dely);
     oy[r] = one_by_det * (sy[r] * sum_delx_sqr - sx[r] * sum_delx_dely);
   }
-  double *d = dq_dst + (size_t)dst_stride * i;
-  store4(d, ox);
-  store4(d + 4, oy);
+  double *d = dst + (size_t)H_STRIDE * i;
+  store4(d + H_DQX, ox);
+  store4(d + H_DQY, oy);
 }
 
 // ------------------------------------------------------------ flux residual
@@ -229,27 +335,35 @@ __device__ __forceinline__ void acc_solve(const Acc &a, bool yform, double G[4])
 // primitive states depend only on the (point, neighbour) pair, so they are
 // computed once per neighbour and feed the x- and y-direction fluxes of the
 // (usually two) sub-stencils the neighbour belongs to.
-template <bool POW>
-__global__ __launch_bounds__(256) void k_flux(int n, const int32_t *__restrict__ off,
-                                              const uint32_t *__restrict__ conn,
-                                              const double *__restrict__ rec,
-                                              const double *__restrict__ nxa,
-                                              const double *__restrict__ nya,
-                                              const uint8_t *__restrict__ flag,
-                                              const double *__restrict__ delta, double power,
-                                              double *__restrict__ fres) {
-  const int i = point_index();
+template <bool TILED, bool POW>
+__global__ __launch_bounds__(256) void k_flux(
+    int n, const int32_t *__restrict__ off, const uint32_t *__restrict__ conn,
+    const uint16_t *__restrict__ conn16, const int32_t *__restrict__ tile_off,
+    const int32_t *__restrict__ tile_idx, const double *__restrict__ hot,
+    const double *__restrict__ qm, const double *__restrict__ nxa,
+    const double *__restrict__ nya, const uint8_t *__restrict__ flag,
+    const double *__restrict__ delta, double power, double *__restrict__ fres) {
+  extern __shared__ double lds[];
+  constexpr int S = 24;  // hot line (16) then qmax, qmin (8)
+  const int b = xcd_block();
+  if (TILED) {
+    load_tile<S>(lds, tile_off, tile_idx, b, hot, H_STRIDE, 0, 8, 0);
+    load_tile<S>(lds, tile_off, tile_idx, b, qm, QM_STRIDE, 0, 4, H_STRIDE);
+    __syncthreads();
+  }
+  const int i = b * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const double *ri = rec + (size_t)R_STRIDE * i;
-  const double2 xy = reinterpret_cast<const double2 *>(ri)[0];
+  const double *hi = TILED ? lds + S * threadIdx.x : hot + (size_t)H_STRIDE * i;
+  const double *mi = TILED ? hi + H_STRIDE : qm + (size_t)QM_STRIDE * i;
+  const double2 xy = reinterpret_cast<const double2 *>(hi)[0];
   const double x_i = xy.x, y_i = xy.y;
   double qi[4], dxi[4], dyi[4], qmxi[4], qmni[4];
-  load4(ri + R_Q, qi);
-  load4(ri + R_DQX, dxi);
-  load4(ri + R_DQY, dyi);
-  load4(ri + R_QMAX, qmxi);
-  load4(ri + R_QMIN, qmni);
-  const double eps2_i = ri[R_EPS2];
+  load4(hi + H_Q, qi);
+  load4(hi + H_DQX, dxi);
+  load4(hi + H_DQY, dyi);
+  load4(mi, qmxi);
+  load4(mi + 4, qmni);
+  const double eps2_i = hi[H_EPS2];
   const double nx = nxa[i], ny = nya[i];
   const double tx = ny, ty = -nx;
   const int type = flag[i];
@@ -262,18 +376,18 @@ __global__ __launch_bounds__(256) void k_flux(int n, const int32_t *__restrict__
   acc_zero(ayn);
   const int e1 = off[i + 1];
   for (int e = off[i]; e < e1; e++) {
-    const uint32_t w = conn[e];
-    const int k = (int)(w & kIndexMask);
-    const uint32_t m = w >> kMaskShift;
-    const double *rk = rec + (size_t)R_STRIDE * k;
-    const double2 xyk = reinterpret_cast<const double2 *>(rk)[0];
+    uint32_t m;
+    const int k = nbr_index(conn, conn16, e, TILED, m);
+    const double *hk = TILED ? lds + S * k : hot + (size_t)H_STRIDE * k;
+    const double *mk = TILED ? hk + H_STRIDE : qm + (size_t)QM_STRIDE * k;
+    const double2 xyk = reinterpret_cast<const double2 *>(hk)[0];
     double qk[4], dxk[4], dyk[4], qmxk[4], qmnk[4];
-    load4(rk + R_Q, qk);
-    load4(rk + R_DQX, dxk);
-    load4(rk + R_DQY, dyk);
-    load4(rk + R_QMAX, qmxk);
-    load4(rk + R_QMIN, qmnk);
-    const double eps2_k = rk[R_EPS2];
+    load4(hk + H_Q, qk);
+    load4(hk + H_DQX, dxk);
+    load4(hk + H_DQY, dyk);
+    load4(mk, qmxk);
+    load4(mk + 4, qmnk);
+    const double eps2_k = hk[H_EPS2];
     const double delx = xyk.x - x_i, dely = xyk.y - y_i;
     const double dels = delx * tx + dely * ty;
     const double deln = delx * nx + dely * ny;
@@ -351,32 +465,6 @@ __global__ __launch_bounds__(256) void k_flux(int n, const int32_t *__restrict__
     for (int j = 0; j < 4; j++) out[j] = d * (Gxp[j] + Gxn[j] + Gyp[j]);
   }
   for (int j = 0; j < 4; j++) fres[(size_t)j * n + i] = out[j];
-}
-
-// ------------------------------------------------------------ local dt
-// state_update_mod.h:448-496: min over the neighbours' states only.
-__global__ __launch_bounds__(256) void k_func_delta(int n, const int32_t *__restrict__ off,
-                                                    const uint32_t *__restrict__ conn,
-                                                    const double *__restrict__ rec,
-                                                    const double *__restrict__ prim, double cfl,
-                                                    double *__restrict__ delta) {
-  const int i = point_index();
-  if (i >= n) return;
-  const double2 xy = reinterpret_cast<const double2 *>(rec + (size_t)R_STRIDE * i)[0];
-  double min_delt = 1.0;
-  const int e1 = off[i + 1];
-  for (int e = off[i]; e < e1; e++) {
-    const int k = (int)(conn[e] & kIndexMask);
-    const double rho = prim[k], u1 = prim[n + k], u2 = prim[2 * n + k], pr = prim[3 * n + k];
-    const double2 xyk = reinterpret_cast<const double2 *>(rec + (size_t)R_STRIDE * k)[0];
-    double dist = (xyk.x - xy.x) * (xyk.x - xy.x) + (xyk.y - xy.y) * (xyk.y - xy.y);
-    dist = sqrt(dist);
-    const double mod_u = sqrt(u1 * u1 + u2 * u2);
-    double delta_t = dist / (mod_u + 3.0 * sqrt(pr / rho));
-    delta_t = cfl * delta_t;
-    if (min_delt > delta_t) min_delt = delta_t;
-  }
-  delta[i] = min_delt;
 }
 
 // ------------------------------------------------------------ state update
@@ -543,7 +631,7 @@ __global__ __launch_bounds__(256) void k_forces(int ns, const int32_t *__restric
                                                 const int32_t *__restrict__ li,
                                                 const int32_t *__restrict__ ri,
                                                 const int32_t *__restrict__ shape,
-                                                const double *__restrict__ rec,
+                                                const double *__restrict__ hot,
                                                 const double *__restrict__ nxa,
                                                 const double *__restrict__ nya,
                                                 const double *__restrict__ prim, int n,
@@ -554,9 +642,9 @@ __global__ __launch_bounds__(256) void k_forces(int ns, const int32_t *__restric
   for (int j = threadIdx.x; j < ns; j += blockDim.x) {
     if (shape[j] != 1) continue;
     const int m = mi[j], l = li[j], r = ri[j];
-    const double2 L = reinterpret_cast<const double2 *>(rec + (size_t)R_STRIDE * l)[0];
-    const double2 Mp = reinterpret_cast<const double2 *>(rec + (size_t)R_STRIDE * m)[0];
-    const double2 Rp = reinterpret_cast<const double2 *>(rec + (size_t)R_STRIDE * r)[0];
+    const double2 L = reinterpret_cast<const double2 *>(hot + (size_t)H_STRIDE * l)[0];
+    const double2 Mp = reinterpret_cast<const double2 *>(hot + (size_t)H_STRIDE * m)[0];
+    const double2 Rp = reinterpret_cast<const double2 *>(hot + (size_t)H_STRIDE * r)[0];
     double ds1 = (Mp.x - L.x) * (Mp.x - L.x) + (Mp.y - L.y) * (Mp.y - L.y);
     ds1 = sqrt(ds1);
     double ds2 = (Rp.x - Mp.x) * (Rp.x - Mp.x) + (Rp.y - Mp.y) * (Rp.y - Mp.y);
@@ -618,54 +706,98 @@ __global__ void k_kat_qtilde(int n, const double *__restrict__ in, double *__res
 
 int blocks_for(int n, int block) { return (n + block - 1) / block; }
 
-void launch_q_variables(const DevCloud &c, const double *prim, double *rec, int block,
+size_t lds_bytes_qderiv(int tile_max) { return sizeof(double) * 6 * (size_t)tile_max; }
+size_t lds_bytes_qinner(int tile_max) { return sizeof(double) * 14 * (size_t)tile_max; }
+size_t lds_bytes_delta(int tile_max) { return sizeof(double) * 4 * (size_t)tile_max; }
+size_t lds_bytes_flux(int tile_max) { return sizeof(double) * 24 * (size_t)tile_max; }
+
+bool prepare_tiled(int tile_max, size_t lds_limit) {
+  if (lds_bytes_flux(tile_max) > lds_limit) return false;
+  struct K { const void *f; size_t bytes; };
+  const K ks[] = {
+      {(const void *)k_q_derivatives<true, false>, lds_bytes_qderiv(tile_max)},
+      {(const void *)k_q_derivatives<true, true>, lds_bytes_qderiv(tile_max)},
+      {(const void *)k_q_inner<true, false>, lds_bytes_qinner(tile_max)},
+      {(const void *)k_q_inner<true, true>, lds_bytes_qinner(tile_max)},
+      {(const void *)k_func_delta<true>, lds_bytes_delta(tile_max)},
+      {(const void *)k_flux<true, false>, lds_bytes_flux(tile_max)},
+      {(const void *)k_flux<true, true>, lds_bytes_flux(tile_max)}};
+  for (const K &k : ks)
+    if (hipFuncSetAttribute(k.f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)k.bytes) !=
+        hipSuccess)
+      return false;
+  return true;
+}
+
+#define TILE_ARGS c.conn_off, c.conn, c.conn16, c.tile_off, c.tile_idx
+
+void launch_q_variables(const DevCloud &c, const double *prim, double *hot_a, double *hot_b,
                         hipStream_t s) {
-  hipLaunchKernelGGL(k_q_variables, dim3(blocks_for(c.n, block)), dim3(block), 0, s, c.n, prim,
-                     rec);
+  hipLaunchKernelGGL(k_q_variables, dim3(blocks_for(c.n, 256)), dim3(256), 0, s, c.n, prim,
+                     hot_a, hot_b);
 }
 
-void launch_q_derivatives(const DevCloud &c, double *rec, double *dq_dst, int dq_stride,
-                          double power, int block, hipStream_t s) {
-  dim3 g(blocks_for(c.n, block)), b(block);
-  if (power == 0.0)
-    hipLaunchKernelGGL(k_q_derivatives<false>, g, b, 0, s, c.n, c.conn_off, c.conn, rec, dq_dst,
-                       dq_stride, power);
+void launch_speed(const DevCloud &c, const double *prim, double *hot, hipStream_t s) {
+  hipLaunchKernelGGL(k_speed, dim3(blocks_for(c.n, 256)), dim3(256), 0, s, c.n, prim, hot);
+}
+
+void launch_func_delta(const DevCloud &c, const double *hot, double cfl, double *delta,
+                       hipStream_t s) {
+  dim3 g(blocks_for(c.n, kTileBlock)), b(kTileBlock);
+  if (c.tiled)
+    hipLaunchKernelGGL(k_func_delta<true>, g, b, lds_bytes_delta(c.tile_max), s, c.n, TILE_ARGS,
+                       hot, cfl, delta);
   else
-    hipLaunchKernelGGL(k_q_derivatives<true>, g, b, 0, s, c.n, c.conn_off, c.conn, rec, dq_dst,
-                       dq_stride, power);
+    hipLaunchKernelGGL(k_func_delta<false>, g, b, 0, s, c.n, TILE_ARGS, hot, cfl, delta);
 }
 
-void launch_q_inner(const DevCloud &c, const double *rec, const double *dq_src, int src_stride,
-                    double *dq_dst, int dst_stride, double power, int block, hipStream_t s) {
-  dim3 g(blocks_for(c.n, block)), b(block);
-  if (power == 0.0)
-    hipLaunchKernelGGL(k_q_inner<false>, g, b, 0, s, c.n, c.conn_off, c.conn, rec, dq_src,
-                       src_stride, dq_dst, dst_stride, power);
+void launch_q_derivatives(const DevCloud &c, double *hot, double *qm, double power,
+                          hipStream_t s) {
+  dim3 g(blocks_for(c.n, kTileBlock)), b(kTileBlock);
+  const size_t L = c.tiled ? lds_bytes_qderiv(c.tile_max) : 0;
+  if (c.tiled && power == 0.0)
+    hipLaunchKernelGGL((k_q_derivatives<true, false>), g, b, L, s, c.n, TILE_ARGS, hot, qm, power);
+  else if (c.tiled)
+    hipLaunchKernelGGL((k_q_derivatives<true, true>), g, b, L, s, c.n, TILE_ARGS, hot, qm, power);
+  else if (power == 0.0)
+    hipLaunchKernelGGL((k_q_derivatives<false, false>), g, b, 0, s, c.n, TILE_ARGS, hot, qm, power);
   else
-    hipLaunchKernelGGL(k_q_inner<true>, g, b, 0, s, c.n, c.conn_off, c.conn, rec, dq_src,
-                       src_stride, dq_dst, dst_stride, power);
+    hipLaunchKernelGGL((k_q_derivatives<false, true>), g, b, 0, s, c.n, TILE_ARGS, hot, qm, power);
 }
 
-void launch_flux(const DevCloud &c, const double *rec, const double *delta, double power,
-                 double *flux_res, int block, hipStream_t s) {
-  dim3 g(blocks_for(c.n, block)), b(block);
-  if (power == 0.0)
-    hipLaunchKernelGGL(k_flux<false>, g, b, 0, s, c.n, c.conn_off, c.conn, rec, c.nx, c.ny,
-                       c.flag, delta, power, flux_res);
+void launch_q_inner(const DevCloud &c, const double *src, double *dst, double power,
+                    hipStream_t s) {
+  dim3 g(blocks_for(c.n, kTileBlock)), b(kTileBlock);
+  const size_t L = c.tiled ? lds_bytes_qinner(c.tile_max) : 0;
+  if (c.tiled && power == 0.0)
+    hipLaunchKernelGGL((k_q_inner<true, false>), g, b, L, s, c.n, TILE_ARGS, src, dst, power);
+  else if (c.tiled)
+    hipLaunchKernelGGL((k_q_inner<true, true>), g, b, L, s, c.n, TILE_ARGS, src, dst, power);
+  else if (power == 0.0)
+    hipLaunchKernelGGL((k_q_inner<false, false>), g, b, 0, s, c.n, TILE_ARGS, src, dst, power);
   else
-    hipLaunchKernelGGL(k_flux<true>, g, b, 0, s, c.n, c.conn_off, c.conn, rec, c.nx, c.ny,
-                       c.flag, delta, power, flux_res);
+    hipLaunchKernelGGL((k_q_inner<false, true>), g, b, 0, s, c.n, TILE_ARGS, src, dst, power);
 }
 
-void launch_func_delta(const DevCloud &c, const double *rec, const double *prim, double cfl,
-                       double *delta, int block, hipStream_t s) {
-  hipLaunchKernelGGL(k_func_delta, dim3(blocks_for(c.n, block)), dim3(block), 0, s, c.n,
-                     c.conn_off, c.conn, rec, prim, cfl, delta);
+void launch_flux(const DevCloud &c, const double *hot, const double *qm, const double *delta,
+                 double power, double *flux_res, hipStream_t s) {
+  dim3 g(blocks_for(c.n, kTileBlock)), b(kTileBlock);
+  const size_t L = c.tiled ? lds_bytes_flux(c.tile_max) : 0;
+#define FLUX_ARGS c.n, TILE_ARGS, hot, qm, c.nx, c.ny, c.flag, delta, power, flux_res
+  if (c.tiled && power == 0.0)
+    hipLaunchKernelGGL((k_flux<true, false>), g, b, L, s, FLUX_ARGS);
+  else if (c.tiled)
+    hipLaunchKernelGGL((k_flux<true, true>), g, b, L, s, FLUX_ARGS);
+  else if (power == 0.0)
+    hipLaunchKernelGGL((k_flux<false, false>), g, b, 0, s, FLUX_ARGS);
+  else
+    hipLaunchKernelGGL((k_flux<false, true>), g, b, 0, s, FLUX_ARGS);
+#undef FLUX_ARGS
 }
 
 void launch_state_update(const DevCloud &c, int rk, double *prim, const double *prim_old,
                          const double *flux_res, const FlowConsts &fc, const Partials &p,
-                         int /*block*/, hipStream_t s) {
+                         hipStream_t s) {
   // fixed 256-thread blocks: the partial layout must not depend on tuning
   hipLaunchKernelGGL(k_state_update, dim3(p.nblocks), dim3(256), 0, s, c.n, rk, prim, prim_old,
                      flux_res, c.nx, c.ny, c.flag, c.perm, fc, p.sum, p.max, p.arg);
@@ -677,10 +809,10 @@ void launch_reduce(const Partials &p, double *out, double *hist_slot, hipStream_
 }
 
 void launch_forces(int n_shape, const int32_t *m, const int32_t *l, const int32_t *r,
-                   const int32_t *shape, const double *rec, const double *nx, const double *ny,
+                   const int32_t *shape, const double *hot, const double *nx, const double *ny,
                    const double *prim, int n, double pr_inf, double qdyn, double *out3,
                    hipStream_t s) {
-  hipLaunchKernelGGL(k_forces, dim3(1), dim3(256), 0, s, n_shape, m, l, r, shape, rec, nx, ny,
+  hipLaunchKernelGGL(k_forces, dim3(1), dim3(256), 0, s, n_shape, m, l, r, shape, hot, nx, ny,
                      prim, n, pr_inf, qdyn, out3);
 }
 

@@ -1,5 +1,5 @@
 // setup.cpp -- one-time host set-up of a cloud: validation, normals,
-// sub-stencil classification and the RCM point order.
+// sub-stencil classification and the device point orders (RCM, Hilbert).
 //
 // Built with -ffp-contract=off: the sub-stencil tests compare dels/deln with
 // zero exactly as the reference does, so the lists are bit-identical to the
@@ -223,6 +223,39 @@ void rcm_order(const lskum_cloud &c, std::vector<int32_t> &perm) {
     }
   }
   perm.assign(order.rbegin(), order.rend());
+}
+
+void hilbert_order(const lskum_cloud &c, std::vector<int32_t> &perm) {
+  const int32_t n = c.n_points;
+  double x0 = c.x[0], x1 = c.x[0], y0 = c.y[0], y1 = c.y[0];
+  for (int32_t i = 1; i < n; i++) {
+    x0 = std::min(x0, c.x[i]); x1 = std::max(x1, c.x[i]);
+    y0 = std::min(y0, c.y[i]); y1 = std::max(y1, c.y[i]);
+  }
+  const int bits = 21;
+  const uint32_t side = 1u << bits;
+  const double sx = (x1 > x0) ? (side - 1) / (x1 - x0) : 0.0;
+  const double sy = (y1 > y0) ? (side - 1) / (y1 - y0) : 0.0;
+  std::vector<std::pair<uint64_t, int32_t>> key(n);
+  for (int32_t i = 0; i < n; i++) {
+    uint32_t xi = (uint32_t)((c.x[i] - x0) * sx), yi = (uint32_t)((c.y[i] - y0) * sy);
+    uint64_t d = 0;
+    for (uint32_t s = side >> 1; s > 0; s >>= 1) {  // xy -> Hilbert distance
+      const uint32_t rx = (xi & s) ? 1u : 0u, ry = (yi & s) ? 1u : 0u;
+      d += (uint64_t)s * s * ((3u * rx) ^ ry);
+      if (ry == 0) {  // rotate the quadrant; only bits below s matter later
+        if (rx == 1) {
+          xi = side - 1 - xi;
+          yi = side - 1 - yi;
+        }
+        std::swap(xi, yi);
+      }
+    }
+    key[i] = {d, i};
+  }
+  std::sort(key.begin(), key.end());
+  perm.resize(n);
+  for (int32_t i = 0; i < n; i++) perm[i] = key[i].second;
 }
 
 }  // namespace lskum

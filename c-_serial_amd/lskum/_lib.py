@@ -17,7 +17,7 @@ ERRORS = {
 }
 
 FP64, MIXED = 0, 1
-REORDER_NONE, REORDER_RCM = 0, 1
+REORDER_NONE, REORDER_RCM, REORDER_HILBERT = 0, 1, 2
 A_PRIM, A_PRIM_OLD, A_Q, A_DQ, A_QM, A_DELTA, A_FLUX_RES, A_NORMALS = range(8)
 K_FLUX, K_QDERIV, K_QINNER, K_STATE, K_DELTA, K_QVAR = range(6)
 
@@ -90,6 +90,7 @@ _SIGS = [
     ("lskum_get_array", C.c_int, [C.c_void_p, C.c_int32, _dp]),
     ("lskum_get_substencil", C.c_int, [C.c_void_p, C.c_int32, _ip, _ip]),
     ("lskum_stream", C.c_void_p, [C.c_void_p]),
+    ("lskum_layout", C.c_int, [C.c_void_p, _ip, _ip, _ip]),
     ("lskum_set_timing", C.c_int, [C.c_void_p, C.c_int32]),
     ("lskum_kernel_time", C.c_int, [C.c_void_p, C.c_int32, _dp, _ip]),
     ("lskum_kat_fluxes", C.c_int, [C.c_int32, _dp, _dp, C.c_int32]),

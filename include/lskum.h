@@ -121,6 +121,8 @@ void lskum_cloud_free(lskum_cloud_h *c);
 
 #define LSKUM_REORDER_NONE 0
 #define LSKUM_REORDER_RCM 1 /* reverse Cuthill-McKee on the neighbour graph */
+#define LSKUM_REORDER_HILBERT 2 /* Hilbert curve over the coordinates:     */
+                                /* compact blocks, neighbour tiles in LDS  */
 
 /*
  * Case and numerics parameters.  The reference hard-codes these as globals
@@ -215,6 +217,10 @@ int lskum_get_substencil(lskum_ctx *ctx, int32_t which, int32_t *counts,
 
 /* Stream the context's kernels run on (a hipStream_t). */
 void *lskum_stream(lskum_ctx *ctx);
+/* Device layout chosen at create: tiled = 1 when the stencil kernels stage
+ * neighbour data in per-block LDS tiles; tile_max = largest tile (points);
+ * order = the LSKUM_REORDER_* in use.  Any pointer may be NULL. */
+int lskum_layout(lskum_ctx *ctx, int32_t *tiled, int32_t *tile_max, int32_t *order);
 
 /* Average duration in ms of the named kernel over the last lskum_iterate
  * call with timing enabled (0 disables).  Kernel ids: LSKUM_K_*. */
