@@ -103,6 +103,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="naca0012_2p5m_k24")
     ap.add_argument("--reorder", default="hilbert", choices=["hilbert", "rcm", "none"])
+    ap.add_argument("--flux-lds", default="auto", choices=["auto", "on", "off"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -119,10 +120,11 @@ def main():
     reorder = {"hilbert": L.REORDER_HILBERT, "rcm": L.REORDER_RCM,
                "none": L.REORDER_NONE}[args.reorder]
     t = time.perf_counter()
-    solver = Solver(cloud, reorder=reorder, **CASE)
-    tiled, tile_max, _ = solver.layout()
-    log(f"context created in {time.perf_counter() - t:.1f}s; LDS tiles {tiled}, "
-        f"largest tile {tile_max} points")
+    flux_lds = {"auto": L.FLUX_LDS_AUTO, "on": L.FLUX_LDS_ON, "off": L.FLUX_LDS_OFF}[args.flux_lds]
+    solver = Solver(cloud, reorder=reorder, flux_lds=flux_lds, **CASE)
+    tiled, flux_tiled, tile_max, _ = solver.layout()
+    log(f"context created in {time.perf_counter() - t:.1f}s; LDS tiles {tiled} (flux "
+        f"{flux_tiled}), largest tile {tile_max} points")
     if args.warmup:
         solver.q_lskum(args.warmup)
     solver.set_timing(True)
@@ -160,7 +162,7 @@ def main():
         "config": {"workload": args.workload, "points": n, "k": 24, "k_mean": round(k_mean, 3),
                    "inner_iterations": 2, "mach": 0.85, "aoa_deg": 1.25, "cfl": 0.2,
                    "vl_const": 5.0, "precision": "fp64", "order": args.reorder,
-                   "lds_tiles": tiled, "tile_max": tile_max},
+                   "lds_tiles": tiled, "flux_lds_tiles": flux_tiled, "tile_max": tile_max},
         "roofline": {"bound": "hbm", "kernel": "k_flux", "achieved": achieved,
                      "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS,
                      "traffic": load_traffic("flux_naca0012_2p5m_k24_traffic.json"),

@@ -23,7 +23,6 @@ using namespace lskum;
 struct lskum_ctx {
   int n = 0;
   lskum_params prm{};
-  int block = 256;
   hipStream_t stream = nullptr;
   std::string err;
   // caller <-> device order
@@ -38,13 +37,14 @@ struct lskum_ctx {
   int cur = 0;
   double *prim = nullptr, *prim_old = nullptr;
   double *flux_res = nullptr, *delta = nullptr, *nx = nullptr, *ny = nullptr;
+  double *coef = nullptr;
   uint8_t *flag = nullptr;
   int32_t *conn_off = nullptr, *perm_d = nullptr;
   uint32_t *conn = nullptr;
   uint16_t *conn16 = nullptr;
   int32_t *tile_off = nullptr, *tile_idx = nullptr;
   int tile_max = 0;
-  bool tiled = false;
+  bool tiled = false, flux_tiled = false;
   Partials part{};
   double *red = nullptr;      // [3] sum, max, argmax of the last state update
   double *hist = nullptr;     // per-iteration sum_res_sqr of lskum_iterate
@@ -79,8 +79,10 @@ struct lskum_ctx {
     c.tile_idx = tile_idx;
     c.tile_max = tile_max;
     c.tiled = tiled;
+    c.flux_tiled = flux_tiled;
     c.nx = nx;
     c.ny = ny;
+    c.coef = coef;
     c.flag = flag;
     c.perm = perm_d;
     return c;
@@ -118,7 +120,7 @@ hipError_t dalloc(T **p, size_t count) {
 
 void release(lskum_ctx *c) {
   if (!c) return;
-  void *ptrs[] = {c->hot[0], c->hot[1], c->qm, c->conn16, c->tile_off, c->tile_idx, c->prim, c->prim_old, c->flux_res, c->delta, c->nx,
+  void *ptrs[] = {c->coef, c->hot[0], c->hot[1], c->qm, c->conn16, c->tile_off, c->tile_idx, c->prim, c->prim_old, c->flux_res, c->delta, c->nx,
                   c->ny, c->flag, c->conn_off, c->perm_d, c->conn, c->part.sum,
                   c->part.max, c->part.arg, c->red, c->hist, c->f_m, c->f_l, c->f_r,
                   c->f_s, c->forces};
@@ -177,10 +179,12 @@ int do_snapshot(lskum_ctx *c) {
   return LSKUM_OK;
 }
 
-int do_func_delta(lskum_ctx *c) {
+// with_speed: first refresh the per-point speed from prim (not needed right
+// after a stage-4 state update, which wrote it)
+int do_func_delta(lskum_ctx *c, bool with_speed) {
   {
     KTimer t(c, LSKUM_K_DELTA);
-    launch_speed(c->dc(), c->prim, c->hot[0], c->stream);
+    if (with_speed) launch_speed(c->dc(), c->prim, c->hot[0], c->stream);
     launch_func_delta(c->dc(), c->hot[0], c->prm.cfl, c->delta, c->stream);
   }
   return check_launch(c, "func_delta");
@@ -222,11 +226,13 @@ int do_flux(lskum_ctx *c) {
   return check_launch(c, "flux_residual");
 }
 
+// the state update also refreshes q (and, after the last stage, the speed
+// func_delta reads), so the next stage needs no separate q_variables pass
 int do_state_update(lskum_ctx *c, int rk, double *hist_slot) {
   {
     KTimer t(c, LSKUM_K_STATE);
     launch_state_update(c->dc(), rk, c->prim, c->prim_old, c->flux_res, c->fc(), c->part,
-                        c->stream);
+                        c->hot[0], c->hot[1], rk == 4, c->stream);
     launch_reduce(c->part, c->red, hist_slot, c->stream);
   }
   return check_launch(c, "state_update");
@@ -285,7 +291,7 @@ int lskum_create(const lskum_cloud *cloud, const lskum_params *params, lskum_ctx
                           p.inner_iterations >= 0 &&
                           (p.precision == LSKUM_FP64 || p.precision == LSKUM_MIXED) &&
                           (p.reorder >= LSKUM_REORDER_NONE && p.reorder <= LSKUM_REORDER_HILBERT) &&
-                          (p.block == 0 || p.block == 64 || p.block == 128 || p.block == 256))) {
+                          (p.flux_lds >= LSKUM_FLUX_LDS_AUTO && p.flux_lds <= LSKUM_FLUX_LDS_OFF))) {
     rc = LSKUM_EINVAL;
     err = "invalid parameters";
   }
@@ -300,7 +306,6 @@ int lskum_create(const lskum_cloud *cloud, const lskum_params *params, lskum_ctx
     return rc;
   }
   c->prm = p;
-  c->block = p.block ? p.block : 256;
   const int32_t n = cloud->n_points;
   c->n = n;
   // set-up on the host: normals and sub-stencils in the caller's order
@@ -361,6 +366,38 @@ int lskum_create(const lskum_cloud *cloud, const lskum_params *params, lskum_ctx
       conn_d[off_h[d] + (t - b)] =
           (uint32_t)c->inv[cloud->conn[t]] | ((uint32_t)c->mask_h[t] << kMaskShift);
   }
+  // least-squares coefficients of each sub-stencil (geometry only): the
+  // sums of interior_fluxes_mod.h:51-61 over the members in conn order, then
+  // (syy, -sxy)/det for the x-form solve and (sxx, -sxy)/det for the y-form
+  std::vector<double> coef_h((size_t)8 * n, 0.0);
+  for (int32_t d = 0; d < n; d++) {
+    const int32_t v = c->perm[d];
+    const double nxv = c->nx_h[v], nyv = c->ny_h[v], tx = nyv, ty = -nxv;
+    double sxx[4] = {0, 0, 0, 0}, syy[4] = {0, 0, 0, 0}, sxy[4] = {0, 0, 0, 0};
+    for (int32_t t = cloud->conn_off[v]; t < cloud->conn_off[v + 1]; t++) {
+      const int32_t k = cloud->conn[t];
+      const double delx = cloud->x[k] - cloud->x[v], dely = cloud->y[k] - cloud->y[v];
+      const double dels = delx * tx + dely * ty, deln = delx * nxv + dely * nyv;
+      double dsw = dels, dnw = deln;
+      if (p.power != 0.0) {
+        const double w = std::pow(std::sqrt(dels * dels + deln * deln), p.power);
+        dsw = dels * w;
+        dnw = deln * w;
+      }
+      for (int sb = 0; sb < 4; sb++) {
+        if (!(c->mask_h[t] & (1u << sb))) continue;
+        sxx[sb] = sxx[sb] + dels * dsw;
+        syy[sb] = syy[sb] + deln * dnw;
+        sxy[sb] = sxy[sb] + dels * dnw;
+      }
+    }
+    for (int sb = 0; sb < 4; sb++) {
+      const double det = sxx[sb] * syy[sb] - sxy[sb] * sxy[sb];
+      const double inv = det != 0.0 ? 1.0 / det : 0.0;
+      coef_h[(size_t)8 * d + 2 * sb] = (sb < 2 ? syy[sb] : sxx[sb]) * inv;
+      coef_h[(size_t)8 * d + 2 * sb + 1] = -sxy[sb] * inv;
+    }
+  }
   // LDS tiles: per block of kTileBlock points, its own points then the
   // sorted other points its lists reach; conn16 holds local indices
   const int nblk = blocks_for(n, kTileBlock);
@@ -411,6 +448,9 @@ int lskum_create(const lskum_cloud *cloud, const lskum_params *params, lskum_ctx
             hipSuccess)
       c->tiled = prepare_tiled(c->tile_max, (size_t)lds_limit);
   }
+  // the flux runs at higher occupancy reading its neighbours through L2
+  // than with a whole-block tile in LDS, so LDS is used only on request
+  c->flux_tiled = c->tiled && p.flux_lds == LSKUM_FLUX_LDS_ON;
   // shape points for the force reduction: the first shape_points points
   // (compute_force_coeffs_mod.h:25-27 takes m = j)
   int shape_points = 0;
@@ -450,6 +490,7 @@ int lskum_create(const lskum_cloud *cloud, const lskum_params *params, lskum_ctx
   ALLOC(c->delta, (size_t)n);
   ALLOC(c->nx, (size_t)n);
   ALLOC(c->ny, (size_t)n);
+  ALLOC(c->coef, (size_t)8 * n);
   ALLOC(c->flag, (size_t)n);
   ALLOC(c->conn_off, (size_t)n + 1);
   ALLOC(c->perm_d, (size_t)n);
@@ -477,6 +518,7 @@ int lskum_create(const lskum_cloud *cloud, const lskum_params *params, lskum_ctx
   std::vector<Up> ups = {{c->hot[0], hot_h.data(), hot_h.size() * sizeof(double)},
                          {c->hot[1], hot_h.data(), hot_h.size() * sizeof(double)},
                          {c->nx, nxd.data(), n * sizeof(double)},
+                         {c->coef, coef_h.data(), coef_h.size() * sizeof(double)},
                          {c->ny, nyd.data(), n * sizeof(double)},
                          {c->flag, flag_h.data(), (size_t)n},
                          {c->conn_off, off_h.data(), (n + 1) * sizeof(int32_t)},
@@ -560,7 +602,7 @@ int lskum_snapshot(lskum_ctx *ctx) {
 
 int lskum_func_delta(lskum_ctx *ctx) {
   CTX_OK(ctx);
-  return do_func_delta(ctx);
+  return do_func_delta(ctx, true);
 }
 
 int lskum_q_variables(lskum_ctx *ctx) {
@@ -641,10 +683,12 @@ int lskum_iterate(lskum_ctx *ctx, int32_t n_iter, double *res_new, double *resid
   }
   const int inner = ctx->prm.inner_iterations;
   for (int t = 0; t < n_iter; t++) {
+    // q and the speed come from the previous stage-4 update, except on the
+    // first iteration of the call (the state may have been set since)
     int rc = do_snapshot(ctx);
-    if (!rc) rc = do_func_delta(ctx);
+    if (!rc) rc = do_func_delta(ctx, t == 0);
     for (int rk = 1; rk <= 4 && !rc; rk++) {
-      rc = do_q_variables(ctx);
+      if (t == 0 && rk == 1) rc = do_q_variables(ctx);
       if (!rc) rc = do_q_derivatives(ctx, inner);
       if (!rc) rc = do_q_inner(ctx, inner);
       if (!rc) rc = do_flux(ctx);
@@ -744,7 +788,7 @@ void *lskum_stream(lskum_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr;
 
 int lskum_layout(lskum_ctx *ctx, int32_t *tiled, int32_t *tile_max, int32_t *order) {
   CTX_OK(ctx);
-  if (tiled) *tiled = ctx->tiled ? 1 : 0;
+  if (tiled) *tiled = (ctx->tiled ? 1 : 0) | (ctx->flux_tiled ? 2 : 0);
   if (tile_max) *tile_max = ctx->tile_max;
   if (order) *order = ctx->prm.reorder;
   return LSKUM_OK;

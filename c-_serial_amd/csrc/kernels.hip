@@ -3,7 +3,7 @@
 // One thread per point, 256 points per block.  Stencil kernels walk the
 // point's CSR neighbour list in the reference's conn order, so every
 // per-point sum is accumulated in the same order as the reference's serial
-// loops (built with -ffp-contract=off).  Neighbour data come from the
+// loops.  Neighbour data come from the
 // block's LDS tile (its own points and every point its lists reach, loaded
 // once per block) when the tiles fit, else straight from the 128-byte hot
 // lines in HBM.
@@ -52,12 +52,28 @@ __device__ __forceinline__ void load_tile(double *lds, const int32_t *__restrict
                                           const int32_t *__restrict__ tile_idx, int b,
                                           const double *__restrict__ src, int src_stride,
                                           int first_chunk, int nchunks, int dst_offset) {
-  const int t0 = tile_off[b], tn = tile_off[b + 1] - t0;
-  for (int t = threadIdx.x; t < tn * nchunks; t += blockDim.x) {
-    const int r = t / nchunks, ch = t - r * nchunks;
-    const double2 v = reinterpret_cast<const double2 *>(
-        src + (size_t)src_stride * tile_idx[t0 + r])[first_chunk + ch];
-    reinterpret_cast<double2 *>(lds + (size_t)STRIDE * r + dst_offset)[ch] = v;
+  // four 16-byte gathers in flight per thread before their LDS stores
+  constexpr int U = 4;
+  const int t0 = tile_off[b], total = (tile_off[b + 1] - t0) * nchunks;
+  for (int base = threadIdx.x; base < total; base += U * blockDim.x) {
+    double2 v[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int t = base + u * blockDim.x;
+      if (t < total) {
+        const int r = t / nchunks, ch = t - r * nchunks;
+        v[u] = reinterpret_cast<const double2 *>(src + (size_t)src_stride *
+                                                           tile_idx[t0 + r])[first_chunk + ch];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int t = base + u * blockDim.x;
+      if (t < total) {
+        const int r = t / nchunks, ch = t - r * nchunks;
+        reinterpret_cast<double2 *>(lds + (size_t)STRIDE * r + dst_offset)[ch] = v[u];
+      }
+    }
   }
 }
 
@@ -294,55 +310,31 @@ __global__ __launch_bounds__(256) void k_q_inner(
 }
 
 // ------------------------------------------------------------ flux residual
-struct Acc {
-  double sxx, syy, sxy, fx[4], fy[4];
-};
-
-__device__ __forceinline__ void acc_zero(Acc &a) {
-  a.sxx = a.syy = a.sxy = 0.0;
-  for (int r = 0; r < 4; r++) a.fx[r] = a.fy[r] = 0.0;
-}
-
-// Adds one neighbour's terms to sub-stencil sums when `on`; adding +0.0 to a
-// sum that started at +0.0 leaves it bit-identical, so masked lanes see the
-// same sequence of sums as the reference's per-sub-stencil loops.
-__device__ __forceinline__ void acc_add(Acc &a, bool on, double ga, double gb, double gc,
-                                        const double Gi[4], const double Gk[4], double dsw,
-                                        double dnw) {
-  a.sxx = a.sxx + (on ? ga : 0.0);
-  a.syy = a.syy + (on ? gb : 0.0);
-  a.sxy = a.sxy + (on ? gc : 0.0);
-  for (int r = 0; r < 4; r++) {
-    const double dG = Gk[r] - Gi[r];
-    a.fx[r] = a.fx[r] + (on ? dG * dsw : 0.0);
-    a.fy[r] = a.fy[r] + (on ? dG * dnw : 0.0);
-  }
-}
-
-// x-form solve (interior_fluxes_mod.h:104-113) or y-form (:324-328)
-__device__ __forceinline__ void acc_solve(const Acc &a, bool yform, double G[4]) {
-  const double det = a.sxx * a.syy - a.sxy * a.sxy;
-  const double one_by_det = 1.0 / det;
-  for (int r = 0; r < 4; r++) {
-    G[r] = yform ? (a.fy[r] * a.sxx - a.fx[r] * a.sxy) * one_by_det
-                 : (a.fx[r] * a.syy - a.fy[r] * a.sxy) * one_by_det;
-  }
-}
-
 // flux_residual_mod.h:10-89 with the ten *_dG* sub-stencil routines of
 // interior_fluxes_mod.h, wall_fluxes_mod.h and outer_fluxes_mod.h fused into
-// one pass over the neighbour list: the limited q~ at both ends and their
-// primitive states depend only on the (point, neighbour) pair, so they are
-// computed once per neighbour and feed the x- and y-direction fluxes of the
-// (usually two) sub-stencils the neighbour belongs to.
+// one pass over the neighbour list.
+//  - The limited q~ at both ends and their primitive states depend only on
+//    the (point, neighbour) pair, so they are computed once per neighbour
+//    and feed the x- and y-direction fluxes of the (usually two)
+//    sub-stencils the neighbour belongs to.
+//  - Each sub-stencil's result G = (Sx*syy - Sy*sxy)/det (x-form,
+//    interior_fluxes_mod.h:104-113) or (Sy*sxx - Sx*sxy)/det (y-form,
+//    :324-328) is linear in the per-neighbour flux differences, and the
+//    geometry sums are constant: with the precomputed coefficients
+//    (a, b) = (syy, -sxy)/det or (sxx, -sxy)/det each neighbour adds
+//    dG * (a*dels_w + b*deln_w) straight into the point's flux residual.
+// coef per point: x+ (a,b), x- (a,b), y+ (a,b), y- (a,b).
+constexpr int COEF_STRIDE = 8;
+
 template <bool TILED, bool POW>
 __global__ __launch_bounds__(256) void k_flux(
     int n, const int32_t *__restrict__ off, const uint32_t *__restrict__ conn,
     const uint16_t *__restrict__ conn16, const int32_t *__restrict__ tile_off,
     const int32_t *__restrict__ tile_idx, const double *__restrict__ hot,
-    const double *__restrict__ qm, const double *__restrict__ nxa,
-    const double *__restrict__ nya, const uint8_t *__restrict__ flag,
-    const double *__restrict__ delta, double power, double *__restrict__ fres) {
+    const double *__restrict__ qm, const double *__restrict__ coef,
+    const double *__restrict__ nxa, const double *__restrict__ nya,
+    const uint8_t *__restrict__ flag, const double *__restrict__ delta, double power,
+    double *__restrict__ fres) {
   extern __shared__ double lds[];
   constexpr int S = 24;  // hot line (16) then qmax, qmin (8)
   const int b = xcd_block();
@@ -357,23 +349,21 @@ __global__ __launch_bounds__(256) void k_flux(
   const double *mi = TILED ? hi + H_STRIDE : qm + (size_t)QM_STRIDE * i;
   const double2 xy = reinterpret_cast<const double2 *>(hi)[0];
   const double x_i = xy.x, y_i = xy.y;
-  double qi[4], dxi[4], dyi[4], qmxi[4], qmni[4];
+  double qi[4], dxi[4], dyi[4], qmxi[4], qmni[4], cf[8];
   load4(hi + H_Q, qi);
   load4(hi + H_DQX, dxi);
   load4(hi + H_DQY, dyi);
   load4(mi, qmxi);
   load4(mi + 4, qmni);
+  load4(coef + (size_t)COEF_STRIDE * i, cf);
+  load4(coef + (size_t)COEF_STRIDE * i + 4, cf + 4);
   const double eps2_i = hi[H_EPS2];
   const double nx = nxa[i], ny = nya[i];
   const double tx = ny, ty = -nx;
   const int type = flag[i];
   // wall: quadrant fluxes with A2neg and Gyn; outer: A2pos and Gyp
   const double s2b = type == 0 ? -1.0 : 1.0;
-  Acc axp, axn, ayp, ayn;
-  acc_zero(axp);
-  acc_zero(axn);
-  acc_zero(ayp);
-  acc_zero(ayn);
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
   const int e1 = off[i + 1];
   for (int e = off[i]; e < e1; e++) {
     uint32_t m;
@@ -397,7 +387,6 @@ __global__ __launch_bounds__(256) void k_flux(
       dsw = dels * weights;
       dnw = deln * weights;
     }
-    const double ga = dels * dsw, gb = deln * dnw, gc = dels * dnw;
     // limited q~ at both ends (interior_fluxes_mod.h:72-83)
     double qti[4], qtk[4];
     for (int r = 0; r < 4; r++) {
@@ -408,11 +397,11 @@ __global__ __launch_bounds__(256) void k_flux(
       qti[r] = qi[r] - 0.5 * phi_i * di;
       qtk[r] = qk[r] - 0.5 * phi_k * dk;
     }
-    double u1, u2, rho, pr;
-    qtilde_to_primitive(qti, u1, u2, rho, pr);
-    const EndTerms<double> ei = end_terms(nx, ny, u1, u2, rho, pr);
-    qtilde_to_primitive(qtk, u1, u2, rho, pr);
-    const EndTerms<double> ek = end_terms(nx, ny, u1, u2, rho, pr);
+    double u1, u2, rho, beta, temp;
+    qtilde_state(qti, u1, u2, rho, beta, temp);
+    const EndTerms<double> ei = end_terms_bt(nx, ny, u1, u2, rho, beta, temp);
+    qtilde_state(qtk, u1, u2, rho, beta, temp);
+    const EndTerms<double> ek = end_terms_bt(nx, ny, u1, u2, rho, beta, temp);
     double Gi[4], Gk[4];
     // x direction: x+ (dels <= 0) or x- (dels >= 0)
     const bool xp = (m & XPOS) != 0;
@@ -424,8 +413,8 @@ __global__ __launch_bounds__(256) void k_flux(
       quad_x(ei, sx, s2b, Gi);
       quad_x(ek, sx, s2b, Gk);
     }
-    acc_add(axp, xp, ga, gb, gc, Gi, Gk, dsw, dnw);
-    acc_add(axn, !xp, ga, gb, gc, Gi, Gk, dsw, dnw);
+    const double wx = xp ? cf[0] * dsw + cf[1] * dnw : cf[2] * dsw + cf[3] * dnw;
+    for (int r = 0; r < 4; r++) acc[r] += (Gk[r] - Gi[r]) * wx;
     if (xp && (m & XNEG)) {  // dels == 0: the neighbour is in both lists
       if (type == 1) {
         split_x(ei, -1.0, Gi);
@@ -434,37 +423,26 @@ __global__ __launch_bounds__(256) void k_flux(
         quad_x(ei, -1.0, s2b, Gi);
         quad_x(ek, -1.0, s2b, Gk);
       }
-      acc_add(axn, true, ga, gb, gc, Gi, Gk, dsw, dnw);
+      const double wn = cf[2] * dsw + cf[3] * dnw;
+      for (int r = 0; r < 4; r++) acc[r] += (Gk[r] - Gi[r]) * wn;
     }
     // y direction: interior y+ / y-, wall y- (all), outer y+ (all)
     const bool yp = type == 1 ? (m & YPOS) != 0 : type == 2;
     const double sy = yp ? 1.0 : -1.0;
     split_y(ei, sy, Gi);
     split_y(ek, sy, Gk);
-    acc_add(ayp, yp, ga, gb, gc, Gi, Gk, dsw, dnw);
-    acc_add(ayn, !yp, ga, gb, gc, Gi, Gk, dsw, dnw);
+    const double wy = yp ? cf[4] * dnw + cf[5] * dsw : cf[6] * dnw + cf[7] * dsw;
+    for (int r = 0; r < 4; r++) acc[r] += (Gk[r] - Gi[r]) * wy;
     if (type == 1 && yp && (m & YNEG)) {  // deln == 0
       split_y(ei, -1.0, Gi);
       split_y(ek, -1.0, Gk);
-      acc_add(ayn, true, ga, gb, gc, Gi, Gk, dsw, dnw);
+      const double wn = cf[6] * dnw + cf[7] * dsw;
+      for (int r = 0; r < 4; r++) acc[r] += (Gk[r] - Gi[r]) * wn;
     }
   }
-  double Gxp[4], Gxn[4], Gyp[4], Gyn[4], out[4];
-  acc_solve(axp, false, Gxp);
-  acc_solve(axn, false, Gxn);
-  const double d = delta[i];
-  if (type == 0) {
-    acc_solve(ayn, true, Gyn);
-    for (int j = 0; j < 4; j++) out[j] = (2.0 * d) * (Gxp[j] + Gxn[j] + Gyn[j]);
-  } else if (type == 1) {
-    acc_solve(ayp, true, Gyp);
-    acc_solve(ayn, true, Gyn);
-    for (int j = 0; j < 4; j++) out[j] = d * (Gxp[j] + Gxn[j] + Gyp[j] + Gyn[j]);
-  } else {
-    acc_solve(ayp, true, Gyp);
-    for (int j = 0; j < 4; j++) out[j] = d * (Gxp[j] + Gxn[j] + Gyp[j]);
-  }
-  for (int j = 0; j < 4; j++) fres[(size_t)j * n + i] = out[j];
+  // flux_residual_mod.h:29 (wall: 2 delta), :45, :85
+  const double d = type == 0 ? 2.0 * delta[i] : delta[i];
+  for (int j = 0; j < 4; j++) fres[(size_t)j * n + i] = d * acc[j];
 }
 
 // ------------------------------------------------------------ state update
@@ -539,7 +517,11 @@ __device__ void block_reduce3(double &s, double &mx, int &arg) {
   arg = sa[0];
 }
 
-// state_update_mod.h:92-283
+// state_update_mod.h:92-283.  It also writes, from the new state, the
+// q-variables of the next stage (q_variables_mod.h:9-34, into both hot-line
+// buffers) and, when `speed` is set, the denominator func_delta divides by
+// (state_update_mod.h:483-485): both are pointwise functions of the state,
+// so computing them here gives the values the separate kernels would.
 __global__ __launch_bounds__(256) void k_state_update(int n, int rk, double *__restrict__ prim,
                                                       const double *__restrict__ prim_old,
                                                       const double *__restrict__ fres,
@@ -549,7 +531,9 @@ __global__ __launch_bounds__(256) void k_state_update(int n, int rk, double *__r
                                                       const int32_t *__restrict__ perm,
                                                       FlowConsts fc, double *__restrict__ psum,
                                                       double *__restrict__ pmax,
-                                                      int32_t *__restrict__ parg) {
+                                                      int32_t *__restrict__ parg,
+                                                      double *__restrict__ hot_a,
+                                                      double *__restrict__ hot_b, int speed) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double res_sqr = 0.0;
   int arg = 0x7fffffff;
@@ -584,11 +568,28 @@ __global__ __launch_bounds__(256) void k_state_update(int n, int rk, double *__r
       res_sqr = (U[0] - temp0) * (U[0] - temp0);
       arg = perm[i];
     }
-    prim[i] = U[0];
     const double temp = 1.0 / U[0];
-    prim[(size_t)n + i] = U[1] * temp;
-    prim[(size_t)2 * n + i] = U[2] * temp;
-    prim[(size_t)3 * n + i] = 0.4 * U[3] - (0.2 * temp) * (U[1] * U[1] + U[2] * U[2]);
+    const double rho = U[0], u1 = U[1] * temp, u2 = U[2] * temp;
+    const double pr = 0.4 * U[3] - (0.2 * temp) * (U[1] * U[1] + U[2] * U[2]);
+    prim[i] = rho;
+    prim[(size_t)n + i] = u1;
+    prim[(size_t)2 * n + i] = u2;
+    prim[(size_t)3 * n + i] = pr;
+    if (hot_a) {
+      const double beta = 0.5 * rho / pr;
+      double q[4];
+      q[0] = log(rho) + (log(beta) * 2.5) - beta * (u1 * u1 + u2 * u2);
+      const double two_times_beta = 2.0 * beta;
+      q[1] = two_times_beta * u1;
+      q[2] = two_times_beta * u2;
+      q[3] = -two_times_beta;
+      store4(hot_a + (size_t)H_STRIDE * i + H_Q, q);
+      store4(hot_b + (size_t)H_STRIDE * i + H_Q, q);
+      if (speed) {
+        const double mod_u = sqrt(u1 * u1 + u2 * u2);
+        hot_a[(size_t)H_STRIDE * i + H_SPEED] = mod_u + 3.0 * sqrt(pr / rho);
+      }
+    }
   }
   double s = res_sqr, mx = res_sqr;
   if (arg == 0x7fffffff) mx = -1.0;
@@ -712,7 +713,7 @@ size_t lds_bytes_delta(int tile_max) { return sizeof(double) * 4 * (size_t)tile_
 size_t lds_bytes_flux(int tile_max) { return sizeof(double) * 24 * (size_t)tile_max; }
 
 bool prepare_tiled(int tile_max, size_t lds_limit) {
-  if (lds_bytes_flux(tile_max) > lds_limit) return false;
+  if (lds_bytes_flux(tile_max) > lds_limit) return false;  // flux needs the most
   struct K { const void *f; size_t bytes; };
   const K ks[] = {
       {(const void *)k_q_derivatives<true, false>, lds_bytes_qderiv(tile_max)},
@@ -782,11 +783,11 @@ void launch_q_inner(const DevCloud &c, const double *src, double *dst, double po
 void launch_flux(const DevCloud &c, const double *hot, const double *qm, const double *delta,
                  double power, double *flux_res, hipStream_t s) {
   dim3 g(blocks_for(c.n, kTileBlock)), b(kTileBlock);
-  const size_t L = c.tiled ? lds_bytes_flux(c.tile_max) : 0;
-#define FLUX_ARGS c.n, TILE_ARGS, hot, qm, c.nx, c.ny, c.flag, delta, power, flux_res
-  if (c.tiled && power == 0.0)
+  const size_t L = c.flux_tiled ? lds_bytes_flux(c.tile_max) : 0;
+#define FLUX_ARGS c.n, TILE_ARGS, hot, qm, c.coef, c.nx, c.ny, c.flag, delta, power, flux_res
+  if (c.flux_tiled && power == 0.0)
     hipLaunchKernelGGL((k_flux<true, false>), g, b, L, s, FLUX_ARGS);
-  else if (c.tiled)
+  else if (c.flux_tiled)
     hipLaunchKernelGGL((k_flux<true, true>), g, b, L, s, FLUX_ARGS);
   else if (power == 0.0)
     hipLaunchKernelGGL((k_flux<false, false>), g, b, 0, s, FLUX_ARGS);
@@ -797,10 +798,11 @@ void launch_flux(const DevCloud &c, const double *hot, const double *qm, const d
 
 void launch_state_update(const DevCloud &c, int rk, double *prim, const double *prim_old,
                          const double *flux_res, const FlowConsts &fc, const Partials &p,
-                         hipStream_t s) {
+                         double *hot_a, double *hot_b, bool speed, hipStream_t s) {
   // fixed 256-thread blocks: the partial layout must not depend on tuning
   hipLaunchKernelGGL(k_state_update, dim3(p.nblocks), dim3(256), 0, s, c.n, rk, prim, prim_old,
-                     flux_res, c.nx, c.ny, c.flag, c.perm, fc, p.sum, p.max, p.arg);
+                     flux_res, c.nx, c.ny, c.flag, c.perm, fc, p.sum, p.max, p.arg, hot_a, hot_b,
+                     speed ? 1 : 0);
 }
 
 void launch_reduce(const Partials &p, double *out, double *hist_slot, hipStream_t s) {

@@ -39,8 +39,11 @@ struct DevCloud {
   const int32_t *tile_off;   // [nblocks + 1]
   const int32_t *tile_idx;   // own points first, then other neighbours
   int tile_max;              // largest tile, entries
-  bool tiled;
+  bool tiled;                // gather kernels use the tiles
+  bool flux_tiled;           // the flux kernel uses them too
   const double *nx, *ny;
+  const double *coef;  // per point: least-squares coefficients of the
+                       // sub-stencils x+, x-, y+, y- (a, b each)
   const uint8_t *flag;
   const int32_t *perm;  // device index -> caller index
 };
@@ -77,9 +80,11 @@ void launch_q_inner(const DevCloud &c, const double *src, double *dst, double po
                     hipStream_t s);
 void launch_flux(const DevCloud &c, const double *hot, const double *qm, const double *delta,
                  double power, double *flux_res, hipStream_t s);
+// also writes q of the new state into hot_a/hot_b (when non-null) and, with
+// speed, the func_delta denominator into hot_a
 void launch_state_update(const DevCloud &c, int rk, double *prim, const double *prim_old,
                          const double *flux_res, const FlowConsts &fc, const Partials &p,
-                         hipStream_t s);
+                         double *hot_a, double *hot_b, bool speed, hipStream_t s);
 // reduces the partials into out[0] = sum, out[1] = max, out[2] = argmax
 // (caller index, as a double), and also hist_slot[0] = sum when non-null
 void launch_reduce(const Partials &p, double *out, double *hist_slot, hipStream_t s);

@@ -1,8 +1,9 @@
 // physics.hpp -- q-LSKUM leaf physics as __device__ functions for gfx950.
 //
-// Each function keeps the reference's operation order (compiled with
-// -ffp-contract=off), so the FP64 form differs from the reference only where
-// the ROCm device libm (exp, log, erf) rounds differently from glibc.
+// The FP64 form keeps the reference's formulas and operation order except
+// where noted; the build contracts a*b+c into FMAs, so results differ from
+// the reference's at the rounding level (the ROCm device exp/log/erf also
+// round differently from glibc by about an ulp).
 //
 // The split/quadrant fluxes are restructured around one set of per-end
 // terms (EndTerms): every flux the reference evaluates for a (point,
@@ -19,6 +20,7 @@ namespace lskum {
 
 constexpr double kPi = 3.14159265358979323846;  // == 4*atan(1.0) in double
 constexpr double kGammaM1 = 1.4 - 1.0;          // gamma_new - 1 as evaluated
+constexpr double kInvGammaM1 = 1.0 / kGammaM1;
 
 // q_variables_mod.h:230-253 (output order u1, u2, rho, pr as there)
 template <typename T>
@@ -34,6 +36,21 @@ __device__ __forceinline__ void qtilde_to_primitive(const T qt[4], T &u1, T &u2,
   pr = rho * temp;
 }
 
+// qtilde_to_primitive that also returns beta and temp = 0.5/beta = p/rho,
+// which the fluxes need again (split_fluxes_mod.h:20,25 recompute them as
+// 0.5*rho/p and p/rho; here they are reused, equal up to rounding), and
+// divides by gamma-1 as a multiplication by its reciprocal.
+template <typename T>
+__device__ __forceinline__ void qtilde_state(const T qt[4], T &u1, T &u2, T &rho, T &beta,
+                                             T &temp) {
+  beta = -qt[3] * T(0.5);
+  temp = T(0.5) / beta;
+  u1 = qt[1] * temp;
+  u2 = qt[2] * temp;
+  const T temp1 = qt[0] + beta * (u1 * u1 + u2 * u2);
+  rho = exp(temp1 - log(beta) * T(kInvGammaM1));
+}
+
 // Terms shared by all split and quadrant fluxes of one state in the frame
 // t = (ny, -nx), n = (nx, ny): split_fluxes_mod.h:14-26 and
 // quadrant_fluxes_mod.h:17-32.
@@ -42,26 +59,33 @@ struct EndTerms {
   T rho, ut, un, pr_by_rho, u_sqr, B1, B2, e1, e2;
 };
 
+// From (beta = rho/2p, p/rho): B1 and B2 share one reciprocal 0.5/sqrt(pi beta).
 template <typename T>
-__device__ __forceinline__ EndTerms<T> end_terms(T nx, T ny, T u1, T u2, T rho,
-                                                 T pr) {
+__device__ __forceinline__ EndTerms<T> end_terms_bt(T nx, T ny, T u1, T u2, T rho, T beta,
+                                                    T pr_by_rho) {
   EndTerms<T> e;
   const T tx = ny, ty = -nx;
   e.rho = rho;
   e.ut = u1 * tx + u2 * ty;
   e.un = u1 * nx + u2 * ny;
-  const T beta = T(0.5) * rho / pr;
   const T sb = sqrt(beta);
   const T S1 = e.ut * sb;
   const T S2 = e.un * sb;
-  const T spb = sqrt(T(kPi) * beta);
-  e.B1 = T(0.5) * exp(-S1 * S1) / spb;
-  e.B2 = T(0.5) * exp(-S2 * S2) / spb;
+  const T h = T(0.5) / sqrt(T(kPi) * beta);
+  e.B1 = exp(-S1 * S1) * h;
+  e.B2 = exp(-S2 * S2) * h;
   e.e1 = erf(S1);
   e.e2 = erf(S2);
-  e.pr_by_rho = pr / rho;
+  e.pr_by_rho = pr_by_rho;
   e.u_sqr = e.ut * e.ut + e.un * e.un;
   return e;
+}
+
+// The same from a primitive state (u1, u2, rho, pr), as the flux functions
+// of split_fluxes_mod.h / quadrant_fluxes_mod.h take it.
+template <typename T>
+__device__ __forceinline__ EndTerms<T> end_terms(T nx, T ny, T u1, T u2, T rho, T pr) {
+  return end_terms_bt(nx, ny, u1, u2, rho, T(0.5) * rho / pr, pr / rho);
 }
 
 // split_fluxes_mod.h:6-43 (s = +1, Gxp) and :45-84 (s = -1, Gxn)

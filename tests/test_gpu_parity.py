@@ -2,9 +2,10 @@
 
 Tolerances are normwise per component, max|gpu - ref| / max|ref| (pointwise
 relative error is ill-posed where a component crosses zero).  The FP64
-kernels keep the reference's operation order, so the only sources of
-difference are the ROCm device exp/log/erf (about an ulp from glibc) and
-the order of the residual reduction:
+kernels keep the reference's formulas; differences come from FMA
+contraction, the flux's reuse of shared terms and precomputed least-squares
+coefficients, the ROCm device exp/log/erf (about an ulp from glibc) and the
+order of the residual reduction:
   - one stage from identical inputs:          <= 1e-12
   - whole iterations vs the reference:        <= 1e-10 (the stated parity bar)
 """
@@ -116,11 +117,13 @@ def test_point_orders_give_bit_identical_states():
     g = golden("k24_m085.npz")
     cloud = lskum.workload("naca0012_3620_k24")
     runs = []
-    for order in [L.REORDER_NONE, L.REORDER_RCM, L.REORDER_HILBERT]:
-        s = _solver(cloud, g["params"], reorder=order)
+    for order, flds in [(L.REORDER_NONE, 0), (L.REORDER_RCM, 0), (L.REORDER_HILBERT, 1),
+                        (L.REORDER_HILBERT, 2)]:
+        s = _solver(cloud, g["params"], reorder=order, flux_lds=flds)
         s.q_lskum(10)
         runs.append((s.layout(), s.prim, s.flux_res, s.dq, s.qm))
-    assert runs[2][0][0], "Hilbert order should give LDS tiles"
+    assert runs[2][0][0] and runs[2][0][1], "Hilbert order should give LDS tiles"
+    assert runs[3][0][0] and not runs[3][0][1]
     for r in runs[1:]:
         for a, b in zip(runs[0][1:], r[1:]):
             np.testing.assert_array_equal(a, b)

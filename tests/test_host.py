@@ -168,10 +168,10 @@ def test_rcm_is_a_permutation_and_narrows_a_shuffled_cloud():
 
 def _params(**kw):
     p = dict(mach=0.63, aoa=2.0, cfl=0.2, vl_const=5.0, power=0.0, inner_iterations=0,
-             precision=0, reorder=0, block=0)
+             precision=0, reorder=0, flux_lds=0)
     p.update(kw)
     return L.Params(p["mach"], p["aoa"], p["cfl"], p["vl_const"], p["power"],
-                    p["inner_iterations"], p["precision"], p["reorder"], p["block"])
+                    p["inner_iterations"], p["precision"], p["reorder"], p["flux_lds"])
 
 
 def _create(cloud, params):
@@ -187,7 +187,7 @@ def _create(cloud, params):
 def test_create_rejects_bad_parameters_before_touching_the_device():
     c = lskum.workload("naca0012_1480_k20")
     for bad in [dict(mach=-1.0), dict(cfl=0.0), dict(inner_iterations=-1), dict(precision=7),
-                dict(reorder=5), dict(block=100)]:
+                dict(reorder=5), dict(flux_lds=9)]:
         rc, msg = _create(c, _params(**bad))
         assert rc == -1, bad
         assert msg
