@@ -103,7 +103,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="naca0012_2p5m_k24")
     ap.add_argument("--reorder", default="hilbert", choices=["hilbert", "rcm", "none"])
-    ap.add_argument("--flux-lds", default="auto", choices=["auto", "on", "off"])
+    ap.add_argument("--lds-tiles", type=int, default=-1,
+                    help="-1 library default, else mask: 1 q_derivatives, 2 q_inner, "
+                         "4 func_delta, 8 flux")
+    ap.add_argument("--tile-block", type=int, default=0, choices=[0, 64, 128, 256])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -120,11 +123,11 @@ def main():
     reorder = {"hilbert": L.REORDER_HILBERT, "rcm": L.REORDER_RCM,
                "none": L.REORDER_NONE}[args.reorder]
     t = time.perf_counter()
-    flux_lds = {"auto": L.FLUX_LDS_AUTO, "on": L.FLUX_LDS_ON, "off": L.FLUX_LDS_OFF}[args.flux_lds]
-    solver = Solver(cloud, reorder=reorder, flux_lds=flux_lds, **CASE)
-    tiled, flux_tiled, tile_max, _ = solver.layout()
-    log(f"context created in {time.perf_counter() - t:.1f}s; LDS tiles {tiled} (flux "
-        f"{flux_tiled}), largest tile {tile_max} points")
+    solver = Solver(cloud, reorder=reorder, lds_tiles=args.lds_tiles, tile_block=args.tile_block,
+                    **CASE)
+    tiles, tile_max, _ = solver.layout()
+    log(f"context created in {time.perf_counter() - t:.1f}s; LDS tile mask {tiles}, "
+        f"largest tile {tile_max} points")
     if args.warmup:
         solver.q_lskum(args.warmup)
     solver.set_timing(True)
@@ -162,7 +165,8 @@ def main():
         "config": {"workload": args.workload, "points": n, "k": 24, "k_mean": round(k_mean, 3),
                    "inner_iterations": 2, "mach": 0.85, "aoa_deg": 1.25, "cfl": 0.2,
                    "vl_const": 5.0, "precision": "fp64", "order": args.reorder,
-                   "lds_tiles": tiled, "flux_lds_tiles": flux_tiled, "tile_max": tile_max},
+                   "lds_tile_mask": tiles, "tile_max": tile_max,
+                   "tile_block": args.tile_block or 256},
         "roofline": {"bound": "hbm", "kernel": "k_flux", "achieved": achieved,
                      "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS,
                      "traffic": load_traffic("flux_naca0012_2p5m_k24_traffic.json"),

@@ -31,9 +31,9 @@ struct lskum_ctx {
   std::vector<int32_t> conn_off_h, conn_h;
   std::vector<uint8_t> mask_h;
   std::vector<double> nx_h, ny_h;
-  // device state: hot lines A/B (which one holds the current derivatives
-  // is `cur`), qmax/qmin, primitive state
-  double *hot[2] = {nullptr, nullptr}, *qm = nullptr;
+  // device state: base records, derivative buffers (dqb[cur] holds the
+  // current derivatives), qmax/qmin, primitive state
+  double *base = nullptr, *dqb[2] = {nullptr, nullptr}, *qm = nullptr;
   int cur = 0;
   double *prim = nullptr, *prim_old = nullptr;
   double *flux_res = nullptr, *delta = nullptr, *nx = nullptr, *ny = nullptr;
@@ -44,7 +44,8 @@ struct lskum_ctx {
   uint16_t *conn16 = nullptr;
   int32_t *tile_off = nullptr, *tile_idx = nullptr;
   int tile_max = 0;
-  bool tiled = false, flux_tiled = false;
+  unsigned tiles = 0;
+  int tile_block = kTileBlockDefault;
   Partials part{};
   double *red = nullptr;      // [3] sum, max, argmax of the last state update
   double *hist = nullptr;     // per-iteration sum_res_sqr of lskum_iterate
@@ -78,8 +79,8 @@ struct lskum_ctx {
     c.tile_off = tile_off;
     c.tile_idx = tile_idx;
     c.tile_max = tile_max;
-    c.tiled = tiled;
-    c.flux_tiled = flux_tiled;
+    c.tiles = tiles;
+    c.tile_block = tile_block;
     c.nx = nx;
     c.ny = ny;
     c.coef = coef;
@@ -120,7 +121,7 @@ hipError_t dalloc(T **p, size_t count) {
 
 void release(lskum_ctx *c) {
   if (!c) return;
-  void *ptrs[] = {c->coef, c->hot[0], c->hot[1], c->qm, c->conn16, c->tile_off, c->tile_idx, c->prim, c->prim_old, c->flux_res, c->delta, c->nx,
+  void *ptrs[] = {c->coef, c->base, c->dqb[0], c->dqb[1], c->qm, c->conn16, c->tile_off, c->tile_idx, c->prim, c->prim_old, c->flux_res, c->delta, c->nx,
                   c->ny, c->flag, c->conn_off, c->perm_d, c->conn, c->part.sum,
                   c->part.max, c->part.arg, c->red, c->hist, c->f_m, c->f_l, c->f_r,
                   c->f_s, c->forces};
@@ -184,8 +185,8 @@ int do_snapshot(lskum_ctx *c) {
 int do_func_delta(lskum_ctx *c, bool with_speed) {
   {
     KTimer t(c, LSKUM_K_DELTA);
-    if (with_speed) launch_speed(c->dc(), c->prim, c->hot[0], c->stream);
-    launch_func_delta(c->dc(), c->hot[0], c->prm.cfl, c->delta, c->stream);
+    if (with_speed) launch_speed(c->dc(), c->prim, c->base, c->stream);
+    launch_func_delta(c->dc(), c->base, c->prm.cfl, c->delta, c->stream);
   }
   return check_launch(c, "func_delta");
 }
@@ -193,26 +194,27 @@ int do_func_delta(lskum_ctx *c, bool with_speed) {
 int do_q_variables(lskum_ctx *c) {
   {
     KTimer t(c, LSKUM_K_QVAR);
-    launch_q_variables(c->dc(), c->prim, c->hot[0], c->hot[1], c->stream);
+    launch_q_variables(c->dc(), c->prim, c->base, c->stream);
   }
   return check_launch(c, "q_variables");
 }
 
-// derivatives into the buffer from which n_later Jacobi passes end in A
+// derivatives into the buffer from which n_later Jacobi passes end in dqb[0]
 int do_q_derivatives(lskum_ctx *c, int n_later) {
   c->cur = n_later % 2;
   {
     KTimer t(c, LSKUM_K_QDERIV);
-    launch_q_derivatives(c->dc(), c->hot[c->cur], c->qm, c->prm.power, c->stream);
+    launch_q_derivatives(c->dc(), c->base, c->dqb[c->cur], c->qm, c->prm.power, c->stream);
   }
   return check_launch(c, "q_derivatives");
 }
 
-// n Jacobi passes, each reading hot[cur] and writing the other buffer
+// n Jacobi passes, each reading dqb[cur] and writing the other buffer
 int do_q_inner(lskum_ctx *c, int n_inner) {
   for (int p = 0; p < n_inner; p++) {
     KTimer t(c, LSKUM_K_QINNER);
-    launch_q_inner(c->dc(), c->hot[c->cur], c->hot[1 - c->cur], c->prm.power, c->stream);
+    launch_q_inner(c->dc(), c->base, c->dqb[c->cur], c->dqb[1 - c->cur], c->prm.power,
+                   c->stream);
     c->cur = 1 - c->cur;
   }
   return check_launch(c, "q_inner");
@@ -221,7 +223,8 @@ int do_q_inner(lskum_ctx *c, int n_inner) {
 int do_flux(lskum_ctx *c) {
   {
     KTimer t(c, LSKUM_K_FLUX);
-    launch_flux(c->dc(), c->hot[c->cur], c->qm, c->delta, c->prm.power, c->flux_res, c->stream);
+    launch_flux(c->dc(), c->base, c->dqb[c->cur], c->qm, c->delta, c->prm.power, c->flux_res,
+                c->stream);
   }
   return check_launch(c, "flux_residual");
 }
@@ -232,7 +235,7 @@ int do_state_update(lskum_ctx *c, int rk, double *hist_slot) {
   {
     KTimer t(c, LSKUM_K_STATE);
     launch_state_update(c->dc(), rk, c->prim, c->prim_old, c->flux_res, c->fc(), c->part,
-                        c->hot[0], c->hot[1], rk == 4, c->stream);
+                        c->base, rk == 4, c->stream);
     launch_reduce(c->part, c->red, hist_slot, c->stream);
   }
   return check_launch(c, "state_update");
@@ -291,7 +294,10 @@ int lskum_create(const lskum_cloud *cloud, const lskum_params *params, lskum_ctx
                           p.inner_iterations >= 0 &&
                           (p.precision == LSKUM_FP64 || p.precision == LSKUM_MIXED) &&
                           (p.reorder >= LSKUM_REORDER_NONE && p.reorder <= LSKUM_REORDER_HILBERT) &&
-                          (p.flux_lds >= LSKUM_FLUX_LDS_AUTO && p.flux_lds <= LSKUM_FLUX_LDS_OFF))) {
+                          (p.lds_tiles == LSKUM_TILES_AUTO ||
+                           (p.lds_tiles >= 0 && p.lds_tiles <= 15)) &&
+                          (p.tile_block == 0 || p.tile_block == 64 || p.tile_block == 128 ||
+                           p.tile_block == 256))) {
     rc = LSKUM_EINVAL;
     err = "invalid parameters";
   }
@@ -306,6 +312,7 @@ int lskum_create(const lskum_cloud *cloud, const lskum_params *params, lskum_ctx
     return rc;
   }
   c->prm = p;
+  c->tile_block = p.tile_block ? p.tile_block : kTileBlockDefault;
   const int32_t n = cloud->n_points;
   c->n = n;
   // set-up on the host: normals and sub-stencils in the caller's order
@@ -346,17 +353,17 @@ int lskum_create(const lskum_cloud *cloud, const lskum_params *params, lskum_ctx
   c->inv.assign(n, 0);
   for (int32_t d = 0; d < n; d++) c->inv[c->perm[d]] = d;
   // host images of the device arrays
-  std::vector<double> hot_h((size_t)H_STRIDE * n, 0.0), nxd(n), nyd(n);
+  std::vector<double> base_h((size_t)B_STRIDE * n, 0.0), nxd(n), nyd(n);
   std::vector<uint8_t> flag_h(n);
   std::vector<int32_t> off_h(n + 1, 0);
   std::vector<uint32_t> conn_d(cloud->conn_off[n]);
   for (int32_t d = 0; d < n; d++) {
     const int32_t v = c->perm[d];
-    hot_h[(size_t)H_STRIDE * d + H_X] = cloud->x[v];
-    hot_h[(size_t)H_STRIDE * d + H_Y] = cloud->y[v];
+    base_h[(size_t)B_STRIDE * d + B_X] = cloud->x[v];
+    base_h[(size_t)B_STRIDE * d + B_Y] = cloud->y[v];
     // limiters_mod.h:82-83: epsi = pow(VL_CONST*min_dist, 3), used squared
     const double epsi = std::pow(p.vl_const * cloud->min_dist[v], 3);
-    hot_h[(size_t)H_STRIDE * d + H_EPS2] = epsi * epsi;
+    base_h[(size_t)B_STRIDE * d + B_EPS2] = epsi * epsi;
     nxd[d] = c->nx_h[v];
     nyd[d] = c->ny_h[v];
     flag_h[d] = (uint8_t)cloud->flag_1[v];
@@ -398,16 +405,17 @@ int lskum_create(const lskum_cloud *cloud, const lskum_params *params, lskum_ctx
       coef_h[(size_t)8 * d + 2 * sb + 1] = -sxy[sb] * inv;
     }
   }
-  // LDS tiles: per block of kTileBlock points, its own points then the
+  // LDS tiles: per block of tile_block points, its own points then the
   // sorted other points its lists reach; conn16 holds local indices
-  const int nblk = blocks_for(n, kTileBlock);
+  const int TB = c->tile_block;
+  const int nblk = blocks_for(n, TB);
   std::vector<int32_t> toff(nblk + 1, 0), tidx;
   std::vector<uint16_t> conn16_h(conn_d.size());
   bool tiles_ok = true;
   {
     std::vector<int32_t> loc(n, -1), extra;
     for (int blk = 0; blk < nblk && tiles_ok; blk++) {
-      const int32_t p0 = blk * kTileBlock, p1 = std::min(n, p0 + kTileBlock);
+      const int32_t p0 = blk * TB, p1 = std::min(n, p0 + TB);
       extra.clear();
       for (int32_t d = p0; d < p1; d++)
         for (int32_t e = off_h[d]; e < off_h[d + 1]; e++) {
@@ -441,16 +449,18 @@ int lskum_create(const lskum_cloud *cloud, const lskum_params *params, lskum_ctx
       for (int32_t k : extra) loc[k] = -1;
     }
   }
-  if (tiles_ok) {
+  // the gather kernels stage neighbours in LDS; the flux runs at higher
+  // occupancy reading its neighbours through L2, so by default it does not
+  const unsigned want = p.lds_tiles == LSKUM_TILES_AUTO
+                            ? (unsigned)(LSKUM_TILE_QDERIV | LSKUM_TILE_QINNER | LSKUM_TILE_DELTA)
+                            : (unsigned)p.lds_tiles;
+  if (tiles_ok && want) {
     int dev = 0, lds_limit = 0;
     if (hipGetDevice(&dev) == hipSuccess &&
         hipDeviceGetAttribute(&lds_limit, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) ==
             hipSuccess)
-      c->tiled = prepare_tiled(c->tile_max, (size_t)lds_limit);
+      c->tiles = prepare_tiled(want, c->tile_max, (size_t)lds_limit);
   }
-  // the flux runs at higher occupancy reading its neighbours through L2
-  // than with a whole-block tile in LDS, so LDS is used only on request
-  c->flux_tiled = c->tiled && p.flux_lds == LSKUM_FLUX_LDS_ON;
   // shape points for the force reduction: the first shape_points points
   // (compute_force_coeffs_mod.h:25-27 takes m = j)
   int shape_points = 0;
@@ -481,8 +491,9 @@ int lskum_create(const lskum_cloud *cloud, const lskum_params *params, lskum_ctx
     return fail(LSKUM_EHIP);
   }
   c->part.nblocks = blocks_for(n, 256);
-  ALLOC(c->hot[0], (size_t)H_STRIDE * n);
-  ALLOC(c->hot[1], (size_t)H_STRIDE * n);
+  ALLOC(c->base, (size_t)B_STRIDE * n);
+  ALLOC(c->dqb[0], (size_t)DQ_STRIDE * n);
+  ALLOC(c->dqb[1], (size_t)DQ_STRIDE * n);
   ALLOC(c->qm, (size_t)QM_STRIDE * n);
   ALLOC(c->prim, (size_t)4 * n);
   ALLOC(c->prim_old, (size_t)4 * n);
@@ -495,7 +506,7 @@ int lskum_create(const lskum_cloud *cloud, const lskum_params *params, lskum_ctx
   ALLOC(c->conn_off, (size_t)n + 1);
   ALLOC(c->perm_d, (size_t)n);
   ALLOC(c->conn, conn_d.size());
-  if (c->tiled) {
+  if (c->tiles) {
     ALLOC(c->conn16, conn16_h.size());
     ALLOC(c->tile_off, toff.size());
     ALLOC(c->tile_idx, tidx.size());
@@ -515,8 +526,7 @@ int lskum_create(const lskum_cloud *cloud, const lskum_params *params, lskum_ctx
     return fail(LSKUM_ENOMEM);
   }
   struct Up { void *dst; const void *src; size_t bytes; };
-  std::vector<Up> ups = {{c->hot[0], hot_h.data(), hot_h.size() * sizeof(double)},
-                         {c->hot[1], hot_h.data(), hot_h.size() * sizeof(double)},
+  std::vector<Up> ups = {{c->base, base_h.data(), base_h.size() * sizeof(double)},
                          {c->nx, nxd.data(), n * sizeof(double)},
                          {c->coef, coef_h.data(), coef_h.size() * sizeof(double)},
                          {c->ny, nyd.data(), n * sizeof(double)},
@@ -528,7 +538,7 @@ int lskum_create(const lskum_cloud *cloud, const lskum_params *params, lskum_ctx
                          {c->f_l, fl.data(), fl.size() * sizeof(int32_t)},
                          {c->f_r, fr.data(), fr.size() * sizeof(int32_t)},
                          {c->f_s, fs.data(), fs.size() * sizeof(int32_t)}};
-  if (c->tiled) {
+  if (c->tiles) {
     ups.push_back({c->conn16, conn16_h.data(), conn16_h.size() * sizeof(uint16_t)});
     ups.push_back({c->tile_off, toff.data(), toff.size() * sizeof(int32_t)});
     ups.push_back({c->tile_idx, tidx.data(), tidx.size() * sizeof(int32_t)});
@@ -541,6 +551,8 @@ int lskum_create(const lskum_cloud *cloud, const lskum_params *params, lskum_ctx
     }
   }
   if ((e = hipMemset(c->qm, 0, sizeof(double) * QM_STRIDE * n)) != hipSuccess ||
+      (e = hipMemset(c->dqb[0], 0, sizeof(double) * DQ_STRIDE * n)) != hipSuccess ||
+      (e = hipMemset(c->dqb[1], 0, sizeof(double) * DQ_STRIDE * n)) != hipSuccess ||
       (e = hipMemset(c->flux_res, 0, sizeof(double) * 4 * n)) != hipSuccess ||
       (e = hipMemset(c->delta, 0, sizeof(double) * n)) != hipSuccess) {
     c->err = std::string("hipMemset: ") + hipGetErrorString(e);
@@ -651,7 +663,7 @@ int lskum_state_update(lskum_ctx *ctx, int32_t rk, double *sum_res_sqr, double *
 int lskum_forces(lskum_ctx *ctx, double *cl, double *cd, double *cm) {
   CTX_OK(ctx);
   const double qdyn = 0.5 * ctx->rho_inf * ctx->prm.mach * ctx->prm.mach;
-  launch_forces(ctx->n_shape, ctx->f_m, ctx->f_l, ctx->f_r, ctx->f_s, ctx->hot[0], ctx->nx, ctx->ny,
+  launch_forces(ctx->n_shape, ctx->f_m, ctx->f_l, ctx->f_r, ctx->f_s, ctx->base, ctx->nx, ctx->ny,
                 ctx->prim, ctx->n, ctx->pr_inf, qdyn, ctx->forces, ctx->stream);
   int rc = check_launch(ctx, "forces");
   if (rc) return rc;
@@ -747,9 +759,9 @@ int lskum_get_array(lskum_ctx *ctx, int32_t which, double *dst) {
   switch (which) {
     case LSKUM_A_PRIM: rc = fetch_plain(ctx, ctx->prim, 4, dev); ncomp = 4; break;
     case LSKUM_A_PRIM_OLD: rc = fetch_plain(ctx, ctx->prim_old, 4, dev); ncomp = 4; break;
-    case LSKUM_A_Q: rc = fetch_record(ctx, ctx->hot[0], H_STRIDE, H_Q, 4, dev); ncomp = 4; break;
+    case LSKUM_A_Q: rc = fetch_record(ctx, ctx->base, B_STRIDE, B_Q, 4, dev); ncomp = 4; break;
     case LSKUM_A_DQ:
-      rc = fetch_record(ctx, ctx->hot[ctx->cur], H_STRIDE, H_DQX, 8, dev);
+      rc = fetch_record(ctx, ctx->dqb[ctx->cur], DQ_STRIDE, 0, 8, dev);
       ncomp = 8;
       break;
     case LSKUM_A_QM: rc = fetch_record(ctx, ctx->qm, QM_STRIDE, 0, 8, dev); ncomp = 8; break;
@@ -788,7 +800,7 @@ void *lskum_stream(lskum_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr;
 
 int lskum_layout(lskum_ctx *ctx, int32_t *tiled, int32_t *tile_max, int32_t *order) {
   CTX_OK(ctx);
-  if (tiled) *tiled = (ctx->tiled ? 1 : 0) | (ctx->flux_tiled ? 2 : 0);
+  if (tiled) *tiled = (int32_t)ctx->tiles;
   if (tile_max) *tile_max = ctx->tile_max;
   if (order) *order = ctx->prm.reorder;
   return LSKUM_OK;

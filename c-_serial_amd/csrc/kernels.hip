@@ -5,8 +5,8 @@
 // per-point sum is accumulated in the same order as the reference's serial
 // loops.  Neighbour data come from the
 // block's LDS tile (its own points and every point its lists reach, loaded
-// once per block) when the tiles fit, else straight from the 128-byte hot
-// lines in HBM.
+// once per block) for the kernels configured so, else straight from the
+// 64-byte per-point records through L2.
 #include <hip/hip_runtime.h>
 
 #include "kernels.hpp"
@@ -53,6 +53,7 @@ __device__ __forceinline__ void load_tile(double *lds, const int32_t *__restrict
                                           const double *__restrict__ src, int src_stride,
                                           int first_chunk, int nchunks, int dst_offset) {
   // four 16-byte gathers in flight per thread before their LDS stores
+  // (measured faster than eight on the 2.5M-point cloud)
   constexpr int U = 4;
   const int t0 = tile_off[b], total = (tile_off[b + 1] - t0) * nchunks;
   for (int base = threadIdx.x; base < total; base += U * blockDim.x) {
@@ -90,10 +91,9 @@ __device__ __forceinline__ int nbr_index(const uint32_t *conn, const uint16_t *c
 }
 
 // ------------------------------------------------------------ q variables
-// q_variables_mod.h:9-34, written into both hot-line buffers
+// q_variables_mod.h:9-34
 __global__ __launch_bounds__(256) void k_q_variables(int n, const double *__restrict__ prim,
-                                                     double *__restrict__ hot_a,
-                                                     double *__restrict__ hot_b) {
+                                                     double *__restrict__ base) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
   const double rho = prim[k], u1 = prim[n + k], u2 = prim[2 * n + k], pr = prim[3 * n + k];
@@ -104,19 +104,18 @@ __global__ __launch_bounds__(256) void k_q_variables(int n, const double *__rest
   q[1] = two_times_beta * u1;
   q[2] = two_times_beta * u2;
   q[3] = -two_times_beta;
-  store4(hot_a + (size_t)H_STRIDE * k + H_Q, q);
-  store4(hot_b + (size_t)H_STRIDE * k + H_Q, q);
+  store4(base + (size_t)B_STRIDE * k + B_Q, q);
 }
 
 // The denominator of state_update_mod.h:485 for the point's own state:
 // func_delta divides each neighbour distance by its neighbour's value.
 __global__ __launch_bounds__(256) void k_speed(int n, const double *__restrict__ prim,
-                                               double *__restrict__ hot) {
+                                               double *__restrict__ base) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
   const double rho = prim[k], u1 = prim[n + k], u2 = prim[2 * n + k], pr = prim[3 * n + k];
   const double mod_u = sqrt(u1 * u1 + u2 * u2);
-  hot[(size_t)H_STRIDE * k + H_SPEED] = mod_u + 3.0 * sqrt(pr / rho);
+  base[(size_t)B_STRIDE * k + B_SPEED] = mod_u + 3.0 * sqrt(pr / rho);
 }
 
 // ------------------------------------------------------------ local dt
@@ -127,7 +126,7 @@ __global__ __launch_bounds__(256) void k_func_delta(int n, const int32_t *__rest
                                                     const uint16_t *__restrict__ conn16,
                                                     const int32_t *__restrict__ tile_off,
                                                     const int32_t *__restrict__ tile_idx,
-                                                    const double *__restrict__ hot, double cfl,
+                                                    const double *__restrict__ base, double cfl,
                                                     double *__restrict__ delta) {
   extern __shared__ double lds[];
   constexpr int S = 4;  // x, y, speed, pad
@@ -135,24 +134,24 @@ __global__ __launch_bounds__(256) void k_func_delta(int n, const int32_t *__rest
   if (TILED) {
     const int t0 = tile_off[b], tn = tile_off[b + 1] - t0;
     for (int t = threadIdx.x; t < tn; t += blockDim.x) {
-      const double *h = hot + (size_t)H_STRIDE * tile_idx[t0 + t];
+      const double *h = base + (size_t)B_STRIDE * tile_idx[t0 + t];
       reinterpret_cast<double2 *>(lds + S * t)[0] = reinterpret_cast<const double2 *>(h)[0];
-      lds[S * t + 2] = h[H_SPEED];
+      lds[S * t + 2] = h[B_SPEED];
     }
     __syncthreads();
   }
   const int i = b * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const double *hi = TILED ? lds + S * threadIdx.x : hot + (size_t)H_STRIDE * i;
+  const double *hi = TILED ? lds + S * threadIdx.x : base + (size_t)B_STRIDE * i;
   const double2 xy = reinterpret_cast<const double2 *>(hi)[0];
   double min_delt = 1.0;
   const int e1 = off[i + 1];
   for (int e = off[i]; e < e1; e++) {
     uint32_t m;
     const int k = nbr_index(conn, conn16, e, TILED, m);
-    const double *hk = TILED ? lds + S * k : hot + (size_t)H_STRIDE * k;
+    const double *hk = TILED ? lds + S * k : base + (size_t)B_STRIDE * k;
     const double2 xyk = reinterpret_cast<const double2 *>(hk)[0];
-    const double speed = TILED ? hk[2] : hk[H_SPEED];
+    const double speed = TILED ? hk[2] : hk[B_SPEED];
     double dist = (xyk.x - xy.x) * (xyk.x - xy.x) + (xyk.y - xy.y) * (xyk.y - xy.y);
     dist = sqrt(dist);
     double delta_t = dist / speed;
@@ -165,28 +164,27 @@ __global__ __launch_bounds__(256) void k_func_delta(int n, const int32_t *__rest
 // ------------------------------------------------------------ q derivatives
 // q_variables_mod.h:36-118: unweighted (power = 0) or pow(dist, power)
 // weighted least squares over the full stencil, and qmax/qmin over the
-// point and its neighbours.  Reads x, y, q of the hot line and writes its
-// dq slots (disjoint from what other threads read) and qm.
+// point and its neighbours.
 template <bool TILED, bool POW>
 __global__ __launch_bounds__(256) void k_q_derivatives(
     int n, const int32_t *__restrict__ off, const uint32_t *__restrict__ conn,
     const uint16_t *__restrict__ conn16, const int32_t *__restrict__ tile_off,
-    const int32_t *__restrict__ tile_idx, double *hot, double *__restrict__ qm_out,
-    double power) {
+    const int32_t *__restrict__ tile_idx, const double *__restrict__ base,
+    double *__restrict__ dq_out, double *__restrict__ qm_out, double power) {
   extern __shared__ double lds[];
   constexpr int S = 6;  // x, y, q[4]
   const int b = xcd_block();
   if (TILED) {
-    load_tile<S>(lds, tile_off, tile_idx, b, hot, H_STRIDE, 0, 3, 0);
+    load_tile<S>(lds, tile_off, tile_idx, b, base, B_STRIDE, 0, 3, 0);
     __syncthreads();
   }
   const int i = b * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const double *hi = TILED ? lds + S * threadIdx.x : hot + (size_t)H_STRIDE * i;
+  const double *hi = TILED ? lds + S * threadIdx.x : base + (size_t)B_STRIDE * i;
   const double2 xy = reinterpret_cast<const double2 *>(hi)[0];
   const double x_i = xy.x, y_i = xy.y;
   double qi[4], qmax[4], qmin[4];
-  load4(hi + H_Q, qi);
+  load4(hi + B_Q, qi);
   double sum_delx_sqr = 0., sum_dely_sqr = 0., sum_delx_dely = 0.;
   double sdx[4] = {0.0, 0.0, 0.0, 0.0}, sdy[4] = {0.0, 0.0, 0.0, 0.0};
   for (int r = 0; r < 4; r++) qmax[r] = qmin[r] = qi[r];
@@ -194,10 +192,10 @@ __global__ __launch_bounds__(256) void k_q_derivatives(
   for (int e = off[i]; e < e1; e++) {
     uint32_t m;
     const int k = nbr_index(conn, conn16, e, TILED, m);
-    const double *hk = TILED ? lds + S * k : hot + (size_t)H_STRIDE * k;
+    const double *hk = TILED ? lds + S * k : base + (size_t)B_STRIDE * k;
     const double2 xyk = reinterpret_cast<const double2 *>(hk)[0];
     double qk[4];
-    load4(hk + H_Q, qk);
+    load4(hk + B_Q, qk);
     for (int r = 0; r < 4; r++) {
       if (qk[r] > qmax[r]) qmax[r] = qk[r];
       if (qk[r] < qmin[r]) qmin[r] = qk[r];
@@ -229,9 +227,9 @@ __global__ __launch_bounds__(256) void k_q_derivatives(
     dqx[r] = (sdx[r] * sum_dely_sqr - sdy[r] * sum_delx_dely) * one_by_det;
     dqy[r] = (sdy[r] * sum_delx_sqr - sdx[r] * sum_delx_dely) * one_by_det;
   }
-  double *w = hot + (size_t)H_STRIDE * i;
-  store4(w + H_DQX, dqx);
-  store4(w + H_DQY, dqy);
+  double *w = dq_out + (size_t)DQ_STRIDE * i;
+  store4(w, dqx);
+  store4(w + 4, dqy);
   double *qmw = qm_out + (size_t)QM_STRIDE * i;
   store4(qmw, qmax);
   store4(qmw + 4, qmin);
@@ -239,41 +237,44 @@ __global__ __launch_bounds__(256) void k_q_derivatives(
 
 // ------------------------------------------------------------ inner passes
 // q_variables_mod.h:120-228: one defect-corrected LS pass, Jacobi (reads
-// src, writes the dq slots of dst: the reference's temp array + copy).
+// src, writes dst: the reference's temp array + copy).
 template <bool TILED, bool POW>
 __global__ __launch_bounds__(256) void k_q_inner(
     int n, const int32_t *__restrict__ off, const uint32_t *__restrict__ conn,
     const uint16_t *__restrict__ conn16, const int32_t *__restrict__ tile_off,
-    const int32_t *__restrict__ tile_idx, const double *__restrict__ src,
-    double *__restrict__ dst, double power) {
+    const int32_t *__restrict__ tile_idx, const double *__restrict__ base,
+    const double *__restrict__ src, double *__restrict__ dst, double power) {
   extern __shared__ double lds[];
   constexpr int S = 14;  // x, y, q[4], dqx[4], dqy[4]
   const int b = xcd_block();
   if (TILED) {
-    load_tile<S>(lds, tile_off, tile_idx, b, src, H_STRIDE, 0, 7, 0);
+    load_tile<S>(lds, tile_off, tile_idx, b, base, B_STRIDE, 0, 3, 0);
+    load_tile<S>(lds, tile_off, tile_idx, b, src, DQ_STRIDE, 0, 4, 6);
     __syncthreads();
   }
   const int i = b * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const double *hi = TILED ? lds + S * threadIdx.x : src + (size_t)H_STRIDE * i;
+  const double *hi = TILED ? lds + S * threadIdx.x : base + (size_t)B_STRIDE * i;
+  const double *di = TILED ? hi + 6 : src + (size_t)DQ_STRIDE * i;
   const double2 xy = reinterpret_cast<const double2 *>(hi)[0];
   const double x_i = xy.x, y_i = xy.y;
   double qi[4], dxi[4], dyi[4];
-  load4(hi + H_Q, qi);
-  load4(hi + H_DQX, dxi);
-  load4(hi + H_DQY, dyi);
+  load4(hi + B_Q, qi);
+  load4(di, dxi);
+  load4(di + 4, dyi);
   double sum_delx_sqr = 0., sum_dely_sqr = 0., sum_delx_dely = 0.;
   double sx[4] = {0.0, 0.0, 0.0, 0.0}, sy[4] = {0.0, 0.0, 0.0, 0.0};
   const int e1 = off[i + 1];
   for (int e = off[i]; e < e1; e++) {
     uint32_t m;
     const int k = nbr_index(conn, conn16, e, TILED, m);
-    const double *hk = TILED ? lds + S * k : src + (size_t)H_STRIDE * k;
+    const double *hk = TILED ? lds + S * k : base + (size_t)B_STRIDE * k;
+    const double *dk = TILED ? hk + 6 : src + (size_t)DQ_STRIDE * k;
     const double2 xyk = reinterpret_cast<const double2 *>(hk)[0];
     double qk[4], dxk[4], dyk[4];
-    load4(hk + H_Q, qk);
-    load4(hk + H_DQX, dxk);
-    load4(hk + H_DQY, dyk);
+    load4(hk + B_Q, qk);
+    load4(dk, dxk);
+    load4(dk + 4, dyk);
     const double delx = xyk.x - x_i, dely = xyk.y - y_i;
     const double weights = POW ? pow(sqrt(delx * delx + dely * dely), power) : 1.0;
     if (POW) {
@@ -304,9 +305,9 @@ __global__ __launch_bounds__(256) void k_q_inner(
     ox[r] = one_by_det * (sx[r] * sum_dely_sqr - sy[r] * sum_delx_dely);
     oy[r] = one_by_det * (sy[r] * sum_delx_sqr - sx[r] * sum_delx_dely);
   }
-  double *d = dst + (size_t)H_STRIDE * i;
-  store4(d + H_DQX, ox);
-  store4(d + H_DQY, oy);
+  double *d = dst + (size_t)DQ_STRIDE * i;
+  store4(d, ox);
+  store4(d + 4, oy);
 }
 
 // ------------------------------------------------------------ flux residual
@@ -330,34 +331,36 @@ template <bool TILED, bool POW>
 __global__ __launch_bounds__(256) void k_flux(
     int n, const int32_t *__restrict__ off, const uint32_t *__restrict__ conn,
     const uint16_t *__restrict__ conn16, const int32_t *__restrict__ tile_off,
-    const int32_t *__restrict__ tile_idx, const double *__restrict__ hot,
-    const double *__restrict__ qm, const double *__restrict__ coef,
-    const double *__restrict__ nxa, const double *__restrict__ nya,
-    const uint8_t *__restrict__ flag, const double *__restrict__ delta, double power,
-    double *__restrict__ fres) {
+    const int32_t *__restrict__ tile_idx, const double *__restrict__ base,
+    const double *__restrict__ dq, const double *__restrict__ qm,
+    const double *__restrict__ coef, const double *__restrict__ nxa,
+    const double *__restrict__ nya, const uint8_t *__restrict__ flag,
+    const double *__restrict__ delta, double power, double *__restrict__ fres) {
   extern __shared__ double lds[];
-  constexpr int S = 24;  // hot line (16) then qmax, qmin (8)
+  constexpr int S = 24;  // base (8), dq (8), qm (8)
   const int b = xcd_block();
   if (TILED) {
-    load_tile<S>(lds, tile_off, tile_idx, b, hot, H_STRIDE, 0, 8, 0);
-    load_tile<S>(lds, tile_off, tile_idx, b, qm, QM_STRIDE, 0, 4, H_STRIDE);
+    load_tile<S>(lds, tile_off, tile_idx, b, base, B_STRIDE, 0, 4, 0);
+    load_tile<S>(lds, tile_off, tile_idx, b, dq, DQ_STRIDE, 0, 4, 8);
+    load_tile<S>(lds, tile_off, tile_idx, b, qm, QM_STRIDE, 0, 4, 16);
     __syncthreads();
   }
   const int i = b * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const double *hi = TILED ? lds + S * threadIdx.x : hot + (size_t)H_STRIDE * i;
-  const double *mi = TILED ? hi + H_STRIDE : qm + (size_t)QM_STRIDE * i;
+  const double *hi = TILED ? lds + S * threadIdx.x : base + (size_t)B_STRIDE * i;
+  const double *di = TILED ? hi + 8 : dq + (size_t)DQ_STRIDE * i;
+  const double *mi = TILED ? hi + 16 : qm + (size_t)QM_STRIDE * i;
   const double2 xy = reinterpret_cast<const double2 *>(hi)[0];
   const double x_i = xy.x, y_i = xy.y;
   double qi[4], dxi[4], dyi[4], qmxi[4], qmni[4], cf[8];
-  load4(hi + H_Q, qi);
-  load4(hi + H_DQX, dxi);
-  load4(hi + H_DQY, dyi);
+  load4(hi + B_Q, qi);
+  load4(di, dxi);
+  load4(di + 4, dyi);
   load4(mi, qmxi);
   load4(mi + 4, qmni);
   load4(coef + (size_t)COEF_STRIDE * i, cf);
   load4(coef + (size_t)COEF_STRIDE * i + 4, cf + 4);
-  const double eps2_i = hi[H_EPS2];
+  const double eps2_i = hi[B_EPS2];
   const double nx = nxa[i], ny = nya[i];
   const double tx = ny, ty = -nx;
   const int type = flag[i];
@@ -368,16 +371,17 @@ __global__ __launch_bounds__(256) void k_flux(
   for (int e = off[i]; e < e1; e++) {
     uint32_t m;
     const int k = nbr_index(conn, conn16, e, TILED, m);
-    const double *hk = TILED ? lds + S * k : hot + (size_t)H_STRIDE * k;
-    const double *mk = TILED ? hk + H_STRIDE : qm + (size_t)QM_STRIDE * k;
+    const double *hk = TILED ? lds + S * k : base + (size_t)B_STRIDE * k;
+    const double *dk = TILED ? hk + 8 : dq + (size_t)DQ_STRIDE * k;
+    const double *mk = TILED ? hk + 16 : qm + (size_t)QM_STRIDE * k;
     const double2 xyk = reinterpret_cast<const double2 *>(hk)[0];
     double qk[4], dxk[4], dyk[4], qmxk[4], qmnk[4];
-    load4(hk + H_Q, qk);
-    load4(hk + H_DQX, dxk);
-    load4(hk + H_DQY, dyk);
+    load4(hk + B_Q, qk);
+    load4(dk, dxk);
+    load4(dk + 4, dyk);
     load4(mk, qmxk);
     load4(mk + 4, qmnk);
-    const double eps2_k = hk[H_EPS2];
+    const double eps2_k = hk[B_EPS2];
     const double delx = xyk.x - x_i, dely = xyk.y - y_i;
     const double dels = delx * tx + dely * ty;
     const double deln = delx * nx + dely * ny;
@@ -518,8 +522,8 @@ __device__ void block_reduce3(double &s, double &mx, int &arg) {
 }
 
 // state_update_mod.h:92-283.  It also writes, from the new state, the
-// q-variables of the next stage (q_variables_mod.h:9-34, into both hot-line
-// buffers) and, when `speed` is set, the denominator func_delta divides by
+// q-variables of the next stage (q_variables_mod.h:9-34) and, when `speed`
+// is set, the denominator func_delta divides by
 // (state_update_mod.h:483-485): both are pointwise functions of the state,
 // so computing them here gives the values the separate kernels would.
 __global__ __launch_bounds__(256) void k_state_update(int n, int rk, double *__restrict__ prim,
@@ -532,8 +536,7 @@ __global__ __launch_bounds__(256) void k_state_update(int n, int rk, double *__r
                                                       FlowConsts fc, double *__restrict__ psum,
                                                       double *__restrict__ pmax,
                                                       int32_t *__restrict__ parg,
-                                                      double *__restrict__ hot_a,
-                                                      double *__restrict__ hot_b, int speed) {
+                                                      double *__restrict__ base, int speed) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double res_sqr = 0.0;
   int arg = 0x7fffffff;
@@ -575,7 +578,7 @@ __global__ __launch_bounds__(256) void k_state_update(int n, int rk, double *__r
     prim[(size_t)n + i] = u1;
     prim[(size_t)2 * n + i] = u2;
     prim[(size_t)3 * n + i] = pr;
-    if (hot_a) {
+    if (base) {
       const double beta = 0.5 * rho / pr;
       double q[4];
       q[0] = log(rho) + (log(beta) * 2.5) - beta * (u1 * u1 + u2 * u2);
@@ -583,11 +586,10 @@ __global__ __launch_bounds__(256) void k_state_update(int n, int rk, double *__r
       q[1] = two_times_beta * u1;
       q[2] = two_times_beta * u2;
       q[3] = -two_times_beta;
-      store4(hot_a + (size_t)H_STRIDE * i + H_Q, q);
-      store4(hot_b + (size_t)H_STRIDE * i + H_Q, q);
+      store4(base + (size_t)B_STRIDE * i + B_Q, q);
       if (speed) {
         const double mod_u = sqrt(u1 * u1 + u2 * u2);
-        hot_a[(size_t)H_STRIDE * i + H_SPEED] = mod_u + 3.0 * sqrt(pr / rho);
+        base[(size_t)B_STRIDE * i + B_SPEED] = mod_u + 3.0 * sqrt(pr / rho);
       }
     }
   }
@@ -632,7 +634,7 @@ __global__ __launch_bounds__(256) void k_forces(int ns, const int32_t *__restric
                                                 const int32_t *__restrict__ li,
                                                 const int32_t *__restrict__ ri,
                                                 const int32_t *__restrict__ shape,
-                                                const double *__restrict__ hot,
+                                                const double *__restrict__ base,
                                                 const double *__restrict__ nxa,
                                                 const double *__restrict__ nya,
                                                 const double *__restrict__ prim, int n,
@@ -643,9 +645,9 @@ __global__ __launch_bounds__(256) void k_forces(int ns, const int32_t *__restric
   for (int j = threadIdx.x; j < ns; j += blockDim.x) {
     if (shape[j] != 1) continue;
     const int m = mi[j], l = li[j], r = ri[j];
-    const double2 L = reinterpret_cast<const double2 *>(hot + (size_t)H_STRIDE * l)[0];
-    const double2 Mp = reinterpret_cast<const double2 *>(hot + (size_t)H_STRIDE * m)[0];
-    const double2 Rp = reinterpret_cast<const double2 *>(hot + (size_t)H_STRIDE * r)[0];
+    const double2 L = reinterpret_cast<const double2 *>(base + (size_t)B_STRIDE * l)[0];
+    const double2 Mp = reinterpret_cast<const double2 *>(base + (size_t)B_STRIDE * m)[0];
+    const double2 Rp = reinterpret_cast<const double2 *>(base + (size_t)B_STRIDE * r)[0];
     double ds1 = (Mp.x - L.x) * (Mp.x - L.x) + (Mp.y - L.y) * (Mp.y - L.y);
     ds1 = sqrt(ds1);
     double ds2 = (Rp.x - Mp.x) * (Rp.x - Mp.x) + (Rp.y - Mp.y) * (Rp.y - Mp.y);
@@ -712,82 +714,91 @@ size_t lds_bytes_qinner(int tile_max) { return sizeof(double) * 14 * (size_t)til
 size_t lds_bytes_delta(int tile_max) { return sizeof(double) * 4 * (size_t)tile_max; }
 size_t lds_bytes_flux(int tile_max) { return sizeof(double) * 24 * (size_t)tile_max; }
 
-bool prepare_tiled(int tile_max, size_t lds_limit) {
-  if (lds_bytes_flux(tile_max) > lds_limit) return false;  // flux needs the most
-  struct K { const void *f; size_t bytes; };
+unsigned prepare_tiled(unsigned tiles, int tile_max, size_t lds_limit) {
+  struct K { unsigned bit; const void *f; size_t bytes; };
   const K ks[] = {
-      {(const void *)k_q_derivatives<true, false>, lds_bytes_qderiv(tile_max)},
-      {(const void *)k_q_derivatives<true, true>, lds_bytes_qderiv(tile_max)},
-      {(const void *)k_q_inner<true, false>, lds_bytes_qinner(tile_max)},
-      {(const void *)k_q_inner<true, true>, lds_bytes_qinner(tile_max)},
-      {(const void *)k_func_delta<true>, lds_bytes_delta(tile_max)},
-      {(const void *)k_flux<true, false>, lds_bytes_flux(tile_max)},
-      {(const void *)k_flux<true, true>, lds_bytes_flux(tile_max)}};
-  for (const K &k : ks)
-    if (hipFuncSetAttribute(k.f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)k.bytes) !=
-        hipSuccess)
-      return false;
-  return true;
+      {TILE_QDERIV, (const void *)k_q_derivatives<true, false>, lds_bytes_qderiv(tile_max)},
+      {TILE_QDERIV, (const void *)k_q_derivatives<true, true>, lds_bytes_qderiv(tile_max)},
+      {TILE_QINNER, (const void *)k_q_inner<true, false>, lds_bytes_qinner(tile_max)},
+      {TILE_QINNER, (const void *)k_q_inner<true, true>, lds_bytes_qinner(tile_max)},
+      {TILE_DELTA, (const void *)k_func_delta<true>, lds_bytes_delta(tile_max)},
+      {TILE_FLUX, (const void *)k_flux<true, false>, lds_bytes_flux(tile_max)},
+      {TILE_FLUX, (const void *)k_flux<true, true>, lds_bytes_flux(tile_max)}};
+  unsigned ok = tiles;
+  for (const K &k : ks) {
+    if (!(tiles & k.bit)) continue;
+    if (k.bytes > lds_limit ||
+        hipFuncSetAttribute(k.f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)k.bytes) !=
+            hipSuccess)
+      ok &= ~k.bit;
+  }
+  return ok;
 }
 
 #define TILE_ARGS c.conn_off, c.conn, c.conn16, c.tile_off, c.tile_idx
 
-void launch_q_variables(const DevCloud &c, const double *prim, double *hot_a, double *hot_b,
-                        hipStream_t s) {
+void launch_q_variables(const DevCloud &c, const double *prim, double *base, hipStream_t s) {
   hipLaunchKernelGGL(k_q_variables, dim3(blocks_for(c.n, 256)), dim3(256), 0, s, c.n, prim,
-                     hot_a, hot_b);
+                     base);
 }
 
-void launch_speed(const DevCloud &c, const double *prim, double *hot, hipStream_t s) {
-  hipLaunchKernelGGL(k_speed, dim3(blocks_for(c.n, 256)), dim3(256), 0, s, c.n, prim, hot);
+void launch_speed(const DevCloud &c, const double *prim, double *base, hipStream_t s) {
+  hipLaunchKernelGGL(k_speed, dim3(blocks_for(c.n, 256)), dim3(256), 0, s, c.n, prim, base);
 }
 
-void launch_func_delta(const DevCloud &c, const double *hot, double cfl, double *delta,
+void launch_func_delta(const DevCloud &c, const double *base, double cfl, double *delta,
                        hipStream_t s) {
-  dim3 g(blocks_for(c.n, kTileBlock)), b(kTileBlock);
-  if (c.tiled)
+  dim3 g(blocks_for(c.n, c.tile_block)), b(c.tile_block);
+  if (c.tiles & TILE_DELTA)
     hipLaunchKernelGGL(k_func_delta<true>, g, b, lds_bytes_delta(c.tile_max), s, c.n, TILE_ARGS,
-                       hot, cfl, delta);
+                       base, cfl, delta);
   else
-    hipLaunchKernelGGL(k_func_delta<false>, g, b, 0, s, c.n, TILE_ARGS, hot, cfl, delta);
+    hipLaunchKernelGGL(k_func_delta<false>, g, b, 0, s, c.n, TILE_ARGS, base, cfl, delta);
 }
 
-void launch_q_derivatives(const DevCloud &c, double *hot, double *qm, double power,
-                          hipStream_t s) {
-  dim3 g(blocks_for(c.n, kTileBlock)), b(kTileBlock);
-  const size_t L = c.tiled ? lds_bytes_qderiv(c.tile_max) : 0;
-  if (c.tiled && power == 0.0)
-    hipLaunchKernelGGL((k_q_derivatives<true, false>), g, b, L, s, c.n, TILE_ARGS, hot, qm, power);
-  else if (c.tiled)
-    hipLaunchKernelGGL((k_q_derivatives<true, true>), g, b, L, s, c.n, TILE_ARGS, hot, qm, power);
+void launch_q_derivatives(const DevCloud &c, const double *base, double *dq, double *qm,
+                          double power, hipStream_t s) {
+  dim3 g(blocks_for(c.n, c.tile_block)), b(c.tile_block);
+  const bool t = (c.tiles & TILE_QDERIV) != 0;
+  const size_t L = t ? lds_bytes_qderiv(c.tile_max) : 0;
+#define QD_ARGS c.n, TILE_ARGS, base, dq, qm, power
+  if (t && power == 0.0)
+    hipLaunchKernelGGL((k_q_derivatives<true, false>), g, b, L, s, QD_ARGS);
+  else if (t)
+    hipLaunchKernelGGL((k_q_derivatives<true, true>), g, b, L, s, QD_ARGS);
   else if (power == 0.0)
-    hipLaunchKernelGGL((k_q_derivatives<false, false>), g, b, 0, s, c.n, TILE_ARGS, hot, qm, power);
+    hipLaunchKernelGGL((k_q_derivatives<false, false>), g, b, 0, s, QD_ARGS);
   else
-    hipLaunchKernelGGL((k_q_derivatives<false, true>), g, b, 0, s, c.n, TILE_ARGS, hot, qm, power);
+    hipLaunchKernelGGL((k_q_derivatives<false, true>), g, b, 0, s, QD_ARGS);
+#undef QD_ARGS
 }
 
-void launch_q_inner(const DevCloud &c, const double *src, double *dst, double power,
-                    hipStream_t s) {
-  dim3 g(blocks_for(c.n, kTileBlock)), b(kTileBlock);
-  const size_t L = c.tiled ? lds_bytes_qinner(c.tile_max) : 0;
-  if (c.tiled && power == 0.0)
-    hipLaunchKernelGGL((k_q_inner<true, false>), g, b, L, s, c.n, TILE_ARGS, src, dst, power);
-  else if (c.tiled)
-    hipLaunchKernelGGL((k_q_inner<true, true>), g, b, L, s, c.n, TILE_ARGS, src, dst, power);
+void launch_q_inner(const DevCloud &c, const double *base, const double *dq_src,
+                    double *dq_dst, double power, hipStream_t s) {
+  dim3 g(blocks_for(c.n, c.tile_block)), b(c.tile_block);
+  const bool t = (c.tiles & TILE_QINNER) != 0;
+  const size_t L = t ? lds_bytes_qinner(c.tile_max) : 0;
+#define QI_ARGS c.n, TILE_ARGS, base, dq_src, dq_dst, power
+  if (t && power == 0.0)
+    hipLaunchKernelGGL((k_q_inner<true, false>), g, b, L, s, QI_ARGS);
+  else if (t)
+    hipLaunchKernelGGL((k_q_inner<true, true>), g, b, L, s, QI_ARGS);
   else if (power == 0.0)
-    hipLaunchKernelGGL((k_q_inner<false, false>), g, b, 0, s, c.n, TILE_ARGS, src, dst, power);
+    hipLaunchKernelGGL((k_q_inner<false, false>), g, b, 0, s, QI_ARGS);
   else
-    hipLaunchKernelGGL((k_q_inner<false, true>), g, b, 0, s, c.n, TILE_ARGS, src, dst, power);
+    hipLaunchKernelGGL((k_q_inner<false, true>), g, b, 0, s, QI_ARGS);
+#undef QI_ARGS
 }
 
-void launch_flux(const DevCloud &c, const double *hot, const double *qm, const double *delta,
-                 double power, double *flux_res, hipStream_t s) {
-  dim3 g(blocks_for(c.n, kTileBlock)), b(kTileBlock);
-  const size_t L = c.flux_tiled ? lds_bytes_flux(c.tile_max) : 0;
-#define FLUX_ARGS c.n, TILE_ARGS, hot, qm, c.coef, c.nx, c.ny, c.flag, delta, power, flux_res
-  if (c.flux_tiled && power == 0.0)
+void launch_flux(const DevCloud &c, const double *base, const double *dq, const double *qm,
+                 const double *delta, double power, double *flux_res, hipStream_t s) {
+  dim3 g(blocks_for(c.n, c.tile_block)), b(c.tile_block);
+  const bool t = (c.tiles & TILE_FLUX) != 0;
+  const size_t L = t ? lds_bytes_flux(c.tile_max) : 0;
+#define FLUX_ARGS c.n, TILE_ARGS, base, dq, qm, c.coef, c.nx, c.ny, c.flag, delta, power, flux_res
+  if (t && power == 0.0)
     hipLaunchKernelGGL((k_flux<true, false>), g, b, L, s, FLUX_ARGS);
-  else if (c.flux_tiled)
+  else if (t)
     hipLaunchKernelGGL((k_flux<true, true>), g, b, L, s, FLUX_ARGS);
   else if (power == 0.0)
     hipLaunchKernelGGL((k_flux<false, false>), g, b, 0, s, FLUX_ARGS);
@@ -798,10 +809,10 @@ void launch_flux(const DevCloud &c, const double *hot, const double *qm, const d
 
 void launch_state_update(const DevCloud &c, int rk, double *prim, const double *prim_old,
                          const double *flux_res, const FlowConsts &fc, const Partials &p,
-                         double *hot_a, double *hot_b, bool speed, hipStream_t s) {
+                         double *base, bool speed, hipStream_t s) {
   // fixed 256-thread blocks: the partial layout must not depend on tuning
   hipLaunchKernelGGL(k_state_update, dim3(p.nblocks), dim3(256), 0, s, c.n, rk, prim, prim_old,
-                     flux_res, c.nx, c.ny, c.flag, c.perm, fc, p.sum, p.max, p.arg, hot_a, hot_b,
+                     flux_res, c.nx, c.ny, c.flag, c.perm, fc, p.sum, p.max, p.arg, base,
                      speed ? 1 : 0);
 }
 
@@ -811,10 +822,10 @@ void launch_reduce(const Partials &p, double *out, double *hist_slot, hipStream_
 }
 
 void launch_forces(int n_shape, const int32_t *m, const int32_t *l, const int32_t *r,
-                   const int32_t *shape, const double *hot, const double *nx, const double *ny,
+                   const int32_t *shape, const double *base, const double *nx, const double *ny,
                    const double *prim, int n, double pr_inf, double qdyn, double *out3,
                    hipStream_t s) {
-  hipLaunchKernelGGL(k_forces, dim3(1), dim3(256), 0, s, n_shape, m, l, r, shape, hot, nx, ny,
+  hipLaunchKernelGGL(k_forces, dim3(1), dim3(256), 0, s, n_shape, m, l, r, shape, base, nx, ny,
                      prim, n, pr_inf, qdyn, out3);
 }
 

@@ -18,7 +18,8 @@ ERRORS = {
 
 FP64, MIXED = 0, 1
 REORDER_NONE, REORDER_RCM, REORDER_HILBERT = 0, 1, 2
-FLUX_LDS_AUTO, FLUX_LDS_ON, FLUX_LDS_OFF = 0, 1, 2
+TILES_AUTO = -1
+TILE_QDERIV, TILE_QINNER, TILE_DELTA, TILE_FLUX = 1, 2, 4, 8
 A_PRIM, A_PRIM_OLD, A_Q, A_DQ, A_QM, A_DELTA, A_FLUX_RES, A_NORMALS = range(8)
 K_FLUX, K_QDERIV, K_QINNER, K_STATE, K_DELTA, K_QVAR = range(6)
 
@@ -57,7 +58,7 @@ class Params(C.Structure):
         ("mach", C.c_double), ("aoa_deg", C.c_double), ("cfl", C.c_double),
         ("vl_const", C.c_double), ("power", C.c_double),
         ("inner_iterations", C.c_int32), ("precision", C.c_int32),
-        ("reorder", C.c_int32), ("flux_lds", C.c_int32),
+        ("reorder", C.c_int32), ("lds_tiles", C.c_int32), ("tile_block", C.c_int32),
     ]
 
 

@@ -30,11 +30,11 @@ _SHAPES = {L.A_PRIM: (4,), L.A_PRIM_OLD: (4,), L.A_Q: (4,), L.A_DQ: (2, 4),
 class Solver:
     def __init__(self, cloud: Cloud, mach=0.63, aoa=2.0, cfl=0.1, vl_const=150.0, power=0.0,
                  inner_iterations=3, precision=L.FP64, reorder=L.REORDER_HILBERT,
-                 flux_lds=L.FLUX_LDS_AUTO):
+                 lds_tiles=L.TILES_AUTO, tile_block=0):
         self.cloud = cloud
         self.n = cloud.n
         self.params = L.Params(mach, aoa, cfl, vl_const, power, inner_iterations, precision,
-                               reorder, flux_lds)
+                               reorder, lds_tiles, tile_block)
         self.inner_iterations = inner_iterations
         view = cloud.view()
         ctx = C.c_void_p()
@@ -188,10 +188,10 @@ class Solver:
         self._chk(L.lib().lskum_set_iteration(self._ctx, it, res_old, 1 if restart else 0))
 
     def layout(self):
-        """(gather kernels tiled, flux tiled, tile_max, order) from lskum_create."""
+        """(mask of kernels using LDS tiles, tile_max, order) from lskum_create."""
         t, m, o = C.c_int32(), C.c_int32(), C.c_int32()
         self._chk(L.lib().lskum_layout(self._ctx, C.byref(t), C.byref(m), C.byref(o)))
-        return bool(t.value & 1), bool(t.value & 2), m.value, o.value
+        return t.value, m.value, o.value
 
     # ------------------------------------------------------------ timing
     def set_timing(self, enable):

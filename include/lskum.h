@@ -139,12 +139,18 @@ typedef struct lskum_params {
   int32_t inner_iterations; /* (data_structure_mod.h:118)                  */
   int32_t precision;        /* LSKUM_FP64 or LSKUM_MIXED                    */
   int32_t reorder;          /* LSKUM_REORDER_*                              */
-  int32_t flux_lds;         /* flux neighbour data: LSKUM_FLUX_LDS_*        */
+  int32_t lds_tiles;        /* kernels that stage neighbour data in per-   */
+                            /* block LDS tiles: LSKUM_TILES_AUTO or a mask */
+                            /* of LSKUM_TILE_* (used when the tiles fit)   */
+  int32_t tile_block;       /* points per block of the stencil kernels (=  */
+                            /* per LDS tile): 64, 128, 256; 0 = default    */
 } lskum_params;
 
-#define LSKUM_FLUX_LDS_AUTO 0 /* library's choice                          */
-#define LSKUM_FLUX_LDS_ON 1   /* from the block's LDS tile (if tiles fit)  */
-#define LSKUM_FLUX_LDS_OFF 2  /* straight from HBM/L2                      */
+#define LSKUM_TILES_AUTO (-1)
+#define LSKUM_TILE_QDERIV 1
+#define LSKUM_TILE_QINNER 2
+#define LSKUM_TILE_DELTA 4
+#define LSKUM_TILE_FLUX 8
 
 /* --------------------------------------------------------------- context */
 typedef struct lskum_ctx lskum_ctx;
@@ -221,9 +227,9 @@ int lskum_get_substencil(lskum_ctx *ctx, int32_t which, int32_t *counts,
 
 /* Stream the context's kernels run on (a hipStream_t). */
 void *lskum_stream(lskum_ctx *ctx);
-/* Device layout chosen at create: tiled bit 0 when the gather kernels stage
- * neighbour data in per-block LDS tiles, bit 1 when the flux does too;
- * tile_max = largest tile (points);
+/* Device layout chosen at create: tiled = mask of the LSKUM_TILE_* kernels
+ * that stage neighbour data in per-block LDS tiles; tile_max = largest tile
+ * (points);
  * order = the LSKUM_REORDER_* in use.  Any pointer may be NULL. */
 int lskum_layout(lskum_ctx *ctx, int32_t *tiled, int32_t *tile_max, int32_t *order);
 

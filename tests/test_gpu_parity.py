@@ -117,13 +117,15 @@ def test_point_orders_give_bit_identical_states():
     g = golden("k24_m085.npz")
     cloud = lskum.workload("naca0012_3620_k24")
     runs = []
-    for order, flds in [(L.REORDER_NONE, 0), (L.REORDER_RCM, 0), (L.REORDER_HILBERT, 1),
-                        (L.REORDER_HILBERT, 2)]:
-        s = _solver(cloud, g["params"], reorder=order, flux_lds=flds)
+    for order, tiles, tb in [(L.REORDER_NONE, 0, 0), (L.REORDER_RCM, -1, 0),
+                             (L.REORDER_HILBERT, 15, 0), (L.REORDER_HILBERT, 0, 0),
+                             (L.REORDER_HILBERT, -1, 0), (L.REORDER_HILBERT, -1, 64),
+                             (L.REORDER_HILBERT, 15, 128)]:
+        s = _solver(cloud, g["params"], reorder=order, lds_tiles=tiles, tile_block=tb)
         s.q_lskum(10)
         runs.append((s.layout(), s.prim, s.flux_res, s.dq, s.qm))
-    assert runs[2][0][0] and runs[2][0][1], "Hilbert order should give LDS tiles"
-    assert runs[3][0][0] and not runs[3][0][1]
+    assert runs[2][0][0] == 15, "Hilbert order should let every kernel use LDS tiles"
+    assert runs[3][0][0] == 0
     for r in runs[1:]:
         for a, b in zip(runs[0][1:], r[1:]):
             np.testing.assert_array_equal(a, b)
@@ -187,7 +189,7 @@ def test_free_stream_preserved_at_full_size():
     cloud = lskum.workload("naca0012_2p5m_k24")
     print("case: free stream, naca0012_2p5m_k24", cloud.n, flush=True)
     s = _solver(cloud, (0.85, 1.25, 0.2, 5.0, 0.0, 2))
-    assert s.layout()[0]
+    assert s.layout()[0] == 7
     s.func_delta()
     s.eval_q_variables()
     s.eval_q_derivatives()
