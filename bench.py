@@ -1,13 +1,17 @@
 #!/usr/bin/env python3
-"""Benchmark: q-LSKUM FP64 iterations on one MI355X.
+"""Benchmark: q-LSKUM iterations on one MI355X, FP64 and mixed precision.
 
 A step is one fpi_solver iteration (snapshot, local time step, four SSPRK
 stages of q-variables, q-derivatives, 2 inner passes, flux residual and state
 update) over the 2,502,000-point NACA0012 cloud (wall 2,000, 500 x 5,000
 O-grid, k = 24), M 0.85, AoA 1.25 deg, CFL 0.2, K 5.  Inputs are
-resident in HBM before the timed region.  Prints one JSON line.
+resident in HBM before the timed region.  `value` is the FP64 form; the
+`mixed` object times the mixed-precision form on the same cloud from the
+same start and states its Cl/Cd/residual differences from the FP64 run.
+Prints one JSON line.
 
   python bench.py [--gpus 1] [--steps 50] [--warmup 5] [--no-cpu-baseline]
+                  [--no-mixed]
 """
 import argparse
 import json
@@ -45,11 +49,46 @@ def bytes_per_point_iteration(k, n_inner):
     return 64 + (60 + 4 * k) + 4 * (64 + (180 + 4 * k) + n_inner * (180 + 4 * k) + (260 + 8 * k) + 148)
 
 
-def flux_bytes_per_point(k):
+def flux_bytes_per_point(k, mixed=False):
     """Compulsory bytes of one flux-residual launch per point: q, dq, qm, x,
     y, nx, ny, min_dist-derived eps, delta, flag in; flux_res out; plus the
-    2k sub-stencil index entries."""
-    return 260 + 8 * k
+    2k sub-stencil index entries.  The mixed form stores q, dq, qm (80 of
+    the 160 bytes) and eps (4 of 8) in FP32."""
+    return (176 if mixed else 260) + 8 * k
+
+
+def run_form(cloud, precision, args, k_mean):
+    """Warmup, then args.steps timed iterations of one form; returns the
+    measurements (the solver is closed)."""
+    reorder = {"hilbert": L.REORDER_HILBERT, "rcm": L.REORDER_RCM,
+               "none": L.REORDER_NONE}[args.reorder]
+    t = time.perf_counter()
+    solver = Solver(cloud, reorder=reorder, lds_tiles=args.lds_tiles, tile_block=args.tile_block,
+                    precision=precision, **CASE)
+    tiles, tile_max, _ = solver.layout()
+    log(f"{'mixed' if precision else 'fp64'} context created in "
+        f"{time.perf_counter() - t:.1f}s; LDS tile mask {tiles}, largest tile {tile_max} points")
+    if args.warmup:
+        solver.q_lskum(args.warmup)
+    solver.set_timing(True)
+    t0 = time.perf_counter()
+    rn, rs = solver.q_lskum(args.steps)  # synchronises its stream at the end
+    elapsed = time.perf_counter() - t0
+    flux_ms, flux_n = solver.kernel_time(L.K_FLUX)
+    kt = {name: solver.kernel_time(kid)[0] for name, kid in
+          [("flux", L.K_FLUX), ("q_derivatives", L.K_QDERIV), ("q_inner", L.K_QINNER),
+           ("state_update", L.K_STATE), ("func_delta", L.K_DELTA), ("q_variables", L.K_QVAR)]}
+    solver.set_timing(False)
+    cl, cd, cm = solver.compute_cl_cd_cm()
+    solver.close()
+    if not np.all(np.isfinite(rn)):
+        raise SystemExit("non-finite residual during the benchmark")
+    n = cloud.n
+    flux_bytes = n * flux_bytes_per_point(k_mean, mixed=bool(precision))
+    return {"elapsed": elapsed, "value": n * args.steps / elapsed,
+            "ms_per_step": 1e3 * elapsed / args.steps, "flux_ms": flux_ms, "flux_n": flux_n,
+            "flux_bytes": flux_bytes, "kernel_avg_ms": kt, "tiles": tiles,
+            "tile_max": tile_max, "res_new": rn, "cl": cl, "cd": cd, "cm": cm}
 
 
 def cpu_baseline(args):
@@ -108,6 +147,7 @@ def main():
                          "4 func_delta, 8 flux")
     ap.add_argument("--tile-block", type=int, default=0, choices=[0, 64, 128, 256])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; the q-LSKUM iteration would shard "
@@ -120,43 +160,18 @@ def main():
     log(f"cloud {args.workload}: {cloud.n} points, {cloud.conn.size} neighbour entries, "
         f"{time.perf_counter() - t:.1f}s")
     k_mean = cloud.conn.size / cloud.n
-    reorder = {"hilbert": L.REORDER_HILBERT, "rcm": L.REORDER_RCM,
-               "none": L.REORDER_NONE}[args.reorder]
-    t = time.perf_counter()
-    solver = Solver(cloud, reorder=reorder, lds_tiles=args.lds_tiles, tile_block=args.tile_block,
-                    **CASE)
-    tiles, tile_max, _ = solver.layout()
-    log(f"context created in {time.perf_counter() - t:.1f}s; LDS tile mask {tiles}, "
-        f"largest tile {tile_max} points")
-    if args.warmup:
-        solver.q_lskum(args.warmup)
-    solver.set_timing(True)
-    t0 = time.perf_counter()
-    rn, rs = solver.q_lskum(args.steps)  # synchronises its stream at the end
-    elapsed = time.perf_counter() - t0
-    flux_ms, flux_n = solver.kernel_time(L.K_FLUX)
-    kt = {name: solver.kernel_time(kid)[0] for name, kid in
-          [("flux", L.K_FLUX), ("q_derivatives", L.K_QDERIV), ("q_inner", L.K_QINNER),
-           ("state_update", L.K_STATE), ("func_delta", L.K_DELTA), ("q_variables", L.K_QVAR)]}
-    solver.set_timing(False)
-    cl, cd, cm = solver.compute_cl_cd_cm()
-    if not np.all(np.isfinite(rn)):
-        raise SystemExit("non-finite residual during the benchmark")
-
+    f = run_form(cloud, L.FP64, args, k_mean)
     n = cloud.n
-    ms_per_step = 1e3 * elapsed / args.steps
-    value = n * args.steps / elapsed
-    flux_bytes = n * flux_bytes_per_point(k_mean)
-    achieved = flux_bytes / (flux_ms * 1e-3) / 1e9
+    achieved = f["flux_bytes"] / (f["flux_ms"] * 1e-3) / 1e9
     it_bytes = n * bytes_per_point_iteration(k_mean, CASE["inner_iterations"])
     out = {
         "metric": METRIC,
-        "value": value,
+        "value": f["value"],
         "unit": "point-iterations/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": f["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -165,18 +180,38 @@ def main():
         "config": {"workload": args.workload, "points": n, "k": 24, "k_mean": round(k_mean, 3),
                    "inner_iterations": 2, "mach": 0.85, "aoa_deg": 1.25, "cfl": 0.2,
                    "vl_const": 5.0, "precision": "fp64", "order": args.reorder,
-                   "lds_tile_mask": tiles, "tile_max": tile_max,
+                   "lds_tile_mask": f["tiles"], "tile_max": f["tile_max"],
                    "tile_block": args.tile_block or 256},
         "roofline": {"bound": "hbm", "kernel": "k_flux", "achieved": achieved,
                      "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS,
                      "traffic": load_traffic("flux_naca0012_2p5m_k24_traffic.json"),
-                     "avg_ms": flux_ms, "launches": flux_n,
-                     "algorithmic_bytes_per_launch": flux_bytes},
-        "iteration_hbm_GBps": it_bytes * args.steps / elapsed / 1e9,
-        "kernel_avg_ms": kt,
-        "final": {"res_new": float(rn[-1]), "residue": float(rs[-1]), "cl": cl, "cd": cd, "cm": cm},
+                     "avg_ms": f["flux_ms"], "launches": f["flux_n"],
+                     "algorithmic_bytes_per_launch": f["flux_bytes"]},
+        "iteration_hbm_GBps": it_bytes * args.steps / f["elapsed"] / 1e9,
+        "kernel_avg_ms": f["kernel_avg_ms"],
+        "final": {"res_new": float(f["res_new"][-1]), "cl": f["cl"], "cd": f["cd"],
+                  "cm": f["cm"]},
     }
-    solver.close()
+    if not args.no_mixed:
+        m = run_form(cloud, L.MIXED, args, k_mean)
+        ach_m = m["flux_bytes"] / (m["flux_ms"] * 1e-3) / 1e9
+        rn64 = f["res_new"]
+        out["mixed"] = {
+            "value": m["value"], "ms_per_step": m["ms_per_step"],
+            "speedup_vs_fp64": m["value"] / f["value"],
+            "dtype": "f32 gathers/fluxes/limiter, f64 solves/state/reductions",
+            "lds_tile_mask": m["tiles"],
+            "flux_roofline": {"achieved": ach_m, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                              "frac": ach_m / HBM_PEAK_GBPS, "avg_ms": m["flux_ms"],
+                              "algorithmic_bytes_per_launch": m["flux_bytes"]},
+            "kernel_avg_ms": m["kernel_avg_ms"],
+            "error_vs_fp64": {"iterations": args.warmup + args.steps,
+                              "abs_dCl": abs(m["cl"] - f["cl"]),
+                              "abs_dCd": abs(m["cd"] - f["cd"]),
+                              "abs_dCm": abs(m["cm"] - f["cm"]),
+                              "res_new_normwise": float(np.abs(m["res_new"] - rn64).max()
+                                                        / np.abs(rn64).max())},
+        }
     if not args.no_cpu_baseline:
         log("cpu baseline: reference serial fpi_solver on a bounded sample ...")
         out["cpu_baseline"] = cpu_baseline(args)

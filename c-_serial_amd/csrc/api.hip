@@ -35,6 +35,11 @@ struct lskum_ctx {
   // current derivatives), qmax/qmin, primitive state
   double *base = nullptr, *dqb[2] = {nullptr, nullptr}, *qm = nullptr;
   int cur = 0;
+  // mixed-precision form: FP64 coordinates, FP32 gathered data (dq32[cur]
+  // holds the current derivatives); dqb/qm are then not allocated
+  bool mixed = false;
+  double *xy = nullptr;
+  float *q32 = nullptr, *dq32[2] = {nullptr, nullptr}, *qm32 = nullptr, *eps32 = nullptr;
   double *prim = nullptr, *prim_old = nullptr;
   double *flux_res = nullptr, *delta = nullptr, *nx = nullptr, *ny = nullptr;
   double *coef = nullptr;
@@ -88,6 +93,14 @@ struct lskum_ctx {
     c.perm = perm_d;
     return c;
   }
+  MixedState ms() const {
+    MixedState m;
+    m.xy = xy;
+    m.q32 = q32;
+    m.qm32 = qm32;
+    m.eps32 = eps32;
+    return m;
+  }
   FlowConsts fc() const {
     FlowConsts f;
     f.cfl = prm.cfl;
@@ -124,7 +137,8 @@ void release(lskum_ctx *c) {
   void *ptrs[] = {c->coef, c->base, c->dqb[0], c->dqb[1], c->qm, c->conn16, c->tile_off, c->tile_idx, c->prim, c->prim_old, c->flux_res, c->delta, c->nx,
                   c->ny, c->flag, c->conn_off, c->perm_d, c->conn, c->part.sum,
                   c->part.max, c->part.arg, c->red, c->hist, c->f_m, c->f_l, c->f_r,
-                  c->f_s, c->forces};
+                  c->f_s, c->forces, c->xy, c->q32, c->dq32[0], c->dq32[1], c->qm32,
+                  c->eps32};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (c->red_h) (void)hipHostFree(c->red_h);
@@ -185,7 +199,7 @@ int do_snapshot(lskum_ctx *c) {
 int do_func_delta(lskum_ctx *c, bool with_speed) {
   {
     KTimer t(c, LSKUM_K_DELTA);
-    if (with_speed) launch_speed(c->dc(), c->prim, c->base, c->stream);
+    if (with_speed) launch_speed(c->dc(), c->prim, c->base, c->stream);  // both forms: FP64
     launch_func_delta(c->dc(), c->base, c->prm.cfl, c->delta, c->stream);
   }
   return check_launch(c, "func_delta");
@@ -194,7 +208,10 @@ int do_func_delta(lskum_ctx *c, bool with_speed) {
 int do_q_variables(lskum_ctx *c) {
   {
     KTimer t(c, LSKUM_K_QVAR);
-    launch_q_variables(c->dc(), c->prim, c->base, c->stream);
+    if (c->mixed)
+      launch_q_variables_m(c->dc(), c->prim, c->q32, c->stream);
+    else
+      launch_q_variables(c->dc(), c->prim, c->base, c->stream);
   }
   return check_launch(c, "q_variables");
 }
@@ -204,7 +221,10 @@ int do_q_derivatives(lskum_ctx *c, int n_later) {
   c->cur = n_later % 2;
   {
     KTimer t(c, LSKUM_K_QDERIV);
-    launch_q_derivatives(c->dc(), c->base, c->dqb[c->cur], c->qm, c->prm.power, c->stream);
+    if (c->mixed)
+      launch_q_derivatives_m(c->dc(), c->ms(), c->dq32[c->cur], c->prm.power, c->stream);
+    else
+      launch_q_derivatives(c->dc(), c->base, c->dqb[c->cur], c->qm, c->prm.power, c->stream);
   }
   return check_launch(c, "q_derivatives");
 }
@@ -213,8 +233,12 @@ int do_q_derivatives(lskum_ctx *c, int n_later) {
 int do_q_inner(lskum_ctx *c, int n_inner) {
   for (int p = 0; p < n_inner; p++) {
     KTimer t(c, LSKUM_K_QINNER);
-    launch_q_inner(c->dc(), c->base, c->dqb[c->cur], c->dqb[1 - c->cur], c->prm.power,
-                   c->stream);
+    if (c->mixed)
+      launch_q_inner_m(c->dc(), c->ms(), c->dq32[c->cur], c->dq32[1 - c->cur], c->prm.power,
+                       c->stream);
+    else
+      launch_q_inner(c->dc(), c->base, c->dqb[c->cur], c->dqb[1 - c->cur], c->prm.power,
+                     c->stream);
     c->cur = 1 - c->cur;
   }
   return check_launch(c, "q_inner");
@@ -223,8 +247,12 @@ int do_q_inner(lskum_ctx *c, int n_inner) {
 int do_flux(lskum_ctx *c) {
   {
     KTimer t(c, LSKUM_K_FLUX);
-    launch_flux(c->dc(), c->base, c->dqb[c->cur], c->qm, c->delta, c->prm.power, c->flux_res,
-                c->stream);
+    if (c->mixed)
+      launch_flux_m(c->dc(), c->ms(), c->dq32[c->cur], c->delta, c->prm.power, c->flux_res,
+                    c->stream);
+    else
+      launch_flux(c->dc(), c->base, c->dqb[c->cur], c->qm, c->delta, c->prm.power,
+                  c->flux_res, c->stream);
   }
   return check_launch(c, "flux_residual");
 }
@@ -235,7 +263,7 @@ int do_state_update(lskum_ctx *c, int rk, double *hist_slot) {
   {
     KTimer t(c, LSKUM_K_STATE);
     launch_state_update(c->dc(), rk, c->prim, c->prim_old, c->flux_res, c->fc(), c->part,
-                        c->base, rk == 4, c->stream);
+                        c->base, c->mixed ? c->q32 : nullptr, rk == 4, c->stream);
     launch_reduce(c->part, c->red, hist_slot, c->stream);
   }
   return check_launch(c, "state_update");
@@ -260,6 +288,18 @@ int fetch_record(lskum_ctx *c, const double *base, int stride, int slot, int nco
                               sizeof(double) * stride, sizeof(double) * ncomp, c->n,
                               hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(c, hipStreamSynchronize(c->stream));
+  for (int d = 0; d < c->n; d++)
+    for (int r = 0; r < ncomp; r++) out[(size_t)r * c->n + d] = tmp[(size_t)d * ncomp + r];
+  return LSKUM_OK;
+}
+
+// fetch a float array of ncomp values per point as out[r][dev] (doubles)
+int fetch_float(lskum_ctx *c, const float *dev, int ncomp, std::vector<double> &out) {
+  std::vector<float> tmp((size_t)ncomp * c->n);
+  HIP_TRY(c, hipMemcpyAsync(tmp.data(), dev, sizeof(float) * tmp.size(), hipMemcpyDeviceToHost,
+                            c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  out.resize(tmp.size());
   for (int d = 0; d < c->n; d++)
     for (int r = 0; r < ncomp; r++) out[(size_t)r * c->n + d] = tmp[(size_t)d * ncomp + r];
   return LSKUM_OK;
@@ -301,10 +341,6 @@ int lskum_create(const lskum_cloud *cloud, const lskum_params *params, lskum_ctx
     rc = LSKUM_EINVAL;
     err = "invalid parameters";
   }
-  if (rc == LSKUM_OK && p.precision == LSKUM_MIXED) {
-    rc = LSKUM_EINVAL;
-    err = "mixed precision is not built into this library";
-  }
   if (rc != LSKUM_OK) {
     // keep the message reachable: return the context in its failed state
     c->err = err;
@@ -312,6 +348,7 @@ int lskum_create(const lskum_cloud *cloud, const lskum_params *params, lskum_ctx
     return rc;
   }
   c->prm = p;
+  c->mixed = p.precision == LSKUM_MIXED;
   c->tile_block = p.tile_block ? p.tile_block : kTileBlockDefault;
   const int32_t n = cloud->n_points;
   c->n = n;
@@ -449,17 +486,24 @@ int lskum_create(const lskum_cloud *cloud, const lskum_params *params, lskum_ctx
       for (int32_t k : extra) loc[k] = -1;
     }
   }
-  // the gather kernels stage neighbours in LDS; the flux runs at higher
-  // occupancy reading its neighbours through L2, so by default it does not
-  const unsigned want = p.lds_tiles == LSKUM_TILES_AUTO
-                            ? (unsigned)(LSKUM_TILE_QDERIV | LSKUM_TILE_QINNER | LSKUM_TILE_DELTA)
-                            : (unsigned)p.lds_tiles;
+  // the gather kernels stage neighbours in LDS; the FP64 flux runs at
+  // higher occupancy reading its neighbours through L2, so by default it
+  // does not (its FP32 records, half the size, do fit at occupancy 2)
+  const unsigned auto_mask = LSKUM_TILE_QDERIV | LSKUM_TILE_QINNER | LSKUM_TILE_DELTA |
+                             (c->mixed ? LSKUM_TILE_FLUX : 0u);
+  const unsigned want =
+      p.lds_tiles == LSKUM_TILES_AUTO ? auto_mask : (unsigned)p.lds_tiles;
   if (tiles_ok && want) {
     int dev = 0, lds_limit = 0;
     if (hipGetDevice(&dev) == hipSuccess &&
         hipDeviceGetAttribute(&lds_limit, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) ==
-            hipSuccess)
-      c->tiles = prepare_tiled(want, c->tile_max, (size_t)lds_limit);
+            hipSuccess) {
+      if (c->mixed)  // func_delta is FP64 in both forms
+        c->tiles = prepare_tiled(want & TILE_DELTA, c->tile_max, (size_t)lds_limit) |
+                   prepare_tiled_m(want & ~TILE_DELTA, c->tile_max, (size_t)lds_limit);
+      else
+        c->tiles = prepare_tiled(want, c->tile_max, (size_t)lds_limit);
+    }
   }
   // shape points for the force reduction: the first shape_points points
   // (compute_force_coeffs_mod.h:25-27 takes m = j)
@@ -492,9 +536,18 @@ int lskum_create(const lskum_cloud *cloud, const lskum_params *params, lskum_ctx
   }
   c->part.nblocks = blocks_for(n, 256);
   ALLOC(c->base, (size_t)B_STRIDE * n);
-  ALLOC(c->dqb[0], (size_t)DQ_STRIDE * n);
-  ALLOC(c->dqb[1], (size_t)DQ_STRIDE * n);
-  ALLOC(c->qm, (size_t)QM_STRIDE * n);
+  if (c->mixed) {
+    ALLOC(c->xy, (size_t)2 * n);
+    ALLOC(c->q32, (size_t)4 * n);
+    ALLOC(c->dq32[0], (size_t)8 * n);
+    ALLOC(c->dq32[1], (size_t)8 * n);
+    ALLOC(c->qm32, (size_t)8 * n);
+    ALLOC(c->eps32, (size_t)n);
+  } else {
+    ALLOC(c->dqb[0], (size_t)DQ_STRIDE * n);
+    ALLOC(c->dqb[1], (size_t)DQ_STRIDE * n);
+    ALLOC(c->qm, (size_t)QM_STRIDE * n);
+  }
   ALLOC(c->prim, (size_t)4 * n);
   ALLOC(c->prim_old, (size_t)4 * n);
   ALLOC(c->flux_res, (size_t)4 * n);
@@ -538,6 +591,19 @@ int lskum_create(const lskum_cloud *cloud, const lskum_params *params, lskum_ctx
                          {c->f_l, fl.data(), fl.size() * sizeof(int32_t)},
                          {c->f_r, fr.data(), fr.size() * sizeof(int32_t)},
                          {c->f_s, fs.data(), fs.size() * sizeof(int32_t)}};
+  std::vector<double> xy_h;
+  std::vector<float> eps_h;
+  if (c->mixed) {
+    xy_h.resize((size_t)2 * n);
+    eps_h.resize(n);
+    for (int32_t d = 0; d < n; d++) {
+      xy_h[2 * (size_t)d] = base_h[(size_t)B_STRIDE * d + B_X];
+      xy_h[2 * (size_t)d + 1] = base_h[(size_t)B_STRIDE * d + B_Y];
+      eps_h[d] = (float)base_h[(size_t)B_STRIDE * d + B_EPS2];
+    }
+    ups.push_back({c->xy, xy_h.data(), xy_h.size() * sizeof(double)});
+    ups.push_back({c->eps32, eps_h.data(), eps_h.size() * sizeof(float)});
+  }
   if (c->tiles) {
     ups.push_back({c->conn16, conn16_h.data(), conn16_h.size() * sizeof(uint16_t)});
     ups.push_back({c->tile_off, toff.data(), toff.size() * sizeof(int32_t)});
@@ -550,9 +616,16 @@ int lskum_create(const lskum_cloud *cloud, const lskum_params *params, lskum_ctx
       return fail(LSKUM_EHIP);
     }
   }
-  if ((e = hipMemset(c->qm, 0, sizeof(double) * QM_STRIDE * n)) != hipSuccess ||
-      (e = hipMemset(c->dqb[0], 0, sizeof(double) * DQ_STRIDE * n)) != hipSuccess ||
-      (e = hipMemset(c->dqb[1], 0, sizeof(double) * DQ_STRIDE * n)) != hipSuccess ||
+  auto zero = [&](void *ptr, size_t bytes) {
+    return ptr ? hipMemset(ptr, 0, bytes) : hipSuccess;
+  };
+  if ((e = zero(c->qm, sizeof(double) * QM_STRIDE * n)) != hipSuccess ||
+      (e = zero(c->dqb[0], sizeof(double) * DQ_STRIDE * n)) != hipSuccess ||
+      (e = zero(c->dqb[1], sizeof(double) * DQ_STRIDE * n)) != hipSuccess ||
+      (e = zero(c->q32, sizeof(float) * 4 * n)) != hipSuccess ||
+      (e = zero(c->qm32, sizeof(float) * 8 * n)) != hipSuccess ||
+      (e = zero(c->dq32[0], sizeof(float) * 8 * n)) != hipSuccess ||
+      (e = zero(c->dq32[1], sizeof(float) * 8 * n)) != hipSuccess ||
       (e = hipMemset(c->flux_res, 0, sizeof(double) * 4 * n)) != hipSuccess ||
       (e = hipMemset(c->delta, 0, sizeof(double) * n)) != hipSuccess) {
     c->err = std::string("hipMemset: ") + hipGetErrorString(e);
@@ -759,12 +832,21 @@ int lskum_get_array(lskum_ctx *ctx, int32_t which, double *dst) {
   switch (which) {
     case LSKUM_A_PRIM: rc = fetch_plain(ctx, ctx->prim, 4, dev); ncomp = 4; break;
     case LSKUM_A_PRIM_OLD: rc = fetch_plain(ctx, ctx->prim_old, 4, dev); ncomp = 4; break;
-    case LSKUM_A_Q: rc = fetch_record(ctx, ctx->base, B_STRIDE, B_Q, 4, dev); ncomp = 4; break;
+    case LSKUM_A_Q:
+      rc = ctx->mixed ? fetch_float(ctx, ctx->q32, 4, dev)
+                      : fetch_record(ctx, ctx->base, B_STRIDE, B_Q, 4, dev);
+      ncomp = 4;
+      break;
     case LSKUM_A_DQ:
-      rc = fetch_record(ctx, ctx->dqb[ctx->cur], DQ_STRIDE, 0, 8, dev);
+      rc = ctx->mixed ? fetch_float(ctx, ctx->dq32[ctx->cur], 8, dev)
+                      : fetch_record(ctx, ctx->dqb[ctx->cur], DQ_STRIDE, 0, 8, dev);
       ncomp = 8;
       break;
-    case LSKUM_A_QM: rc = fetch_record(ctx, ctx->qm, QM_STRIDE, 0, 8, dev); ncomp = 8; break;
+    case LSKUM_A_QM:
+      rc = ctx->mixed ? fetch_float(ctx, ctx->qm32, 8, dev)
+                      : fetch_record(ctx, ctx->qm, QM_STRIDE, 0, 8, dev);
+      ncomp = 8;
+      break;
     case LSKUM_A_DELTA: rc = fetch_plain(ctx, ctx->delta, 1, dev); ncomp = 1; break;
     case LSKUM_A_FLUX_RES: rc = fetch_plain(ctx, ctx->flux_res, 4, dev); ncomp = 4; break;
     case LSKUM_A_NORMALS:
@@ -828,7 +910,7 @@ int lskum_kernel_time(lskum_ctx *ctx, int32_t kernel, double *avg_ms, int32_t *l
 }
 
 int lskum_kat_fluxes(int32_t n, const double *in6, double *out, int32_t mixed) {
-  if (n <= 0 || !in6 || !out || mixed) return LSKUM_EINVAL;
+  if (n <= 0 || !in6 || !out) return LSKUM_EINVAL;
   double *din = nullptr, *dout = nullptr;
   if (dalloc(&din, (size_t)6 * n) != hipSuccess) return LSKUM_ENOMEM;
   if (dalloc(&dout, (size_t)32 * n) != hipSuccess) {
@@ -838,7 +920,10 @@ int lskum_kat_fluxes(int32_t n, const double *in6, double *out, int32_t mixed) {
   int rc = LSKUM_OK;
   if (hipMemcpy(din, in6, sizeof(double) * 6 * n, hipMemcpyHostToDevice) != hipSuccess) rc = LSKUM_EHIP;
   if (!rc) {
-    launch_kat_fluxes(n, din, dout, nullptr);
+    if (mixed)
+      launch_kat_fluxes_m(n, din, dout, nullptr);
+    else
+      launch_kat_fluxes(n, din, dout, nullptr);
     if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) rc = LSKUM_EHIP;
   }
   if (!rc && hipMemcpy(out, dout, sizeof(double) * 32 * n, hipMemcpyDeviceToHost) != hipSuccess)

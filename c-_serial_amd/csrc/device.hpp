@@ -1,0 +1,64 @@
+// device.hpp -- device helpers shared by the FP64 and mixed-precision kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "kernels.hpp"
+
+namespace lskum {
+
+// sub-stencil bits of a conn entry (x+, x-, y+, y-)
+constexpr uint32_t XPOS = 1u, XNEG = 2u, YPOS = 4u, YNEG = 8u;
+
+// blockIdx -> block of points such that each of the 8 XCDs (which receive
+// workgroups round-robin) works on one contiguous range of points and keeps
+// its neighbours' records in its own L2.
+__device__ __forceinline__ int xcd_block() {
+  const int nb = gridDim.x, b = blockIdx.x;
+  const int q = nb >> 3, r = nb & 7, xcd = b & 7, local = b >> 3;
+  return xcd < r ? xcd * (q + 1) + local : r * (q + 1) + (xcd - r) * q + local;
+}
+
+__device__ __forceinline__ void load4(const double *p, double v[4]) {
+  const double2 a = reinterpret_cast<const double2 *>(p)[0];
+  const double2 b = reinterpret_cast<const double2 *>(p)[1];
+  v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
+}
+
+__device__ __forceinline__ void store4(double *p, const double v[4]) {
+  reinterpret_cast<double2 *>(p)[0] = make_double2(v[0], v[1]);
+  reinterpret_cast<double2 *>(p)[1] = make_double2(v[2], v[3]);
+}
+
+__device__ __forceinline__ void unpack4(const float4 a, float v[4]) {
+  v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+}
+
+// a - b as one rounded subtraction that the compiler may not fuse with the
+// product that produced a (a*b' - c in one FMA would turn the difference of
+// two equal fluxes into the rounding error of one of them, and uniform flow
+// would no longer give an exactly zero residual).  Needs the build's
+// -ffp-contract=fast-honor-pragmas.
+template <typename T>
+__device__ __forceinline__ T diff_unfused(T a, T b) {
+#pragma clang fp contract(off)
+  return a - b;
+}
+
+// Neighbour of conn entry e: a device index (global form) or an index into
+// the block's LDS tile (tiled form), with the entry's sub-stencil bits.
+__device__ __forceinline__ int nbr_index(const uint32_t *conn, const uint16_t *conn16, int e,
+                                         bool tiled, uint32_t &mask) {
+  if (tiled) {
+    const uint16_t w = conn16[e];
+    mask = w >> kLocalShift;
+    return w & kLocalMask;
+  }
+  const uint32_t w = conn[e];
+  mask = w >> kMaskShift;
+  return (int)(w & kIndexMask);
+}
+
+}  // namespace lskum

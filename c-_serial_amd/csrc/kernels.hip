@@ -9,38 +9,13 @@
 // 64-byte per-point records through L2.
 #include <hip/hip_runtime.h>
 
+#include "device.hpp"
 #include "kernels.hpp"
 #include "physics.hpp"
 
 namespace lskum {
 
 namespace {
-
-constexpr uint32_t XPOS = 1u, XNEG = 2u, YPOS = 4u, YNEG = 8u;
-
-// blockIdx -> block of points such that each of the 8 XCDs (which receive
-// workgroups round-robin) works on one contiguous range of points and keeps
-// its neighbours' records in its own L2.
-__device__ __forceinline__ int xcd_block() {
-  const int nb = gridDim.x, b = blockIdx.x;
-  const int q = nb >> 3, r = nb & 7, xcd = b & 7, local = b >> 3;
-  return xcd < r ? xcd * (q + 1) + local : r * (q + 1) + (xcd - r) * q + local;
-}
-
-__device__ __forceinline__ int point_index() {
-  return xcd_block() * blockDim.x + threadIdx.x;
-}
-
-__device__ __forceinline__ void load4(const double *p, double v[4]) {
-  const double2 a = reinterpret_cast<const double2 *>(p)[0];
-  const double2 b = reinterpret_cast<const double2 *>(p)[1];
-  v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
-}
-
-__device__ __forceinline__ void store4(double *p, const double v[4]) {
-  reinterpret_cast<double2 *>(p)[0] = make_double2(v[0], v[1]);
-  reinterpret_cast<double2 *>(p)[1] = make_double2(v[2], v[3]);
-}
 
 // ------------------------------------------------------------ tiles
 // A block's LDS tile holds its own 256 points first (local index =
@@ -76,18 +51,6 @@ __device__ __forceinline__ void load_tile(double *lds, const int32_t *__restrict
       }
     }
   }
-}
-
-__device__ __forceinline__ int nbr_index(const uint32_t *conn, const uint16_t *conn16, int e,
-                                         bool tiled, uint32_t &mask) {
-  if (tiled) {
-    const uint16_t w = conn16[e];
-    mask = w >> kLocalShift;
-    return w & kLocalMask;
-  }
-  const uint32_t w = conn[e];
-  mask = w >> kMaskShift;
-  return (int)(w & kIndexMask);
 }
 
 // ------------------------------------------------------------ q variables
@@ -418,7 +381,7 @@ __global__ __launch_bounds__(256) void k_flux(
       quad_x(ek, sx, s2b, Gk);
     }
     const double wx = xp ? cf[0] * dsw + cf[1] * dnw : cf[2] * dsw + cf[3] * dnw;
-    for (int r = 0; r < 4; r++) acc[r] += (Gk[r] - Gi[r]) * wx;
+    for (int r = 0; r < 4; r++) acc[r] += diff_unfused(Gk[r], Gi[r]) * wx;
     if (xp && (m & XNEG)) {  // dels == 0: the neighbour is in both lists
       if (type == 1) {
         split_x(ei, -1.0, Gi);
@@ -428,7 +391,7 @@ __global__ __launch_bounds__(256) void k_flux(
         quad_x(ek, -1.0, s2b, Gk);
       }
       const double wn = cf[2] * dsw + cf[3] * dnw;
-      for (int r = 0; r < 4; r++) acc[r] += (Gk[r] - Gi[r]) * wn;
+      for (int r = 0; r < 4; r++) acc[r] += diff_unfused(Gk[r], Gi[r]) * wn;
     }
     // y direction: interior y+ / y-, wall y- (all), outer y+ (all)
     const bool yp = type == 1 ? (m & YPOS) != 0 : type == 2;
@@ -436,12 +399,12 @@ __global__ __launch_bounds__(256) void k_flux(
     split_y(ei, sy, Gi);
     split_y(ek, sy, Gk);
     const double wy = yp ? cf[4] * dnw + cf[5] * dsw : cf[6] * dnw + cf[7] * dsw;
-    for (int r = 0; r < 4; r++) acc[r] += (Gk[r] - Gi[r]) * wy;
+    for (int r = 0; r < 4; r++) acc[r] += diff_unfused(Gk[r], Gi[r]) * wy;
     if (type == 1 && yp && (m & YNEG)) {  // deln == 0
       split_y(ei, -1.0, Gi);
       split_y(ek, -1.0, Gk);
       const double wn = cf[6] * dnw + cf[7] * dsw;
-      for (int r = 0; r < 4; r++) acc[r] += (Gk[r] - Gi[r]) * wn;
+      for (int r = 0; r < 4; r++) acc[r] += diff_unfused(Gk[r], Gi[r]) * wn;
     }
   }
   // flux_residual_mod.h:29 (wall: 2 delta), :45, :85
@@ -536,7 +499,8 @@ __global__ __launch_bounds__(256) void k_state_update(int n, int rk, double *__r
                                                       FlowConsts fc, double *__restrict__ psum,
                                                       double *__restrict__ pmax,
                                                       int32_t *__restrict__ parg,
-                                                      double *__restrict__ base, int speed) {
+                                                      double *__restrict__ base,
+                                                      float4 *__restrict__ q32, int speed) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double res_sqr = 0.0;
   int arg = 0x7fffffff;
@@ -578,7 +542,7 @@ __global__ __launch_bounds__(256) void k_state_update(int n, int rk, double *__r
     prim[(size_t)n + i] = u1;
     prim[(size_t)2 * n + i] = u2;
     prim[(size_t)3 * n + i] = pr;
-    if (base) {
+    if (base || q32) {
       const double beta = 0.5 * rho / pr;
       double q[4];
       q[0] = log(rho) + (log(beta) * 2.5) - beta * (u1 * u1 + u2 * u2);
@@ -586,11 +550,14 @@ __global__ __launch_bounds__(256) void k_state_update(int n, int rk, double *__r
       q[1] = two_times_beta * u1;
       q[2] = two_times_beta * u2;
       q[3] = -two_times_beta;
-      store4(base + (size_t)B_STRIDE * i + B_Q, q);
-      if (speed) {
-        const double mod_u = sqrt(u1 * u1 + u2 * u2);
-        base[(size_t)B_STRIDE * i + B_SPEED] = mod_u + 3.0 * sqrt(pr / rho);
-      }
+      if (q32)  // mixed form: q rounded once, from the FP64 state
+        q32[i] = make_float4((float)q[0], (float)q[1], (float)q[2], (float)q[3]);
+      else
+        store4(base + (size_t)B_STRIDE * i + B_Q, q);
+    }
+    if (base && speed) {
+      const double mod_u = sqrt(u1 * u1 + u2 * u2);
+      base[(size_t)B_STRIDE * i + B_SPEED] = mod_u + 3.0 * sqrt(pr / rho);
     }
   }
   double s = res_sqr, mx = res_sqr;
@@ -809,11 +776,11 @@ void launch_flux(const DevCloud &c, const double *base, const double *dq, const 
 
 void launch_state_update(const DevCloud &c, int rk, double *prim, const double *prim_old,
                          const double *flux_res, const FlowConsts &fc, const Partials &p,
-                         double *base, bool speed, hipStream_t s) {
+                         double *base, float *q32, bool speed, hipStream_t s) {
   // fixed 256-thread blocks: the partial layout must not depend on tuning
   hipLaunchKernelGGL(k_state_update, dim3(p.nblocks), dim3(256), 0, s, c.n, rk, prim, prim_old,
                      flux_res, c.nx, c.ny, c.flag, c.perm, fc, p.sum, p.max, p.arg, base,
-                     speed ? 1 : 0);
+                     reinterpret_cast<float4 *>(q32), speed ? 1 : 0);
 }
 
 void launch_reduce(const Partials &p, double *out, double *hist_slot, hipStream_t s) {

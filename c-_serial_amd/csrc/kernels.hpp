@@ -81,11 +81,12 @@ void launch_q_inner(const DevCloud &c, const double *base, const double *dq_src,
                     double *dq_dst, double power, hipStream_t s);
 void launch_flux(const DevCloud &c, const double *base, const double *dq, const double *qm,
                  const double *delta, double power, double *flux_res, hipStream_t s);
-// also writes q of the new state into base (when non-null) and, with
-// speed, the func_delta denominator
+// also writes q of the new state, as FP32 into q32 when non-null (mixed
+// form), else into base when non-null, and, with speed, the func_delta
+// denominator into base
 void launch_state_update(const DevCloud &c, int rk, double *prim, const double *prim_old,
                          const double *flux_res, const FlowConsts &fc, const Partials &p,
-                         double *base, bool speed, hipStream_t s);
+                         double *base, float *q32, bool speed, hipStream_t s);
 // reduces the partials into out[0] = sum, out[1] = max, out[2] = argmax
 // (caller index, as a double), and also hist_slot[0] = sum when non-null
 void launch_reduce(const Partials &p, double *out, double *hist_slot, hipStream_t s);
@@ -95,5 +96,28 @@ void launch_forces(int n_shape, const int32_t *m, const int32_t *l, const int32_
                    double *out3, hipStream_t s);
 void launch_kat_fluxes(int n, const double *in6, double *out, hipStream_t s);
 void launch_kat_qtilde(int n, const double *qt4, double *out, hipStream_t s);
+
+// ---------------------------------------------- mixed-precision form
+// Per-point arrays (device order) the mixed kernels gather: coordinates in
+// FP64, everything else in FP32.  dq buffers are [n][8] floats (dq/dx 0..3,
+// dq/dy 0..3), qm32 is [n][8] (qmax 0..3, qmin 0..3).
+struct MixedState {
+  const double *xy;  // [n][2]
+  float *q32;        // [n][4]
+  float *qm32;       // [n][8]
+  const float *eps32;  // [n], (pow(K*min_dist,3))^2 rounded to FP32
+};
+size_t lds_bytes_qderiv_m(int tile_max);
+size_t lds_bytes_qinner_m(int tile_max);
+size_t lds_bytes_flux_m(int tile_max);
+unsigned prepare_tiled_m(unsigned tiles, int tile_max, size_t lds_limit);
+void launch_q_variables_m(const DevCloud &c, const double *prim, float *q32, hipStream_t s);
+void launch_q_derivatives_m(const DevCloud &c, const MixedState &m, float *dq, double power,
+                            hipStream_t s);
+void launch_q_inner_m(const DevCloud &c, const MixedState &m, const float *src, float *dst,
+                      double power, hipStream_t s);
+void launch_flux_m(const DevCloud &c, const MixedState &m, const float *dq, const double *delta,
+                   double power, double *flux_res, hipStream_t s);
+void launch_kat_fluxes_m(int n, const double *in6, double *out, hipStream_t s);
 
 }  // namespace lskum

@@ -1,0 +1,473 @@
+// kernels_mixed.hip -- q-LSKUM stage kernels, mixed-precision form (gfx950).
+//
+// Neighbour data are gathered in FP32: q (float4), dq/dx and dq/dy (two
+// float4), qmax/qmin (two float4) and the limiter's eps^2 (float).  The
+// coordinates stay FP64 so that a neighbour offset dx = x_k - x_i is exact
+// before it is rounded to FP32: at a cosine-clustered leading edge |dx| is
+// ~1e-6 chord, and an FP32 difference of absolute coordinates would keep two
+// digits of it.  The limiter, q~ -> primitive and the split/quadrant fluxes
+// run in FP32.  FP64 is kept for
+//  - the least-squares geometry sums, determinants and 2x2 solves (here in
+//    q_derivatives and the inner passes; the flux applies its per-point
+//    FP64 coefficients, computed once on the host),
+//  - the accumulation of every per-point derivative and flux sum,
+//  - the state, the SSPRK update and the residual and force reductions
+//    (kernels.hip, shared with the FP64 form).
+// Differences are formed directly where FP32 would otherwise cancel: the
+// limiter takes del_neg = -0.5*(dq . dx) rather than q~ - q, and an inner
+// pass sums (q_k - q_i) - 0.5*(dx . (dq_k - dq_i)) rather than the
+// difference of two reconstructed values.
+#include <hip/hip_runtime.h>
+
+#include "device.hpp"
+#include "kernels.hpp"
+#include "physics.hpp"
+
+namespace lskum {
+
+namespace {
+
+// Copies one tile's records of CH 16-byte chunks into LDS:
+// dst[r*CH + ch] = src[idx[r]*CH + ch], four loads in flight per thread.
+template <int CH>
+__device__ __forceinline__ void tile_copy16(uint4 *dst, const uint4 *__restrict__ src,
+                                            const int32_t *__restrict__ idx, int count) {
+  constexpr int U = 4;
+  const int total = count * CH;
+  for (int base = threadIdx.x; base < total; base += U * blockDim.x) {
+    uint4 v[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int t = base + u * blockDim.x;
+      if (t < total) {
+        const int r = t / CH, ch = t - r * CH;
+        v[u] = src[(size_t)CH * idx[r] + ch];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int t = base + u * blockDim.x;
+      if (t < total) dst[t] = v[u];
+    }
+  }
+}
+
+__device__ __forceinline__ void tile_copy4(float *dst, const float *__restrict__ src,
+                                           const int32_t *__restrict__ idx, int count) {
+  for (int t = threadIdx.x; t < count; t += blockDim.x) dst[t] = src[idx[t]];
+}
+
+// ------------------------------------------------------------ q variables
+// q_variables_mod.h:9-34 in FP64, rounded once to FP32
+__global__ __launch_bounds__(256) void k_q_variables_m(int n, const double *__restrict__ prim,
+                                                       float4 *__restrict__ q32) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const double rho = prim[k], u1 = prim[n + k], u2 = prim[2 * n + k], pr = prim[3 * n + k];
+  const double beta = 0.5 * rho / pr;
+  const double q0 = log(rho) + (log(beta) * 2.5) - beta * (u1 * u1 + u2 * u2);
+  const double two_times_beta = 2.0 * beta;
+  q32[k] = make_float4((float)q0, (float)(two_times_beta * u1), (float)(two_times_beta * u2),
+                       (float)(-two_times_beta));
+}
+
+// ------------------------------------------------------------ q derivatives
+// q_variables_mod.h:36-118: FP32 gathers of q, FP64 geometry sums, solve
+// and accumulation; qmax/qmin over FP32 values (rounding is monotone, so
+// they are the rounded FP64 extrema).
+template <bool TILED, bool POW>
+__global__ __launch_bounds__(256) void k_q_derivatives_m(
+    int n, const int32_t *__restrict__ off, const uint32_t *__restrict__ conn,
+    const uint16_t *__restrict__ conn16, const int32_t *__restrict__ tile_off,
+    const int32_t *__restrict__ tile_idx, int tile_max, const double2 *__restrict__ xy,
+    const float4 *__restrict__ q32, float4 *__restrict__ dq_out, float4 *__restrict__ qm_out,
+    double power) {
+  extern __shared__ uint4 lds4[];
+  double2 *lxy = reinterpret_cast<double2 *>(lds4);
+  float4 *lq = reinterpret_cast<float4 *>(lds4 + tile_max);
+  const int b = xcd_block();
+  if (TILED) {
+    const int t0 = tile_off[b], tn = tile_off[b + 1] - t0;
+    tile_copy16<1>(lds4, reinterpret_cast<const uint4 *>(xy), tile_idx + t0, tn);
+    tile_copy16<1>(lds4 + tile_max, reinterpret_cast<const uint4 *>(q32), tile_idx + t0, tn);
+    __syncthreads();
+  }
+  const int i = b * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double2 *gxy = TILED ? lxy : xy;
+  const float4 *gq = TILED ? lq : q32;
+  const int li = TILED ? (int)threadIdx.x : i;
+  const double2 pi = gxy[li];
+  float qi[4], qmax[4], qmin[4];
+  unpack4(gq[li], qi);
+  double sum_delx_sqr = 0., sum_dely_sqr = 0., sum_delx_dely = 0.;
+  double sdx[4] = {0.0, 0.0, 0.0, 0.0}, sdy[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int r = 0; r < 4; r++) qmax[r] = qmin[r] = qi[r];
+  const int e1 = off[i + 1];
+  for (int e = off[i]; e < e1; e++) {
+    uint32_t m;
+    const int k = nbr_index(conn, conn16, e, TILED, m);
+    const double2 pk = gxy[k];
+    float qk[4];
+    unpack4(gq[k], qk);
+    for (int r = 0; r < 4; r++) {
+      qmax[r] = qk[r] > qmax[r] ? qk[r] : qmax[r];
+      qmin[r] = qk[r] < qmin[r] ? qk[r] : qmin[r];
+    }
+    const double delx = pk.x - pi.x, dely = pk.y - pi.y;
+    const double weights = POW ? pow(sqrt(delx * delx + dely * dely), power) : 1.0;
+    const double wx = POW ? weights * delx : delx, wy = POW ? weights * dely : dely;
+    sum_delx_sqr = sum_delx_sqr + wx * delx;
+    sum_dely_sqr = sum_dely_sqr + wy * dely;
+    sum_delx_dely = sum_delx_dely + wx * dely;
+    for (int r = 0; r < 4; r++) {
+      const double d = (double)(qk[r] - qi[r]);
+      sdx[r] = sdx[r] + wx * d;
+      sdy[r] = sdy[r] + wy * d;
+    }
+  }
+  const double det = sum_delx_sqr * sum_dely_sqr - sum_delx_dely * sum_delx_dely;
+  const double one_by_det = 1.0 / det;
+  float dqx[4], dqy[4];
+  for (int r = 0; r < 4; r++) {
+    dqx[r] = (float)((sdx[r] * sum_dely_sqr - sdy[r] * sum_delx_dely) * one_by_det);
+    dqy[r] = (float)((sdy[r] * sum_delx_sqr - sdx[r] * sum_delx_dely) * one_by_det);
+  }
+  dq_out[2 * (size_t)i] = make_float4(dqx[0], dqx[1], dqx[2], dqx[3]);
+  dq_out[2 * (size_t)i + 1] = make_float4(dqy[0], dqy[1], dqy[2], dqy[3]);
+  qm_out[2 * (size_t)i] = make_float4(qmax[0], qmax[1], qmax[2], qmax[3]);
+  qm_out[2 * (size_t)i + 1] = make_float4(qmin[0], qmin[1], qmin[2], qmin[3]);
+}
+
+// ------------------------------------------------------------ inner passes
+// q_variables_mod.h:120-228, one Jacobi pass (src -> dst), FP32 gathers and
+// corrected differences, FP64 sums and solve.
+template <bool TILED, bool POW>
+__global__ __launch_bounds__(256) void k_q_inner_m(
+    int n, const int32_t *__restrict__ off, const uint32_t *__restrict__ conn,
+    const uint16_t *__restrict__ conn16, const int32_t *__restrict__ tile_off,
+    const int32_t *__restrict__ tile_idx, int tile_max, const double2 *__restrict__ xy,
+    const float4 *__restrict__ q32, const float4 *__restrict__ src, float4 *__restrict__ dst,
+    double power) {
+  extern __shared__ uint4 lds4[];
+  double2 *lxy = reinterpret_cast<double2 *>(lds4);
+  float4 *lq = reinterpret_cast<float4 *>(lds4 + tile_max);
+  float4 *ld = reinterpret_cast<float4 *>(lds4 + 2 * tile_max);
+  const int b = xcd_block();
+  if (TILED) {
+    const int t0 = tile_off[b], tn = tile_off[b + 1] - t0;
+    tile_copy16<1>(lds4, reinterpret_cast<const uint4 *>(xy), tile_idx + t0, tn);
+    tile_copy16<1>(lds4 + tile_max, reinterpret_cast<const uint4 *>(q32), tile_idx + t0, tn);
+    tile_copy16<2>(lds4 + 2 * tile_max, reinterpret_cast<const uint4 *>(src), tile_idx + t0, tn);
+    __syncthreads();
+  }
+  const int i = b * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double2 *gxy = TILED ? lxy : xy;
+  const float4 *gq = TILED ? lq : q32;
+  const float4 *gd = TILED ? ld : src;
+  const int li = TILED ? (int)threadIdx.x : i;
+  const double2 pi = gxy[li];
+  float qi[4], dxi[4], dyi[4];
+  unpack4(gq[li], qi);
+  unpack4(gd[2 * li], dxi);
+  unpack4(gd[2 * li + 1], dyi);
+  double sum_delx_sqr = 0., sum_dely_sqr = 0., sum_delx_dely = 0.;
+  double sx[4] = {0.0, 0.0, 0.0, 0.0}, sy[4] = {0.0, 0.0, 0.0, 0.0};
+  const int e1 = off[i + 1];
+  for (int e = off[i]; e < e1; e++) {
+    uint32_t m;
+    const int k = nbr_index(conn, conn16, e, TILED, m);
+    const double2 pk = gxy[k];
+    float qk[4], dxk[4], dyk[4];
+    unpack4(gq[k], qk);
+    unpack4(gd[2 * k], dxk);
+    unpack4(gd[2 * k + 1], dyk);
+    const double delx = pk.x - pi.x, dely = pk.y - pi.y;
+    const float fx = (float)delx, fy = (float)dely;
+    const double weights = POW ? pow(sqrt(delx * delx + dely * dely), power) : 1.0;
+    const double wx = POW ? weights * delx : delx, wy = POW ? weights * dely : dely;
+    sum_delx_sqr = sum_delx_sqr + wx * delx;
+    sum_dely_sqr = sum_dely_sqr + wy * dely;
+    sum_delx_dely = sum_delx_dely + wx * dely;
+    for (int r = 0; r < 4; r++) {
+      const float d =
+          (qk[r] - qi[r]) - 0.5f * (fx * (dxk[r] - dxi[r]) + fy * (dyk[r] - dyi[r]));
+      sx[r] = sx[r] + wx * (double)d;
+      sy[r] = sy[r] + wy * (double)d;
+    }
+  }
+  const double det = sum_delx_sqr * sum_dely_sqr - sum_delx_dely * sum_delx_dely;
+  const double one_by_det = 1.0 / det;
+  float ox[4], oy[4];
+  for (int r = 0; r < 4; r++) {
+    ox[r] = (float)(one_by_det * (sx[r] * sum_dely_sqr - sy[r] * sum_delx_dely));
+    oy[r] = (float)(one_by_det * (sy[r] * sum_delx_sqr - sx[r] * sum_delx_dely));
+  }
+  dst[2 * (size_t)i] = make_float4(ox[0], ox[1], ox[2], ox[3]);
+  dst[2 * (size_t)i + 1] = make_float4(oy[0], oy[1], oy[2], oy[3]);
+}
+
+// ------------------------------------------------------------ flux residual
+// flux_residual_mod.h:10-89 and the *_dG* sub-stencil routines, as the FP64
+// kernel (one pass over the neighbour list, shared end terms, per-point
+// least-squares coefficients), with the reconstruction, limiter, q~ ->
+// primitive and fluxes in FP32 and the coefficient products and sums in
+// FP64.
+constexpr int COEF_STRIDE = 8;
+
+template <bool TILED, bool POW>
+__global__ __launch_bounds__(256) void k_flux_m(
+    int n, const int32_t *__restrict__ off, const uint32_t *__restrict__ conn,
+    const uint16_t *__restrict__ conn16, const int32_t *__restrict__ tile_off,
+    const int32_t *__restrict__ tile_idx, int tile_max, const double2 *__restrict__ xy,
+    const float4 *__restrict__ q32, const float4 *__restrict__ dq32,
+    const float4 *__restrict__ qm32, const float *__restrict__ eps32,
+    const double *__restrict__ coef, const double *__restrict__ nxa,
+    const double *__restrict__ nya, const uint8_t *__restrict__ flag,
+    const double *__restrict__ delta, double power, double *__restrict__ fres) {
+  // LDS tile: xy (16 B), q (16 B), dq (32 B), qm (32 B), eps2 (4 B) per entry
+  extern __shared__ uint4 lds4[];
+  double2 *lxy = reinterpret_cast<double2 *>(lds4);
+  float4 *lq = reinterpret_cast<float4 *>(lds4 + tile_max);
+  float4 *ld = reinterpret_cast<float4 *>(lds4 + 2 * tile_max);
+  float4 *lm = reinterpret_cast<float4 *>(lds4 + 4 * tile_max);
+  float *le = reinterpret_cast<float *>(lds4 + 6 * tile_max);
+  const int b = xcd_block();
+  if (TILED) {
+    const int t0 = tile_off[b], tn = tile_off[b + 1] - t0;
+    const int32_t *idx = tile_idx + t0;
+    tile_copy16<1>(lds4, reinterpret_cast<const uint4 *>(xy), idx, tn);
+    tile_copy16<1>(lds4 + tile_max, reinterpret_cast<const uint4 *>(q32), idx, tn);
+    tile_copy16<2>(lds4 + 2 * tile_max, reinterpret_cast<const uint4 *>(dq32), idx, tn);
+    tile_copy16<2>(lds4 + 4 * tile_max, reinterpret_cast<const uint4 *>(qm32), idx, tn);
+    tile_copy4(le, eps32, idx, tn);
+    __syncthreads();
+  }
+  const int i = b * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double2 *gxy = TILED ? lxy : xy;
+  const float4 *gq = TILED ? lq : q32;
+  const float4 *gd = TILED ? ld : dq32;
+  const float4 *gm = TILED ? lm : qm32;
+  const float *ge = TILED ? le : eps32;
+  const int li = TILED ? (int)threadIdx.x : i;
+  const double2 pi = gxy[li];
+  float qi[4], dxi[4], dyi[4], dmaxi[4], dmini[4];
+  unpack4(gq[li], qi);
+  unpack4(gd[2 * li], dxi);
+  unpack4(gd[2 * li + 1], dyi);
+  unpack4(gm[2 * li], dmaxi);
+  unpack4(gm[2 * li + 1], dmini);
+  for (int r = 0; r < 4; r++) {
+    dmaxi[r] -= qi[r];
+    dmini[r] -= qi[r];
+  }
+  const float eps2_i = ge[li];
+  double cf[8];
+  load4(coef + (size_t)COEF_STRIDE * i, cf);
+  load4(coef + (size_t)COEF_STRIDE * i + 4, cf + 4);
+  const double nx = nxa[i], ny = nya[i];
+  const double tx = ny, ty = -nx;
+  const float nxf = (float)nx, nyf = (float)ny;
+  const int type = flag[i];
+  const float s2b = type == 0 ? -1.0f : 1.0f;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  const int e1 = off[i + 1];
+  for (int e = off[i]; e < e1; e++) {
+    uint32_t m;
+    const int k = nbr_index(conn, conn16, e, TILED, m);
+    const double2 pk = gxy[k];
+    float qk[4], dxk[4], dyk[4], dmaxk[4], dmink[4];
+    unpack4(gq[k], qk);
+    unpack4(gd[2 * k], dxk);
+    unpack4(gd[2 * k + 1], dyk);
+    unpack4(gm[2 * k], dmaxk);
+    unpack4(gm[2 * k + 1], dmink);
+    const float eps2_k = ge[k];
+    const double delx = pk.x - pi.x, dely = pk.y - pi.y;
+    const double dels = delx * tx + dely * ty;
+    const double deln = delx * nx + dely * ny;
+    double dsw = dels, dnw = deln;
+    if (POW) {
+      const double weights = pow(sqrt(dels * dels + deln * deln), power);
+      dsw = dels * weights;
+      dnw = deln * weights;
+    }
+    const float fx = (float)delx, fy = (float)dely;
+    // limited q~ at both ends (interior_fluxes_mod.h:72-83, limiters_mod.h:55-104)
+    float qti[4], qtk[4];
+    for (int r = 0; r < 4; r++) {
+      const float di = fx * dxi[r] + fy * dyi[r];
+      const float dk = fx * dxk[r] + fy * dyk[r];
+      const float phi_i = venkat_phi_delta(-0.5f * di, dmaxi[r], dmini[r], eps2_i);
+      const float phi_k =
+          venkat_phi_delta(-0.5f * dk, dmaxk[r] - qk[r], dmink[r] - qk[r], eps2_k);
+      qti[r] = qi[r] - 0.5f * phi_i * di;
+      qtk[r] = qk[r] - 0.5f * phi_k * dk;
+    }
+    float u1, u2, rho, beta, temp;
+    qtilde_state(qti, u1, u2, rho, beta, temp);
+    const EndTerms<float> ei = end_terms_bt(nxf, nyf, u1, u2, rho, beta, temp);
+    qtilde_state(qtk, u1, u2, rho, beta, temp);
+    const EndTerms<float> ek = end_terms_bt(nxf, nyf, u1, u2, rho, beta, temp);
+    float Gi[4], Gk[4];
+    const bool xp = (m & XPOS) != 0;
+    const float sx = xp ? 1.0f : -1.0f;
+    if (type == 1) {
+      split_x(ei, sx, Gi);
+      split_x(ek, sx, Gk);
+    } else {
+      quad_x(ei, sx, s2b, Gi);
+      quad_x(ek, sx, s2b, Gk);
+    }
+    const double wx = xp ? cf[0] * dsw + cf[1] * dnw : cf[2] * dsw + cf[3] * dnw;
+    for (int r = 0; r < 4; r++) acc[r] += (double)diff_unfused(Gk[r], Gi[r]) * wx;
+    if (xp && (m & XNEG)) {  // dels == 0: the neighbour is in both lists
+      if (type == 1) {
+        split_x(ei, -1.0f, Gi);
+        split_x(ek, -1.0f, Gk);
+      } else {
+        quad_x(ei, -1.0f, s2b, Gi);
+        quad_x(ek, -1.0f, s2b, Gk);
+      }
+      const double wn = cf[2] * dsw + cf[3] * dnw;
+      for (int r = 0; r < 4; r++) acc[r] += (double)diff_unfused(Gk[r], Gi[r]) * wn;
+    }
+    const bool yp = type == 1 ? (m & YPOS) != 0 : type == 2;
+    const float sy = yp ? 1.0f : -1.0f;
+    split_y(ei, sy, Gi);
+    split_y(ek, sy, Gk);
+    const double wy = yp ? cf[4] * dnw + cf[5] * dsw : cf[6] * dnw + cf[7] * dsw;
+    for (int r = 0; r < 4; r++) acc[r] += (double)diff_unfused(Gk[r], Gi[r]) * wy;
+    if (type == 1 && yp && (m & YNEG)) {  // deln == 0
+      split_y(ei, -1.0f, Gi);
+      split_y(ek, -1.0f, Gk);
+      const double wn = cf[6] * dnw + cf[7] * dsw;
+      for (int r = 0; r < 4; r++) acc[r] += (double)diff_unfused(Gk[r], Gi[r]) * wn;
+    }
+  }
+  const double d = type == 0 ? 2.0 * delta[i] : delta[i];
+  for (int j = 0; j < 4; j++) fres[(size_t)j * n + i] = d * acc[j];
+}
+
+// ------------------------------------------------------------ KAT
+// The eight flux functions in FP32 on FP32-rounded inputs.
+__global__ void k_kat_fluxes_m(int n, const double *__restrict__ in, double *__restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double *v = in + 6 * (size_t)i;
+  const EndTerms<float> e = end_terms((float)v[0], (float)v[1], (float)v[2], (float)v[3],
+                                      (float)v[4], (float)v[5]);
+  float G[8][4];
+  split_x(e, 1.0f, G[0]);
+  split_x(e, -1.0f, G[1]);
+  split_y(e, 1.0f, G[2]);
+  split_y(e, -1.0f, G[3]);
+  quad_x(e, -1.0f, -1.0f, G[4]);
+  quad_x(e, 1.0f, -1.0f, G[5]);
+  quad_x(e, 1.0f, 1.0f, G[6]);
+  quad_x(e, -1.0f, 1.0f, G[7]);
+  for (int f = 0; f < 8; f++)
+    for (int r = 0; r < 4; r++) out[32 * (size_t)i + 4 * f + r] = (double)G[f][r];
+}
+
+}  // namespace
+
+size_t lds_bytes_qderiv_m(int tile_max) { return 32 * (size_t)tile_max; }
+size_t lds_bytes_qinner_m(int tile_max) { return 64 * (size_t)tile_max; }
+size_t lds_bytes_flux_m(int tile_max) { return 100 * (size_t)tile_max; }
+
+unsigned prepare_tiled_m(unsigned tiles, int tile_max, size_t lds_limit) {
+  struct K { unsigned bit; const void *f; size_t bytes; };
+  const K ks[] = {
+      {TILE_QDERIV, (const void *)k_q_derivatives_m<true, false>, lds_bytes_qderiv_m(tile_max)},
+      {TILE_QDERIV, (const void *)k_q_derivatives_m<true, true>, lds_bytes_qderiv_m(tile_max)},
+      {TILE_QINNER, (const void *)k_q_inner_m<true, false>, lds_bytes_qinner_m(tile_max)},
+      {TILE_QINNER, (const void *)k_q_inner_m<true, true>, lds_bytes_qinner_m(tile_max)},
+      {TILE_FLUX, (const void *)k_flux_m<true, false>, lds_bytes_flux_m(tile_max)},
+      {TILE_FLUX, (const void *)k_flux_m<true, true>, lds_bytes_flux_m(tile_max)}};
+  unsigned ok = tiles;
+  for (const K &k : ks) {
+    if (!(tiles & k.bit)) continue;
+    if (k.bytes > lds_limit ||
+        hipFuncSetAttribute(k.f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)k.bytes) !=
+            hipSuccess)
+      ok &= ~k.bit;
+  }
+  return ok;
+}
+
+#define TILE_ARGS_M c.conn_off, c.conn, c.conn16, c.tile_off, c.tile_idx, c.tile_max
+
+void launch_q_variables_m(const DevCloud &c, const double *prim, float *q32, hipStream_t s) {
+  hipLaunchKernelGGL(k_q_variables_m, dim3(blocks_for(c.n, 256)), dim3(256), 0, s, c.n, prim,
+                     reinterpret_cast<float4 *>(q32));
+}
+
+void launch_q_derivatives_m(const DevCloud &c, const MixedState &m, float *dq, double power,
+                            hipStream_t s) {
+  dim3 g(blocks_for(c.n, c.tile_block)), b(c.tile_block);
+  const bool t = (c.tiles & TILE_QDERIV) != 0;
+  const size_t L = t ? lds_bytes_qderiv_m(c.tile_max) : 0;
+#define QD_ARGS                                                                           \
+  c.n, TILE_ARGS_M, reinterpret_cast<const double2 *>(m.xy),                              \
+      reinterpret_cast<const float4 *>(m.q32), reinterpret_cast<float4 *>(dq),            \
+      reinterpret_cast<float4 *>(m.qm32), power
+  if (t && power == 0.0)
+    hipLaunchKernelGGL((k_q_derivatives_m<true, false>), g, b, L, s, QD_ARGS);
+  else if (t)
+    hipLaunchKernelGGL((k_q_derivatives_m<true, true>), g, b, L, s, QD_ARGS);
+  else if (power == 0.0)
+    hipLaunchKernelGGL((k_q_derivatives_m<false, false>), g, b, 0, s, QD_ARGS);
+  else
+    hipLaunchKernelGGL((k_q_derivatives_m<false, true>), g, b, 0, s, QD_ARGS);
+#undef QD_ARGS
+}
+
+void launch_q_inner_m(const DevCloud &c, const MixedState &m, const float *src, float *dst,
+                      double power, hipStream_t s) {
+  dim3 g(blocks_for(c.n, c.tile_block)), b(c.tile_block);
+  const bool t = (c.tiles & TILE_QINNER) != 0;
+  const size_t L = t ? lds_bytes_qinner_m(c.tile_max) : 0;
+#define QI_ARGS                                                                           \
+  c.n, TILE_ARGS_M, reinterpret_cast<const double2 *>(m.xy),                              \
+      reinterpret_cast<const float4 *>(m.q32), reinterpret_cast<const float4 *>(src),     \
+      reinterpret_cast<float4 *>(dst), power
+  if (t && power == 0.0)
+    hipLaunchKernelGGL((k_q_inner_m<true, false>), g, b, L, s, QI_ARGS);
+  else if (t)
+    hipLaunchKernelGGL((k_q_inner_m<true, true>), g, b, L, s, QI_ARGS);
+  else if (power == 0.0)
+    hipLaunchKernelGGL((k_q_inner_m<false, false>), g, b, 0, s, QI_ARGS);
+  else
+    hipLaunchKernelGGL((k_q_inner_m<false, true>), g, b, 0, s, QI_ARGS);
+#undef QI_ARGS
+}
+
+void launch_flux_m(const DevCloud &c, const MixedState &m, const float *dq, const double *delta,
+                   double power, double *flux_res, hipStream_t s) {
+  dim3 g(blocks_for(c.n, c.tile_block)), b(c.tile_block);
+  const bool t = (c.tiles & TILE_FLUX) != 0;
+  const size_t L = t ? lds_bytes_flux_m(c.tile_max) : 0;
+#define FLUX_ARGS                                                                          \
+  c.n, TILE_ARGS_M, reinterpret_cast<const double2 *>(m.xy),                               \
+      reinterpret_cast<const float4 *>(m.q32), reinterpret_cast<const float4 *>(dq),       \
+      reinterpret_cast<const float4 *>(m.qm32), m.eps32, c.coef, c.nx, c.ny, c.flag, delta, \
+      power, flux_res
+  if (t && power == 0.0)
+    hipLaunchKernelGGL((k_flux_m<true, false>), g, b, L, s, FLUX_ARGS);
+  else if (t)
+    hipLaunchKernelGGL((k_flux_m<true, true>), g, b, L, s, FLUX_ARGS);
+  else if (power == 0.0)
+    hipLaunchKernelGGL((k_flux_m<false, false>), g, b, 0, s, FLUX_ARGS);
+  else
+    hipLaunchKernelGGL((k_flux_m<false, true>), g, b, 0, s, FLUX_ARGS);
+#undef FLUX_ARGS
+}
+
+void launch_kat_fluxes_m(int n, const double *in6, double *out, hipStream_t s) {
+  hipLaunchKernelGGL(k_kat_fluxes_m, dim3(blocks_for(n, 128)), dim3(128), 0, s, n, in6, out);
+}
+
+}  // namespace lskum

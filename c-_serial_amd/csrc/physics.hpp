@@ -157,4 +157,19 @@ __device__ __forceinline__ T venkat_phi(T qtilde, T q, T qmax, T qmin, T eps2) {
   return temp < T(1) ? temp : T(1);
 }
 
+// The same limiter from the increments themselves: del_neg = q~ - q =
+// -0.5*(dq . dx) and dmax = qmax - q, dmin = qmin - q.  In FP32, forming q~
+// first and subtracting q again would lose the increment's low bits to the
+// size of q; this form keeps them.
+template <typename T>
+__device__ __forceinline__ T venkat_phi_delta(T del_neg, T dmax, T dmin, T eps2) {
+  if (fabs(del_neg) <= T(10e-6)) return T(1.0);
+  const T del_pos = del_neg > T(0) ? dmax : dmin;
+  const T dp2 = del_pos * del_pos;
+  const T num = (dp2 + eps2) * del_neg + T(2.0) * del_neg * del_neg * del_pos;
+  const T den = (dp2 + T(2.0) * del_neg * del_neg + del_neg * del_pos + eps2) * del_neg;
+  const T temp = num / den;
+  return temp < T(1) ? temp : T(1);
+}
+
 }  // namespace lskum

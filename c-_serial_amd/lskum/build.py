@@ -18,16 +18,17 @@ INCLUDE = os.path.join(REPO, "include")
 LIB_PATH = os.path.join(PKG_DIR, "liblskum.so")
 BUILD_DIR = os.path.join(ROOT, "_build")
 
-HIP_SOURCES = ["kernels.hip", "api.hip"]
+HIP_SOURCES = ["kernels.hip", "kernels_mixed.hip", "api.hip"]
 CXX_SOURCES = ["setup.cpp", "cloud.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("LSKUM_ARCH", "gfx950")
 
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
-    # device FP64 may fuse a*b+c into FMAs (rounding-level differences from
-    # the reference, inside the parity bar); nothing else is relaxed
-    "-ffp-contract=fast", "-fno-fast-math",
+    # device code may fuse a*b+c into FMAs (rounding-level differences from
+    # the reference, inside the parity bar) except where a pragma forbids
+    # it; nothing else is relaxed
+    "-ffp-contract=fast-honor-pragmas", "-fno-fast-math",
     # packed FP32 forms are kept out of all device code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-I" + INCLUDE, "-I" + CSRC,

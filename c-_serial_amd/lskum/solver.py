@@ -203,12 +203,13 @@ class Solver:
         return ms.value, cnt.value
 
 
-def kat_fluxes(rows6):
-    """Device flux functions on rows (nx, ny, u1, u2, rho, pr) -> [n, 8, 4]."""
+def kat_fluxes(rows6, mixed=False):
+    """Device flux functions on rows (nx, ny, u1, u2, rho, pr) -> [n, 8, 4];
+    mixed: the FP32 functions of the mixed form, on FP32-rounded inputs."""
     rows6 = np.ascontiguousarray(rows6, dtype=np.float64)
     n = rows6.shape[0]
     out = np.zeros((n, 8, 4))
-    L.check(L.lib().lskum_kat_fluxes(n, L.dptr(rows6), L.dptr(out), 0))
+    L.check(L.lib().lskum_kat_fluxes(n, L.dptr(rows6), L.dptr(out), 1 if mixed else 0))
     return out
 
 

@@ -117,7 +117,12 @@ void lskum_cloud_free(lskum_cloud_h *c);
 
 /* ------------------------------------------------------------ parameters */
 #define LSKUM_FP64 0  /* every operation in FP64, in the reference's order  */
-#define LSKUM_MIXED 1 /* FP32 gathers/fluxes/limiter, FP64 solves/updates  */
+/* Mixed: neighbour q, dq, qmax/qmin and eps^2 stored and gathered in FP32;
+ * limiter, q~ -> primitive and split/quadrant fluxes in FP32; coordinates,
+ * least-squares sums/determinants/solves, flux accumulation, the state,
+ * SSPRK update, local time step and all reductions in FP64.  q, dq and qm
+ * read back through lskum_get_array are the FP32 values widened. */
+#define LSKUM_MIXED 1
 
 #define LSKUM_REORDER_NONE 0
 #define LSKUM_REORDER_RCM 1 /* reverse Cuthill-McKee on the neighbour graph */
@@ -251,7 +256,8 @@ int lskum_kernel_time(lskum_ctx *ctx, int32_t kernel, double *avg_ms,
  * quadrant_fluxes_mod.h:6,57,108,161) on n input rows
  * (nx, ny, u1, u2, rho, pr) -> out[n][8][4], order Gxp, Gxn, Gyp, Gyn, GxI,
  * GxII, GxIII, GxIV; and qtilde_to_primitive (q_variables_mod.h:230) on
- * n rows of q~[4] -> prim_out[n][4] as (u1, u2, rho, pr).
+ * n rows of q~[4] -> prim_out[n][4] as (u1, u2, rho, pr).  mixed != 0
+ * evaluates the FP32 flux functions of LSKUM_MIXED on FP32-rounded inputs.
  */
 int lskum_kat_fluxes(int32_t n, const double *in6, double *out, int32_t mixed);
 int lskum_kat_qtilde(int32_t n, const double *qt4, double *prim_out);
