@@ -95,10 +95,10 @@ def cpu_baseline(args):
     """The reference's own serial fpi_solver (oracle/_ref/ref_driver, built
     from the reference sources) on a bounded sample of the workload: a
     100,400-point cloud of the same generator family (wall 400, 100 x 1,000
-    O-grid, k = 24) with the same numerics, 3 iterations, 1 core."""
+    O-grid, k = 24) with the same numerics, 8 iterations (about 20 s), 1 core."""
     sample = dict(naca="0012", n_wall=400, n_radial=100, n_circ=1000, stretch=0.0, k=24, seed=2)
     cloud = lskum.Cloud.generate(**sample)
-    iters = 3
+    iters = 8
     driver = os.path.join(REPO, "oracle", "_ref", "ref_driver")
     desc = (f"{cloud.n}-point NACA0012 cloud of the same generator family (wall 400, "
             f"100 x 1,000, k 24), same numerics, {iters} iterations, 1 thread")
