@@ -11,7 +11,7 @@ same start and states its Cl/Cd/residual differences from the FP64 run.
 Prints one JSON line.
 
   python bench.py [--gpus 1] [--steps 50] [--warmup 5] [--no-cpu-baseline]
-                  [--no-mixed]
+                  [--no-mixed] [--workload batch]
 """
 import argparse
 import json
@@ -135,6 +135,64 @@ def load_traffic(name):
     return None
 
 
+def run_batch(args):
+    """The many-case workload: 64 flow conditions over 8 NACA 4-digit clouds
+    of 50k-800k points (k = 20, inner passes 0, CFL 0.2, K 5), all cases
+    advanced together, each iteration one captured graph replay per 10
+    iterations, with the layout chosen by lskum_autotune on one case per
+    cloud.  value = sum of case points x iterations / s.  Two cases are
+    rerun alone and must match their batch results bit for bit; cases whose
+    residual turned non-finite are listed (they run on as NaN without
+    touching the others)."""
+    from lskum.batch import Batch, autotune, batch_workload
+    num = dict(cfl=0.2, vl_const=5.0, power=0.0, inner_iterations=0)
+    t = time.perf_counter()
+    clouds, cases = batch_workload()
+    log(f"batch: {len(clouds)} clouds {[c.n for c in clouds]}, {len(cases)} cases, "
+        f"{time.perf_counter() - t:.1f}s")
+    reps = [next(j for j, c in enumerate(cases) if c[0] == ci) for ci in range(len(clouds))]
+    best, trials = autotune(clouds, [cases[j] for j in reps], iters=5, **num)
+    log(f"autotune: best {best}")
+    b = Batch(clouds, cases, tile_block=best["tile_block"], lds_tiles=best["lds_tiles"],
+              graph_iters=10, **num)
+    if args.warmup:
+        b.q_lskum(args.warmup)
+    t0 = time.perf_counter()
+    rn, _ = b.q_lskum(args.steps)
+    elapsed = time.perf_counter() - t0
+    forces = b.compute_cl_cd_cm()
+    prims = b.prim()
+    failed = {int(j): int(it) for j, it in enumerate(b.failed_at()) if it}
+    pts = int(b.points.sum())
+    b.close()
+    ok = np.isfinite(forces).all(axis=1)
+    same = {}
+    for j in (0, len(cases) - 1):
+        ci, mach, aoa = cases[j]
+        s = Solver(clouds[ci], mach=mach, aoa=aoa, tile_block=best["tile_block"],
+                   lds_tiles=best["lds_tiles"], **num)
+        s.q_lskum(args.warmup)
+        srn, _ = s.q_lskum(args.steps)
+        same[j] = bool(np.array_equal(srn, rn[j]) and
+                       np.array_equal(np.array(s.compute_cl_cd_cm()), forces[j]) and
+                       np.array_equal(s.prim, prims[j]))
+        s.close()
+    return {
+        "metric": METRIC, "value": pts * args.steps / elapsed, "unit": "point-iterations/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "batch_64cases_8clouds_50k-800k_k20", "cases": len(cases),
+                   "clouds": [c.n for c in clouds], "case_points": pts, "graph_iters": 10,
+                   "tile_block": best["tile_block"], "lds_tile_mask": best["lds_tiles"]},
+        "autotune": {"best": best, "trials": trials},
+        "cases_bit_identical_to_single_runs": same,
+        "cases_failed": failed,
+        "cl_cd": {"min_cl": float(forces[ok, 0].min()), "max_cl": float(forces[ok, 0].max()),
+                  "min_cd": float(forces[ok, 1].min()), "max_cd": float(forces[ok, 1].max())},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -155,6 +213,9 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "batch":
+        print(json.dumps(run_batch(args)), flush=True)
+        return
     t = time.perf_counter()
     cloud = lskum.workload(args.workload)
     log(f"cloud {args.workload}: {cloud.n} points, {cloud.conn.size} neighbour entries, "

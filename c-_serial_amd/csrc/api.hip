@@ -4,6 +4,15 @@
 // q_lskum_mod.h:28-37): each stage is one stream-ordered kernel launch on the
 // context's stream; only the per-iteration residual sums come back, once,
 // at the end of lskum_iterate.
+//
+// A context runs n_cases independent cases (one for lskum_create, many for
+// lskum_create_batch).  Cases are concatenated in device memory, each padded
+// to whole 256-point blocks when there are several, so that every stage is
+// one launch over all cases: the stencil kernels need no notion of a case
+// (a case's neighbour lists only reach its own points), the state update
+// reads its case's flow constants per block, and the residual and force
+// reductions run one block per case.  Padding points have no neighbours and
+// stay at their initial state.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -20,17 +29,45 @@
 
 using namespace lskum;
 
+namespace {
+
+// One cloud's set-up on the host, in its device order.
+struct CloudLayout {
+  int32_t n = 0;
+  std::vector<int32_t> perm, inv;           // perm[dev] = caller, inv[caller] = dev
+  std::vector<int32_t> conn_off_h, conn_h;  // caller order, for sub-stencil queries
+  std::vector<uint8_t> mask_h;
+  std::vector<double> nx_h, ny_h;
+  // device order
+  std::vector<double> x, y, eps2, nx, ny, coef;
+  std::vector<uint8_t> flag;
+  std::vector<int32_t> off;
+  std::vector<uint32_t> conn;               // local index | mask << kMaskShift
+  std::vector<int32_t> fm, fl, fr, fs;      // force panels (device indices)
+};
+
+struct CaseInfo {
+  int cloud = 0;
+  int32_t p0 = 0;  // first device point
+  int32_t n = 0;   // points (the cloud's)
+  int32_t np = 0;  // points with padding
+  int32_t s0 = 0, s1 = 0;  // force panel range
+  double mach = 0.0, aoa = 0.0, theta = 0.0, q_inf[4] = {0, 0, 0, 0};
+  double res_old = 0.0;
+  int32_t max_res_point = 0;
+  int32_t failed_at = 0;  // first iteration with a non-finite residual (0: none)
+};
+
+}  // namespace
+
 struct lskum_ctx {
-  int n = 0;
+  int n = 0;  // device points, all cases with padding
   lskum_params prm{};
   hipStream_t stream = nullptr;
   std::string err;
-  // caller <-> device order
-  std::vector<int32_t> perm, inv;  // perm[dev] = caller, inv[caller] = dev
-  // host copy of the caller-order cloud (for sub-stencil queries)
-  std::vector<int32_t> conn_off_h, conn_h;
-  std::vector<uint8_t> mask_h;
-  std::vector<double> nx_h, ny_h;
+  std::vector<CloudLayout> clouds;
+  std::vector<CaseInfo> cases;
+  int n_cases() const { return (int)cases.size(); }
   // device state: base records, derivative buffers (dqb[cur] holds the
   // current derivatives), qmax/qmin, primitive state
   double *base = nullptr, *dqb[2] = {nullptr, nullptr}, *qm = nullptr;
@@ -51,25 +88,35 @@ struct lskum_ctx {
   int tile_max = 0;
   unsigned tiles = 0;
   int tile_block = kTileBlockDefault;
+  // cases: flow constants, 256-block -> case map, partial and force ranges
+  FlowConsts *fcs = nullptr;
+  uint16_t *block_case = nullptr;  // NULL for one case
+  int32_t *seg = nullptr;          // NULL for one case
+  int32_t *soff = nullptr;
+  double *qdyn = nullptr;
   Partials part{};
-  double *red = nullptr;      // [3] sum, max, argmax of the last state update
-  double *hist = nullptr;     // per-iteration sum_res_sqr of lskum_iterate
+  double *red = nullptr;      // [n_cases][3] sum, max, argmax of the last state update
+  double *hist = nullptr;     // [n_cases][hist_cap] per-iteration sum_res_sqr
   int hist_cap = 0;
-  double *red_h = nullptr;    // pinned host mirror of red / hist
+  int32_t *counter = nullptr; // iteration slot of hist within an lskum_iterate call
+  double *red_h = nullptr;    // pinned host mirror of red / forces
   int32_t *f_m = nullptr, *f_l = nullptr, *f_r = nullptr, *f_s = nullptr;
-  int n_shape = 0;
-  double *forces = nullptr;   // [3] H, V, pitch moment
-  // case
-  double theta = 0.0, q_inf[4] = {0, 0, 0, 0};
+  double *forces = nullptr;   // [n_cases][3] H, V, pitch moment
   double rho_inf = 1.0, pr_inf = 1.0 / 1.4;
   // iteration bookkeeping (q_lskum_mod.h, fpi_solver_mod.h:138-147)
   int it = 0;
-  double res_old = 0.0;
   int restart = 0;
-  int32_t max_res_point = 0;
+  // q and the func_delta speed hold for the current state (a stage-4 update
+  // refreshes both; setting the state invalidates them)
+  bool q_fresh = false, speed_fresh = false;
+  // graph capture of whole iterations
+  int graph_iters = 0;
+  hipGraphExec_t gexec = nullptr;
+  int gexec_iters = 0;
+  double *gexec_hist = nullptr;
+  bool capturing = false;
   // timing
   bool timing = false;
-  hipEvent_t ev[2] = {nullptr, nullptr};
   double k_ms[6] = {0, 0, 0, 0, 0, 0};
   int k_cnt[6] = {0, 0, 0, 0, 0, 0};
   std::vector<hipEvent_t> ev_pool;
@@ -101,16 +148,6 @@ struct lskum_ctx {
     m.eps32 = eps32;
     return m;
   }
-  FlowConsts fc() const {
-    FlowConsts f;
-    f.cfl = prm.cfl;
-    f.power = prm.power;
-    f.u1_inf = q_inf[1];
-    f.u2_inf = q_inf[2];
-    f.rho_inf = rho_inf;
-    f.pr_inf = pr_inf;
-    return f;
-  }
 };
 
 namespace {
@@ -132,13 +169,22 @@ hipError_t dalloc(T **p, size_t count) {
   return hipMalloc((void **)p, sizeof(T) * (count ? count : 1));
 }
 
+void drop_graph(lskum_ctx *c) {
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  c->gexec = nullptr;
+  c->gexec_iters = 0;
+  c->gexec_hist = nullptr;
+}
+
 void release(lskum_ctx *c) {
   if (!c) return;
-  void *ptrs[] = {c->coef, c->base, c->dqb[0], c->dqb[1], c->qm, c->conn16, c->tile_off, c->tile_idx, c->prim, c->prim_old, c->flux_res, c->delta, c->nx,
-                  c->ny, c->flag, c->conn_off, c->perm_d, c->conn, c->part.sum,
-                  c->part.max, c->part.arg, c->red, c->hist, c->f_m, c->f_l, c->f_r,
-                  c->f_s, c->forces, c->xy, c->q32, c->dq32[0], c->dq32[1], c->qm32,
-                  c->eps32};
+  drop_graph(c);
+  void *ptrs[] = {c->coef, c->base, c->dqb[0], c->dqb[1], c->qm, c->conn16, c->tile_off,
+                  c->tile_idx, c->prim, c->prim_old, c->flux_res, c->delta, c->nx, c->ny,
+                  c->flag, c->conn_off, c->perm_d, c->conn, c->part.sum, c->part.max,
+                  c->part.arg, c->red, c->hist, c->f_m, c->f_l, c->f_r, c->f_s, c->forces,
+                  c->xy, c->q32, c->dq32[0], c->dq32[1], c->qm32, c->eps32, c->fcs,
+                  c->block_case, c->seg, c->soff, c->qdyn, c->counter};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (c->red_h) (void)hipHostFree(c->red_h);
@@ -146,13 +192,14 @@ void release(lskum_ctx *c) {
   if (c->stream) (void)hipStreamDestroy(c->stream);
 }
 
-// Kernel timing: an event pair around a launch when enabled.
+// Kernel timing: an event pair around a launch when enabled (never inside a
+// graph capture).
 struct KTimer {
   lskum_ctx *c;
   int id;
   int idx = -1;
   KTimer(lskum_ctx *ctx, int kid) : c(ctx), id(kid) {
-    if (!c->timing) return;
+    if (!c->timing || c->capturing) return;
     idx = (int)c->ev_pending.size() * 2;
     while ((int)c->ev_pool.size() < idx + 2) {
       hipEvent_t e;
@@ -189,17 +236,19 @@ int check_launch(lskum_ctx *c, const char *what) {
 
 // prim_old <- prim
 int do_snapshot(lskum_ctx *c) {
-  HIP_TRY(c, hipMemcpyAsync(c->prim_old, c->prim, sizeof(double) * 4 * c->n,
-                            hipMemcpyDeviceToDevice, c->stream));
-  return LSKUM_OK;
+  launch_copy(c->prim, c->prim_old, (size_t)4 * c->n, c->stream);
+  return check_launch(c, "snapshot");
 }
 
-// with_speed: first refresh the per-point speed from prim (not needed right
-// after a stage-4 state update, which wrote it)
-int do_func_delta(lskum_ctx *c, bool with_speed) {
+int do_speed(lskum_ctx *c) {
+  launch_speed(c->dc(), c->prim, c->base, c->stream);  // both forms: FP64
+  c->speed_fresh = true;
+  return check_launch(c, "speed");
+}
+
+int do_func_delta(lskum_ctx *c) {
   {
     KTimer t(c, LSKUM_K_DELTA);
-    if (with_speed) launch_speed(c->dc(), c->prim, c->base, c->stream);  // both forms: FP64
     launch_func_delta(c->dc(), c->base, c->prm.cfl, c->delta, c->stream);
   }
   return check_launch(c, "func_delta");
@@ -213,10 +262,11 @@ int do_q_variables(lskum_ctx *c) {
     else
       launch_q_variables(c->dc(), c->prim, c->base, c->stream);
   }
+  c->q_fresh = true;
   return check_launch(c, "q_variables");
 }
 
-// derivatives into the buffer from which n_later Jacobi passes end in dqb[0]
+// derivatives into the buffer from which n_later Jacobi passes end in buffer 0
 int do_q_derivatives(lskum_ctx *c, int n_later) {
   c->cur = n_later % 2;
   {
@@ -229,7 +279,7 @@ int do_q_derivatives(lskum_ctx *c, int n_later) {
   return check_launch(c, "q_derivatives");
 }
 
-// n Jacobi passes, each reading dqb[cur] and writing the other buffer
+// n Jacobi passes, each reading buffer cur and writing the other one
 int do_q_inner(lskum_ctx *c, int n_inner) {
   for (int p = 0; p < n_inner; p++) {
     KTimer t(c, LSKUM_K_QINNER);
@@ -257,26 +307,90 @@ int do_flux(lskum_ctx *c) {
   return check_launch(c, "flux_residual");
 }
 
-// the state update also refreshes q (and, after the last stage, the speed
-// func_delta reads), so the next stage needs no separate q_variables pass
-int do_state_update(lskum_ctx *c, int rk, double *hist_slot) {
+// The state update also refreshes q (and, after the last stage, the speed
+// func_delta reads), so the next stage needs no separate q_variables pass.
+// with_hist: record each case's sum in hist at the slot the counter holds.
+int do_state_update(lskum_ctx *c, int rk, bool with_hist) {
   {
     KTimer t(c, LSKUM_K_STATE);
-    launch_state_update(c->dc(), rk, c->prim, c->prim_old, c->flux_res, c->fc(), c->part,
-                        c->base, c->mixed ? c->q32 : nullptr, rk == 4, c->stream);
-    launch_reduce(c->part, c->red, hist_slot, c->stream);
+    launch_state_update(c->dc(), rk, c->prim, c->prim_old, c->flux_res, c->fcs, c->block_case,
+                        c->part, c->base, c->mixed ? c->q32 : nullptr, rk == 4, c->stream);
+    launch_reduce(c->part, c->n_cases(), c->seg, c->red, with_hist ? c->hist : nullptr,
+                  c->hist_cap, 0, c->counter, c->stream);
   }
+  c->q_fresh = true;
+  c->speed_fresh = rk == 4;
   return check_launch(c, "state_update");
 }
 
-void permute_in(const lskum_ctx *c, const double *src, double *dst, int ncomp) {
-  for (int r = 0; r < ncomp; r++)
-    for (int d = 0; d < c->n; d++) dst[(size_t)r * c->n + d] = src[(size_t)r * c->n + c->perm[d]];
+// One fpi_solver iteration (fpi_solver_mod.h:14-150) with q and the speed
+// already fresh; advances the hist slot counter.
+int run_iteration(lskum_ctx *c) {
+  const int inner = c->prm.inner_iterations;
+  int rc = do_snapshot(c);
+  if (!rc) rc = do_func_delta(c);
+  for (int rk = 1; rk <= 4 && !rc; rk++) {
+    rc = do_q_derivatives(c, inner);
+    if (!rc) rc = do_q_inner(c, inner);
+    if (!rc) rc = do_flux(c);
+    if (!rc) rc = do_state_update(c, rk, rk == 4);
+  }
+  if (!rc) {
+    launch_tick(c->counter, c->stream);
+    rc = check_launch(c, "tick");
+  }
+  return rc;
 }
 
+// Captures gexec_iters = K whole iterations as one graph.
+int capture_graph(lskum_ctx *c, int K) {
+  drop_graph(c);
+  HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  c->capturing = true;
+  int rc = LSKUM_OK;
+  for (int k = 0; k < K && !rc; k++) rc = run_iteration(c);
+  c->capturing = false;
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(c->stream, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return set_err(c, LSKUM_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  e = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) {
+    c->gexec = nullptr;
+    return set_err(c, LSKUM_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  }
+  c->gexec_iters = K;
+  c->gexec_hist = c->hist;
+  return LSKUM_OK;
+}
+
+// per-point arrays of all cases <-> the caller's layout: case after case,
+// each [ncomp][n_case] in the case's caller order
 void permute_out(const lskum_ctx *c, const double *src, double *dst, int ncomp) {
-  for (int r = 0; r < ncomp; r++)
-    for (int d = 0; d < c->n; d++) dst[(size_t)r * c->n + c->perm[d]] = src[(size_t)r * c->n + d];
+  size_t o = 0;
+  for (const CaseInfo &cs : c->cases) {
+    const CloudLayout &L = c->clouds[cs.cloud];
+    for (int r = 0; r < ncomp; r++)
+      for (int32_t d = 0; d < cs.n; d++)
+        dst[o + (size_t)r * cs.n + L.perm[d]] = src[(size_t)r * c->n + cs.p0 + d];
+    o += (size_t)ncomp * cs.n;
+  }
+}
+
+// the inverse; padding points keep what dst held
+void permute_in(const lskum_ctx *c, const double *src, double *dst, int ncomp) {
+  size_t o = 0;
+  for (const CaseInfo &cs : c->cases) {
+    const CloudLayout &L = c->clouds[cs.cloud];
+    for (int r = 0; r < ncomp; r++)
+      for (int32_t d = 0; d < cs.n; d++)
+        dst[(size_t)r * c->n + cs.p0 + d] = src[o + (size_t)r * cs.n + L.perm[d]];
+    o += (size_t)ncomp * cs.n;
+  }
 }
 
 // fetch ncomp consecutive slots of a per-point record array: out[r][dev]
@@ -313,114 +427,92 @@ int fetch_plain(lskum_ctx *c, const double *dev, int ncomp, std::vector<double> 
   return LSKUM_OK;
 }
 
-}  // namespace
-
-extern "C" {
-
-const char *lskum_last_error(const lskum_ctx *ctx) {
-  return ctx ? ctx->err.c_str() : "null context";
+bool params_ok(const lskum_params &p) {
+  return p.mach > 0.0 && p.cfl > 0.0 && p.vl_const >= 0.0 && p.inner_iterations >= 0 &&
+         (p.precision == LSKUM_FP64 || p.precision == LSKUM_MIXED) &&
+         (p.reorder >= LSKUM_REORDER_NONE && p.reorder <= LSKUM_REORDER_HILBERT) &&
+         (p.lds_tiles == LSKUM_TILES_AUTO || (p.lds_tiles >= 0 && p.lds_tiles <= 15)) &&
+         (p.tile_block == 0 || p.tile_block == 64 || p.tile_block == 128 ||
+          p.tile_block == 256);
 }
 
-int lskum_create(const lskum_cloud *cloud, const lskum_params *params, lskum_ctx **out) {
-  if (!out) return LSKUM_EINVAL;
-  *out = nullptr;
-  if (!cloud || !params) return LSKUM_EINVAL;
-  lskum_ctx *c = new (std::nothrow) lskum_ctx;
-  if (!c) return LSKUM_ENOMEM;
-  std::string err;
-  int rc = check_cloud(*cloud, &err);
-  const lskum_params &p = *params;
-  if (rc == LSKUM_OK && !(p.mach > 0.0 && p.cfl > 0.0 && p.vl_const >= 0.0 &&
-                          p.inner_iterations >= 0 &&
-                          (p.precision == LSKUM_FP64 || p.precision == LSKUM_MIXED) &&
-                          (p.reorder >= LSKUM_REORDER_NONE && p.reorder <= LSKUM_REORDER_HILBERT) &&
-                          (p.lds_tiles == LSKUM_TILES_AUTO ||
-                           (p.lds_tiles >= 0 && p.lds_tiles <= 15)) &&
-                          (p.tile_block == 0 || p.tile_block == 64 || p.tile_block == 128 ||
-                           p.tile_block == 256))) {
-    rc = LSKUM_EINVAL;
-    err = "invalid parameters";
-  }
-  if (rc != LSKUM_OK) {
-    // keep the message reachable: return the context in its failed state
-    c->err = err;
-    *out = c;
-    return rc;
-  }
-  c->prm = p;
-  c->mixed = p.precision == LSKUM_MIXED;
-  c->tile_block = p.tile_block ? p.tile_block : kTileBlockDefault;
-  const int32_t n = cloud->n_points;
-  c->n = n;
-  // set-up on the host: normals and sub-stencils in the caller's order
-  compute_normals(*cloud, c->nx_h, c->ny_h);
-  generate_connectivity(*cloud, c->nx_h, c->ny_h, c->mask_h);
+// Host set-up of one cloud: normals and sub-stencils in the caller's order,
+// stencil validation, device order, device-order images, least-squares
+// coefficients, force panels.
+int build_layout(const lskum_cloud &cloud, const lskum_params &p, CloudLayout &L,
+                 std::string &err) {
+  const int32_t n = cloud.n_points;
+  L.n = n;
+  compute_normals(cloud, L.nx_h, L.ny_h);
+  generate_connectivity(cloud, L.nx_h, L.ny_h, L.mask_h);
   for (int32_t i = 0; i < n; i++) {
-    double r = substencil_min_rel_det(*cloud, c->nx_h, c->ny_h, c->mask_h, i);
+    double r = substencil_min_rel_det(cloud, L.nx_h, L.ny_h, L.mask_h, i);
     if (!(r > 0.0)) {
       char m[160];
       std::snprintf(m, sizeof m,
                     "degenerate sub-stencil at point %d (flag_1 %d): empty or collinear", i,
-                    cloud->flag_1[i]);
-      c->err = m;
-      *out = c;
+                    cloud.flag_1[i]);
+      err = m;
       return LSKUM_ESTENCIL;
     }
   }
-  c->conn_off_h.assign(cloud->conn_off, cloud->conn_off + n + 1);
-  c->conn_h.assign(cloud->conn, cloud->conn + cloud->conn_off[n]);
+  L.conn_off_h.assign(cloud.conn_off, cloud.conn_off + n + 1);
+  L.conn_h.assign(cloud.conn, cloud.conn + cloud.conn_off[n]);
   // device order: Hilbert (compact 256-point blocks whose neighbour sets fit
   // an LDS tile), or RCM / the caller's order grouped by point type
-  std::vector<int32_t> base;
+  std::vector<int32_t> order;
   if (p.reorder == LSKUM_REORDER_HILBERT) {
-    hilbert_order(*cloud, c->perm);
+    hilbert_order(cloud, L.perm);
   } else {
     if (p.reorder == LSKUM_REORDER_RCM) {
-      rcm_order(*cloud, base);
+      rcm_order(cloud, order);
     } else {
-      base.resize(n);
-      for (int32_t i = 0; i < n; i++) base[i] = i;
+      order.resize(n);
+      for (int32_t i = 0; i < n; i++) order[i] = i;
     }
-    c->perm.clear();
-    c->perm.reserve(n);
+    L.perm.clear();
+    L.perm.reserve(n);
     for (int t = 0; t < 3; t++)
-      for (int32_t v : base)
-        if (cloud->flag_1[v] == t) c->perm.push_back(v);
+      for (int32_t v : order)
+        if (cloud.flag_1[v] == t) L.perm.push_back(v);
   }
-  c->inv.assign(n, 0);
-  for (int32_t d = 0; d < n; d++) c->inv[c->perm[d]] = d;
-  // host images of the device arrays
-  std::vector<double> base_h((size_t)B_STRIDE * n, 0.0), nxd(n), nyd(n);
-  std::vector<uint8_t> flag_h(n);
-  std::vector<int32_t> off_h(n + 1, 0);
-  std::vector<uint32_t> conn_d(cloud->conn_off[n]);
+  L.inv.assign(n, 0);
+  for (int32_t d = 0; d < n; d++) L.inv[L.perm[d]] = d;
+  L.x.resize(n);
+  L.y.resize(n);
+  L.eps2.resize(n);
+  L.nx.resize(n);
+  L.ny.resize(n);
+  L.flag.resize(n);
+  L.off.assign(n + 1, 0);
+  L.conn.resize(cloud.conn_off[n]);
   for (int32_t d = 0; d < n; d++) {
-    const int32_t v = c->perm[d];
-    base_h[(size_t)B_STRIDE * d + B_X] = cloud->x[v];
-    base_h[(size_t)B_STRIDE * d + B_Y] = cloud->y[v];
+    const int32_t v = L.perm[d];
+    L.x[d] = cloud.x[v];
+    L.y[d] = cloud.y[v];
     // limiters_mod.h:82-83: epsi = pow(VL_CONST*min_dist, 3), used squared
-    const double epsi = std::pow(p.vl_const * cloud->min_dist[v], 3);
-    base_h[(size_t)B_STRIDE * d + B_EPS2] = epsi * epsi;
-    nxd[d] = c->nx_h[v];
-    nyd[d] = c->ny_h[v];
-    flag_h[d] = (uint8_t)cloud->flag_1[v];
-    const int32_t b = cloud->conn_off[v], e = cloud->conn_off[v + 1];
-    off_h[d + 1] = off_h[d] + (e - b);
+    const double epsi = std::pow(p.vl_const * cloud.min_dist[v], 3);
+    L.eps2[d] = epsi * epsi;
+    L.nx[d] = L.nx_h[v];
+    L.ny[d] = L.ny_h[v];
+    L.flag[d] = (uint8_t)cloud.flag_1[v];
+    const int32_t b = cloud.conn_off[v], e = cloud.conn_off[v + 1];
+    L.off[d + 1] = L.off[d] + (e - b);
     for (int32_t t = b; t < e; t++)
-      conn_d[off_h[d] + (t - b)] =
-          (uint32_t)c->inv[cloud->conn[t]] | ((uint32_t)c->mask_h[t] << kMaskShift);
+      L.conn[L.off[d] + (t - b)] =
+          (uint32_t)L.inv[cloud.conn[t]] | ((uint32_t)L.mask_h[t] << kMaskShift);
   }
   // least-squares coefficients of each sub-stencil (geometry only): the
   // sums of interior_fluxes_mod.h:51-61 over the members in conn order, then
   // (syy, -sxy)/det for the x-form solve and (sxx, -sxy)/det for the y-form
-  std::vector<double> coef_h((size_t)8 * n, 0.0);
+  L.coef.assign((size_t)8 * n, 0.0);
   for (int32_t d = 0; d < n; d++) {
-    const int32_t v = c->perm[d];
-    const double nxv = c->nx_h[v], nyv = c->ny_h[v], tx = nyv, ty = -nxv;
+    const int32_t v = L.perm[d];
+    const double nxv = L.nx_h[v], nyv = L.ny_h[v], tx = nyv, ty = -nxv;
     double sxx[4] = {0, 0, 0, 0}, syy[4] = {0, 0, 0, 0}, sxy[4] = {0, 0, 0, 0};
-    for (int32_t t = cloud->conn_off[v]; t < cloud->conn_off[v + 1]; t++) {
-      const int32_t k = cloud->conn[t];
-      const double delx = cloud->x[k] - cloud->x[v], dely = cloud->y[k] - cloud->y[v];
+    for (int32_t t = cloud.conn_off[v]; t < cloud.conn_off[v + 1]; t++) {
+      const int32_t k = cloud.conn[t];
+      const double delx = cloud.x[k] - cloud.x[v], dely = cloud.y[k] - cloud.y[v];
       const double dels = delx * tx + dely * ty, deln = delx * nxv + dely * nyv;
       double dsw = dels, dnw = deln;
       if (p.power != 0.0) {
@@ -429,7 +521,7 @@ int lskum_create(const lskum_cloud *cloud, const lskum_params *params, lskum_ctx
         dnw = deln * w;
       }
       for (int sb = 0; sb < 4; sb++) {
-        if (!(c->mask_h[t] & (1u << sb))) continue;
+        if (!(L.mask_h[t] & (1u << sb))) continue;
         sxx[sb] = sxx[sb] + dels * dsw;
         syy[sb] = syy[sb] + deln * dnw;
         sxy[sb] = sxy[sb] + dels * dnw;
@@ -438,9 +530,124 @@ int lskum_create(const lskum_cloud *cloud, const lskum_params *params, lskum_ctx
     for (int sb = 0; sb < 4; sb++) {
       const double det = sxx[sb] * syy[sb] - sxy[sb] * sxy[sb];
       const double inv = det != 0.0 ? 1.0 / det : 0.0;
-      coef_h[(size_t)8 * d + 2 * sb] = (sb < 2 ? syy[sb] : sxx[sb]) * inv;
-      coef_h[(size_t)8 * d + 2 * sb + 1] = -sxy[sb] * inv;
+      L.coef[(size_t)8 * d + 2 * sb] = (sb < 2 ? syy[sb] : sxx[sb]) * inv;
+      L.coef[(size_t)8 * d + 2 * sb + 1] = -sxy[sb] * inv;
     }
+  }
+  // force panels: the first shape_points points (compute_force_coeffs_mod.h:
+  // 25-27 takes m = j)
+  int shape_points = 0;
+  for (int32_t i = 0; i < n; i++)
+    if (cloud.flag_2[i] > 0) shape_points++;
+  for (int j = 0; j < shape_points; j++) {
+    if (cloud.flag_1[j] == 1) continue;  // no left/right: cannot contribute a panel
+    L.fm.push_back(L.inv[j]);
+    L.fl.push_back(L.inv[cloud.left[j]]);
+    L.fr.push_back(L.inv[cloud.right[j]]);
+    L.fs.push_back(cloud.flag_2[j]);
+  }
+  return LSKUM_OK;
+}
+
+// The context of n_cases cases over n_clouds clouds.
+int create_impl(const lskum_cloud *clouds, int32_t n_clouds, const lskum_case *cases,
+                int32_t n_cases, const lskum_params *params, lskum_ctx **out) {
+  if (!out) return LSKUM_EINVAL;
+  *out = nullptr;
+  if (!clouds || !cases || !params || n_clouds < 1 || n_cases < 1) return LSKUM_EINVAL;
+  lskum_ctx *c = new (std::nothrow) lskum_ctx;
+  if (!c) return LSKUM_ENOMEM;
+  auto fail = [&](int code, const std::string &m) {
+    c->err = m;
+    *out = c;  // keep the message reachable: return the context in its failed state
+    return code;
+  };
+  const lskum_params &p = *params;
+  if (!params_ok(p)) return fail(LSKUM_EINVAL, "invalid parameters");
+  if (n_cases > 65535) return fail(LSKUM_EINVAL, "more than 65535 cases");
+  for (int32_t j = 0; j < n_cases; j++)
+    if (cases[j].cloud < 0 || cases[j].cloud >= n_clouds || !(cases[j].mach > 0.0))
+      return fail(LSKUM_EINVAL, "case " + std::to_string(j) + ": bad cloud index or mach");
+  c->prm = p;
+  c->mixed = p.precision == LSKUM_MIXED;
+  c->tile_block = p.tile_block ? p.tile_block : kTileBlockDefault;
+  c->clouds.resize(n_clouds);
+  for (int32_t ci = 0; ci < n_clouds; ci++) {
+    std::string err;
+    int rc = check_cloud(clouds[ci], &err);
+    if (!rc) rc = build_layout(clouds[ci], p, c->clouds[ci], err);
+    if (rc) return fail(rc, n_clouds > 1 ? "cloud " + std::to_string(ci) + ": " + err : err);
+  }
+  // concatenation: each case padded to whole 256-point blocks when batched
+  const int pad = n_cases > 1 ? 256 : 1;
+  const double pi = 4 * std::atan(1.0);
+  int64_t total = 0, entries = 0;
+  c->cases.resize(n_cases);
+  for (int32_t j = 0; j < n_cases; j++) {
+    CaseInfo &cs = c->cases[j];
+    const CloudLayout &L = c->clouds[cases[j].cloud];
+    cs.cloud = cases[j].cloud;
+    cs.p0 = (int32_t)total;
+    cs.n = L.n;
+    cs.np = (L.n + pad - 1) / pad * pad;
+    cs.mach = cases[j].mach;
+    cs.aoa = cases[j].aoa_deg;
+    // initial_conditions_mod.h:23-37 with parameter_mod.h:13-30
+    cs.theta = cs.aoa * pi / 180;
+    cs.q_inf[0] = c->rho_inf;
+    cs.q_inf[1] = cs.mach * std::cos(cs.theta);
+    cs.q_inf[2] = cs.mach * std::sin(cs.theta);
+    cs.q_inf[3] = c->pr_inf;
+    total += cs.np;
+    entries += L.off[L.n];
+  }
+  if (total > (int64_t)kIndexMask || entries > (int64_t)INT32_MAX)
+    return fail(LSKUM_EINVAL, "batch too large for 28-bit point / 31-bit entry indices");
+  const int32_t n = (int32_t)total;
+  c->n = n;
+  std::vector<double> base_h((size_t)B_STRIDE * n, 0.0), nxd(n, 0.0), nyd(n, 1.0),
+      coef_h((size_t)8 * n, 0.0), prim0((size_t)4 * n);
+  std::vector<uint8_t> flag_h(n, 1);
+  std::vector<int32_t> off_h(n + 1, 0), perm_h(n);
+  std::vector<uint32_t> conn_d((size_t)entries);
+  std::vector<int32_t> fm, fl, fr, fs;
+  std::vector<int32_t> soff_h(n_cases + 1, 0);
+  for (int32_t j = 0; j < n_cases; j++) {
+    CaseInfo &cs = c->cases[j];
+    const CloudLayout &L = c->clouds[cs.cloud];
+    for (int32_t d = 0; d < cs.np; d++) {
+      const int32_t g = cs.p0 + d;
+      for (int r = 0; r < 4; r++) prim0[(size_t)r * n + g] = cs.q_inf[r];
+      if (d >= cs.n) {  // padding: no neighbours, interior frame, its own state
+        off_h[g + 1] = off_h[g];
+        perm_h[g] = cs.n + (d - cs.n);
+        continue;
+      }
+      base_h[(size_t)B_STRIDE * g + B_X] = L.x[d];
+      base_h[(size_t)B_STRIDE * g + B_Y] = L.y[d];
+      base_h[(size_t)B_STRIDE * g + B_EPS2] = L.eps2[d];
+      nxd[g] = L.nx[d];
+      nyd[g] = L.ny[d];
+      flag_h[g] = L.flag[d];
+      perm_h[g] = L.perm[d];
+      std::copy(L.coef.begin() + (size_t)8 * d, L.coef.begin() + (size_t)8 * (d + 1),
+                coef_h.begin() + (size_t)8 * g);
+      const int32_t cnt = L.off[d + 1] - L.off[d];
+      off_h[g + 1] = off_h[g] + cnt;
+      for (int32_t t = 0; t < cnt; t++) {
+        const uint32_t w = L.conn[L.off[d] + t];
+        conn_d[off_h[g] + t] = ((w & kIndexMask) + (uint32_t)cs.p0) | (w & ~kIndexMask);
+      }
+    }
+    cs.s0 = (int32_t)fm.size();
+    for (size_t k = 0; k < L.fm.size(); k++) {
+      fm.push_back(L.fm[k] + cs.p0);
+      fl.push_back(L.fl[k] + cs.p0);
+      fr.push_back(L.fr[k] + cs.p0);
+      fs.push_back(L.fs[k]);
+    }
+    cs.s1 = (int32_t)fm.size();
+    soff_h[j + 1] = cs.s1;
   }
   // LDS tiles: per block of tile_block points, its own points then the
   // sorted other points its lists reach; conn16 holds local indices
@@ -505,36 +712,34 @@ int lskum_create(const lskum_cloud *cloud, const lskum_params *params, lskum_ctx
         c->tiles = prepare_tiled(want, c->tile_max, (size_t)lds_limit);
     }
   }
-  // shape points for the force reduction: the first shape_points points
-  // (compute_force_coeffs_mod.h:25-27 takes m = j)
-  int shape_points = 0;
-  for (int32_t i = 0; i < n; i++)
-    if (cloud->flag_2[i] > 0) shape_points++;
-  std::vector<int32_t> fm, fl, fr, fs;
-  for (int j = 0; j < shape_points; j++) {
-    if (cloud->flag_1[j] == 1) continue;  // no left/right: cannot contribute a panel
-    fm.push_back(c->inv[j]);
-    fl.push_back(c->inv[cloud->left[j]]);
-    fr.push_back(c->inv[cloud->right[j]]);
-    fs.push_back(cloud->flag_2[j]);
-  }
-  c->n_shape = (int)fm.size();
-  // device allocation and upload
-  auto fail = [&](int code) {
-    *out = c;
-    return code;
-  };
-  hipError_t e;
-#define ALLOC(ptr, count)                                                   \
-  if ((e = dalloc(&(ptr), (count))) != hipSuccess) {                        \
-    c->err = std::string("hipMalloc " #ptr ": ") + hipGetErrorString(e);   \
-    return fail(LSKUM_ENOMEM);                                              \
-  }
-  if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess) {
-    c->err = std::string("hipStreamCreate: ") + hipGetErrorString(e);
-    return fail(LSKUM_EHIP);
-  }
+  // per-case constants and reduction ranges
   c->part.nblocks = blocks_for(n, 256);
+  std::vector<FlowConsts> fcs_h(n_cases);
+  std::vector<double> qdyn_h(n_cases);
+  std::vector<int32_t> seg_h(n_cases + 1, 0);
+  std::vector<uint16_t> bcase_h;
+  if (n_cases > 1) bcase_h.resize(c->part.nblocks);
+  for (int32_t j = 0; j < n_cases; j++) {
+    const CaseInfo &cs = c->cases[j];
+    FlowConsts &f = fcs_h[j];
+    f.cfl = p.cfl;
+    f.power = p.power;
+    f.u1_inf = cs.q_inf[1];
+    f.u2_inf = cs.q_inf[2];
+    f.rho_inf = c->rho_inf;
+    f.pr_inf = c->pr_inf;
+    qdyn_h[j] = 0.5 * c->rho_inf * cs.mach * cs.mach;
+    seg_h[j] = cs.p0 / 256;
+    seg_h[j + 1] = n_cases > 1 ? (cs.p0 + cs.np) / 256 : c->part.nblocks;
+    for (int32_t b = seg_h[j]; b < seg_h[j + 1] && n_cases > 1; b++) bcase_h[b] = (uint16_t)j;
+  }
+  // device allocation and upload
+  hipError_t e;
+#define ALLOC(ptr, count)                                                              \
+  if ((e = dalloc(&(ptr), (count))) != hipSuccess)                                     \
+    return fail(LSKUM_ENOMEM, std::string("hipMalloc " #ptr ": ") + hipGetErrorString(e));
+  if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess)
+    return fail(LSKUM_EHIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
   ALLOC(c->base, (size_t)B_STRIDE * n);
   if (c->mixed) {
     ALLOC(c->xy, (size_t)2 * n);
@@ -567,17 +772,24 @@ int lskum_create(const lskum_cloud *cloud, const lskum_params *params, lskum_ctx
   ALLOC(c->part.sum, (size_t)c->part.nblocks);
   ALLOC(c->part.max, (size_t)c->part.nblocks);
   ALLOC(c->part.arg, (size_t)c->part.nblocks);
-  ALLOC(c->red, 3);
-  ALLOC(c->forces, 3);
+  ALLOC(c->red, (size_t)3 * n_cases);
+  ALLOC(c->forces, (size_t)3 * n_cases);
+  ALLOC(c->fcs, (size_t)n_cases);
+  ALLOC(c->qdyn, (size_t)n_cases);
+  ALLOC(c->soff, (size_t)n_cases + 1);
+  ALLOC(c->counter, 1);
+  if (n_cases > 1) {
+    ALLOC(c->seg, (size_t)n_cases + 1);
+    ALLOC(c->block_case, bcase_h.size());
+  }
   ALLOC(c->f_m, fm.size());
   ALLOC(c->f_l, fm.size());
   ALLOC(c->f_r, fm.size());
   ALLOC(c->f_s, fm.size());
 #undef ALLOC
-  if ((e = hipHostMalloc((void **)&c->red_h, sizeof(double) * 8)) != hipSuccess) {
-    c->err = std::string("hipHostMalloc: ") + hipGetErrorString(e);
-    return fail(LSKUM_ENOMEM);
-  }
+  if ((e = hipHostMalloc((void **)&c->red_h, sizeof(double) * 6 * (size_t)n_cases)) !=
+      hipSuccess)
+    return fail(LSKUM_ENOMEM, std::string("hipHostMalloc: ") + hipGetErrorString(e));
   struct Up { void *dst; const void *src; size_t bytes; };
   std::vector<Up> ups = {{c->base, base_h.data(), base_h.size() * sizeof(double)},
                          {c->nx, nxd.data(), n * sizeof(double)},
@@ -585,12 +797,21 @@ int lskum_create(const lskum_cloud *cloud, const lskum_params *params, lskum_ctx
                          {c->ny, nyd.data(), n * sizeof(double)},
                          {c->flag, flag_h.data(), (size_t)n},
                          {c->conn_off, off_h.data(), (n + 1) * sizeof(int32_t)},
-                         {c->perm_d, c->perm.data(), n * sizeof(int32_t)},
+                         {c->perm_d, perm_h.data(), n * sizeof(int32_t)},
                          {c->conn, conn_d.data(), conn_d.size() * sizeof(uint32_t)},
                          {c->f_m, fm.data(), fm.size() * sizeof(int32_t)},
                          {c->f_l, fl.data(), fl.size() * sizeof(int32_t)},
                          {c->f_r, fr.data(), fr.size() * sizeof(int32_t)},
-                         {c->f_s, fs.data(), fs.size() * sizeof(int32_t)}};
+                         {c->f_s, fs.data(), fs.size() * sizeof(int32_t)},
+                         {c->fcs, fcs_h.data(), fcs_h.size() * sizeof(FlowConsts)},
+                         {c->qdyn, qdyn_h.data(), qdyn_h.size() * sizeof(double)},
+                         {c->soff, soff_h.data(), soff_h.size() * sizeof(int32_t)},
+                         {c->prim, prim0.data(), prim0.size() * sizeof(double)},
+                         {c->prim_old, prim0.data(), prim0.size() * sizeof(double)}};
+  if (n_cases > 1) {
+    ups.push_back({c->seg, seg_h.data(), seg_h.size() * sizeof(int32_t)});
+    ups.push_back({c->block_case, bcase_h.data(), bcase_h.size() * sizeof(uint16_t)});
+  }
   std::vector<double> xy_h;
   std::vector<float> eps_h;
   if (c->mixed) {
@@ -611,10 +832,8 @@ int lskum_create(const lskum_cloud *cloud, const lskum_params *params, lskum_ctx
   }
   for (const Up &u : ups) {
     if (!u.bytes) continue;
-    if ((e = hipMemcpy(u.dst, u.src, u.bytes, hipMemcpyHostToDevice)) != hipSuccess) {
-      c->err = std::string("hipMemcpy: ") + hipGetErrorString(e);
-      return fail(LSKUM_EHIP);
-    }
+    if ((e = hipMemcpy(u.dst, u.src, u.bytes, hipMemcpyHostToDevice)) != hipSuccess)
+      return fail(LSKUM_EHIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
   }
   auto zero = [&](void *ptr, size_t bytes) {
     return ptr ? hipMemset(ptr, 0, bytes) : hipSuccess;
@@ -627,29 +846,33 @@ int lskum_create(const lskum_cloud *cloud, const lskum_params *params, lskum_ctx
       (e = zero(c->dq32[0], sizeof(float) * 8 * n)) != hipSuccess ||
       (e = zero(c->dq32[1], sizeof(float) * 8 * n)) != hipSuccess ||
       (e = hipMemset(c->flux_res, 0, sizeof(double) * 4 * n)) != hipSuccess ||
-      (e = hipMemset(c->delta, 0, sizeof(double) * n)) != hipSuccess) {
-    c->err = std::string("hipMemset: ") + hipGetErrorString(e);
-    return fail(LSKUM_EHIP);
-  }
-  // initial_conditions_mod.h:23-37 with parameter_mod.h:13-30
-  const double pi = 4 * std::atan(1.0);
-  c->theta = p.aoa_deg * pi / 180;
-  c->q_inf[0] = c->rho_inf;
-  c->q_inf[1] = p.mach * std::cos(c->theta);
-  c->q_inf[2] = p.mach * std::sin(c->theta);
-  c->q_inf[3] = c->pr_inf;
-  std::vector<double> prim0((size_t)4 * n);
-  for (int r = 0; r < 4; r++)
-    for (int32_t i = 0; i < n; i++) prim0[(size_t)r * n + i] = c->q_inf[r];
-  if ((e = hipMemcpy(c->prim, prim0.data(), prim0.size() * sizeof(double),
-                     hipMemcpyHostToDevice)) != hipSuccess ||
-      (e = hipMemcpy(c->prim_old, prim0.data(), prim0.size() * sizeof(double),
-                     hipMemcpyHostToDevice)) != hipSuccess) {
-    c->err = std::string("hipMemcpy prim: ") + hipGetErrorString(e);
-    return fail(LSKUM_EHIP);
-  }
+      (e = hipMemset(c->delta, 0, sizeof(double) * n)) != hipSuccess ||
+      (e = hipMemset(c->counter, 0, sizeof(int32_t))) != hipSuccess)
+    return fail(LSKUM_EHIP, std::string("hipMemset: ") + hipGetErrorString(e));
   *out = c;
   return LSKUM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *lskum_last_error(const lskum_ctx *ctx) {
+  return ctx ? ctx->err.c_str() : "null context";
+}
+
+int lskum_create(const lskum_cloud *cloud, const lskum_params *params, lskum_ctx **out) {
+  if (!params) return LSKUM_EINVAL;
+  lskum_case one;
+  one.cloud = 0;
+  one.mach = params->mach;
+  one.aoa_deg = params->aoa_deg;
+  return create_impl(cloud, 1, &one, 1, params, out);
+}
+
+int lskum_create_batch(const lskum_cloud *clouds, int32_t n_clouds, const lskum_case *cases,
+                       int32_t n_cases, const lskum_params *params, lskum_ctx **out) {
+  return create_impl(clouds, n_clouds, cases, n_cases, params, out);
 }
 
 void lskum_destroy(lskum_ctx *ctx) {
@@ -665,14 +888,25 @@ void lskum_destroy(lskum_ctx *ctx) {
     if (!(ctx)->stream) return set_err(ctx, LSKUM_EINVAL, "context failed to initialise"); \
   } while (0)
 
+int lskum_case_count(lskum_ctx *ctx, int32_t *n_cases, int32_t *points) {
+  CTX_OK(ctx);
+  if (n_cases) *n_cases = ctx->n_cases();
+  if (points)
+    for (int j = 0; j < ctx->n_cases(); j++) points[j] = ctx->cases[j].n;
+  return LSKUM_OK;
+}
+
 int lskum_set_state(lskum_ctx *ctx, const double *prim) {
   CTX_OK(ctx);
   if (!prim) return set_err(ctx, LSKUM_EINVAL, "null prim");
-  std::vector<double> tmp((size_t)4 * ctx->n);
+  std::vector<double> tmp;
+  int rc = fetch_plain(ctx, ctx->prim, 4, tmp);  // keeps the padding points' state
+  if (rc) return rc;
   permute_in(ctx, prim, tmp.data(), 4);
   HIP_TRY(ctx, hipMemcpyAsync(ctx->prim, tmp.data(), tmp.size() * sizeof(double),
                               hipMemcpyHostToDevice, ctx->stream));
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->q_fresh = ctx->speed_fresh = false;
   return LSKUM_OK;
 }
 
@@ -687,7 +921,8 @@ int lskum_snapshot(lskum_ctx *ctx) {
 
 int lskum_func_delta(lskum_ctx *ctx) {
   CTX_OK(ctx);
-  return do_func_delta(ctx, true);
+  int rc = do_speed(ctx);
+  return rc ? rc : do_func_delta(ctx);
 }
 
 int lskum_q_variables(lskum_ctx *ctx) {
@@ -715,43 +950,62 @@ int lskum_state_update(lskum_ctx *ctx, int32_t rk, double *sum_res_sqr, double *
                        int32_t *max_res_point) {
   CTX_OK(ctx);
   if (rk < 1 || rk > 4) return set_err(ctx, LSKUM_EINVAL, "rk outside 1..4");
-  int rc = do_state_update(ctx, rk, nullptr);
+  int rc = do_state_update(ctx, rk, false);
   if (rc) return rc;
   if (!sum_res_sqr && !max_res && !max_res_point) return LSKUM_OK;
-  HIP_TRY(ctx, hipMemcpyAsync(ctx->red_h, ctx->red, sizeof(double) * 3, hipMemcpyDeviceToHost,
-                              ctx->stream));
+  const int nc = ctx->n_cases();
+  HIP_TRY(ctx, hipMemcpyAsync(ctx->red_h, ctx->red, sizeof(double) * 3 * nc,
+                              hipMemcpyDeviceToHost, ctx->stream));
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-  // state_update_mod.h:106-107,159-163: max_res starts at 0 and the point
-  // only moves on a strictly larger value
-  const double mx = ctx->red_h[1] > 0.0 ? ctx->red_h[1] : 0.0;
-  if (ctx->red_h[1] > 0.0) ctx->max_res_point = (int32_t)ctx->red_h[2];
-  if (sum_res_sqr) *sum_res_sqr = ctx->red_h[0];
-  if (max_res) *max_res = mx;
-  if (max_res_point) *max_res_point = ctx->max_res_point;
-  if (!std::isfinite(ctx->red_h[0]))
+  bool finite = true;
+  for (int j = 0; j < nc; j++) {
+    const double *r = ctx->red_h + 3 * j;
+    // state_update_mod.h:106-107,159-163: max_res starts at 0 and the point
+    // only moves on a strictly larger value
+    if (r[1] > 0.0) ctx->cases[j].max_res_point = (int32_t)r[2];
+    if (sum_res_sqr) sum_res_sqr[j] = r[0];
+    if (max_res) max_res[j] = r[1] > 0.0 ? r[1] : 0.0;
+    if (max_res_point) max_res_point[j] = ctx->cases[j].max_res_point;
+    if (!std::isfinite(r[0])) finite = false;
+  }
+  if (!finite && nc == 1)  // a batch reports NaN for a failed case instead
     return set_err(ctx, LSKUM_ENONFINITE, "residual sum is not finite");
   return LSKUM_OK;
 }
 
 int lskum_forces(lskum_ctx *ctx, double *cl, double *cd, double *cm) {
   CTX_OK(ctx);
-  const double qdyn = 0.5 * ctx->rho_inf * ctx->prm.mach * ctx->prm.mach;
-  launch_forces(ctx->n_shape, ctx->f_m, ctx->f_l, ctx->f_r, ctx->f_s, ctx->base, ctx->nx, ctx->ny,
-                ctx->prim, ctx->n, ctx->pr_inf, qdyn, ctx->forces, ctx->stream);
+  const int nc = ctx->n_cases();
+  launch_forces(nc, ctx->soff, ctx->f_m, ctx->f_l, ctx->f_r, ctx->f_s, ctx->base, ctx->nx,
+                ctx->ny, ctx->prim, ctx->n, ctx->pr_inf, ctx->qdyn, ctx->forces, ctx->stream);
   int rc = check_launch(ctx, "forces");
   if (rc) return rc;
-  HIP_TRY(ctx, hipMemcpyAsync(ctx->red_h + 3, ctx->forces, sizeof(double) * 3,
-                              hipMemcpyDeviceToHost, ctx->stream));
+  double *h = ctx->red_h + 3 * nc;
+  HIP_TRY(ctx, hipMemcpyAsync(h, ctx->forces, sizeof(double) * 3 * nc, hipMemcpyDeviceToHost,
+                              ctx->stream));
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-  const double H = ctx->red_h[3], V = ctx->red_h[4], P = ctx->red_h[5];
-  // compute_force_coeffs_mod.h:62-67
-  const double l = V * std::cos(ctx->theta) - H * std::sin(ctx->theta);
-  const double d = H * std::cos(ctx->theta) + V * std::sin(ctx->theta);
-  if (cl) *cl = l;
-  if (cd) *cd = d;
-  if (cm) *cm = P;
-  if (!std::isfinite(l) || !std::isfinite(d) || !std::isfinite(P))
+  bool finite = true;
+  for (int j = 0; j < nc; j++) {
+    const double H = h[3 * j], V = h[3 * j + 1], P = h[3 * j + 2], th = ctx->cases[j].theta;
+    // compute_force_coeffs_mod.h:62-67
+    const double l = V * std::cos(th) - H * std::sin(th);
+    const double d = H * std::cos(th) + V * std::sin(th);
+    if (cl) cl[j] = l;
+    if (cd) cd[j] = d;
+    if (cm) cm[j] = P;
+    if (!std::isfinite(l) || !std::isfinite(d) || !std::isfinite(P)) finite = false;
+  }
+  if (!finite && nc == 1)  // a batch reports NaN for a failed case instead
     return set_err(ctx, LSKUM_ENONFINITE, "force coefficients are not finite");
+  return LSKUM_OK;
+}
+
+int lskum_set_graph(lskum_ctx *ctx, int32_t iters_per_graph) {
+  CTX_OK(ctx);
+  if (iters_per_graph < 0 || iters_per_graph > 1000)
+    return set_err(ctx, LSKUM_EINVAL, "iters_per_graph outside 0..1000");
+  if (iters_per_graph != ctx->gexec_iters) drop_graph(ctx);
+  ctx->graph_iters = iters_per_graph;
   return LSKUM_OK;
 }
 
@@ -759,59 +1013,85 @@ int lskum_iterate(lskum_ctx *ctx, int32_t n_iter, double *res_new, double *resid
   CTX_OK(ctx);
   if (n_iter < 0) return set_err(ctx, LSKUM_EINVAL, "n_iter < 0");
   if (n_iter == 0) return LSKUM_OK;
+  const int nc = ctx->n_cases();
   if (ctx->hist_cap < n_iter) {
+    drop_graph(ctx);  // a captured graph writes into the old buffer
     if (ctx->hist) (void)hipFree(ctx->hist);
     ctx->hist = nullptr;
     ctx->hist_cap = 0;
-    HIP_TRY(ctx, dalloc(&ctx->hist, (size_t)n_iter));
+    HIP_TRY(ctx, dalloc(&ctx->hist, (size_t)n_iter * nc));
     ctx->hist_cap = n_iter;
   }
-  const int inner = ctx->prm.inner_iterations;
-  for (int t = 0; t < n_iter; t++) {
-    // q and the speed come from the previous stage-4 update, except on the
-    // first iteration of the call (the state may have been set since)
-    int rc = do_snapshot(ctx);
-    if (!rc) rc = do_func_delta(ctx, t == 0);
-    for (int rk = 1; rk <= 4 && !rc; rk++) {
-      if (t == 0 && rk == 1) rc = do_q_variables(ctx);
-      if (!rc) rc = do_q_derivatives(ctx, inner);
-      if (!rc) rc = do_q_inner(ctx, inner);
-      if (!rc) rc = do_flux(ctx);
-      if (!rc) rc = do_state_update(ctx, rk, rk == 4 ? ctx->hist + t : nullptr);
-    }
-    if (rc) return rc;
-  }
-  std::vector<double> sums(n_iter);
-  HIP_TRY(ctx, hipMemcpyAsync(sums.data(), ctx->hist, sizeof(double) * n_iter,
-                              hipMemcpyDeviceToHost, ctx->stream));
-  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-  int rc = collect_timing(ctx);
+  int rc = LSKUM_OK;
+  // q and the speed of the current state: from the last stage-4 update, else
+  // computed here
+  if (!ctx->speed_fresh) rc = do_speed(ctx);
+  if (!rc && !ctx->q_fresh) rc = do_q_variables(ctx);
   if (rc) return rc;
-  bool finite = true;
+  launch_set_int(ctx->counter, 0, ctx->stream);
+  int done = 0;
+  const int K = ctx->graph_iters;
+  if (K > 0 && n_iter >= K && !ctx->timing) {
+    if (!ctx->gexec || ctx->gexec_iters != K || ctx->gexec_hist != ctx->hist)
+      rc = capture_graph(ctx, K);
+    for (; !rc && done + K <= n_iter; done += K) {
+      hipError_t e = hipGraphLaunch(ctx->gexec, ctx->stream);
+      if (e != hipSuccess) rc = set_err(ctx, LSKUM_EHIP, std::string("hipGraphLaunch: ") + hipGetErrorString(e));
+    }
+  }
+  for (; !rc && done < n_iter; done++) rc = run_iteration(ctx);
+  if (rc) return rc;
+  std::vector<double> sums((size_t)nc * n_iter);
+  HIP_TRY(ctx, hipMemcpy2DAsync(sums.data(), sizeof(double) * n_iter, ctx->hist,
+                                sizeof(double) * ctx->hist_cap, sizeof(double) * n_iter, nc,
+                                hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  rc = collect_timing(ctx);
+  if (rc) return rc;
+  int bad = -1;
   for (int t = 0; t < n_iter; t++) {
+    for (int j = 0; j < nc; j++)
+      if (!std::isfinite(sums[(size_t)j * n_iter + t]) && !ctx->cases[j].failed_at)
+        ctx->cases[j].failed_at = ctx->it + 1 + t;
     // fpi_solver_mod.h:138-147
     const int it = ctx->it + 1;
-    const double rn = std::sqrt(sums[t]) / ctx->n;
-    double rs;
-    if (it <= 2 && ctx->restart == 0) {
-      ctx->res_old = rn;
-      rs = 0;
-    } else {
-      rs = std::log10(rn / ctx->res_old);
+    for (int j = 0; j < nc; j++) {
+      CaseInfo &cs = ctx->cases[j];
+      const double s = sums[(size_t)j * n_iter + t];
+      const double rn = std::sqrt(s) / cs.n;
+      double rs;
+      if (it <= 2 && ctx->restart == 0) {
+        cs.res_old = rn;
+        rs = 0;
+      } else {
+        rs = std::log10(rn / cs.res_old);
+      }
+      if (res_new) res_new[(size_t)j * n_iter + t] = rn;
+      if (residue) residue[(size_t)j * n_iter + t] = rs;
+      if (!std::isfinite(s) && bad < 0) bad = j;
     }
     ctx->it = it;
-    if (res_new) res_new[t] = rn;
-    if (residue) residue[t] = rs;
-    if (!std::isfinite(sums[t])) finite = false;
   }
-  if (!finite) return set_err(ctx, LSKUM_ENONFINITE, "residual became non-finite");
+  // A batch's cases share no points, stencils or reductions, so a case that
+  // diverges cannot disturb the others: the batch goes on and
+  // lskum_case_status names the failed cases.  A single case fails the call.
+  if (bad >= 0 && nc == 1)
+    return set_err(ctx, LSKUM_ENONFINITE, "residual became non-finite");
+  return LSKUM_OK;
+}
+
+int lskum_case_status(lskum_ctx *ctx, int32_t *failed_at) {
+  CTX_OK(ctx);
+  if (!failed_at) return set_err(ctx, LSKUM_EINVAL, "null output");
+  for (int j = 0; j < ctx->n_cases(); j++) failed_at[j] = ctx->cases[j].failed_at;
   return LSKUM_OK;
 }
 
 int lskum_get_iteration(lskum_ctx *ctx, int32_t *it, double *res_old) {
   CTX_OK(ctx);
   if (it) *it = ctx->it;
-  if (res_old) *res_old = ctx->res_old;
+  if (res_old)
+    for (int j = 0; j < ctx->n_cases(); j++) res_old[j] = ctx->cases[j].res_old;
   return LSKUM_OK;
 }
 
@@ -819,7 +1099,7 @@ int lskum_set_iteration(lskum_ctx *ctx, int32_t it, double res_old, int32_t rest
   CTX_OK(ctx);
   if (it < 0) return set_err(ctx, LSKUM_EINVAL, "it < 0");
   ctx->it = it;
-  ctx->res_old = res_old;
+  for (CaseInfo &cs : ctx->cases) cs.res_old = res_old;
   ctx->restart = restart ? 1 : 0;
   return LSKUM_OK;
 }
@@ -849,10 +1129,16 @@ int lskum_get_array(lskum_ctx *ctx, int32_t which, double *dst) {
       break;
     case LSKUM_A_DELTA: rc = fetch_plain(ctx, ctx->delta, 1, dev); ncomp = 1; break;
     case LSKUM_A_FLUX_RES: rc = fetch_plain(ctx, ctx->flux_res, 4, dev); ncomp = 4; break;
-    case LSKUM_A_NORMALS:
-      std::memcpy(dst, ctx->nx_h.data(), sizeof(double) * ctx->n);
-      std::memcpy(dst + ctx->n, ctx->ny_h.data(), sizeof(double) * ctx->n);
+    case LSKUM_A_NORMALS: {
+      size_t o = 0;
+      for (const CaseInfo &cs : ctx->cases) {
+        const CloudLayout &L = ctx->clouds[cs.cloud];
+        std::memcpy(dst + o, L.nx_h.data(), sizeof(double) * cs.n);
+        std::memcpy(dst + o + cs.n, L.ny_h.data(), sizeof(double) * cs.n);
+        o += 2 * (size_t)cs.n;
+      }
       return LSKUM_OK;
+    }
     default: return set_err(ctx, LSKUM_EINVAL, "unknown array id");
   }
   if (rc) return rc;
@@ -863,13 +1149,14 @@ int lskum_get_array(lskum_ctx *ctx, int32_t which, double *dst) {
 int lskum_get_substencil(lskum_ctx *ctx, int32_t which, int32_t *counts, int32_t *members) {
   CTX_OK(ctx);
   if (which < 0 || which > 3 || !counts) return set_err(ctx, LSKUM_EINVAL, "bad arguments");
+  const CloudLayout &L = ctx->clouds[ctx->cases[0].cloud];
   const uint8_t bit = (uint8_t)(1u << which);
   int64_t w = 0;
-  for (int32_t i = 0; i < ctx->n; i++) {
+  for (int32_t i = 0; i < L.n; i++) {
     int32_t cnt = 0;
-    for (int32_t e = ctx->conn_off_h[i]; e < ctx->conn_off_h[i + 1]; e++) {
-      if (!(ctx->mask_h[e] & bit)) continue;
-      if (members) members[w] = ctx->conn_h[e];
+    for (int32_t e = L.conn_off_h[i]; e < L.conn_off_h[i + 1]; e++) {
+      if (!(L.mask_h[e] & bit)) continue;
+      if (members) members[w] = L.conn_h[e];
       w++;
       cnt++;
     }
@@ -907,6 +1194,62 @@ int lskum_kernel_time(lskum_ctx *ctx, int32_t kernel, double *avg_ms, int32_t *l
   if (avg_ms) *avg_ms = ctx->k_cnt[kernel] ? ctx->k_ms[kernel] / ctx->k_cnt[kernel] : 0.0;
   if (launches) *launches = ctx->k_cnt[kernel];
   return LSKUM_OK;
+}
+
+int lskum_autotune(const lskum_cloud *clouds, int32_t n_clouds, const lskum_case *cases,
+                   int32_t n_cases, const lskum_params *params, int32_t iters,
+                   lskum_tune_result *best, lskum_tune_result *trials, int32_t max_trials,
+                   int32_t *n_trials) {
+  if (!params || !best || iters < 1 || iters > 1000) return LSKUM_EINVAL;
+  const int32_t blocks[3] = {64, 128, 256};
+  const int32_t masks[2] = {LSKUM_TILE_QDERIV | LSKUM_TILE_QINNER | LSKUM_TILE_DELTA,
+                            LSKUM_TILE_QDERIV | LSKUM_TILE_QINNER | LSKUM_TILE_DELTA |
+                                LSKUM_TILE_FLUX};
+  int32_t count = 0;
+  best->ms_per_iter = -1.0;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    if (e0) (void)hipEventDestroy(e0);
+    return LSKUM_EHIP;
+  }
+  int rc = LSKUM_OK;
+  for (int32_t tb : blocks) {
+    for (int32_t mask : masks) {
+      lskum_params p = *params;
+      p.tile_block = tb;
+      p.lds_tiles = mask;
+      lskum_ctx *ctx = nullptr;
+      rc = create_impl(clouds, n_clouds, cases, n_cases, &p, &ctx);
+      if (rc) {
+        lskum_destroy(ctx);
+        break;
+      }
+      lskum_tune_result r;
+      r.tile_block = tb;
+      r.lds_tiles = (int32_t)ctx->tiles;  // what fits: a tile too big for LDS is dropped
+      // one graph of `iters` iterations, after one warm-up replay of it
+      rc = lskum_set_graph(ctx, iters);
+      if (!rc) rc = lskum_iterate(ctx, iters, nullptr, nullptr);
+      if (!rc && hipEventRecord(e0, ctx->stream) != hipSuccess) rc = LSKUM_EHIP;
+      if (!rc) rc = lskum_iterate(ctx, iters, nullptr, nullptr);
+      if (!rc && hipEventRecord(e1, ctx->stream) != hipSuccess) rc = LSKUM_EHIP;
+      float ms = 0.f;
+      if (!rc && (hipEventSynchronize(e1) != hipSuccess ||
+                  hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+        rc = LSKUM_EHIP;
+      lskum_destroy(ctx);
+      if (rc) break;
+      r.ms_per_iter = ms / iters;
+      if (trials && count < max_trials) trials[count] = r;
+      count++;
+      if (best->ms_per_iter < 0.0 || r.ms_per_iter < best->ms_per_iter) *best = r;
+    }
+    if (rc) break;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (n_trials) *n_trials = count;
+  return rc;
 }
 
 int lskum_kat_fluxes(int32_t n, const double *in6, double *out, int32_t mixed) {

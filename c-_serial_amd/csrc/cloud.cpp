@@ -339,11 +339,19 @@ int lskum_cloud_generate(const lskum_gen_params *p, lskum_cloud_h **out) {
     g[r] = (q == 1.0) ? (double)r / L : (std::pow(q, r) - 1.0) / (std::pow(q, L) - 1.0);
   g[L] = 1.0;
 
+  // The far-field circle is parametrised from the direction of the camber
+  // line at the trailing edge (0 for a symmetric section), so that the rings
+  // leave the trailing-edge wedge along its bisector rather than along +x,
+  // which for an aft-cambered section runs along its lower surface.
+  double yc_te, beta;
+  af.camber(1.0, &yc_te, &beta);
+  beta = std::atan(beta);
   auto ring_pos = [&](double s, double gg, double *X, double *Y) {
     double sx, sy;
     arc.at(s, &sx, &sy);
-    double fx = 0.5 + R * std::cos(2.0 * kPi * s);
-    double fy = -R * std::sin(2.0 * kPi * s);
+    const double th = -2.0 * kPi * s + beta;
+    double fx = 0.5 + R * std::cos(th);
+    double fy = R * std::sin(th);
     *X = (1.0 - gg) * sx + gg * fx;
     *Y = (1.0 - gg) * sy + gg * fy;
   };
@@ -397,6 +405,73 @@ int lskum_cloud_generate(const lskum_gen_params *p, lskum_cloud_h **out) {
       }
     }
   }
+  // Ring points inside the body.  The rings blend each surface point toward
+  // the far-field circle, which at the trailing edge lies along +x; where a
+  // cambered section's lower surface descends toward the trailing edge, that
+  // direction enters the thin wedge, and near-wall ring points land inside
+  // the body.  Such a point moves to the foot of its nearest wall edge plus
+  // its ring's wall distance along the edge's outward normal.  (A symmetric
+  // section's surfaces approach the trailing edge from outside that
+  // direction, so none of its points move.)
+  {
+    double area2 = 0.0;  // twice the signed area of the wall polygon
+    for (int32_t e = 0; e < nw; e++) {
+      const int32_t b = (e + 1) % nw;
+      area2 += c->x[e] * c->y[b] - c->x[b] * c->y[e];
+    }
+    const double sgn = area2 > 0.0 ? 1.0 : -1.0;  // outward = sgn * (dy, -dx)
+    double bx0 = 1e300, bx1 = -1e300, by0 = 1e300, by1 = -1e300;
+    for (int32_t e = 0; e < nw; e++) {
+      bx0 = std::min(bx0, c->x[e]); bx1 = std::max(bx1, c->x[e]);
+      by0 = std::min(by0, c->y[e]); by1 = std::max(by1, c->y[e]);
+    }
+    auto inside = [&](double px, double py) {
+      bool in = false;
+      for (int32_t e = 0; e < nw; e++) {
+        const int32_t b = (e + 1) % nw;
+        const double ya = c->y[e], yb = c->y[b];
+        if ((ya > py) != (yb > py) &&
+            px < c->x[e] + (py - ya) * (c->x[b] - c->x[e]) / (yb - ya))
+          in = !in;
+      }
+      return in;
+    };
+    for (int32_t r = 1; r < L; r++) {
+#pragma omp parallel for schedule(dynamic, 64)
+      for (int32_t j = 0; j < M; j++) {
+        const int32_t id = nw + (r - 1) * M + j;
+        const double px = c->x[id], py = c->y[id];
+        // the ring's distance from the wall at this arc position
+        const double s = (j + 0.5) / M;
+        double sx, sy, fx_far, fy_far;
+        ring_pos(s, 0.0, &sx, &sy);
+        ring_pos(s, 1.0, &fx_far, &fy_far);
+        const double dist = g[r] * std::hypot(fx_far - sx, fy_far - sy);
+        if (px < bx0 || px > bx1 || py < by0 || py > by1 || !inside(px, py)) continue;
+        int32_t best = 0;
+        double bd = 1e300, fx = 0.0, fy = 0.0;
+        for (int32_t e = 0; e < nw; e++) {
+          const int32_t b = (e + 1) % nw;
+          const double ex = c->x[b] - c->x[e], ey = c->y[b] - c->y[e];
+          const double ee = ex * ex + ey * ey;
+          double t = ee > 0.0 ? ((px - c->x[e]) * ex + (py - c->y[e]) * ey) / ee : 0.0;
+          t = std::max(0.0, std::min(1.0, t));
+          const double qx = c->x[e] + t * ex, qy = c->y[e] + t * ey;
+          const double d = (px - qx) * (px - qx) + (py - qy) * (py - qy);
+          if (d < bd) { bd = d; best = e; fx = qx; fy = qy; }
+        }
+        const int32_t b = (best + 1) % nw;
+        const double ex = c->x[b] - c->x[best], ey = c->y[b] - c->y[best];
+        const double el = std::hypot(ex, ey);
+        if (!(el > 0.0)) continue;
+        const double nxp = fx + dist * sgn * ey / el, nyp = fy - dist * sgn * ex / el;
+        if (!inside(nxp, nyp)) {
+          c->x[id] = nxp;
+          c->y[id] = nyp;
+        }
+      }
+    }
+  }
   // neighbours: k nearest visible points within radius_factor * h, ordered
   // by (distance, index), then balanced so that every sub-stencil the flux
   // needs (x+/x- along the point's tangent; y+/y- for interior points) has
@@ -429,7 +504,7 @@ int lskum_cloud_generate(const lskum_gen_params *p, lskum_cloud_h **out) {
     for (int32_t i = 0; i < n; i++) {
       const double rr = p->radius_factor * h[i];
       tree.query(i, 4 * k, 4.0 * rr * rr, cand, visible);
-      const int nc = (int)cand.size();
+      int nc = (int)cand.size();
       used.assign(nc, 0);
       // sub-stencil membership of each candidate in i's (t, n) frame
       const double tx = nyv[i], ty = -nxv[i], nxi = nxv[i], nyi = nyv[i];
@@ -495,6 +570,65 @@ int lskum_cloud_generate(const lskum_gen_params *p, lskum_cloud_h **out) {
             }
             if (ok) { taken--; break; }
             used[a] = 1;
+          }
+        }
+      }
+      // Fallback where the nearest candidates are nearly collinear, as for
+      // first-ring points at a thin trailing edge, whose rings keep the
+      // wall's tangential clustering but sit a full radial spacing apart:
+      // widen the candidate set and add, one at a time, the candidate that
+      // raises the sub-stencil's determinant most, dropping the farthest
+      // member whose removal leaves every needed sub-stencil with kMinSub
+      // members and no lower determinant than the target (or than it had).
+      for (unsigned sb = 1; sb <= 8; sb <<= 1) {
+        if (!(need & sb)) continue;
+        for (int widen = 1; widen <= 2 && rel_det(sb) < kMinRelDet; widen++) {
+          std::vector<int32_t> ids;
+          for (int a = 0; a < nc; a++)
+            if (used[a]) ids.push_back(cand[a].second);
+          const double scale = (double)(2 << widen);  // 4x, then 8x the radius
+          tree.query(i, 4 * k << (2 * widen), scale * scale * rr * rr, cand, visible);
+          nc = (int)cand.size();
+          used.assign(nc, 0);
+          for (int a = 0; a < nc; a++)
+            used[a] = std::find(ids.begin(), ids.end(), cand[a].second) != ids.end();
+          for (int step = 0; step < 8 && rel_det(sb) < kMinRelDet; step++) {
+            const double cur = rel_det(sb);
+            int best = -1;
+            double best_det = cur;
+            for (int a = 0; a < nc; a++) {
+              if (used[a] || !(bits(a) & sb)) continue;
+              used[a] = 1;
+              const double d = rel_det(sb);
+              used[a] = 0;
+              if (d > best_det) { best_det = d; best = a; }
+            }
+            if (best < 0) break;
+            used[best] = 1;
+            if (++taken <= k) continue;
+            double before[4];
+            for (int t = 0; t < 4; t++) before[t] = (need >> t & 1u) ? rel_det(1u << t) : 0.0;
+            bool dropped = false;
+            for (int a = nc - 1; a >= 0 && !dropped; a--) {
+              if (!used[a] || a == best) continue;
+              used[a] = 0;
+              bool ok = true;
+              for (int t = 0; t < 4 && ok; t++) {
+                if (!(need >> t & 1u)) continue;
+                ok = rel_det(1u << t) >= std::min(before[t], kMinRelDet);
+              }
+              if (ok) {
+                taken--;
+                dropped = true;
+              } else {
+                used[a] = 1;
+              }
+            }
+            if (!dropped) {  // nothing can make room: keep the list as it was
+              used[best] = 0;
+              taken--;
+              break;
+            }
           }
         }
       }

@@ -9,6 +9,8 @@
 // 64-byte per-point records through L2.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "device.hpp"
 #include "kernels.hpp"
 #include "physics.hpp"
@@ -184,7 +186,9 @@ __global__ __launch_bounds__(256) void k_q_derivatives(
     }
   }
   const double det = sum_delx_sqr * sum_dely_sqr - sum_delx_dely * sum_delx_dely;
-  const double one_by_det = 1.0 / det;
+  // det > 0 for every validated point; 0 only for a batch's padding points,
+  // which have no neighbours and must not produce NaNs
+  const double one_by_det = det != 0.0 ? 1.0 / det : 0.0;
   double dqx[4], dqy[4];
   for (int r = 0; r < 4; r++) {
     dqx[r] = (sdx[r] * sum_dely_sqr - sdy[r] * sum_delx_dely) * one_by_det;
@@ -262,7 +266,9 @@ __global__ __launch_bounds__(256) void k_q_inner(
     }
   }
   const double det = sum_delx_sqr * sum_dely_sqr - sum_delx_dely * sum_delx_dely;
-  const double one_by_det = 1.0 / det;
+  // det > 0 for every validated point; 0 only for a batch's padding points,
+  // which have no neighbours and must not produce NaNs
+  const double one_by_det = det != 0.0 ? 1.0 / det : 0.0;
   double ox[4], oy[4];
   for (int r = 0; r < 4; r++) {
     ox[r] = one_by_det * (sx[r] * sum_dely_sqr - sy[r] * sum_delx_dely);
@@ -496,12 +502,16 @@ __global__ __launch_bounds__(256) void k_state_update(int n, int rk, double *__r
                                                       const double *__restrict__ nya,
                                                       const uint8_t *__restrict__ flag,
                                                       const int32_t *__restrict__ perm,
-                                                      FlowConsts fc, double *__restrict__ psum,
+                                                      const FlowConsts *__restrict__ fcs,
+                                                      const uint16_t *__restrict__ block_case,
+                                                      double *__restrict__ psum,
                                                       double *__restrict__ pmax,
                                                       int32_t *__restrict__ parg,
                                                       double *__restrict__ base,
                                                       float4 *__restrict__ q32, int speed) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  // a batch pads every case to whole 256-point blocks: one case per block
+  const FlowConsts fc = fcs[block_case ? block_case[blockIdx.x] : 0];
   double res_sqr = 0.0;
   int arg = 0x7fffffff;
   if (i < n) {
@@ -570,13 +580,22 @@ __global__ __launch_bounds__(256) void k_state_update(int n, int rk, double *__r
   }
 }
 
-__global__ __launch_bounds__(256) void k_reduce(int nb, const double *__restrict__ psum,
+// One block per case: reduces the case's partials [seg[c], seg[c+1]) in a
+// fixed order into out[3c..3c+2] = (sum, max, argmax as a double) and, when
+// hist is given, hist[c*hist_stride + slot] = sum, with slot = *slot_ptr
+// when slot_ptr is given (a graph replays the same launch every iteration).
+__global__ __launch_bounds__(256) void k_reduce(const int32_t *__restrict__ seg, int nb,
+                                                const double *__restrict__ psum,
                                                 const double *__restrict__ pmax,
                                                 const int32_t *__restrict__ parg,
-                                                double *__restrict__ out, double *__restrict__ hist) {
+                                                double *__restrict__ out, double *__restrict__ hist,
+                                                int hist_stride, int slot,
+                                                const int32_t *__restrict__ slot_ptr) {
+  const int c = blockIdx.x;
+  const int b0 = seg ? seg[c] : 0, b1 = seg ? seg[c + 1] : nb;
   double s = 0.0, mx = -1.0;
   int arg = 0x7fffffff;
-  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+  for (int b = b0 + threadIdx.x; b < b1; b += blockDim.x) {
     s = s + psum[b];
     const double v = pmax[b];
     const int a = parg[b];
@@ -587,17 +606,34 @@ __global__ __launch_bounds__(256) void k_reduce(int nb, const double *__restrict
   }
   block_reduce3(s, mx, arg);
   if (threadIdx.x == 0) {
-    out[0] = s;
-    out[1] = mx;
-    out[2] = (double)arg;
-    if (hist) hist[0] = s;
+    out[3 * c] = s;
+    out[3 * c + 1] = mx;
+    out[3 * c + 2] = (double)arg;
+    if (hist) hist[(size_t)c * hist_stride + (slot_ptr ? *slot_ptr : slot)] = s;
   }
+}
+
+__global__ void k_tick(int32_t *counter) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) counter[0] += 1;
+}
+
+__global__ void k_set_int(int32_t *p, int32_t v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) p[0] = v;
+}
+
+// dst <- src (graph-safe replacement of a device-to-device memcpy)
+__global__ __launch_bounds__(256) void k_copy(size_t n, const double2 *__restrict__ src,
+                                              double2 *__restrict__ dst) {
+  for (size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x; j < n;
+       j += (size_t)gridDim.x * blockDim.x)
+    dst[j] = src[j];
 }
 
 // ------------------------------------------------------------ forces
 // compute_force_coeffs_mod.h:25-60 for shape 1: H, V and the pitching
 // moment about (0.25, 0), reduced in a fixed order by one block.
-__global__ __launch_bounds__(256) void k_forces(int ns, const int32_t *__restrict__ mi,
+__global__ __launch_bounds__(256) void k_forces(const int32_t *__restrict__ soff,
+                                                const int32_t *__restrict__ mi,
                                                 const int32_t *__restrict__ li,
                                                 const int32_t *__restrict__ ri,
                                                 const int32_t *__restrict__ shape,
@@ -605,11 +641,15 @@ __global__ __launch_bounds__(256) void k_forces(int ns, const int32_t *__restric
                                                 const double *__restrict__ nxa,
                                                 const double *__restrict__ nya,
                                                 const double *__restrict__ prim, int n,
-                                                double pr_inf, double qdyn,
+                                                double pr_inf,
+                                                const double *__restrict__ qdyns,
                                                 double *__restrict__ out) {
+  // one block per case: its shape points are [soff[c], soff[c+1])
   __shared__ double sh[3][256];
+  const int c = blockIdx.x;
+  const double qdyn = qdyns[c];
   double H = 0.0, V = 0.0, P = 0.0;
-  for (int j = threadIdx.x; j < ns; j += blockDim.x) {
+  for (int j = soff[c] + threadIdx.x; j < soff[c + 1]; j += blockDim.x) {
     if (shape[j] != 1) continue;
     const int m = mi[j], l = li[j], r = ri[j];
     const double2 L = reinterpret_cast<const double2 *>(base + (size_t)B_STRIDE * l)[0];
@@ -637,7 +677,7 @@ __global__ __launch_bounds__(256) void k_forces(int ns, const int32_t *__restric
       for (int c = 0; c < 3; c++) sh[c][t] = sh[c][t] + sh[c][t + w];
     __syncthreads();
   }
-  if (t < 3) out[t] = sh[t][0];
+  if (t < 3) out[3 * c + t] = sh[t][0];
 }
 
 // ------------------------------------------------------------ KAT
@@ -775,25 +815,43 @@ void launch_flux(const DevCloud &c, const double *base, const double *dq, const 
 }
 
 void launch_state_update(const DevCloud &c, int rk, double *prim, const double *prim_old,
-                         const double *flux_res, const FlowConsts &fc, const Partials &p,
-                         double *base, float *q32, bool speed, hipStream_t s) {
+                         const double *flux_res, const FlowConsts *fcs,
+                         const uint16_t *block_case, const Partials &p, double *base,
+                         float *q32, bool speed, hipStream_t s) {
   // fixed 256-thread blocks: the partial layout must not depend on tuning
   hipLaunchKernelGGL(k_state_update, dim3(p.nblocks), dim3(256), 0, s, c.n, rk, prim, prim_old,
-                     flux_res, c.nx, c.ny, c.flag, c.perm, fc, p.sum, p.max, p.arg, base,
-                     reinterpret_cast<float4 *>(q32), speed ? 1 : 0);
+                     flux_res, c.nx, c.ny, c.flag, c.perm, fcs, block_case, p.sum, p.max, p.arg,
+                     base, reinterpret_cast<float4 *>(q32), speed ? 1 : 0);
 }
 
-void launch_reduce(const Partials &p, double *out, double *hist_slot, hipStream_t s) {
-  hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, s, p.nblocks, p.sum, p.max, p.arg, out,
-                     hist_slot);
-}
-
-void launch_forces(int n_shape, const int32_t *m, const int32_t *l, const int32_t *r,
-                   const int32_t *shape, const double *base, const double *nx, const double *ny,
-                   const double *prim, int n, double pr_inf, double qdyn, double *out3,
+void launch_reduce(const Partials &p, int n_cases, const int32_t *seg, double *out,
+                   double *hist, int hist_stride, int slot, const int32_t *slot_ptr,
                    hipStream_t s) {
-  hipLaunchKernelGGL(k_forces, dim3(1), dim3(256), 0, s, n_shape, m, l, r, shape, base, nx, ny,
-                     prim, n, pr_inf, qdyn, out3);
+  hipLaunchKernelGGL(k_reduce, dim3(n_cases), dim3(256), 0, s, seg, p.nblocks, p.sum, p.max,
+                     p.arg, out, hist, hist_stride, slot, slot_ptr);
+}
+
+void launch_tick(int32_t *counter, hipStream_t s) {
+  hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, s, counter);
+}
+
+void launch_set_int(int32_t *p, int32_t v, hipStream_t s) {
+  hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, s, p, v);
+}
+
+void launch_copy(const double *src, double *dst, size_t count, hipStream_t s) {
+  const size_t n2 = count / 2;  // callers copy whole [4][n] double arrays
+  const int blocks = (int)std::min<size_t>((n2 + 255) / 256, 8192);
+  hipLaunchKernelGGL(k_copy, dim3(blocks > 0 ? blocks : 1), dim3(256), 0, s, n2,
+                     reinterpret_cast<const double2 *>(src), reinterpret_cast<double2 *>(dst));
+}
+
+void launch_forces(int n_cases, const int32_t *soff, const int32_t *m, const int32_t *l,
+                   const int32_t *r, const int32_t *shape, const double *base, const double *nx,
+                   const double *ny, const double *prim, int n, double pr_inf,
+                   const double *qdyn, double *out3, hipStream_t s) {
+  hipLaunchKernelGGL(k_forces, dim3(n_cases), dim3(256), 0, s, soff, m, l, r, shape, base, nx,
+                     ny, prim, n, pr_inf, qdyn, out3);
 }
 
 void launch_kat_fluxes(int n, const double *in6, double *out, hipStream_t s) {

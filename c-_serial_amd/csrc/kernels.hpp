@@ -84,16 +84,28 @@ void launch_flux(const DevCloud &c, const double *base, const double *dq, const 
 // also writes q of the new state, as FP32 into q32 when non-null (mixed
 // form), else into base when non-null, and, with speed, the func_delta
 // denominator into base
+// fcs[case] holds each case's flow constants; block_case[b] the case of
+// 256-point block b (NULL: one case)
 void launch_state_update(const DevCloud &c, int rk, double *prim, const double *prim_old,
-                         const double *flux_res, const FlowConsts &fc, const Partials &p,
-                         double *base, float *q32, bool speed, hipStream_t s);
-// reduces the partials into out[0] = sum, out[1] = max, out[2] = argmax
-// (caller index, as a double), and also hist_slot[0] = sum when non-null
-void launch_reduce(const Partials &p, double *out, double *hist_slot, hipStream_t s);
-void launch_forces(int n_shape, const int32_t *m, const int32_t *l, const int32_t *r,
-                   const int32_t *shape, const double *base, const double *nx,
-                   const double *ny, const double *prim, int n, double pr_inf, double qdyn,
-                   double *out3, hipStream_t s);
+                         const double *flux_res, const FlowConsts *fcs,
+                         const uint16_t *block_case, const Partials &p, double *base,
+                         float *q32, bool speed, hipStream_t s);
+// per case c (partials seg[c]..seg[c+1]-1, or all of them when seg is NULL):
+// out[3c] = sum, out[3c+1] = max, out[3c+2] = argmax (caller index, as a
+// double), and, when hist is non-null, hist[c*hist_stride + slot] = sum with
+// slot read from slot_ptr when that is non-null
+void launch_reduce(const Partials &p, int n_cases, const int32_t *seg, double *out,
+                   double *hist, int hist_stride, int slot, const int32_t *slot_ptr,
+                   hipStream_t s);
+void launch_tick(int32_t *counter, hipStream_t s);      // counter[0] += 1
+void launch_set_int(int32_t *p, int32_t v, hipStream_t s);
+// count doubles (even) from src to dst by a kernel (graph-safe)
+void launch_copy(const double *src, double *dst, size_t count, hipStream_t s);
+// per case c: H, V, pitch moment over shape entries soff[c]..soff[c+1]-1
+void launch_forces(int n_cases, const int32_t *soff, const int32_t *m, const int32_t *l,
+                   const int32_t *r, const int32_t *shape, const double *base, const double *nx,
+                   const double *ny, const double *prim, int n, double pr_inf,
+                   const double *qdyn, double *out3, hipStream_t s);
 void launch_kat_fluxes(int n, const double *in6, double *out, hipStream_t s);
 void launch_kat_qtilde(int n, const double *qt4, double *out, hipStream_t s);
 

@@ -127,7 +127,9 @@ __global__ __launch_bounds__(256) void k_q_derivatives_m(
     }
   }
   const double det = sum_delx_sqr * sum_dely_sqr - sum_delx_dely * sum_delx_dely;
-  const double one_by_det = 1.0 / det;
+  // det > 0 for every validated point; 0 only for a batch's padding points,
+  // which have no neighbours and must not produce NaNs
+  const double one_by_det = det != 0.0 ? 1.0 / det : 0.0;
   float dqx[4], dqy[4];
   for (int r = 0; r < 4; r++) {
     dqx[r] = (float)((sdx[r] * sum_dely_sqr - sdy[r] * sum_delx_dely) * one_by_det);
@@ -198,7 +200,9 @@ __global__ __launch_bounds__(256) void k_q_inner_m(
     }
   }
   const double det = sum_delx_sqr * sum_dely_sqr - sum_delx_dely * sum_delx_dely;
-  const double one_by_det = 1.0 / det;
+  // det > 0 for every validated point; 0 only for a batch's padding points,
+  // which have no neighbours and must not produce NaNs
+  const double one_by_det = det != 0.0 ? 1.0 / det : 0.0;
   float ox[4], oy[4];
   for (int r = 0; r < 4; r++) {
     ox[r] = (float)(one_by_det * (sx[r] * sum_dely_sqr - sy[r] * sum_delx_dely));

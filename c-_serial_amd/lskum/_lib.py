@@ -62,6 +62,14 @@ class Params(C.Structure):
     ]
 
 
+class Case(C.Structure):
+    _fields_ = [("cloud", C.c_int32), ("mach", C.c_double), ("aoa_deg", C.c_double)]
+
+
+class TuneResult(C.Structure):
+    _fields_ = [("tile_block", C.c_int32), ("lds_tiles", C.c_int32), ("ms_per_iter", C.c_double)]
+
+
 # (name, restype, argtypes)
 _SIGS = [
     ("lskum_cloud_generate", C.c_int, [C.POINTER(GenParams), C.POINTER(C.c_void_p)]),
@@ -74,6 +82,14 @@ _SIGS = [
     ("lskum_cloud_rcm", C.c_int, [C.c_void_p, _ip]),
     ("lskum_cloud_free", None, [C.c_void_p]),
     ("lskum_create", C.c_int, [C.POINTER(CloudView), C.POINTER(Params), C.POINTER(C.c_void_p)]),
+    ("lskum_create_batch", C.c_int, [C.POINTER(CloudView), C.c_int32, C.POINTER(Case),
+                                     C.c_int32, C.POINTER(Params), C.POINTER(C.c_void_p)]),
+    ("lskum_case_count", C.c_int, [C.c_void_p, _ip, _ip]),
+    ("lskum_case_status", C.c_int, [C.c_void_p, _ip]),
+    ("lskum_set_graph", C.c_int, [C.c_void_p, C.c_int32]),
+    ("lskum_autotune", C.c_int, [C.POINTER(CloudView), C.c_int32, C.POINTER(Case), C.c_int32,
+                                 C.POINTER(Params), C.c_int32, C.POINTER(TuneResult),
+                                 C.POINTER(TuneResult), C.c_int32, _ip]),
     ("lskum_destroy", None, [C.c_void_p]),
     ("lskum_last_error", C.c_char_p, [C.c_void_p]),
     ("lskum_set_state", C.c_int, [C.c_void_p, _dp]),

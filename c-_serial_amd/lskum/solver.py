@@ -187,6 +187,10 @@ class Solver:
     def set_iteration(self, it, res_old, restart=True):
         self._chk(L.lib().lskum_set_iteration(self._ctx, it, res_old, 1 if restart else 0))
 
+    def set_graph(self, iters_per_graph):
+        """Run iterations as captured HIP graphs of this many iterations (0: off)."""
+        self._chk(L.lib().lskum_set_graph(self._ctx, iters_per_graph))
+
     def layout(self):
         """(mask of kernels using LDS tiles, tile_max, order) from lskum_create."""
         t, m, o = C.c_int32(), C.c_int32(), C.c_int32()

@@ -19,6 +19,12 @@
  *  - Calls on one context are ordered on its HIP stream.  Only calls that
  *    return host scalars or host arrays synchronise.
  *  - One context per host thread; contexts are independent.
+ *  - A context runs n_cases independent cases: one from lskum_create, many
+ *    from lskum_create_batch.  Per-case outputs are arrays of n_cases
+ *    (case-major where they are per case and per iteration); per-point arrays
+ *    hold the cases one after the other, each in its own [comp][n_case]
+ *    layout and caller order.  With one case every array reduces to the
+ *    reference's shape.
  */
 #ifndef LSKUM_H
 #define LSKUM_H
@@ -160,6 +166,14 @@ typedef struct lskum_params {
 /* --------------------------------------------------------------- context */
 typedef struct lskum_ctx lskum_ctx;
 
+/* One case of a batch: a cloud (index into the clouds array) and its flow
+ * condition; the other numerics are the batch's lskum_params. */
+typedef struct lskum_case {
+  int32_t cloud;
+  double mach;
+  double aoa_deg;
+} lskum_case;
+
 /*
  * Copies the cloud to the device, computes normals (point_normals_mod.h:17-55)
  * and the x+/x-/y+/y- sub-stencils (generate_connectivity_mod.h:5-249) on the
@@ -169,10 +183,26 @@ typedef struct lskum_ctx lskum_ctx;
  */
 int lskum_create(const lskum_cloud *cloud, const lskum_params *params,
                  lskum_ctx **out);
+/*
+ * Many flow conditions over several clouds in one context (no reference
+ * counterpart: the reference runs one case per process).  Every stage call
+ * and lskum_iterate advance all cases together, one kernel launch per stage
+ * for the whole batch; each case's arithmetic is that of a context of its
+ * own (bit-identical results).  params->mach / aoa_deg are ignored in favour
+ * of the cases'.
+ */
+int lskum_create_batch(const lskum_cloud *clouds, int32_t n_clouds,
+                       const lskum_case *cases, int32_t n_cases,
+                       const lskum_params *params, lskum_ctx **out);
+/* Number of cases and each case's point count (points may be NULL). */
+int lskum_case_count(lskum_ctx *ctx, int32_t *n_cases, int32_t *points);
+/* Per case, the first iteration whose residual was non-finite (0: none). */
+int lskum_case_status(lskum_ctx *ctx, int32_t *failed_at);
 void lskum_destroy(lskum_ctx *ctx);
 const char *lskum_last_error(const lskum_ctx *ctx);
 
-/* prim is [4][n]: rho, u1, u2, p (point.prim, data_structure_mod.h:26). */
+/* prim is [4][n] per case: rho, u1, u2, p (point.prim,
+ * data_structure_mod.h:26). */
 int lskum_set_state(lskum_ctx *ctx, const double *prim);
 int lskum_get_state(lskum_ctx *ctx, double *prim);
 
@@ -189,12 +219,13 @@ int lskum_q_derivatives(lskum_ctx *ctx);
 int lskum_q_inner(lskum_ctx *ctx, int32_t n_inner);
 /* flux_res (flux_residual_mod.h:10 cal_flux_residual). */
 int lskum_flux_residual(lskum_ctx *ctx);
-/* SSPRK stage rk in 1..4 (state_update_mod.h:92 state_update); returns the
- * globals sum_res_sqr, max_res, max_res_point (0-based) it sets.  Any
- * pointer may be NULL (then the call does not synchronise). */
+/* SSPRK stage rk in 1..4 (state_update_mod.h:92 state_update); returns,
+ * per case, the globals sum_res_sqr, max_res, max_res_point (0-based) it
+ * sets.  Any pointer may be NULL (then the call does not synchronise). */
 int lskum_state_update(lskum_ctx *ctx, int32_t rk, double *sum_res_sqr,
                        double *max_res, int32_t *max_res_point);
-/* Cl, Cd, Cm of shape 1 (compute_force_coeffs_mod.h:7 compute_cl_cd_cm). */
+/* Cl, Cd, Cm of shape 1 per case (compute_force_coeffs_mod.h:7
+ * compute_cl_cd_cm). */
 int lskum_forces(lskum_ctx *ctx, double *cl, double *cd, double *cm);
 
 /*
@@ -204,12 +235,23 @@ int lskum_forces(lskum_ctx *ctx, double *cl, double *cd, double *cm);
  * numbered it0+1 .. it0+n_iter (q_lskum_mod.h:33).  res_new[i] =
  * sqrt(sum_res_sqr)/n and residue[i] = log10(res_new/res_old) with res_old
  * latched at it <= 2 (fpi_solver_mod.h:138-146), unless restart is set.
- * Either output may be NULL.
+ * Outputs are [n_cases][n_iter]; either may be NULL.  A non-finite residual
+ * fails a single-case context with LSKUM_ENONFINITE; in a batch the case
+ * goes on as NaN without touching the others (no case shares points,
+ * stencils or reductions with another) and lskum_case_status reports it.
  */
 int lskum_iterate(lskum_ctx *ctx, int32_t n_iter, double *res_new,
                   double *residue);
-/* Iteration counter and res_old, for restart (q_lskum_mod.h:29-37,
- * fpi_solver_mod.h:139). restart != 0 keeps res_old across it <= 2. */
+/*
+ * Run whole iterations as captured HIP graphs of iters_per_graph iterations
+ * each (0: launch every kernel from the host, the default).  Graphs are
+ * captured on first use and replayed; kernel timing (lskum_set_timing)
+ * runs the launches from the host instead.
+ */
+int lskum_set_graph(lskum_ctx *ctx, int32_t iters_per_graph);
+/* Iteration counter and res_old (per case), for restart
+ * (q_lskum_mod.h:29-37, fpi_solver_mod.h:139); set_iteration gives every
+ * case the same res_old.  restart != 0 keeps res_old across it <= 2. */
 int lskum_get_iteration(lskum_ctx *ctx, int32_t *it, double *res_old);
 int lskum_set_iteration(lskum_ctx *ctx, int32_t it, double res_old,
                         int32_t restart);
@@ -224,11 +266,35 @@ int lskum_set_iteration(lskum_ctx *ctx, int32_t it, double res_old,
 #define LSKUM_A_FLUX_RES 6 /* [4][n]                      */
 #define LSKUM_A_NORMALS 7  /* [2][n]: nx, ny              */
 int lskum_get_array(lskum_ctx *ctx, int32_t which, double *host_dst);
-/* Sub-stencil lists as made by generate_connectivity, caller order.
+/* Sub-stencil lists as made by generate_connectivity, caller order (of the
+ * first case's cloud).
  * which: 0 xpos, 1 xneg, 2 ypos, 3 yneg.  counts[n]; members are written
  * to conn_out in point order, conn order within a point. */
 int lskum_get_substencil(lskum_ctx *ctx, int32_t which, int32_t *counts,
                          int32_t *members);
+
+/*
+ * Autotuner (no reference counterpart): builds the given cases once per
+ * candidate layout -- points per block of the stencil kernels 64, 128 or
+ * 256 (the block shape; one point per thread, whose neighbour list the
+ * thread walks), and the flux's neighbour data staged in LDS or read
+ * through L2 -- times one captured graph of `iters` whole iterations after
+ * a warm-up replay, and returns the fastest in *best.  Per-point arithmetic
+ * does not depend on the layout, so the choice changes speed only.
+ * trials (up to max_trials entries) receives every candidate; lds_tiles is
+ * the mask that fitted.  Pass a representative subset of a large batch (for
+ * example one case per cloud): the set-up runs once per candidate.
+ */
+typedef struct lskum_tune_result {
+  int32_t tile_block;
+  int32_t lds_tiles;
+  double ms_per_iter;
+} lskum_tune_result;
+int lskum_autotune(const lskum_cloud *clouds, int32_t n_clouds,
+                   const lskum_case *cases, int32_t n_cases,
+                   const lskum_params *params, int32_t iters,
+                   lskum_tune_result *best, lskum_tune_result *trials,
+                   int32_t max_trials, int32_t *n_trials);
 
 /* Stream the context's kernels run on (a hipStream_t). */
 void *lskum_stream(lskum_ctx *ctx);

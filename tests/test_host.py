@@ -215,3 +215,84 @@ def test_create_rejects_out_of_range_neighbours():
     bad = lskum.Cloud(c.x, c.y, c.left, c.right, c.flag_1, c.flag_2, c.min_dist, c.conn_off, conn)
     rc, _ = _create(bad, _params())
     assert rc == -1
+
+
+def _inside_wall_polygon(c, pts):
+    nw = int((c.flag_1 == 0).sum())
+    ax, ay = c.x[:nw], c.y[:nw]
+    bx, by = np.roll(ax, -1), np.roll(ay, -1)
+    inside = []
+    for i in pts:
+        px, py = c.x[i], c.y[i]
+        span = (ay > py) != (by > py)
+        xc = ax + (py - ay) * (bx - ax) / np.where(span, by - ay, 1.0)
+        inside.append(int(np.count_nonzero(span & (px < xc))) % 2 == 1)
+    return np.array(inside)
+
+
+def _used_rel_dets(c):
+    """Per point, the smallest det/(sxx*syy) over the sub-stencils the flux
+    uses (interior: x+, x-, y+, y-; wall and outer: x+, x-)."""
+    nx, ny, mask = c.connectivity()
+    n = c.n
+    pt = np.repeat(np.arange(n), np.diff(c.conn_off))
+    k = c.conn
+    dx, dy = c.x[k] - c.x[pt], c.y[k] - c.y[pt]
+    ds = dx * ny[pt] - dy * nx[pt]
+    dn = dx * nx[pt] + dy * ny[pt]
+    out = np.full(n, np.inf)
+    for sb in range(4):
+        m = (mask & (1 << sb)) != 0
+        sxx = np.bincount(pt[m], ds[m] * ds[m], n)
+        syy = np.bincount(pt[m], dn[m] * dn[m], n)
+        sxy = np.bincount(pt[m], ds[m] * dn[m], n)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            r = (sxx * syy - sxy * sxy) / (sxx * syy)
+        used = np.ones(n, bool) if sb < 2 else c.flag_1 == 1
+        out = np.where(used, np.minimum(out, r), out)
+    return out
+
+
+def test_cambered_cloud_has_no_points_inside_the_body():
+    # a cambered section's lower surface descends toward the trailing edge,
+    # where blending toward the far field would put first-ring points inside
+    c = lskum.Cloud.generate(naca="4612", n_wall=180, n_radial=45, n_circ=450, stretch=0.0,
+                             k=20, seed=4007)
+    nw = 180
+    box = ((c.x >= c.x[:nw].min()) & (c.x <= c.x[:nw].max()) & (c.y >= c.y[:nw].min()) &
+           (c.y <= c.y[:nw].max()) & (c.flag_1 == 1))
+    pts = np.nonzero(box)[0]
+    assert len(pts) > 0
+    assert not _inside_wall_polygon(c, pts).any()
+
+
+def test_cambered_cloud_substencils_meet_the_balance_target():
+    c = lskum.Cloud.generate(naca="4612", n_wall=180, n_radial=45, n_circ=450, stretch=0.0,
+                             k=20, seed=4007)
+    assert _used_rel_dets(c).min() >= 0.05 * (1 - 1e-9)
+
+
+def test_create_batch_validates_cases_and_names_the_cloud():
+    a = lskum.workload("naca0012_1480_k20")
+    b = lskum.workload("naca0012_3620_k24")
+    conn = b.conn.copy()
+    conn[10] = b.n + 5
+    bad = lskum.Cloud(b.x, b.y, b.left, b.right, b.flag_1, b.flag_2, b.min_dist, b.conn_off, conn)
+
+    def create(clouds, cases):
+        views = (L.CloudView * len(clouds))(*[c.view() for c in clouds])
+        cs = (L.Case * len(cases))(*[L.Case(*x) for x in cases])
+        ctx = C.c_void_p()
+        rc = L.lib().lskum_create_batch(views, len(clouds), cs, len(cases),
+                                        C.byref(_params()), C.byref(ctx))
+        msg = L.lib().lskum_last_error(ctx).decode() if ctx else ""
+        if ctx:
+            L.lib().lskum_destroy(ctx)
+        return rc, msg
+
+    rc, msg = create([a], [(0, 0.8, 1.0), (1, 0.8, 1.0)])
+    assert rc == -1 and "case 1" in msg
+    rc, msg = create([a], [(0, -0.5, 1.0)])
+    assert rc == -1 and "case 0" in msg
+    rc, msg = create([a, bad], [(0, 0.8, 1.0), (1, 0.7, 2.0)])
+    assert rc == -1 and "cloud 1" in msg
