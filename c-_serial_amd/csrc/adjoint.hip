@@ -1,0 +1,567 @@
+// adjoint.hip -- discrete adjoint (reverse) and tangent-linear kernels of the
+// flux residual with respect to the primitive state, FP64, power = 0.
+//
+// The map differentiated is one cal_flux_residual evaluation as a function
+// of prim, with the local time step delta held fixed (flux_residual_mod.h:
+// 10-89 after q_variables_mod.h:9-228):
+//   prim -> q (eval_q_variables) -> dq, qmax/qmin (eval_q_derivatives)
+//        -> n inner passes (eval_q_inner_loop) -> flux_res.
+// The reverse pass recomputes q, dq and qmax/qmin from prim (the checkpoint
+// is prim alone; nothing of a forward run is stored) and then runs each
+// stage backwards:
+//  - flux: per (point, neighbour) pair, the 4x4 Jacobians of the summed
+//    sub-stencil fluxes with respect to q~ at each end, by forward-mode dual
+//    numbers through the same templated leaf physics the forward kernel
+//    uses, then the limiter and reconstruction partials by hand.  The
+//    neighbour's share of each pair goes to a per-edge buffer and a
+//    transpose-CSR gather sums it per point in a fixed order (no atomics,
+//    deterministic).
+//  - qmax/qmin: routed to the point each extremum came from.
+//  - inner passes and q_derivatives: linear least-squares stencils, whose
+//    transposes are applied by the same gather pattern.
+//  - q_variables: the pointwise 4x4 Jacobian transposed.
+// The tangent-linear map runs the same chain forwards (the linear stencils
+// through the forward kernels themselves) for the duality check.
+#include <hip/hip_runtime.h>
+
+#include "device.hpp"
+#include "kernels.hpp"
+#include "physics.hpp"
+// after physics.hpp: the physics templates look up exp/log/sqrt/erf for
+// double at their definition and find the Dual overloads by argument
+#include "dual.hpp"
+
+namespace lskum {
+
+namespace {
+
+constexpr int COEF_STRIDE = 8;
+
+// limiters_mod.h:55-104 for one component (del_neg outside the threshold)
+// with d phi/d del_neg (pn) and d phi/d del_pos (pp); phi = 1 and zero
+// partials where the ratio reaches 1.
+__device__ __forceinline__ double venkat_phi_d(double N, double P, double eps2, double &pn,
+                                               double &pp) {
+  pn = pp = 0.0;
+  const double num = (P * P + eps2) * N + 2.0 * N * N * P;
+  const double den = (P * P + 2.0 * N * N + N * P + eps2) * N;
+  const double t = num / den;
+  if (!(t < 1.0)) return 1.0;
+  const double num_n = P * P + eps2 + 4.0 * N * P;
+  const double den_n = 6.0 * N * N + 2.0 * N * P + P * P + eps2;
+  const double num_p = 2.0 * P * N + 2.0 * N * N;
+  const double den_p = (2.0 * P + N) * N;
+  const double inv2 = 1.0 / (den * den);
+  pn = (num_n * den - num * den_n) * inv2;
+  pp = (num_p * den - num * den_p) * inv2;
+  return t;
+}
+
+// q~ = q - 0.5 phi d (interior_fluxes_mod.h:72-83) with its partials with
+// respect to q, the increment d = dx.dq, and the qmax or qmin it used.
+struct Recon {
+  double qt, dq, dd, dm;
+  bool use_max;
+};
+
+__device__ __forceinline__ Recon recon(double q, double d, double qmax, double qmin,
+                                       double eps2) {
+  Recon o;
+  const double N = (q - 0.5 * d) - q;
+  double phi = 1.0, pn = 0.0, pp = 0.0;
+  o.use_max = N > 0.0;
+  if (fabs(N) > 10e-6) phi = venkat_phi_d(N, (N > 0.0 ? qmax : qmin) - q, eps2, pn, pp);
+  o.qt = q - 0.5 * phi * d;
+  o.dq = 1.0 + 0.5 * d * pp;           // through del_pos = qm - q
+  o.dd = -0.5 * phi + 0.25 * d * pn;   // del_neg = -0.5 d
+  o.dm = -0.5 * d * pp;
+  return o;
+}
+
+// Sum over the sub-stencil fluxes a (point, neighbour) pair feeds of
+// w_s G_s(e), for one end's terms e: the selection of the FP64 flux kernel
+// (x+ or x-, both on a tie; interior y+ / y-, wall y-, outer y+).
+template <typename T>
+__device__ __forceinline__ void end_sum(const EndTerms<T> &e, int type, uint32_t m, double s2b,
+                                        const double cf[8], double dsw, double dnw, T out[4]) {
+  T G[4];
+  const bool xp = (m & XPOS) != 0;
+  if (type == 1)
+    split_x(e, T(xp ? 1.0 : -1.0), G);
+  else
+    quad_x(e, T(xp ? 1.0 : -1.0), T(s2b), G);
+  const double wx = xp ? cf[0] * dsw + cf[1] * dnw : cf[2] * dsw + cf[3] * dnw;
+  for (int r = 0; r < 4; r++) out[r] = G[r] * wx;
+  if (xp && (m & XNEG)) {
+    if (type == 1)
+      split_x(e, T(-1.0), G);
+    else
+      quad_x(e, T(-1.0), T(s2b), G);
+    const double wn = cf[2] * dsw + cf[3] * dnw;
+    for (int r = 0; r < 4; r++) out[r] = out[r] + G[r] * wn;
+  }
+  const bool yp = type == 1 ? (m & YPOS) != 0 : type == 2;
+  split_y(e, T(yp ? 1.0 : -1.0), G);
+  const double wy = yp ? cf[4] * dnw + cf[5] * dsw : cf[6] * dnw + cf[7] * dsw;
+  for (int r = 0; r < 4; r++) out[r] = out[r] + G[r] * wy;
+  if (type == 1 && yp && (m & YNEG)) {
+    split_y(e, T(-1.0), G);
+    const double wn = cf[6] * dnw + cf[7] * dsw;
+    for (int r = 0; r < 4; r++) out[r] = out[r] + G[r] * wn;
+  }
+}
+
+// end_sum for the state q~ with N seeded directions: the value and, per
+// direction, its derivative along the seed.
+template <int N>
+__device__ __forceinline__ void end_sum_dual(const Dual<N> qt[4], double nx, double ny, int type,
+                                             uint32_t m, double s2b, const double cf[8],
+                                             double dsw, double dnw, Dual<N> out[4]) {
+  Dual<N> u1, u2, rho, beta, temp;
+  qtilde_state(qt, u1, u2, rho, beta, temp);
+  const EndTerms<Dual<N>> e = end_terms_bt(Dual<N>(nx), Dual<N>(ny), u1, u2, rho, beta, temp);
+  end_sum(e, type, m, s2b, cf, dsw, dnw, out);
+}
+
+__device__ __forceinline__ void load8(const double *p, double a[4], double b[4]) {
+  load4(p, a);
+  load4(p + 4, b);
+}
+
+// ------------------------------------------------------------ extremum origin
+// the point each qmax/qmin component came from (q_variables_mod.h:60-80:
+// the point itself, replaced by a neighbour only on a strictly larger /
+// smaller value, in conn order): arg[8i + r] max, arg[8i + 4 + r] min
+__global__ __launch_bounds__(256) void k_argext(int n, const int32_t *__restrict__ off,
+                                                const uint32_t *__restrict__ conn,
+                                                const double *__restrict__ base,
+                                                int32_t *__restrict__ arg) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double qi[4], mx[4], mn[4];
+  int am[4], an[4];
+  load4(base + (size_t)B_STRIDE * i + B_Q, qi);
+  for (int r = 0; r < 4; r++) {
+    mx[r] = mn[r] = qi[r];
+    am[r] = an[r] = i;
+  }
+  for (int e = off[i]; e < off[i + 1]; e++) {
+    const int k = (int)(conn[e] & kIndexMask);
+    double qk[4];
+    load4(base + (size_t)B_STRIDE * k + B_Q, qk);
+    for (int r = 0; r < 4; r++) {
+      if (qk[r] > mx[r]) { mx[r] = qk[r]; am[r] = k; }
+      if (qk[r] < mn[r]) { mn[r] = qk[r]; an[r] = k; }
+    }
+  }
+  for (int r = 0; r < 4; r++) {
+    arg[8 * (size_t)i + r] = am[r];
+    arg[8 * (size_t)i + 4 + r] = an[r];
+  }
+}
+
+// ------------------------------------------------------------ flux, reverse
+// Per point i, over its neighbours: the adjoint of q~ at both ends of each
+// pair, pushed through the reconstruction.  i's own share accumulates in
+// registers (own_q, own_dq, own_qm); the neighbour's goes to the edge's
+// record: c_q[4], c_d[4] (per unit of d, times dx/dy at the gather),
+// c_max[4], c_min[4].
+__global__ __launch_bounds__(256) void k_flux_vjp(
+    int n, const int32_t *__restrict__ off, const uint32_t *__restrict__ conn,
+    const double *__restrict__ base, const double *__restrict__ dq,
+    const double *__restrict__ qm, const double *__restrict__ coef,
+    const double *__restrict__ nxa, const double *__restrict__ nya,
+    const uint8_t *__restrict__ flag, const double *__restrict__ delta,
+    const double *__restrict__ lam, double *__restrict__ own_q, double *__restrict__ own_dq,
+    double *__restrict__ own_qm, double *__restrict__ edge) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int type = flag[i];
+  // flux_res = c delta acc with c = 2 on the wall (flux_residual_mod.h:29)
+  const double sc = (type == 0 ? 2.0 : 1.0) * delta[i];
+  double la[4];
+  for (int r = 0; r < 4; r++) la[r] = sc * lam[(size_t)r * n + i];
+  const double *hi = base + (size_t)B_STRIDE * i;
+  const double x_i = hi[B_X], y_i = hi[B_Y], eps2_i = hi[B_EPS2];
+  double qi[4], dxi[4], dyi[4], qmxi[4], qmni[4], cf[8];
+  load4(hi + B_Q, qi);
+  load8(dq + (size_t)DQ_STRIDE * i, dxi, dyi);
+  load8(qm + (size_t)QM_STRIDE * i, qmxi, qmni);
+  load8(coef + (size_t)COEF_STRIDE * i, cf, cf + 4);
+  const double nx = nxa[i], ny = nya[i], tx = ny, ty = -nx;
+  const double s2b = type == 0 ? -1.0 : 1.0;
+  double aq[4] = {0, 0, 0, 0}, adx[4] = {0, 0, 0, 0}, ady[4] = {0, 0, 0, 0};
+  double amx[4] = {0, 0, 0, 0}, amn[4] = {0, 0, 0, 0};
+  for (int e = off[i]; e < off[i + 1]; e++) {
+    const uint32_t w = conn[e];
+    const int k = (int)(w & kIndexMask);
+    const uint32_t m = w >> kMaskShift;
+    const double *hk = base + (size_t)B_STRIDE * k;
+    double qk[4], dxk[4], dyk[4], qmxk[4], qmnk[4];
+    load4(hk + B_Q, qk);
+    load8(dq + (size_t)DQ_STRIDE * k, dxk, dyk);
+    load8(qm + (size_t)QM_STRIDE * k, qmxk, qmnk);
+    const double delx = hk[B_X] - x_i, dely = hk[B_Y] - y_i;
+    const double dels = delx * tx + dely * ty, deln = delx * nx + dely * ny;
+    Recon Ri[4], Rk[4];
+    for (int r = 0; r < 4; r++) {
+      Ri[r] = recon(qi[r], delx * dxi[r] + dely * dyi[r], qmxi[r], qmni[r], eps2_i);
+      Rk[r] = recon(qk[r], delx * dxk[r] + dely * dyk[r], qmxk[r], qmnk[r], hk[B_EPS2]);
+    }
+    // acc += end_sum(k end) - end_sum(i end): the adjoint of q~ at each end
+    double a_i[4], a_k[4];
+    {
+      Dual<4> qt[4], S[4];
+      for (int r = 0; r < 4; r++) {
+        qt[r] = Dual<4>(Ri[r].qt);
+        qt[r].d[r] = 1.0;
+      }
+      end_sum_dual(qt, nx, ny, type, m, s2b, cf, dels, deln, S);
+      for (int c = 0; c < 4; c++) {
+        double t = 0.0;
+        for (int r = 0; r < 4; r++) t += la[r] * S[r].d[c];
+        a_i[c] = -t;
+      }
+    }
+    {
+      Dual<4> qt[4], S[4];
+      for (int r = 0; r < 4; r++) {
+        qt[r] = Dual<4>(Rk[r].qt);
+        qt[r].d[r] = 1.0;
+      }
+      end_sum_dual(qt, nx, ny, type, m, s2b, cf, dels, deln, S);
+      for (int c = 0; c < 4; c++) {
+        double t = 0.0;
+        for (int r = 0; r < 4; r++) t += la[r] * S[r].d[c];
+        a_k[c] = t;
+      }
+    }
+    double *rec = edge + 16 * (size_t)e;
+    for (int r = 0; r < 4; r++) {
+      aq[r] += a_i[r] * Ri[r].dq;
+      const double ad = a_i[r] * Ri[r].dd;
+      adx[r] += ad * delx;
+      ady[r] += ad * dely;
+      if (Ri[r].use_max)
+        amx[r] += a_i[r] * Ri[r].dm;
+      else
+        amn[r] += a_i[r] * Ri[r].dm;
+      rec[r] = a_k[r] * Rk[r].dq;
+      rec[4 + r] = a_k[r] * Rk[r].dd;
+      rec[8 + r] = Rk[r].use_max ? a_k[r] * Rk[r].dm : 0.0;
+      rec[12 + r] = Rk[r].use_max ? 0.0 : a_k[r] * Rk[r].dm;
+    }
+  }
+  store4(own_q + 4 * (size_t)i, aq);
+  store4(own_dq + 8 * (size_t)i, adx);
+  store4(own_dq + 8 * (size_t)i + 4, ady);
+  store4(own_qm + 8 * (size_t)i, amx);
+  store4(own_qm + 8 * (size_t)i + 4, amn);
+}
+
+// Per point j: its own share plus the edge records of every pair in which
+// it is the neighbour (transpose CSR, owners ascending).
+__global__ __launch_bounds__(256) void k_flux_vjp_gather(
+    int n, const int32_t *__restrict__ tr_off, const int32_t *__restrict__ tr_edge,
+    const int32_t *__restrict__ tr_owner, const double *__restrict__ base,
+    const double *__restrict__ edge, const double *__restrict__ own_q,
+    const double *__restrict__ own_dq, const double *__restrict__ own_qm,
+    double *__restrict__ lq, double *__restrict__ ldq, double *__restrict__ lqm) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  double aq[4], adx[4], ady[4], amx[4], amn[4];
+  load4(own_q + 4 * (size_t)j, aq);
+  load8(own_dq + 8 * (size_t)j, adx, ady);
+  load8(own_qm + 8 * (size_t)j, amx, amn);
+  const double x_j = base[(size_t)B_STRIDE * j + B_X], y_j = base[(size_t)B_STRIDE * j + B_Y];
+  for (int t = tr_off[j]; t < tr_off[j + 1]; t++) {
+    const int i = tr_owner[t];
+    const double *rec = edge + 16 * (size_t)tr_edge[t];
+    const double delx = x_j - base[(size_t)B_STRIDE * i + B_X];
+    const double dely = y_j - base[(size_t)B_STRIDE * i + B_Y];
+    for (int r = 0; r < 4; r++) {
+      aq[r] += rec[r];
+      adx[r] += rec[4 + r] * delx;
+      ady[r] += rec[4 + r] * dely;
+      amx[r] += rec[8 + r];
+      amn[r] += rec[12 + r];
+    }
+  }
+  store4(lq + 4 * (size_t)j, aq);
+  store4(ldq + 8 * (size_t)j, adx);
+  store4(ldq + 8 * (size_t)j + 4, ady);
+  store4(lqm + 8 * (size_t)j, amx);
+  store4(lqm + 8 * (size_t)j + 4, amn);
+}
+
+// ------------------------------------------------------------ LS stencils, reverse
+// ls_i = M_i^-T lo_i: the adjoint of the sums (sx, sy) a least-squares pass
+// solves, from the adjoint of its output (q_variables_mod.h:94-112 and
+// :176-210 solve with the same 2x2 matrix of the point's geometry).
+__global__ __launch_bounds__(256) void k_ls_adj(int n, const int32_t *__restrict__ off,
+                                                const uint32_t *__restrict__ conn,
+                                                const double *__restrict__ base,
+                                                const double *__restrict__ lo,
+                                                double *__restrict__ ls) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double x_i = base[(size_t)B_STRIDE * i + B_X], y_i = base[(size_t)B_STRIDE * i + B_Y];
+  double sxx = 0.0, syy = 0.0, sxy = 0.0;
+  for (int e = off[i]; e < off[i + 1]; e++) {
+    const int k = (int)(conn[e] & kIndexMask);
+    const double delx = base[(size_t)B_STRIDE * k + B_X] - x_i;
+    const double dely = base[(size_t)B_STRIDE * k + B_Y] - y_i;
+    sxx = sxx + delx * delx;
+    syy = syy + dely * dely;
+    sxy = sxy + delx * dely;
+  }
+  const double det = sxx * syy - sxy * sxy;
+  const double inv = det != 0.0 ? 1.0 / det : 0.0;
+  double ox[4], oy[4], sx[4], sy[4];
+  load8(lo + 8 * (size_t)i, ox, oy);
+  for (int r = 0; r < 4; r++) {
+    sx[r] = (syy * ox[r] - sxy * oy[r]) * inv;
+    sy[r] = (sxx * oy[r] - sxy * ox[r]) * inv;
+  }
+  store4(ls + 8 * (size_t)i, sx);
+  store4(ls + 8 * (size_t)i + 4, sy);
+}
+
+// Per point j, from the sums' adjoints ls of every point: the adjoint of
+// q (added to lq) and, for an inner pass (with_dq), of the derivatives the
+// pass read (written to ldq_in).  Each pair (i, k) contributes
+// t = dx.ls_i per component to q_k and -t to q_i, and for an inner pass
+// -0.5 dx t to dq_k and +0.5 dx t to dq_i.  For q_derivatives (with_ext)
+// the adjoints of qmax/qmin are routed to the point each came from.
+__global__ __launch_bounds__(256) void k_ls_adj_gather(
+    int n, const int32_t *__restrict__ off, const uint32_t *__restrict__ conn,
+    const int32_t *__restrict__ tr_off, const int32_t *__restrict__ tr_owner,
+    const double *__restrict__ base, const double *__restrict__ ls, int with_dq,
+    double *__restrict__ ldq_in, const int32_t *__restrict__ arg,
+    const double *__restrict__ lqm, double *__restrict__ lq) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const double x_j = base[(size_t)B_STRIDE * j + B_X], y_j = base[(size_t)B_STRIDE * j + B_Y];
+  double sx[4], sy[4], aq[4], adx[4] = {0, 0, 0, 0}, ady[4] = {0, 0, 0, 0};
+  load8(ls + 8 * (size_t)j, sx, sy);
+  load4(lq + 4 * (size_t)j, aq);
+  // pairs (j, k) of j's own list
+  for (int e = off[j]; e < off[j + 1]; e++) {
+    const int k = (int)(conn[e] & kIndexMask);
+    const double delx = base[(size_t)B_STRIDE * k + B_X] - x_j;
+    const double dely = base[(size_t)B_STRIDE * k + B_Y] - y_j;
+    for (int r = 0; r < 4; r++) {
+      const double t = delx * sx[r] + dely * sy[r];
+      aq[r] -= t;
+      adx[r] += 0.5 * delx * t;
+      ady[r] += 0.5 * dely * t;
+    }
+  }
+  // pairs (i, j) in which j is the neighbour
+  int prev = -1;
+  for (int tt = tr_off[j]; tt < tr_off[j + 1]; tt++) {
+    const int i = tr_owner[tt];
+    const double delx = x_j - base[(size_t)B_STRIDE * i + B_X];
+    const double dely = y_j - base[(size_t)B_STRIDE * i + B_Y];
+    double si[4], ti[4];
+    load8(ls + 8 * (size_t)i, si, ti);
+    for (int r = 0; r < 4; r++) {
+      const double t = delx * si[r] + dely * ti[r];
+      aq[r] += t;
+      adx[r] -= 0.5 * delx * t;
+      ady[r] -= 0.5 * dely * t;
+    }
+    if (arg && i != prev) {  // a repeated neighbour entry routes an extremum once
+      for (int r = 0; r < 8; r++)
+        if (arg[8 * (size_t)i + r] == j) aq[r & 3] += lqm[8 * (size_t)i + r];
+    }
+    prev = i;
+  }
+  if (arg)
+    for (int r = 0; r < 8; r++)
+      if (arg[8 * (size_t)j + r] == j) aq[r & 3] += lqm[8 * (size_t)j + r];
+  store4(lq + 4 * (size_t)j, aq);
+  if (with_dq) {
+    store4(ldq_in + 8 * (size_t)j, adx);
+    store4(ldq_in + 8 * (size_t)j + 4, ady);
+  }
+}
+
+// ------------------------------------------------------------ q variables
+// d q / d prim (q_variables_mod.h:9-34) at one state, prim order (rho, u1,
+// u2, p): J[r][c] = d q_r / d prim_c.
+__device__ __forceinline__ void qvar_jac(const double p[4], double J[4][4]) {
+  const double rho = p[0], u1 = p[1], u2 = p[2], pr = p[3];
+  const double beta = 0.5 * rho / pr, uu = u1 * u1 + u2 * u2;
+  J[0][0] = (3.5 - beta * uu) / rho;
+  J[0][1] = -2.0 * beta * u1;
+  J[0][2] = -2.0 * beta * u2;
+  J[0][3] = (beta * uu - 2.5) / pr;
+  J[1][0] = 2.0 * beta * u1 / rho;
+  J[1][1] = 2.0 * beta;
+  J[1][2] = 0.0;
+  J[1][3] = -2.0 * beta * u1 / pr;
+  J[2][0] = 2.0 * beta * u2 / rho;
+  J[2][1] = 0.0;
+  J[2][2] = 2.0 * beta;
+  J[2][3] = -2.0 * beta * u2 / pr;
+  J[3][0] = -2.0 * beta / rho;
+  J[3][1] = 0.0;
+  J[3][2] = 0.0;
+  J[3][3] = 2.0 * beta / pr;
+}
+
+__global__ __launch_bounds__(256) void k_qvar_vjp(int n, const double *__restrict__ prim,
+                                                  const double *__restrict__ lq,
+                                                  double *__restrict__ grad) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double p[4], J[4][4], a[4];
+  for (int c = 0; c < 4; c++) p[c] = prim[(size_t)c * n + i];
+  qvar_jac(p, J);
+  load4(lq + 4 * (size_t)i, a);
+  for (int c = 0; c < 4; c++)
+    grad[(size_t)c * n + i] = J[0][c] * a[0] + J[1][c] * a[1] + J[2][c] * a[2] + J[3][c] * a[3];
+}
+
+// ------------------------------------------------------------ tangent-linear
+// q tangent into the q slots of a copy of the base records (so that the
+// forward least-squares kernels apply the linear stencils to it)
+__global__ __launch_bounds__(256) void k_qvar_jvp(int n, const double *__restrict__ prim,
+                                                  const double *__restrict__ v,
+                                                  double *__restrict__ tbase) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double p[4], J[4][4], t[4];
+  for (int c = 0; c < 4; c++) p[c] = prim[(size_t)c * n + i];
+  qvar_jac(p, J);
+  for (int r = 0; r < 4; r++) {
+    t[r] = 0.0;
+    for (int c = 0; c < 4; c++) t[r] += J[r][c] * v[(size_t)c * n + i];
+  }
+  store4(tbase + (size_t)B_STRIDE * i + B_Q, t);
+}
+
+// qmax/qmin tangents: the tangent of the point each extremum came from
+__global__ __launch_bounds__(256) void k_ext_jvp(int n, const int32_t *__restrict__ arg,
+                                                 const double *__restrict__ tbase,
+                                                 double *__restrict__ tqm) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int r = 0; r < 8; r++)
+    tqm[8 * (size_t)i + r] = tbase[(size_t)B_STRIDE * arg[8 * (size_t)i + r] + B_Q + (r & 3)];
+}
+
+// flux tangent: per pair, the tangent of q~ at each end through the
+// reconstruction, then one dual direction through the summed fluxes
+__global__ __launch_bounds__(256) void k_flux_jvp(
+    int n, const int32_t *__restrict__ off, const uint32_t *__restrict__ conn,
+    const double *__restrict__ base, const double *__restrict__ dq,
+    const double *__restrict__ qm, const double *__restrict__ coef,
+    const double *__restrict__ nxa, const double *__restrict__ nya,
+    const uint8_t *__restrict__ flag, const double *__restrict__ delta,
+    const double *__restrict__ tbase, const double *__restrict__ tdq,
+    const double *__restrict__ tqm, double *__restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int type = flag[i];
+  const double *hi = base + (size_t)B_STRIDE * i;
+  const double x_i = hi[B_X], y_i = hi[B_Y], eps2_i = hi[B_EPS2];
+  double qi[4], dxi[4], dyi[4], qmxi[4], qmni[4], cf[8];
+  double tqi[4], tdxi[4], tdyi[4], tmxi[4], tmni[4];
+  load4(hi + B_Q, qi);
+  load8(dq + (size_t)DQ_STRIDE * i, dxi, dyi);
+  load8(qm + (size_t)QM_STRIDE * i, qmxi, qmni);
+  load8(coef + (size_t)COEF_STRIDE * i, cf, cf + 4);
+  load4(tbase + (size_t)B_STRIDE * i + B_Q, tqi);
+  load8(tdq + (size_t)DQ_STRIDE * i, tdxi, tdyi);
+  load8(tqm + 8 * (size_t)i, tmxi, tmni);
+  const double nx = nxa[i], ny = nya[i], tx = ny, ty = -nx;
+  const double s2b = type == 0 ? -1.0 : 1.0;
+  double acc[4] = {0, 0, 0, 0};
+  for (int e = off[i]; e < off[i + 1]; e++) {
+    const uint32_t w = conn[e];
+    const int k = (int)(w & kIndexMask);
+    const uint32_t m = w >> kMaskShift;
+    const double *hk = base + (size_t)B_STRIDE * k;
+    double qk[4], dxk[4], dyk[4], qmxk[4], qmnk[4], tqk[4], tdxk[4], tdyk[4], tmxk[4], tmnk[4];
+    load4(hk + B_Q, qk);
+    load8(dq + (size_t)DQ_STRIDE * k, dxk, dyk);
+    load8(qm + (size_t)QM_STRIDE * k, qmxk, qmnk);
+    load4(tbase + (size_t)B_STRIDE * k + B_Q, tqk);
+    load8(tdq + (size_t)DQ_STRIDE * k, tdxk, tdyk);
+    load8(tqm + 8 * (size_t)k, tmxk, tmnk);
+    const double delx = hk[B_X] - x_i, dely = hk[B_Y] - y_i;
+    const double dels = delx * tx + dely * ty, deln = delx * nx + dely * ny;
+    Dual<1> qti[4], qtk[4];
+    for (int r = 0; r < 4; r++) {
+      const Recon a = recon(qi[r], delx * dxi[r] + dely * dyi[r], qmxi[r], qmni[r], eps2_i);
+      const Recon b = recon(qk[r], delx * dxk[r] + dely * dyk[r], qmxk[r], qmnk[r], hk[B_EPS2]);
+      qti[r] = Dual<1>(a.qt);
+      qti[r].d[0] = a.dq * tqi[r] + a.dd * (delx * tdxi[r] + dely * tdyi[r]) +
+                    a.dm * (a.use_max ? tmxi[r] : tmni[r]);
+      qtk[r] = Dual<1>(b.qt);
+      qtk[r].d[0] = b.dq * tqk[r] + b.dd * (delx * tdxk[r] + dely * tdyk[r]) +
+                    b.dm * (b.use_max ? tmxk[r] : tmnk[r]);
+    }
+    Dual<1> Si[4], Sk[4];
+    end_sum_dual(qti, nx, ny, type, m, s2b, cf, dels, deln, Si);
+    end_sum_dual(qtk, nx, ny, type, m, s2b, cf, dels, deln, Sk);
+    for (int r = 0; r < 4; r++) acc[r] += Sk[r].d[0] - Si[r].d[0];
+  }
+  const double sc = (type == 0 ? 2.0 : 1.0) * delta[i];
+  for (int r = 0; r < 4; r++) out[(size_t)r * n + i] = sc * acc[r];
+}
+
+int blocks_of(int n) { return (n + 255) / 256; }
+
+}  // namespace
+
+void launch_argext(const DevCloud &c, const double *base, int32_t *arg, hipStream_t s) {
+  hipLaunchKernelGGL(k_argext, dim3(blocks_of(c.n)), dim3(256), 0, s, c.n, c.conn_off, c.conn,
+                     base, arg);
+}
+
+void launch_flux_vjp(const DevCloud &c, const AdjointBuffers &a, const double *base,
+                     const double *dq, const double *qm, const double *delta, const double *lam,
+                     hipStream_t s) {
+  hipLaunchKernelGGL(k_flux_vjp, dim3(blocks_of(c.n)), dim3(256), 0, s, c.n, c.conn_off, c.conn,
+                     base, dq, qm, c.coef, c.nx, c.ny, c.flag, delta, lam, a.own_q, a.own_dq,
+                     a.own_qm, a.edge);
+  hipLaunchKernelGGL(k_flux_vjp_gather, dim3(blocks_of(c.n)), dim3(256), 0, s, c.n, a.tr_off,
+                     a.tr_edge, a.tr_owner, base, a.edge, a.own_q, a.own_dq, a.own_qm, a.lq,
+                     a.ldq[0], a.lqm);
+}
+
+void launch_ls_vjp(const DevCloud &c, const AdjointBuffers &a, const double *base,
+                   const double *lo, double *ldq_in, bool extrema, hipStream_t s) {
+  hipLaunchKernelGGL(k_ls_adj, dim3(blocks_of(c.n)), dim3(256), 0, s, c.n, c.conn_off, c.conn,
+                     base, lo, a.ls);
+  hipLaunchKernelGGL(k_ls_adj_gather, dim3(blocks_of(c.n)), dim3(256), 0, s, c.n, c.conn_off,
+                     c.conn, a.tr_off, a.tr_owner, base, a.ls, extrema ? 0 : 1, ldq_in,
+                     extrema ? a.arg : nullptr, a.lqm, a.lq);
+}
+
+void launch_qvar_vjp(const DevCloud &c, const AdjointBuffers &a, const double *prim,
+                     double *grad, hipStream_t s) {
+  hipLaunchKernelGGL(k_qvar_vjp, dim3(blocks_of(c.n)), dim3(256), 0, s, c.n, prim, a.lq, grad);
+}
+
+void launch_qvar_jvp(const DevCloud &c, const double *prim, const double *v, double *tbase,
+                     hipStream_t s) {
+  hipLaunchKernelGGL(k_qvar_jvp, dim3(blocks_of(c.n)), dim3(256), 0, s, c.n, prim, v, tbase);
+}
+
+void launch_ext_jvp(const DevCloud &c, const int32_t *arg, const double *tbase, double *tqm,
+                    hipStream_t s) {
+  hipLaunchKernelGGL(k_ext_jvp, dim3(blocks_of(c.n)), dim3(256), 0, s, c.n, arg, tbase, tqm);
+}
+
+void launch_flux_jvp(const DevCloud &c, const double *base, const double *dq, const double *qm,
+                     const double *delta, const double *tbase, const double *tdq,
+                     const double *tqm, double *out, hipStream_t s) {
+  hipLaunchKernelGGL(k_flux_jvp, dim3(blocks_of(c.n)), dim3(256), 0, s, c.n, c.conn_off, c.conn,
+                     base, dq, qm, c.coef, c.nx, c.ny, c.flag, delta, tbase, tdq, tqm, out);
+}
+
+}  // namespace lskum

@@ -115,6 +115,9 @@ struct lskum_ctx {
   int gexec_iters = 0;
   double *gexec_hist = nullptr;
   bool capturing = false;
+  // adjoint and tangent working arrays, allocated on first use
+  AdjointBuffers adj{};
+  bool adj_ready = false;
   // timing
   bool timing = false;
   double k_ms[6] = {0, 0, 0, 0, 0, 0};
@@ -186,6 +189,12 @@ void release(lskum_ctx *c) {
                   c->xy, c->q32, c->dq32[0], c->dq32[1], c->qm32, c->eps32, c->fcs,
                   c->block_case, c->seg, c->soff, c->qdyn, c->counter};
   for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  void *aptrs[] = {c->adj.own_q, c->adj.own_dq, c->adj.own_qm, c->adj.edge, c->adj.lq,
+                   c->adj.ldq[0], c->adj.ldq[1], c->adj.lqm, c->adj.ls, c->adj.arg,
+                   c->adj.tr_off, c->adj.tr_edge, c->adj.tr_owner, c->adj.tbase, c->adj.tdq[0],
+                   c->adj.tdq[1], c->adj.tqm, c->adj.vec_in, c->adj.vec_out};
+  for (void *p : aptrs)
     if (p) (void)hipFree(p);
   if (c->red_h) (void)hipHostFree(c->red_h);
   for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
@@ -1193,6 +1202,165 @@ int lskum_kernel_time(lskum_ctx *ctx, int32_t kernel, double *avg_ms, int32_t *l
   if (rc) return rc;
   if (avg_ms) *avg_ms = ctx->k_cnt[kernel] ? ctx->k_ms[kernel] / ctx->k_cnt[kernel] : 0.0;
   if (launches) *launches = ctx->k_cnt[kernel];
+  return LSKUM_OK;
+}
+
+// Adjoint working arrays and the transpose of conn (single-case FP64
+// contexts with power = 0).
+static int prepare_adjoint(lskum_ctx *c) {
+  if (c->n_cases() != 1 || c->mixed || c->prm.power != 0.0)
+    return set_err(c, LSKUM_EINVAL,
+                   "the adjoint and tangent need a single-case FP64 context with power 0");
+  if (c->adj_ready) return LSKUM_OK;
+  const CloudLayout &L = c->clouds[0];
+  const int32_t n = c->n;
+  const size_t E = (size_t)L.off[n];
+  std::vector<int32_t> tr_off(n + 1, 0), tr_edge(E), tr_owner(E);
+  for (size_t e = 0; e < E; e++) tr_off[(L.conn[e] & kIndexMask) + 1]++;
+  for (int32_t j = 0; j < n; j++) tr_off[j + 1] += tr_off[j];
+  std::vector<int32_t> pos(tr_off.begin(), tr_off.end() - 1);
+  for (int32_t i = 0; i < n; i++)
+    for (int32_t e = L.off[i]; e < L.off[i + 1]; e++) {
+      const int32_t j = (int32_t)(L.conn[e] & kIndexMask);
+      tr_edge[pos[j]] = e;
+      tr_owner[pos[j]++] = i;
+    }
+  AdjointBuffers &a = c->adj;
+  hipError_t e;
+#define AALLOC(ptr, count)                                                   \
+  if ((e = dalloc(&(ptr), (count))) != hipSuccess)                           \
+    return set_err(c, LSKUM_ENOMEM, std::string("hipMalloc " #ptr ": ") + hipGetErrorString(e));
+  AALLOC(a.own_q, (size_t)4 * n);
+  AALLOC(a.own_dq, (size_t)8 * n);
+  AALLOC(a.own_qm, (size_t)8 * n);
+  AALLOC(a.edge, 16 * E);
+  AALLOC(a.lq, (size_t)4 * n);
+  AALLOC(a.ldq[0], (size_t)8 * n);
+  AALLOC(a.ldq[1], (size_t)8 * n);
+  AALLOC(a.lqm, (size_t)8 * n);
+  AALLOC(a.ls, (size_t)8 * n);
+  AALLOC(a.arg, (size_t)8 * n);
+  AALLOC(a.tr_off, (size_t)n + 1);
+  AALLOC(a.tr_edge, E);
+  AALLOC(a.tr_owner, E);
+  AALLOC(a.tbase, (size_t)B_STRIDE * n);
+  AALLOC(a.tdq[0], (size_t)DQ_STRIDE * n);
+  AALLOC(a.tdq[1], (size_t)DQ_STRIDE * n);
+  AALLOC(a.tqm, (size_t)8 * n);
+  AALLOC(a.vec_in, (size_t)4 * n);
+  AALLOC(a.vec_out, (size_t)4 * n);
+#undef AALLOC
+  HIP_TRY(c, hipMemcpy(a.tr_off, tr_off.data(), tr_off.size() * sizeof(int32_t),
+                       hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(a.tr_edge, tr_edge.data(), E * sizeof(int32_t), hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(a.tr_owner, tr_owner.data(), E * sizeof(int32_t), hipMemcpyHostToDevice));
+  c->adj_ready = true;
+  return LSKUM_OK;
+}
+
+// q, dq (after the inner passes) and qmax/qmin of the current state, and
+// the origin of every extremum: the checkpoint recomputation
+static int recompute_stages(lskum_ctx *c) {
+  const int inner = c->prm.inner_iterations;
+  int rc = do_q_variables(c);
+  if (!rc) rc = do_q_derivatives(c, inner);
+  if (!rc) rc = do_q_inner(c, inner);
+  if (rc) return rc;
+  launch_argext(c->dc(), c->base, c->adj.arg, c->stream);
+  return check_launch(c, "argext");
+}
+
+static int upload_caller(lskum_ctx *c, const double *src, double *dev) {
+  std::vector<double> tmp((size_t)4 * c->n, 0.0);
+  permute_in(c, src, tmp.data(), 4);
+  HIP_TRY(c, hipMemcpyAsync(dev, tmp.data(), tmp.size() * sizeof(double), hipMemcpyHostToDevice,
+                            c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return LSKUM_OK;
+}
+
+static int download_caller(lskum_ctx *c, const double *dev, double *dst) {
+  std::vector<double> tmp;
+  int rc = fetch_plain(c, dev, 4, tmp);
+  if (rc) return rc;
+  permute_out(c, tmp.data(), dst, 4);
+  return LSKUM_OK;
+}
+
+int lskum_flux_vjp(lskum_ctx *ctx, const double *lam, double *grad) {
+  CTX_OK(ctx);
+  if (!lam || !grad) return set_err(ctx, LSKUM_EINVAL, "null argument");
+  int rc = prepare_adjoint(ctx);
+  if (!rc) rc = recompute_stages(ctx);
+  if (!rc) rc = upload_caller(ctx, lam, ctx->adj.vec_in);
+  if (rc) return rc;
+  const DevCloud dc = ctx->dc();
+  AdjointBuffers &a = ctx->adj;
+  launch_flux_vjp(dc, a, ctx->base, ctx->dqb[ctx->cur], ctx->qm, ctx->delta, a.vec_in,
+                  ctx->stream);
+  int b = 0;  // ldq[b]: adjoint of the derivatives the flux read
+  for (int p = 0; p < ctx->prm.inner_iterations; p++) {
+    launch_ls_vjp(dc, a, ctx->base, a.ldq[b], a.ldq[1 - b], false, ctx->stream);
+    b = 1 - b;
+  }
+  launch_ls_vjp(dc, a, ctx->base, a.ldq[b], nullptr, true, ctx->stream);
+  launch_qvar_vjp(dc, a, ctx->prim, a.vec_out, ctx->stream);
+  rc = check_launch(ctx, "flux_vjp");
+  return rc ? rc : download_caller(ctx, a.vec_out, grad);
+}
+
+int lskum_flux_jvp(lskum_ctx *ctx, const double *v, double *out) {
+  CTX_OK(ctx);
+  if (!v || !out) return set_err(ctx, LSKUM_EINVAL, "null argument");
+  int rc = prepare_adjoint(ctx);
+  if (!rc) rc = recompute_stages(ctx);
+  if (!rc) rc = upload_caller(ctx, v, ctx->adj.vec_in);
+  if (rc) return rc;
+  const DevCloud dc = ctx->dc();
+  AdjointBuffers &a = ctx->adj;
+  const int inner = ctx->prm.inner_iterations;
+  HIP_TRY(ctx, hipMemcpyAsync(a.tbase, ctx->base, sizeof(double) * B_STRIDE * ctx->n,
+                              hipMemcpyDeviceToDevice, ctx->stream));
+  launch_qvar_jvp(dc, ctx->prim, a.vec_in, a.tbase, ctx->stream);
+  // the least-squares stencils are linear: the forward kernels map tangents
+  int t = inner % 2;
+  launch_q_derivatives(dc, a.tbase, a.tdq[t], a.tqm, 0.0, ctx->stream);
+  launch_ext_jvp(dc, a.arg, a.tbase, a.tqm, ctx->stream);
+  for (int p = 0; p < inner; p++) {
+    launch_q_inner(dc, a.tbase, a.tdq[t], a.tdq[1 - t], 0.0, ctx->stream);
+    t = 1 - t;
+  }
+  launch_flux_jvp(dc, ctx->base, ctx->dqb[ctx->cur], ctx->qm, ctx->delta, a.tbase, a.tdq[t],
+                  a.tqm, a.vec_out, ctx->stream);
+  rc = check_launch(ctx, "flux_jvp");
+  return rc ? rc : download_caller(ctx, a.vec_out, out);
+}
+
+int lskum_forces_grad(lskum_ctx *ctx, double *dcl, double *dcd, double *dcm) {
+  CTX_OK(ctx);
+  if (ctx->n_cases() != 1) return set_err(ctx, LSKUM_EINVAL, "single-case contexts only");
+  const CloudLayout &L = ctx->clouds[0];
+  const CaseInfo &cs = ctx->cases[0];
+  const double qdyn = 0.5 * ctx->rho_inf * cs.mach * cs.mach;
+  const double ct = std::cos(cs.theta), st = std::sin(cs.theta);
+  double *outs[3] = {dcl, dcd, dcm};
+  for (double *o : outs)
+    if (o) std::fill(o, o + 4 * (size_t)L.n, 0.0);
+  // compute_force_coeffs_mod.h:25-66: cp = -(p_m - p_inf)/qdyn per panel
+  for (size_t j = 0; j < L.fm.size(); j++) {
+    if (L.fs[j] != 1) continue;
+    const int32_t m = L.fm[j], l = L.fl[j], r = L.fr[j];
+    const double ds1 = std::sqrt((L.x[m] - L.x[l]) * (L.x[m] - L.x[l]) +
+                                 (L.y[m] - L.y[l]) * (L.y[m] - L.y[l]));
+    const double ds2 = std::sqrt((L.x[r] - L.x[m]) * (L.x[r] - L.x[m]) +
+                                 (L.y[r] - L.y[m]) * (L.y[r] - L.y[m]));
+    const double ds = 0.5 * (ds1 + ds2), dcp = -1.0 / qdyn;
+    const double nx = L.nx[m], ny = L.ny[m];
+    const size_t at = 3 * (size_t)L.n + L.perm[m];  // the pressure of point m
+    if (dcl) dcl[at] += (ny * ct - nx * st) * ds * dcp;
+    if (dcd) dcd[at] += (nx * ct + ny * st) * ds * dcp;
+    if (dcm) dcm[at] += (-ny * ds * (L.x[m] - 0.25) + nx * ds * L.y[m]) * dcp;
+  }
   return LSKUM_OK;
 }
 

@@ -132,4 +132,38 @@ void launch_flux_m(const DevCloud &c, const MixedState &m, const float *dq, cons
                    double power, double *flux_res, hipStream_t s);
 void launch_kat_fluxes_m(int n, const double *in6, double *out, hipStream_t s);
 
+// ---------------------------------------------- adjoint and tangent (FP64)
+// Working arrays of the reverse and tangent passes (device order): own_*
+// and edge hold the flux pairs' shares (edge: [n_edges][16]); lq [n][4],
+// ldq [n][8] (two, for the inner passes), lqm [n][8], ls [n][8] are
+// adjoints of q, dq, qmax/qmin and of the least-squares sums; arg [n][8]
+// the point each qmax/qmin came from; tr_* the transpose of conn (for each
+// point, the edges that name it, owners ascending); tbase, tdq, tqm the
+// tangents (tbase a copy of the base records with tangent q); vec_in /
+// vec_out [4][n] staging.
+struct AdjointBuffers {
+  double *own_q, *own_dq, *own_qm, *edge, *lq, *ldq[2], *lqm, *ls;
+  int32_t *arg, *tr_off, *tr_edge, *tr_owner;
+  double *tbase, *tdq[2], *tqm, *vec_in, *vec_out;
+};
+void launch_argext(const DevCloud &c, const double *base, int32_t *arg, hipStream_t s);
+// flux reverse: lam [4][n] (adjoint of flux_res) -> lq, ldq[0], lqm
+void launch_flux_vjp(const DevCloud &c, const AdjointBuffers &a, const double *base,
+                     const double *dq, const double *qm, const double *delta, const double *lam,
+                     hipStream_t s);
+// least-squares pass reverse from the adjoint lo of its output: adds to lq;
+// an inner pass (extrema false) writes the adjoint of its input
+// derivatives to ldq_in; q_derivatives (extrema true) routes lqm
+void launch_ls_vjp(const DevCloud &c, const AdjointBuffers &a, const double *base,
+                   const double *lo, double *ldq_in, bool extrema, hipStream_t s);
+void launch_qvar_vjp(const DevCloud &c, const AdjointBuffers &a, const double *prim,
+                     double *grad, hipStream_t s);
+void launch_qvar_jvp(const DevCloud &c, const double *prim, const double *v, double *tbase,
+                     hipStream_t s);
+void launch_ext_jvp(const DevCloud &c, const int32_t *arg, const double *tbase, double *tqm,
+                    hipStream_t s);
+void launch_flux_jvp(const DevCloud &c, const double *base, const double *dq, const double *qm,
+                     const double *delta, const double *tbase, const double *tdq,
+                     const double *tqm, double *out, hipStream_t s);
+
 }  // namespace lskum

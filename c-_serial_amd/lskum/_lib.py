@@ -111,6 +111,9 @@ _SIGS = [
     ("lskum_layout", C.c_int, [C.c_void_p, _ip, _ip, _ip]),
     ("lskum_set_timing", C.c_int, [C.c_void_p, C.c_int32]),
     ("lskum_kernel_time", C.c_int, [C.c_void_p, C.c_int32, _dp, _ip]),
+    ("lskum_flux_vjp", C.c_int, [C.c_void_p, _dp, _dp]),
+    ("lskum_flux_jvp", C.c_int, [C.c_void_p, _dp, _dp]),
+    ("lskum_forces_grad", C.c_int, [C.c_void_p, _dp, _dp, _dp]),
     ("lskum_kat_fluxes", C.c_int, [C.c_int32, _dp, _dp, C.c_int32]),
     ("lskum_kat_qtilde", C.c_int, [C.c_int32, _dp, _dp]),
 ]

@@ -171,6 +171,38 @@ class Solver:
             self.res_old = self.iteration()[1]
         return rn[:n_iter], rs[:n_iter]
 
+    # -------------------------------------------------- adjoint / tangent
+    def flux_vjp(self, lam):
+        """J^T lam for J = d flux_res / d prim at the current state (delta
+        held fixed); lam [4, n] -> [4, n]."""
+        lam = np.ascontiguousarray(lam, dtype=np.float64).reshape(4, self.n)
+        out = np.empty((4, self.n))
+        self._chk(L.lib().lskum_flux_vjp(self._ctx, L.dptr(lam), L.dptr(out)))
+        return out
+
+    def flux_jvp(self, v):
+        """J v for J = d flux_res / d prim at the current state."""
+        v = np.ascontiguousarray(v, dtype=np.float64).reshape(4, self.n)
+        out = np.empty((4, self.n))
+        self._chk(L.lib().lskum_flux_jvp(self._ctx, L.dptr(v), L.dptr(out)))
+        return out
+
+    def forces_grad(self):
+        """(dCl, dCd, dCm) / d prim, each [4, n]."""
+        g = [np.empty((4, self.n)) for _ in range(3)]
+        self._chk(L.lib().lskum_forces_grad(self._ctx, *[L.dptr(a) for a in g]))
+        return tuple(g)
+
+    def residual_of(self, prim):
+        """flux_res at the state prim with delta held at its last value: the
+        map lskum_flux_vjp / lskum_flux_jvp differentiate."""
+        self.set_state(prim)
+        self.eval_q_variables()
+        self.eval_q_derivatives()
+        self.eval_q_inner_loop(self.inner_iterations)
+        self.cal_flux_residual()
+        return self.flux_res
+
     def fpi_solver(self, it=None):
         """One iteration (fpi_solver_mod.h:14-150); `it` must be the next
         iteration number when given."""

@@ -274,6 +274,27 @@ int lskum_get_substencil(lskum_ctx *ctx, int32_t which, int32_t *counts,
                          int32_t *members);
 
 /*
+ * Discrete adjoint and tangent of the flux residual (no reference
+ * counterpart).  The map is one cal_flux_residual evaluation
+ * (flux_residual_mod.h:10) as a function of the primitive state prim, after
+ * eval_q_variables, eval_q_derivatives and the inner passes
+ * (q_variables_mod.h:9-228), with the local time step delta held at its
+ * last func_delta value.  Both recompute q, dq and qmax/qmin from the
+ * context's current state (lskum_set_state), so nothing of a forward run is
+ * stored.  Single-case FP64 contexts with power = 0 (else LSKUM_EINVAL).
+ *   lskum_flux_vjp: grad = J^T lam, lam and grad [4][n];
+ *   lskum_flux_jvp: out = J v, v and out [4][n].
+ * The limiter's switches (|del_neg| <= 1e-5, the ratio reaching 1) and the
+ * qmax/qmin selections are differentiated on the branch the state takes.
+ */
+int lskum_flux_vjp(lskum_ctx *ctx, const double *lam, double *grad);
+int lskum_flux_jvp(lskum_ctx *ctx, const double *v, double *out);
+/* Gradients of Cl, Cd, Cm (compute_force_coeffs_mod.h:7-69) with respect
+ * to prim, each [4][n] (only the pressure row is non-zero); any output may
+ * be NULL.  Single-case contexts. */
+int lskum_forces_grad(lskum_ctx *ctx, double *dcl, double *dcd, double *dcm);
+
+/*
  * Autotuner (no reference counterpart): builds the given cases once per
  * candidate layout -- points per block of the stencil kernels 64, 128 or
  * 256 (the block shape; one point per thread, whose neighbour list the
