@@ -37,12 +37,13 @@ __device__ __forceinline__ void load_tile(double *lds, const int32_t *__restrict
     double2 v[U];
 #pragma unroll
     for (int u = 0; u < U; u++) {
-      const int t = base + u * blockDim.x;
-      if (t < total) {
-        const int r = t / nchunks, ch = t - r * nchunks;
-        v[u] = reinterpret_cast<const double2 *>(src + (size_t)src_stride *
-                                                           tile_idx[t0 + r])[first_chunk + ch];
-      }
+      // every lane loads (past the end: the last chunk again), so the
+      // array is fully written and stays in registers
+      int t = base + u * blockDim.x;
+      t = t < total ? t : total - 1;
+      const int r = t / nchunks, ch = t - r * nchunks;
+      v[u] = reinterpret_cast<const double2 *>(src + (size_t)src_stride *
+                                                         tile_idx[t0 + r])[first_chunk + ch];
     }
 #pragma unroll
     for (int u = 0; u < U; u++) {

@@ -38,11 +38,12 @@ __device__ __forceinline__ void tile_copy16(uint4 *dst, const uint4 *__restrict_
     uint4 v[U];
 #pragma unroll
     for (int u = 0; u < U; u++) {
-      const int t = base + u * blockDim.x;
-      if (t < total) {
-        const int r = t / CH, ch = t - r * CH;
-        v[u] = src[(size_t)CH * idx[r] + ch];
-      }
+      // every lane loads (past the end: the last chunk again), so the
+      // array is fully written and stays in registers
+      int t = base + u * blockDim.x;
+      t = t < total ? t : total - 1;
+      const int r = t / CH, ch = t - r * CH;
+      v[u] = src[(size_t)CH * idx[r] + ch];
     }
 #pragma unroll
     for (int u = 0; u < U; u++) {
