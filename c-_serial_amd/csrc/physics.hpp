@@ -8,8 +8,10 @@
 // The split/quadrant fluxes are restructured around one set of per-end
 // terms (EndTerms): every flux the reference evaluates for a (point,
 // neighbour) pair -- two sub-stencils per neighbour, so two to four fluxes
-// per end -- shares beta, sqrt(beta), sqrt(pi*beta), exp(-S^2) and erf(S),
-// which are computed once.  A flux's +/- variant is selected by a sign s =
+// per end -- shares beta, sqrt(beta), 0.5/sqrt(pi*beta), exp(-S^2) and
+// erf(S), which are computed once.  Quotients in the limiter and in
+// q~ -> primitive use qdiv (a refined hardware reciprocal), within an ulp or
+// two of the IEEE quotient.  A flux's +/- variant is selected by a sign s =
 // +-1.0: 0.5*(1 + s*erf) and x + s*y are bit-identical to the reference's
 // 0.5*(1 - erf) and x - y because multiplying by -1.0 is exact.
 #pragma once
@@ -21,6 +23,31 @@ namespace lskum {
 constexpr double kPi = 3.14159265358979323846;  // == 4*atan(1.0) in double
 constexpr double kGammaM1 = 1.4 - 1.0;          // gamma_new - 1 as evaluated
 constexpr double kInvGammaM1 = 1.0 / kGammaM1;
+constexpr double kInvSqrtPi = 0.56418958354775628695;  // 1/sqrt(pi)
+
+// a / b from the hardware reciprocal: two Newton steps and a final residual
+// correction (the refinement the correctly rounded division also performs,
+// without its scaling for operands near the ends of the exponent range);
+// within an ulp or two of a / b for the normal-range operands it gets here.
+// Other types (the dual numbers of the derivative kernels) divide plainly.
+template <typename T>
+__device__ __forceinline__ T qdiv(T a, T b) {
+  return a / b;
+}
+template <>
+__device__ __forceinline__ double qdiv(double a, double b) {
+  double r = __builtin_amdgcn_rcp(b);
+  r = fma(fma(-b, r, 1.0), r, r);
+  r = fma(fma(-b, r, 1.0), r, r);
+  const double q = a * r;
+  return fma(fma(-b, q, a), r, q);
+}
+template <>
+__device__ __forceinline__ float qdiv(float a, float b) {
+  const float r = __builtin_amdgcn_rcpf(b);
+  const float q = a * r;
+  return fmaf(fmaf(-b, q, a), r, q);
+}
 
 // q_variables_mod.h:230-253 (output order u1, u2, rho, pr as there)
 template <typename T>
@@ -44,7 +71,7 @@ template <typename T>
 __device__ __forceinline__ void qtilde_state(const T qt[4], T &u1, T &u2, T &rho, T &beta,
                                              T &temp) {
   beta = -qt[3] * T(0.5);
-  temp = T(0.5) / beta;
+  temp = qdiv(T(0.5), beta);
   u1 = qt[1] * temp;
   u2 = qt[2] * temp;
   const T temp1 = qt[0] + beta * (u1 * u1 + u2 * u2);
@@ -59,7 +86,9 @@ struct EndTerms {
   T rho, ut, un, pr_by_rho, u_sqr, B1, B2, e1, e2;
 };
 
-// From (beta = rho/2p, p/rho): B1 and B2 share one reciprocal 0.5/sqrt(pi beta).
+// From (beta = rho/2p, p/rho).  B1 and B2 share h = 0.5/sqrt(pi beta),
+// formed without a second square root or a division: p/rho = 0.5/beta, so
+// h = (1/sqrt(pi)) sqrt(beta) (p/rho).
 template <typename T>
 __device__ __forceinline__ EndTerms<T> end_terms_bt(T nx, T ny, T u1, T u2, T rho, T beta,
                                                     T pr_by_rho) {
@@ -71,7 +100,7 @@ __device__ __forceinline__ EndTerms<T> end_terms_bt(T nx, T ny, T u1, T u2, T rh
   const T sb = sqrt(beta);
   const T S1 = e.ut * sb;
   const T S2 = e.un * sb;
-  const T h = T(0.5) / sqrt(T(kPi) * beta);
+  const T h = T(kInvSqrtPi) * sb * pr_by_rho;
   e.B1 = exp(-S1 * S1) * h;
   e.B2 = exp(-S2 * S2) * h;
   e.e1 = erf(S1);
@@ -153,7 +182,7 @@ __device__ __forceinline__ T venkat_phi(T qtilde, T q, T qmax, T qmin, T eps2) {
   T den = del_pos * del_pos + T(2.0) * del_neg * del_neg;
   den = den + del_neg * del_pos + eps2;
   den = den * del_neg;
-  const T temp = num / den;
+  const T temp = qdiv(num, den);
   return temp < T(1) ? temp : T(1);
 }
 
@@ -168,7 +197,7 @@ __device__ __forceinline__ T venkat_phi_delta(T del_neg, T dmax, T dmin, T eps2)
   const T dp2 = del_pos * del_pos;
   const T num = (dp2 + eps2) * del_neg + T(2.0) * del_neg * del_neg * del_pos;
   const T den = (dp2 + T(2.0) * del_neg * del_neg + del_neg * del_pos + eps2) * del_neg;
-  const T temp = num / den;
+  const T temp = qdiv(num, den);
   return temp < T(1) ? temp : T(1);
 }
 
