@@ -117,9 +117,9 @@ template <int N>
 __device__ __forceinline__ void end_sum_dual(const Dual<N> qt[4], double nx, double ny, int type,
                                              uint32_t m, double s2b, const double cf[8],
                                              double dsw, double dnw, Dual<N> out[4]) {
-  Dual<N> u1, u2, rho, beta, temp;
-  qtilde_state(qt, u1, u2, rho, beta, temp);
-  const EndTerms<Dual<N>> e = end_terms_bt(Dual<N>(nx), Dual<N>(ny), u1, u2, rho, beta, temp);
+  Dual<N> u1, u2, rho, sb, temp;
+  qtilde_state(qt, u1, u2, rho, sb, temp);
+  const EndTerms<Dual<N>> e = end_terms_sb(Dual<N>(nx), Dual<N>(ny), u1, u2, rho, sb, temp);
   end_sum(e, type, m, s2b, cf, dsw, dnw, out);
 }
 

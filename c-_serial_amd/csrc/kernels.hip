@@ -371,11 +371,11 @@ __global__ __launch_bounds__(256) void k_flux(
       qti[r] = qi[r] - 0.5 * phi_i * di;
       qtk[r] = qk[r] - 0.5 * phi_k * dk;
     }
-    double u1, u2, rho, beta, temp;
-    qtilde_state(qti, u1, u2, rho, beta, temp);
-    const EndTerms<double> ei = end_terms_bt(nx, ny, u1, u2, rho, beta, temp);
-    qtilde_state(qtk, u1, u2, rho, beta, temp);
-    const EndTerms<double> ek = end_terms_bt(nx, ny, u1, u2, rho, beta, temp);
+    double u1, u2, rho, sb, temp;
+    qtilde_state(qti, u1, u2, rho, sb, temp);
+    const EndTerms<double> ei = end_terms_sb(nx, ny, u1, u2, rho, sb, temp);
+    qtilde_state(qtk, u1, u2, rho, sb, temp);
+    const EndTerms<double> ek = end_terms_sb(nx, ny, u1, u2, rho, sb, temp);
     double Gi[4], Gk[4];
     // x direction: x+ (dels <= 0) or x- (dels >= 0)
     const bool xp = (m & XPOS) != 0;

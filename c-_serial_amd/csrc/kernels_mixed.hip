@@ -311,11 +311,11 @@ __global__ __launch_bounds__(256) void k_flux_m(
       qti[r] = qi[r] - 0.5f * phi_i * di;
       qtk[r] = qk[r] - 0.5f * phi_k * dk;
     }
-    float u1, u2, rho, beta, temp;
-    qtilde_state(qti, u1, u2, rho, beta, temp);
-    const EndTerms<float> ei = end_terms_bt(nxf, nyf, u1, u2, rho, beta, temp);
-    qtilde_state(qtk, u1, u2, rho, beta, temp);
-    const EndTerms<float> ek = end_terms_bt(nxf, nyf, u1, u2, rho, beta, temp);
+    float u1, u2, rho, sb, temp;
+    qtilde_state(qti, u1, u2, rho, sb, temp);
+    const EndTerms<float> ei = end_terms_sb(nxf, nyf, u1, u2, rho, sb, temp);
+    qtilde_state(qtk, u1, u2, rho, sb, temp);
+    const EndTerms<float> ek = end_terms_sb(nxf, nyf, u1, u2, rho, sb, temp);
     float Gi[4], Gk[4];
     const bool xp = (m & XPOS) != 0;
     const float sx = xp ? 1.0f : -1.0f;

@@ -63,19 +63,22 @@ __device__ __forceinline__ void qtilde_to_primitive(const T qt[4], T &u1, T &u2,
   pr = rho * temp;
 }
 
-// qtilde_to_primitive that also returns beta and temp = 0.5/beta = p/rho,
-// which the fluxes need again (split_fluxes_mod.h:20,25 recompute them as
-// 0.5*rho/p and p/rho; here they are reused, equal up to rounding), and
-// divides by gamma-1 as a multiplication by its reciprocal.
+// qtilde_to_primitive that also returns sqrt(beta) and temp = 0.5/beta =
+// p/rho, which the fluxes need again (split_fluxes_mod.h:20-25 recompute
+// beta = 0.5*rho/p and p/rho; here they are reused, equal up to rounding).
+// rho = exp(temp1 - log(beta)/(gamma-1)) is evaluated as exp(temp1) /
+// (beta^2 sqrt(beta)): 1/(gamma-1) = 2.5 for gamma = 1.4, so the logarithm
+// becomes a quotient of the square root the end terms need anyway.
 template <typename T>
-__device__ __forceinline__ void qtilde_state(const T qt[4], T &u1, T &u2, T &rho, T &beta,
+__device__ __forceinline__ void qtilde_state(const T qt[4], T &u1, T &u2, T &rho, T &sb,
                                              T &temp) {
-  beta = -qt[3] * T(0.5);
+  const T beta = -qt[3] * T(0.5);
   temp = qdiv(T(0.5), beta);
   u1 = qt[1] * temp;
   u2 = qt[2] * temp;
   const T temp1 = qt[0] + beta * (u1 * u1 + u2 * u2);
-  rho = exp(temp1 - log(beta) * T(kInvGammaM1));
+  sb = sqrt(beta);
+  rho = exp(temp1) * qdiv(T(1.0), beta * beta * sb);
 }
 
 // Terms shared by all split and quadrant fluxes of one state in the frame
@@ -86,18 +89,17 @@ struct EndTerms {
   T rho, ut, un, pr_by_rho, u_sqr, B1, B2, e1, e2;
 };
 
-// From (beta = rho/2p, p/rho).  B1 and B2 share h = 0.5/sqrt(pi beta),
-// formed without a second square root or a division: p/rho = 0.5/beta, so
-// h = (1/sqrt(pi)) sqrt(beta) (p/rho).
+// From (sqrt(beta), p/rho), beta = rho/2p.  B1 and B2 share h =
+// 0.5/sqrt(pi beta), formed without a square root or a division: p/rho =
+// 0.5/beta, so h = (1/sqrt(pi)) sqrt(beta) (p/rho).
 template <typename T>
-__device__ __forceinline__ EndTerms<T> end_terms_bt(T nx, T ny, T u1, T u2, T rho, T beta,
+__device__ __forceinline__ EndTerms<T> end_terms_sb(T nx, T ny, T u1, T u2, T rho, T sb,
                                                     T pr_by_rho) {
   EndTerms<T> e;
   const T tx = ny, ty = -nx;
   e.rho = rho;
   e.ut = u1 * tx + u2 * ty;
   e.un = u1 * nx + u2 * ny;
-  const T sb = sqrt(beta);
   const T S1 = e.ut * sb;
   const T S2 = e.un * sb;
   const T h = T(kInvSqrtPi) * sb * pr_by_rho;
@@ -114,7 +116,7 @@ __device__ __forceinline__ EndTerms<T> end_terms_bt(T nx, T ny, T u1, T u2, T rh
 // of split_fluxes_mod.h / quadrant_fluxes_mod.h take it.
 template <typename T>
 __device__ __forceinline__ EndTerms<T> end_terms(T nx, T ny, T u1, T u2, T rho, T pr) {
-  return end_terms_bt(nx, ny, u1, u2, rho, T(0.5) * rho / pr, pr / rho);
+  return end_terms_sb(nx, ny, u1, u2, rho, sqrt(T(0.5) * rho / pr), pr / rho);
 }
 
 // split_fluxes_mod.h:6-43 (s = +1, Gxp) and :45-84 (s = -1, Gxn)
