@@ -105,6 +105,9 @@ _SIGS = [
     ("lskum_iterate", C.c_int, [C.c_void_p, C.c_int32, _dp, _dp]),
     ("lskum_get_iteration", C.c_int, [C.c_void_p, _ip, _dp]),
     ("lskum_set_iteration", C.c_int, [C.c_void_p, C.c_int32, C.c_double, C.c_int32]),
+    ("lskum_restart_write", C.c_int, [C.c_char_p, C.POINTER(CloudView), C.c_int32, C.c_double,
+                                      _dp]),
+    ("lskum_restart_read", C.c_int, [C.c_char_p, C.c_int32, _ip, _dp, _dp]),
     ("lskum_get_array", C.c_int, [C.c_void_p, C.c_int32, _dp]),
     ("lskum_get_substencil", C.c_int, [C.c_void_p, C.c_int32, _ip, _ip]),
     ("lskum_stream", C.c_void_p, [C.c_void_p]),

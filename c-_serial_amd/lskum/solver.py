@@ -223,6 +223,25 @@ class Solver:
         """Run iterations as captured HIP graphs of this many iterations (0: off)."""
         self._chk(L.lib().lskum_set_graph(self._ctx, iters_per_graph))
 
+    def write_restart(self, path):
+        """restart.dat of the current state and iteration counter (the
+        layout initial_conditions_mod.h:7-21 reads)."""
+        it, res_old = self.iteration()
+        view = self.cloud.view()
+        prim = self.prim
+        L.check(L.lib().lskum_restart_write(str(path).encode(), C.byref(view), it, res_old,
+                                            L.dptr(prim)))
+
+    def read_restart(self, path):
+        """Resume from a restart file: its state, iteration counter and
+        res_old, with the res_old latch kept (the reference's restart = 1)."""
+        it, ro = C.c_int32(), C.c_double()
+        prim = np.zeros((4, self.n))
+        L.check(L.lib().lskum_restart_read(str(path).encode(), self.n, C.byref(it),
+                                           C.byref(ro), L.dptr(prim)))
+        self.set_state(prim)
+        self.set_iteration(it.value, ro.value, restart=True)
+
     def layout(self):
         """(mask of kernels using LDS tiles, tile_max, order) from lskum_create."""
         t, m, o = C.c_int32(), C.c_int32(), C.c_int32()

@@ -256,6 +256,24 @@ int lskum_get_iteration(lskum_ctx *ctx, int32_t *it, double *res_old);
 int lskum_set_iteration(lskum_ctx *ctx, int32_t it, double res_old,
                         int32_t restart);
 
+/*
+ * Restart files in the reference's restart.dat layout
+ * (initial_conditions_mod.h:7-21 reads it): a header "n itr res_old", then
+ * nine numbers per point of which the reader skips five and takes
+ * prim[1..4].  The writer fills the skipped fields with x, y, flag_1,
+ * flag_2 and min_dist (the reference has no writer for this layout).  prim
+ * is [4][n] in the cloud's order; 17 significant digits round-trip exactly.
+ * The reader fails with LSKUM_EIO on a missing or short file and
+ * LSKUM_EINVAL when the header's point count is not n.  A context resumes
+ * from a file with lskum_set_state(prim) and lskum_set_iteration(itr,
+ * res_old, 1), the reference's restart = 1 (q_lskum_mod.h:29-33,
+ * fpi_solver_mod.h:139).
+ */
+int lskum_restart_write(const char *path, const lskum_cloud *cloud, int32_t it,
+                        double res_old, const double *prim);
+int lskum_restart_read(const char *path, int32_t n, int32_t *it, double *res_old,
+                       double *prim);
+
 /* Arrays for tests and post-processing; sizes in doubles. */
 #define LSKUM_A_PRIM 0     /* [4][n]                      */
 #define LSKUM_A_PRIM_OLD 1 /* [4][n]                      */

@@ -215,3 +215,21 @@ def test_full_size_iterations_match_oracle():
     o.compute_cl_cd_cm()
     sc = o.scalars()
     np.testing.assert_allclose([s.Cl, s.Cd], [sc["cl"], sc["cd"]], rtol=1e-11)
+
+
+def test_restart_resumes_bit_for_bit(tmp_path):
+    # q_lskum_mod.h:29-33 / fpi_solver_mod.h:139 with restart = 1: the
+    # iteration count and res_old come from the file, the latch is kept
+    cloud = lskum.workload("naca0012_3620_k24")
+    params = (0.85, 1.25, 0.2, 5.0, 0.0, 2)
+    a = _solver(cloud, params)
+    a.q_lskum(15)
+    a.write_restart(tmp_path / "restart.dat")
+    ra = a.q_lskum(15)
+    b = _solver(cloud, params)
+    b.read_restart(tmp_path / "restart.dat")
+    assert b.iteration()[0] == a.iteration()[0] - 15, b.iteration()
+    rb = b.q_lskum(15)
+    np.testing.assert_array_equal(ra[0], rb[0])
+    np.testing.assert_array_equal(ra[1], rb[1])
+    np.testing.assert_array_equal(a.prim, b.prim)

@@ -296,3 +296,48 @@ def test_create_batch_validates_cases_and_names_the_cloud():
     assert rc == -1 and "case 0" in msg
     rc, msg = create([a, bad], [(0, 0.8, 1.0), (1, 0.7, 2.0)])
     assert rc == -1 and "cloud 1" in msg
+
+
+def _restart_write(cloud, path, it, res_old, prim):
+    v = cloud.view()
+    prim = np.ascontiguousarray(prim, dtype=np.float64)
+    return L.lib().lskum_restart_write(str(path).encode(), C.byref(v), it, res_old, L.dptr(prim))
+
+
+def _restart_read(path, n):
+    it, ro = C.c_int32(), C.c_double()
+    prim = np.zeros((4, n))
+    rc = L.lib().lskum_restart_read(str(path).encode(), n, C.byref(it), C.byref(ro),
+                                    L.dptr(prim))
+    return rc, it.value, ro.value, prim
+
+
+def test_restart_file_round_trip_and_layout(tmp_path):
+    c = lskum.workload("naca0012_1480_k20")
+    rng = np.random.default_rng(3)
+    prim = np.stack([1 + 0.1 * rng.random(c.n), rng.random(c.n), rng.random(c.n) - 0.5,
+                     0.7 + 0.1 * rng.random(c.n)]) * (1 + 1e-13 * rng.standard_normal(c.n))
+    p = tmp_path / "restart.dat"
+    assert _restart_write(c, p, 137, 3.25e-7, prim) == 0
+    rc, it, ro, back = _restart_read(p, c.n)
+    assert rc == 0 and it == 137 and ro == 3.25e-7
+    np.testing.assert_array_equal(back, prim)
+    # the layout restart_sol reads (initial_conditions_mod.h:12-17): a header
+    # "n itr res_old", then nine numbers per point, prim in the last four
+    tok = p.read_text().split()
+    assert int(float(tok[0])) == c.n and int(tok[1]) == 137 and float(tok[2]) == 3.25e-7
+    body = np.array(tok[3:], dtype=float).reshape(c.n, 9)
+    np.testing.assert_array_equal(body[:, 5:].T, prim)
+    np.testing.assert_array_equal(body[:, 0], c.x)
+
+
+def test_restart_reader_errors(tmp_path):
+    c = lskum.workload("naca0012_1480_k20")
+    prim = np.ones((4, c.n))
+    assert _restart_read(tmp_path / "missing.dat", c.n)[0] == -4
+    p = tmp_path / "restart.dat"
+    assert _restart_write(c, p, 5, 1e-6, prim) == 0
+    assert _restart_read(p, c.n + 1)[0] == -1       # a different cloud
+    lines = p.read_text().splitlines()
+    (tmp_path / "short.dat").write_text("\n".join(lines[:100]) + "\n")
+    assert _restart_read(tmp_path / "short.dat", c.n)[0] == -4
