@@ -80,11 +80,22 @@ __device__ __forceinline__ Recon recon(double q, double d, double qmax, double q
 
 // Sum over the sub-stencil fluxes a (point, neighbour) pair feeds of
 // w_s G_s(e), for one end's terms e: the selection of the FP64 flux kernel
-// (x+ or x-, both on a tie; interior y+ / y-, wall y-, outer y+).
+// (x+ or x-, both on a tie; interior y+ / y-, wall y-, outer y+).  With la
+// given, also g[j] = sum_r la_r G_j,r (values) of each flux summed: j = 0
+// the x flux, 1 the x- flux of a tie, 2 the y flux, 3 the y- flux of a tie
+// (0 where the pair has none).
 template <typename T>
 __device__ __forceinline__ void end_sum(const EndTerms<T> &e, int type, uint32_t m, double s2b,
-                                        const double cf[8], double dsw, double dnw, T out[4]) {
+                                        const double cf[8], double dsw, double dnw, T out[4],
+                                        const double *la = nullptr, double *g = nullptr) {
   T G[4];
+  auto contract = [&](int j) {
+    if (!la) return;
+    double t = 0.0;
+    for (int r = 0; r < 4; r++) t += la[r] * vpart(G[r]);
+    g[j] = t;
+  };
+  if (la) g[0] = g[1] = g[2] = g[3] = 0.0;
   const bool xp = (m & XPOS) != 0;
   if (type == 1)
     split_x(e, T(xp ? 1.0 : -1.0), G);
@@ -92,6 +103,7 @@ __device__ __forceinline__ void end_sum(const EndTerms<T> &e, int type, uint32_t
     quad_x(e, T(xp ? 1.0 : -1.0), T(s2b), G);
   const double wx = xp ? cf[0] * dsw + cf[1] * dnw : cf[2] * dsw + cf[3] * dnw;
   for (int r = 0; r < 4; r++) out[r] = G[r] * wx;
+  contract(0);
   if (xp && (m & XNEG)) {
     if (type == 1)
       split_x(e, T(-1.0), G);
@@ -99,15 +111,18 @@ __device__ __forceinline__ void end_sum(const EndTerms<T> &e, int type, uint32_t
       quad_x(e, T(-1.0), T(s2b), G);
     const double wn = cf[2] * dsw + cf[3] * dnw;
     for (int r = 0; r < 4; r++) out[r] = out[r] + G[r] * wn;
+    contract(1);
   }
   const bool yp = type == 1 ? (m & YPOS) != 0 : type == 2;
   split_y(e, T(yp ? 1.0 : -1.0), G);
   const double wy = yp ? cf[4] * dnw + cf[5] * dsw : cf[6] * dnw + cf[7] * dsw;
   for (int r = 0; r < 4; r++) out[r] = out[r] + G[r] * wy;
+  contract(2);
   if (type == 1 && yp && (m & YNEG)) {
     split_y(e, T(-1.0), G);
     const double wn = cf[6] * dnw + cf[7] * dsw;
     for (int r = 0; r < 4; r++) out[r] = out[r] + G[r] * wn;
+    contract(3);
   }
 }
 
@@ -116,11 +131,32 @@ __device__ __forceinline__ void end_sum(const EndTerms<T> &e, int type, uint32_t
 template <int N>
 __device__ __forceinline__ void end_sum_dual(const Dual<N> qt[4], double nx, double ny, int type,
                                              uint32_t m, double s2b, const double cf[8],
-                                             double dsw, double dnw, Dual<N> out[4]) {
+                                             double dsw, double dnw, Dual<N> out[4],
+                                             const double *la = nullptr, double *g = nullptr) {
   Dual<N> u1, u2, rho, sb, temp;
   qtilde_state(qt, u1, u2, rho, sb, temp);
   const EndTerms<Dual<N>> e = end_terms_sb(Dual<N>(nx), Dual<N>(ny), u1, u2, rho, sb, temp);
-  end_sum(e, type, m, s2b, cf, dsw, dnw, out);
+  end_sum(e, type, m, s2b, cf, dsw, dnw, out, la, g);
+}
+
+// d/d(nx, ny) of sum_r la_r S_r at one end through the frame (t, n) of its
+// fluxes, the weights held fixed.
+__device__ __forceinline__ void frame_partial(const Recon R[4], double nx, double ny, int type,
+                                              uint32_t m, double s2b, const double cf[8],
+                                              double dsw, double dnw, const double la[4],
+                                              double out[2]) {
+  Dual<2> qt[4], u1, u2, rho, sb, temp, dnx(nx), dny(ny), S[4];
+  for (int r = 0; r < 4; r++) qt[r] = Dual<2>(R[r].qt);
+  dnx.d[0] = 1.0;
+  dny.d[1] = 1.0;
+  qtilde_state(qt, u1, u2, rho, sb, temp);
+  const EndTerms<Dual<2>> e = end_terms_sb(dnx, dny, u1, u2, rho, sb, temp);
+  end_sum(e, type, m, s2b, cf, dsw, dnw, S);
+  out[0] = out[1] = 0.0;
+  for (int r = 0; r < 4; r++) {
+    out[0] += la[r] * S[r].d[0];
+    out[1] += la[r] * S[r].d[1];
+  }
 }
 
 __device__ __forceinline__ void load8(const double *p, double a[4], double b[4]) {
@@ -166,6 +202,16 @@ __global__ __launch_bounds__(256) void k_argext(int n, const int32_t *__restrict
 // registers (own_q, own_dq, own_qm); the neighbour's goes to the edge's
 // record: c_q[4], c_d[4] (per unit of d, times dx/dy at the gather),
 // c_max[4], c_min[4].
+//
+// GEO adds the adjoint of the geometry the flux reads, accumulated (+=)
+// into per-edge and per-point arrays owned by i alone:
+//   geo_edge[e] (d/d delx, d/d dely of pair e): through the increments
+//     d = delx dqx + dely dqy of both reconstructions and through the
+//     weights' dels = delx ny - dely nx, deln = delx nx + dely ny;
+//   geo_n[i] (d/d nx, d/d ny): through dels, deln and the flux frame;
+//   geo_cf[i] (d/d of the eight sub-stencil coefficients);
+//   geo_delta[i] (d/d delta).
+template <bool GEO>
 __global__ __launch_bounds__(256) void k_flux_vjp(
     int n, const int32_t *__restrict__ off, const uint32_t *__restrict__ conn,
     const double *__restrict__ base, const double *__restrict__ dq,
@@ -173,14 +219,20 @@ __global__ __launch_bounds__(256) void k_flux_vjp(
     const double *__restrict__ nxa, const double *__restrict__ nya,
     const uint8_t *__restrict__ flag, const double *__restrict__ delta,
     const double *__restrict__ lam, double *__restrict__ own_q, double *__restrict__ own_dq,
-    double *__restrict__ own_qm, double *__restrict__ edge) {
+    double *__restrict__ own_qm, double *__restrict__ edge, double *__restrict__ geo_edge,
+    double *__restrict__ geo_n, double *__restrict__ geo_cf, double *__restrict__ geo_delta) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int type = flag[i];
   // flux_res = c delta acc with c = 2 on the wall (flux_residual_mod.h:29)
-  const double sc = (type == 0 ? 2.0 : 1.0) * delta[i];
-  double la[4];
-  for (int r = 0; r < 4; r++) la[r] = sc * lam[(size_t)r * n + i];
+  const double cw = type == 0 ? 2.0 : 1.0;
+  const double sc = cw * delta[i];
+  double la[4], lc[4];
+  for (int r = 0; r < 4; r++) {
+    la[r] = sc * lam[(size_t)r * n + i];
+    lc[r] = cw * lam[(size_t)r * n + i];
+  }
+  double gcf[8] = {0, 0, 0, 0, 0, 0, 0, 0}, gnx = 0.0, gny = 0.0, gdel = 0.0;
   const double *hi = base + (size_t)B_STRIDE * i;
   const double x_i = hi[B_X], y_i = hi[B_Y], eps2_i = hi[B_EPS2];
   double qi[4], dxi[4], dyi[4], qmxi[4], qmni[4], cf[8];
@@ -209,19 +261,21 @@ __global__ __launch_bounds__(256) void k_flux_vjp(
       Rk[r] = recon(qk[r], delx * dxk[r] + dely * dyk[r], qmxk[r], qmnk[r], hk[B_EPS2]);
     }
     // acc += end_sum(k end) - end_sum(i end): the adjoint of q~ at each end
-    double a_i[4], a_k[4];
+    double a_i[4], a_k[4], g_i[4], g_k[4], v_i = 0.0, v_k = 0.0;
     {
       Dual<4> qt[4], S[4];
       for (int r = 0; r < 4; r++) {
         qt[r] = Dual<4>(Ri[r].qt);
         qt[r].d[r] = 1.0;
       }
-      end_sum_dual(qt, nx, ny, type, m, s2b, cf, dels, deln, S);
+      end_sum_dual(qt, nx, ny, type, m, s2b, cf, dels, deln, S, GEO ? la : nullptr, g_i);
       for (int c = 0; c < 4; c++) {
         double t = 0.0;
         for (int r = 0; r < 4; r++) t += la[r] * S[r].d[c];
         a_i[c] = -t;
       }
+      if (GEO)
+        for (int r = 0; r < 4; r++) v_i += lc[r] * S[r].v;
     }
     {
       Dual<4> qt[4], S[4];
@@ -229,12 +283,53 @@ __global__ __launch_bounds__(256) void k_flux_vjp(
         qt[r] = Dual<4>(Rk[r].qt);
         qt[r].d[r] = 1.0;
       }
-      end_sum_dual(qt, nx, ny, type, m, s2b, cf, dels, deln, S);
+      end_sum_dual(qt, nx, ny, type, m, s2b, cf, dels, deln, S, GEO ? la : nullptr, g_k);
       for (int c = 0; c < 4; c++) {
         double t = 0.0;
         for (int r = 0; r < 4; r++) t += la[r] * S[r].d[c];
         a_k[c] = t;
       }
+      if (GEO)
+        for (int r = 0; r < 4; r++) v_k += lc[r] * S[r].v;
+    }
+    if (GEO) {
+      gdel += v_k - v_i;
+      // weights: x flux a dels + b deln, y flux a deln + b dels, with (a, b)
+      // the coefficients of the sub-stencil the flux belongs to
+      double g[4], bs = 0.0, bn = 0.0;
+      for (int j = 0; j < 4; j++) g[j] = g_k[j] - g_i[j];
+      const bool xp = (m & XPOS) != 0;
+      const bool yp = type == 1 ? (m & YPOS) != 0 : type == 2;
+      const int cx = xp ? 0 : 2, cy = yp ? 4 : 6;
+      gcf[cx] += g[0] * dels;
+      gcf[cx + 1] += g[0] * deln;
+      bs += g[0] * cf[cx];
+      bn += g[0] * cf[cx + 1];
+      gcf[2] += g[1] * dels;
+      gcf[3] += g[1] * deln;
+      bs += g[1] * cf[2];
+      bn += g[1] * cf[3];
+      gcf[cy] += g[2] * deln;
+      gcf[cy + 1] += g[2] * dels;
+      bn += g[2] * cf[cy];
+      bs += g[2] * cf[cy + 1];
+      gcf[6] += g[3] * deln;
+      gcf[7] += g[3] * dels;
+      bn += g[3] * cf[6];
+      bs += g[3] * cf[7];
+      double fi[2], fk[2];
+      frame_partial(Ri, nx, ny, type, m, s2b, cf, dels, deln, la, fi);
+      frame_partial(Rk, nx, ny, type, m, s2b, cf, dels, deln, la, fk);
+      gnx += -bs * dely + bn * delx + (fk[0] - fi[0]);
+      gny += bs * delx + bn * dely + (fk[1] - fi[1]);
+      double gx = bs * ny + bn * nx, gy = -bs * nx + bn * ny;
+      for (int r = 0; r < 4; r++) {
+        const double ai = a_i[r] * Ri[r].dd, ak = a_k[r] * Rk[r].dd;
+        gx += ai * dxi[r] + ak * dxk[r];
+        gy += ai * dyi[r] + ak * dyk[r];
+      }
+      geo_edge[2 * (size_t)e] += gx;
+      geo_edge[2 * (size_t)e + 1] += gy;
     }
     double *rec = edge + 16 * (size_t)e;
     for (int r = 0; r < 4; r++) {
@@ -257,6 +352,12 @@ __global__ __launch_bounds__(256) void k_flux_vjp(
   store4(own_dq + 8 * (size_t)i + 4, ady);
   store4(own_qm + 8 * (size_t)i, amx);
   store4(own_qm + 8 * (size_t)i + 4, amn);
+  if (GEO) {
+    for (int j = 0; j < 8; j++) geo_cf[8 * (size_t)i + j] += gcf[j];
+    geo_n[2 * (size_t)i] += gnx;
+    geo_n[2 * (size_t)i + 1] += gny;
+    geo_delta[i] += gdel;
+  }
 }
 
 // Per point j: its own share plus the edge records of every pair in which
@@ -298,11 +399,24 @@ __global__ __launch_bounds__(256) void k_flux_vjp_gather(
 // ls_i = M_i^-T lo_i: the adjoint of the sums (sx, sy) a least-squares pass
 // solves, from the adjoint of its output (q_variables_mod.h:94-112 and
 // :176-210 solve with the same 2x2 matrix of the point's geometry).
+//
+// GEO adds, per pair e = (i, k) with rho = (delx, dely), the derivative of
+// sum_r lo_r . out_r with respect to rho (accumulated into geo_edge[e]):
+// out_r = M^-1 s_r with M = sum rho rho^T and s_r = sum rho dq_r, so
+//   d/d rho = sum_r ls_r (dq_r - rho . out_r) - (ls_r . rho) (out_r + 0.5 h_r)
+// where dq_r is the pair's difference the pass sums (q_k - q_i, less
+// 0.5 rho . h_r for an inner pass) and h_r = dq_in_k,r - dq_in_i,r the
+// difference of the derivatives an inner pass reads (0 for q_derivatives,
+// dq_in = NULL).
+template <bool GEO>
 __global__ __launch_bounds__(256) void k_ls_adj(int n, const int32_t *__restrict__ off,
                                                 const uint32_t *__restrict__ conn,
                                                 const double *__restrict__ base,
                                                 const double *__restrict__ lo,
-                                                double *__restrict__ ls) {
+                                                double *__restrict__ ls,
+                                                const double *__restrict__ out,
+                                                const double *__restrict__ dq_in,
+                                                double *__restrict__ geo_edge) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double x_i = base[(size_t)B_STRIDE * i + B_X], y_i = base[(size_t)B_STRIDE * i + B_Y];
@@ -325,6 +439,31 @@ __global__ __launch_bounds__(256) void k_ls_adj(int n, const int32_t *__restrict
   }
   store4(ls + 8 * (size_t)i, sx);
   store4(ls + 8 * (size_t)i + 4, sy);
+  if (GEO) {
+    double qi[4], px[4], py[4], hxi[4] = {0, 0, 0, 0}, hyi[4] = {0, 0, 0, 0};
+    load4(base + (size_t)B_STRIDE * i + B_Q, qi);
+    load8(out + (size_t)DQ_STRIDE * i, px, py);
+    if (dq_in) load8(dq_in + (size_t)DQ_STRIDE * i, hxi, hyi);
+    for (int e = off[i]; e < off[i + 1]; e++) {
+      const int k = (int)(conn[e] & kIndexMask);
+      const double delx = base[(size_t)B_STRIDE * k + B_X] - x_i;
+      const double dely = base[(size_t)B_STRIDE * k + B_Y] - y_i;
+      double qk[4], hxk[4] = {0, 0, 0, 0}, hyk[4] = {0, 0, 0, 0};
+      load4(base + (size_t)B_STRIDE * k + B_Q, qk);
+      if (dq_in) load8(dq_in + (size_t)DQ_STRIDE * k, hxk, hyk);
+      double gx = 0.0, gy = 0.0;
+      for (int r = 0; r < 4; r++) {
+        const double hx = hxk[r] - hxi[r], hy = hyk[r] - hyi[r];
+        const double d = (qk[r] - qi[r]) - 0.5 * (delx * hx + dely * hy);
+        const double lr = delx * sx[r] + dely * sy[r];
+        const double ro = delx * px[r] + dely * py[r];
+        gx += sx[r] * (d - ro) - lr * (px[r] + 0.5 * hx);
+        gy += sy[r] * (d - ro) - lr * (py[r] + 0.5 * hy);
+      }
+      geo_edge[2 * (size_t)e] += gx;
+      geo_edge[2 * (size_t)e + 1] += gy;
+    }
+  }
 }
 
 // Per point j, from the sums' adjoints ls of every point: the adjoint of
@@ -411,17 +550,20 @@ __device__ __forceinline__ void qvar_jac(const double p[4], double J[4][4]) {
   J[3][3] = 2.0 * beta / pr;
 }
 
+// grad = J^T lq (accumulate: grad += J^T lq)
 __global__ __launch_bounds__(256) void k_qvar_vjp(int n, const double *__restrict__ prim,
                                                   const double *__restrict__ lq,
-                                                  double *__restrict__ grad) {
+                                                  double *__restrict__ grad, int accumulate) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double p[4], J[4][4], a[4];
   for (int c = 0; c < 4; c++) p[c] = prim[(size_t)c * n + i];
   qvar_jac(p, J);
   load4(lq + 4 * (size_t)i, a);
-  for (int c = 0; c < 4; c++)
-    grad[(size_t)c * n + i] = J[0][c] * a[0] + J[1][c] * a[1] + J[2][c] * a[2] + J[3][c] * a[3];
+  for (int c = 0; c < 4; c++) {
+    const double g = J[0][c] * a[0] + J[1][c] * a[1] + J[2][c] * a[2] + J[3][c] * a[3];
+    grad[(size_t)c * n + i] = accumulate ? grad[(size_t)c * n + i] + g : g;
+  }
 }
 
 // ------------------------------------------------------------ tangent-linear
@@ -524,27 +666,38 @@ void launch_argext(const DevCloud &c, const double *base, int32_t *arg, hipStrea
 
 void launch_flux_vjp(const DevCloud &c, const AdjointBuffers &a, const double *base,
                      const double *dq, const double *qm, const double *delta, const double *lam,
-                     hipStream_t s) {
-  hipLaunchKernelGGL(k_flux_vjp, dim3(blocks_of(c.n)), dim3(256), 0, s, c.n, c.conn_off, c.conn,
-                     base, dq, qm, c.coef, c.nx, c.ny, c.flag, delta, lam, a.own_q, a.own_dq,
-                     a.own_qm, a.edge);
+                     const GeoAdjoint *geo, hipStream_t s) {
+  if (geo)
+    hipLaunchKernelGGL(k_flux_vjp<true>, dim3(blocks_of(c.n)), dim3(256), 0, s, c.n, c.conn_off,
+                       c.conn, base, dq, qm, c.coef, c.nx, c.ny, c.flag, delta, lam, a.own_q,
+                       a.own_dq, a.own_qm, a.edge, geo->edge, geo->nrm, geo->coef, geo->delta);
+  else
+    hipLaunchKernelGGL(k_flux_vjp<false>, dim3(blocks_of(c.n)), dim3(256), 0, s, c.n,
+                       c.conn_off, c.conn, base, dq, qm, c.coef, c.nx, c.ny, c.flag, delta, lam,
+                       a.own_q, a.own_dq, a.own_qm, a.edge, nullptr, nullptr, nullptr, nullptr);
   hipLaunchKernelGGL(k_flux_vjp_gather, dim3(blocks_of(c.n)), dim3(256), 0, s, c.n, a.tr_off,
                      a.tr_edge, a.tr_owner, base, a.edge, a.own_q, a.own_dq, a.own_qm, a.lq,
                      a.ldq[0], a.lqm);
 }
 
 void launch_ls_vjp(const DevCloud &c, const AdjointBuffers &a, const double *base,
-                   const double *lo, double *ldq_in, bool extrema, hipStream_t s) {
-  hipLaunchKernelGGL(k_ls_adj, dim3(blocks_of(c.n)), dim3(256), 0, s, c.n, c.conn_off, c.conn,
-                     base, lo, a.ls);
+                   const double *lo, double *ldq_in, bool extrema, const double *out,
+                   const double *dq_in, double *geo_edge, hipStream_t s) {
+  if (geo_edge)
+    hipLaunchKernelGGL(k_ls_adj<true>, dim3(blocks_of(c.n)), dim3(256), 0, s, c.n, c.conn_off,
+                       c.conn, base, lo, a.ls, out, dq_in, geo_edge);
+  else
+    hipLaunchKernelGGL(k_ls_adj<false>, dim3(blocks_of(c.n)), dim3(256), 0, s, c.n, c.conn_off,
+                       c.conn, base, lo, a.ls, nullptr, nullptr, nullptr);
   hipLaunchKernelGGL(k_ls_adj_gather, dim3(blocks_of(c.n)), dim3(256), 0, s, c.n, c.conn_off,
                      c.conn, a.tr_off, a.tr_owner, base, a.ls, extrema ? 0 : 1, ldq_in,
                      extrema ? a.arg : nullptr, a.lqm, a.lq);
 }
 
 void launch_qvar_vjp(const DevCloud &c, const AdjointBuffers &a, const double *prim,
-                     double *grad, hipStream_t s) {
-  hipLaunchKernelGGL(k_qvar_vjp, dim3(blocks_of(c.n)), dim3(256), 0, s, c.n, prim, a.lq, grad);
+                     double *grad, bool accumulate, hipStream_t s) {
+  hipLaunchKernelGGL(k_qvar_vjp, dim3(blocks_of(c.n)), dim3(256), 0, s, c.n, prim, a.lq, grad,
+                     accumulate ? 1 : 0);
 }
 
 void launch_qvar_jvp(const DevCloud &c, const double *prim, const double *v, double *tbase,

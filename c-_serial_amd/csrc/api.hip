@@ -27,149 +27,15 @@
 #include "kernels.hpp"
 #include "lskum.h"
 
+#include "context.hpp"
+
 using namespace lskum;
 
-namespace {
-
-// One cloud's set-up on the host, in its device order.
-struct CloudLayout {
-  int32_t n = 0;
-  std::vector<int32_t> perm, inv;           // perm[dev] = caller, inv[caller] = dev
-  std::vector<int32_t> conn_off_h, conn_h;  // caller order, for sub-stencil queries
-  std::vector<uint8_t> mask_h;
-  std::vector<double> nx_h, ny_h;
-  // device order
-  std::vector<double> x, y, eps2, nx, ny, coef;
-  std::vector<uint8_t> flag;
-  std::vector<int32_t> off;
-  std::vector<uint32_t> conn;               // local index | mask << kMaskShift
-  std::vector<int32_t> fm, fl, fr, fs;      // force panels (device indices)
-};
-
-struct CaseInfo {
-  int cloud = 0;
-  int32_t p0 = 0;  // first device point
-  int32_t n = 0;   // points (the cloud's)
-  int32_t np = 0;  // points with padding
-  int32_t s0 = 0, s1 = 0;  // force panel range
-  double mach = 0.0, aoa = 0.0, theta = 0.0, q_inf[4] = {0, 0, 0, 0};
-  double res_old = 0.0;
-  int32_t max_res_point = 0;
-  int32_t failed_at = 0;  // first iteration with a non-finite residual (0: none)
-};
-
-}  // namespace
-
-struct lskum_ctx {
-  int n = 0;  // device points, all cases with padding
-  lskum_params prm{};
-  hipStream_t stream = nullptr;
-  std::string err;
-  std::vector<CloudLayout> clouds;
-  std::vector<CaseInfo> cases;
-  int n_cases() const { return (int)cases.size(); }
-  // device state: base records, derivative buffers (dqb[cur] holds the
-  // current derivatives), qmax/qmin, primitive state
-  double *base = nullptr, *dqb[2] = {nullptr, nullptr}, *qm = nullptr;
-  int cur = 0;
-  // mixed-precision form: FP64 coordinates, FP32 gathered data (dq32[cur]
-  // holds the current derivatives); dqb/qm are then not allocated
-  bool mixed = false;
-  double *xy = nullptr;
-  float *q32 = nullptr, *dq32[2] = {nullptr, nullptr}, *qm32 = nullptr, *eps32 = nullptr;
-  double *prim = nullptr, *prim_old = nullptr;
-  double *flux_res = nullptr, *delta = nullptr, *nx = nullptr, *ny = nullptr;
-  double *coef = nullptr;
-  uint8_t *flag = nullptr;
-  int32_t *conn_off = nullptr, *perm_d = nullptr;
-  uint32_t *conn = nullptr;
-  uint16_t *conn16 = nullptr;
-  int32_t *tile_off = nullptr, *tile_idx = nullptr;
-  int tile_max = 0;
-  unsigned tiles = 0;
-  int tile_block = kTileBlockDefault;
-  // cases: flow constants, 256-block -> case map, partial and force ranges
-  FlowConsts *fcs = nullptr;
-  uint16_t *block_case = nullptr;  // NULL for one case
-  int32_t *seg = nullptr;          // NULL for one case
-  int32_t *soff = nullptr;
-  double *qdyn = nullptr;
-  Partials part{};
-  double *red = nullptr;      // [n_cases][3] sum, max, argmax of the last state update
-  double *hist = nullptr;     // [n_cases][hist_cap] per-iteration sum_res_sqr
-  int hist_cap = 0;
-  int32_t *counter = nullptr; // iteration slot of hist within an lskum_iterate call
-  double *red_h = nullptr;    // pinned host mirror of red / forces
-  int32_t *f_m = nullptr, *f_l = nullptr, *f_r = nullptr, *f_s = nullptr;
-  double *forces = nullptr;   // [n_cases][3] H, V, pitch moment
-  double rho_inf = 1.0, pr_inf = 1.0 / 1.4;
-  // iteration bookkeeping (q_lskum_mod.h, fpi_solver_mod.h:138-147)
-  int it = 0;
-  int restart = 0;
-  // q and the func_delta speed hold for the current state (a stage-4 update
-  // refreshes both; setting the state invalidates them)
-  bool q_fresh = false, speed_fresh = false;
-  // graph capture of whole iterations
-  int graph_iters = 0;
-  hipGraphExec_t gexec = nullptr;
-  int gexec_iters = 0;
-  double *gexec_hist = nullptr;
-  bool capturing = false;
-  // adjoint and tangent working arrays, allocated on first use
-  AdjointBuffers adj{};
-  bool adj_ready = false;
-  // timing
-  bool timing = false;
-  double k_ms[6] = {0, 0, 0, 0, 0, 0};
-  int k_cnt[6] = {0, 0, 0, 0, 0, 0};
-  std::vector<hipEvent_t> ev_pool;
-  std::vector<std::pair<int, int>> ev_pending;  // (kernel id, event index)
-
-  DevCloud dc() const {
-    DevCloud c;
-    c.n = n;
-    c.conn_off = conn_off;
-    c.conn = conn;
-    c.conn16 = conn16;
-    c.tile_off = tile_off;
-    c.tile_idx = tile_idx;
-    c.tile_max = tile_max;
-    c.tiles = tiles;
-    c.tile_block = tile_block;
-    c.nx = nx;
-    c.ny = ny;
-    c.coef = coef;
-    c.flag = flag;
-    c.perm = perm_d;
-    return c;
-  }
-  MixedState ms() const {
-    MixedState m;
-    m.xy = xy;
-    m.q32 = q32;
-    m.qm32 = qm32;
-    m.eps32 = eps32;
-    return m;
-  }
-};
-
-namespace {
+namespace lskum {
 
 int set_err(lskum_ctx *c, int code, const std::string &m) {
   if (c) c->err = m;
   return code;
-}
-
-#define HIP_TRY(ctx, call)                                                       \
-  do {                                                                           \
-    hipError_t e_ = (call);                                                      \
-    if (e_ != hipSuccess)                                                        \
-      return set_err(ctx, LSKUM_EHIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
-  } while (0)
-
-template <typename T>
-hipError_t dalloc(T **p, size_t count) {
-  return hipMalloc((void **)p, sizeof(T) * (count ? count : 1));
 }
 
 void drop_graph(lskum_ctx *c) {
@@ -445,6 +311,43 @@ bool params_ok(const lskum_params &p) {
           p.tile_block == 256);
 }
 
+// Least-squares coefficients of each sub-stencil (geometry only): the sums
+// of interior_fluxes_mod.h:51-61 over the members in conn order, then
+// (syy, -sxy)/det for the x-form solve and (sxx, -sxy)/det for the y-form.
+void compute_coef(const CloudLayout &L, const double *x, const double *y, const double *nxh,
+                  const double *nyh, double power, std::vector<double> &coef) {
+  const int32_t n = L.n;
+  coef.assign((size_t)8 * n, 0.0);
+  for (int32_t d = 0; d < n; d++) {
+    const int32_t v = L.perm[d];
+    const double nxv = nxh[v], nyv = nyh[v], tx = nyv, ty = -nxv;
+    double sxx[4] = {0, 0, 0, 0}, syy[4] = {0, 0, 0, 0}, sxy[4] = {0, 0, 0, 0};
+    for (int32_t t = L.conn_off_h[v]; t < L.conn_off_h[v + 1]; t++) {
+      const int32_t k = L.conn_h[t];
+      const double delx = x[k] - x[v], dely = y[k] - y[v];
+      const double dels = delx * tx + dely * ty, deln = delx * nxv + dely * nyv;
+      double dsw = dels, dnw = deln;
+      if (power != 0.0) {
+        const double w = std::pow(std::sqrt(dels * dels + deln * deln), power);
+        dsw = dels * w;
+        dnw = deln * w;
+      }
+      for (int sb = 0; sb < 4; sb++) {
+        if (!(L.mask_h[t] & (1u << sb))) continue;
+        sxx[sb] = sxx[sb] + dels * dsw;
+        syy[sb] = syy[sb] + deln * dnw;
+        sxy[sb] = sxy[sb] + dels * dnw;
+      }
+    }
+    for (int sb = 0; sb < 4; sb++) {
+      const double det = sxx[sb] * syy[sb] - sxy[sb] * sxy[sb];
+      const double inv = det != 0.0 ? 1.0 / det : 0.0;
+      coef[(size_t)8 * d + 2 * sb] = (sb < 2 ? syy[sb] : sxx[sb]) * inv;
+      coef[(size_t)8 * d + 2 * sb + 1] = -sxy[sb] * inv;
+    }
+  }
+}
+
 // Host set-up of one cloud: normals and sub-stencils in the caller's order,
 // stencil validation, device order, device-order images, least-squares
 // coefficients, force panels.
@@ -511,38 +414,12 @@ int build_layout(const lskum_cloud &cloud, const lskum_params &p, CloudLayout &L
       L.conn[L.off[d] + (t - b)] =
           (uint32_t)L.inv[cloud.conn[t]] | ((uint32_t)L.mask_h[t] << kMaskShift);
   }
-  // least-squares coefficients of each sub-stencil (geometry only): the
-  // sums of interior_fluxes_mod.h:51-61 over the members in conn order, then
-  // (syy, -sxy)/det for the x-form solve and (sxx, -sxy)/det for the y-form
-  L.coef.assign((size_t)8 * n, 0.0);
-  for (int32_t d = 0; d < n; d++) {
-    const int32_t v = L.perm[d];
-    const double nxv = L.nx_h[v], nyv = L.ny_h[v], tx = nyv, ty = -nxv;
-    double sxx[4] = {0, 0, 0, 0}, syy[4] = {0, 0, 0, 0}, sxy[4] = {0, 0, 0, 0};
-    for (int32_t t = cloud.conn_off[v]; t < cloud.conn_off[v + 1]; t++) {
-      const int32_t k = cloud.conn[t];
-      const double delx = cloud.x[k] - cloud.x[v], dely = cloud.y[k] - cloud.y[v];
-      const double dels = delx * tx + dely * ty, deln = delx * nxv + dely * nyv;
-      double dsw = dels, dnw = deln;
-      if (p.power != 0.0) {
-        const double w = std::pow(std::sqrt(dels * dels + deln * deln), p.power);
-        dsw = dels * w;
-        dnw = deln * w;
-      }
-      for (int sb = 0; sb < 4; sb++) {
-        if (!(L.mask_h[t] & (1u << sb))) continue;
-        sxx[sb] = sxx[sb] + dels * dsw;
-        syy[sb] = syy[sb] + deln * dnw;
-        sxy[sb] = sxy[sb] + dels * dnw;
-      }
-    }
-    for (int sb = 0; sb < 4; sb++) {
-      const double det = sxx[sb] * syy[sb] - sxy[sb] * sxy[sb];
-      const double inv = det != 0.0 ? 1.0 / det : 0.0;
-      L.coef[(size_t)8 * d + 2 * sb] = (sb < 2 ? syy[sb] : sxx[sb]) * inv;
-      L.coef[(size_t)8 * d + 2 * sb + 1] = -sxy[sb] * inv;
-    }
-  }
+  L.left_h.assign(cloud.left, cloud.left + n);
+  L.right_h.assign(cloud.right, cloud.right + n);
+  L.flag1_h.assign(cloud.flag_1, cloud.flag_1 + n);
+  L.flag2_h.assign(cloud.flag_2, cloud.flag_2 + n);
+  L.min_dist_h.assign(cloud.min_dist, cloud.min_dist + n);
+  compute_coef(L, cloud.x, cloud.y, L.nx_h.data(), L.ny_h.data(), p.power, L.coef);
   // force panels: the first shape_points points (compute_force_coeffs_mod.h:
   // 25-27 takes m = j)
   int shape_points = 0;
@@ -862,7 +739,89 @@ int create_impl(const lskum_cloud *clouds, int32_t n_clouds, const lskum_case *c
   return LSKUM_OK;
 }
 
-}  // namespace
+// Adjoint working arrays and the transpose of conn (single-case FP64
+// contexts with power = 0).
+int prepare_adjoint(lskum_ctx *c) {
+  if (c->n_cases() != 1 || c->mixed || c->prm.power != 0.0)
+    return set_err(c, LSKUM_EINVAL,
+                   "the adjoint and tangent need a single-case FP64 context with power 0");
+  if (c->adj_ready) return LSKUM_OK;
+  const CloudLayout &L = c->clouds[0];
+  const int32_t n = c->n;
+  const size_t E = (size_t)L.off[n];
+  std::vector<int32_t> tr_off(n + 1, 0), tr_edge(E), tr_owner(E);
+  for (size_t e = 0; e < E; e++) tr_off[(L.conn[e] & kIndexMask) + 1]++;
+  for (int32_t j = 0; j < n; j++) tr_off[j + 1] += tr_off[j];
+  std::vector<int32_t> pos(tr_off.begin(), tr_off.end() - 1);
+  for (int32_t i = 0; i < n; i++)
+    for (int32_t e = L.off[i]; e < L.off[i + 1]; e++) {
+      const int32_t j = (int32_t)(L.conn[e] & kIndexMask);
+      tr_edge[pos[j]] = e;
+      tr_owner[pos[j]++] = i;
+    }
+  AdjointBuffers &a = c->adj;
+  hipError_t e;
+#define AALLOC(ptr, count)                                                   \
+  if ((e = dalloc(&(ptr), (count))) != hipSuccess)                           \
+    return set_err(c, LSKUM_ENOMEM, std::string("hipMalloc " #ptr ": ") + hipGetErrorString(e));
+  AALLOC(a.own_q, (size_t)4 * n);
+  AALLOC(a.own_dq, (size_t)8 * n);
+  AALLOC(a.own_qm, (size_t)8 * n);
+  AALLOC(a.edge, 16 * E);
+  AALLOC(a.lq, (size_t)4 * n);
+  AALLOC(a.ldq[0], (size_t)8 * n);
+  AALLOC(a.ldq[1], (size_t)8 * n);
+  AALLOC(a.lqm, (size_t)8 * n);
+  AALLOC(a.ls, (size_t)8 * n);
+  AALLOC(a.arg, (size_t)8 * n);
+  AALLOC(a.tr_off, (size_t)n + 1);
+  AALLOC(a.tr_edge, E);
+  AALLOC(a.tr_owner, E);
+  AALLOC(a.tbase, (size_t)B_STRIDE * n);
+  AALLOC(a.tdq[0], (size_t)DQ_STRIDE * n);
+  AALLOC(a.tdq[1], (size_t)DQ_STRIDE * n);
+  AALLOC(a.tqm, (size_t)8 * n);
+  AALLOC(a.vec_in, (size_t)4 * n);
+  AALLOC(a.vec_out, (size_t)4 * n);
+#undef AALLOC
+  HIP_TRY(c, hipMemcpy(a.tr_off, tr_off.data(), tr_off.size() * sizeof(int32_t),
+                       hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(a.tr_edge, tr_edge.data(), E * sizeof(int32_t), hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(a.tr_owner, tr_owner.data(), E * sizeof(int32_t), hipMemcpyHostToDevice));
+  c->adj_ready = true;
+  return LSKUM_OK;
+}
+
+// q, dq (after the inner passes) and qmax/qmin of the current state, and
+// the origin of every extremum: the checkpoint recomputation
+int recompute_stages(lskum_ctx *c) {
+  const int inner = c->prm.inner_iterations;
+  int rc = do_q_variables(c);
+  if (!rc) rc = do_q_derivatives(c, inner);
+  if (!rc) rc = do_q_inner(c, inner);
+  if (rc) return rc;
+  launch_argext(c->dc(), c->base, c->adj.arg, c->stream);
+  return check_launch(c, "argext");
+}
+
+int upload_caller(lskum_ctx *c, const double *src, double *dev) {
+  std::vector<double> tmp((size_t)4 * c->n, 0.0);
+  permute_in(c, src, tmp.data(), 4);
+  HIP_TRY(c, hipMemcpyAsync(dev, tmp.data(), tmp.size() * sizeof(double), hipMemcpyHostToDevice,
+                            c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return LSKUM_OK;
+}
+
+int download_caller(lskum_ctx *c, const double *dev, double *dst) {
+  std::vector<double> tmp;
+  int rc = fetch_plain(c, dev, 4, tmp);
+  if (rc) return rc;
+  permute_out(c, tmp.data(), dst, 4);
+  return LSKUM_OK;
+}
+
+}  // namespace lskum
 
 extern "C" {
 
@@ -890,12 +849,6 @@ void lskum_destroy(lskum_ctx *ctx) {
   release(ctx);
   delete ctx;
 }
-
-#define CTX_OK(ctx)                                        \
-  do {                                                     \
-    if (!(ctx)) return LSKUM_EINVAL;                       \
-    if (!(ctx)->stream) return set_err(ctx, LSKUM_EINVAL, "context failed to initialise"); \
-  } while (0)
 
 int lskum_case_count(lskum_ctx *ctx, int32_t *n_cases, int32_t *points) {
   CTX_OK(ctx);
@@ -1205,88 +1158,6 @@ int lskum_kernel_time(lskum_ctx *ctx, int32_t kernel, double *avg_ms, int32_t *l
   return LSKUM_OK;
 }
 
-// Adjoint working arrays and the transpose of conn (single-case FP64
-// contexts with power = 0).
-static int prepare_adjoint(lskum_ctx *c) {
-  if (c->n_cases() != 1 || c->mixed || c->prm.power != 0.0)
-    return set_err(c, LSKUM_EINVAL,
-                   "the adjoint and tangent need a single-case FP64 context with power 0");
-  if (c->adj_ready) return LSKUM_OK;
-  const CloudLayout &L = c->clouds[0];
-  const int32_t n = c->n;
-  const size_t E = (size_t)L.off[n];
-  std::vector<int32_t> tr_off(n + 1, 0), tr_edge(E), tr_owner(E);
-  for (size_t e = 0; e < E; e++) tr_off[(L.conn[e] & kIndexMask) + 1]++;
-  for (int32_t j = 0; j < n; j++) tr_off[j + 1] += tr_off[j];
-  std::vector<int32_t> pos(tr_off.begin(), tr_off.end() - 1);
-  for (int32_t i = 0; i < n; i++)
-    for (int32_t e = L.off[i]; e < L.off[i + 1]; e++) {
-      const int32_t j = (int32_t)(L.conn[e] & kIndexMask);
-      tr_edge[pos[j]] = e;
-      tr_owner[pos[j]++] = i;
-    }
-  AdjointBuffers &a = c->adj;
-  hipError_t e;
-#define AALLOC(ptr, count)                                                   \
-  if ((e = dalloc(&(ptr), (count))) != hipSuccess)                           \
-    return set_err(c, LSKUM_ENOMEM, std::string("hipMalloc " #ptr ": ") + hipGetErrorString(e));
-  AALLOC(a.own_q, (size_t)4 * n);
-  AALLOC(a.own_dq, (size_t)8 * n);
-  AALLOC(a.own_qm, (size_t)8 * n);
-  AALLOC(a.edge, 16 * E);
-  AALLOC(a.lq, (size_t)4 * n);
-  AALLOC(a.ldq[0], (size_t)8 * n);
-  AALLOC(a.ldq[1], (size_t)8 * n);
-  AALLOC(a.lqm, (size_t)8 * n);
-  AALLOC(a.ls, (size_t)8 * n);
-  AALLOC(a.arg, (size_t)8 * n);
-  AALLOC(a.tr_off, (size_t)n + 1);
-  AALLOC(a.tr_edge, E);
-  AALLOC(a.tr_owner, E);
-  AALLOC(a.tbase, (size_t)B_STRIDE * n);
-  AALLOC(a.tdq[0], (size_t)DQ_STRIDE * n);
-  AALLOC(a.tdq[1], (size_t)DQ_STRIDE * n);
-  AALLOC(a.tqm, (size_t)8 * n);
-  AALLOC(a.vec_in, (size_t)4 * n);
-  AALLOC(a.vec_out, (size_t)4 * n);
-#undef AALLOC
-  HIP_TRY(c, hipMemcpy(a.tr_off, tr_off.data(), tr_off.size() * sizeof(int32_t),
-                       hipMemcpyHostToDevice));
-  HIP_TRY(c, hipMemcpy(a.tr_edge, tr_edge.data(), E * sizeof(int32_t), hipMemcpyHostToDevice));
-  HIP_TRY(c, hipMemcpy(a.tr_owner, tr_owner.data(), E * sizeof(int32_t), hipMemcpyHostToDevice));
-  c->adj_ready = true;
-  return LSKUM_OK;
-}
-
-// q, dq (after the inner passes) and qmax/qmin of the current state, and
-// the origin of every extremum: the checkpoint recomputation
-static int recompute_stages(lskum_ctx *c) {
-  const int inner = c->prm.inner_iterations;
-  int rc = do_q_variables(c);
-  if (!rc) rc = do_q_derivatives(c, inner);
-  if (!rc) rc = do_q_inner(c, inner);
-  if (rc) return rc;
-  launch_argext(c->dc(), c->base, c->adj.arg, c->stream);
-  return check_launch(c, "argext");
-}
-
-static int upload_caller(lskum_ctx *c, const double *src, double *dev) {
-  std::vector<double> tmp((size_t)4 * c->n, 0.0);
-  permute_in(c, src, tmp.data(), 4);
-  HIP_TRY(c, hipMemcpyAsync(dev, tmp.data(), tmp.size() * sizeof(double), hipMemcpyHostToDevice,
-                            c->stream));
-  HIP_TRY(c, hipStreamSynchronize(c->stream));
-  return LSKUM_OK;
-}
-
-static int download_caller(lskum_ctx *c, const double *dev, double *dst) {
-  std::vector<double> tmp;
-  int rc = fetch_plain(c, dev, 4, tmp);
-  if (rc) return rc;
-  permute_out(c, tmp.data(), dst, 4);
-  return LSKUM_OK;
-}
-
 int lskum_flux_vjp(lskum_ctx *ctx, const double *lam, double *grad) {
   CTX_OK(ctx);
   if (!lam || !grad) return set_err(ctx, LSKUM_EINVAL, "null argument");
@@ -1296,15 +1167,17 @@ int lskum_flux_vjp(lskum_ctx *ctx, const double *lam, double *grad) {
   if (rc) return rc;
   const DevCloud dc = ctx->dc();
   AdjointBuffers &a = ctx->adj;
-  launch_flux_vjp(dc, a, ctx->base, ctx->dqb[ctx->cur], ctx->qm, ctx->delta, a.vec_in,
+  launch_flux_vjp(dc, a, ctx->base, ctx->dqb[ctx->cur], ctx->qm, ctx->delta, a.vec_in, nullptr,
                   ctx->stream);
   int b = 0;  // ldq[b]: adjoint of the derivatives the flux read
   for (int p = 0; p < ctx->prm.inner_iterations; p++) {
-    launch_ls_vjp(dc, a, ctx->base, a.ldq[b], a.ldq[1 - b], false, ctx->stream);
+    launch_ls_vjp(dc, a, ctx->base, a.ldq[b], a.ldq[1 - b], false, nullptr, nullptr, nullptr,
+                  ctx->stream);
     b = 1 - b;
   }
-  launch_ls_vjp(dc, a, ctx->base, a.ldq[b], nullptr, true, ctx->stream);
-  launch_qvar_vjp(dc, a, ctx->prim, a.vec_out, ctx->stream);
+  launch_ls_vjp(dc, a, ctx->base, a.ldq[b], nullptr, true, nullptr, nullptr, nullptr,
+                ctx->stream);
+  launch_qvar_vjp(dc, a, ctx->prim, a.vec_out, false, ctx->stream);
   rc = check_launch(ctx, "flux_vjp");
   return rc ? rc : download_caller(ctx, a.vec_out, grad);
 }

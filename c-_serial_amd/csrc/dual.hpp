@@ -12,6 +12,13 @@
 
 namespace lskum {
 
+// the double overloads stay visible beside the Dual ones declared here
+using ::erf;
+using ::exp;
+using ::fabs;
+using ::log;
+using ::sqrt;
+
 template <int N>
 struct Dual {
   double v;
@@ -98,6 +105,37 @@ __device__ __forceinline__ Dual<N> erf(const Dual<N> &a) {
   const double g = 1.1283791670955126 * ::exp(-a.v * a.v);
   for (int j = 0; j < N; j++) r.d[j] = g * a.d[j];
   return r;
+}
+
+// Comparisons look at the values: a branch (a limiter switch, a min or max
+// changing hands) is taken as the state takes it, and the derivative is that
+// of the branch taken.
+template <int N>
+__device__ __forceinline__ bool operator<(const Dual<N> &a, const Dual<N> &b) {
+  return a.v < b.v;
+}
+template <int N>
+__device__ __forceinline__ bool operator>(const Dual<N> &a, const Dual<N> &b) {
+  return a.v > b.v;
+}
+template <int N>
+__device__ __forceinline__ bool operator<=(const Dual<N> &a, const Dual<N> &b) {
+  return a.v <= b.v;
+}
+template <int N>
+__device__ __forceinline__ bool operator!=(const Dual<N> &a, const Dual<N> &b) {
+  return a.v != b.v;
+}
+template <int N>
+__device__ __forceinline__ Dual<N> fabs(const Dual<N> &a) {
+  return a.v < 0.0 ? -a : a;
+}
+
+// the value of a scalar of either kind
+__device__ __forceinline__ double vpart(double a) { return a; }
+template <int N>
+__device__ __forceinline__ double vpart(const Dual<N> &a) {
+  return a.v;
 }
 
 }  // namespace lskum

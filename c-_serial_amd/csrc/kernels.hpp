@@ -146,18 +146,30 @@ struct AdjointBuffers {
   int32_t *arg, *tr_off, *tr_edge, *tr_owner;
   double *tbase, *tdq[2], *tqm, *vec_in, *vec_out;
 };
+// Geometry adjoints the reverse kernels accumulate (+=), device order:
+// edge [n_edges][2] d/d(delx, dely) of each (point, neighbour) pair, nrm
+// [n][2] d/d(nx, ny), coef [n][8] d/d of the sub-stencil coefficients,
+// delta [n] d/d of the local time step.
+struct GeoAdjoint {
+  double *edge, *nrm, *coef, *delta;
+};
 void launch_argext(const DevCloud &c, const double *base, int32_t *arg, hipStream_t s);
-// flux reverse: lam [4][n] (adjoint of flux_res) -> lq, ldq[0], lqm
+// flux reverse: lam [4][n] (adjoint of flux_res) -> lq, ldq[0], lqm, and
+// with geo non-null the geometry adjoints (+=)
 void launch_flux_vjp(const DevCloud &c, const AdjointBuffers &a, const double *base,
                      const double *dq, const double *qm, const double *delta, const double *lam,
-                     hipStream_t s);
+                     const GeoAdjoint *geo, hipStream_t s);
 // least-squares pass reverse from the adjoint lo of its output: adds to lq;
 // an inner pass (extrema false) writes the adjoint of its input
-// derivatives to ldq_in; q_derivatives (extrema true) routes lqm
+// derivatives to ldq_in; q_derivatives (extrema true) routes lqm.  With
+// geo_edge non-null also the pairs' geometry adjoints (+=), from the pass's
+// output derivatives `out` and, for an inner pass, its input dq_in.
 void launch_ls_vjp(const DevCloud &c, const AdjointBuffers &a, const double *base,
-                   const double *lo, double *ldq_in, bool extrema, hipStream_t s);
+                   const double *lo, double *ldq_in, bool extrema, const double *out,
+                   const double *dq_in, double *geo_edge, hipStream_t s);
+// grad = J^T lq, or grad += J^T lq
 void launch_qvar_vjp(const DevCloud &c, const AdjointBuffers &a, const double *prim,
-                     double *grad, hipStream_t s);
+                     double *grad, bool accumulate, hipStream_t s);
 void launch_qvar_jvp(const DevCloud &c, const double *prim, const double *v, double *tbase,
                      hipStream_t s);
 void launch_ext_jvp(const DevCloud &c, const int32_t *arg, const double *tbase, double *tqm,
@@ -165,5 +177,42 @@ void launch_ext_jvp(const DevCloud &c, const int32_t *arg, const double *tbase, 
 void launch_flux_jvp(const DevCloud &c, const double *base, const double *dq, const double *qm,
                      const double *delta, const double *tbase, const double *tdq,
                      const double *tqm, double *out, hipStream_t s);
+
+// ---------------------------------------------- whole iterations (trajectory.hip)
+// Stage update reverse at every point (FP64, one case): from pbar, the
+// adjoint of the state the stage wrote, to pbar = the adjoint of the
+// stage's input state (overwritten), pold_bar += that of the iteration's
+// start state, rbar = that of the flux residual, nrm_bar += that of the
+// normals.
+void launch_su_vjp(const DevCloud &c, int rk, const double *prim, const double *prim_old,
+                   const double *fres, const FlowConsts *fc, double *pbar, double *pold_bar,
+                   double *rbar, double *nrm_bar, hipStream_t s);
+// func_delta reverse from dbar [n]: the pair that set each point's delta
+// gets its geometry adjoint (geo_edge +=) and the speed of its neighbour
+// (base: the speed of the state prim) passes it on to pbar (+=).
+void launch_delta_vjp(const DevCloud &c, const AdjointBuffers &a, const double *base,
+                      const double *prim, double cfl, const double *dbar, int32_t *amin_e,
+                      double *amin_v, double *geo_edge, double *pbar, hipStream_t s);
+// y += x over count doubles
+void launch_axpy(size_t count, const double *x, double *y, hipStream_t s);
+
+// Forward-mode (value, tangent) run of whole iterations, every array device
+// order in pairs {value, derivative along the seeded direction}.
+struct Tan {
+  double v, d;
+};
+struct TanState {
+  Tan *x, *y, *nx, *ny, *coef;  // [n], [n][8]
+  Tan *prim, *prim_old, *fres;  // [4][n]
+  Tan *q, *speed, *delta;       // [n][4], [n], [n]
+  Tan *dq[2], *qm;              // [n][8]
+};
+// normals of the wall and outer points (lr: [n][2] device indices of their
+// left and right neighbours) and sub-stencil coefficients from x, y
+void launch_tan_geometry(const DevCloud &c, const int32_t *lr, const TanState &t,
+                         hipStream_t s);
+// one fpi_solver iteration (eps2 read from base, power 0)
+void launch_tan_iteration(const DevCloud &c, const double *base, const FlowConsts *fc,
+                          double cfl, int inner, const TanState &t, hipStream_t s);
 
 }  // namespace lskum

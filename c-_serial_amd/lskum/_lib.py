@@ -117,6 +117,11 @@ _SIGS = [
     ("lskum_flux_vjp", C.c_int, [C.c_void_p, _dp, _dp]),
     ("lskum_flux_jvp", C.c_int, [C.c_void_p, _dp, _dp]),
     ("lskum_forces_grad", C.c_int, [C.c_void_p, _dp, _dp, _dp]),
+    ("lskum_set_coordinates", C.c_int, [C.c_void_p, _dp, _dp]),
+    ("lskum_shape_vjp", C.c_int, [C.c_void_p, C.c_int32, _dp, _dp, _dp, _dp, _dp]),
+    ("lskum_shape_jvp", C.c_int, [C.c_void_p, C.c_int32, _dp, _dp, _dp, _dp, _dp, _dp]),
+    ("lskum_hicks_henne_fields", C.c_int, [C.POINTER(CloudView), C.c_int32, C.c_double,
+                                           C.c_double, C.c_int32, C.c_double, _dp]),
     ("lskum_kat_fluxes", C.c_int, [C.c_int32, _dp, _dp, C.c_int32]),
     ("lskum_kat_qtilde", C.c_int, [C.c_int32, _dp, _dp]),
 ]

@@ -18,8 +18,9 @@ INCLUDE = os.path.join(REPO, "include")
 LIB_PATH = os.path.join(PKG_DIR, "liblskum.so")
 BUILD_DIR = os.path.join(ROOT, "_build")
 
-HIP_SOURCES = ["kernels.hip", "kernels_mixed.hip", "adjoint.hip", "api.hip"]
-CXX_SOURCES = ["setup.cpp", "cloud.cpp", "restart.cpp"]
+HIP_SOURCES = ["kernels.hip", "kernels_mixed.hip", "adjoint.hip", "trajectory.hip", "api.hip",
+               "shape_api.hip"]
+CXX_SOURCES = ["setup.cpp", "cloud.cpp", "restart.cpp", "shape.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("LSKUM_ARCH", "gfx950")
 

@@ -193,6 +193,51 @@ class Solver:
         self._chk(L.lib().lskum_forces_grad(self._ctx, *[L.dptr(a) for a in g]))
         return tuple(g)
 
+    # -------------------------------------------------- shape sensitivities
+    def set_coordinates(self, x, y):
+        """Moves the points to (x, y), keeping the neighbour lists, their
+        sub-stencil split and min_dist; normals and least-squares
+        coefficients follow."""
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(self.n)
+        y = np.ascontiguousarray(y, dtype=np.float64).reshape(self.n)
+        self._chk(L.lib().lskum_set_coordinates(self._ctx, L.dptr(x), L.dptr(y)))
+
+    def shape_vjp(self, n_iter, w=(1.0, 0.0, 0.0)):
+        """J = w . (Cl, Cd, Cm) after n_iter iterations from the current
+        state, and its gradients with respect to the coordinates and the
+        starting state by the discrete adjoint: (J, x_bar, y_bar, prim_bar)."""
+        w = np.asarray(w, dtype=np.float64).reshape(3).copy()
+        J = C.c_double()
+        xb, yb, pb = np.empty(self.n), np.empty(self.n), np.empty((4, self.n))
+        self._chk(L.lib().lskum_shape_vjp(self._ctx, n_iter, L.dptr(w), C.byref(J),
+                                          L.dptr(xb), L.dptr(yb), L.dptr(pb)))
+        return J.value, xb, yb, pb
+
+    def shape_jvp(self, n_iter, w=(1.0, 0.0, 0.0), dx=None, dy=None, dprim=None):
+        """(J, dJ) with dJ the derivative of J along the coordinate direction
+        (dx, dy) and the starting-state direction dprim (None: zero), by a
+        forward-mode run of the same iterations."""
+        w = np.asarray(w, dtype=np.float64).reshape(3).copy()
+
+        def arr(a, shape):
+            return None if a is None else np.ascontiguousarray(a, dtype=np.float64).reshape(shape)
+
+        dx, dy, dprim = arr(dx, self.n), arr(dy, self.n), arr(dprim, (4, self.n))
+        ptr = lambda a: None if a is None else L.dptr(a)
+        J, dJ = C.c_double(), C.c_double()
+        self._chk(L.lib().lskum_shape_jvp(self._ctx, n_iter, L.dptr(w), ptr(dx), ptr(dy),
+                                          ptr(dprim), C.byref(J), C.byref(dJ)))
+        return J.value, dJ.value
+
+    def objective_after(self, n_iter, w=(1.0, 0.0, 0.0)):
+        """J = w . (Cl, Cd, Cm) after n_iter plain iterations from the current
+        state, which is restored afterwards (the iteration counter is not)."""
+        p0 = self.prim
+        self.q_lskum(n_iter)
+        cl, cd, cm = self.compute_cl_cd_cm()
+        self.set_state(p0)
+        return float(np.dot(w, (cl, cd, cm)))
+
     def residual_of(self, prim):
         """flux_res at the state prim with delta held at its last value: the
         map lskum_flux_vjp / lskum_flux_jvp differentiate."""

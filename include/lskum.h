@@ -313,6 +313,58 @@ int lskum_flux_jvp(lskum_ctx *ctx, const double *v, double *out);
 int lskum_forces_grad(lskum_ctx *ctx, double *dcl, double *dcd, double *dcm);
 
 /*
+ * Shape sensitivities (no reference counterpart).
+ *
+ * lskum_set_coordinates moves the points of a single-case context to
+ * (x, y), caller order, and recomputes what the set-up derives from them:
+ * the wall and outer normals (point_normals_mod.h:17-55) and the
+ * sub-stencil least-squares coefficients.  The neighbour lists, their
+ * sub-stencil split (generate_connectivity_mod.h) and min_dist stay as
+ * built.  A move that leaves a sub-stencil degenerate is refused
+ * (LSKUM_ESTENCIL) and the context keeps its coordinates.
+ *
+ * The objective is J = w[0] Cl + w[1] Cd + w[2] Cm of the state that n_iter
+ * whole iterations (fpi_solver_mod.h:14-150) reach from the context's
+ * current state P0, as a function of the coordinates and of P0:
+ *   lskum_shape_vjp: J, its gradient x_bar, y_bar [n] (caller order) and,
+ *     if prim_bar is non-NULL, [4][n] with respect to P0, by the discrete
+ *     adjoint of the n_iter iterations -- one reverse sweep that keeps the
+ *     state at the start of every iteration (32 bytes per point per
+ *     iteration on the device) and recomputes each stage's q-derivatives,
+ *     limiter and fluxes from it;
+ *   lskum_shape_jvp: J and dJ = dJ/d(x, y) . (dx, dy) + dJ/dP0 . dprim
+ *     (any of dx, dy, dprim may be NULL for zero), by a forward-mode run of
+ *     the same iterations in dual numbers -- independent derivative code
+ *     that checks the adjoint.
+ * Neither changes the context's state or iteration counter.  min_dist (the
+ * limiter's epsilon) and the sub-stencil split are held fixed, and the
+ * limiter's switches, the qmax/qmin and func_delta minima are
+ * differentiated on the branch the state takes.  Single-case FP64 contexts
+ * with power = 0 (else LSKUM_EINVAL).
+ */
+int lskum_set_coordinates(lskum_ctx *ctx, const double *x, const double *y);
+int lskum_shape_vjp(lskum_ctx *ctx, int32_t n_iter, const double w[3], double *J,
+                    double *x_bar, double *y_bar, double *prim_bar);
+int lskum_shape_jvp(lskum_ctx *ctx, int32_t n_iter, const double w[3], const double *dx,
+                    const double *dy, const double *dprim, double *J, double *dJ);
+
+/*
+ * Hicks-Henne design variables of a cloud whose wall points come first in
+ * the generator's order (clockwise from the trailing edge, lower surface
+ * first): bump j has its peak at p_j, evenly spaced over [p_lo, p_hi],
+ * b_j(x) = sin(pi x^(ln 0.5/ln p_j))^width_exponent, n_per_side bumps on
+ * each surface, moving it outward (variables 0..n_per_side-1 the upper
+ * surface, +y; the rest the lower, -y).  Other points follow by
+ * inverse-distance weights (|p - w|^-3 over the wall points), faded by
+ * (1 - d/decay_radius)^2 with d their distance from the wall.  Writes
+ * dy[2 n_per_side][n]: the y displacement of every point per unit
+ * amplitude (x does not move).  No reference counterpart.
+ */
+int lskum_hicks_henne_fields(const lskum_cloud *cloud, int32_t n_per_side, double p_lo,
+                             double p_hi, int32_t width_exponent, double decay_radius,
+                             double *dy);
+
+/*
  * Autotuner (no reference counterpart): builds the given cases once per
  * candidate layout -- points per block of the stencil kernels 64, 128 or
  * 256 (the block shape; one point per thread, whose neighbour list the
