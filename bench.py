@@ -11,7 +11,7 @@ same start and states its Cl/Cd/residual differences from the FP64 run.
 Prints one JSON line.
 
   python bench.py [--gpus 1] [--steps 50] [--warmup 5] [--no-cpu-baseline]
-                  [--no-mixed] [--workload batch]
+                  [--no-mixed] [--workload batch|shape]
 """
 import argparse
 import json
@@ -193,6 +193,87 @@ def run_batch(args):
     }
 
 
+def run_shape(args):
+    """Shape gradients (the adjoint workload): the 1,000,940-point NACA0012
+    cloud (wall 1,500, 806 x 1,240 O-grid, k = 24) at M 0.8, AoA 1.25 deg,
+    CFL 0.2, the reference's limiter constant K = 150, no inner passes,
+    advanced `--converge` iterations from free stream;
+    20 Hicks-Henne bumps (10 per surface, peaks over x/c 0.05..0.95, width
+    exponent 3, inverse-distance propagation within 0.5 chord of the wall).
+    Timed: one adjoint sweep over `--adjoint-iters` iterations per objective
+    (Cl, Cd), each giving all 20 derivatives.  value = gradients
+    (objectives x design variables) per second of adjoint time.  Checked on
+    sample variables: the tangent-linear run and central differences
+    (h = 1e-6) of the plain iterations, with the one-sided differences that
+    show whether a limiter switch lies within +-h."""
+    from lskum import shape
+    t = time.perf_counter()
+    cloud = lskum.workload("naca0012_1m_k24")
+    log(f"shape: cloud {cloud.n} points, {time.perf_counter() - t:.1f}s")
+    s = Solver(cloud, mach=0.8, aoa=1.25, cfl=0.2, vl_const=150.0, power=0.0,
+               inner_iterations=0)
+    t = time.perf_counter()
+    done = 0
+    while done < args.converge:
+        step = min(500, args.converge - done)
+        rn, rs = s.q_lskum(step)
+        done += step
+        log(f"shape: converge {done}/{args.converge}: res_new {rn[-1]:.3e} residue {rs[-1]:.2f}")
+    conv_s = time.perf_counter() - t
+    t = time.perf_counter()
+    fields = shape.hicks_henne_fields(cloud, n_per_side=10, p_lo=0.05, p_hi=0.95,
+                                      width_exponent=3, decay_radius=0.5)
+    log(f"shape: fields {fields.shape} in {time.perf_counter() - t:.1f}s")
+    K = args.adjoint_iters
+    objectives = {"cl": (1.0, 0.0, 0.0), "cd": (0.0, 1.0, 0.0)}
+    grads, Js, sweep_s = {}, {}, {}
+    for name, w in objectives.items():
+        t = time.perf_counter()
+        Js[name], grads[name] = shape.design_gradient(s, fields, K, w)
+        sweep_s[name] = time.perf_counter() - t
+        log(f"shape: adjoint sweep {name} over {K} iterations in {sweep_s[name]:.1f}s")
+    t = time.perf_counter()
+    s.objective_after(K)
+    fwd_s = time.perf_counter() - t
+    checks = {}
+    for name, j in (("cl", 3), ("cd", 3), ("cl", 16)):
+        w = objectives[name]
+        t = time.perf_counter()
+        tan = shape.design_tangent(s, fields[j], K, w)
+        tan_s = time.perf_counter() - t
+        fd, fwd, bwd = shape.design_fd(s, fields[j], K, w, h=1e-6, one_sided=True)
+        g = float(grads[name][j])
+        checks[f"{name}_var{j}"] = {
+            "adjoint": g, "tangent": tan, "fd_central": fd, "fd_forward": fwd,
+            "fd_backward": bwd, "rel_err_tangent": abs(tan - g) / abs(g),
+            "rel_err_fd": abs(fd - g) / abs(g),
+            "one_sided_agree": abs(fwd - bwd) <= 1e-5 * abs(fd), "tangent_s": tan_s}
+        log(f"shape: check {name} var {j}: {checks[f'{name}_var{j}']}")
+    total = sum(sweep_s.values())
+    n_grads = len(objectives) * fields.shape[0]
+    return {
+        "metric": METRIC, "value": n_grads / total, "unit": "gradients/s",
+        "n_gpus": 1, "steps": K, "warmup": 0, "ms_per_step": 1e3 * total / (len(objectives) * K),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "shape_naca0012_1m_k24", "points": cloud.n, "k": 24,
+                   "inner_iterations": 0, "mach": 0.8, "aoa_deg": 1.25, "cfl": 0.2,
+                   "vl_const": 150.0, "converge_iterations": args.converge,
+                   "adjoint_iterations": K, "design_variables": int(fields.shape[0]),
+                   "bumps": "10 upper + 10 lower, x/c 0.05..0.95, exponent 3",
+                   "propagation": "inverse distance |d|^-3, faded to 0 at 0.5 chord"},
+        "state": {"res_new": float(s.res_new), "residue": float(s.residue),
+                  "converge_s": conv_s},
+        "objectives": Js,
+        "adjoint_sweep_s": sweep_s,
+        "adjoint_point_iterations_per_s": cloud.n * K * len(objectives) / total,
+        "forward_s_same_iterations": fwd_s,
+        "adjoint_cost_vs_forward": total / len(objectives) / fwd_s,
+        "gradients": {k: [float(v) for v in g] for k, g in grads.items()},
+        "checks": checks,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -206,6 +287,10 @@ def main():
     ap.add_argument("--tile-block", type=int, default=0, choices=[0, 64, 128, 256])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--converge", type=int, default=10000,
+                    help="shape workload: iterations before the adjoint")
+    ap.add_argument("--adjoint-iters", type=int, default=300,
+                    help="shape workload: iterations the adjoint differentiates")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; the q-LSKUM iteration would shard "
@@ -215,6 +300,9 @@ def main():
 
     if args.workload == "batch":
         print(json.dumps(run_batch(args)), flush=True)
+        return
+    if args.workload == "shape":
+        print(json.dumps(run_shape(args)), flush=True)
         return
     t = time.perf_counter()
     cloud = lskum.workload(args.workload)

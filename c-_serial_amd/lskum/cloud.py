@@ -146,6 +146,12 @@ WORKLOADS = {
     # NACA0012, 2,000 wall, 500 x 5,000, k = 24 (stretch matched to spacing)
     "naca0012_2p5m_k24": dict(naca="0012", n_wall=2000, n_radial=500, n_circ=5000,
                               stretch=0.0, k=24, seed=2),
+    # NACA0012, 1,500 wall, 806 x 1,240 near-isotropic O-grid, k = 24 (the
+    # shape-gradient cloud; a 266 x 3,750 layout of the same size, far
+    # cells 14 times longer radially than around, loses stability within
+    # a few hundred iterations at M 0.8)
+    "naca0012_1m_k24": dict(naca="0012", n_wall=1500, n_radial=806, n_circ=1240,
+                            stretch=0.0, k=24, seed=5),
     # NACA0012, 4,000 wall, 1,000 x 10,000, k = 24
     "naca0012_10m_k24": dict(naca="0012", n_wall=4000, n_radial=1000, n_circ=10000,
                              stretch=0.0, k=24, seed=3),

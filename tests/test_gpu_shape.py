@@ -11,8 +11,7 @@ min_dist held as built.  Checks:
     sweep equals the tangent run's dJ in random directions, to rounding --
     the two share no derivative code;
   - central differences (h = 1e-6) of the plain iterations along
-    Hicks-Henne design fields, where the forward and backward differences
-    agree (a limiter switch within +-h gives two one-sided slopes).
+    Hicks-Henne design fields, with the reference's limiter constant.
 """
 import numpy as np
 import pytest
@@ -103,17 +102,26 @@ def test_adjoint_tangent_duality(name, inner, w):
         assert abs(rev - dJ) <= 1e-10 * max(abs(dJ), scale)
 
 
-def test_design_gradient_matches_central_differences():
-    cloud, s = _developed("naca0012_3620_k24", 0, 60)
+@pytest.mark.parametrize("inner", [0, 2])
+def test_design_gradient_matches_central_differences(inner):
+    # K = 150, the reference's VL_CONST: epsilon^2 = (K min_dist)^6 is large
+    # next to the squared limiter increments, so phi stays within about
+    # (del/epsilon)^2 of 1 and its jump at |del_neg| = 1e-5 is negligible --
+    # the iterations are smooth enough in the coordinates for h = 1e-6
+    # differences.  (With K = 5 that jump is of order one where the stencil
+    # has an extremum, and differences across it measure the jump.)
+    cloud = lskum.workload("naca0012_3620_k24")
+    s = Solver(cloud, mach=0.8, aoa=1.25, cfl=0.2, vl_const=150.0, inner_iterations=inner)
+    s.q_lskum(60)
     fields = shape.hicks_henne_fields(cloud, n_per_side=10)
     n_iter = 10
     for w in [(1.0, 0.0, 0.0), (0.0, 1.0, 0.0)]:
         J, g = shape.design_gradient(s, fields, n_iter, w)
         for j in (2, 7, 12, 17):
-            fd, fwd, bwd = shape.design_fd(s, fields[j], n_iter, w, h=1e-6, one_sided=True)
+            fd = shape.design_fd(s, fields[j], n_iter, w, h=1e-6)
             tan = shape.design_tangent(s, fields[j], n_iter, w)
-            print(f"case: design FD w={w} var {j}: adjoint {g[j]:.10e} tangent {tan:.10e} "
-                  f"central {fd:.10e} one-sided {fwd:.6e} {bwd:.6e}", flush=True)
+            print(f"case: design FD inner={inner} w={w} var {j}: adjoint {g[j]:.12e} "
+                  f"tangent {tan:.12e} central {fd:.12e} rel {abs(fd - g[j]) / abs(g[j]):.2e}",
+                  flush=True)
             assert abs(tan - g[j]) <= 1e-10 * abs(g[j])
-            if abs(fwd - bwd) <= 1e-5 * abs(fd):  # no switch within +-h
-                assert abs(fd - g[j]) <= 1e-5 * abs(g[j])
+            assert abs(fd - g[j]) <= 1e-5 * abs(g[j])
