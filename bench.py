@@ -196,8 +196,10 @@ def run_batch(args):
 def run_shape(args):
     """Shape gradients (the adjoint workload): the 1,000,940-point NACA0012
     cloud (wall 1,500, 806 x 1,240 O-grid, k = 24) at M 0.8, AoA 1.25 deg,
-    CFL 0.2, the reference's limiter constant K = 150, no inner passes,
-    advanced `--converge` iterations from free stream;
+    the reference's limiter constant K = 150, no inner passes, CFL 0.1 (at
+    0.2 the trailing-edge point, whose nearest wall neighbour lies 4.4e-6
+    chords away, runs away after about 4,000 iterations), advanced
+    `--converge` iterations from free stream;
     20 Hicks-Henne bumps (10 per surface, peaks over x/c 0.05..0.95, width
     exponent 3, inverse-distance propagation within 0.5 chord of the wall).
     Timed: one adjoint sweep over `--adjoint-iters` iterations per objective
@@ -210,7 +212,7 @@ def run_shape(args):
     t = time.perf_counter()
     cloud = lskum.workload("naca0012_1m_k24")
     log(f"shape: cloud {cloud.n} points, {time.perf_counter() - t:.1f}s")
-    s = Solver(cloud, mach=0.8, aoa=1.25, cfl=0.2, vl_const=150.0, power=0.0,
+    s = Solver(cloud, mach=0.8, aoa=1.25, cfl=0.1, vl_const=150.0, power=0.0,
                inner_iterations=0)
     t = time.perf_counter()
     done = 0
@@ -257,7 +259,7 @@ def run_shape(args):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": "shape_naca0012_1m_k24", "points": cloud.n, "k": 24,
-                   "inner_iterations": 0, "mach": 0.8, "aoa_deg": 1.25, "cfl": 0.2,
+                   "inner_iterations": 0, "mach": 0.8, "aoa_deg": 1.25, "cfl": 0.1,
                    "vl_const": 150.0, "converge_iterations": args.converge,
                    "adjoint_iterations": K, "design_variables": int(fields.shape[0]),
                    "bumps": "10 upper + 10 lower, x/c 0.05..0.95, exponent 3",
@@ -287,7 +289,7 @@ def main():
     ap.add_argument("--tile-block", type=int, default=0, choices=[0, 64, 128, 256])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
-    ap.add_argument("--converge", type=int, default=10000,
+    ap.add_argument("--converge", type=int, default=8000,
                     help="shape workload: iterations before the adjoint")
     ap.add_argument("--adjoint-iters", type=int, default=300,
                     help="shape workload: iterations the adjoint differentiates")
