@@ -9,13 +9,13 @@
 // The reverse pass recomputes q, dq and qmax/qmin from prim (the checkpoint
 // is prim alone; nothing of a forward run is stored) and then runs each
 // stage backwards:
-//  - flux: per (point, neighbour) pair, the 4x4 Jacobians of the summed
-//    sub-stencil fluxes with respect to q~ at each end, by forward-mode dual
-//    numbers through the same templated leaf physics the forward kernel
-//    uses, then the limiter and reconstruction partials by hand.  The
-//    neighbour's share of each pair goes to a per-edge buffer and a
-//    transpose-CSR gather sums it per point in a fixed order (no atomics,
-//    deterministic).
+//  - flux: per (point, neighbour) pair, each end's summed sub-stencil
+//    fluxes run forward and then in reverse (the split and quadrant fluxes,
+//    the kinetic terms and q~ -> primitive, each reversed by hand), giving
+//    the adjoint of q~ at both ends and of the flux frame; then the limiter
+//    and reconstruction partials.  The neighbour's share of each pair goes
+//    to a per-edge buffer and a transpose-CSR gather sums it per point in a
+//    fixed order (no atomics, deterministic).
 //  - qmax/qmin: routed to the point each extremum came from.
 //  - inner passes and q_derivatives: linear least-squares stencils, whose
 //    transposes are applied by the same gather pattern.
@@ -80,22 +80,11 @@ __device__ __forceinline__ Recon recon(double q, double d, double qmax, double q
 
 // Sum over the sub-stencil fluxes a (point, neighbour) pair feeds of
 // w_s G_s(e), for one end's terms e: the selection of the FP64 flux kernel
-// (x+ or x-, both on a tie; interior y+ / y-, wall y-, outer y+).  With la
-// given, also g[j] = sum_r la_r G_j,r (values) of each flux summed: j = 0
-// the x flux, 1 the x- flux of a tie, 2 the y flux, 3 the y- flux of a tie
-// (0 where the pair has none).
+// (x+ or x-, both on a tie; interior y+ / y-, wall y-, outer y+).
 template <typename T>
 __device__ __forceinline__ void end_sum(const EndTerms<T> &e, int type, uint32_t m, double s2b,
-                                        const double cf[8], double dsw, double dnw, T out[4],
-                                        const double *la = nullptr, double *g = nullptr) {
+                                        const double cf[8], double dsw, double dnw, T out[4]) {
   T G[4];
-  auto contract = [&](int j) {
-    if (!la) return;
-    double t = 0.0;
-    for (int r = 0; r < 4; r++) t += la[r] * vpart(G[r]);
-    g[j] = t;
-  };
-  if (la) g[0] = g[1] = g[2] = g[3] = 0.0;
   const bool xp = (m & XPOS) != 0;
   if (type == 1)
     split_x(e, T(xp ? 1.0 : -1.0), G);
@@ -103,7 +92,6 @@ __device__ __forceinline__ void end_sum(const EndTerms<T> &e, int type, uint32_t
     quad_x(e, T(xp ? 1.0 : -1.0), T(s2b), G);
   const double wx = xp ? cf[0] * dsw + cf[1] * dnw : cf[2] * dsw + cf[3] * dnw;
   for (int r = 0; r < 4; r++) out[r] = G[r] * wx;
-  contract(0);
   if (xp && (m & XNEG)) {
     if (type == 1)
       split_x(e, T(-1.0), G);
@@ -111,52 +99,208 @@ __device__ __forceinline__ void end_sum(const EndTerms<T> &e, int type, uint32_t
       quad_x(e, T(-1.0), T(s2b), G);
     const double wn = cf[2] * dsw + cf[3] * dnw;
     for (int r = 0; r < 4; r++) out[r] = out[r] + G[r] * wn;
-    contract(1);
   }
   const bool yp = type == 1 ? (m & YPOS) != 0 : type == 2;
   split_y(e, T(yp ? 1.0 : -1.0), G);
   const double wy = yp ? cf[4] * dnw + cf[5] * dsw : cf[6] * dnw + cf[7] * dsw;
   for (int r = 0; r < 4; r++) out[r] = out[r] + G[r] * wy;
-  contract(2);
   if (type == 1 && yp && (m & YNEG)) {
     split_y(e, T(-1.0), G);
     const double wn = cf[6] * dnw + cf[7] * dsw;
     for (int r = 0; r < 4; r++) out[r] = out[r] + G[r] * wn;
-    contract(3);
   }
 }
 
 // end_sum for the state q~ with N seeded directions: the value and, per
-// direction, its derivative along the seed.
+// direction, its derivative along the seed (the tangent kernel's Dual<1>).
 template <int N>
 __device__ __forceinline__ void end_sum_dual(const Dual<N> qt[4], double nx, double ny, int type,
                                              uint32_t m, double s2b, const double cf[8],
-                                             double dsw, double dnw, Dual<N> out[4],
-                                             const double *la = nullptr, double *g = nullptr) {
+                                             double dsw, double dnw, Dual<N> out[4]) {
   Dual<N> u1, u2, rho, sb, temp;
   qtilde_state(qt, u1, u2, rho, sb, temp);
   const EndTerms<Dual<N>> e = end_terms_sb(Dual<N>(nx), Dual<N>(ny), u1, u2, rho, sb, temp);
-  end_sum(e, type, m, s2b, cf, dsw, dnw, out, la, g);
+  end_sum(e, type, m, s2b, cf, dsw, dnw, out);
 }
 
-// d/d(nx, ny) of sum_r la_r S_r at one end through the frame (t, n) of its
-// fluxes, the weights held fixed.
-__device__ __forceinline__ void frame_partial(const Recon R[4], double nx, double ny, int type,
-                                              uint32_t m, double s2b, const double cf[8],
-                                              double dsw, double dnw, const double la[4],
-                                              double out[2]) {
-  Dual<2> qt[4], u1, u2, rho, sb, temp, dnx(nx), dny(ny), S[4];
-  for (int r = 0; r < 4; r++) qt[r] = Dual<2>(R[r].qt);
-  dnx.d[0] = 1.0;
-  dny.d[1] = 1.0;
+// ------------------------------------------------------------ leaf physics, reverse
+// Adjoints of the per-end terms (EndTerms fields), accumulated by the
+// reverse flux functions below.
+struct EndBar {
+  double rho = 0.0, ut = 0.0, un = 0.0, prr = 0.0, usq = 0.0, B1 = 0.0, B2 = 0.0, e1 = 0.0,
+         e2 = 0.0;
+};
+
+// split_x (split_fluxes_mod.h:6-84) reversed: gb is the adjoint of G[4].
+// G_r = rho P_r with P0 = ut A1 + s B1, P1 = (p/rho + ut^2) A1 + s ut B1,
+// P2 = ut un A1 + s un B1, P3 = ut (7p/rho + |u|^2) A1 / 2 + s (6p/rho +
+// |u|^2) B1 / 2, A1 = (1 + s erf S1) / 2.
+__device__ __forceinline__ void split_x_adj(const EndTerms<double> &e, double s,
+                                            const double gb[4], EndBar &b) {
+  const double A1 = 0.5 * (1.0 + s * e.e1);
+  const double t1 = e.pr_by_rho + e.ut * e.ut;
+  const double t7 = 7.0 * e.pr_by_rho + e.u_sqr, t6 = 6.0 * e.pr_by_rho + e.u_sqr;
+  const double P0 = e.ut * A1 + s * e.B1;
+  const double P1 = t1 * A1 + s * (e.ut * e.B1);
+  const double P2 = e.ut * e.un * A1 + s * (e.un * e.B1);
+  const double P3 = 0.5 * e.ut * t7 * A1 + s * (0.5 * t6 * e.B1);
+  b.rho += gb[0] * P0 + gb[1] * P1 + gb[2] * P2 + gb[3] * P3;
+  const double R0 = e.rho * gb[0], R1 = e.rho * gb[1], R2 = e.rho * gb[2], R3 = e.rho * gb[3];
+  const double A1b = R0 * e.ut + R1 * t1 + R2 * (e.ut * e.un) + R3 * (0.5 * e.ut * t7);
+  b.ut += R0 * A1 + R1 * (2.0 * e.ut * A1 + s * e.B1) + R2 * (e.un * A1) + R3 * (0.5 * t7 * A1);
+  b.un += R2 * (e.ut * A1 + s * e.B1);
+  b.prr += R1 * A1 + R3 * (3.5 * e.ut * A1 + 3.0 * s * e.B1);
+  b.usq += R3 * (0.5 * e.ut * A1 + 0.5 * s * e.B1);
+  b.B1 += s * (R0 + R1 * e.ut + R2 * e.un + R3 * (0.5 * t6));
+  b.e1 += A1b * (0.5 * s);
+}
+
+// split_y (split_fluxes_mod.h:86-164) reversed: the same with (un, A2, B2)
+// and the momentum rows exchanged.
+__device__ __forceinline__ void split_y_adj(const EndTerms<double> &e, double s,
+                                            const double gb[4], EndBar &b) {
+  const double A2 = 0.5 * (1.0 + s * e.e2);
+  const double t1 = e.pr_by_rho + e.un * e.un;
+  const double t7 = 7.0 * e.pr_by_rho + e.u_sqr, t6 = 6.0 * e.pr_by_rho + e.u_sqr;
+  const double P0 = e.un * A2 + s * e.B2;
+  const double P1 = e.ut * e.un * A2 + s * (e.ut * e.B2);
+  const double P2 = t1 * A2 + s * (e.un * e.B2);
+  const double P3 = 0.5 * e.un * t7 * A2 + s * (0.5 * t6 * e.B2);
+  b.rho += gb[0] * P0 + gb[1] * P1 + gb[2] * P2 + gb[3] * P3;
+  const double R0 = e.rho * gb[0], R1 = e.rho * gb[1], R2 = e.rho * gb[2], R3 = e.rho * gb[3];
+  const double A2b = R0 * e.un + R1 * (e.ut * e.un) + R2 * t1 + R3 * (0.5 * e.un * t7);
+  b.un += R0 * A2 + R1 * (e.ut * A2) + R2 * (2.0 * e.un * A2 + s * e.B2) + R3 * (0.5 * t7 * A2);
+  b.ut += R1 * (e.un * A2 + s * e.B2);
+  b.prr += R2 * A2 + R3 * (3.5 * e.un * A2 + 3.0 * s * e.B2);
+  b.usq += R3 * (0.5 * e.un * A2 + 0.5 * s * e.B2);
+  b.B2 += s * (R0 + R1 * e.ut + R2 * e.un + R3 * (0.5 * t6));
+  b.e2 += A2b * (0.5 * s);
+}
+
+// quad_x (quadrant_fluxes_mod.h:6-212) reversed.
+__device__ __forceinline__ void quad_x_adj(const EndTerms<double> &e, double s1, double s2,
+                                           const double gb[4], EndBar &b) {
+  const double A1 = 0.5 * (1.0 + s1 * e.e1), A2 = 0.5 * (1.0 + s2 * e.e2);
+  const double a1b1 = e.ut * A1 + s1 * e.B1;
+  const double t1 = e.pr_by_rho + e.ut * e.ut;
+  const double t2 = t1 * A1 + s1 * (e.ut * e.B1);
+  const double t2b = e.un * A2 + s2 * e.B2;
+  const double t7 = 7.0 * e.pr_by_rho + e.u_sqr, t6 = 6.0 * e.pr_by_rho + e.u_sqr;
+  const double t2c = 0.5 * e.ut * t7 * A1, t3 = 0.5 * e.B1 * t6;
+  const double M = t2c + s1 * t3;
+  // G0 = rho A2 a1b1, G1 = rho A2 t2, G2 = rho a1b1 t2b,
+  // G3 = rho A2 M + s2 (rho un B2 a1b1) / 2
+  double rhob = gb[0] * A2 * a1b1 + gb[1] * A2 * t2 + gb[2] * a1b1 * t2b + gb[3] * A2 * M;
+  double A2b = e.rho * (gb[0] * a1b1 + gb[1] * t2 + gb[3] * M);
+  double a1b1b = e.rho * (gb[0] * A2 + gb[2] * t2b);
+  const double t2bar = gb[1] * e.rho * A2, t2bb = gb[2] * e.rho * a1b1;
+  const double t2cb = gb[3] * e.rho * A2, t3b = t2cb * s1, t4b = gb[3] * s2;
+  double A1b = t2bar * t1, utb = t2bar * (s1 * e.B1), B1b = t2bar * (s1 * e.ut);
+  const double t1b = t2bar * A1;
+  double prrb = t1b, unb = t2bb * A2, B2b = t2bb * s2, usqb = 0.0;
+  utb += t1b * 2.0 * e.ut;
+  A2b += t2bb * e.un;
+  utb += t2cb * (0.5 * t7 * A1);
+  const double t7b = t2cb * (0.5 * e.ut * A1);
+  A1b += t2cb * (0.5 * e.ut * t7);
+  prrb += 7.0 * t7b;
+  usqb += t7b;
+  B1b += t3b * (0.5 * t6);
+  const double t6b = t3b * (0.5 * e.B1);
+  prrb += 6.0 * t6b;
+  usqb += t6b;
+  rhob += t4b * (0.5 * e.un * e.B2 * a1b1);
+  unb += t4b * (0.5 * e.rho * e.B2 * a1b1);
+  B2b += t4b * (0.5 * e.rho * e.un * a1b1);
+  a1b1b += t4b * (0.5 * e.rho * e.un * e.B2);
+  utb += a1b1b * A1;
+  A1b += a1b1b * e.ut;
+  B1b += a1b1b * s1;
+  b.rho += rhob;
+  b.ut += utb;
+  b.un += unb;
+  b.prr += prrb;
+  b.usq += usqb;
+  b.B1 += B1b;
+  b.B2 += B2b;
+  b.e1 += A1b * (0.5 * s1);
+  b.e2 += A2b * (0.5 * s2);
+}
+
+// One end of a pair run forward and then in reverse.  From its limited
+// state q~, the sum S[r] of the fluxes of end_sum (with the same
+// selection), per flux j the contraction g[j] = sum_r la_r G_j,r, and from
+// the adjoint weights sgn * la_r * w_j of those fluxes (sgn = -1 at the
+// point's own end, +1 at the neighbour's) the adjoint of q~ (qb) and of the
+// normal through the flux frame (nb).
+__device__ __forceinline__ void end_adj(const double qt[4], double nx, double ny, int type,
+                                        uint32_t m, double s2b, const double cf[8], double dels,
+                                        double deln, const double la[4], double sgn,
+                                        double qb[4], double nb[2], double g[4], double S[4]) {
+  double u1, u2, rho, sb, temp;
   qtilde_state(qt, u1, u2, rho, sb, temp);
-  const EndTerms<Dual<2>> e = end_terms_sb(dnx, dny, u1, u2, rho, sb, temp);
-  end_sum(e, type, m, s2b, cf, dsw, dnw, S);
-  out[0] = out[1] = 0.0;
-  for (int r = 0; r < 4; r++) {
-    out[0] += la[r] * S[r].d[0];
-    out[1] += la[r] * S[r].d[1];
-  }
+  const EndTerms<double> e = end_terms_sb(nx, ny, u1, u2, rho, sb, temp);
+  EndBar b;
+  for (int r = 0; r < 4; r++) S[r] = g[r] = 0.0;
+  // kind 0 split_x, 1 quad_x, 2 split_y
+  auto flux = [&](int j, int kind, double s1, double w) {
+    double G[4], gb[4], t = 0.0;
+    if (kind == 0)
+      split_x(e, s1, G);
+    else if (kind == 1)
+      quad_x(e, s1, s2b, G);
+    else
+      split_y(e, s1, G);
+    for (int r = 0; r < 4; r++) {
+      S[r] += G[r] * w;
+      t += la[r] * G[r];
+      gb[r] = sgn * la[r] * w;
+    }
+    g[j] = t;
+    if (kind == 0)
+      split_x_adj(e, s1, gb, b);
+    else if (kind == 1)
+      quad_x_adj(e, s1, s2b, gb, b);
+    else
+      split_y_adj(e, s1, gb, b);
+  };
+  const bool xp = (m & XPOS) != 0;
+  const int kx = type == 1 ? 0 : 1;
+  flux(0, kx, xp ? 1.0 : -1.0, xp ? cf[0] * dels + cf[1] * deln : cf[2] * dels + cf[3] * deln);
+  if (xp && (m & XNEG)) flux(1, kx, -1.0, cf[2] * dels + cf[3] * deln);
+  const bool yp = type == 1 ? (m & YPOS) != 0 : type == 2;
+  flux(2, 2, yp ? 1.0 : -1.0, yp ? cf[4] * deln + cf[5] * dels : cf[6] * deln + cf[7] * dels);
+  if (type == 1 && yp && (m & YNEG)) flux(3, 2, -1.0, cf[6] * deln + cf[7] * dels);
+  // end_terms_sb reversed: S1 = ut sb, S2 = un sb, h = sb (p/rho) / sqrt(pi),
+  // B = exp(-S^2) h, e = erf(S), |u|^2 = ut^2 + un^2
+  const double h = kInvSqrtPi * sb * temp;
+  const double S1 = e.ut * sb, S2 = e.un * sb;
+  const double X1 = e.B1 / h, X2 = e.B2 / h;  // exp(-S1^2), exp(-S2^2)
+  constexpr double kTwoBySqrtPi = 1.1283791670955126;
+  const double hb = b.B1 * X1 + b.B2 * X2;
+  const double S1b = X1 * (-2.0 * S1 * (b.B1 * h) + kTwoBySqrtPi * b.e1);
+  const double S2b = X2 * (-2.0 * S2 * (b.B2 * h) + kTwoBySqrtPi * b.e2);
+  const double utb = b.ut + 2.0 * e.ut * b.usq + S1b * sb;
+  const double unb = b.un + 2.0 * e.un * b.usq + S2b * sb;
+  const double sbb = S1b * e.ut + S2b * e.un + hb * kInvSqrtPi * temp;
+  double tempb = b.prr + hb * kInvSqrtPi * sb;
+  // ut = u1 ny - u2 nx, un = u1 nx + u2 ny
+  double u1b = utb * ny + unb * nx, u2b = -utb * nx + unb * ny;
+  nb[0] = -utb * u2 + unb * u1;
+  nb[1] = utb * u1 + unb * u2;
+  // qtilde_state reversed: beta = -q3/2, p/rho = 1/(2 beta), u = q1..2 p/rho,
+  // rho = exp(q0 + beta |u|^2) beta^-2.5 (sb = sqrt(beta))
+  const double beta = -qt[3] * 0.5;
+  const double T1b = b.rho * rho;
+  double betab = T1b * (-2.5 / beta) + T1b * (u1 * u1 + u2 * u2) + sbb * 0.5 / sb;
+  u1b += T1b * 2.0 * beta * u1;
+  u2b += T1b * 2.0 * beta * u2;
+  qb[0] = T1b;
+  qb[1] = u1b * temp;
+  qb[2] = u2b * temp;
+  tempb += u1b * qt[1] + u2b * qt[2];
+  betab += -tempb * temp / beta;
+  qb[3] = -0.5 * betab;
 }
 
 __device__ __forceinline__ void load8(const double *p, double a[4], double b[4]) {
@@ -201,7 +345,8 @@ __global__ __launch_bounds__(256) void k_argext(int n, const int32_t *__restrict
 // pair, pushed through the reconstruction.  i's own share accumulates in
 // registers (own_q, own_dq, own_qm); the neighbour's goes to the edge's
 // record: c_q[4], c_d[4] (per unit of d, times dx/dy at the gather),
-// c_max[4], c_min[4].
+// c_max[4], c_min[4], stored at the edge's position in the transpose lists
+// so that the gather reads each point's records contiguously.
 //
 // GEO adds the adjoint of the geometry the flux reads, accumulated (+=)
 // into per-edge and per-point arrays owned by i alone:
@@ -219,8 +364,9 @@ __global__ __launch_bounds__(256) void k_flux_vjp(
     const double *__restrict__ nxa, const double *__restrict__ nya,
     const uint8_t *__restrict__ flag, const double *__restrict__ delta,
     const double *__restrict__ lam, double *__restrict__ own_q, double *__restrict__ own_dq,
-    double *__restrict__ own_qm, double *__restrict__ edge, double *__restrict__ geo_edge,
-    double *__restrict__ geo_n, double *__restrict__ geo_cf, double *__restrict__ geo_delta) {
+    double *__restrict__ own_qm, double *__restrict__ edge, const int32_t *__restrict__ tr_pos,
+    double *__restrict__ geo_edge, double *__restrict__ geo_n, double *__restrict__ geo_cf,
+    double *__restrict__ geo_delta) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int type = flag[i];
@@ -261,39 +407,16 @@ __global__ __launch_bounds__(256) void k_flux_vjp(
       Rk[r] = recon(qk[r], delx * dxk[r] + dely * dyk[r], qmxk[r], qmnk[r], hk[B_EPS2]);
     }
     // acc += end_sum(k end) - end_sum(i end): the adjoint of q~ at each end
-    double a_i[4], a_k[4], g_i[4], g_k[4], v_i = 0.0, v_k = 0.0;
+    double a_i[4], a_k[4], g_i[4], g_k[4], S_i[4], S_k[4], n_i[2], n_k[2];
     {
-      Dual<4> qt[4], S[4];
-      for (int r = 0; r < 4; r++) {
-        qt[r] = Dual<4>(Ri[r].qt);
-        qt[r].d[r] = 1.0;
-      }
-      end_sum_dual(qt, nx, ny, type, m, s2b, cf, dels, deln, S, GEO ? la : nullptr, g_i);
-      for (int c = 0; c < 4; c++) {
-        double t = 0.0;
-        for (int r = 0; r < 4; r++) t += la[r] * S[r].d[c];
-        a_i[c] = -t;
-      }
-      if (GEO)
-        for (int r = 0; r < 4; r++) v_i += lc[r] * S[r].v;
-    }
-    {
-      Dual<4> qt[4], S[4];
-      for (int r = 0; r < 4; r++) {
-        qt[r] = Dual<4>(Rk[r].qt);
-        qt[r].d[r] = 1.0;
-      }
-      end_sum_dual(qt, nx, ny, type, m, s2b, cf, dels, deln, S, GEO ? la : nullptr, g_k);
-      for (int c = 0; c < 4; c++) {
-        double t = 0.0;
-        for (int r = 0; r < 4; r++) t += la[r] * S[r].d[c];
-        a_k[c] = t;
-      }
-      if (GEO)
-        for (int r = 0; r < 4; r++) v_k += lc[r] * S[r].v;
+      double qt[4];
+      for (int r = 0; r < 4; r++) qt[r] = Ri[r].qt;
+      end_adj(qt, nx, ny, type, m, s2b, cf, dels, deln, la, -1.0, a_i, n_i, g_i, S_i);
+      for (int r = 0; r < 4; r++) qt[r] = Rk[r].qt;
+      end_adj(qt, nx, ny, type, m, s2b, cf, dels, deln, la, 1.0, a_k, n_k, g_k, S_k);
     }
     if (GEO) {
-      gdel += v_k - v_i;
+      for (int r = 0; r < 4; r++) gdel += lc[r] * (S_k[r] - S_i[r]);
       // weights: x flux a dels + b deln, y flux a deln + b dels, with (a, b)
       // the coefficients of the sub-stencil the flux belongs to
       double g[4], bs = 0.0, bn = 0.0;
@@ -317,11 +440,8 @@ __global__ __launch_bounds__(256) void k_flux_vjp(
       gcf[7] += g[3] * dels;
       bn += g[3] * cf[6];
       bs += g[3] * cf[7];
-      double fi[2], fk[2];
-      frame_partial(Ri, nx, ny, type, m, s2b, cf, dels, deln, la, fi);
-      frame_partial(Rk, nx, ny, type, m, s2b, cf, dels, deln, la, fk);
-      gnx += -bs * dely + bn * delx + (fk[0] - fi[0]);
-      gny += bs * delx + bn * dely + (fk[1] - fi[1]);
+      gnx += -bs * dely + bn * delx + (n_k[0] + n_i[0]);
+      gny += bs * delx + bn * dely + (n_k[1] + n_i[1]);
       double gx = bs * ny + bn * nx, gy = -bs * nx + bn * ny;
       for (int r = 0; r < 4; r++) {
         const double ai = a_i[r] * Ri[r].dd, ak = a_k[r] * Rk[r].dd;
@@ -331,7 +451,7 @@ __global__ __launch_bounds__(256) void k_flux_vjp(
       geo_edge[2 * (size_t)e] += gx;
       geo_edge[2 * (size_t)e + 1] += gy;
     }
-    double *rec = edge + 16 * (size_t)e;
+    double *rec = edge + 16 * (size_t)tr_pos[e];
     for (int r = 0; r < 4; r++) {
       aq[r] += a_i[r] * Ri[r].dq;
       const double ad = a_i[r] * Ri[r].dd;
@@ -363,8 +483,7 @@ __global__ __launch_bounds__(256) void k_flux_vjp(
 // Per point j: its own share plus the edge records of every pair in which
 // it is the neighbour (transpose CSR, owners ascending).
 __global__ __launch_bounds__(256) void k_flux_vjp_gather(
-    int n, const int32_t *__restrict__ tr_off, const int32_t *__restrict__ tr_edge,
-    const int32_t *__restrict__ tr_owner, const double *__restrict__ base,
+    int n, const int32_t *__restrict__ tr_off, const int32_t *__restrict__ tr_owner, const double *__restrict__ base,
     const double *__restrict__ edge, const double *__restrict__ own_q,
     const double *__restrict__ own_dq, const double *__restrict__ own_qm,
     double *__restrict__ lq, double *__restrict__ ldq, double *__restrict__ lqm) {
@@ -377,7 +496,7 @@ __global__ __launch_bounds__(256) void k_flux_vjp_gather(
   const double x_j = base[(size_t)B_STRIDE * j + B_X], y_j = base[(size_t)B_STRIDE * j + B_Y];
   for (int t = tr_off[j]; t < tr_off[j + 1]; t++) {
     const int i = tr_owner[t];
-    const double *rec = edge + 16 * (size_t)tr_edge[t];
+    const double *rec = edge + 16 * (size_t)t;
     const double delx = x_j - base[(size_t)B_STRIDE * i + B_X];
     const double dely = y_j - base[(size_t)B_STRIDE * i + B_Y];
     for (int r = 0; r < 4; r++) {
@@ -670,14 +789,16 @@ void launch_flux_vjp(const DevCloud &c, const AdjointBuffers &a, const double *b
   if (geo)
     hipLaunchKernelGGL(k_flux_vjp<true>, dim3(blocks_of(c.n)), dim3(256), 0, s, c.n, c.conn_off,
                        c.conn, base, dq, qm, c.coef, c.nx, c.ny, c.flag, delta, lam, a.own_q,
-                       a.own_dq, a.own_qm, a.edge, geo->edge, geo->nrm, geo->coef, geo->delta);
+                       a.own_dq, a.own_qm, a.edge, a.tr_pos, geo->edge, geo->nrm, geo->coef,
+                       geo->delta);
   else
     hipLaunchKernelGGL(k_flux_vjp<false>, dim3(blocks_of(c.n)), dim3(256), 0, s, c.n,
                        c.conn_off, c.conn, base, dq, qm, c.coef, c.nx, c.ny, c.flag, delta, lam,
-                       a.own_q, a.own_dq, a.own_qm, a.edge, nullptr, nullptr, nullptr, nullptr);
+                       a.own_q, a.own_dq, a.own_qm, a.edge, a.tr_pos, nullptr, nullptr, nullptr,
+                       nullptr);
   hipLaunchKernelGGL(k_flux_vjp_gather, dim3(blocks_of(c.n)), dim3(256), 0, s, c.n, a.tr_off,
-                     a.tr_edge, a.tr_owner, base, a.edge, a.own_q, a.own_dq, a.own_qm, a.lq,
-                     a.ldq[0], a.lqm);
+                     a.tr_owner, base, a.edge, a.own_q, a.own_dq, a.own_qm, a.lq, a.ldq[0],
+                     a.lqm);
 }
 
 void launch_ls_vjp(const DevCloud &c, const AdjointBuffers &a, const double *base,

@@ -58,7 +58,8 @@ void release(lskum_ctx *c) {
     if (p) (void)hipFree(p);
   void *aptrs[] = {c->adj.own_q, c->adj.own_dq, c->adj.own_qm, c->adj.edge, c->adj.lq,
                    c->adj.ldq[0], c->adj.ldq[1], c->adj.lqm, c->adj.ls, c->adj.arg,
-                   c->adj.tr_off, c->adj.tr_edge, c->adj.tr_owner, c->adj.tbase, c->adj.tdq[0],
+                   c->adj.tr_off, c->adj.tr_edge, c->adj.tr_owner, c->adj.tr_pos, c->adj.tbase,
+                   c->adj.tdq[0],
                    c->adj.tdq[1], c->adj.tqm, c->adj.vec_in, c->adj.vec_out};
   for (void *p : aptrs)
     if (p) (void)hipFree(p);
@@ -749,7 +750,7 @@ int prepare_adjoint(lskum_ctx *c) {
   const CloudLayout &L = c->clouds[0];
   const int32_t n = c->n;
   const size_t E = (size_t)L.off[n];
-  std::vector<int32_t> tr_off(n + 1, 0), tr_edge(E), tr_owner(E);
+  std::vector<int32_t> tr_off(n + 1, 0), tr_edge(E), tr_owner(E), tr_pos(E);
   for (size_t e = 0; e < E; e++) tr_off[(L.conn[e] & kIndexMask) + 1]++;
   for (int32_t j = 0; j < n; j++) tr_off[j + 1] += tr_off[j];
   std::vector<int32_t> pos(tr_off.begin(), tr_off.end() - 1);
@@ -757,6 +758,7 @@ int prepare_adjoint(lskum_ctx *c) {
     for (int32_t e = L.off[i]; e < L.off[i + 1]; e++) {
       const int32_t j = (int32_t)(L.conn[e] & kIndexMask);
       tr_edge[pos[j]] = e;
+      tr_pos[e] = pos[j];
       tr_owner[pos[j]++] = i;
     }
   AdjointBuffers &a = c->adj;
@@ -777,6 +779,7 @@ int prepare_adjoint(lskum_ctx *c) {
   AALLOC(a.tr_off, (size_t)n + 1);
   AALLOC(a.tr_edge, E);
   AALLOC(a.tr_owner, E);
+  AALLOC(a.tr_pos, E);
   AALLOC(a.tbase, (size_t)B_STRIDE * n);
   AALLOC(a.tdq[0], (size_t)DQ_STRIDE * n);
   AALLOC(a.tdq[1], (size_t)DQ_STRIDE * n);
@@ -788,6 +791,7 @@ int prepare_adjoint(lskum_ctx *c) {
                        hipMemcpyHostToDevice));
   HIP_TRY(c, hipMemcpy(a.tr_edge, tr_edge.data(), E * sizeof(int32_t), hipMemcpyHostToDevice));
   HIP_TRY(c, hipMemcpy(a.tr_owner, tr_owner.data(), E * sizeof(int32_t), hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(a.tr_pos, tr_pos.data(), E * sizeof(int32_t), hipMemcpyHostToDevice));
   c->adj_ready = true;
   return LSKUM_OK;
 }

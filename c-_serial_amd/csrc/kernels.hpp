@@ -138,12 +138,13 @@ void launch_kat_fluxes_m(int n, const double *in6, double *out, hipStream_t s);
 // ldq [n][8] (two, for the inner passes), lqm [n][8], ls [n][8] are
 // adjoints of q, dq, qmax/qmin and of the least-squares sums; arg [n][8]
 // the point each qmax/qmin came from; tr_* the transpose of conn (for each
-// point, the edges that name it, owners ascending); tbase, tdq, tqm the
+// point, the edges that name it, owners ascending; tr_pos[e] the position of
+// edge e in those lists, where its record in `edge` is kept); tbase, tdq, tqm the
 // tangents (tbase a copy of the base records with tangent q); vec_in /
 // vec_out [4][n] staging.
 struct AdjointBuffers {
   double *own_q, *own_dq, *own_qm, *edge, *lq, *ldq[2], *lqm, *ls;
-  int32_t *arg, *tr_off, *tr_edge, *tr_owner;
+  int32_t *arg, *tr_off, *tr_edge, *tr_owner, *tr_pos;
   double *tbase, *tdq[2], *tqm, *vec_in, *vec_out;
 };
 // Geometry adjoints the reverse kernels accumulate (+=), device order:
