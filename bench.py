@@ -206,8 +206,8 @@ def run_shape(args):
     (Cl, Cd), each giving all 20 derivatives.  value = gradients
     (objectives x design variables) per second of adjoint time.  Checked on
     sample variables: the tangent-linear run and central differences
-    (h = 1e-6) of the plain iterations, with the one-sided differences that
-    show whether a limiter switch lies within +-h."""
+    (h = 1e-6) of the plain iterations; the spread of the one-sided
+    differences is reported beside them (h J'' even where J is smooth)."""
     from lskum import shape
     t = time.perf_counter()
     cloud = lskum.workload("naca0012_1m_k24")
@@ -249,7 +249,7 @@ def run_shape(args):
             "adjoint": g, "tangent": tan, "fd_central": fd, "fd_forward": fwd,
             "fd_backward": bwd, "rel_err_tangent": abs(tan - g) / abs(g),
             "rel_err_fd": abs(fd - g) / abs(g),
-            "one_sided_agree": abs(fwd - bwd) <= 1e-5 * abs(fd), "tangent_s": tan_s}
+            "one_sided_spread_rel": abs(fwd - bwd) / abs(fd), "tangent_s": tan_s}
         log(f"shape: check {name} var {j}: {checks[f'{name}_var{j}']}")
     total = sum(sweep_s.values())
     n_grads = len(objectives) * fields.shape[0]
