@@ -11,7 +11,7 @@ same start and states its Cl/Cd/residual differences from the FP64 run.
 Prints one JSON line.
 
   python bench.py [--gpus 1] [--steps 50] [--warmup 5] [--no-cpu-baseline]
-                  [--no-mixed] [--workload batch|shape]
+                  [--no-mixed] [--workload batch|shape|mixed10m]
 """
 import argparse
 import json
@@ -193,6 +193,58 @@ def run_batch(args):
     }
 
 
+def run_mixed10m(args):
+    """Config 3: the 10,004,000-point NACA0012 cloud (wall 4,000, 1,000 x
+    10,000 O-grid, k = 24) at M 0.63, AoA 2 deg, CFL 0.2, K 5, no inner
+    passes, `--steps` iterations from free stream in FP64 and then in the
+    mixed form from the same start.  value = the mixed form's point-
+    iterations/s; the line states its speed-up and its Cl, Cd, Cm and
+    residual-history differences from the FP64 run (the oracle of this
+    config)."""
+    t = time.perf_counter()
+    cloud = lskum.workload("naca0012_10m_k24")
+    log(f"mixed10m: cloud {cloud.n} points, {time.perf_counter() - t:.1f}s")
+    runs = {}
+    for name, precision in (("fp64", L.FP64), ("mixed", L.MIXED)):
+        s = Solver(cloud, mach=0.63, aoa=2.0, cfl=0.2, vl_const=5.0, power=0.0,
+                   inner_iterations=0, precision=precision)
+        s.set_graph(50)
+        rn_all, elapsed, done = [], 0.0, 0
+        while done < args.steps:
+            step = min(250, args.steps - done)
+            t0 = time.perf_counter()
+            rn, _ = s.q_lskum(step)
+            elapsed += time.perf_counter() - t0
+            done += step
+            rn_all.append(rn)
+            log(f"mixed10m {name}: {done}/{args.steps} res_new {rn[-1]:.4e} "
+                f"{1e3 * elapsed / done:.2f} ms/iteration")
+        runs[name] = {"res_new": np.concatenate(rn_all), "elapsed": elapsed,
+                      "forces": s.compute_cl_cd_cm()}
+        s.close()
+    f, m = runs["fp64"], runs["mixed"]
+    n = cloud.n
+    return {
+        "metric": METRIC, "value": n * args.steps / m["elapsed"], "unit": "point-iterations/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": 0,
+        "ms_per_step": 1e3 * m["elapsed"] / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 gathers/fluxes/limiter, f64 solves/state/reductions", "data": "synthetic",
+        "config": {"workload": "naca0012_10m_k24", "points": n, "k": 24, "inner_iterations": 0,
+                   "mach": 0.63, "aoa_deg": 2.0, "cfl": 0.2, "vl_const": 5.0,
+                   "graph_iters": 50},
+        "fp64": {"value": n * args.steps / f["elapsed"],
+                 "ms_per_step": 1e3 * f["elapsed"] / args.steps,
+                 "cl_cd_cm": list(f["forces"]), "final_res_new": float(f["res_new"][-1])},
+        "speedup_vs_fp64": f["elapsed"] / m["elapsed"],
+        "error_vs_fp64": {"abs_dCl": abs(m["forces"][0] - f["forces"][0]),
+                          "abs_dCd": abs(m["forces"][1] - f["forces"][1]),
+                          "abs_dCm": abs(m["forces"][2] - f["forces"][2]),
+                          "res_new_normwise": float(np.abs(m["res_new"] - f["res_new"]).max()
+                                                    / np.abs(f["res_new"]).max())},
+    }
+
+
 def run_shape(args):
     """Shape gradients (the adjoint workload): the 1,000,940-point NACA0012
     cloud (wall 1,500, 806 x 1,240 O-grid, k = 24) at M 0.8, AoA 1.25 deg,
@@ -305,6 +357,9 @@ def main():
         return
     if args.workload == "shape":
         print(json.dumps(run_shape(args)), flush=True)
+        return
+    if args.workload == "mixed10m":
+        print(json.dumps(run_mixed10m(args)), flush=True)
         return
     t = time.perf_counter()
     cloud = lskum.workload(args.workload)
