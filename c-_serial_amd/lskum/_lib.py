@@ -117,6 +117,10 @@ _SIGS = [
     ("lskum_flux_vjp", C.c_int, [C.c_void_p, _dp, _dp]),
     ("lskum_flux_jvp", C.c_int, [C.c_void_p, _dp, _dp]),
     ("lskum_forces_grad", C.c_int, [C.c_void_p, _dp, _dp, _dp]),
+    ("lskum_compute_entropy", C.c_int, [C.c_void_p, _dp, _dp]),
+    ("lskum_compute_enstrophy", C.c_int, [C.c_void_p, _dp, _dp, _dp]),
+    ("lskum_stagnation_pressure", C.c_int, [C.c_void_p, _dp, _ip]),
+    ("lskum_objective_J", C.c_int, [C.c_void_p, _dp]),
     ("lskum_set_coordinates", C.c_int, [C.c_void_p, _dp, _dp]),
     ("lskum_shape_vjp", C.c_int, [C.c_void_p, C.c_int32, _dp, _dp, _dp, _dp, _dp]),
     ("lskum_shape_jvp", C.c_int, [C.c_void_p, C.c_int32, _dp, _dp, _dp, _dp, _dp, _dp]),

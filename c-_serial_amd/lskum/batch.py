@@ -88,6 +88,34 @@ class Batch:
     def prim(self):
         return self.get(L.A_PRIM, 4)
 
+    def diagnostics(self, which):
+        """Per case: "entropy" -> ([entropy per point], totals), "enstrophy" ->
+        ([vorticity per point], totals), "stagnation" -> ([n_cases, 4] values,
+        [n_cases, 2] points), "J" -> [n_cases] stagnation-pressure losses
+        (compute_entropy_mod.h, compute_enstrophy_mod.h, stagnation_values_mod.h)."""
+        nc, tot = len(self.cases), int(self.points.sum())
+
+        def split(a):
+            return np.split(a, np.cumsum(self.points)[:-1])
+
+        if which == "entropy":
+            e, t = np.zeros(tot), np.zeros(nc)
+            self._chk(L.lib().lskum_compute_entropy(self._ctx, L.dptr(e), L.dptr(t)))
+            return split(e), t
+        if which == "enstrophy":
+            v, t = np.zeros(tot), np.zeros(nc)
+            self._chk(L.lib().lskum_compute_enstrophy(self._ctx, L.dptr(v), None, L.dptr(t)))
+            return split(v), t
+        if which == "stagnation":
+            vals, idx = np.zeros((nc, 4)), np.zeros((nc, 2), np.int32)
+            self._chk(L.lib().lskum_stagnation_pressure(self._ctx, L.dptr(vals), L.iptr(idx)))
+            return vals, idx
+        if which == "J":
+            J = np.zeros(nc)
+            self._chk(L.lib().lskum_objective_J(self._ctx, L.dptr(J)))
+            return J
+        raise ValueError(which)
+
     def set_timing(self, enable):
         self._chk(L.lib().lskum_set_timing(self._ctx, 1 if enable else 0))
 

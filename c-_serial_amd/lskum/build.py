@@ -19,7 +19,7 @@ LIB_PATH = os.path.join(PKG_DIR, "liblskum.so")
 BUILD_DIR = os.path.join(ROOT, "_build")
 
 HIP_SOURCES = ["kernels.hip", "kernels_mixed.hip", "adjoint.hip", "trajectory.hip", "api.hip",
-               "shape_api.hip"]
+               "shape_api.hip", "diagnostics.hip"]
 CXX_SOURCES = ["setup.cpp", "cloud.cpp", "restart.cpp", "shape.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("LSKUM_ARCH", "gfx950")

@@ -193,6 +193,33 @@ class Solver:
         self._chk(L.lib().lskum_forces_grad(self._ctx, *[L.dptr(a) for a in g]))
         return tuple(g)
 
+    # -------------------------------------------------- diagnostics
+    def compute_entropy(self):
+        """(entropy [n], total) -- compute_entropy_mod.h:5-24"""
+        e, t = np.empty(self.n), C.c_double()
+        self._chk(L.lib().lskum_compute_entropy(self._ctx, L.dptr(e), C.byref(t)))
+        return e, t.value
+
+    def compute_enstrophy(self):
+        """(vorticity [n], vorticity_sqr [n], total) -- compute_enstrophy_mod.h:5-76"""
+        v, v2, t = np.empty(self.n), np.empty(self.n), C.c_double()
+        self._chk(L.lib().lskum_compute_enstrophy(self._ctx, L.dptr(v), L.dptr(v2),
+                                                  C.byref(t)))
+        return v, v2, t.value
+
+    def stagnation_pressure(self):
+        """((pMin, pMax, pMin/p0_inf, pMax/p0_inf), (indexMin, indexMax)) --
+        stagnation_values_mod.h:8-46, indices 0-based"""
+        vals, idx = np.empty(4), np.empty(2, np.int32)
+        self._chk(L.lib().lskum_stagnation_pressure(self._ctx, L.dptr(vals), L.iptr(idx)))
+        return vals, idx
+
+    def objective_J(self):
+        """stagnation-pressure loss -- stagnation_values_mod.h:49-80"""
+        J = C.c_double()
+        self._chk(L.lib().lskum_objective_J(self._ctx, C.byref(J)))
+        return J.value
+
     # -------------------------------------------------- shape sensitivities
     def set_coordinates(self, x, y):
         """Moves the points to (x, y), keeping the neighbour lists, their

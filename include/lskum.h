@@ -313,6 +313,34 @@ int lskum_flux_jvp(lskum_ctx *ctx, const double *v, double *out);
 int lskum_forces_grad(lskum_ctx *ctx, double *dcl, double *dcd, double *dcm);
 
 /*
+ * Diagnostics of the current state (per case for a batch: per-point arrays
+ * case after case in each case's caller order, totals one per case; any
+ * output may be NULL):
+ *   lskum_compute_entropy     compute_entropy_mod.h:5-24: entropy[k] =
+ *       |ln(p/rho^gamma) - ln p_inf|, total_entropy = their sum (the
+ *       reference forms the sum and discards it);
+ *   lskum_compute_enstrophy   compute_enstrophy_mod.h:5-76: vorticity =
+ *       du2/dx - du1/dy by the (power-weighted) least squares over each
+ *       point's neighbours, vorticity_sqr, total_enstrophy = its sum;
+ *   lskum_stagnation_pressure stagnation_values_mod.h:8-46: values[4] =
+ *       pMin, pMax, pMin/p0_inf, pMax/p0_inf of p0 = p (1 + (gamma-1)/2
+ *       M^2)^(gamma/(gamma-1)), index[2] = their points (0-based, the first
+ *       in point order, as the reference keeps them; it prints them with a
+ *       mismatched %f);
+ *   lskum_objective_J         stagnation_values_mod.h:49-80: the loss
+ *       sum (p0_inf - p0)^2 / (p0_inf^2 n).  The reference multiplies an
+ *       uninitialised total_p0 instead of the sum it forms; this returns
+ *       the sum's value.
+ * Sums are reduced in a fixed order, so results are deterministic; they
+ * differ from the reference's serial sums at the rounding level.
+ */
+int lskum_compute_entropy(lskum_ctx *ctx, double *entropy, double *total_entropy);
+int lskum_compute_enstrophy(lskum_ctx *ctx, double *vorticity, double *vorticity_sqr,
+                            double *total_enstrophy);
+int lskum_stagnation_pressure(lskum_ctx *ctx, double *values, int32_t *index);
+int lskum_objective_J(lskum_ctx *ctx, double *J);
+
+/*
  * Shape sensitivities (no reference counterpart).
  *
  * lskum_set_coordinates moves the points of a single-case context to

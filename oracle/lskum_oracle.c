@@ -755,6 +755,119 @@ void orc_q_lskum(orc_ctx *c, int n_iter, double *res_new, double *residue) {
   }
 }
 
+/* ---------------------------------------------------------------- diagnostics */
+
+/* compute_entropy_mod.h:5-24: per point |log(p / rho^gamma) - log(pr_inf)|;
+ * returns the sum (the reference's total_entropy) */
+double orc_compute_entropy(const orc_ctx *c, double *entropy) {
+  const int n = c->n;
+  double total_entropy = 0.;
+  const double temp2 = log(c->pr_inf);
+  for (int k = 0; k < n; k++) {
+    double temp1 = pow(PRIM(c, 0, k), c->gamma_new);
+    temp1 = PRIM(c, 3, k) / temp1;
+    temp1 = log(temp1);
+    if (entropy) entropy[k] = fabs(temp1 - temp2);
+    total_entropy = total_entropy + fabs(temp1 - temp2);
+  }
+  return total_entropy;
+}
+
+/* compute_enstrophy_mod.h:5-76: vorticity du2/dx - du1/dy from the weighted
+ * least squares over the point's neighbours, its square; returns the sum */
+double orc_compute_enstrophy(const orc_ctx *c, double *vorticity, double *vorticity_sqr) {
+  const int n = c->n;
+  double total_enstrophy = 0;
+  for (int i = 0; i < n; i++) {
+    const double x_i = c->x[i], y_i = c->y[i];
+    double sum_delx_sqr = 0., sum_dely_sqr = 0., sum_delx_dely = 0.;
+    double sum_delx_delu1 = 0., sum_dely_delu1 = 0., sum_delx_delu2 = 0., sum_dely_delu2 = 0.;
+    for (int e = c->conn_off[i]; e < c->conn_off[i + 1]; e++) {
+      const int nbh = c->conn[e];
+      const double delx = c->x[nbh] - x_i, dely = c->y[nbh] - y_i;
+      const double dist = sqrt(delx * delx + dely * dely);
+      const double weights = pow(dist, c->power);
+      sum_delx_sqr = sum_delx_sqr + delx * delx * weights;
+      sum_dely_sqr = sum_dely_sqr + dely * dely * weights;
+      sum_delx_dely = sum_delx_dely + delx * dely * weights;
+      sum_delx_delu1 = sum_delx_delu1 + weights * delx * (PRIM(c, 1, nbh) - PRIM(c, 1, i));
+      sum_delx_delu2 = sum_delx_delu2 + weights * delx * (PRIM(c, 2, nbh) - PRIM(c, 2, i));
+      sum_dely_delu1 = sum_dely_delu1 + weights * dely * (PRIM(c, 1, nbh) - PRIM(c, 1, i));
+      sum_dely_delu2 = sum_dely_delu2 + weights * dely * (PRIM(c, 2, nbh) - PRIM(c, 2, i));
+    }
+    const double det = sum_delx_sqr * sum_dely_sqr - sum_delx_dely * sum_delx_dely;
+    const double one_by_det = 1.0 / det;
+    const double du2_dx =
+        (sum_delx_delu2 * sum_dely_sqr - sum_dely_delu2 * sum_delx_dely) * one_by_det;
+    const double du1_dy =
+        (sum_dely_delu1 * sum_delx_sqr - sum_delx_delu1 * sum_delx_dely) * one_by_det;
+    const double temp = du2_dx - du1_dy;
+    if (vorticity) vorticity[i] = temp;
+    if (vorticity_sqr) vorticity_sqr[i] = temp * temp;
+    total_enstrophy = total_enstrophy + temp * temp;
+  }
+  return total_enstrophy;
+}
+
+/* stagnation pressure of one state, stagnation_values_mod.h:29-31 */
+static double stag_p0(const orc_ctx *c, int i, double gammaPower) {
+  const double g = c->gamma_new;
+  const double angle = sqrt(g * PRIM(c, 3, i) / PRIM(c, 0, i));
+  const double mach_t = sqrt(pow(PRIM(c, 1, i), 2) + pow(PRIM(c, 2, i), 2)) / angle;
+  return PRIM(c, 3, i) * (pow((1 + ((g - 1) / 2) * mach_t * mach_t), gammaPower));
+}
+
+static double stag_p0_inf(const orc_ctx *c, double gammaPower) {
+  const double g = c->gamma_new;
+  return c->pr_inf * (pow((1 + ((g - 1) / 2) * c->mach * c->mach), gammaPower));
+}
+
+/* stagnation_values_mod.h:8-46: out = pMin, pMax, pMin/p0_inf, pMax/p0_inf;
+ * idx = the (0-based) points of the first minimum and maximum */
+void orc_stagnation_pressure(const orc_ctx *c, double out[4], int idx[2]) {
+  const double gammaPower = c->gamma_new / (c->gamma_new - 1);
+  const double p0_inf = stag_p0_inf(c, gammaPower);
+  double pMin = 0, pMax = 0;
+  int indexMin = 0, indexMax = 0;
+  for (int i = 0; i < c->n; i++) {
+    const double p0 = stag_p0(c, i, gammaPower);
+    if (i == 0) {
+      pMin = p0;
+      indexMin = i;
+      indexMax = i;
+      pMax = p0;
+    } else if (p0 < pMin) {
+      pMin = p0;
+      indexMin = i;
+    } else if (p0 > pMax) {
+      pMax = p0;
+      indexMax = i;
+    }
+  }
+  out[0] = pMin;
+  out[1] = pMax;
+  out[2] = pMin / p0_inf;
+  out[3] = pMax / p0_inf;
+  idx[0] = indexMin;
+  idx[1] = indexMax;
+}
+
+/* stagnation_values_mod.h:49-80, the loss sum (p0_inf - p0)^2 / (p0_inf^2
+ * plen) with plen = max_points (meshfree_solver_mod.h:58).  The reference
+ * forms p0_sum and then multiplies an uninitialised total_p0 by the
+ * constant; this returns p0_sum times the constant. */
+double orc_objective_J(const orc_ctx *c) {
+  const double gammaPower = c->gamma_new / (c->gamma_new - 1);
+  const double p0_inf = stag_p0_inf(c, gammaPower);
+  const double constant = 1 / (pow(p0_inf, 2) * c->n);
+  double p0_sum = 0.0;
+  for (int i = 0; i < c->n; i++) {
+    const double p0 = stag_p0(c, i, gammaPower);
+    p0_sum = p0_sum + pow((p0_inf - p0), 2);
+  }
+  return p0_sum * constant;
+}
+
 double *orc_array(orc_ctx *c, int which, int *len) {
   int n = c->n;
   switch (which) {

@@ -78,6 +78,13 @@ void orc_flux(int which, double nx, double ny, double u1, double u2,
 void orc_venkat_limiter(const orc_ctx *c, const double qtilde[4],
                         double phi[4], int k);
 
+/* diagnostics (compute_entropy_mod.h, compute_enstrophy_mod.h,
+ * stagnation_values_mod.h); per-point outputs may be NULL */
+double orc_compute_entropy(const orc_ctx *c, double *entropy);
+double orc_compute_enstrophy(const orc_ctx *c, double *vorticity, double *vorticity_sqr);
+void orc_stagnation_pressure(const orc_ctx *c, double out[4], int idx[2]);
+double orc_objective_J(const orc_ctx *c);
+
 void orc_scalars(const orc_ctx *c, double out[11]);
 void orc_set_iteration(orc_ctx *c, int it, double res_old, int restart);
 

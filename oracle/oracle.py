@@ -49,6 +49,13 @@ def lib():
         h.orc_sub_list.argtypes = [C.c_void_p, C.c_int, C.c_int]
         h.orc_scalars.argtypes = [C.c_void_p, _dp]
         h.orc_set_iteration.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int]
+        h.orc_compute_entropy.argtypes = [C.c_void_p, _dp]
+        h.orc_compute_entropy.restype = C.c_double
+        h.orc_compute_enstrophy.argtypes = [C.c_void_p, _dp, _dp]
+        h.orc_compute_enstrophy.restype = C.c_double
+        h.orc_stagnation_pressure.argtypes = [C.c_void_p, _dp, _ip]
+        h.orc_objective_J.argtypes = [C.c_void_p]
+        h.orc_objective_J.restype = C.c_double
         _lib = h
     return _lib
 
@@ -103,6 +110,32 @@ class Oracle:
 
     def snapshot(self):
         self.set("prim_old", self.get("prim"))
+
+    # diagnostics of the current state
+    def compute_entropy(self):
+        """(entropy [n], total) -- compute_entropy_mod.h:5-24"""
+        e = np.zeros(self.n)
+        t = lib().orc_compute_entropy(self.c, e.ctypes.data_as(_dp))
+        return e, t
+
+    def compute_enstrophy(self):
+        """(vorticity [n], vorticity_sqr [n], total) -- compute_enstrophy_mod.h:5-76"""
+        v, v2 = np.zeros(self.n), np.zeros(self.n)
+        t = lib().orc_compute_enstrophy(self.c, v.ctypes.data_as(_dp), v2.ctypes.data_as(_dp))
+        return v, v2, t
+
+    def stagnation_pressure(self):
+        """(pMin, pMax, pMin/p0_inf, pMax/p0_inf), (indexMin, indexMax) 0-based --
+        stagnation_values_mod.h:8-46"""
+        out = np.zeros(4)
+        idx = np.zeros(2, np.int32)
+        lib().orc_stagnation_pressure(self.c, out.ctypes.data_as(_dp), idx.ctypes.data_as(_ip))
+        return out, idx
+
+    def objective_J(self):
+        """stagnation-pressure loss, stagnation_values_mod.h:49-80 (the sum the
+        reference forms, in place of its uninitialised total_p0)"""
+        return lib().orc_objective_J(self.c)
 
     def q_lskum(self, n_iter):
         rn = np.zeros(n_iter)

@@ -19,6 +19,10 @@
 // Usage:
 //   ref_driver run GRID OUTDIR --iters N [--cfl C] [--vl K] [--inner I]
 //              [--mach M --aoa A] [--stages it1,it2,...] [--dump it1,...]
+//              [--diag 1]
+// --diag runs the reference's diagnostics on the final state: entropy and
+// enstrophy are dumped; stagnation_pressure() prints its own line (its last
+// two fields are ints printed with %f, so only the first four mean anything).
 //   ref_driver kat DATAFILE NROWS OUTFILE
 #include "split_fluxes_mod.h"
 #include "quadrant_fluxes_mod.h"
@@ -29,6 +33,9 @@
 #include "initial_conditions_mod.h"
 #include "q_lskum_mod.h"
 #include "compute_force_coeffs_mod.h"
+#include "compute_entropy_mod.h"
+#include "compute_enstrophy_mod.h"
+#include "stagnation_values_mod.h"
 
 #include <chrono>
 #include <cstdio>
@@ -143,7 +150,7 @@ static int cmd_run(int argc, char **argv) {
   if (argc < 4) return 2;
   const char *grid = argv[2];
   g_out = argv[3];
-  int iters = 1;
+  int iters = 1, diag = 0;
   double mach_o = -1, aoa_o = 0;
   std::set<int> stages, dumps;
   for (int a = 4; a < argc; a++) {
@@ -158,6 +165,7 @@ static int cmd_run(int argc, char **argv) {
     else if (k == "--aoa") aoa_o = std::atof(v);
     else if (k == "--stages") stages = parse_list(v);
     else if (k == "--dump") dumps = parse_list(v);
+    else if (k == "--diag") diag = std::atoi(v);
     else return 2;
   }
   int width = read_grid(grid);
@@ -267,6 +275,20 @@ static int cmd_run(int argc, char **argv) {
   std::cout.rdbuf(saved);
   dump_raw("history", hist);
   dump_raw("loop_seconds", {secs});
+  if (diag) {
+    compute_entropy();
+    compute_enstrophy();
+    dump_raw("diag_entropy", std::vector<double>(point.entropy.begin() + 1, point.entropy.end()));
+    dump_raw("diag_vorticity",
+             std::vector<double>(point.vorticity.begin() + 1, point.vorticity.end()));
+    dump_raw("diag_vorticity_sqr",
+             std::vector<double>(point.vorticity_sqr.begin() + 1, point.vorticity_sqr.end()));
+    dump_raw("diag_total_enstrophy", {total_enstrophy});
+    std::fflush(stdout);
+    stagnation_pressure();
+    std::printf("\n");
+    std::fflush(stdout);
+  }
   return 0;
 }
 

@@ -15,6 +15,10 @@ come from the reference's own functions run by oracle/ref_driver.cpp.
                      20 iterations: history, final prim and flux_res
   c1_1000.npz        naca0012_9760_k20, config 1, 1000 iterations: history,
                      final prim and SHA-256 digests of every final array
+  diag_small.npz     naca0012_1480_k20, config-1 numerics, 40 iterations:
+                     final prim and the reference's diagnostics of it
+                     (entropy, vorticity, its square and total enstrophy;
+                     the first four values stagnation_pressure() prints)
 """
 import argparse
 import hashlib
@@ -44,13 +48,15 @@ def grid_sha(cloud, path):
         return hashlib.sha256(f.read()).hexdigest()
 
 
-def run_ref(cloud, iters, cfl, vl, inner, extra=()):
+def run_ref(cloud, iters, cfl, vl, inner, extra=(), stdout=None):
     d = tempfile.mkdtemp(prefix="lskum_ref_")
     g = os.path.join(d, "grid.txt")
     gsha = grid_sha(cloud, g)
     cmd = [DRIVER, "run", g, d, "--iters", str(iters), "--cfl", repr(cfl), "--vl", repr(vl),
            "--inner", str(inner), "--dump", str(iters), *extra]
-    subprocess.run(cmd, check=True, cwd=d)
+    proc = subprocess.run(cmd, check=True, cwd=d, capture_output=True, text=True)
+    if stdout is not None:
+        stdout.append(proc.stdout)
     load = lambda name: np.fromfile(os.path.join(d, name + ".bin"))
     return d, gsha, load
 
@@ -119,6 +125,21 @@ def c1():
         sub_members_sha=sha(load("substencil_members").astype(np.int32)))
 
 
+def diag():
+    cloud = lskum.workload("naca0012_1480_k20")
+    n = cloud.n
+    out = []
+    d, gsha, load = run_ref(cloud, 40, 0.2, 5.0, 0, ("--diag", "1"), stdout=out)
+    line = [ln for ln in out[0].splitlines() if ln.startswith("Stagnation values are")][0]
+    stag = np.array([float(v) for v in line.split()[3:7]])
+    np.savez_compressed(
+        os.path.join(HERE, "diag_small.npz"), grid_sha=gsha,
+        params=np.array([0.63, 2.0, 0.2, 5.0, 0.0, 0]),
+        prim=load("it40_prim").reshape(4, n), entropy=load("diag_entropy"),
+        vorticity=load("diag_vorticity"), vorticity_sqr=load("diag_vorticity_sqr"),
+        total_enstrophy=load("diag_total_enstrophy")[0], stagnation_printed=stag)
+
+
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--c1", action="store_true", help="also the 1000-iteration config-1 run")
@@ -126,8 +147,8 @@ if __name__ == "__main__":
     a = ap.parse_args()
     if not os.path.isfile(DRIVER):
         sys.exit("build the reference harness first: make -C oracle ref")
-    jobs = {"kat": kat, "small": small_c1, "k24": k24, "c1": c1}
-    todo = a.only.split(",") if a.only else ["kat", "small", "k24"] + (["c1"] if a.c1 else [])
+    jobs = {"kat": kat, "small": small_c1, "k24": k24, "c1": c1, "diag": diag}
+    todo = a.only.split(",") if a.only else ["kat", "small", "k24", "diag"] + (["c1"] if a.c1 else [])
     for j in todo:
         jobs[j]()
         print("wrote", j, flush=True)

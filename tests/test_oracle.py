@@ -345,3 +345,82 @@ def test_thread_count_does_not_change_results(small):
     ra = a.q_lskum(5)
     rb = b.q_lskum(5)
     assert np.array_equal(ra[0], rb[0]) and np.array_equal(a.get("prim"), b.get("prim"))
+
+
+# ---------------------------------------------------------------- diagnostics
+def test_diagnostics_match_reference(tmp_path):
+    """compute_entropy, compute_enstrophy and stagnation_pressure of the
+    reference run on its own final state (diag_small.npz)."""
+    g = golden("diag_small.npz")
+    cloud = lskum.workload("naca0012_1480_k20")
+    assert _grid_sha(cloud, tmp_path) == str(g["grid_sha"])
+    o = make_oracle(cloud, g["params"])
+    o.set("prim", g["prim"])
+    e, _ = o.compute_entropy()
+    v, v2, tot = o.compute_enstrophy()
+    assert np.array_equal(e, g["entropy"])
+    assert np.array_equal(v, g["vorticity"]) and np.array_equal(v2, g["vorticity_sqr"])
+    assert tot == float(g["total_enstrophy"])
+    stag, idx = o.stagnation_pressure()
+    # the reference prints these four with %f
+    assert np.allclose(stag, g["stagnation_printed"], rtol=0, atol=5e-7)
+
+
+def _uniform(cloud, rho, u1, u2, p):
+    return np.tile(np.array([rho, u1, u2, p])[:, None], (1, cloud.n))
+
+
+def test_stagnation_pressure_closed_forms():
+    cloud = lskum.workload("naca0012_1480_k20")
+    o = make_oracle(cloud, (0.63, 2.0, 0.2, 5.0, 0.0, 0))
+    g = 1.4
+    # isentropic stagnation ratio p0/p = (1 + (g-1)/2 M^2)^(g/(g-1)): at
+    # M = 1, 1.2^3.5 = 1.89293; at M = 2, 1.8^3.5 = 7.82445
+    for M, ratio in ((1.0, 1.8929291587378819), (2.0, 7.824449066867264)):
+        p, rho = 0.7, 1.3
+        a = math.sqrt(g * p / rho)
+        o.set("prim", _uniform(cloud, rho, M * a * 0.6, M * a * 0.8, p))
+        stag, idx = o.stagnation_pressure()
+        assert stag[0] == pytest.approx(ratio * p, rel=1e-13)
+        assert stag[0] == stag[1] and tuple(idx) == (0, 0)  # ties keep the first point
+    # free stream: p0 = p0_inf everywhere, J = 0
+    o.set("prim", _uniform(cloud, 1.0, 0.63 * math.cos(math.radians(2)),
+                           0.63 * math.sin(math.radians(2)), 1 / 1.4))
+    stag, _ = o.stagnation_pressure()
+    assert stag[2] == pytest.approx(1.0, abs=1e-14) and abs(o.objective_J()) < 1e-28
+    # one point brought to rest: its p0 is its p, and J is that one term
+    prim = o.get("prim")
+    prim[1:3, 5] = 0.0
+    o.set("prim", prim)
+    p0_inf = (1 / 1.4) * (1 + 0.2 * 0.63 ** 2) ** 3.5
+    assert o.objective_J() == pytest.approx((p0_inf - 1 / 1.4) ** 2 / (p0_inf ** 2 * cloud.n),
+                                            rel=1e-10)
+    stag, idx = o.stagnation_pressure()
+    assert idx[0] == 5 and stag[0] == pytest.approx(1 / 1.4, rel=1e-15)
+
+
+def test_entropy_and_vorticity_closed_forms():
+    cloud = lskum.workload("naca0012_1480_k20")
+    o = make_oracle(cloud, (0.63, 2.0, 0.2, 5.0, 0.0, 0))
+    x, y = cloud.x, cloud.y
+    # along the isentrope p = p_inf rho^gamma the entropy measure is 0; off
+    # it, |ln(p / rho^g) - ln p_inf|
+    rho = 1.0 + 0.2 * np.sin(x)
+    prim = np.stack([rho, 0 * x, 0 * x, (1 / 1.4) * rho ** 1.4])
+    o.set("prim", prim)
+    e, tot = o.compute_entropy()
+    assert np.abs(e).max() < 1e-14
+    prim[3] *= math.e ** 0.25
+    o.set("prim", prim)
+    e, tot = o.compute_entropy()
+    assert np.allclose(e, 0.25, atol=1e-14) and tot == pytest.approx(0.25 * cloud.n)
+    # solid-body rotation u = w (-y, x): vorticity 2w at every point (the
+    # least squares are exact for linear fields); a strain u = (x, -y) has none
+    w = 0.37
+    o.set("prim", np.stack([1 + 0 * x, -w * y, w * x, 0.7 + 0 * x]))
+    v, v2, tot = o.compute_enstrophy()
+    assert np.allclose(v, 2 * w, rtol=0, atol=1e-11)
+    assert tot == pytest.approx(cloud.n * (2 * w) ** 2, rel=1e-10)
+    o.set("prim", np.stack([1 + 0 * x, x, -y, 0.7 + 0 * x]))
+    v, v2, tot = o.compute_enstrophy()
+    assert np.abs(v).max() < 1e-11
