@@ -108,6 +108,9 @@ _SIGS = [
     ("lskum_restart_write", C.c_int, [C.c_char_p, C.POINTER(CloudView), C.c_int32, C.c_double,
                                       _dp]),
     ("lskum_restart_read", C.c_int, [C.c_char_p, C.c_int32, _ip, _dp, _dp]),
+    ("lskum_solution_write", C.c_int, [C.c_char_p, C.POINTER(CloudView), C.c_int32,
+                                       C.c_double, _dp, _dp]),
+    ("lskum_cp_write", C.c_int, [C.c_char_p, C.POINTER(CloudView), _dp, C.c_double]),
     ("lskum_get_array", C.c_int, [C.c_void_p, C.c_int32, _dp]),
     ("lskum_get_substencil", C.c_int, [C.c_void_p, C.c_int32, _ip, _ip]),
     ("lskum_stream", C.c_void_p, [C.c_void_p]),

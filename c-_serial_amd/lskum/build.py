@@ -20,7 +20,7 @@ BUILD_DIR = os.path.join(ROOT, "_build")
 
 HIP_SOURCES = ["kernels.hip", "kernels_mixed.hip", "adjoint.hip", "trajectory.hip", "api.hip",
                "shape_api.hip", "diagnostics.hip"]
-CXX_SOURCES = ["setup.cpp", "cloud.cpp", "restart.cpp", "shape.cpp"]
+CXX_SOURCES = ["setup.cpp", "cloud.cpp", "restart.cpp", "output.cpp", "shape.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("LSKUM_ARCH", "gfx950")
 

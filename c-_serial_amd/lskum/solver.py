@@ -304,6 +304,21 @@ class Solver:
         L.check(L.lib().lskum_restart_write(str(path).encode(), C.byref(view), it, res_old,
                                             L.dptr(prim)))
 
+    def write_solution(self, path):
+        """output.dat of the current state with its entropy
+        (post_processing_mod.h:7-27, one point per line)."""
+        it, res_old = self.iteration()
+        view = self.cloud.view()
+        entropy, _ = self.compute_entropy()
+        L.check(L.lib().lskum_solution_write(str(path).encode(), C.byref(view), it, res_old,
+                                             L.dptr(self.prim), L.dptr(entropy)))
+
+    def write_cp(self, path):
+        """Cp along the shape points (compute_force_coeffs_mod.h:20-21,54)."""
+        view = self.cloud.view()
+        L.check(L.lib().lskum_cp_write(str(path).encode(), C.byref(view), L.dptr(self.prim),
+                                       self.params.mach))
+
     def read_restart(self, path):
         """Resume from a restart file: its state, iteration counter and
         res_old, with the res_old latch kept (the reference's restart = 1)."""

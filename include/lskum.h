@@ -341,6 +341,26 @@ int lskum_stagnation_pressure(lskum_ctx *ctx, double *values, int32_t *index);
 int lskum_objective_J(lskum_ctx *ctx, double *J);
 
 /*
+ * Output files of a state (host code; prim [4][n] caller order, as from
+ * lskum_get_state), the reference's writers with their defects fixed:
+ *   lskum_solution_write  post_processing_mod.h:7-27: a header line
+ *       "n it res_old", then per point "x y flag_1 rho u1 u2 p Mach entropy",
+ *       one point per line at 17 significant digits (the reference writes
+ *       no separator between points, at 6 digits), Mach = |u| / sqrt(gamma
+ *       p / rho) (the reference takes sqrt(|u|^2 / c)); entropy may be NULL
+ *       (zeros, as the reference writes before compute_entropy has run);
+ *   lskum_cp_write  the Cp file of compute_force_coeffs_mod.h:20-21,54:
+ *       per shape point (the first shape_points points, as the force loop
+ *       takes them) "flag_2 x cp", cp = -(p - p_inf) / (rho_inf mach^2 / 2)
+ *       (the reference's `fout << a, b, c;` writes only flag_2).
+ * LSKUM_EIO on a file that cannot be written.
+ */
+int lskum_solution_write(const char *path, const lskum_cloud *cloud, int32_t it,
+                         double res_old, const double *prim, const double *entropy);
+int lskum_cp_write(const char *path, const lskum_cloud *cloud, const double *prim,
+                   double mach);
+
+/*
  * Shape sensitivities (no reference counterpart).
  *
  * lskum_set_coordinates moves the points of a single-case context to

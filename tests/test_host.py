@@ -341,3 +341,56 @@ def test_restart_reader_errors(tmp_path):
     lines = p.read_text().splitlines()
     (tmp_path / "short.dat").write_text("\n".join(lines[:100]) + "\n")
     assert _restart_read(tmp_path / "short.dat", c.n)[0] == -4
+
+
+def test_solution_file_layout_and_mach(tmp_path):
+    """output.dat (post_processing_mod.h:7-27, defects fixed): one point per
+    line, full precision, Mach = |u|/c."""
+    c = lskum.workload("naca0012_1480_k20")
+    rng = np.random.default_rng(5)
+    rho = 1 + 0.2 * rng.random(c.n)
+    p = 0.7 + 0.1 * rng.random(c.n)
+    a = np.sqrt(1.4 * p / rho)
+    mach = 0.1 + 2 * rng.random(c.n)
+    ang = 2 * np.pi * rng.random(c.n)
+    prim = np.ascontiguousarray(np.stack([rho, mach * a * np.cos(ang), mach * a * np.sin(ang), p]))
+    ent = rng.random(c.n)
+    v = c.view()
+    path = tmp_path / "output.dat"
+    assert L.lib().lskum_solution_write(str(path).encode(), C.byref(v), 42, 1.5e-7,
+                                        L.dptr(prim), L.dptr(ent)) == 0
+    lines = path.read_text().splitlines()
+    assert lines[0].split() == [str(c.n), "42", "1.4999999999999999e-07"]
+    body = np.array([ln.split() for ln in lines[1:]], dtype=float)
+    assert body.shape == (c.n, 9)
+    np.testing.assert_array_equal(body[:, 0], c.x)
+    np.testing.assert_array_equal(body[:, 2], c.flag_1)
+    np.testing.assert_array_equal(body[:, 3:7].T, prim)
+    np.testing.assert_allclose(body[:, 7], mach, rtol=1e-14)   # |u| / c
+    np.testing.assert_array_equal(body[:, 8], ent)
+    # no entropy given: zeros, as before compute_entropy has run
+    assert L.lib().lskum_solution_write(str(path).encode(), C.byref(v), 0, 0.0, L.dptr(prim),
+                                        None) == 0
+    assert np.all(np.loadtxt(path, skiprows=1)[:, 8] == 0.0)
+    assert L.lib().lskum_solution_write(str(tmp_path / "no" / "dir.dat").encode(), C.byref(v),
+                                        0, 0.0, L.dptr(prim), None) == -4
+
+
+def test_cp_file(tmp_path):
+    """Cp file (compute_force_coeffs_mod.h:20-21,54, defect fixed): flag_2, x,
+    cp = -(p - p_inf)/(M^2/2) for the shape points in the force loop's order."""
+    c = lskum.workload("naca0012_1480_k20")
+    nw = int(np.sum(c.flag_2 > 0))
+    p = np.full(c.n, 1 / 1.4)
+    p[:nw] = np.linspace(0.3, 1.2, nw)
+    prim = np.ascontiguousarray(np.stack([np.ones(c.n), np.zeros(c.n), np.zeros(c.n), p]))
+    v = c.view()
+    path = tmp_path / "cp_file"
+    M = 0.63
+    assert L.lib().lskum_cp_write(str(path).encode(), C.byref(v), L.dptr(prim), M) == 0
+    t = np.loadtxt(path)
+    assert t.shape == (nw, 3)
+    np.testing.assert_array_equal(t[:, 0], c.flag_2[:nw])
+    np.testing.assert_array_equal(t[:, 1], c.x[:nw])
+    np.testing.assert_allclose(t[:, 2], -(p[:nw] - 1 / 1.4) / (0.5 * M * M), rtol=1e-15, atol=0)
+    assert L.lib().lskum_cp_write(str(path).encode(), C.byref(v), L.dptr(prim), 0.0) == -1
