@@ -96,4 +96,15 @@ def test_solution_and_cp_files_of_a_state(tmp_path):
     np.testing.assert_allclose(cp[:, 2], -(s.prim[3, :nw] - 1 / 1.4) / (0.5 * 0.63 ** 2),
                                rtol=1e-14)
     # the force integral of these Cp values gives the library's coefficients
-    assert np.isfinite(cp).all()
+    # (compute_force_coeffs_mod.h:25-66: panel length from the left and right
+    # neighbours, the wall normal)
+    m = np.arange(nw)
+    l, r = cloud.left[m], cloud.right[m]
+    ds = 0.5 * (np.hypot(cloud.x[m] - cloud.x[l], cloud.y[m] - cloud.y[l]) +
+                np.hypot(cloud.x[r] - cloud.x[m], cloud.y[r] - cloud.y[m]))
+    nx, ny = s.normals[0, m], s.normals[1, m]
+    H, V = np.sum(cp[:, 2] * nx * ds), np.sum(cp[:, 2] * ny * ds)
+    th = np.radians(2.0)
+    cl, cd, _ = s.compute_cl_cd_cm()
+    assert V * np.cos(th) - H * np.sin(th) == pytest.approx(cl, rel=1e-12)
+    assert H * np.cos(th) + V * np.sin(th) == pytest.approx(cd, rel=1e-12)
