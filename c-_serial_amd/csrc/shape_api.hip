@@ -56,6 +56,33 @@ void force_weights(double theta, const double w[3], double &wH, double &wV, doub
   wP = w[2];
 }
 
+// The stagnation-pressure loss of a state (stagnation_values_mod.h:49-80,
+// the sum the reference forms times its constant 1/(p0_inf^2 n)) and, with
+// grad, weight times its gradient added to grad [4][n]: p0 = p A^g with
+// A = 1 + (gamma-1)/2 M^2, M^2 = |u|^2 rho / (gamma p), g = gamma/(gamma-1).
+double stagnation_loss(const double *prim, int32_t n, double mach, double pr_inf, double weight,
+                       double *grad) {
+  const double gm = 1.4, gp = gm / (gm - 1), c1 = (gm - 1) / 2;
+  const double p0_inf = pr_inf * std::pow(1 + c1 * mach * mach, gp);
+  const double constant = 1 / (std::pow(p0_inf, 2) * n);
+  double sum = 0.0;
+  for (int32_t i = 0; i < n; i++) {
+    const double rho = prim[i], u1 = prim[(size_t)n + i], u2 = prim[(size_t)2 * n + i];
+    const double pr = prim[(size_t)3 * n + i];
+    const double uu = u1 * u1 + u2 * u2, M2 = uu * rho / (gm * pr);
+    const double A = 1 + c1 * M2, Ag = std::pow(A, gp), p0 = pr * Ag;
+    sum += (p0_inf - p0) * (p0_inf - p0);
+    if (!grad) continue;
+    const double dp0 = weight * (-2.0 * (p0_inf - p0) * constant);
+    const double dM2 = dp0 * pr * gp * std::pow(A, gp - 1) * c1;  // d/dM2 of weight J
+    grad[i] += dM2 * (uu / (gm * pr));
+    grad[(size_t)n + i] += dM2 * (2.0 * u1 * rho / (gm * pr));
+    grad[(size_t)2 * n + i] += dM2 * (2.0 * u2 * rho / (gm * pr));
+    grad[(size_t)3 * n + i] += dp0 * Ag + dM2 * (-uu * rho / (gm * pr * pr));
+  }
+  return sum * constant;
+}
+
 // one fpi_solver iteration without the residual history
 int plain_iteration(lskum_ctx *c) {
   const int inner = c->prm.inner_iterations;
@@ -112,6 +139,8 @@ HD hsqrt(HD a) {
   const double r = std::sqrt(a.v);
   return {r, 0.5 * a.d / r};
 }
+HD operator/(HD a, HD b) { return {a.v / b.v, (a.d - a.v / b.v * b.d) / b.v}; }
+HD hpow(HD a, double e) { return {std::pow(a.v, e), e * std::pow(a.v, e - 1) * a.d}; }
 
 }  // namespace
 
@@ -178,7 +207,7 @@ int lskum_set_coordinates(lskum_ctx *ctx, const double *x, const double *y) {
   return LSKUM_OK;
 }
 
-int lskum_shape_vjp(lskum_ctx *ctx, int32_t n_iter, const double w[3], double *J,
+int lskum_shape_vjp(lskum_ctx *ctx, int32_t n_iter, const double w[4], double *J,
                     double *x_bar, double *y_bar, double *prim_bar) {
   CTX_OK(ctx);
   if (!w || n_iter < 0) return set_err(ctx, LSKUM_EINVAL, "null weights or n_iter < 0");
@@ -251,6 +280,9 @@ int lskum_shape_vjp(lskum_ctx *ctx, int32_t n_iter, const double w[3], double *J
     xb[r] += dsb * bx / ds2;
     yb[r] += dsb * by / ds2;
   }
+  // the stagnation-pressure loss: pointwise in P_N, no geometry
+  if (w[3] != 0.0)
+    Jv += w[3] * stagnation_loss(pN.data(), n, cs.mach, ctx->pr_inf, w[3], pb.data());
   HIP_TRY(ctx, hipMemcpyAsync(pbar, pb.data(), n4 * sizeof(double), hipMemcpyHostToDevice, s));
   HIP_TRY(ctx, hipMemsetAsync(g.edge, 0, 2 * E * sizeof(double), s));
   HIP_TRY(ctx, hipMemsetAsync(g.nrm, 0, (size_t)2 * n * sizeof(double), s));
@@ -399,7 +431,7 @@ int lskum_shape_vjp(lskum_ctx *ctx, int32_t n_iter, const double w[3], double *J
   return LSKUM_OK;
 }
 
-int lskum_shape_jvp(lskum_ctx *ctx, int32_t n_iter, const double w[3], const double *dx,
+int lskum_shape_jvp(lskum_ctx *ctx, int32_t n_iter, const double w[4], const double *dx,
                     const double *dy, const double *dprim, double *J, double *dJ) {
   CTX_OK(ctx);
   if (!w || n_iter < 0) return set_err(ctx, LSKUM_EINVAL, "null weights or n_iter < 0");
@@ -469,8 +501,25 @@ int lskum_shape_jvp(lskum_ctx *ctx, int32_t n_iter, const double w[3], const dou
   const double th = cs.theta;
   const double l = V.v * std::cos(th) - H.v * std::sin(th);
   const double d = H.v * std::cos(th) + V.v * std::sin(th);
-  if (J) *J = w[0] * l + w[1] * d + w[2] * P.v;
-  if (dJ) *dJ = wH * H.d + wV * V.d + wP * P.d;
+  double Jv = w[0] * l + w[1] * d + w[2] * P.v, dJv = wH * H.d + wV * V.d + wP * P.d;
+  if (w[3] != 0.0) {  // the stagnation-pressure loss on (value, derivative) pairs
+    const double gm = 1.4, gp = gm / (gm - 1), c1 = (gm - 1) / 2;
+    const double p0_inf = ctx->pr_inf * std::pow(1 + c1 * cs.mach * cs.mach, gp);
+    HD sum;
+    for (int32_t i = 0; i < n; i++) {
+      const HD rho = hd(ph[i]), u1 = hd(ph[(size_t)n + i]), u2 = hd(ph[(size_t)2 * n + i]);
+      const HD pr = hd(ph[(size_t)3 * n + i]);
+      const HD M2 = (u1 * u1 + u2 * u2) * rho / (gm * pr);
+      const HD p0 = pr * hpow(HD{1.0, 0.0} + c1 * M2, gp);
+      const HD t = HD{p0_inf, 0.0} - p0;
+      sum = sum + t * t;
+    }
+    const double constant = 1 / (std::pow(p0_inf, 2) * n);
+    Jv += w[3] * sum.v * constant;
+    dJv += w[3] * sum.d * constant;
+  }
+  if (J) *J = Jv;
+  if (dJ) *dJ = dJv;
   return LSKUM_OK;
 }
 

@@ -229,11 +229,20 @@ class Solver:
         y = np.ascontiguousarray(y, dtype=np.float64).reshape(self.n)
         self._chk(L.lib().lskum_set_coordinates(self._ctx, L.dptr(x), L.dptr(y)))
 
+    @staticmethod
+    def _weights(w):
+        """(Cl, Cd, Cm[, stagnation-pressure loss]) weights as 4 doubles"""
+        out = np.zeros(4)
+        w = np.asarray(w, dtype=np.float64).ravel()
+        out[:len(w)] = w
+        return out
+
     def shape_vjp(self, n_iter, w=(1.0, 0.0, 0.0)):
-        """J = w . (Cl, Cd, Cm) after n_iter iterations from the current
-        state, and its gradients with respect to the coordinates and the
-        starting state by the discrete adjoint: (J, x_bar, y_bar, prim_bar)."""
-        w = np.asarray(w, dtype=np.float64).reshape(3).copy()
+        """J = w . (Cl, Cd, Cm, J_p0) after n_iter iterations from the current
+        state (J_p0 the stagnation-pressure loss of objective_J; w may omit
+        trailing zeros), and its gradients with respect to the coordinates and
+        the starting state by the discrete adjoint: (J, x_bar, y_bar, prim_bar)."""
+        w = self._weights(w)
         J = C.c_double()
         xb, yb, pb = np.empty(self.n), np.empty(self.n), np.empty((4, self.n))
         self._chk(L.lib().lskum_shape_vjp(self._ctx, n_iter, L.dptr(w), C.byref(J),
@@ -244,7 +253,7 @@ class Solver:
         """(J, dJ) with dJ the derivative of J along the coordinate direction
         (dx, dy) and the starting-state direction dprim (None: zero), by a
         forward-mode run of the same iterations."""
-        w = np.asarray(w, dtype=np.float64).reshape(3).copy()
+        w = self._weights(w)
 
         def arr(a, shape):
             return None if a is None else np.ascontiguousarray(a, dtype=np.float64).reshape(shape)
@@ -257,13 +266,16 @@ class Solver:
         return J.value, dJ.value
 
     def objective_after(self, n_iter, w=(1.0, 0.0, 0.0)):
-        """J = w . (Cl, Cd, Cm) after n_iter plain iterations from the current
-        state, which is restored afterwards (the iteration counter is not)."""
+        """J = w . (Cl, Cd, Cm, J_p0) after n_iter plain iterations from the
+        current state, which is restored afterwards (the iteration counter is
+        not)."""
+        w = self._weights(w)
         p0 = self.prim
         self.q_lskum(n_iter)
         cl, cd, cm = self.compute_cl_cd_cm()
+        jp0 = self.objective_J() if w[3] != 0.0 else 0.0
         self.set_state(p0)
-        return float(np.dot(w, (cl, cd, cm)))
+        return float(np.dot(w, (cl, cd, cm, jp0)))
 
     def residual_of(self, prim):
         """flux_res at the state prim with delta held at its last value: the

@@ -371,9 +371,11 @@ int lskum_cp_write(const char *path, const lskum_cloud *cloud, const double *pri
  * built.  A move that leaves a sub-stencil degenerate is refused
  * (LSKUM_ESTENCIL) and the context keeps its coordinates.
  *
- * The objective is J = w[0] Cl + w[1] Cd + w[2] Cm of the state that n_iter
+ * The objective is J = w[0] Cl + w[1] Cd + w[2] Cm + w[3] J_p0 of the state
+ * that n_iter
  * whole iterations (fpi_solver_mod.h:14-150) reach from the context's
- * current state P0, as a function of the coordinates and of P0:
+ * current state P0, as a function of the coordinates and of P0, with J_p0
+ * the stagnation-pressure loss of lskum_objective_J:
  *   lskum_shape_vjp: J, its gradient x_bar, y_bar [n] (caller order) and,
  *     if prim_bar is non-NULL, [4][n] with respect to P0, by the discrete
  *     adjoint of the n_iter iterations -- one reverse sweep that keeps the
@@ -391,9 +393,9 @@ int lskum_cp_write(const char *path, const lskum_cloud *cloud, const double *pri
  * with power = 0 (else LSKUM_EINVAL).
  */
 int lskum_set_coordinates(lskum_ctx *ctx, const double *x, const double *y);
-int lskum_shape_vjp(lskum_ctx *ctx, int32_t n_iter, const double w[3], double *J,
+int lskum_shape_vjp(lskum_ctx *ctx, int32_t n_iter, const double w[4], double *J,
                     double *x_bar, double *y_bar, double *prim_bar);
-int lskum_shape_jvp(lskum_ctx *ctx, int32_t n_iter, const double w[3], const double *dx,
+int lskum_shape_jvp(lskum_ctx *ctx, int32_t n_iter, const double w[4], const double *dx,
                     const double *dy, const double *dprim, double *J, double *dJ);
 
 /*

@@ -125,3 +125,31 @@ def test_design_gradient_matches_central_differences(inner):
                   flush=True)
             assert abs(tan - g[j]) <= 1e-10 * abs(g[j])
             assert abs(fd - g[j]) <= 1e-5 * abs(g[j])
+
+
+def test_stagnation_loss_objective():
+    # J_p0, the stagnation-pressure loss, as the adjoint's objective (alone
+    # and mixed with Cl): duality with the tangent run and central
+    # differences in the starting state and the coordinates
+    cloud = lskum.workload("naca0012_1480_k20")
+    s = Solver(cloud, mach=0.8, aoa=1.25, cfl=0.2, vl_const=150.0, inner_iterations=0)
+    s.q_lskum(40)
+    rng = np.random.default_rng(9)
+    p0 = s.prim
+    for w in [(0.0, 0.0, 0.0, 1.0), (0.5, 0.0, 0.0, 30.0)]:
+        J, xb, yb, pb = s.shape_vjp(5, w)
+        assert J == pytest.approx(s.objective_after(5, w), rel=1e-14)
+        dy = rng.standard_normal(cloud.n) * 1e-3 * (np.hypot(cloud.x - 0.5, cloud.y) < 1)
+        dp = rng.standard_normal((4, cloud.n)) * np.abs(p0) * 1e-3
+        _, dJ = s.shape_jvp(5, w, dy=dy, dprim=dp)
+        rev = float(yb @ dy + np.sum(pb * dp))
+        print(f"case: stagnation-loss objective w={w}: rev {rev:.15e} tan {dJ:.15e}", flush=True)
+        assert abs(rev - dJ) <= 1e-10 * abs(dJ)
+        h = 1e-6
+        s.set_state(p0 + h * dp)
+        jp = s.objective_after(5, w)
+        s.set_state(p0 - h * dp)
+        jm = s.objective_after(5, w)
+        s.set_state(p0)
+        fd_p = (jp - jm) / (2 * h)
+        assert fd_p == pytest.approx(float(np.sum(pb * dp)), rel=1e-5)
