@@ -580,18 +580,27 @@ int create_impl(const lskum_cloud *clouds, int32_t n_clouds, const lskum_case *c
       for (int32_t k : extra) loc[k] = -1;
     }
   }
-  // the gather kernels stage neighbours in LDS; the FP64 flux runs at
-  // higher occupancy reading its neighbours through L2, so by default it
-  // does not (its FP32 records, half the size, do fit at occupancy 2)
+  // The gather kernels stage neighbours in LDS.  The FP64 flux stages part
+  // of each neighbour record (dq, qmax/qmin) when the tiles of as many
+  // blocks as its registers allow per CU fit in the CU's LDS together;
+  // otherwise the tile would lower its occupancy, and it reads every record
+  // through L2.  The mixed form's FP32 records always fit at its occupancy.
+  int dev = 0, lds_limit = 0, lds_per_cu = 0;
+  const bool dev_ok =
+      hipGetDevice(&dev) == hipSuccess &&
+      hipDeviceGetAttribute(&lds_limit, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) ==
+          hipSuccess &&
+      hipDeviceGetAttribute(&lds_per_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor,
+                            dev) == hipSuccess;
+  const int flux_blocks = dev_ok ? flux_blocks_per_cu(TB) : 0;
+  const bool flux_fits = flux_blocks > 0 && (size_t)flux_blocks * lds_bytes_flux(c->tile_max) <=
+                                                (size_t)lds_per_cu;
   const unsigned auto_mask = LSKUM_TILE_QDERIV | LSKUM_TILE_QINNER | LSKUM_TILE_DELTA |
-                             (c->mixed ? LSKUM_TILE_FLUX : 0u);
+                             (c->mixed || flux_fits ? LSKUM_TILE_FLUX : 0u);
   const unsigned want =
       p.lds_tiles == LSKUM_TILES_AUTO ? auto_mask : (unsigned)p.lds_tiles;
   if (tiles_ok && want) {
-    int dev = 0, lds_limit = 0;
-    if (hipGetDevice(&dev) == hipSuccess &&
-        hipDeviceGetAttribute(&lds_limit, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) ==
-            hipSuccess) {
+    if (dev_ok) {
       if (c->mixed)  // func_delta is FP64 in both forms
         c->tiles = prepare_tiled(want & TILE_DELTA, c->tile_max, (size_t)lds_limit) |
                    prepare_tiled_m(want & ~TILE_DELTA, c->tile_max, (size_t)lds_limit);

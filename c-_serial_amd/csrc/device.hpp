@@ -61,4 +61,21 @@ __device__ __forceinline__ int nbr_index(const uint32_t *conn, const uint16_t *c
   return (int)(w & kIndexMask);
 }
 
+// The raw conn entry e (tiled: the 16-bit local form) and its parts.  The
+// stencil kernels load the entry of neighbour e + 2 while neighbour e is
+// processed and split it only when it is used, so no instruction waits on
+// the load before then.
+__device__ __forceinline__ uint32_t conn_raw(const uint32_t *conn, const uint16_t *conn16, int e,
+                                             bool tiled) {
+  return tiled ? (uint32_t)conn16[e] : conn[e];
+}
+
+__device__ __forceinline__ int conn_index(uint32_t w, bool tiled) {
+  return tiled ? (int)(w & kLocalMask) : (int)(w & kIndexMask);
+}
+
+__device__ __forceinline__ uint32_t conn_mask(uint32_t w, bool tiled) {
+  return tiled ? w >> kLocalShift : w >> kMaskShift;
+}
+
 }  // namespace lskum

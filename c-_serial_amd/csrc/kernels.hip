@@ -29,9 +29,9 @@ __device__ __forceinline__ void load_tile(double *lds, const int32_t *__restrict
                                           const int32_t *__restrict__ tile_idx, int b,
                                           const double *__restrict__ src, int src_stride,
                                           int first_chunk, int nchunks, int dst_offset) {
-  // four 16-byte gathers in flight per thread before their LDS stores
-  // (measured faster than eight on the 2.5M-point cloud)
-  constexpr int U = 4;
+  // sixteen 16-byte gathers in flight per thread before their LDS stores
+  // (on the 2.5M-point cloud q_inner takes 0.375 ms with 16, 0.40 with 4)
+  constexpr int U = 16;
   const int t0 = tile_off[b], total = (tile_off[b + 1] - t0) * nchunks;
   for (int base = threadIdx.x; base < total; base += U * blockDim.x) {
     double2 v[U];
@@ -111,18 +111,33 @@ __global__ __launch_bounds__(256) void k_func_delta(int n, const int32_t *__rest
   const double *hi = TILED ? lds + S * threadIdx.x : base + (size_t)B_STRIDE * i;
   const double2 xy = reinterpret_cast<const double2 *>(hi)[0];
   double min_delt = 1.0;
-  const int e1 = off[i + 1];
-  for (int e = off[i]; e < e1; e++) {
-    uint32_t m;
-    const int k = nbr_index(conn, conn16, e, TILED, m);
+  // neighbour e + 1's record and e + 2's conn entry load while e is used
+  auto fetch = [&](uint32_t w, double2 &xyk, double &speed) {
+    const int k = conn_index(w, TILED);
     const double *hk = TILED ? lds + S * k : base + (size_t)B_STRIDE * k;
-    const double2 xyk = reinterpret_cast<const double2 *>(hk)[0];
-    const double speed = TILED ? hk[2] : hk[B_SPEED];
+    xyk = reinterpret_cast<const double2 *>(hk)[0];
+    speed = TILED ? hk[2] : hk[B_SPEED];
+  };
+  const int e0 = off[i], last = off[i + 1] - 1;
+  double2 xyk = {0.0, 0.0};
+  double speed = 1.0;
+  uint32_t w_n = 0;
+  if (e0 <= last) {
+    fetch(conn_raw(conn, conn16, e0, TILED), xyk, speed);
+    w_n = conn_raw(conn, conn16, min(e0 + 1, last), TILED);
+  }
+  for (int e = e0; e <= last; e++) {
+    double2 xy_n;
+    double speed_n;
+    fetch(w_n, xy_n, speed_n);
+    w_n = conn_raw(conn, conn16, min(e + 2, last), TILED);
     double dist = (xyk.x - xy.x) * (xyk.x - xy.x) + (xyk.y - xy.y) * (xyk.y - xy.y);
     dist = sqrt(dist);
     double delta_t = dist / speed;
     delta_t = cfl * delta_t;
     if (min_delt > delta_t) min_delt = delta_t;
+    xyk = xy_n;
+    speed = speed_n;
   }
   delta[i] = min_delt;
 }
@@ -154,14 +169,26 @@ __global__ __launch_bounds__(256) void k_q_derivatives(
   double sum_delx_sqr = 0., sum_dely_sqr = 0., sum_delx_dely = 0.;
   double sdx[4] = {0.0, 0.0, 0.0, 0.0}, sdy[4] = {0.0, 0.0, 0.0, 0.0};
   for (int r = 0; r < 4; r++) qmax[r] = qmin[r] = qi[r];
-  const int e1 = off[i + 1];
-  for (int e = off[i]; e < e1; e++) {
-    uint32_t m;
-    const int k = nbr_index(conn, conn16, e, TILED, m);
+  // neighbour e + 1's record and e + 2's conn entry load while e is used
+  auto fetch = [&](uint32_t w, double2 &xyk, double qk[4]) {
+    const int k = conn_index(w, TILED);
     const double *hk = TILED ? lds + S * k : base + (size_t)B_STRIDE * k;
-    const double2 xyk = reinterpret_cast<const double2 *>(hk)[0];
-    double qk[4];
+    xyk = reinterpret_cast<const double2 *>(hk)[0];
     load4(hk + B_Q, qk);
+  };
+  const int e0 = off[i], last = off[i + 1] - 1;
+  double2 xyk = {0.0, 0.0};
+  double qk[4] = {0.0, 0.0, 0.0, 0.0};
+  uint32_t w_n = 0;
+  if (e0 <= last) {
+    fetch(conn_raw(conn, conn16, e0, TILED), xyk, qk);
+    w_n = conn_raw(conn, conn16, min(e0 + 1, last), TILED);
+  }
+  for (int e = e0; e <= last; e++) {
+    double2 xy_n;
+    double q_n[4];
+    fetch(w_n, xy_n, q_n);
+    w_n = conn_raw(conn, conn16, min(e + 2, last), TILED);
     for (int r = 0; r < 4; r++) {
       if (qk[r] > qmax[r]) qmax[r] = qk[r];
       if (qk[r] < qmin[r]) qmin[r] = qk[r];
@@ -185,6 +212,8 @@ __global__ __launch_bounds__(256) void k_q_derivatives(
         sdy[r] = sdy[r] + dely * (qk[r] - qi[r]);
       }
     }
+    xyk = xy_n;
+    for (int r = 0; r < 4; r++) qk[r] = q_n[r];
   }
   const double det = sum_delx_sqr * sum_dely_sqr - sum_delx_dely * sum_delx_dely;
   // det > 0 for every validated point; 0 only for a batch's padding points,
@@ -232,17 +261,29 @@ __global__ __launch_bounds__(256) void k_q_inner(
   load4(di + 4, dyi);
   double sum_delx_sqr = 0., sum_dely_sqr = 0., sum_delx_dely = 0.;
   double sx[4] = {0.0, 0.0, 0.0, 0.0}, sy[4] = {0.0, 0.0, 0.0, 0.0};
-  const int e1 = off[i + 1];
-  for (int e = off[i]; e < e1; e++) {
-    uint32_t m;
-    const int k = nbr_index(conn, conn16, e, TILED, m);
+  // neighbour e + 1's record and e + 2's conn entry load while e is used
+  auto fetch = [&](uint32_t w, double2 &xyk, double qk[4], double dxk[4], double dyk[4]) {
+    const int k = conn_index(w, TILED);
     const double *hk = TILED ? lds + S * k : base + (size_t)B_STRIDE * k;
     const double *dk = TILED ? hk + 6 : src + (size_t)DQ_STRIDE * k;
-    const double2 xyk = reinterpret_cast<const double2 *>(hk)[0];
-    double qk[4], dxk[4], dyk[4];
+    xyk = reinterpret_cast<const double2 *>(hk)[0];
     load4(hk + B_Q, qk);
     load4(dk, dxk);
     load4(dk + 4, dyk);
+  };
+  const int e0 = off[i], last = off[i + 1] - 1;
+  double2 xyk = {0.0, 0.0};
+  double qk[4] = {}, dxk[4] = {}, dyk[4] = {};
+  uint32_t w_n = 0;
+  if (e0 <= last) {
+    fetch(conn_raw(conn, conn16, e0, TILED), xyk, qk, dxk, dyk);
+    w_n = conn_raw(conn, conn16, min(e0 + 1, last), TILED);
+  }
+  for (int e = e0; e <= last; e++) {
+    double2 xy_n;
+    double q_n[4], dx_n[4], dy_n[4];
+    fetch(w_n, xy_n, q_n, dx_n, dy_n);
+    w_n = conn_raw(conn, conn16, min(e + 2, last), TILED);
     const double delx = xyk.x - x_i, dely = xyk.y - y_i;
     const double weights = POW ? pow(sqrt(delx * delx + dely * dely), power) : 1.0;
     if (POW) {
@@ -264,6 +305,12 @@ __global__ __launch_bounds__(256) void k_q_inner(
         sx[r] = sx[r] + (delx * (temp2 - temp1));
         sy[r] = sy[r] + (dely * (temp2 - temp1));
       }
+    }
+    xyk = xy_n;
+    for (int r = 0; r < 4; r++) {
+      qk[r] = q_n[r];
+      dxk[r] = dx_n[r];
+      dyk[r] = dy_n[r];
     }
   }
   const double det = sum_delx_sqr * sum_dely_sqr - sum_delx_dely * sum_delx_dely;
@@ -297,6 +344,66 @@ __global__ __launch_bounds__(256) void k_q_inner(
 // coef per point: x+ (a,b), x- (a,b), y+ (a,b), y- (a,b).
 constexpr int COEF_STRIDE = 8;
 
+// The flux's LDS tile holds, per tile entry, the point's dq (8 doubles) and
+// qmax/qmin (8 doubles) as eight 16-byte chunks; its 64-byte base record
+// (x, y, q, eps2) is read through L2.  The tile then takes 128 bytes per
+// entry, so two blocks of a 605-entry tile share a CU's LDS and the kernel
+// keeps the two waves per SIMD its registers allow, while the bytes it
+// gathers through L2 per neighbour fall from 192 to 64.  Chunk c of entry r
+// is stored at position c ^ (r & 7): lanes reading the same chunk of
+// different entries then fall on different LDS banks.
+constexpr int FLUX_TILE_CHUNKS = 8;
+
+__device__ __forceinline__ void load_flux_tile(double2 *lds, const int32_t *__restrict__ tile_off,
+                                               const int32_t *__restrict__ tile_idx, int b,
+                                               const double *__restrict__ dq,
+                                               const double *__restrict__ qm) {
+  constexpr int U = 4;
+  const int t0 = tile_off[b], total = (tile_off[b + 1] - t0) * FLUX_TILE_CHUNKS;
+  for (int base = threadIdx.x; base < total; base += U * blockDim.x) {
+    double2 v[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      int t = base + u * blockDim.x;
+      t = t < total ? t : total - 1;
+      const int r = t >> 3, ch = t & 7;
+      const size_t g = (size_t)tile_idx[t0 + r];
+      const double *src = ch < 4 ? dq + DQ_STRIDE * g : qm + QM_STRIDE * g;
+      v[u] = reinterpret_cast<const double2 *>(src)[ch & 3];
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int t = base + u * blockDim.x;
+      if (t < total) {
+        const int r = t >> 3, ch = t & 7;
+        lds[FLUX_TILE_CHUNKS * r + (ch ^ (r & 7))] = v[u];
+      }
+    }
+  }
+}
+
+// A point's base record (x, y, q[4], eps2, speed) as four 16-byte loads.
+struct BaseRec {
+  double2 xy, q01, q23, eps_speed;
+};
+
+__device__ __forceinline__ BaseRec load_base(const double *__restrict__ base, uint32_t k) {
+  const double2 *p = reinterpret_cast<const double2 *>(base + (size_t)B_STRIDE * k);
+  return {p[0], p[1], p[2], p[3]};
+}
+
+// entry r's dq/dx, dq/dy, qmax, qmin from the flux tile
+__device__ __forceinline__ void flux_tile_entry(const double2 *lds, int r, double dx[4],
+                                                double dy[4], double qmx[4], double qmn[4]) {
+  double2 c[FLUX_TILE_CHUNKS];
+#pragma unroll
+  for (int j = 0; j < FLUX_TILE_CHUNKS; j++) c[j] = lds[FLUX_TILE_CHUNKS * r + (j ^ (r & 7))];
+  dx[0] = c[0].x; dx[1] = c[0].y; dx[2] = c[1].x; dx[3] = c[1].y;
+  dy[0] = c[2].x; dy[1] = c[2].y; dy[2] = c[3].x; dy[3] = c[3].y;
+  qmx[0] = c[4].x; qmx[1] = c[4].y; qmx[2] = c[5].x; qmx[3] = c[5].y;
+  qmn[0] = c[6].x; qmn[1] = c[6].y; qmn[2] = c[7].x; qmn[3] = c[7].y;
+}
+
 template <bool TILED, bool POW>
 __global__ __launch_bounds__(256) void k_flux(
     int n, const int32_t *__restrict__ off, const uint32_t *__restrict__ conn,
@@ -306,28 +413,29 @@ __global__ __launch_bounds__(256) void k_flux(
     const double *__restrict__ coef, const double *__restrict__ nxa,
     const double *__restrict__ nya, const uint8_t *__restrict__ flag,
     const double *__restrict__ delta, double power, double *__restrict__ fres) {
-  extern __shared__ double lds[];
-  constexpr int S = 24;  // base (8), dq (8), qm (8)
+  extern __shared__ double2 lds2[];
   const int b = xcd_block();
   if (TILED) {
-    load_tile<S>(lds, tile_off, tile_idx, b, base, B_STRIDE, 0, 4, 0);
-    load_tile<S>(lds, tile_off, tile_idx, b, dq, DQ_STRIDE, 0, 4, 8);
-    load_tile<S>(lds, tile_off, tile_idx, b, qm, QM_STRIDE, 0, 4, 16);
+    load_flux_tile(lds2, tile_off, tile_idx, b, dq, qm);
     __syncthreads();
   }
   const int i = b * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const double *hi = TILED ? lds + S * threadIdx.x : base + (size_t)B_STRIDE * i;
-  const double *di = TILED ? hi + 8 : dq + (size_t)DQ_STRIDE * i;
-  const double *mi = TILED ? hi + 16 : qm + (size_t)QM_STRIDE * i;
+  const double *hi = base + (size_t)B_STRIDE * i;
   const double2 xy = reinterpret_cast<const double2 *>(hi)[0];
   const double x_i = xy.x, y_i = xy.y;
   double qi[4], dxi[4], dyi[4], qmxi[4], qmni[4], cf[8];
   load4(hi + B_Q, qi);
-  load4(di, dxi);
-  load4(di + 4, dyi);
-  load4(mi, qmxi);
-  load4(mi + 4, qmni);
+  if (TILED) {
+    flux_tile_entry(lds2, threadIdx.x, dxi, dyi, qmxi, qmni);
+  } else {
+    const double *di = dq + (size_t)DQ_STRIDE * i;
+    const double *mi = qm + (size_t)QM_STRIDE * i;
+    load4(di, dxi);
+    load4(di + 4, dyi);
+    load4(mi, qmxi);
+    load4(mi + 4, qmni);
+  }
   load4(coef + (size_t)COEF_STRIDE * i, cf);
   load4(coef + (size_t)COEF_STRIDE * i + 4, cf + 4);
   const double eps2_i = hi[B_EPS2];
@@ -337,21 +445,44 @@ __global__ __launch_bounds__(256) void k_flux(
   // wall: quadrant fluxes with A2neg and Gyn; outer: A2pos and Gyp
   const double s2b = type == 0 ? -1.0 : 1.0;
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  const int e1 = off[i + 1];
-  for (int e = off[i]; e < e1; e++) {
-    uint32_t m;
-    const int k = nbr_index(conn, conn16, e, TILED, m);
-    const double *hk = TILED ? lds + S * k : base + (size_t)B_STRIDE * k;
-    const double *dk = TILED ? hk + 8 : dq + (size_t)DQ_STRIDE * k;
-    const double *mk = TILED ? hk + 16 : qm + (size_t)QM_STRIDE * k;
-    const double2 xyk = reinterpret_cast<const double2 *>(hk)[0];
-    double qk[4], dxk[4], dyk[4], qmxk[4], qmnk[4];
-    load4(hk + B_Q, qk);
-    load4(dk, dxk);
-    load4(dk + 4, dyk);
-    load4(mk, qmxk);
-    load4(mk + 4, qmnk);
-    const double eps2_k = hk[B_EPS2];
+  // Software pipeline over the neighbour list: while neighbour e is
+  // computed, the base record of e + 1 and the conn entries of e + 2 are in
+  // flight (indices clamped to the last entry), so their L2 latency hides
+  // behind a whole neighbour's arithmetic.
+  const int e0 = off[i], last = off[i + 1] - 1;
+  uint32_t w_c = 0, w_n = 0;
+  uint16_t l_c = 0, l_n = 0;
+  BaseRec r_c = {};
+  if (e0 <= last) {
+    w_c = conn[e0];
+    w_n = conn[min(e0 + 1, last)];
+    if (TILED) {
+      l_c = conn16[e0];
+      l_n = conn16[min(e0 + 1, last)];
+    }
+    r_c = load_base(base, w_c & kIndexMask);
+  }
+  for (int e = e0; e <= last; e++) {
+    const BaseRec r_n = load_base(base, w_n & kIndexMask);
+    const int enn = min(e + 2, last);
+    const uint32_t w_nn = conn[enn];
+    const uint16_t l_nn = TILED ? conn16[enn] : 0;
+    const uint32_t m = w_c >> kMaskShift;
+    const int k = (int)(w_c & kIndexMask);
+    const double2 xyk = r_c.xy;
+    double qk[4] = {r_c.q01.x, r_c.q01.y, r_c.q23.x, r_c.q23.y};
+    double dxk[4], dyk[4], qmxk[4], qmnk[4];
+    if (TILED) {
+      flux_tile_entry(lds2, l_c & kLocalMask, dxk, dyk, qmxk, qmnk);
+    } else {
+      const double *dk = dq + (size_t)DQ_STRIDE * k;
+      const double *mk = qm + (size_t)QM_STRIDE * k;
+      load4(dk, dxk);
+      load4(dk + 4, dyk);
+      load4(mk, qmxk);
+      load4(mk + 4, qmnk);
+    }
+    const double eps2_k = r_c.eps_speed.x;
     const double delx = xyk.x - x_i, dely = xyk.y - y_i;
     const double dels = delx * tx + dely * ty;
     const double deln = delx * nx + dely * ny;
@@ -413,6 +544,11 @@ __global__ __launch_bounds__(256) void k_flux(
       const double wn = cf[6] * dnw + cf[7] * dsw;
       for (int r = 0; r < 4; r++) acc[r] += diff_unfused(Gk[r], Gi[r]) * wn;
     }
+    w_c = w_n;
+    w_n = w_nn;
+    l_c = l_n;
+    l_n = l_nn;
+    r_c = r_n;
   }
   // flux_residual_mod.h:29 (wall: 2 delta), :45, :85
   const double d = type == 0 ? 2.0 * delta[i] : delta[i];
@@ -720,7 +856,17 @@ int blocks_for(int n, int block) { return (n + block - 1) / block; }
 size_t lds_bytes_qderiv(int tile_max) { return sizeof(double) * 6 * (size_t)tile_max; }
 size_t lds_bytes_qinner(int tile_max) { return sizeof(double) * 14 * (size_t)tile_max; }
 size_t lds_bytes_delta(int tile_max) { return sizeof(double) * 4 * (size_t)tile_max; }
-size_t lds_bytes_flux(int tile_max) { return sizeof(double) * 24 * (size_t)tile_max; }
+size_t lds_bytes_flux(int tile_max) {
+  return sizeof(double2) * FLUX_TILE_CHUNKS * (size_t)tile_max;
+}
+
+int flux_blocks_per_cu(int block) {
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_flux<false, false>, block, 0) !=
+      hipSuccess)
+    return 0;
+  return nb;
+}
 
 unsigned prepare_tiled(unsigned tiles, int tile_max, size_t lds_limit) {
   struct K { unsigned bit; const void *f; size_t bytes; };

@@ -67,6 +67,8 @@ size_t lds_bytes_qderiv(int tile_max);
 size_t lds_bytes_qinner(int tile_max);
 size_t lds_bytes_delta(int tile_max);
 size_t lds_bytes_flux(int tile_max);
+// blocks of `block` threads the FP64 flux's registers let one CU hold
+int flux_blocks_per_cu(int block);
 // raises the dynamic-LDS limits of the kernels in `tiles`; returns the
 // subset whose tiles fit in lds_limit bytes
 unsigned prepare_tiled(unsigned tiles, int tile_max, size_t lds_limit);

@@ -53,6 +53,32 @@ __device__ __forceinline__ void tile_copy16(uint4 *dst, const uint4 *__restrict_
   }
 }
 
+// The flux's tile: per entry q, dq/dx, dq/dy, qmax, qmin as five 16-byte
+// chunks (dst[5r + ch]).
+__device__ __forceinline__ void tile_copy_flux(uint4 *dst, const uint4 *__restrict__ q,
+                                               const uint4 *__restrict__ dq,
+                                               const uint4 *__restrict__ qm,
+                                               const int32_t *__restrict__ idx, int count) {
+  constexpr int U = 4;
+  const int total = count * 5;
+  for (int base = threadIdx.x; base < total; base += U * blockDim.x) {
+    uint4 v[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      int t = base + u * blockDim.x;
+      t = t < total ? t : total - 1;
+      const int r = t / 5, ch = t - 5 * r;
+      const size_t g = (size_t)idx[r];
+      v[u] = ch == 0 ? q[g] : ch < 3 ? dq[2 * g + (ch - 1)] : qm[2 * g + (ch - 3)];
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int t = base + u * blockDim.x;
+      if (t < total) dst[t] = v[u];
+    }
+  }
+}
+
 __device__ __forceinline__ void tile_copy4(float *dst, const float *__restrict__ src,
                                            const int32_t *__restrict__ idx, int count) {
   for (int t = threadIdx.x; t < count; t += blockDim.x) dst[t] = src[idx[t]];
@@ -104,13 +130,24 @@ __global__ __launch_bounds__(256) void k_q_derivatives_m(
   double sum_delx_sqr = 0., sum_dely_sqr = 0., sum_delx_dely = 0.;
   double sdx[4] = {0.0, 0.0, 0.0, 0.0}, sdy[4] = {0.0, 0.0, 0.0, 0.0};
   for (int r = 0; r < 4; r++) qmax[r] = qmin[r] = qi[r];
-  const int e1 = off[i + 1];
-  for (int e = off[i]; e < e1; e++) {
-    uint32_t m;
-    const int k = nbr_index(conn, conn16, e, TILED, m);
-    const double2 pk = gxy[k];
+  // neighbour e + 1's record and e + 2's conn entry load while e is used
+  const int e0 = off[i], last = off[i + 1] - 1;
+  double2 pk = {0.0, 0.0};
+  float4 qk4 = {0.0f, 0.0f, 0.0f, 0.0f};
+  uint32_t w_n = 0;
+  if (e0 <= last) {
+    const int k0 = conn_index(conn_raw(conn, conn16, e0, TILED), TILED);
+    pk = gxy[k0];
+    qk4 = gq[k0];
+    w_n = conn_raw(conn, conn16, min(e0 + 1, last), TILED);
+  }
+  for (int e = e0; e <= last; e++) {
+    const int kn = conn_index(w_n, TILED);
+    const double2 pk_n = gxy[kn];
+    const float4 qk4_n = gq[kn];
+    w_n = conn_raw(conn, conn16, min(e + 2, last), TILED);
     float qk[4];
-    unpack4(gq[k], qk);
+    unpack4(qk4, qk);
     for (int r = 0; r < 4; r++) {
       qmax[r] = qk[r] > qmax[r] ? qk[r] : qmax[r];
       qmin[r] = qk[r] < qmin[r] ? qk[r] : qmin[r];
@@ -126,6 +163,8 @@ __global__ __launch_bounds__(256) void k_q_derivatives_m(
       sdx[r] = sdx[r] + wx * d;
       sdy[r] = sdy[r] + wy * d;
     }
+    pk = pk_n;
+    qk4 = qk4_n;
   }
   const double det = sum_delx_sqr * sum_dely_sqr - sum_delx_dely * sum_delx_dely;
   // det > 0 for every validated point; 0 only for a batch's padding points,
@@ -177,15 +216,28 @@ __global__ __launch_bounds__(256) void k_q_inner_m(
   unpack4(gd[2 * li + 1], dyi);
   double sum_delx_sqr = 0., sum_dely_sqr = 0., sum_delx_dely = 0.;
   double sx[4] = {0.0, 0.0, 0.0, 0.0}, sy[4] = {0.0, 0.0, 0.0, 0.0};
-  const int e1 = off[i + 1];
-  for (int e = off[i]; e < e1; e++) {
-    uint32_t m;
-    const int k = nbr_index(conn, conn16, e, TILED, m);
-    const double2 pk = gxy[k];
+  // neighbour e + 1's record and e + 2's conn entry load while e is used
+  const int e0 = off[i], last = off[i + 1] - 1;
+  double2 pk = {0.0, 0.0};
+  float4 qk4 = {}, dxk4 = {}, dyk4 = {};
+  uint32_t w_n = 0;
+  if (e0 <= last) {
+    const int k0 = conn_index(conn_raw(conn, conn16, e0, TILED), TILED);
+    pk = gxy[k0];
+    qk4 = gq[k0];
+    dxk4 = gd[2 * k0];
+    dyk4 = gd[2 * k0 + 1];
+    w_n = conn_raw(conn, conn16, min(e0 + 1, last), TILED);
+  }
+  for (int e = e0; e <= last; e++) {
+    const int kn = conn_index(w_n, TILED);
+    const double2 pk_n = gxy[kn];
+    const float4 qk4_n = gq[kn], dxk4_n = gd[2 * kn], dyk4_n = gd[2 * kn + 1];
+    w_n = conn_raw(conn, conn16, min(e + 2, last), TILED);
     float qk[4], dxk[4], dyk[4];
-    unpack4(gq[k], qk);
-    unpack4(gd[2 * k], dxk);
-    unpack4(gd[2 * k + 1], dyk);
+    unpack4(qk4, qk);
+    unpack4(dxk4, dxk);
+    unpack4(dyk4, dyk);
     const double delx = pk.x - pi.x, dely = pk.y - pi.y;
     const float fx = (float)delx, fy = (float)dely;
     const double weights = POW ? pow(sqrt(delx * delx + dely * dely), power) : 1.0;
@@ -199,6 +251,10 @@ __global__ __launch_bounds__(256) void k_q_inner_m(
       sx[r] = sx[r] + wx * (double)d;
       sy[r] = sy[r] + wy * (double)d;
     }
+    pk = pk_n;
+    qk4 = qk4_n;
+    dxk4 = dxk4_n;
+    dyk4 = dyk4_n;
   }
   const double det = sum_delx_sqr * sum_dely_sqr - sum_delx_dely * sum_delx_dely;
   // det > 0 for every validated point; 0 only for a batch's padding points,
@@ -231,44 +287,47 @@ __global__ __launch_bounds__(256) void k_flux_m(
     const double *__restrict__ coef, const double *__restrict__ nxa,
     const double *__restrict__ nya, const uint8_t *__restrict__ flag,
     const double *__restrict__ delta, double power, double *__restrict__ fres) {
-  // LDS tile: xy (16 B), q (16 B), dq (32 B), qm (32 B), eps2 (4 B) per entry
+  // LDS tile: q, dq, qm (80 B) and eps2 (4 B) per entry; the FP64
+  // coordinates (16 B) are read through L2.  At a 605-entry tile three
+  // blocks share a CU, the occupancy the kernel's registers allow.
   extern __shared__ uint4 lds4[];
-  double2 *lxy = reinterpret_cast<double2 *>(lds4);
-  float4 *lq = reinterpret_cast<float4 *>(lds4 + tile_max);
-  float4 *ld = reinterpret_cast<float4 *>(lds4 + 2 * tile_max);
-  float4 *lm = reinterpret_cast<float4 *>(lds4 + 4 * tile_max);
-  float *le = reinterpret_cast<float *>(lds4 + 6 * tile_max);
+  const float4 *lt = reinterpret_cast<const float4 *>(lds4);
+  float *le = reinterpret_cast<float *>(lds4 + 5 * tile_max);
   const int b = xcd_block();
   if (TILED) {
     const int t0 = tile_off[b], tn = tile_off[b + 1] - t0;
     const int32_t *idx = tile_idx + t0;
-    tile_copy16<1>(lds4, reinterpret_cast<const uint4 *>(xy), idx, tn);
-    tile_copy16<1>(lds4 + tile_max, reinterpret_cast<const uint4 *>(q32), idx, tn);
-    tile_copy16<2>(lds4 + 2 * tile_max, reinterpret_cast<const uint4 *>(dq32), idx, tn);
-    tile_copy16<2>(lds4 + 4 * tile_max, reinterpret_cast<const uint4 *>(qm32), idx, tn);
+    tile_copy_flux(lds4, reinterpret_cast<const uint4 *>(q32),
+                   reinterpret_cast<const uint4 *>(dq32), reinterpret_cast<const uint4 *>(qm32),
+                   idx, tn);
     tile_copy4(le, eps32, idx, tn);
     __syncthreads();
   }
   const int i = b * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const double2 *gxy = TILED ? lxy : xy;
-  const float4 *gq = TILED ? lq : q32;
-  const float4 *gd = TILED ? ld : dq32;
-  const float4 *gm = TILED ? lm : qm32;
-  const float *ge = TILED ? le : eps32;
-  const int li = TILED ? (int)threadIdx.x : i;
-  const double2 pi = gxy[li];
+  const double2 pi = xy[i];
   float qi[4], dxi[4], dyi[4], dmaxi[4], dmini[4];
-  unpack4(gq[li], qi);
-  unpack4(gd[2 * li], dxi);
-  unpack4(gd[2 * li + 1], dyi);
-  unpack4(gm[2 * li], dmaxi);
-  unpack4(gm[2 * li + 1], dmini);
+  float eps2_i;
+  if (TILED) {
+    const int li = 5 * (int)threadIdx.x;
+    unpack4(lt[li], qi);
+    unpack4(lt[li + 1], dxi);
+    unpack4(lt[li + 2], dyi);
+    unpack4(lt[li + 3], dmaxi);
+    unpack4(lt[li + 4], dmini);
+    eps2_i = le[threadIdx.x];
+  } else {
+    unpack4(q32[i], qi);
+    unpack4(dq32[2 * i], dxi);
+    unpack4(dq32[2 * i + 1], dyi);
+    unpack4(qm32[2 * i], dmaxi);
+    unpack4(qm32[2 * i + 1], dmini);
+    eps2_i = eps32[i];
+  }
   for (int r = 0; r < 4; r++) {
     dmaxi[r] -= qi[r];
     dmini[r] -= qi[r];
   }
-  const float eps2_i = ge[li];
   double cf[8];
   load4(coef + (size_t)COEF_STRIDE * i, cf);
   load4(coef + (size_t)COEF_STRIDE * i + 4, cf + 4);
@@ -278,18 +337,46 @@ __global__ __launch_bounds__(256) void k_flux_m(
   const int type = flag[i];
   const float s2b = type == 0 ? -1.0f : 1.0f;
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  const int e1 = off[i + 1];
-  for (int e = off[i]; e < e1; e++) {
-    uint32_t m;
-    const int k = nbr_index(conn, conn16, e, TILED, m);
-    const double2 pk = gxy[k];
+  // neighbour e + 1's record and e + 2's conn entry load while e is used
+  struct Rec {
+    double2 p;
+    float4 q, dx, dy, mx, mn;
+    float eps2;
+  };
+  // the global entry (coordinates) and, tiled, the local one (the rest)
+  auto fetch = [&](uint32_t wg, uint32_t wl) -> Rec {
+    const int kg = conn_index(wg, false);
+    if (TILED) {
+      const int kl = conn_index(wl, true);
+      const float4 *t = lt + 5 * kl;
+      return {xy[kg], t[0], t[1], t[2], t[3], t[4], le[kl]};
+    }
+    return {xy[kg], q32[kg], dq32[2 * kg], dq32[2 * kg + 1], qm32[2 * kg], qm32[2 * kg + 1],
+            eps32[kg]};
+  };
+  const int e0 = off[i], last = off[i + 1] - 1;
+  Rec rc = {};
+  uint32_t w_c = 0, w_n = 0, l_n = 0;
+  if (e0 <= last) {
+    w_c = conn[e0];
+    rc = fetch(w_c, TILED ? conn16[e0] : 0u);
+    w_n = conn[min(e0 + 1, last)];
+    l_n = TILED ? conn16[min(e0 + 1, last)] : 0u;
+  }
+  for (int e = e0; e <= last; e++) {
+    const Rec rn = fetch(w_n, l_n);
+    const int enn = min(e + 2, last);
+    const uint32_t w_nn = conn[enn];
+    const uint32_t l_nn = TILED ? conn16[enn] : 0u;
+    const uint32_t m = conn_mask(w_c, false);
+    const double2 pk = rc.p;
     float qk[4], dxk[4], dyk[4], dmaxk[4], dmink[4];
-    unpack4(gq[k], qk);
-    unpack4(gd[2 * k], dxk);
-    unpack4(gd[2 * k + 1], dyk);
-    unpack4(gm[2 * k], dmaxk);
-    unpack4(gm[2 * k + 1], dmink);
-    const float eps2_k = ge[k];
+    unpack4(rc.q, qk);
+    unpack4(rc.dx, dxk);
+    unpack4(rc.dy, dyk);
+    unpack4(rc.mx, dmaxk);
+    unpack4(rc.mn, dmink);
+    const float eps2_k = rc.eps2;
     const double delx = pk.x - pi.x, dely = pk.y - pi.y;
     const double dels = delx * tx + dely * ty;
     const double deln = delx * nx + dely * ny;
@@ -351,6 +438,10 @@ __global__ __launch_bounds__(256) void k_flux_m(
       const double wn = cf[6] * dnw + cf[7] * dsw;
       for (int r = 0; r < 4; r++) acc[r] += (double)diff_unfused(Gk[r], Gi[r]) * wn;
     }
+    rc = rn;
+    w_c = w_n;
+    w_n = w_nn;
+    l_n = l_nn;
   }
   const double d = type == 0 ? 2.0 * delta[i] : delta[i];
   for (int j = 0; j < 4; j++) fres[(size_t)j * n + i] = d * acc[j];
@@ -381,7 +472,7 @@ __global__ void k_kat_fluxes_m(int n, const double *__restrict__ in, double *__r
 
 size_t lds_bytes_qderiv_m(int tile_max) { return 32 * (size_t)tile_max; }
 size_t lds_bytes_qinner_m(int tile_max) { return 64 * (size_t)tile_max; }
-size_t lds_bytes_flux_m(int tile_max) { return 100 * (size_t)tile_max; }
+size_t lds_bytes_flux_m(int tile_max) { return 84 * (size_t)tile_max; }
 
 unsigned prepare_tiled_m(unsigned tiles, int tile_max, size_t lds_limit) {
   struct K { unsigned bit; const void *f; size_t bytes; };
