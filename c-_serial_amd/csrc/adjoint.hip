@@ -316,7 +316,7 @@ __global__ __launch_bounds__(256) void k_argext(int n, const int32_t *__restrict
                                                 const uint32_t *__restrict__ conn,
                                                 const double *__restrict__ base,
                                                 int32_t *__restrict__ arg) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = xcd_block() * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double qi[4], mx[4], mn[4];
   int am[4], an[4];
@@ -325,14 +325,27 @@ __global__ __launch_bounds__(256) void k_argext(int n, const int32_t *__restrict
     mx[r] = mn[r] = qi[r];
     am[r] = an[r] = i;
   }
-  for (int e = off[i]; e < off[i + 1]; e++) {
-    const int k = (int)(conn[e] & kIndexMask);
-    double qk[4];
-    load4(base + (size_t)B_STRIDE * k + B_Q, qk);
+  // neighbour e + 1's q and e + 2's conn entry load while e is compared
+  const int e0 = off[i], last = off[i + 1] - 1;
+  double qk[4] = {0, 0, 0, 0};
+  uint32_t w_c = 0, w_n = 0;
+  if (e0 <= last) {
+    w_c = conn[e0];
+    load4(base + (size_t)B_STRIDE * (w_c & kIndexMask) + B_Q, qk);
+    w_n = conn[min(e0 + 1, last)];
+  }
+  for (int e = e0; e <= last; e++) {
+    double qn[4];
+    load4(base + (size_t)B_STRIDE * (w_n & kIndexMask) + B_Q, qn);
+    const uint32_t w_nn = conn[min(e + 2, last)];
+    const int k = (int)(w_c & kIndexMask);
     for (int r = 0; r < 4; r++) {
       if (qk[r] > mx[r]) { mx[r] = qk[r]; am[r] = k; }
       if (qk[r] < mn[r]) { mn[r] = qk[r]; an[r] = k; }
     }
+    for (int r = 0; r < 4; r++) qk[r] = qn[r];
+    w_c = w_n;
+    w_n = w_nn;
   }
   for (int r = 0; r < 4; r++) {
     arg[8 * (size_t)i + r] = am[r];
@@ -356,9 +369,16 @@ __global__ __launch_bounds__(256) void k_argext(int n, const int32_t *__restrict
 //   geo_n[i] (d/d nx, d/d ny): through dels, deln and the flux frame;
 //   geo_cf[i] (d/d of the eight sub-stencil coefficients);
 //   geo_delta[i] (d/d delta).
-template <bool GEO>
+//
+// TILED stages the block's neighbour records (base, dq, qm: 192 B per tile
+// entry) in LDS first.  The kernel's registers allow one wave per SIMD, one
+// block per CU, so the tile costs no occupancy, and it takes the three
+// dependent gathers of every neighbour off the critical path.
+template <bool GEO, bool TILED>
 __global__ __launch_bounds__(256) void k_flux_vjp(
     int n, const int32_t *__restrict__ off, const uint32_t *__restrict__ conn,
+    const uint16_t *__restrict__ conn16, const int32_t *__restrict__ tile_off,
+    const int32_t *__restrict__ tile_idx,
     const double *__restrict__ base, const double *__restrict__ dq,
     const double *__restrict__ qm, const double *__restrict__ coef,
     const double *__restrict__ nxa, const double *__restrict__ nya,
@@ -367,7 +387,16 @@ __global__ __launch_bounds__(256) void k_flux_vjp(
     double *__restrict__ own_qm, double *__restrict__ edge, const int32_t *__restrict__ tr_pos,
     double *__restrict__ geo_edge, double *__restrict__ geo_n, double *__restrict__ geo_cf,
     double *__restrict__ geo_delta) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  extern __shared__ double lds[];
+  constexpr int S = 24;  // base (8), dq (8), qm (8)
+  const int b = xcd_block();
+  if (TILED) {
+    load_tile<S>(lds, tile_off, tile_idx, b, base, B_STRIDE, 0, 4, 0);
+    load_tile<S>(lds, tile_off, tile_idx, b, dq, DQ_STRIDE, 0, 4, 8);
+    load_tile<S>(lds, tile_off, tile_idx, b, qm, QM_STRIDE, 0, 4, 16);
+    __syncthreads();
+  }
+  const int i = b * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int type = flag[i];
   // flux_res = c delta acc with c = 2 on the wall (flux_residual_mod.h:29)
@@ -379,26 +408,26 @@ __global__ __launch_bounds__(256) void k_flux_vjp(
     lc[r] = cw * lam[(size_t)r * n + i];
   }
   double gcf[8] = {0, 0, 0, 0, 0, 0, 0, 0}, gnx = 0.0, gny = 0.0, gdel = 0.0;
-  const double *hi = base + (size_t)B_STRIDE * i;
+  const double *hi = TILED ? lds + S * threadIdx.x : base + (size_t)B_STRIDE * i;
   const double x_i = hi[B_X], y_i = hi[B_Y], eps2_i = hi[B_EPS2];
   double qi[4], dxi[4], dyi[4], qmxi[4], qmni[4], cf[8];
   load4(hi + B_Q, qi);
-  load8(dq + (size_t)DQ_STRIDE * i, dxi, dyi);
-  load8(qm + (size_t)QM_STRIDE * i, qmxi, qmni);
+  load8(TILED ? hi + 8 : dq + (size_t)DQ_STRIDE * i, dxi, dyi);
+  load8(TILED ? hi + 16 : qm + (size_t)QM_STRIDE * i, qmxi, qmni);
   load8(coef + (size_t)COEF_STRIDE * i, cf, cf + 4);
   const double nx = nxa[i], ny = nya[i], tx = ny, ty = -nx;
   const double s2b = type == 0 ? -1.0 : 1.0;
   double aq[4] = {0, 0, 0, 0}, adx[4] = {0, 0, 0, 0}, ady[4] = {0, 0, 0, 0};
   double amx[4] = {0, 0, 0, 0}, amn[4] = {0, 0, 0, 0};
   for (int e = off[i]; e < off[i + 1]; e++) {
-    const uint32_t w = conn[e];
-    const int k = (int)(w & kIndexMask);
-    const uint32_t m = w >> kMaskShift;
-    const double *hk = base + (size_t)B_STRIDE * k;
+    const uint32_t w = conn_raw(conn, conn16, e, TILED);
+    const int k = conn_index(w, TILED);
+    const uint32_t m = conn_mask(w, TILED);
+    const double *hk = TILED ? lds + S * k : base + (size_t)B_STRIDE * k;
     double qk[4], dxk[4], dyk[4], qmxk[4], qmnk[4];
     load4(hk + B_Q, qk);
-    load8(dq + (size_t)DQ_STRIDE * k, dxk, dyk);
-    load8(qm + (size_t)QM_STRIDE * k, qmxk, qmnk);
+    load8(TILED ? hk + 8 : dq + (size_t)DQ_STRIDE * k, dxk, dyk);
+    load8(TILED ? hk + 16 : qm + (size_t)QM_STRIDE * k, qmxk, qmnk);
     const double delx = hk[B_X] - x_i, dely = hk[B_Y] - y_i;
     const double dels = delx * tx + dely * ty, deln = delx * nx + dely * ny;
     Recon Ri[4], Rk[4];
@@ -487,25 +516,45 @@ __global__ __launch_bounds__(256) void k_flux_vjp_gather(
     const double *__restrict__ edge, const double *__restrict__ own_q,
     const double *__restrict__ own_dq, const double *__restrict__ own_qm,
     double *__restrict__ lq, double *__restrict__ ldq, double *__restrict__ lqm) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = xcd_block() * blockDim.x + threadIdx.x;
   if (j >= n) return;
   double aq[4], adx[4], ady[4], amx[4], amn[4];
   load4(own_q + 4 * (size_t)j, aq);
   load8(own_dq + 8 * (size_t)j, adx, ady);
   load8(own_qm + 8 * (size_t)j, amx, amn);
   const double x_j = base[(size_t)B_STRIDE * j + B_X], y_j = base[(size_t)B_STRIDE * j + B_Y];
-  for (int t = tr_off[j]; t < tr_off[j + 1]; t++) {
-    const int i = tr_owner[t];
+  // record t + 1 with its owner's coordinates, and owner t + 2's index, load
+  // while record t is added
+  const int t0 = tr_off[j], tlast = tr_off[j + 1] - 1;
+  auto fetch = [&](int t, int i, double2 &p, double r16[16]) {
+    p = reinterpret_cast<const double2 *>(base + (size_t)B_STRIDE * i)[0];
     const double *rec = edge + 16 * (size_t)t;
-    const double delx = x_j - base[(size_t)B_STRIDE * i + B_X];
-    const double dely = y_j - base[(size_t)B_STRIDE * i + B_Y];
+    load8(rec, r16, r16 + 4);
+    load8(rec + 8, r16 + 8, r16 + 12);
+  };
+  double2 po = {0.0, 0.0};
+  double rc[16] = {};
+  int i_n = 0;
+  if (t0 <= tlast) {
+    fetch(t0, tr_owner[t0], po, rc);
+    i_n = tr_owner[min(t0 + 1, tlast)];
+  }
+  for (int t = t0; t <= tlast; t++) {
+    double2 po_n;
+    double rn[16];
+    fetch(min(t + 1, tlast), i_n, po_n, rn);
+    i_n = tr_owner[min(t + 2, tlast)];
+    const double delx = x_j - po.x;
+    const double dely = y_j - po.y;
     for (int r = 0; r < 4; r++) {
-      aq[r] += rec[r];
-      adx[r] += rec[4 + r] * delx;
-      ady[r] += rec[4 + r] * dely;
-      amx[r] += rec[8 + r];
-      amn[r] += rec[12 + r];
+      aq[r] += rc[r];
+      adx[r] += rc[4 + r] * delx;
+      ady[r] += rc[4 + r] * dely;
+      amx[r] += rc[8 + r];
+      amn[r] += rc[12 + r];
     }
+    po = po_n;
+    for (int r = 0; r < 16; r++) rc[r] = rn[r];
   }
   store4(lq + 4 * (size_t)j, aq);
   store4(ldq + 8 * (size_t)j, adx);
@@ -536,17 +585,32 @@ __global__ __launch_bounds__(256) void k_ls_adj(int n, const int32_t *__restrict
                                                 const double *__restrict__ out,
                                                 const double *__restrict__ dq_in,
                                                 double *__restrict__ geo_edge) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = xcd_block() * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double x_i = base[(size_t)B_STRIDE * i + B_X], y_i = base[(size_t)B_STRIDE * i + B_Y];
+  const int e0 = off[i], last = off[i + 1] - 1;
+  // neighbour e + 1's coordinates and e + 2's conn entry load while e is used
+  auto xy_of = [&](uint32_t w) {
+    return reinterpret_cast<const double2 *>(base + (size_t)B_STRIDE * (w & kIndexMask))[0];
+  };
   double sxx = 0.0, syy = 0.0, sxy = 0.0;
-  for (int e = off[i]; e < off[i + 1]; e++) {
-    const int k = (int)(conn[e] & kIndexMask);
-    const double delx = base[(size_t)B_STRIDE * k + B_X] - x_i;
-    const double dely = base[(size_t)B_STRIDE * k + B_Y] - y_i;
-    sxx = sxx + delx * delx;
-    syy = syy + dely * dely;
-    sxy = sxy + delx * dely;
+  {
+    double2 pk = {0.0, 0.0};
+    uint32_t w_n = 0;
+    if (e0 <= last) {
+      pk = xy_of(conn[e0]);
+      w_n = conn[min(e0 + 1, last)];
+    }
+    for (int e = e0; e <= last; e++) {
+      const double2 pk_n = xy_of(w_n);
+      w_n = conn[min(e + 2, last)];
+      const double delx = pk.x - x_i;
+      const double dely = pk.y - y_i;
+      sxx = sxx + delx * delx;
+      syy = syy + dely * dely;
+      sxy = sxy + delx * dely;
+      pk = pk_n;
+    }
   }
   const double det = sxx * syy - sxy * sxy;
   const double inv = det != 0.0 ? 1.0 / det : 0.0;
@@ -563,12 +627,30 @@ __global__ __launch_bounds__(256) void k_ls_adj(int n, const int32_t *__restrict
     load4(base + (size_t)B_STRIDE * i + B_Q, qi);
     load8(out + (size_t)DQ_STRIDE * i, px, py);
     if (dq_in) load8(dq_in + (size_t)DQ_STRIDE * i, hxi, hyi);
-    for (int e = off[i]; e < off[i + 1]; e++) {
-      const int k = (int)(conn[e] & kIndexMask);
-      const double delx = base[(size_t)B_STRIDE * k + B_X] - x_i;
-      const double dely = base[(size_t)B_STRIDE * k + B_Y] - y_i;
-      double qk[4], hxk[4] = {0, 0, 0, 0}, hyk[4] = {0, 0, 0, 0};
-      load4(base + (size_t)B_STRIDE * k + B_Q, qk);
+    // neighbour e + 1's base record and e + 2's conn entry load while e is
+    // used (its dq_in is read when it is used)
+    auto rec_of = [&](uint32_t w, double2 &p, double q[4]) {
+      const double *h = base + (size_t)B_STRIDE * (w & kIndexMask);
+      p = reinterpret_cast<const double2 *>(h)[0];
+      load4(h + B_Q, q);
+    };
+    double2 pk = {0.0, 0.0};
+    double qk[4] = {0, 0, 0, 0};
+    uint32_t w_c = 0, w_n = 0;
+    if (e0 <= last) {
+      w_c = conn[e0];
+      rec_of(w_c, pk, qk);
+      w_n = conn[min(e0 + 1, last)];
+    }
+    for (int e = e0; e <= last; e++) {
+      double2 pk_n;
+      double qk_n[4];
+      rec_of(w_n, pk_n, qk_n);
+      const uint32_t w_nn = conn[min(e + 2, last)];
+      const int k = (int)(w_c & kIndexMask);
+      const double delx = pk.x - x_i;
+      const double dely = pk.y - y_i;
+      double hxk[4] = {0, 0, 0, 0}, hyk[4] = {0, 0, 0, 0};
       if (dq_in) load8(dq_in + (size_t)DQ_STRIDE * k, hxk, hyk);
       double gx = 0.0, gy = 0.0;
       for (int r = 0; r < 4; r++) {
@@ -581,6 +663,10 @@ __global__ __launch_bounds__(256) void k_ls_adj(int n, const int32_t *__restrict
       }
       geo_edge[2 * (size_t)e] += gx;
       geo_edge[2 * (size_t)e + 1] += gy;
+      pk = pk_n;
+      for (int r = 0; r < 4; r++) qk[r] = qk_n[r];
+      w_c = w_n;
+      w_n = w_nn;
     }
   }
 }
@@ -597,17 +683,30 @@ __global__ __launch_bounds__(256) void k_ls_adj_gather(
     const double *__restrict__ base, const double *__restrict__ ls, int with_dq,
     double *__restrict__ ldq_in, const int32_t *__restrict__ arg,
     const double *__restrict__ lqm, double *__restrict__ lq) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = xcd_block() * blockDim.x + threadIdx.x;
   if (j >= n) return;
   const double x_j = base[(size_t)B_STRIDE * j + B_X], y_j = base[(size_t)B_STRIDE * j + B_Y];
   double sx[4], sy[4], aq[4], adx[4] = {0, 0, 0, 0}, ady[4] = {0, 0, 0, 0};
   load8(ls + 8 * (size_t)j, sx, sy);
   load4(lq + 4 * (size_t)j, aq);
-  // pairs (j, k) of j's own list
-  for (int e = off[j]; e < off[j + 1]; e++) {
-    const int k = (int)(conn[e] & kIndexMask);
-    const double delx = base[(size_t)B_STRIDE * k + B_X] - x_j;
-    const double dely = base[(size_t)B_STRIDE * k + B_Y] - y_j;
+  // pairs (j, k) of j's own list; neighbour e + 1's coordinates and e + 2's
+  // conn entry load while e is used
+  const int e0 = off[j], last = off[j + 1] - 1;
+  auto xy_of = [&](uint32_t w) {
+    return reinterpret_cast<const double2 *>(base + (size_t)B_STRIDE * (w & kIndexMask))[0];
+  };
+  double2 pk = {0.0, 0.0};
+  uint32_t w_n = 0;
+  if (e0 <= last) {
+    pk = xy_of(conn[e0]);
+    w_n = conn[min(e0 + 1, last)];
+  }
+  for (int e = e0; e <= last; e++) {
+    const double2 pk_n = xy_of(w_n);
+    w_n = conn[min(e + 2, last)];
+    const double delx = pk.x - x_j;
+    const double dely = pk.y - y_j;
+    pk = pk_n;
     for (int r = 0; r < 4; r++) {
       const double t = delx * sx[r] + dely * sy[r];
       aq[r] -= t;
@@ -615,14 +714,39 @@ __global__ __launch_bounds__(256) void k_ls_adj_gather(
       ady[r] += 0.5 * dely * t;
     }
   }
-  // pairs (i, j) in which j is the neighbour
+  // pairs (i, j) in which j is the neighbour; owner t + 1's coordinates and
+  // sums' adjoints and t + 2's index load while t is used
+  const int t0 = tr_off[j], tlast = tr_off[j + 1] - 1;
+  auto own_of = [&](int i, double2 &p, double s8[8]) {
+    p = reinterpret_cast<const double2 *>(base + (size_t)B_STRIDE * i)[0];
+    load8(ls + 8 * (size_t)i, s8, s8 + 4);
+  };
+  double2 po = {0.0, 0.0};
+  double so[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  int i_c = 0, i_n = 0;
+  if (t0 <= tlast) {
+    i_c = tr_owner[t0];
+    own_of(i_c, po, so);
+    i_n = tr_owner[min(t0 + 1, tlast)];
+  }
   int prev = -1;
-  for (int tt = tr_off[j]; tt < tr_off[j + 1]; tt++) {
-    const int i = tr_owner[tt];
-    const double delx = x_j - base[(size_t)B_STRIDE * i + B_X];
-    const double dely = y_j - base[(size_t)B_STRIDE * i + B_Y];
+  for (int tt = t0; tt <= tlast; tt++) {
+    double2 po_n;
+    double so_n[8];
+    own_of(i_n, po_n, so_n);
+    const int i_nn = tr_owner[min(tt + 2, tlast)];
+    const int i = i_c;
+    const double delx = x_j - po.x;
+    const double dely = y_j - po.y;
     double si[4], ti[4];
-    load8(ls + 8 * (size_t)i, si, ti);
+    for (int r = 0; r < 4; r++) {
+      si[r] = so[r];
+      ti[r] = so[4 + r];
+    }
+    po = po_n;
+    for (int r = 0; r < 8; r++) so[r] = so_n[r];
+    i_c = i_n;
+    i_n = i_nn;
     for (int r = 0; r < 4; r++) {
       const double t = delx * si[r] + dely * ti[r];
       aq[r] += t;
@@ -673,7 +797,7 @@ __device__ __forceinline__ void qvar_jac(const double p[4], double J[4][4]) {
 __global__ __launch_bounds__(256) void k_qvar_vjp(int n, const double *__restrict__ prim,
                                                   const double *__restrict__ lq,
                                                   double *__restrict__ grad, int accumulate) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = xcd_block() * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double p[4], J[4][4], a[4];
   for (int c = 0; c < 4; c++) p[c] = prim[(size_t)c * n + i];
@@ -691,7 +815,7 @@ __global__ __launch_bounds__(256) void k_qvar_vjp(int n, const double *__restric
 __global__ __launch_bounds__(256) void k_qvar_jvp(int n, const double *__restrict__ prim,
                                                   const double *__restrict__ v,
                                                   double *__restrict__ tbase) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = xcd_block() * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double p[4], J[4][4], t[4];
   for (int c = 0; c < 4; c++) p[c] = prim[(size_t)c * n + i];
@@ -707,7 +831,7 @@ __global__ __launch_bounds__(256) void k_qvar_jvp(int n, const double *__restric
 __global__ __launch_bounds__(256) void k_ext_jvp(int n, const int32_t *__restrict__ arg,
                                                  const double *__restrict__ tbase,
                                                  double *__restrict__ tqm) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = xcd_block() * blockDim.x + threadIdx.x;
   if (i >= n) return;
   for (int r = 0; r < 8; r++)
     tqm[8 * (size_t)i + r] = tbase[(size_t)B_STRIDE * arg[8 * (size_t)i + r] + B_Q + (r & 3)];
@@ -723,7 +847,7 @@ __global__ __launch_bounds__(256) void k_flux_jvp(
     const uint8_t *__restrict__ flag, const double *__restrict__ delta,
     const double *__restrict__ tbase, const double *__restrict__ tdq,
     const double *__restrict__ tqm, double *__restrict__ out) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = xcd_block() * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int type = flag[i];
   const double *hi = base + (size_t)B_STRIDE * i;
@@ -783,19 +907,43 @@ void launch_argext(const DevCloud &c, const double *base, int32_t *arg, hipStrea
                      base, arg);
 }
 
+size_t lds_bytes_flux_vjp(int tile_max) { return sizeof(double) * 24 * (size_t)tile_max; }
+
+bool prepare_flux_vjp_tiles(const DevCloud &c, size_t lds_limit) {
+  if (!c.tiles || c.tile_max <= 0) return false;
+  const size_t L = lds_bytes_flux_vjp(c.tile_max);
+  if (L > lds_limit) return false;
+  const void *fs[2] = {(const void *)k_flux_vjp<true, true>,
+                       (const void *)k_flux_vjp<false, true>};
+  for (const void *f : fs)
+    if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)L) != hipSuccess)
+      return false;
+  return true;
+}
+
 void launch_flux_vjp(const DevCloud &c, const AdjointBuffers &a, const double *base,
                      const double *dq, const double *qm, const double *delta, const double *lam,
                      const GeoAdjoint *geo, hipStream_t s) {
-  if (geo)
-    hipLaunchKernelGGL(k_flux_vjp<true>, dim3(blocks_of(c.n)), dim3(256), 0, s, c.n, c.conn_off,
-                       c.conn, base, dq, qm, c.coef, c.nx, c.ny, c.flag, delta, lam, a.own_q,
-                       a.own_dq, a.own_qm, a.edge, a.tr_pos, geo->edge, geo->nrm, geo->coef,
-                       geo->delta);
-  else
-    hipLaunchKernelGGL(k_flux_vjp<false>, dim3(blocks_of(c.n)), dim3(256), 0, s, c.n,
-                       c.conn_off, c.conn, base, dq, qm, c.coef, c.nx, c.ny, c.flag, delta, lam,
-                       a.own_q, a.own_dq, a.own_qm, a.edge, a.tr_pos, nullptr, nullptr, nullptr,
-                       nullptr);
+#define VJP_ARGS(G)                                                                       \
+  c.n, c.conn_off, c.conn, c.conn16, c.tile_off, c.tile_idx, base, dq, qm, c.coef, c.nx,  \
+      c.ny, c.flag, delta, lam, a.own_q, a.own_dq, a.own_qm, a.edge, a.tr_pos,            \
+      G ? geo->edge : nullptr, G ? geo->nrm : nullptr, G ? geo->coef : nullptr,           \
+      G ? geo->delta : nullptr
+  if (a.vjp_tiled) {
+    const dim3 g(blocks_for(c.n, c.tile_block)), b(c.tile_block);
+    const size_t L = lds_bytes_flux_vjp(c.tile_max);
+    if (geo)
+      hipLaunchKernelGGL((k_flux_vjp<true, true>), g, b, L, s, VJP_ARGS(true));
+    else
+      hipLaunchKernelGGL((k_flux_vjp<false, true>), g, b, L, s, VJP_ARGS(false));
+  } else if (geo) {
+    hipLaunchKernelGGL((k_flux_vjp<true, false>), dim3(blocks_of(c.n)), dim3(256), 0, s,
+                       VJP_ARGS(true));
+  } else {
+    hipLaunchKernelGGL((k_flux_vjp<false, false>), dim3(blocks_of(c.n)), dim3(256), 0, s,
+                       VJP_ARGS(false));
+  }
+#undef VJP_ARGS
   hipLaunchKernelGGL(k_flux_vjp_gather, dim3(blocks_of(c.n)), dim3(256), 0, s, c.n, a.tr_off,
                      a.tr_owner, base, a.edge, a.own_q, a.own_dq, a.own_qm, a.lq, a.ldq[0],
                      a.lqm);

@@ -801,6 +801,12 @@ int prepare_adjoint(lskum_ctx *c) {
   HIP_TRY(c, hipMemcpy(a.tr_edge, tr_edge.data(), E * sizeof(int32_t), hipMemcpyHostToDevice));
   HIP_TRY(c, hipMemcpy(a.tr_owner, tr_owner.data(), E * sizeof(int32_t), hipMemcpyHostToDevice));
   HIP_TRY(c, hipMemcpy(a.tr_pos, tr_pos.data(), E * sizeof(int32_t), hipMemcpyHostToDevice));
+  int dev = 0, lds_limit = 0;
+  a.vjp_tiled =
+      hipGetDevice(&dev) == hipSuccess &&
+      hipDeviceGetAttribute(&lds_limit, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) ==
+          hipSuccess &&
+      prepare_flux_vjp_tiles(c->dc(), (size_t)lds_limit);
   c->adj_ready = true;
   return LSKUM_OK;
 }

@@ -19,43 +19,6 @@ namespace lskum {
 
 namespace {
 
-// ------------------------------------------------------------ tiles
-// A block's LDS tile holds its own 256 points first (local index =
-// threadIdx.x), then the other points its neighbour lists reach; tiled conn
-// entries carry the local index.  The cooperative load happens before any
-// thread leaves, so the barrier is reached by the whole block.
-template <int STRIDE>
-__device__ __forceinline__ void load_tile(double *lds, const int32_t *__restrict__ tile_off,
-                                          const int32_t *__restrict__ tile_idx, int b,
-                                          const double *__restrict__ src, int src_stride,
-                                          int first_chunk, int nchunks, int dst_offset) {
-  // sixteen 16-byte gathers in flight per thread before their LDS stores
-  // (on the 2.5M-point cloud q_inner takes 0.375 ms with 16, 0.40 with 4)
-  constexpr int U = 16;
-  const int t0 = tile_off[b], total = (tile_off[b + 1] - t0) * nchunks;
-  for (int base = threadIdx.x; base < total; base += U * blockDim.x) {
-    double2 v[U];
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-      // every lane loads (past the end: the last chunk again), so the
-      // array is fully written and stays in registers
-      int t = base + u * blockDim.x;
-      t = t < total ? t : total - 1;
-      const int r = t / nchunks, ch = t - r * nchunks;
-      v[u] = reinterpret_cast<const double2 *>(src + (size_t)src_stride *
-                                                         tile_idx[t0 + r])[first_chunk + ch];
-    }
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-      const int t = base + u * blockDim.x;
-      if (t < total) {
-        const int r = t / nchunks, ch = t - r * nchunks;
-        reinterpret_cast<double2 *>(lds + (size_t)STRIDE * r + dst_offset)[ch] = v[u];
-      }
-    }
-  }
-}
-
 // ------------------------------------------------------------ q variables
 // q_variables_mod.h:9-34
 __global__ __launch_bounds__(256) void k_q_variables(int n, const double *__restrict__ prim,

@@ -148,7 +148,12 @@ struct AdjointBuffers {
   double *own_q, *own_dq, *own_qm, *edge, *lq, *ldq[2], *lqm, *ls;
   int32_t *arg, *tr_off, *tr_edge, *tr_owner, *tr_pos;
   double *tbase, *tdq[2], *tqm, *vec_in, *vec_out;
+  bool vjp_tiled;  // the flux reverse stages neighbour records in LDS
 };
+// LDS bytes of the flux reverse's tile, and whether it can use the cloud's
+// tiles (they exist and fit in lds_limit; sets the kernels' LDS attribute)
+size_t lds_bytes_flux_vjp(int tile_max);
+bool prepare_flux_vjp_tiles(const DevCloud &c, size_t lds_limit);
 // Geometry adjoints the reverse kernels accumulate (+=), device order:
 // edge [n_edges][2] d/d(delx, dely) of each (point, neighbour) pair, nrm
 // [n][2] d/d(nx, ny), coef [n][8] d/d of the sub-stencil coefficients,

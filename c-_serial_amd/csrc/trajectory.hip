@@ -42,7 +42,7 @@ __global__ __launch_bounds__(256) void k_su_vjp(int n, int rk, const double *__r
                                                 double *__restrict__ pold_bar,
                                                 double *__restrict__ rbar,
                                                 double *__restrict__ nrm_bar) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = xcd_block() * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const FlowConsts fc = fcs[0];
   const int type = flag[i];
@@ -100,7 +100,7 @@ __global__ __launch_bounds__(256) void k_delta_vjp(int n, const int32_t *__restr
                                                    int32_t *__restrict__ amin_e,
                                                    double *__restrict__ amin_v,
                                                    double *__restrict__ geo_edge) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = xcd_block() * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double x_i = base[(size_t)B_STRIDE * i + B_X], y_i = base[(size_t)B_STRIDE * i + B_Y];
   double min_delt = 1.0, bx = 0.0, by = 0.0, bd = 1.0, bs = 1.0;
@@ -140,7 +140,7 @@ __global__ __launch_bounds__(256) void k_delta_vjp_gather(
     const int32_t *__restrict__ tr_owner, const int32_t *__restrict__ amin_e,
     const double *__restrict__ amin_v, const double *__restrict__ prim,
     double *__restrict__ pbar) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = xcd_block() * blockDim.x + threadIdx.x;
   if (j >= n) return;
   double sb = 0.0;
   for (int t = tr_off[j]; t < tr_off[j + 1]; t++) {
@@ -173,7 +173,7 @@ __global__ __launch_bounds__(256) void t_normals(int n, const uint8_t *__restric
                                                  const D1 *__restrict__ x,
                                                  const D1 *__restrict__ y, D1 *__restrict__ nx,
                                                  D1 *__restrict__ ny) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = xcd_block() * blockDim.x + threadIdx.x;
   if (i >= n) return;
   if (flag[i] == 1) {
     nx[i] = D1(0.0);
@@ -197,7 +197,7 @@ __global__ __launch_bounds__(256) void t_coef(int n, const int32_t *__restrict__
                                               const D1 *__restrict__ x, const D1 *__restrict__ y,
                                               const D1 *__restrict__ nxa,
                                               const D1 *__restrict__ nya, D1 *__restrict__ coef) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = xcd_block() * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const D1 nx = nxa[i], ny = nya[i], tx = ny, ty = -nx;
   D1 sxx[4], syy[4], sxy[4];
@@ -225,7 +225,7 @@ __global__ __launch_bounds__(256) void t_coef(int n, const int32_t *__restrict__
 // q_variables_mod.h:9-34
 __global__ __launch_bounds__(256) void t_qvar(int n, const D1 *__restrict__ prim,
                                               D1 *__restrict__ q) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = xcd_block() * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const D1 rho = prim[i], u1 = prim[(size_t)n + i], u2 = prim[(size_t)2 * n + i];
   const D1 pr = prim[(size_t)3 * n + i];
@@ -240,7 +240,7 @@ __global__ __launch_bounds__(256) void t_qvar(int n, const D1 *__restrict__ prim
 // state_update_mod.h:483-485
 __global__ __launch_bounds__(256) void t_speed(int n, const D1 *__restrict__ prim,
                                                D1 *__restrict__ speed) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = xcd_block() * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const D1 rho = prim[i], u1 = prim[(size_t)n + i], u2 = prim[(size_t)2 * n + i];
   const D1 pr = prim[(size_t)3 * n + i];
@@ -255,7 +255,7 @@ __global__ __launch_bounds__(256) void t_delta(int n, const int32_t *__restrict_
                                                const D1 *__restrict__ x, const D1 *__restrict__ y,
                                                const D1 *__restrict__ speed, double cfl,
                                                D1 *__restrict__ delta) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = xcd_block() * blockDim.x + threadIdx.x;
   if (i >= n) return;
   D1 min_delt(1.0);
   for (int e = off[i]; e < off[i + 1]; e++) {
@@ -275,7 +275,7 @@ __global__ __launch_bounds__(256) void t_qder(int n, const int32_t *__restrict__
                                               const D1 *__restrict__ x, const D1 *__restrict__ y,
                                               const D1 *__restrict__ q, D1 *__restrict__ dq,
                                               D1 *__restrict__ qm) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = xcd_block() * blockDim.x + threadIdx.x;
   if (i >= n) return;
   D1 qi[4], qmax[4], qmin[4], sdx[4], sdy[4], sxx, syy, sxy;
   for (int r = 0; r < 4; r++) qi[r] = qmax[r] = qmin[r] = q[4 * (size_t)i + r];
@@ -309,7 +309,7 @@ __global__ __launch_bounds__(256) void t_inner(int n, const int32_t *__restrict_
                                                const D1 *__restrict__ x, const D1 *__restrict__ y,
                                                const D1 *__restrict__ q,
                                                const D1 *__restrict__ src, D1 *__restrict__ dst) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = xcd_block() * blockDim.x + threadIdx.x;
   if (i >= n) return;
   D1 sxx, syy, sxy, sx[4], sy[4];
   for (int e = off[i]; e < off[i + 1]; e++) {
@@ -344,7 +344,7 @@ __global__ __launch_bounds__(256) void t_flux(
     const D1 *__restrict__ dq, const D1 *__restrict__ qm, const D1 *__restrict__ coef,
     const D1 *__restrict__ nxa, const D1 *__restrict__ nya, const uint8_t *__restrict__ flag,
     const double *__restrict__ base, const D1 *__restrict__ delta, D1 *__restrict__ fres) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = xcd_block() * blockDim.x + threadIdx.x;
   if (i >= n) return;
   D1 qi[4], dxi[4], dyi[4], qmxi[4], qmni[4], cf[8], acc[4];
   for (int r = 0; r < 4; r++) {
@@ -429,7 +429,7 @@ __global__ __launch_bounds__(256) void t_state_update(int n, int rk, D1 *__restr
                                                       const D1 *__restrict__ nya,
                                                       const uint8_t *__restrict__ flag,
                                                       const FlowConsts *__restrict__ fcs) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = xcd_block() * blockDim.x + threadIdx.x;
   if (i >= n) return;
   D1 p1[4], p2[4], R[4], out[4];
   for (int r = 0; r < 4; r++) {
