@@ -580,11 +580,12 @@ int create_impl(const lskum_cloud *clouds, int32_t n_clouds, const lskum_case *c
       for (int32_t k : extra) loc[k] = -1;
     }
   }
-  // The gather kernels stage neighbours in LDS.  The FP64 flux stages part
-  // of each neighbour record (dq, qmax/qmin) when the tiles of as many
-  // blocks as its registers allow per CU fit in the CU's LDS together;
-  // otherwise the tile would lower its occupancy, and it reads every record
-  // through L2.  The mixed form's FP32 records always fit at its occupancy.
+  // The gather kernels stage neighbours in LDS.  The flux stages part of
+  // each neighbour record (FP64: dq, qmax/qmin; mixed: all but the
+  // coordinates), at most flux_cap entries of a tile: as many as let a CU
+  // hold the blocks the kernel's registers allow (a larger tile's further
+  // entries are read through L2).  Without room for a block's own points it
+  // reads every record through L2.
   int dev = 0, lds_limit = 0, lds_per_cu = 0;
   const bool dev_ok =
       hipGetDevice(&dev) == hipSuccess &&
@@ -592,21 +593,21 @@ int create_impl(const lskum_cloud *clouds, int32_t n_clouds, const lskum_case *c
           hipSuccess &&
       hipDeviceGetAttribute(&lds_per_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor,
                             dev) == hipSuccess;
-  const int flux_blocks = dev_ok ? flux_blocks_per_cu(TB) : 0;
-  const bool flux_fits = flux_blocks > 0 && (size_t)flux_blocks * lds_bytes_flux(c->tile_max) <=
-                                                (size_t)lds_per_cu;
+  if (tiles_ok && dev_ok)
+    c->flux_cap =
+        c->mixed ? flux_tile_cap_m(TB, c->tile_max, (size_t)lds_limit, (size_t)lds_per_cu)
+                 : flux_tile_cap(TB, c->tile_max, (size_t)lds_limit, (size_t)lds_per_cu);
   const unsigned auto_mask = LSKUM_TILE_QDERIV | LSKUM_TILE_QINNER | LSKUM_TILE_DELTA |
-                             (c->mixed || flux_fits ? LSKUM_TILE_FLUX : 0u);
-  const unsigned want =
-      p.lds_tiles == LSKUM_TILES_AUTO ? auto_mask : (unsigned)p.lds_tiles;
-  if (tiles_ok && want) {
-    if (dev_ok) {
-      if (c->mixed)  // func_delta is FP64 in both forms
-        c->tiles = prepare_tiled(want & TILE_DELTA, c->tile_max, (size_t)lds_limit) |
-                   prepare_tiled_m(want & ~TILE_DELTA, c->tile_max, (size_t)lds_limit);
-      else
-        c->tiles = prepare_tiled(want, c->tile_max, (size_t)lds_limit);
-    }
+                             (c->flux_cap > 0 ? LSKUM_TILE_FLUX : 0u);
+  unsigned want = p.lds_tiles == LSKUM_TILES_AUTO ? auto_mask : (unsigned)p.lds_tiles;
+  if (c->flux_cap <= 0) want &= ~(unsigned)LSKUM_TILE_FLUX;
+  if (tiles_ok && want && dev_ok) {
+    if (c->mixed)  // func_delta is FP64 in both forms
+      c->tiles = prepare_tiled(want & TILE_DELTA, c->tile_max, c->flux_cap, (size_t)lds_limit) |
+                 prepare_tiled_m(want & ~TILE_DELTA, c->tile_max, c->flux_cap,
+                                 (size_t)lds_limit);
+    else
+      c->tiles = prepare_tiled(want, c->tile_max, c->flux_cap, (size_t)lds_limit);
   }
   // per-case constants and reduction ranges
   c->part.nblocks = blocks_for(n, 256);
@@ -1147,6 +1148,13 @@ int lskum_get_substencil(lskum_ctx *ctx, int32_t which, int32_t *counts, int32_t
 }
 
 void *lskum_stream(lskum_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+int lskum_flux_tile(lskum_ctx *ctx, int32_t *cap) {
+  CTX_OK(ctx);
+  if (!cap) return set_err(ctx, LSKUM_EINVAL, "null argument");
+  *cap = (ctx->tiles & TILE_FLUX) ? ctx->flux_cap : 0;
+  return LSKUM_OK;
+}
 
 int lskum_layout(lskum_ctx *ctx, int32_t *tiled, int32_t *tile_max, int32_t *order) {
   CTX_OK(ctx);

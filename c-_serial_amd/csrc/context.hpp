@@ -74,6 +74,7 @@ struct lskum_ctx {
   int tile_max = 0;
   unsigned tiles = 0;
   int tile_block = kTileBlockDefault;
+  int flux_cap = 0;  // tile entries the flux stages (0: it reads through L2)
   // cases: flow constants, 256-block -> case map, partial and force ranges
   FlowConsts *fcs = nullptr;
   uint16_t *block_case = nullptr;  // NULL for one case
@@ -122,6 +123,7 @@ struct lskum_ctx {
     c.tile_max = tile_max;
     c.tiles = tiles;
     c.tile_block = tile_block;
+    c.flux_cap = flux_cap;
     c.nx = nx;
     c.ny = ny;
     c.coef = coef;

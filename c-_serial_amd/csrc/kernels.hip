@@ -371,13 +371,20 @@ template <bool TILED, bool POW>
 __global__ __launch_bounds__(256) void k_flux(
     int n, const int32_t *__restrict__ off, const uint32_t *__restrict__ conn,
     const uint16_t *__restrict__ conn16, const int32_t *__restrict__ tile_off,
-    const int32_t *__restrict__ tile_idx, const double *__restrict__ base,
+    const int32_t *__restrict__ tile_idx, int cap, const double *__restrict__ base,
     const double *__restrict__ dq, const double *__restrict__ qm,
     const double *__restrict__ coef, const double *__restrict__ nxa,
     const double *__restrict__ nya, const uint8_t *__restrict__ flag,
     const double *__restrict__ delta, double power, double *__restrict__ fres) {
+  // Tiled, a launch takes the blocks whose tile has at most cap entries;
+  // untiled with cap > 0, the others (their neighbours through L2); untiled
+  // with cap = 0, every block.
   extern __shared__ double2 lds2[];
   const int b = xcd_block();
+  if (TILED || cap > 0) {
+    const int tn = tile_off[b + 1] - tile_off[b];
+    if (TILED ? tn > cap : tn <= cap) return;
+  }
   if (TILED) {
     load_flux_tile(lds2, tile_off, tile_idx, b, dq, qm);
     __syncthreads();
@@ -823,15 +830,24 @@ size_t lds_bytes_flux(int tile_max) {
   return sizeof(double2) * FLUX_TILE_CHUNKS * (size_t)tile_max;
 }
 
-int flux_blocks_per_cu(int block) {
-  int nb = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_flux<false, false>, block, 0) !=
-      hipSuccess)
+int flux_tile_cap(int block, int tile_max, size_t lds_limit, size_t lds_per_cu) {
+  int nb_regs = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb_regs, k_flux<true, false>, block, 0) !=
+          hipSuccess ||
+      nb_regs <= 0)
     return 0;
-  return nb;
+  // per block: the CU's LDS shared by nb_regs blocks, less 2 KB of
+  // allocation granularity, and at most what one block may allocate
+  size_t per_block = lds_per_cu / (size_t)nb_regs;
+  per_block = per_block > 2048 ? per_block - 2048 : 0;
+  if (per_block > lds_limit) per_block = lds_limit;
+  const size_t entry = lds_bytes_flux(1);
+  int cap = (int)(per_block / entry);
+  if (cap > tile_max) cap = tile_max;
+  return cap >= block ? cap : 0;
 }
 
-unsigned prepare_tiled(unsigned tiles, int tile_max, size_t lds_limit) {
+unsigned prepare_tiled(unsigned tiles, int tile_max, int flux_cap, size_t lds_limit) {
   struct K { unsigned bit; const void *f; size_t bytes; };
   const K ks[] = {
       {TILE_QDERIV, (const void *)k_q_derivatives<true, false>, lds_bytes_qderiv(tile_max)},
@@ -839,8 +855,8 @@ unsigned prepare_tiled(unsigned tiles, int tile_max, size_t lds_limit) {
       {TILE_QINNER, (const void *)k_q_inner<true, false>, lds_bytes_qinner(tile_max)},
       {TILE_QINNER, (const void *)k_q_inner<true, true>, lds_bytes_qinner(tile_max)},
       {TILE_DELTA, (const void *)k_func_delta<true>, lds_bytes_delta(tile_max)},
-      {TILE_FLUX, (const void *)k_flux<true, false>, lds_bytes_flux(tile_max)},
-      {TILE_FLUX, (const void *)k_flux<true, true>, lds_bytes_flux(tile_max)}};
+      {TILE_FLUX, (const void *)k_flux<true, false>, lds_bytes_flux(flux_cap)},
+      {TILE_FLUX, (const void *)k_flux<true, true>, lds_bytes_flux(flux_cap)}};
   unsigned ok = tiles;
   for (const K &k : ks) {
     if (!(tiles & k.bit)) continue;
@@ -911,16 +927,22 @@ void launch_flux(const DevCloud &c, const double *base, const double *dq, const 
                  const double *delta, double power, double *flux_res, hipStream_t s) {
   dim3 g(blocks_for(c.n, c.tile_block)), b(c.tile_block);
   const bool t = (c.tiles & TILE_FLUX) != 0;
-  const size_t L = t ? lds_bytes_flux(c.tile_max) : 0;
-#define FLUX_ARGS c.n, TILE_ARGS, base, dq, qm, c.coef, c.nx, c.ny, c.flag, delta, power, flux_res
+  const size_t L = t ? lds_bytes_flux(c.flux_cap) : 0;
+#define FLUX_ARGS \
+  c.n, TILE_ARGS, cap, base, dq, qm, c.coef, c.nx, c.ny, c.flag, delta, power, flux_res
+  // tiled: the blocks whose tile fits the cap, then (if any tile is larger)
+  // the rest through L2
+  const int cap = t ? c.flux_cap : 0;
   if (t && power == 0.0)
     hipLaunchKernelGGL((k_flux<true, false>), g, b, L, s, FLUX_ARGS);
   else if (t)
     hipLaunchKernelGGL((k_flux<true, true>), g, b, L, s, FLUX_ARGS);
-  else if (power == 0.0)
-    hipLaunchKernelGGL((k_flux<false, false>), g, b, 0, s, FLUX_ARGS);
-  else
-    hipLaunchKernelGGL((k_flux<false, true>), g, b, 0, s, FLUX_ARGS);
+  if (!t || c.tile_max > cap) {
+    if (power == 0.0)
+      hipLaunchKernelGGL((k_flux<false, false>), g, b, 0, s, FLUX_ARGS);
+    else
+      hipLaunchKernelGGL((k_flux<false, true>), g, b, 0, s, FLUX_ARGS);
+  }
 #undef FLUX_ARGS
 }
 

@@ -41,6 +41,7 @@ struct DevCloud {
   int tile_max;              // largest tile, entries
   unsigned tiles;            // LSKUM_TILE_* bits of the kernels using them
   int tile_block;            // points per block of the stencil kernels
+  int flux_cap;              // entries of a tile the flux stages (<= tile_max)
   const double *nx, *ny;
   const double *coef;  // per point: least-squares coefficients of the
                        // sub-stencils x+, x-, y+, y- (a, b each)
@@ -67,11 +68,15 @@ size_t lds_bytes_qderiv(int tile_max);
 size_t lds_bytes_qinner(int tile_max);
 size_t lds_bytes_delta(int tile_max);
 size_t lds_bytes_flux(int tile_max);
-// blocks of `block` threads the FP64 flux's registers let one CU hold
-int flux_blocks_per_cu(int block);
+// The number of tile entries the FP64 flux (mixed: flux_tile_cap_m) stages
+// per block so that a CU (lds_per_cu bytes of LDS) still holds as many of
+// its blocks as its registers allow; 0 when not even the block's own points
+// fit.  A block whose tile is larger reads the entries beyond the cap
+// through L2.
+int flux_tile_cap(int block, int tile_max, size_t lds_limit, size_t lds_per_cu);
 // raises the dynamic-LDS limits of the kernels in `tiles`; returns the
 // subset whose tiles fit in lds_limit bytes
-unsigned prepare_tiled(unsigned tiles, int tile_max, size_t lds_limit);
+unsigned prepare_tiled(unsigned tiles, int tile_max, int flux_cap, size_t lds_limit);
 
 void launch_q_variables(const DevCloud &c, const double *prim, double *base, hipStream_t s);
 void launch_speed(const DevCloud &c, const double *prim, double *base, hipStream_t s);
@@ -124,7 +129,8 @@ struct MixedState {
 size_t lds_bytes_qderiv_m(int tile_max);
 size_t lds_bytes_qinner_m(int tile_max);
 size_t lds_bytes_flux_m(int tile_max);
-unsigned prepare_tiled_m(unsigned tiles, int tile_max, size_t lds_limit);
+int flux_tile_cap_m(int block, int tile_max, size_t lds_limit, size_t lds_per_cu);
+unsigned prepare_tiled_m(unsigned tiles, int tile_max, int flux_cap, size_t lds_limit);
 void launch_q_variables_m(const DevCloud &c, const double *prim, float *q32, hipStream_t s);
 void launch_q_derivatives_m(const DevCloud &c, const MixedState &m, float *dq, double power,
                             hipStream_t s);

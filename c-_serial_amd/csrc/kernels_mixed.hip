@@ -281,19 +281,26 @@ template <bool TILED, bool POW>
 __global__ __launch_bounds__(256) void k_flux_m(
     int n, const int32_t *__restrict__ off, const uint32_t *__restrict__ conn,
     const uint16_t *__restrict__ conn16, const int32_t *__restrict__ tile_off,
-    const int32_t *__restrict__ tile_idx, int tile_max, const double2 *__restrict__ xy,
+    const int32_t *__restrict__ tile_idx, int cap, const double2 *__restrict__ xy,
     const float4 *__restrict__ q32, const float4 *__restrict__ dq32,
     const float4 *__restrict__ qm32, const float *__restrict__ eps32,
     const double *__restrict__ coef, const double *__restrict__ nxa,
     const double *__restrict__ nya, const uint8_t *__restrict__ flag,
     const double *__restrict__ delta, double power, double *__restrict__ fres) {
   // LDS tile: q, dq, qm (80 B) and eps2 (4 B) per entry; the FP64
-  // coordinates (16 B) are read through L2.  At a 605-entry tile three
-  // blocks share a CU, the occupancy the kernel's registers allow.
+  // coordinates (16 B) are read through L2.  cap entries (LDS for cap
+  // entries per block) let a CU hold the three blocks the kernel's
+  // registers allow.  Tiled, a launch takes the blocks whose tile has at
+  // most cap entries; untiled with cap > 0, the others; untiled with cap =
+  // 0, every block.
   extern __shared__ uint4 lds4[];
   const float4 *lt = reinterpret_cast<const float4 *>(lds4);
-  float *le = reinterpret_cast<float *>(lds4 + 5 * tile_max);
+  float *le = reinterpret_cast<float *>(lds4 + 5 * cap);
   const int b = xcd_block();
+  if (TILED || cap > 0) {
+    const int tn = tile_off[b + 1] - tile_off[b];
+    if (TILED ? tn > cap : tn <= cap) return;
+  }
   if (TILED) {
     const int t0 = tile_off[b], tn = tile_off[b + 1] - t0;
     const int32_t *idx = tile_idx + t0;
@@ -474,15 +481,32 @@ size_t lds_bytes_qderiv_m(int tile_max) { return 32 * (size_t)tile_max; }
 size_t lds_bytes_qinner_m(int tile_max) { return 64 * (size_t)tile_max; }
 size_t lds_bytes_flux_m(int tile_max) { return 84 * (size_t)tile_max; }
 
-unsigned prepare_tiled_m(unsigned tiles, int tile_max, size_t lds_limit) {
+int flux_tile_cap_m(int block, int tile_max, size_t lds_limit, size_t lds_per_cu) {
+  int nb_regs = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb_regs, k_flux_m<true, false>, block, 0) !=
+          hipSuccess ||
+      nb_regs <= 0)
+    return 0;
+  // per block: the CU's LDS shared by nb_regs blocks, less 2 KB of
+  // allocation granularity, and at most what one block may allocate
+  size_t per_block = lds_per_cu / (size_t)nb_regs;
+  per_block = per_block > 2048 ? per_block - 2048 : 0;
+  if (per_block > lds_limit) per_block = lds_limit;
+  const size_t entry = lds_bytes_flux_m(1);
+  int cap = (int)(per_block / entry);
+  if (cap > tile_max) cap = tile_max;
+  return cap >= block ? cap : 0;
+}
+
+unsigned prepare_tiled_m(unsigned tiles, int tile_max, int flux_cap, size_t lds_limit) {
   struct K { unsigned bit; const void *f; size_t bytes; };
   const K ks[] = {
       {TILE_QDERIV, (const void *)k_q_derivatives_m<true, false>, lds_bytes_qderiv_m(tile_max)},
       {TILE_QDERIV, (const void *)k_q_derivatives_m<true, true>, lds_bytes_qderiv_m(tile_max)},
       {TILE_QINNER, (const void *)k_q_inner_m<true, false>, lds_bytes_qinner_m(tile_max)},
       {TILE_QINNER, (const void *)k_q_inner_m<true, true>, lds_bytes_qinner_m(tile_max)},
-      {TILE_FLUX, (const void *)k_flux_m<true, false>, lds_bytes_flux_m(tile_max)},
-      {TILE_FLUX, (const void *)k_flux_m<true, true>, lds_bytes_flux_m(tile_max)}};
+      {TILE_FLUX, (const void *)k_flux_m<true, false>, lds_bytes_flux_m(flux_cap)},
+      {TILE_FLUX, (const void *)k_flux_m<true, true>, lds_bytes_flux_m(flux_cap)}};
   unsigned ok = tiles;
   for (const K &k : ks) {
     if (!(tiles & k.bit)) continue;
@@ -545,20 +569,26 @@ void launch_flux_m(const DevCloud &c, const MixedState &m, const float *dq, cons
                    double power, double *flux_res, hipStream_t s) {
   dim3 g(blocks_for(c.n, c.tile_block)), b(c.tile_block);
   const bool t = (c.tiles & TILE_FLUX) != 0;
-  const size_t L = t ? lds_bytes_flux_m(c.tile_max) : 0;
+  const size_t L = t ? lds_bytes_flux_m(c.flux_cap) : 0;
+  const int cap = t ? c.flux_cap : 0;
 #define FLUX_ARGS                                                                          \
-  c.n, TILE_ARGS_M, reinterpret_cast<const double2 *>(m.xy),                               \
+  c.n, c.conn_off, c.conn, c.conn16, c.tile_off, c.tile_idx, cap,                          \
+      reinterpret_cast<const double2 *>(m.xy),                                             \
       reinterpret_cast<const float4 *>(m.q32), reinterpret_cast<const float4 *>(dq),       \
       reinterpret_cast<const float4 *>(m.qm32), m.eps32, c.coef, c.nx, c.ny, c.flag, delta, \
       power, flux_res
+  // tiled: the blocks whose tile fits the cap, then (if any tile is larger)
+  // the rest through L2
   if (t && power == 0.0)
     hipLaunchKernelGGL((k_flux_m<true, false>), g, b, L, s, FLUX_ARGS);
   else if (t)
     hipLaunchKernelGGL((k_flux_m<true, true>), g, b, L, s, FLUX_ARGS);
-  else if (power == 0.0)
-    hipLaunchKernelGGL((k_flux_m<false, false>), g, b, 0, s, FLUX_ARGS);
-  else
-    hipLaunchKernelGGL((k_flux_m<false, true>), g, b, 0, s, FLUX_ARGS);
+  if (!t || c.tile_max > cap) {
+    if (power == 0.0)
+      hipLaunchKernelGGL((k_flux_m<false, false>), g, b, 0, s, FLUX_ARGS);
+    else
+      hipLaunchKernelGGL((k_flux_m<false, true>), g, b, 0, s, FLUX_ARGS);
+  }
 #undef FLUX_ARGS
 }
 

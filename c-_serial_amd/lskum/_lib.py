@@ -115,6 +115,7 @@ _SIGS = [
     ("lskum_get_substencil", C.c_int, [C.c_void_p, C.c_int32, _ip, _ip]),
     ("lskum_stream", C.c_void_p, [C.c_void_p]),
     ("lskum_layout", C.c_int, [C.c_void_p, _ip, _ip, _ip]),
+    ("lskum_flux_tile", C.c_int, [C.c_void_p, _ip]),
     ("lskum_set_timing", C.c_int, [C.c_void_p, C.c_int32]),
     ("lskum_kernel_time", C.c_int, [C.c_void_p, C.c_int32, _dp, _ip]),
     ("lskum_flux_vjp", C.c_int, [C.c_void_p, _dp, _dp]),

@@ -347,6 +347,12 @@ class Solver:
         self._chk(L.lib().lskum_layout(self._ctx, C.byref(t), C.byref(m), C.byref(o)))
         return t.value, m.value, o.value
 
+    def flux_tile(self):
+        """Tile entries the flux stages in LDS per block (lskum_flux_tile)."""
+        cap = C.c_int32()
+        self._chk(L.lib().lskum_flux_tile(self._ctx, C.byref(cap)))
+        return cap.value
+
     # ------------------------------------------------------------ timing
     def set_timing(self, enable):
         self._chk(L.lib().lskum_set_timing(self._ctx, 1 if enable else 0))

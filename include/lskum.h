@@ -445,6 +445,12 @@ void *lskum_stream(lskum_ctx *ctx);
  * order = the LSKUM_REORDER_* in use.  Any pointer may be NULL. */
 int lskum_layout(lskum_ctx *ctx, int32_t *tiled, int32_t *tile_max, int32_t *order);
 
+/* Tile entries the flux stages in LDS per block (no reference counterpart):
+ * as many as let a CU hold the blocks the flux kernel's registers allow.  A
+ * block whose tile is larger reads its further neighbours through L2; 0
+ * when the flux reads every neighbour record through L2. */
+int lskum_flux_tile(lskum_ctx *ctx, int32_t *cap);
+
 /* Average duration in ms of the named kernel over the last lskum_iterate
  * call with timing enabled (0 disables).  Kernel ids: LSKUM_K_*. */
 #define LSKUM_K_FLUX 0
