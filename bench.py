@@ -125,14 +125,30 @@ def cpu_baseline(args):
             "cpu": platform.processor() or platform.machine()}
 
 
-def load_traffic(name):
-    """HBM bytes per flux launch measured by rocprofv3 --pmc (FETCH_SIZE and
-    WRITE_SIZE runs, copy-kernel calibrated), if committed under profiles/."""
+def load_traffic(name, key="traffic_bytes_per_launch"):
+    """A figure per flux launch measured by rocprofv3 --pmc and committed
+    under profiles/: HBM bytes (FETCH_SIZE and WRITE_SIZE runs, copy-kernel
+    calibrated) or VALU wave-instructions (SQ_INSTS_VALU)."""
     p = os.path.join(REPO, "profiles", name)
     if os.path.isfile(p):
         with open(p) as f:
-            return json.load(f).get("traffic_bytes_per_launch")
+            return json.load(f).get(key)
     return None
+
+
+# MI355X: 256 CUs x 4 SIMDs, 2.4 GHz peak engine clock; a wave64 VALU
+# instruction occupies a 16-lane SIMD for 4 cycles
+SIMDS, CLOCK_HZ, VALU_CYCLES = 1024, 2.4e9, 4
+
+
+def valu_bound(name, avg_ms):
+    """The flux's VALU issue time at its counted wave-instructions against its
+    measured average duration (the bound the FP64 flux actually sits at)."""
+    insts = load_traffic(name, "valu_wave_instructions_per_launch")
+    if not insts:
+        return None
+    issue_ms = insts * VALU_CYCLES / SIMDS / CLOCK_HZ * 1e3
+    return {"wave_instructions": insts, "issue_ms": issue_ms, "frac": issue_ms / avg_ms}
 
 
 def run_batch(args):
@@ -392,7 +408,8 @@ def main():
                      "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS,
                      "traffic": load_traffic("flux_naca0012_2p5m_k24_traffic.json"),
                      "avg_ms": f["flux_ms"], "launches": f["flux_n"],
-                     "algorithmic_bytes_per_launch": f["flux_bytes"]},
+                     "algorithmic_bytes_per_launch": f["flux_bytes"],
+                     "valu": valu_bound("flux_naca0012_2p5m_k24_traffic.json", f["flux_ms"])},
         "iteration_hbm_GBps": it_bytes * args.steps / f["elapsed"] / 1e9,
         "kernel_avg_ms": f["kernel_avg_ms"],
         "final": {"res_new": float(f["res_new"][-1]), "cl": f["cl"], "cd": f["cd"],
