@@ -53,14 +53,14 @@ void release(lskum_ctx *c) {
                   c->flag, c->conn_off, c->perm_d, c->conn, c->part.sum, c->part.max,
                   c->part.arg, c->red, c->hist, c->f_m, c->f_l, c->f_r, c->f_s, c->forces,
                   c->xy, c->q32, c->dq32[0], c->dq32[1], c->qm32, c->eps32, c->fcs,
-                  c->block_case, c->seg, c->soff, c->qdyn, c->counter};
+                  c->block_case, c->seg, c->soff, c->qdyn, c->counter, c->qsum};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   void *aptrs[] = {c->adj.own_q, c->adj.own_dq, c->adj.own_qm, c->adj.edge, c->adj.lq,
                    c->adj.ldq[0], c->adj.ldq[1], c->adj.lqm, c->adj.ls, c->adj.arg,
                    c->adj.tr_off, c->adj.tr_edge, c->adj.tr_owner, c->adj.tr_pos, c->adj.tbase,
                    c->adj.tdq[0],
-                   c->adj.tdq[1], c->adj.tqm, c->adj.vec_in, c->adj.vec_out};
+                   c->adj.tdq[1], c->adj.tqm, c->adj.tqs, c->adj.vec_in, c->adj.vec_out};
   for (void *p : aptrs)
     if (p) (void)hipFree(p);
   if (c->red_h) (void)hipHostFree(c->red_h);
@@ -143,14 +143,15 @@ int do_q_variables(lskum_ctx *c) {
 }
 
 // derivatives into the buffer from which n_later Jacobi passes end in buffer 0
-int do_q_derivatives(lskum_ctx *c, int n_later) {
+int do_q_derivatives(lskum_ctx *c, int n_later, bool keep_sums) {
   c->cur = n_later % 2;
   {
     KTimer t(c, LSKUM_K_QDERIV);
     if (c->mixed)
       launch_q_derivatives_m(c->dc(), c->ms(), c->dq32[c->cur], c->prm.power, c->stream);
     else
-      launch_q_derivatives(c->dc(), c->base, c->dqb[c->cur], c->qm, c->prm.power, c->stream);
+      launch_q_derivatives(c->dc(), c->base, c->dqb[c->cur], c->qm, keep_sums ? c->qsum : nullptr,
+                           c->prm.power, c->stream);
   }
   return check_launch(c, "q_derivatives");
 }
@@ -163,8 +164,8 @@ int do_q_inner(lskum_ctx *c, int n_inner) {
       launch_q_inner_m(c->dc(), c->ms(), c->dq32[c->cur], c->dq32[1 - c->cur], c->prm.power,
                        c->stream);
     else
-      launch_q_inner(c->dc(), c->base, c->dqb[c->cur], c->dqb[1 - c->cur], c->prm.power,
-                     c->stream);
+      launch_q_inner(c->dc(), c->base, c->qsum, c->dqb[c->cur], c->dqb[1 - c->cur],
+                     c->prm.power, c->stream);
     c->cur = 1 - c->cur;
   }
   return check_launch(c, "q_inner");
@@ -206,7 +207,7 @@ int run_iteration(lskum_ctx *c) {
   int rc = do_snapshot(c);
   if (!rc) rc = do_func_delta(c);
   for (int rk = 1; rk <= 4 && !rc; rk++) {
-    rc = do_q_derivatives(c, inner);
+    rc = do_q_derivatives(c, inner, inner > 0);
     if (!rc) rc = do_q_inner(c, inner);
     if (!rc) rc = do_flux(c);
     if (!rc) rc = do_state_update(c, rk, rk == 4);
@@ -649,6 +650,7 @@ int create_impl(const lskum_cloud *clouds, int32_t n_clouds, const lskum_case *c
     ALLOC(c->dqb[0], (size_t)DQ_STRIDE * n);
     ALLOC(c->dqb[1], (size_t)DQ_STRIDE * n);
     ALLOC(c->qm, (size_t)QM_STRIDE * n);
+    ALLOC(c->qsum, (size_t)8 * n);
   }
   ALLOC(c->prim, (size_t)4 * n);
   ALLOC(c->prim_old, (size_t)4 * n);
@@ -794,6 +796,7 @@ int prepare_adjoint(lskum_ctx *c) {
   AALLOC(a.tdq[0], (size_t)DQ_STRIDE * n);
   AALLOC(a.tdq[1], (size_t)DQ_STRIDE * n);
   AALLOC(a.tqm, (size_t)8 * n);
+  AALLOC(a.tqs, (size_t)8 * n);
   AALLOC(a.vec_in, (size_t)4 * n);
   AALLOC(a.vec_out, (size_t)4 * n);
 #undef AALLOC
@@ -817,7 +820,7 @@ int prepare_adjoint(lskum_ctx *c) {
 int recompute_stages(lskum_ctx *c) {
   const int inner = c->prm.inner_iterations;
   int rc = do_q_variables(c);
-  if (!rc) rc = do_q_derivatives(c, inner);
+  if (!rc) rc = do_q_derivatives(c, inner, inner > 0);
   if (!rc) rc = do_q_inner(c, inner);
   if (rc) return rc;
   launch_argext(c->dc(), c->base, c->adj.arg, c->stream);
@@ -914,7 +917,7 @@ int lskum_q_variables(lskum_ctx *ctx) {
 
 int lskum_q_derivatives(lskum_ctx *ctx) {
   CTX_OK(ctx);
-  return do_q_derivatives(ctx, 0);
+  return do_q_derivatives(ctx, 0, true);
 }
 
 int lskum_q_inner(lskum_ctx *ctx, int32_t n_inner) {
@@ -1224,10 +1227,10 @@ int lskum_flux_jvp(lskum_ctx *ctx, const double *v, double *out) {
   launch_qvar_jvp(dc, ctx->prim, a.vec_in, a.tbase, ctx->stream);
   // the least-squares stencils are linear: the forward kernels map tangents
   int t = inner % 2;
-  launch_q_derivatives(dc, a.tbase, a.tdq[t], a.tqm, 0.0, ctx->stream);
+  launch_q_derivatives(dc, a.tbase, a.tdq[t], a.tqm, a.tqs, 0.0, ctx->stream);
   launch_ext_jvp(dc, a.arg, a.tbase, a.tqm, ctx->stream);
   for (int p = 0; p < inner; p++) {
-    launch_q_inner(dc, a.tbase, a.tdq[t], a.tdq[1 - t], 0.0, ctx->stream);
+    launch_q_inner(dc, a.tbase, a.tqs, a.tdq[t], a.tdq[1 - t], 0.0, ctx->stream);
     t = 1 - t;
   }
   launch_flux_jvp(dc, ctx->base, ctx->dqb[ctx->cur], ctx->qm, ctx->delta, a.tbase, a.tdq[t],

@@ -57,6 +57,7 @@ struct lskum_ctx {
   // device state: base records, derivative buffers (dqb[cur] holds the
   // current derivatives), qmax/qmin, primitive state
   double *base = nullptr, *dqb[2] = {nullptr, nullptr}, *qm = nullptr;
+  double *qsum = nullptr;  // FP64: q_derivatives' q sums, [n][8], for the inner passes
   int cur = 0;
   // mixed-precision form: FP64 coordinates, FP32 gathered data (dq32[cur]
   // holds the current derivatives); dqb/qm are then not allocated
@@ -169,7 +170,9 @@ int do_snapshot(lskum_ctx *c);
 int do_speed(lskum_ctx *c);
 int do_func_delta(lskum_ctx *c);
 int do_q_variables(lskum_ctx *c);
-int do_q_derivatives(lskum_ctx *c, int n_later);
+// keep_sums (FP64 form): also write qsum, which an inner pass that follows
+// reads (the single-stage entry point cannot know, so it always writes them)
+int do_q_derivatives(lskum_ctx *c, int n_later, bool keep_sums);
 int do_q_inner(lskum_ctx *c, int n_inner);
 int do_flux(lskum_ctx *c);
 int do_state_update(lskum_ctx *c, int rk, bool with_hist);

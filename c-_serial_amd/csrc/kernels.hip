@@ -114,7 +114,8 @@ __global__ __launch_bounds__(256) void k_q_derivatives(
     int n, const int32_t *__restrict__ off, const uint32_t *__restrict__ conn,
     const uint16_t *__restrict__ conn16, const int32_t *__restrict__ tile_off,
     const int32_t *__restrict__ tile_idx, const double *__restrict__ base,
-    double *__restrict__ dq_out, double *__restrict__ qm_out, double power) {
+    double *__restrict__ dq_out, double *__restrict__ qm_out, double *__restrict__ qs_out,
+    double power) {
   extern __shared__ double lds[];
   constexpr int S = 6;  // x, y, q[4]
   const int b = xcd_block();
@@ -193,59 +194,67 @@ __global__ __launch_bounds__(256) void k_q_derivatives(
   double *qmw = qm_out + (size_t)QM_STRIDE * i;
   store4(qmw, qmax);
   store4(qmw + 4, qmin);
+  if (qs_out) {  // the q sums the inner passes reuse
+    store4(qs_out + 8 * (size_t)i, sdx);
+    store4(qs_out + 8 * (size_t)i + 4, sdy);
+  }
 }
 
 // ------------------------------------------------------------ inner passes
 // q_variables_mod.h:120-228: one defect-corrected LS pass, Jacobi (reads
-// src, writes dst: the reference's temp array + copy).
+// src, writes dst: the reference's temp array + copy).  Per pair the pass
+// sums w dx ((q_k - q_i) - 0.5 dx.(dq_k - dq_i)) (dx = (delx, dely)); the
+// first part, w dx (q_k - q_i), is q_derivatives' sum, which does not
+// change between passes and is read from qs, so a pass reads only the
+// neighbours' coordinates and derivatives (80 B per tile entry, against
+// 112 B with q, which lets three blocks share a CU instead of two).
 template <bool TILED, bool POW>
 __global__ __launch_bounds__(256) void k_q_inner(
     int n, const int32_t *__restrict__ off, const uint32_t *__restrict__ conn,
     const uint16_t *__restrict__ conn16, const int32_t *__restrict__ tile_off,
     const int32_t *__restrict__ tile_idx, const double *__restrict__ base,
-    const double *__restrict__ src, double *__restrict__ dst, double power) {
+    const double *__restrict__ qs, const double *__restrict__ src, double *__restrict__ dst,
+    double power) {
   extern __shared__ double lds[];
-  constexpr int S = 14;  // x, y, q[4], dqx[4], dqy[4]
+  constexpr int S = 10;  // x, y, dqx[4], dqy[4]
   const int b = xcd_block();
   if (TILED) {
-    load_tile<S>(lds, tile_off, tile_idx, b, base, B_STRIDE, 0, 3, 0);
-    load_tile<S>(lds, tile_off, tile_idx, b, src, DQ_STRIDE, 0, 4, 6);
+    load_tile<S>(lds, tile_off, tile_idx, b, base, B_STRIDE, 0, 1, 0);
+    load_tile<S>(lds, tile_off, tile_idx, b, src, DQ_STRIDE, 0, 4, 2);
     __syncthreads();
   }
   const int i = b * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double *hi = TILED ? lds + S * threadIdx.x : base + (size_t)B_STRIDE * i;
-  const double *di = TILED ? hi + 6 : src + (size_t)DQ_STRIDE * i;
+  const double *di = TILED ? hi + 2 : src + (size_t)DQ_STRIDE * i;
   const double2 xy = reinterpret_cast<const double2 *>(hi)[0];
   const double x_i = xy.x, y_i = xy.y;
-  double qi[4], dxi[4], dyi[4];
-  load4(hi + B_Q, qi);
+  double dxi[4], dyi[4];
   load4(di, dxi);
   load4(di + 4, dyi);
   double sum_delx_sqr = 0., sum_dely_sqr = 0., sum_delx_dely = 0.;
   double sx[4] = {0.0, 0.0, 0.0, 0.0}, sy[4] = {0.0, 0.0, 0.0, 0.0};
   // neighbour e + 1's record and e + 2's conn entry load while e is used
-  auto fetch = [&](uint32_t w, double2 &xyk, double qk[4], double dxk[4], double dyk[4]) {
+  auto fetch = [&](uint32_t w, double2 &xyk, double dxk[4], double dyk[4]) {
     const int k = conn_index(w, TILED);
     const double *hk = TILED ? lds + S * k : base + (size_t)B_STRIDE * k;
-    const double *dk = TILED ? hk + 6 : src + (size_t)DQ_STRIDE * k;
+    const double *dk = TILED ? hk + 2 : src + (size_t)DQ_STRIDE * k;
     xyk = reinterpret_cast<const double2 *>(hk)[0];
-    load4(hk + B_Q, qk);
     load4(dk, dxk);
     load4(dk + 4, dyk);
   };
   const int e0 = off[i], last = off[i + 1] - 1;
   double2 xyk = {0.0, 0.0};
-  double qk[4] = {}, dxk[4] = {}, dyk[4] = {};
+  double dxk[4] = {}, dyk[4] = {};
   uint32_t w_n = 0;
   if (e0 <= last) {
-    fetch(conn_raw(conn, conn16, e0, TILED), xyk, qk, dxk, dyk);
+    fetch(conn_raw(conn, conn16, e0, TILED), xyk, dxk, dyk);
     w_n = conn_raw(conn, conn16, min(e0 + 1, last), TILED);
   }
   for (int e = e0; e <= last; e++) {
     double2 xy_n;
-    double q_n[4], dx_n[4], dy_n[4];
-    fetch(w_n, xy_n, q_n, dx_n, dy_n);
+    double dx_n[4], dy_n[4];
+    fetch(w_n, xy_n, dx_n, dy_n);
     w_n = conn_raw(conn, conn16, min(e + 2, last), TILED);
     const double delx = xyk.x - x_i, dely = xyk.y - y_i;
     const double weights = POW ? pow(sqrt(delx * delx + dely * dely), power) : 1.0;
@@ -259,21 +268,28 @@ __global__ __launch_bounds__(256) void k_q_inner(
       sum_delx_dely = sum_delx_dely + delx * dely;
     }
     for (int r = 0; r < 4; r++) {
-      const double temp1 = qi[r] - 0.5 * (delx * dxi[r] + dely * dyi[r]);
-      const double temp2 = qk[r] - 0.5 * (delx * dxk[r] + dely * dyk[r]);
+      const double c = -0.5 * (delx * (dxk[r] - dxi[r]) + dely * (dyk[r] - dyi[r]));
       if (POW) {
-        sx[r] = sx[r] + (weights * delx * (temp2 - temp1));
-        sy[r] = sy[r] + (weights * dely * (temp2 - temp1));
+        sx[r] = sx[r] + (weights * delx * c);
+        sy[r] = sy[r] + (weights * dely * c);
       } else {
-        sx[r] = sx[r] + (delx * (temp2 - temp1));
-        sy[r] = sy[r] + (dely * (temp2 - temp1));
+        sx[r] = sx[r] + (delx * c);
+        sy[r] = sy[r] + (dely * c);
       }
     }
     xyk = xy_n;
     for (int r = 0; r < 4; r++) {
-      qk[r] = q_n[r];
       dxk[r] = dx_n[r];
       dyk[r] = dy_n[r];
+    }
+  }
+  {
+    double qx[4], qy[4];
+    load4(qs + 8 * (size_t)i, qx);
+    load4(qs + 8 * (size_t)i + 4, qy);
+    for (int r = 0; r < 4; r++) {
+      sx[r] = qx[r] + sx[r];
+      sy[r] = qy[r] + sy[r];
     }
   }
   const double det = sum_delx_sqr * sum_dely_sqr - sum_delx_dely * sum_delx_dely;
@@ -824,7 +840,7 @@ __global__ void k_kat_qtilde(int n, const double *__restrict__ in, double *__res
 int blocks_for(int n, int block) { return (n + block - 1) / block; }
 
 size_t lds_bytes_qderiv(int tile_max) { return sizeof(double) * 6 * (size_t)tile_max; }
-size_t lds_bytes_qinner(int tile_max) { return sizeof(double) * 14 * (size_t)tile_max; }
+size_t lds_bytes_qinner(int tile_max) { return sizeof(double) * 10 * (size_t)tile_max; }
 size_t lds_bytes_delta(int tile_max) { return sizeof(double) * 4 * (size_t)tile_max; }
 size_t lds_bytes_flux(int tile_max) {
   return sizeof(double2) * FLUX_TILE_CHUNKS * (size_t)tile_max;
@@ -890,11 +906,11 @@ void launch_func_delta(const DevCloud &c, const double *base, double cfl, double
 }
 
 void launch_q_derivatives(const DevCloud &c, const double *base, double *dq, double *qm,
-                          double power, hipStream_t s) {
+                          double *qs, double power, hipStream_t s) {
   dim3 g(blocks_for(c.n, c.tile_block)), b(c.tile_block);
   const bool t = (c.tiles & TILE_QDERIV) != 0;
   const size_t L = t ? lds_bytes_qderiv(c.tile_max) : 0;
-#define QD_ARGS c.n, TILE_ARGS, base, dq, qm, power
+#define QD_ARGS c.n, TILE_ARGS, base, dq, qm, qs, power
   if (t && power == 0.0)
     hipLaunchKernelGGL((k_q_derivatives<true, false>), g, b, L, s, QD_ARGS);
   else if (t)
@@ -906,12 +922,12 @@ void launch_q_derivatives(const DevCloud &c, const double *base, double *dq, dou
 #undef QD_ARGS
 }
 
-void launch_q_inner(const DevCloud &c, const double *base, const double *dq_src,
-                    double *dq_dst, double power, hipStream_t s) {
+void launch_q_inner(const DevCloud &c, const double *base, const double *qs,
+                    const double *dq_src, double *dq_dst, double power, hipStream_t s) {
   dim3 g(blocks_for(c.n, c.tile_block)), b(c.tile_block);
   const bool t = (c.tiles & TILE_QINNER) != 0;
   const size_t L = t ? lds_bytes_qinner(c.tile_max) : 0;
-#define QI_ARGS c.n, TILE_ARGS, base, dq_src, dq_dst, power
+#define QI_ARGS c.n, TILE_ARGS, base, qs, dq_src, dq_dst, power
   if (t && power == 0.0)
     hipLaunchKernelGGL((k_q_inner<true, false>), g, b, L, s, QI_ARGS);
   else if (t)

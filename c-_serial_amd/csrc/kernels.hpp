@@ -82,10 +82,12 @@ void launch_q_variables(const DevCloud &c, const double *prim, double *base, hip
 void launch_speed(const DevCloud &c, const double *prim, double *base, hipStream_t s);
 void launch_func_delta(const DevCloud &c, const double *base, double cfl, double *delta,
                        hipStream_t s);
+// q_derivatives also writes qs [n][8]: sum w dx (q_k - q_i) and sum w dy
+// (q_k - q_i), which the inner passes reuse (qs may be null when none run)
 void launch_q_derivatives(const DevCloud &c, const double *base, double *dq, double *qm,
-                          double power, hipStream_t s);
-void launch_q_inner(const DevCloud &c, const double *base, const double *dq_src,
-                    double *dq_dst, double power, hipStream_t s);
+                          double *qs, double power, hipStream_t s);
+void launch_q_inner(const DevCloud &c, const double *base, const double *qs,
+                    const double *dq_src, double *dq_dst, double power, hipStream_t s);
 void launch_flux(const DevCloud &c, const double *base, const double *dq, const double *qm,
                  const double *delta, double power, double *flux_res, hipStream_t s);
 // also writes q of the new state, as FP32 into q32 when non-null (mixed
@@ -154,6 +156,7 @@ struct AdjointBuffers {
   double *own_q, *own_dq, *own_qm, *edge, *lq, *ldq[2], *lqm, *ls;
   int32_t *arg, *tr_off, *tr_edge, *tr_owner, *tr_pos;
   double *tbase, *tdq[2], *tqm, *vec_in, *vec_out;
+  double *tqs;     // tangent of q_derivatives' q sums
   bool vjp_tiled;  // the flux reverse stages neighbour records in LDS
 };
 // LDS bytes of the flux reverse's tile, and whether it can use the cloud's

@@ -89,7 +89,7 @@ int plain_iteration(lskum_ctx *c) {
   int rc = do_snapshot(c);
   if (!rc) rc = do_func_delta(c);
   for (int rk = 1; rk <= 4 && !rc; rk++) {
-    rc = do_q_derivatives(c, inner);
+    rc = do_q_derivatives(c, inner, inner > 0);
     if (!rc) rc = do_q_inner(c, inner);
     if (!rc) rc = do_flux(c);
     if (!rc) rc = do_state_update(c, rk, false);
@@ -298,7 +298,7 @@ int lskum_shape_vjp(lskum_ctx *ctx, int32_t n_iter, const double w[4], double *J
     if (!rc) rc = do_func_delta(ctx);
     for (int rk = 1; rk <= 4 && !rc; rk++) {
       rc = copy(ctx->prim, stage + n4 * (rk - 1), n4);
-      if (!rc) rc = do_q_derivatives(ctx, inner);
+      if (!rc) rc = do_q_derivatives(ctx, inner, inner > 0);
       if (!rc) rc = do_q_inner(ctx, inner);
       if (!rc) rc = do_flux(ctx);
       if (!rc) rc = copy(ctx->flux_res, sres + n4 * (rk - 1), n4);
@@ -312,9 +312,9 @@ int lskum_shape_vjp(lskum_ctx *ctx, int32_t n_iter, const double w[4], double *J
       launch_su_vjp(dc, rk, Ps, P0, sres + n4 * (rk - 1), ctx->fcs, pbar, pold, rbar, g.nrm, s);
       // the stage's q, derivatives of every pass, qmax/qmin and their origins
       launch_q_variables(dc, Ps, ctx->base, s);
-      launch_q_derivatives(dc, ctx->base, dq_of(0), ctx->qm, 0.0, s);
+      launch_q_derivatives(dc, ctx->base, dq_of(0), ctx->qm, ctx->qsum, 0.0, s);
       for (int p = 0; p < inner; p++)
-        launch_q_inner(dc, ctx->base, dq_of(p), dq_of(p + 1), 0.0, s);
+        launch_q_inner(dc, ctx->base, ctx->qsum, dq_of(p), dq_of(p + 1), 0.0, s);
       launch_argext(dc, ctx->base, a.arg, s);
       launch_flux_vjp(dc, a, ctx->base, dq_of(inner), ctx->qm, ctx->delta, rbar, &g, s);
       int b = 0;
