@@ -122,7 +122,34 @@ def cpu_baseline(args):
             kind = "port"
     return {"value": cloud.n * iters / secs, "unit": "point-iterations/s", "cores": 1,
             "kind": kind, "sample": desc, "seconds": round(secs, 3),
-            "cpu": platform.processor() or platform.machine()}
+            "cpu": platform.processor() or platform.machine(), "cpu_model": cpu_model(),
+            "host_cpus": os.cpu_count(),
+            "build": ref_build_flags() if kind == "reference" else "oracle C restatement"}
+
+
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), or None."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def ref_build_flags():
+    """The compiler line the reference's serial solver was built with
+    (RFLAGS of oracle/Makefile)."""
+    try:
+        with open(os.path.join(REPO, "oracle", "Makefile")) as f:
+            for line in f:
+                if line.startswith("RFLAGS"):
+                    return "g++ " + line.split("=", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def load_traffic(name, key="traffic_bytes_per_launch"):
