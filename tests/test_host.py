@@ -394,3 +394,31 @@ def test_cp_file(tmp_path):
     np.testing.assert_array_equal(t[:, 1], c.x[:nw])
     np.testing.assert_allclose(t[:, 2], -(p[:nw] - 1 / 1.4) / (0.5 * M * M), rtol=1e-15, atol=0)
     assert L.lib().lskum_cp_write(str(path).encode(), C.byref(v), L.dptr(prim), 0.0) == -1
+
+
+def test_erf_from_exponential_constants():
+    # physics.hpp's double erf_x, 1 - exp(-s^2) P(t)/Q(t) (t = min(|s|, 6)),
+    # evaluated here in double with the constants read from the source and
+    # compared with the C library's erf: within 6e-16 absolute on [-8, 8]
+    import math
+    import re
+    src = open(os.path.join(REPO, "c-_serial_amd", "csrc", "physics.hpp")).read()
+    body = src[src.index("__device__ __forceinline__ double erf_x("):]
+    body = body[:body.index("\n}\n")]
+    p = [float(x) for x in re.findall(r"p = fma\(p, t, ([0-9.e+-]+)\)", body)]
+    q = [float(x) for x in re.findall(r"q = fma\(q, t, ([0-9.e+-]+)\)", body)]
+    p0 = float(re.search(r"double p = ([0-9.e+-]+);", body).group(1))
+    q0 = float(re.search(r"double q = ([0-9.e+-]+);", body).group(1))
+    assert len(p) == 8 and len(q) == 8
+    worst = 0.0
+    for s in np.linspace(-8.0, 8.0, 40001):
+        t = min(abs(s), 6.0)
+        pp, qq = p0, q0
+        for c in p:
+            pp = pp * t + c
+        for c in q:
+            qq = qq * t + c
+        e = 1.0 - math.exp(-s * s) * (pp / qq)
+        e = -e if s < 0 else e
+        worst = max(worst, abs(e - math.erf(s)))
+    assert worst <= 6e-16, worst
