@@ -308,51 +308,6 @@ __device__ __forceinline__ void load8(const double *p, double a[4], double b[4])
   load4(p + 4, b);
 }
 
-// ------------------------------------------------------------ extremum origin
-// the point each qmax/qmin component came from (q_variables_mod.h:60-80:
-// the point itself, replaced by a neighbour only on a strictly larger /
-// smaller value, in conn order): arg[8i + r] max, arg[8i + 4 + r] min
-__global__ __launch_bounds__(256) void k_argext(int n, const int32_t *__restrict__ off,
-                                                const uint32_t *__restrict__ conn,
-                                                const double *__restrict__ base,
-                                                int32_t *__restrict__ arg) {
-  const int i = xcd_block() * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  double qi[4], mx[4], mn[4];
-  int am[4], an[4];
-  load4(base + (size_t)B_STRIDE * i + B_Q, qi);
-  for (int r = 0; r < 4; r++) {
-    mx[r] = mn[r] = qi[r];
-    am[r] = an[r] = i;
-  }
-  // neighbour e + 1's q and e + 2's conn entry load while e is compared
-  const int e0 = off[i], last = off[i + 1] - 1;
-  double qk[4] = {0, 0, 0, 0};
-  uint32_t w_c = 0, w_n = 0;
-  if (e0 <= last) {
-    w_c = conn[e0];
-    load4(base + (size_t)B_STRIDE * (w_c & kIndexMask) + B_Q, qk);
-    w_n = conn[min(e0 + 1, last)];
-  }
-  for (int e = e0; e <= last; e++) {
-    double qn[4];
-    load4(base + (size_t)B_STRIDE * (w_n & kIndexMask) + B_Q, qn);
-    const uint32_t w_nn = conn[min(e + 2, last)];
-    const int k = (int)(w_c & kIndexMask);
-    for (int r = 0; r < 4; r++) {
-      if (qk[r] > mx[r]) { mx[r] = qk[r]; am[r] = k; }
-      if (qk[r] < mn[r]) { mn[r] = qk[r]; an[r] = k; }
-    }
-    for (int r = 0; r < 4; r++) qk[r] = qn[r];
-    w_c = w_n;
-    w_n = w_nn;
-  }
-  for (int r = 0; r < 4; r++) {
-    arg[8 * (size_t)i + r] = am[r];
-    arg[8 * (size_t)i + 4 + r] = an[r];
-  }
-}
-
 // ------------------------------------------------------------ flux, reverse
 // Per point i, over its neighbours: the adjoint of q~ at both ends of each
 // pair, pushed through the reconstruction.  i's own share accumulates in
@@ -566,7 +521,8 @@ __global__ __launch_bounds__(256) void k_flux_vjp_gather(
 // ------------------------------------------------------------ LS stencils, reverse
 // ls_i = M_i^-T lo_i: the adjoint of the sums (sx, sy) a least-squares pass
 // solves, from the adjoint of its output (q_variables_mod.h:94-112 and
-// :176-210 solve with the same 2x2 matrix of the point's geometry).
+// :176-210 solve with the same 2x2 matrix of the point's geometry, whose
+// sums lsum the recomputed q_derivatives stored).
 //
 // GEO adds, per pair e = (i, k) with rho = (delx, dely), the derivative of
 // sum_r lo_r . out_r with respect to rho (accumulated into geo_edge[e]):
@@ -580,6 +536,7 @@ template <bool GEO>
 __global__ __launch_bounds__(256) void k_ls_adj(int n, const int32_t *__restrict__ off,
                                                 const uint32_t *__restrict__ conn,
                                                 const double *__restrict__ base,
+                                                const double *__restrict__ lsum,
                                                 const double *__restrict__ lo,
                                                 double *__restrict__ ls,
                                                 const double *__restrict__ out,
@@ -589,29 +546,9 @@ __global__ __launch_bounds__(256) void k_ls_adj(int n, const int32_t *__restrict
   if (i >= n) return;
   const double x_i = base[(size_t)B_STRIDE * i + B_X], y_i = base[(size_t)B_STRIDE * i + B_Y];
   const int e0 = off[i], last = off[i + 1] - 1;
-  // neighbour e + 1's coordinates and e + 2's conn entry load while e is used
-  auto xy_of = [&](uint32_t w) {
-    return reinterpret_cast<const double2 *>(base + (size_t)B_STRIDE * (w & kIndexMask))[0];
-  };
-  double sxx = 0.0, syy = 0.0, sxy = 0.0;
-  {
-    double2 pk = {0.0, 0.0};
-    uint32_t w_n = 0;
-    if (e0 <= last) {
-      pk = xy_of(conn[e0]);
-      w_n = conn[min(e0 + 1, last)];
-    }
-    for (int e = e0; e <= last; e++) {
-      const double2 pk_n = xy_of(w_n);
-      w_n = conn[min(e + 2, last)];
-      const double delx = pk.x - x_i;
-      const double dely = pk.y - y_i;
-      sxx = sxx + delx * delx;
-      syy = syy + dely * dely;
-      sxy = sxy + delx * dely;
-      pk = pk_n;
-    }
-  }
+  // the geometry sums the recomputed q_derivatives formed
+  const double sxx = lsum[4 * (size_t)i], syy = lsum[4 * (size_t)i + 1],
+               sxy = lsum[4 * (size_t)i + 2];
   const double det = sxx * syy - sxy * sxy;
   const double inv = det != 0.0 ? 1.0 / det : 0.0;
   double ox[4], oy[4], sx[4], sy[4];
@@ -902,11 +839,6 @@ int blocks_of(int n) { return (n + 255) / 256; }
 
 }  // namespace
 
-void launch_argext(const DevCloud &c, const double *base, int32_t *arg, hipStream_t s) {
-  hipLaunchKernelGGL(k_argext, dim3(blocks_of(c.n)), dim3(256), 0, s, c.n, c.conn_off, c.conn,
-                     base, arg);
-}
-
 size_t lds_bytes_flux_vjp(int tile_max) { return sizeof(double) * 24 * (size_t)tile_max; }
 
 bool prepare_flux_vjp_tiles(const DevCloud &c, size_t lds_limit) {
@@ -954,10 +886,10 @@ void launch_ls_vjp(const DevCloud &c, const AdjointBuffers &a, const double *bas
                    const double *dq_in, double *geo_edge, hipStream_t s) {
   if (geo_edge)
     hipLaunchKernelGGL(k_ls_adj<true>, dim3(blocks_of(c.n)), dim3(256), 0, s, c.n, c.conn_off,
-                       c.conn, base, lo, a.ls, out, dq_in, geo_edge);
+                       c.conn, base, a.lsum, lo, a.ls, out, dq_in, geo_edge);
   else
     hipLaunchKernelGGL(k_ls_adj<false>, dim3(blocks_of(c.n)), dim3(256), 0, s, c.n, c.conn_off,
-                       c.conn, base, lo, a.ls, nullptr, nullptr, nullptr);
+                       c.conn, base, a.lsum, lo, a.ls, nullptr, nullptr, nullptr);
   hipLaunchKernelGGL(k_ls_adj_gather, dim3(blocks_of(c.n)), dim3(256), 0, s, c.n, c.conn_off,
                      c.conn, a.tr_off, a.tr_owner, base, a.ls, extrema ? 0 : 1, ldq_in,
                      extrema ? a.arg : nullptr, a.lqm, a.lq);

@@ -60,7 +60,8 @@ void release(lskum_ctx *c) {
                    c->adj.ldq[0], c->adj.ldq[1], c->adj.lqm, c->adj.ls, c->adj.arg,
                    c->adj.tr_off, c->adj.tr_edge, c->adj.tr_owner, c->adj.tr_pos, c->adj.tbase,
                    c->adj.tdq[0],
-                   c->adj.tdq[1], c->adj.tqm, c->adj.tqs, c->adj.vec_in, c->adj.vec_out};
+                   c->adj.tdq[1], c->adj.tqm, c->adj.tqs, c->adj.lsum, c->adj.vec_in,
+                   c->adj.vec_out};
   for (void *p : aptrs)
     if (p) (void)hipFree(p);
   if (c->red_h) (void)hipHostFree(c->red_h);
@@ -797,6 +798,7 @@ int prepare_adjoint(lskum_ctx *c) {
   AALLOC(a.tdq[1], (size_t)DQ_STRIDE * n);
   AALLOC(a.tqm, (size_t)8 * n);
   AALLOC(a.tqs, (size_t)8 * n);
+  AALLOC(a.lsum, (size_t)4 * n);
   AALLOC(a.vec_in, (size_t)4 * n);
   AALLOC(a.vec_out, (size_t)4 * n);
 #undef AALLOC
@@ -820,11 +822,14 @@ int prepare_adjoint(lskum_ctx *c) {
 int recompute_stages(lskum_ctx *c) {
   const int inner = c->prm.inner_iterations;
   int rc = do_q_variables(c);
-  if (!rc) rc = do_q_derivatives(c, inner, inner > 0);
-  if (!rc) rc = do_q_inner(c, inner);
   if (rc) return rc;
-  launch_argext(c->dc(), c->base, c->adj.arg, c->stream);
-  return check_launch(c, "argext");
+  // q_derivatives with the extrema's origins, into the buffer from which
+  // the inner passes end in buffer 0
+  c->cur = inner % 2;
+  launch_q_derivatives_arg(c->dc(), c->base, c->dqb[c->cur], c->qm, inner > 0 ? c->qsum : nullptr,
+                           c->adj.arg, c->adj.lsum, c->stream);
+  rc = check_launch(c, "q_derivatives");
+  return rc ? rc : do_q_inner(c, inner);
 }
 
 int upload_caller(lskum_ctx *c, const double *src, double *dev) {

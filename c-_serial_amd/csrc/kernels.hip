@@ -109,13 +109,19 @@ __global__ __launch_bounds__(256) void k_func_delta(int n, const int32_t *__rest
 // q_variables_mod.h:36-118: unweighted (power = 0) or pow(dist, power)
 // weighted least squares over the full stencil, and qmax/qmin over the
 // point and its neighbours.
-template <bool TILED, bool POW>
+// ARG (the adjoint's recomputation) also records, per qmax/qmin
+// component, the point the extremum came from (q_variables_mod.h:60-80:
+// the point itself, replaced by a neighbour only on a strictly larger /
+// smaller value, in conn order; device index), which the reverse sweep
+// routes the extrema's adjoints to, and the geometry sums (sxx, syy, sxy)
+// the least-squares reverse needs.
+template <bool TILED, bool POW, bool ARG = false>
 __global__ __launch_bounds__(256) void k_q_derivatives(
     int n, const int32_t *__restrict__ off, const uint32_t *__restrict__ conn,
     const uint16_t *__restrict__ conn16, const int32_t *__restrict__ tile_off,
     const int32_t *__restrict__ tile_idx, const double *__restrict__ base,
     double *__restrict__ dq_out, double *__restrict__ qm_out, double *__restrict__ qs_out,
-    double power) {
+    double power, int32_t *__restrict__ arg = nullptr, double *__restrict__ lsum = nullptr) {
   extern __shared__ double lds[];
   constexpr int S = 6;  // x, y, q[4]
   const int b = xcd_block();
@@ -133,6 +139,8 @@ __global__ __launch_bounds__(256) void k_q_derivatives(
   double sum_delx_sqr = 0., sum_dely_sqr = 0., sum_delx_dely = 0.;
   double sdx[4] = {0.0, 0.0, 0.0, 0.0}, sdy[4] = {0.0, 0.0, 0.0, 0.0};
   for (int r = 0; r < 4; r++) qmax[r] = qmin[r] = qi[r];
+  int am[4] = {i, i, i, i}, an[4] = {i, i, i, i};
+  const int32_t *tix = TILED ? tile_idx + tile_off[b] : nullptr;
   // neighbour e + 1's record and e + 2's conn entry load while e is used
   auto fetch = [&](uint32_t w, double2 &xyk, double qk[4]) {
     const int k = conn_index(w, TILED);
@@ -148,14 +156,26 @@ __global__ __launch_bounds__(256) void k_q_derivatives(
     fetch(conn_raw(conn, conn16, e0, TILED), xyk, qk);
     w_n = conn_raw(conn, conn16, min(e0 + 1, last), TILED);
   }
+  uint32_t w_c = e0 <= last ? conn_raw(conn, conn16, e0, TILED) : 0u;
   for (int e = e0; e <= last; e++) {
     double2 xy_n;
     double q_n[4];
     fetch(w_n, xy_n, q_n);
+    const uint32_t w_k = w_c;
+    w_c = w_n;
     w_n = conn_raw(conn, conn16, min(e + 2, last), TILED);
-    for (int r = 0; r < 4; r++) {
-      if (qk[r] > qmax[r]) qmax[r] = qk[r];
-      if (qk[r] < qmin[r]) qmin[r] = qk[r];
+    if (ARG) {
+      const int kl = conn_index(w_k, TILED);
+      const int kg = TILED ? tix[kl] : kl;
+      for (int r = 0; r < 4; r++) {
+        if (qk[r] > qmax[r]) { qmax[r] = qk[r]; am[r] = kg; }
+        if (qk[r] < qmin[r]) { qmin[r] = qk[r]; an[r] = kg; }
+      }
+    } else {
+      for (int r = 0; r < 4; r++) {
+        if (qk[r] > qmax[r]) qmax[r] = qk[r];
+        if (qk[r] < qmin[r]) qmin[r] = qk[r];
+      }
     }
     const double delx = xyk.x - x_i, dely = xyk.y - y_i;
     if (POW) {
@@ -197,6 +217,15 @@ __global__ __launch_bounds__(256) void k_q_derivatives(
   if (qs_out) {  // the q sums the inner passes reuse
     store4(qs_out + 8 * (size_t)i, sdx);
     store4(qs_out + 8 * (size_t)i + 4, sdy);
+  }
+  if (ARG) {
+    for (int r = 0; r < 4; r++) {
+      arg[8 * (size_t)i + r] = am[r];
+      arg[8 * (size_t)i + 4 + r] = an[r];
+    }
+    lsum[4 * (size_t)i] = sum_delx_sqr;
+    lsum[4 * (size_t)i + 1] = sum_dely_sqr;
+    lsum[4 * (size_t)i + 2] = sum_delx_dely;
   }
 }
 
@@ -868,6 +897,7 @@ unsigned prepare_tiled(unsigned tiles, int tile_max, int flux_cap, size_t lds_li
   const K ks[] = {
       {TILE_QDERIV, (const void *)k_q_derivatives<true, false>, lds_bytes_qderiv(tile_max)},
       {TILE_QDERIV, (const void *)k_q_derivatives<true, true>, lds_bytes_qderiv(tile_max)},
+      {TILE_QDERIV, (const void *)k_q_derivatives<true, false, true>, lds_bytes_qderiv(tile_max)},
       {TILE_QINNER, (const void *)k_q_inner<true, false>, lds_bytes_qinner(tile_max)},
       {TILE_QINNER, (const void *)k_q_inner<true, true>, lds_bytes_qinner(tile_max)},
       {TILE_DELTA, (const void *)k_func_delta<true>, lds_bytes_delta(tile_max)},
@@ -920,6 +950,17 @@ void launch_q_derivatives(const DevCloud &c, const double *base, double *dq, dou
   else
     hipLaunchKernelGGL((k_q_derivatives<false, true>), g, b, 0, s, QD_ARGS);
 #undef QD_ARGS
+}
+
+void launch_q_derivatives_arg(const DevCloud &c, const double *base, double *dq, double *qm,
+                              double *qs, int32_t *arg, double *lsum, hipStream_t s) {
+  dim3 g(blocks_for(c.n, c.tile_block)), b(c.tile_block);
+  if (c.tiles & TILE_QDERIV)
+    hipLaunchKernelGGL((k_q_derivatives<true, false, true>), g, b, lds_bytes_qderiv(c.tile_max),
+                       s, c.n, TILE_ARGS, base, dq, qm, qs, 0.0, arg, lsum);
+  else
+    hipLaunchKernelGGL((k_q_derivatives<false, false, true>), g, b, 0, s, c.n, TILE_ARGS, base,
+                       dq, qm, qs, 0.0, arg, lsum);
 }
 
 void launch_q_inner(const DevCloud &c, const double *base, const double *qs,

@@ -88,6 +88,11 @@ void launch_q_derivatives(const DevCloud &c, const double *base, double *dq, dou
                           double *qs, double power, hipStream_t s);
 void launch_q_inner(const DevCloud &c, const double *base, const double *qs,
                     const double *dq_src, double *dq_dst, double power, hipStream_t s);
+// q_derivatives (power 0) that also writes arg [n][8], per qmax/qmin
+// component the device index of the point it came from, and lsum [n][4],
+// the geometry sums (sxx, syy, sxy) of the least-squares solve
+void launch_q_derivatives_arg(const DevCloud &c, const double *base, double *dq, double *qm,
+                              double *qs, int32_t *arg, double *lsum, hipStream_t s);
 void launch_flux(const DevCloud &c, const double *base, const double *dq, const double *qm,
                  const double *delta, double power, double *flux_res, hipStream_t s);
 // also writes q of the new state, as FP32 into q32 when non-null (mixed
@@ -157,6 +162,7 @@ struct AdjointBuffers {
   int32_t *arg, *tr_off, *tr_edge, *tr_owner, *tr_pos;
   double *tbase, *tdq[2], *tqm, *vec_in, *vec_out;
   double *tqs;     // tangent of q_derivatives' q sums
+  double *lsum;    // per point sxx, syy, sxy of the least-squares solve
   bool vjp_tiled;  // the flux reverse stages neighbour records in LDS
 };
 // LDS bytes of the flux reverse's tile, and whether it can use the cloud's
@@ -170,7 +176,6 @@ bool prepare_flux_vjp_tiles(const DevCloud &c, size_t lds_limit);
 struct GeoAdjoint {
   double *edge, *nrm, *coef, *delta;
 };
-void launch_argext(const DevCloud &c, const double *base, int32_t *arg, hipStream_t s);
 // flux reverse: lam [4][n] (adjoint of flux_res) -> lq, ldq[0], lqm, and
 // with geo non-null the geometry adjoints (+=)
 void launch_flux_vjp(const DevCloud &c, const AdjointBuffers &a, const double *base,
