@@ -1,21 +1,23 @@
 """Writes the golden fixtures of tests/golden/ from the compiled reference.
 
 Run in the build container only (it needs the reference sources to build
-oracle/_ref/ref_driver):  python tests/golden/make_golden.py [--c1]
+oracle/_ref/ref_driver):  python tests/golden/make_golden.py [--long]
 
 Every fixture is data: the clouds come from the project generator (their
 partGrid SHA-256 is stored so a generator change is caught), the outputs
 come from the reference's own functions run by oracle/ref_driver.cpp.
   kat_data.npz       first 1000 rows of the reference's data.dat and, per
                      row, the eight flux functions and qtilde_to_primitive
-  small_c1.npz       naca0012_1480_k20, config-1 numerics (CFL 0.2, K 5, no
-                     inner passes), 40 iterations: history, final arrays,
+  small_m063.npz     naca0012_1480_k20, M 0.63, AoA 2, CFL 0.2, K 5, no inner
+                     passes, 40 iterations: history, final arrays,
                      stage-by-stage flux_res/prim at iteration 30
   k24_m085.npz       naca0012_3620_k24, M 0.85, AoA 1.25, 2 inner passes,
                      20 iterations: history, final prim and flux_res
-  c1_1000.npz        naca0012_9760_k20, config 1, 1000 iterations: history,
-                     final prim and SHA-256 digests of every final array
-  diag_small.npz     naca0012_1480_k20, config-1 numerics, 40 iterations:
+  naca0012_9760_m063_1000it.npz
+                     naca0012_9760_k20 with the same numerics, 1000
+                     iterations: history, final prim and SHA-256 digests of
+                     every final array
+  diag_small.npz     naca0012_1480_k20, the same numerics, 40 iterations:
                      final prim and the reference's diagnostics of it
                      (entropy, vorticity, its square and total enstrophy;
                      the first four values stagnation_pressure() prints)
@@ -71,7 +73,7 @@ def kat():
                         fluxes=res[:, :32].reshape(1000, 8, 4), qtilde_prim=res[:, 32:])
 
 
-def small_c1():
+def small_m063():
     cloud = lskum.workload("naca0012_1480_k20")
     n = cloud.n
     d, gsha, load = run_ref(cloud, 40, 0.2, 5.0, 0, ("--stages", "30"))
@@ -86,7 +88,7 @@ def small_c1():
     stage["rk1_qm"] = load("it30_rk1_qm").reshape(2, 4, n)
     stage["delta30"] = load("it30_delta")
     np.savez_compressed(
-        os.path.join(HERE, "small_c1.npz"), grid_sha=gsha,
+        os.path.join(HERE, "small_m063.npz"), grid_sha=gsha,
         params=np.array([0.63, 2.0, 0.2, 5.0, 0.0, 0]),
         history=load("history").reshape(-1, 5),
         prim=load("it40_prim").reshape(4, n), q=load("it40_q").reshape(4, n),
@@ -111,13 +113,13 @@ def k24():
         sub_members_sha=sha(load("substencil_members").astype(np.int32)))
 
 
-def c1():
+def long_9760():
     cloud = lskum.workload("naca0012_9760_k20")
     n = cloud.n
     d, gsha, load = run_ref(cloud, 1000, 0.2, 5.0, 0)
     final = {nm: load("it1000_" + nm) for nm in ["prim", "q", "dq", "qm", "flux_res", "delta"]}
     np.savez_compressed(
-        os.path.join(HERE, "c1_1000.npz"), grid_sha=gsha,
+        os.path.join(HERE, "naca0012_9760_m063_1000it.npz"), grid_sha=gsha,
         params=np.array([0.63, 2.0, 0.2, 5.0, 0.0, 0]),
         history=load("history").reshape(-1, 5), prim=final["prim"].reshape(4, n),
         **{nm + "_sha": sha(v) for nm, v in final.items()},
@@ -142,13 +144,15 @@ def diag():
 
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
-    ap.add_argument("--c1", action="store_true", help="also the 1000-iteration config-1 run")
+    ap.add_argument("--long", action="store_true",
+                    help="also the 1000-iteration run on the 9,760-point cloud")
     ap.add_argument("--only", default="")
     a = ap.parse_args()
     if not os.path.isfile(DRIVER):
         sys.exit("build the reference harness first: make -C oracle ref")
-    jobs = {"kat": kat, "small": small_c1, "k24": k24, "c1": c1, "diag": diag}
-    todo = a.only.split(",") if a.only else ["kat", "small", "k24", "diag"] + (["c1"] if a.c1 else [])
+    jobs = {"kat": kat, "small": small_m063, "k24": k24, "long": long_9760, "diag": diag}
+    todo = (a.only.split(",") if a.only else
+            ["kat", "small", "k24", "diag"] + (["long"] if a.long else []))
     for j in todo:
         jobs[j]()
         print("wrote", j, flush=True)

@@ -46,7 +46,7 @@ def test_mixed_fluxes_on_reference_known_answers():
 
 
 def test_mixed_stages_against_oracle():
-    g = golden("small_c1.npz")
+    g = golden("small_m063.npz")
     cloud = lskum.workload("naca0012_1480_k20")
     params = (0.63, 2.0, 0.2, 5.0, 0.0, 2)
     print("case: mixed stage errors naca0012_1480_k20", flush=True)
@@ -97,11 +97,11 @@ def test_mixed_free_stream_is_exact():
     assert not np.any(s.flux_res)
 
 
-def test_mixed_config1_1000_iterations_forces():
+def test_mixed_9760_points_1000_iterations_forces():
     # |dCl|, |dCd| against the reference's own FP64 run, every 100 iterations
-    g = golden("c1_1000.npz")
+    g = golden("naca0012_9760_m063_1000it.npz")
     cloud = lskum.workload("naca0012_9760_k20")
-    print("case: mixed config 1, 1000 iterations", flush=True)
+    print("case: mixed naca0012_9760_k20 M0.63, 1000 iterations", flush=True)
     s = _solver(cloud, g["params"])
     h = g["history"]
     rn_all = []

@@ -51,7 +51,7 @@ def test_device_fluxes_match_reference_known_answers():
 
 
 def test_stages_match_oracle_from_a_developed_state():
-    g = golden("small_c1.npz")
+    g = golden("small_m063.npz")
     cloud = lskum.workload("naca0012_1480_k20")
     params = (0.63, 2.0, 0.2, 5.0, 0.0, 2)
     print("case: stage parity naca0012_1480_k20", flush=True)
@@ -96,7 +96,7 @@ def test_stages_match_oracle_from_a_developed_state():
 
 @pytest.mark.parametrize("reorder", [L.REORDER_NONE, L.REORDER_RCM, L.REORDER_HILBERT])
 def test_iterations_match_reference_small(reorder):
-    g = golden("small_c1.npz")
+    g = golden("small_m063.npz")
     cloud = lskum.workload("naca0012_1480_k20")
     print("case: 40 iterations naca0012_1480_k20 reorder", reorder, flush=True)
     s = _solver(cloud, g["params"], reorder=reorder)
@@ -144,10 +144,10 @@ def test_iterations_match_reference_k24_inner_passes():
     assert normwise(s.prim, g["prim"]) <= ITER_TOL
 
 
-def test_config1_1000_iterations_match_reference():
-    g = golden("c1_1000.npz")
+def test_9760_points_1000_iterations_match_reference():
+    g = golden("naca0012_9760_m063_1000it.npz")
     cloud = lskum.workload("naca0012_9760_k20")
-    print("case: config 1, 1000 iterations", flush=True)
+    print("case: naca0012_9760_k20 M0.63, 1000 iterations", flush=True)
     s = _solver(cloud, g["params"])
     h = g["history"]
     rn_all = []

@@ -119,7 +119,7 @@ def _sub_lists(mask, cloud, which):
     return counts, np.concatenate(mem)
 
 
-@pytest.mark.parametrize("name,fixture", [("naca0012_1480_k20", "small_c1.npz"),
+@pytest.mark.parametrize("name,fixture", [("naca0012_1480_k20", "small_m063.npz"),
                                           ("naca0012_3620_k24", "k24_m085.npz")])
 def test_host_substencils_equal_oracle_and_reference(name, fixture):
     cloud = lskum.workload(name)

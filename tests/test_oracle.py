@@ -48,7 +48,7 @@ def test_kat_fluxes_match_reference():
 
 
 def test_small_cloud_stage_by_stage_bit_exact(tmp_path):
-    g = golden("small_c1.npz")
+    g = golden("small_m063.npz")
     cloud = lskum.workload("naca0012_1480_k20")
     assert _grid_sha(cloud, tmp_path) == str(g["grid_sha"])
     o = make_oracle(cloud, g["params"])
@@ -125,8 +125,8 @@ def test_k24_inner_passes_other_mach_bit_exact(tmp_path):
 
 
 @pytest.mark.slow
-def test_config1_1000_iterations_bit_exact(tmp_path):
-    g = golden("c1_1000.npz")
+def test_9760_points_1000_iterations_bit_exact(tmp_path):
+    g = golden("naca0012_9760_m063_1000it.npz")
     cloud = lskum.workload("naca0012_9760_k20")
     assert _grid_sha(cloud, tmp_path) == str(g["grid_sha"])
     o = make_oracle(cloud, g["params"], threads=8)
