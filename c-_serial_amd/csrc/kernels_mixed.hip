@@ -27,62 +27,85 @@ namespace lskum {
 
 namespace {
 
-// Copies one tile's records of CH 16-byte chunks into LDS:
-// dst[r*CH + ch] = src[idx[r]*CH + ch], four loads in flight per thread.
-template <int CH>
-__device__ __forceinline__ void tile_copy16(uint4 *dst, const uint4 *__restrict__ src,
+// One tile's records gathered from up to three arrays in one round of loads
+// per thread: entry r takes C0 16-byte chunks of s0 (s0[idx[r] C0 + c])
+// into d0[r C0 + c], C1 of s1 into d1 and C2 of s2 into d2.  Every load is
+// issued before any LDS store, so a block waits for memory once rather than
+// once per array.
+template <int C0, int C1, int C2>
+__device__ __forceinline__ void tile_gather(uint4 *d0, const uint4 *__restrict__ s0, uint4 *d1,
+                                            const uint4 *__restrict__ s1, uint4 *d2,
+                                            const uint4 *__restrict__ s2,
                                             const int32_t *__restrict__ idx, int count) {
-  constexpr int U = 4;
+  constexpr int CH = C0 + C1 + C2, U = 8;
   const int total = count * CH;
   for (int base = threadIdx.x; base < total; base += U * blockDim.x) {
     uint4 v[U];
 #pragma unroll
     for (int u = 0; u < U; u++) {
-      // every lane loads (past the end: the last chunk again), so the
-      // array is fully written and stays in registers
       int t = base + u * blockDim.x;
       t = t < total ? t : total - 1;
-      const int r = t / CH, ch = t - r * CH;
-      v[u] = src[(size_t)CH * idx[r] + ch];
+      const int r = t / CH, c = t - r * CH;
+      const size_t g = (size_t)idx[r];
+      const uint4 *p = s0 + C0 * g + c;
+      if constexpr (C2 > 0) {
+        if (c >= C0 + C1) p = s2 + C2 * g + (c - C0 - C1);
+      }
+      if (c >= C0 && c < C0 + C1) p = s1 + C1 * g + (c - C0);
+      v[u] = *p;
     }
 #pragma unroll
     for (int u = 0; u < U; u++) {
       const int t = base + u * blockDim.x;
-      if (t < total) dst[t] = v[u];
+      if (t < total) {
+        const int r = t / CH, c = t - r * CH;
+        uint4 *q = d0 + C0 * r + c;
+        if constexpr (C2 > 0) {
+          if (c >= C0 + C1) q = d2 + C2 * r + (c - C0 - C1);
+        }
+        if (c >= C0 && c < C0 + C1) q = d1 + C1 * r + (c - C0);
+        *q = v[u];
+      }
     }
   }
 }
 
 // The flux's tile: per entry q, dq/dx, dq/dy, qmax, qmin as five 16-byte
-// chunks (dst[5r + ch]).
-__device__ __forceinline__ void tile_copy_flux(uint4 *dst, const uint4 *__restrict__ q,
+// chunks (dst[5r + ch]) and eps2 (eps[r]), in one round of loads.
+__device__ __forceinline__ void tile_copy_flux(uint4 *dst, float *eps, const uint4 *__restrict__ q,
                                                const uint4 *__restrict__ dq,
                                                const uint4 *__restrict__ qm,
+                                               const float *__restrict__ eps32,
                                                const int32_t *__restrict__ idx, int count) {
-  constexpr int U = 4;
-  const int total = count * 5;
+  constexpr int U = 12;  // a 605-entry tile's 3,630 loads in one round
+  const int total = count * 6;
   for (int base = threadIdx.x; base < total; base += U * blockDim.x) {
     uint4 v[U];
 #pragma unroll
     for (int u = 0; u < U; u++) {
       int t = base + u * blockDim.x;
       t = t < total ? t : total - 1;
-      const int r = t / 5, ch = t - 5 * r;
+      const int r = t / 6, ch = t - 6 * r;
       const size_t g = (size_t)idx[r];
-      v[u] = ch == 0 ? q[g] : ch < 3 ? dq[2 * g + (ch - 1)] : qm[2 * g + (ch - 3)];
+      if (ch == 5)
+        v[u] = make_uint4(__float_as_uint(eps32[g]), 0u, 0u, 0u);
+      else
+        v[u] = ch == 0 ? q[g] : ch < 3 ? dq[2 * g + (ch - 1)] : qm[2 * g + (ch - 3)];
     }
 #pragma unroll
     for (int u = 0; u < U; u++) {
       const int t = base + u * blockDim.x;
-      if (t < total) dst[t] = v[u];
+      if (t < total) {
+        const int r = t / 6, ch = t - 6 * r;
+        if (ch == 5)
+          eps[r] = __uint_as_float(v[u].x);
+        else
+          dst[5 * r + ch] = v[u];
+      }
     }
   }
 }
 
-__device__ __forceinline__ void tile_copy4(float *dst, const float *__restrict__ src,
-                                           const int32_t *__restrict__ idx, int count) {
-  for (int t = threadIdx.x; t < count; t += blockDim.x) dst[t] = src[idx[t]];
-}
 
 // ------------------------------------------------------------ q variables
 // q_variables_mod.h:9-34 in FP64, rounded once to FP32
@@ -115,8 +138,9 @@ __global__ __launch_bounds__(256) void k_q_derivatives_m(
   const int b = xcd_block();
   if (TILED) {
     const int t0 = tile_off[b], tn = tile_off[b + 1] - t0;
-    tile_copy16<1>(lds4, reinterpret_cast<const uint4 *>(xy), tile_idx + t0, tn);
-    tile_copy16<1>(lds4 + tile_max, reinterpret_cast<const uint4 *>(q32), tile_idx + t0, tn);
+    tile_gather<1, 1, 0>(lds4, reinterpret_cast<const uint4 *>(xy), lds4 + tile_max,
+                         reinterpret_cast<const uint4 *>(q32), nullptr, nullptr,
+                         tile_idx + t0, tn);
     __syncthreads();
   }
   const int i = b * blockDim.x + threadIdx.x;
@@ -198,9 +222,9 @@ __global__ __launch_bounds__(256) void k_q_inner_m(
   const int b = xcd_block();
   if (TILED) {
     const int t0 = tile_off[b], tn = tile_off[b + 1] - t0;
-    tile_copy16<1>(lds4, reinterpret_cast<const uint4 *>(xy), tile_idx + t0, tn);
-    tile_copy16<1>(lds4 + tile_max, reinterpret_cast<const uint4 *>(q32), tile_idx + t0, tn);
-    tile_copy16<2>(lds4 + 2 * tile_max, reinterpret_cast<const uint4 *>(src), tile_idx + t0, tn);
+    tile_gather<1, 1, 2>(lds4, reinterpret_cast<const uint4 *>(xy), lds4 + tile_max,
+                         reinterpret_cast<const uint4 *>(q32), lds4 + 2 * tile_max,
+                         reinterpret_cast<const uint4 *>(src), tile_idx + t0, tn);
     __syncthreads();
   }
   const int i = b * blockDim.x + threadIdx.x;
@@ -304,10 +328,9 @@ __global__ __launch_bounds__(256) void k_flux_m(
   if (TILED) {
     const int t0 = tile_off[b], tn = tile_off[b + 1] - t0;
     const int32_t *idx = tile_idx + t0;
-    tile_copy_flux(lds4, reinterpret_cast<const uint4 *>(q32),
+    tile_copy_flux(lds4, le, reinterpret_cast<const uint4 *>(q32),
                    reinterpret_cast<const uint4 *>(dq32), reinterpret_cast<const uint4 *>(qm32),
-                   idx, tn);
-    tile_copy4(le, eps32, idx, tn);
+                   eps32, idx, tn);
     __syncthreads();
   }
   const int i = b * blockDim.x + threadIdx.x;
