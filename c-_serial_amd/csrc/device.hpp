@@ -66,14 +66,14 @@ __device__ __forceinline__ int nbr_index(const uint32_t *conn, const uint16_t *c
 // threadIdx.x), then the other points its neighbour lists reach; tiled conn
 // entries carry the local index.  The cooperative load happens before any
 // thread leaves, so the barrier is reached by the whole block.
-template <int STRIDE>
+// U 16-byte gathers are in flight per thread before their LDS stores; a
+// caller sizes U so that one round covers the tiles it expects (on the
+// 2.5M-point cloud q_inner took 0.375 ms with 16, 0.40 with 4).
+template <int STRIDE, int U = 16>
 __device__ __forceinline__ void load_tile(double *lds, const int32_t *__restrict__ tile_off,
                                           const int32_t *__restrict__ tile_idx, int b,
                                           const double *__restrict__ src, int src_stride,
                                           int first_chunk, int nchunks, int dst_offset) {
-  // sixteen 16-byte gathers in flight per thread before their LDS stores
-  // (on the 2.5M-point cloud q_inner takes 0.375 ms with 16, 0.40 with 4)
-  constexpr int U = 16;
   const int t0 = tile_off[b], total = (tile_off[b + 1] - t0) * nchunks;
   for (int base = threadIdx.x; base < total; base += U * blockDim.x) {
     double2 v[U];

@@ -126,7 +126,7 @@ __global__ __launch_bounds__(256) void k_q_derivatives(
   constexpr int S = 6;  // x, y, q[4]
   const int b = xcd_block();
   if (TILED) {
-    load_tile<S>(lds, tile_off, tile_idx, b, base, B_STRIDE, 0, 3, 0);
+    load_tile<S, 8>(lds, tile_off, tile_idx, b, base, B_STRIDE, 0, 3, 0);  // 682 entries a round
     __syncthreads();
   }
   const int i = b * blockDim.x + threadIdx.x;
@@ -248,8 +248,10 @@ __global__ __launch_bounds__(256) void k_q_inner(
   constexpr int S = 10;  // x, y, dqx[4], dqy[4]
   const int b = xcd_block();
   if (TILED) {
-    load_tile<S>(lds, tile_off, tile_idx, b, base, B_STRIDE, 0, 1, 0);
-    load_tile<S>(lds, tile_off, tile_idx, b, src, DQ_STRIDE, 0, 4, 2);
+    // one round each for tiles of up to 768 entries (3 and 10 chunks per
+    // thread at 256 threads)
+    load_tile<S, 3>(lds, tile_off, tile_idx, b, base, B_STRIDE, 0, 1, 0);
+    load_tile<S, 10>(lds, tile_off, tile_idx, b, src, DQ_STRIDE, 0, 4, 2);
     __syncthreads();
   }
   const int i = b * blockDim.x + threadIdx.x;
