@@ -165,8 +165,8 @@ int do_q_inner(lskum_ctx *c, int n_inner) {
       launch_q_inner_m(c->dc(), c->ms(), c->dq32[c->cur], c->dq32[1 - c->cur], c->prm.power,
                        c->stream);
     else
-      launch_q_inner(c->dc(), c->base, c->qsum, c->dqb[c->cur], c->dqb[1 - c->cur],
-                     c->prm.power, c->stream);
+      launch_q_inner(c->dc(), c->qsum, c->dqb[c->cur], c->dqb[1 - c->cur], c->prm.power,
+                     c->stream);
     c->cur = 1 - c->cur;
   }
   return check_launch(c, "q_inner");
@@ -640,8 +640,8 @@ int create_impl(const lskum_cloud *clouds, int32_t n_clouds, const lskum_case *c
   if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess)
     return fail(LSKUM_EHIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
   ALLOC(c->base, (size_t)B_STRIDE * n);
+  ALLOC(c->xy, (size_t)2 * n);
   if (c->mixed) {
-    ALLOC(c->xy, (size_t)2 * n);
     ALLOC(c->q32, (size_t)4 * n);
     ALLOC(c->dq32[0], (size_t)8 * n);
     ALLOC(c->dq32[1], (size_t)8 * n);
@@ -712,17 +712,16 @@ int create_impl(const lskum_cloud *clouds, int32_t n_clouds, const lskum_case *c
     ups.push_back({c->seg, seg_h.data(), seg_h.size() * sizeof(int32_t)});
     ups.push_back({c->block_case, bcase_h.data(), bcase_h.size() * sizeof(uint16_t)});
   }
-  std::vector<double> xy_h;
+  std::vector<double> xy_h((size_t)2 * n);
+  for (int32_t d = 0; d < n; d++) {
+    xy_h[2 * (size_t)d] = base_h[(size_t)B_STRIDE * d + B_X];
+    xy_h[2 * (size_t)d + 1] = base_h[(size_t)B_STRIDE * d + B_Y];
+  }
+  ups.push_back({c->xy, xy_h.data(), xy_h.size() * sizeof(double)});
   std::vector<float> eps_h;
   if (c->mixed) {
-    xy_h.resize((size_t)2 * n);
     eps_h.resize(n);
-    for (int32_t d = 0; d < n; d++) {
-      xy_h[2 * (size_t)d] = base_h[(size_t)B_STRIDE * d + B_X];
-      xy_h[2 * (size_t)d + 1] = base_h[(size_t)B_STRIDE * d + B_Y];
-      eps_h[d] = (float)base_h[(size_t)B_STRIDE * d + B_EPS2];
-    }
-    ups.push_back({c->xy, xy_h.data(), xy_h.size() * sizeof(double)});
+    for (int32_t d = 0; d < n; d++) eps_h[d] = (float)base_h[(size_t)B_STRIDE * d + B_EPS2];
     ups.push_back({c->eps32, eps_h.data(), eps_h.size() * sizeof(float)});
   }
   if (c->tiles) {
@@ -1235,7 +1234,7 @@ int lskum_flux_jvp(lskum_ctx *ctx, const double *v, double *out) {
   launch_q_derivatives(dc, a.tbase, a.tdq[t], a.tqm, a.tqs, 0.0, ctx->stream);
   launch_ext_jvp(dc, a.arg, a.tbase, a.tqm, ctx->stream);
   for (int p = 0; p < inner; p++) {
-    launch_q_inner(dc, a.tbase, a.tqs, a.tdq[t], a.tdq[1 - t], 0.0, ctx->stream);
+    launch_q_inner(dc, a.tqs, a.tdq[t], a.tdq[1 - t], 0.0, ctx->stream);
     t = 1 - t;
   }
   launch_flux_jvp(dc, ctx->base, ctx->dqb[ctx->cur], ctx->qm, ctx->delta, a.tbase, a.tdq[t],

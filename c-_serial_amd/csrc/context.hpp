@@ -125,6 +125,7 @@ struct lskum_ctx {
     c.tiles = tiles;
     c.tile_block = tile_block;
     c.flux_cap = flux_cap;
+    c.xy = xy;
     c.nx = nx;
     c.ny = ny;
     c.coef = coef;

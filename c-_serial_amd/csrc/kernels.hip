@@ -241,7 +241,7 @@ template <bool TILED, bool POW>
 __global__ __launch_bounds__(256) void k_q_inner(
     int n, const int32_t *__restrict__ off, const uint32_t *__restrict__ conn,
     const uint16_t *__restrict__ conn16, const int32_t *__restrict__ tile_off,
-    const int32_t *__restrict__ tile_idx, const double *__restrict__ base,
+    const int32_t *__restrict__ tile_idx, const double *__restrict__ xy,
     const double *__restrict__ qs, const double *__restrict__ src, double *__restrict__ dst,
     double power) {
   extern __shared__ double lds[];
@@ -249,17 +249,18 @@ __global__ __launch_bounds__(256) void k_q_inner(
   const int b = xcd_block();
   if (TILED) {
     // one round each for tiles of up to 768 entries (3 and 10 chunks per
-    // thread at 256 threads)
-    load_tile<S, 3>(lds, tile_off, tile_idx, b, base, B_STRIDE, 0, 1, 0);
+    // thread at 256 threads); the coordinates come from the compact xy
+    // array, four points to a 64-byte line
+    load_tile<S, 3>(lds, tile_off, tile_idx, b, xy, 2, 0, 1, 0);
     load_tile<S, 10>(lds, tile_off, tile_idx, b, src, DQ_STRIDE, 0, 4, 2);
     __syncthreads();
   }
   const int i = b * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const double *hi = TILED ? lds + S * threadIdx.x : base + (size_t)B_STRIDE * i;
+  const double *hi = TILED ? lds + S * threadIdx.x : xy + 2 * (size_t)i;
   const double *di = TILED ? hi + 2 : src + (size_t)DQ_STRIDE * i;
-  const double2 xy = reinterpret_cast<const double2 *>(hi)[0];
-  const double x_i = xy.x, y_i = xy.y;
+  const double2 pi = reinterpret_cast<const double2 *>(hi)[0];
+  const double x_i = pi.x, y_i = pi.y;
   double dxi[4], dyi[4];
   load4(di, dxi);
   load4(di + 4, dyi);
@@ -268,7 +269,7 @@ __global__ __launch_bounds__(256) void k_q_inner(
   // neighbour e + 1's record and e + 2's conn entry load while e is used
   auto fetch = [&](uint32_t w, double2 &xyk, double dxk[4], double dyk[4]) {
     const int k = conn_index(w, TILED);
-    const double *hk = TILED ? lds + S * k : base + (size_t)B_STRIDE * k;
+    const double *hk = TILED ? lds + S * k : xy + 2 * (size_t)k;
     const double *dk = TILED ? hk + 2 : src + (size_t)DQ_STRIDE * k;
     xyk = reinterpret_cast<const double2 *>(hk)[0];
     load4(dk, dxk);
@@ -858,12 +859,13 @@ __global__ void k_kat_qtilde(int n, const double *__restrict__ in, double *__res
   if (i >= n) return;
   double qt[4];
   for (int r = 0; r < 4; r++) qt[r] = in[4 * (size_t)i + r];
-  double u1, u2, rho, pr;
-  qtilde_to_primitive(qt, u1, u2, rho, pr);
+  // the form the flux kernels use
+  double u1, u2, rho, sb, temp;
+  qtilde_state(qt, u1, u2, rho, sb, temp);
   out[4 * (size_t)i + 0] = u1;
   out[4 * (size_t)i + 1] = u2;
   out[4 * (size_t)i + 2] = rho;
-  out[4 * (size_t)i + 3] = pr;
+  out[4 * (size_t)i + 3] = rho * temp;
 }
 
 }  // namespace
@@ -965,12 +967,12 @@ void launch_q_derivatives_arg(const DevCloud &c, const double *base, double *dq,
                        dq, qm, qs, 0.0, arg, lsum);
 }
 
-void launch_q_inner(const DevCloud &c, const double *base, const double *qs,
-                    const double *dq_src, double *dq_dst, double power, hipStream_t s) {
+void launch_q_inner(const DevCloud &c, const double *qs, const double *dq_src, double *dq_dst,
+                    double power, hipStream_t s) {
   dim3 g(blocks_for(c.n, c.tile_block)), b(c.tile_block);
   const bool t = (c.tiles & TILE_QINNER) != 0;
   const size_t L = t ? lds_bytes_qinner(c.tile_max) : 0;
-#define QI_ARGS c.n, TILE_ARGS, base, qs, dq_src, dq_dst, power
+#define QI_ARGS c.n, TILE_ARGS, c.xy, qs, dq_src, dq_dst, power
   if (t && power == 0.0)
     hipLaunchKernelGGL((k_q_inner<true, false>), g, b, L, s, QI_ARGS);
   else if (t)

@@ -42,6 +42,7 @@ struct DevCloud {
   unsigned tiles;            // LSKUM_TILE_* bits of the kernels using them
   int tile_block;            // points per block of the stencil kernels
   int flux_cap;              // entries of a tile the flux stages (<= tile_max)
+  const double *xy;   // [n][2] coordinates (also in the base records)
   const double *nx, *ny;
   const double *coef;  // per point: least-squares coefficients of the
                        // sub-stencils x+, x-, y+, y- (a, b each)
@@ -86,8 +87,8 @@ void launch_func_delta(const DevCloud &c, const double *base, double cfl, double
 // (q_k - q_i), which the inner passes reuse (qs may be null when none run)
 void launch_q_derivatives(const DevCloud &c, const double *base, double *dq, double *qm,
                           double *qs, double power, hipStream_t s);
-void launch_q_inner(const DevCloud &c, const double *base, const double *qs,
-                    const double *dq_src, double *dq_dst, double power, hipStream_t s);
+void launch_q_inner(const DevCloud &c, const double *qs, const double *dq_src, double *dq_dst,
+                    double power, hipStream_t s);
 // q_derivatives (power 0) that also writes arg [n][8], per qmax/qmin
 // component the device index of the point it came from, and lsum [n][4],
 // the geometry sums (sxx, syy, sxy) of the least-squares solve

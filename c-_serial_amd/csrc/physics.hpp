@@ -49,22 +49,43 @@ __device__ __forceinline__ float qdiv(float a, float b) {
   return fmaf(fmaf(-b, q, a), r, q);
 }
 
-// q_variables_mod.h:230-253 (output order u1, u2, rho, pr as there)
+// a / b without qdiv's last correction: the reciprocal after its two Newton
+// steps times a, within two ulps.  For quotients whose last bits do not
+// matter downstream (the limiter ratio, erf's rational).
 template <typename T>
-__device__ __forceinline__ void qtilde_to_primitive(const T qt[4], T &u1, T &u2,
-                                                    T &rho, T &pr) {
-  const T beta = -qt[3] * T(0.5);
-  const T temp = T(0.5) / beta;
-  u1 = qt[1] * temp;
-  u2 = qt[2] * temp;
-  const T temp1 = qt[0] + beta * (u1 * u1 + u2 * u2);
-  const T temp2 = temp1 - (log(beta) / T(kGammaM1));
-  rho = exp(temp2);
-  pr = rho * temp;
+__device__ __forceinline__ T qdiv_loose(T a, T b) {
+  return qdiv(a, b);
+}
+template <>
+__device__ __forceinline__ double qdiv_loose(double a, double b) {
+  double r = __builtin_amdgcn_rcp(b);
+  r = fma(fma(-b, r, 1.0), r, r);
+  r = fma(fma(-b, r, 1.0), r, r);
+  return a * r;
+}
+template <>
+__device__ __forceinline__ float qdiv_loose(float a, float b) {
+  return a * __builtin_amdgcn_rcpf(b);
 }
 
-// qtilde_to_primitive that also returns sqrt(beta) and temp = 0.5/beta =
-// p/rho, which the fluxes need again (split_fluxes_mod.h:20-25 recompute
+// 1/sqrt(b) for normal b > 0: the hardware estimate and two Newton steps
+// y <- y + y (1/2 - (b/2) y^2), each of which squares the relative error of
+// the last; within an ulp or two.
+__device__ __forceinline__ double rsqrt_nr(double b) {
+  double y = __builtin_amdgcn_rsq(b);
+  const double hb = 0.5 * b;
+  y = fma(y, fma(-hb * y, y, 0.5), y);
+  y = fma(y, fma(-hb * y, y, 0.5), y);
+  return y;
+}
+// In FP32 the estimate is within an ulp or so; one step rounds it off.
+__device__ __forceinline__ float rsqrt_nr(float b) {
+  float y = __builtin_amdgcn_rsqf(b);
+  return fmaf(y, fmaf(-0.5f * b * y, y, 0.5f), y);
+}
+
+// qtilde_to_primitive (q_variables_mod.h:230-253) that also returns
+// sqrt(beta) and temp = 0.5/beta = p/rho, which the fluxes need again (split_fluxes_mod.h:20-25 recompute
 // beta = 0.5*rho/p and p/rho; here they are reused, equal up to rounding).
 // rho = exp(temp1 - log(beta)/(gamma-1)) is evaluated as exp(temp1) /
 // (beta^2 sqrt(beta)): 1/(gamma-1) = 2.5 for gamma = 1.4, so the logarithm
@@ -79,6 +100,32 @@ __device__ __forceinline__ void qtilde_state(const T qt[4], T &u1, T &u2, T &rho
   const T temp1 = qt[0] + beta * (u1 * u1 + u2 * u2);
   sb = sqrt(beta);
   rho = exp(temp1) * qdiv(T(1.0), beta * beta * sb);
+}
+// For doubles and floats, one y = 1/sqrt(beta) gives all three: 0.5/beta = y^2/2,
+// sqrt(beta) = beta y and beta^-2.5 = y^5 (a few ulps in all), in place of
+// a square root and two divisions.
+template <typename T>
+__device__ __forceinline__ void qtilde_state_rsqrt(const T qt[4], T &u1, T &u2, T &rho, T &sb,
+                                                   T &temp) {
+  const T beta = -qt[3] * T(0.5);
+  const T y = rsqrt_nr(beta);
+  const T y2 = y * y;
+  temp = T(0.5) * y2;
+  u1 = qt[1] * temp;
+  u2 = qt[2] * temp;
+  const T temp1 = qt[0] + beta * (u1 * u1 + u2 * u2);
+  sb = beta * y;
+  rho = exp(temp1) * (y2 * y2 * y);
+}
+template <>
+__device__ __forceinline__ void qtilde_state(const double qt[4], double &u1, double &u2,
+                                             double &rho, double &sb, double &temp) {
+  qtilde_state_rsqrt(qt, u1, u2, rho, sb, temp);
+}
+template <>
+__device__ __forceinline__ void qtilde_state(const float qt[4], float &u1, float &u2,
+                                             float &rho, float &sb, float &temp) {
+  qtilde_state_rsqrt(qt, u1, u2, rho, sb, temp);
 }
 
 // erf(s) given x = exp(-s^2), which the kinetic terms compute anyway.  For
@@ -115,7 +162,7 @@ __device__ __forceinline__ double erf_x(double s, double x) {
   q = fma(q, t, 3.5495262609381824);
   q = fma(q, t, 2.797342582242985);
   q = fma(q, t, 1.0);
-  const double e = fma(-x, qdiv(p, q), 1.0);
+  const double e = fma(-x, qdiv_loose(p, q), 1.0);
   return s < 0.0 ? -e : e;
 }
 
@@ -211,20 +258,28 @@ __device__ __forceinline__ void quad_x(const EndTerms<T> &e, T s1, T s2, T G[4])
   G[3] = e.rho * A2 * (temp2 + s1 * temp3) + s2 * temp4;
 }
 
+// The Venkatakrishnan ratio of limiters_mod.h:85-99.  There num and den
+// both carry a factor del_neg, which the callers have already shown to be
+// non-zero (|del_neg| > 1e-5); cancelled, with a = del_pos^2 + eps2,
+//   phi = min(1, (a + 2 dn dp) / (a + dn (2 dn + dp))).
+// dp has the sign of dn (qmax - q >= 0 for dn > 0, qmin - q <= 0 for
+// dn < 0), so every term is non-negative and nothing cancels.
+template <typename T>
+__device__ __forceinline__ T venkat_ratio(T dn, T dp, T eps2) {
+  const T a = dp * dp + eps2;
+  const T num = a + T(2.0) * dn * dp;
+  const T den = a + dn * (T(2.0) * dn + dp);
+  const T temp = qdiv_loose(num, den);
+  return temp < T(1) ? temp : T(1);
+}
+
 // limiters_mod.h:55-104 for one component; eps2 = (pow(K*min_dist,3))^2 is
 // precomputed per point on the host with the reference's pow call.
 template <typename T>
 __device__ __forceinline__ T venkat_phi(T qtilde, T q, T qmax, T qmin, T eps2) {
   const T del_neg = qtilde - q;
   if (fabs(del_neg) <= T(10e-6)) return T(1.0);
-  const T del_pos = del_neg > T(0) ? qmax - q : qmin - q;
-  T num = (del_pos * del_pos) + eps2;
-  num = num * del_neg + T(2.0) * del_neg * del_neg * del_pos;
-  T den = del_pos * del_pos + T(2.0) * del_neg * del_neg;
-  den = den + del_neg * del_pos + eps2;
-  den = den * del_neg;
-  const T temp = qdiv(num, den);
-  return temp < T(1) ? temp : T(1);
+  return venkat_ratio(del_neg, del_neg > T(0) ? qmax - q : qmin - q, eps2);
 }
 
 // The same limiter from the increments themselves: del_neg = q~ - q =
@@ -234,12 +289,7 @@ __device__ __forceinline__ T venkat_phi(T qtilde, T q, T qmax, T qmin, T eps2) {
 template <typename T>
 __device__ __forceinline__ T venkat_phi_delta(T del_neg, T dmax, T dmin, T eps2) {
   if (fabs(del_neg) <= T(10e-6)) return T(1.0);
-  const T del_pos = del_neg > T(0) ? dmax : dmin;
-  const T dp2 = del_pos * del_pos;
-  const T num = (dp2 + eps2) * del_neg + T(2.0) * del_neg * del_neg * del_pos;
-  const T den = (dp2 + T(2.0) * del_neg * del_neg + del_neg * del_pos + eps2) * del_neg;
-  const T temp = qdiv(num, den);
-  return temp < T(1) ? temp : T(1);
+  return venkat_ratio(del_neg, del_neg > T(0) ? dmax : dmin, eps2);
 }
 
 }  // namespace lskum

@@ -314,7 +314,7 @@ int lskum_shape_vjp(lskum_ctx *ctx, int32_t n_iter, const double w[4], double *J
       launch_q_variables(dc, Ps, ctx->base, s);
       launch_q_derivatives_arg(dc, ctx->base, dq_of(0), ctx->qm, ctx->qsum, a.arg, a.lsum, s);
       for (int p = 0; p < inner; p++)
-        launch_q_inner(dc, ctx->base, ctx->qsum, dq_of(p), dq_of(p + 1), 0.0, s);
+        launch_q_inner(dc, ctx->qsum, dq_of(p), dq_of(p + 1), 0.0, s);
       launch_flux_vjp(dc, a, ctx->base, dq_of(inner), ctx->qm, ctx->delta, rbar, &g, s);
       int b = 0;
       for (int p = inner - 1; p >= 0; p--) {

@@ -374,7 +374,8 @@ def kat_fluxes(rows6, mixed=False):
 
 
 def kat_qtilde(qt):
-    """Device qtilde_to_primitive on rows q~[4] -> [n, 4] (u1, u2, rho, pr)."""
+    """Device qtilde_to_primitive, in the form the flux kernels evaluate it,
+    on rows q~[4] -> [n, 4] (u1, u2, rho, pr)."""
     qt = np.ascontiguousarray(qt, dtype=np.float64)
     n = qt.shape[0]
     out = np.zeros((n, 4))

@@ -468,8 +468,9 @@ int lskum_kernel_time(lskum_ctx *ctx, int32_t kernel, double *avg_ms,
  * eight flux functions (split_fluxes_mod.h:6,45,86,127 and
  * quadrant_fluxes_mod.h:6,57,108,161) on n input rows
  * (nx, ny, u1, u2, rho, pr) -> out[n][8][4], order Gxp, Gxn, Gyp, Gyn, GxI,
- * GxII, GxIII, GxIV; and qtilde_to_primitive (q_variables_mod.h:230) on
- * n rows of q~[4] -> prim_out[n][4] as (u1, u2, rho, pr).  mixed != 0
+ * GxII, GxIII, GxIV; and qtilde_to_primitive (q_variables_mod.h:230), in
+ * the form the flux kernels evaluate it, on n rows of q~[4] ->
+ * prim_out[n][4] as (u1, u2, rho, pr).  mixed != 0
  * evaluates the FP32 flux functions of LSKUM_MIXED on FP32-rounded inputs.
  */
 int lskum_kat_fluxes(int32_t n, const double *in6, double *out, int32_t mixed);
