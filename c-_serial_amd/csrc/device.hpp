@@ -115,4 +115,48 @@ __device__ __forceinline__ uint32_t conn_mask(uint32_t w, bool tiled) {
   return tiled ? w >> kLocalShift : w >> kMaskShift;
 }
 
+// The next DEPTH raw conn entries of one lane's list, in registers.  The
+// stencil loops fetch neighbour e + 1's record while e is processed; pop
+// hands them entry e + 1 and issues the load of entry e + 1 + DEPTH
+// (indices clamped to the last entry), so each conn load has DEPTH
+// neighbours' work to arrive in.  The loops unroll by DEPTH, which turns
+// the shift of the ring into a renaming of registers (a register copy
+// would wait for its load).  With the conn arrays far larger than L2,
+// one neighbour's work is not enough to cover the load on the kernels that
+// run few waves per SIMD.
+//
+// The loads are relaxed wavefront-scope atomics, which compile to the same
+// plain global loads.  A plain load would be simple enough for the compiler
+// to fold the ring of loaded values back into a ring of indices and issue
+// each load where its value is used, undoing the distance.
+template <int DEPTH>
+struct ConnAhead {
+  uint32_t w[DEPTH] = {};
+  __device__ __forceinline__ static uint32_t load(const uint32_t *conn, const uint16_t *conn16,
+                                                  bool tiled, int e) {
+    return tiled ? (uint32_t)__hip_atomic_load(conn16 + e, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_WAVEFRONT)
+                 : __hip_atomic_load(conn + e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+  }
+  // entries first .. first + DEPTH - 1; first - 1 <= last
+  __device__ __forceinline__ void start(const uint32_t *conn, const uint16_t *conn16,
+                                        bool tiled, int first, int last) {
+#pragma unroll
+    for (int d = 0; d < DEPTH; d++) w[d] = load(conn, conn16, tiled, min(first + d, last));
+  }
+  __device__ __forceinline__ uint32_t pop(const uint32_t *conn, const uint16_t *conn16,
+                                          bool tiled, int e_load, int last) {
+    const uint32_t head = w[0];
+#pragma unroll
+    for (int d = 0; d + 1 < DEPTH; d++) w[d] = w[d + 1];
+    w[DEPTH - 1] = load(conn, conn16, tiled, min(e_load, last));
+    return head;
+  }
+};
+// Depths measured on a 2.5M-point cloud with 24 neighbours a point (shallower
+// and deeper rings were slower).  func_delta and the mixed q_derivatives ran
+// slower with any ring and load their conn entries one neighbour ahead.
+constexpr int kAheadQDeriv = 3;
+constexpr int kAheadQInner = 4;
+
 }  // namespace lskum

@@ -141,7 +141,7 @@ __global__ __launch_bounds__(256) void k_q_derivatives(
   for (int r = 0; r < 4; r++) qmax[r] = qmin[r] = qi[r];
   int am[4] = {i, i, i, i}, an[4] = {i, i, i, i};
   const int32_t *tix = TILED ? tile_idx + tile_off[b] : nullptr;
-  // neighbour e + 1's record and e + 2's conn entry load while e is used
+  // neighbour e + 1's record loads while e is used (conn entries further ahead)
   auto fetch = [&](uint32_t w, double2 &xyk, double qk[4]) {
     const int k = conn_index(w, TILED);
     const double *hk = TILED ? lds + S * k : base + (size_t)B_STRIDE * k;
@@ -151,19 +151,21 @@ __global__ __launch_bounds__(256) void k_q_derivatives(
   const int e0 = off[i], last = off[i + 1] - 1;
   double2 xyk = {0.0, 0.0};
   double qk[4] = {0.0, 0.0, 0.0, 0.0};
-  uint32_t w_n = 0;
+  ConnAhead<kAheadQDeriv> ahead;
+  uint32_t w_c = 0;
   if (e0 <= last) {
-    fetch(conn_raw(conn, conn16, e0, TILED), xyk, qk);
-    w_n = conn_raw(conn, conn16, min(e0 + 1, last), TILED);
+    w_c = conn_raw(conn, conn16, e0, TILED);
+    fetch(w_c, xyk, qk);
+    ahead.start(conn, conn16, TILED, e0 + 1, last);
   }
-  uint32_t w_c = e0 <= last ? conn_raw(conn, conn16, e0, TILED) : 0u;
+#pragma unroll kAheadQDeriv
   for (int e = e0; e <= last; e++) {
     double2 xy_n;
     double q_n[4];
+    const uint32_t w_n = ahead.pop(conn, conn16, TILED, e + 1 + kAheadQDeriv, last);
     fetch(w_n, xy_n, q_n);
     const uint32_t w_k = w_c;
     w_c = w_n;
-    w_n = conn_raw(conn, conn16, min(e + 2, last), TILED);
     if (ARG) {
       const int kl = conn_index(w_k, TILED);
       const int kg = TILED ? tix[kl] : kl;
@@ -266,7 +268,7 @@ __global__ __launch_bounds__(256) void k_q_inner(
   load4(di + 4, dyi);
   double sum_delx_sqr = 0., sum_dely_sqr = 0., sum_delx_dely = 0.;
   double sx[4] = {0.0, 0.0, 0.0, 0.0}, sy[4] = {0.0, 0.0, 0.0, 0.0};
-  // neighbour e + 1's record and e + 2's conn entry load while e is used
+  // neighbour e + 1's record loads while e is used (conn entries further ahead)
   auto fetch = [&](uint32_t w, double2 &xyk, double dxk[4], double dyk[4]) {
     const int k = conn_index(w, TILED);
     const double *hk = TILED ? lds + S * k : xy + 2 * (size_t)k;
@@ -278,16 +280,16 @@ __global__ __launch_bounds__(256) void k_q_inner(
   const int e0 = off[i], last = off[i + 1] - 1;
   double2 xyk = {0.0, 0.0};
   double dxk[4] = {}, dyk[4] = {};
-  uint32_t w_n = 0;
+  ConnAhead<kAheadQInner> ahead;
   if (e0 <= last) {
     fetch(conn_raw(conn, conn16, e0, TILED), xyk, dxk, dyk);
-    w_n = conn_raw(conn, conn16, min(e0 + 1, last), TILED);
+    ahead.start(conn, conn16, TILED, e0 + 1, last);
   }
+#pragma unroll kAheadQInner
   for (int e = e0; e <= last; e++) {
     double2 xy_n;
     double dx_n[4], dy_n[4];
-    fetch(w_n, xy_n, dx_n, dy_n);
-    w_n = conn_raw(conn, conn16, min(e + 2, last), TILED);
+    fetch(ahead.pop(conn, conn16, TILED, e + 1 + kAheadQInner, last), xy_n, dx_n, dy_n);
     const double delx = xyk.x - x_i, dely = xyk.y - y_i;
     const double weights = POW ? pow(sqrt(delx * delx + dely * dely), power) : 1.0;
     if (POW) {

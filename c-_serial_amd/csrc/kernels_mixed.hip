@@ -240,24 +240,24 @@ __global__ __launch_bounds__(256) void k_q_inner_m(
   unpack4(gd[2 * li + 1], dyi);
   double sum_delx_sqr = 0., sum_dely_sqr = 0., sum_delx_dely = 0.;
   double sx[4] = {0.0, 0.0, 0.0, 0.0}, sy[4] = {0.0, 0.0, 0.0, 0.0};
-  // neighbour e + 1's record and e + 2's conn entry load while e is used
+  // neighbour e + 1's record loads while e is used (conn entries further ahead)
   const int e0 = off[i], last = off[i + 1] - 1;
   double2 pk = {0.0, 0.0};
   float4 qk4 = {}, dxk4 = {}, dyk4 = {};
-  uint32_t w_n = 0;
+  ConnAhead<kAheadQInner> ahead;
   if (e0 <= last) {
     const int k0 = conn_index(conn_raw(conn, conn16, e0, TILED), TILED);
     pk = gxy[k0];
     qk4 = gq[k0];
     dxk4 = gd[2 * k0];
     dyk4 = gd[2 * k0 + 1];
-    w_n = conn_raw(conn, conn16, min(e0 + 1, last), TILED);
+    ahead.start(conn, conn16, TILED, e0 + 1, last);
   }
+#pragma unroll kAheadQInner
   for (int e = e0; e <= last; e++) {
-    const int kn = conn_index(w_n, TILED);
+    const int kn = conn_index(ahead.pop(conn, conn16, TILED, e + 1 + kAheadQInner, last), TILED);
     const double2 pk_n = gxy[kn];
     const float4 qk4_n = gq[kn], dxk4_n = gd[2 * kn], dyk4_n = gd[2 * kn + 1];
-    w_n = conn_raw(conn, conn16, min(e + 2, last), TILED);
     float qk[4], dxk[4], dyk[4];
     unpack4(qk4, qk);
     unpack4(dxk4, dxk);
