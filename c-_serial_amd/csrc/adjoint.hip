@@ -467,8 +467,8 @@ __global__ __launch_bounds__(256) void k_flux_vjp(
 // Per point j: its own share plus the edge records of every pair in which
 // it is the neighbour (transpose CSR, owners ascending).
 __global__ __launch_bounds__(256) void k_flux_vjp_gather(
-    int n, const int32_t *__restrict__ tr_off, const int32_t *__restrict__ tr_owner, const double *__restrict__ base,
-    const double *__restrict__ edge, const double *__restrict__ own_q,
+    int n, const int32_t *__restrict__ tr_off, const int32_t *__restrict__ tr_owner,
+    const double2 *__restrict__ xy, const double *__restrict__ edge, const double *__restrict__ own_q,
     const double *__restrict__ own_dq, const double *__restrict__ own_qm,
     double *__restrict__ lq, double *__restrict__ ldq, double *__restrict__ lqm) {
   const int j = xcd_block() * blockDim.x + threadIdx.x;
@@ -477,12 +477,12 @@ __global__ __launch_bounds__(256) void k_flux_vjp_gather(
   load4(own_q + 4 * (size_t)j, aq);
   load8(own_dq + 8 * (size_t)j, adx, ady);
   load8(own_qm + 8 * (size_t)j, amx, amn);
-  const double x_j = base[(size_t)B_STRIDE * j + B_X], y_j = base[(size_t)B_STRIDE * j + B_Y];
+  const double x_j = xy[j].x, y_j = xy[j].y;
   // record t + 1 with its owner's coordinates, and owner t + 2's index, load
   // while record t is added
   const int t0 = tr_off[j], tlast = tr_off[j + 1] - 1;
   auto fetch = [&](int t, int i, double2 &p, double r16[16]) {
-    p = reinterpret_cast<const double2 *>(base + (size_t)B_STRIDE * i)[0];
+    p = xy[i];
     const double *rec = edge + 16 * (size_t)t;
     load8(rec, r16, r16 + 4);
     load8(rec + 8, r16 + 8, r16 + 12);
@@ -617,21 +617,19 @@ __global__ __launch_bounds__(256) void k_ls_adj(int n, const int32_t *__restrict
 __global__ __launch_bounds__(256) void k_ls_adj_gather(
     int n, const int32_t *__restrict__ off, const uint32_t *__restrict__ conn,
     const int32_t *__restrict__ tr_off, const int32_t *__restrict__ tr_owner,
-    const double *__restrict__ base, const double *__restrict__ ls, int with_dq,
+    const double2 *__restrict__ xy, const double *__restrict__ ls, int with_dq,
     double *__restrict__ ldq_in, const int32_t *__restrict__ arg,
     const double *__restrict__ lqm, double *__restrict__ lq) {
   const int j = xcd_block() * blockDim.x + threadIdx.x;
   if (j >= n) return;
-  const double x_j = base[(size_t)B_STRIDE * j + B_X], y_j = base[(size_t)B_STRIDE * j + B_Y];
+  const double x_j = xy[j].x, y_j = xy[j].y;
   double sx[4], sy[4], aq[4], adx[4] = {0, 0, 0, 0}, ady[4] = {0, 0, 0, 0};
   load8(ls + 8 * (size_t)j, sx, sy);
   load4(lq + 4 * (size_t)j, aq);
   // pairs (j, k) of j's own list; neighbour e + 1's coordinates and e + 2's
   // conn entry load while e is used
   const int e0 = off[j], last = off[j + 1] - 1;
-  auto xy_of = [&](uint32_t w) {
-    return reinterpret_cast<const double2 *>(base + (size_t)B_STRIDE * (w & kIndexMask))[0];
-  };
+  auto xy_of = [&](uint32_t w) { return xy[w & kIndexMask]; };
   double2 pk = {0.0, 0.0};
   uint32_t w_n = 0;
   if (e0 <= last) {
@@ -655,7 +653,7 @@ __global__ __launch_bounds__(256) void k_ls_adj_gather(
   // sums' adjoints and t + 2's index load while t is used
   const int t0 = tr_off[j], tlast = tr_off[j + 1] - 1;
   auto own_of = [&](int i, double2 &p, double s8[8]) {
-    p = reinterpret_cast<const double2 *>(base + (size_t)B_STRIDE * i)[0];
+    p = xy[i];
     load8(ls + 8 * (size_t)i, s8, s8 + 4);
   };
   double2 po = {0.0, 0.0};
@@ -877,8 +875,8 @@ void launch_flux_vjp(const DevCloud &c, const AdjointBuffers &a, const double *b
   }
 #undef VJP_ARGS
   hipLaunchKernelGGL(k_flux_vjp_gather, dim3(blocks_of(c.n)), dim3(256), 0, s, c.n, a.tr_off,
-                     a.tr_owner, base, a.edge, a.own_q, a.own_dq, a.own_qm, a.lq, a.ldq[0],
-                     a.lqm);
+                     a.tr_owner, reinterpret_cast<const double2 *>(c.xy), a.edge, a.own_q,
+                     a.own_dq, a.own_qm, a.lq, a.ldq[0], a.lqm);
 }
 
 void launch_ls_vjp(const DevCloud &c, const AdjointBuffers &a, const double *base,
@@ -891,8 +889,8 @@ void launch_ls_vjp(const DevCloud &c, const AdjointBuffers &a, const double *bas
     hipLaunchKernelGGL(k_ls_adj<false>, dim3(blocks_of(c.n)), dim3(256), 0, s, c.n, c.conn_off,
                        c.conn, base, a.lsum, lo, a.ls, nullptr, nullptr, nullptr);
   hipLaunchKernelGGL(k_ls_adj_gather, dim3(blocks_of(c.n)), dim3(256), 0, s, c.n, c.conn_off,
-                     c.conn, a.tr_off, a.tr_owner, base, a.ls, extrema ? 0 : 1, ldq_in,
-                     extrema ? a.arg : nullptr, a.lqm, a.lq);
+                     c.conn, a.tr_off, a.tr_owner, reinterpret_cast<const double2 *>(c.xy), a.ls,
+                     extrema ? 0 : 1, ldq_in, extrema ? a.arg : nullptr, a.lqm, a.lq);
 }
 
 void launch_qvar_vjp(const DevCloud &c, const AdjointBuffers &a, const double *prim,
