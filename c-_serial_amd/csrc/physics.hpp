@@ -49,9 +49,10 @@ __device__ __forceinline__ float qdiv(float a, float b) {
   return fmaf(fmaf(-b, q, a), r, q);
 }
 
-// e^a.  Floats take the hardware base-2 exponential of a log2(e): the
-// rounding of the product costs |a| 6e-8 in relative terms, as much as a
-// has already lost in FP32, in place of the library's range-reduced expf.
+// e^a as the fluxes use it (other types: the library exp).  Floats take the
+// hardware base-2 exponential of a log2(e): the rounding of the product
+// costs |a| 6e-8 in relative terms, as much as a has already lost in FP32,
+// in place of the library's range-reduced expf.
 template <typename T>
 __device__ __forceinline__ T exp_k(T a) {
   return exp(a);
@@ -59,6 +60,33 @@ __device__ __forceinline__ T exp_k(T a) {
 template <>
 __device__ __forceinline__ float exp_k(float a) {
   return __builtin_amdgcn_exp2f(a * 1.44269504088896341f);
+}
+// Doubles: a = n ln2 + r with n = rint(a / ln2) and |r| <= ln2/2 (r by two
+// FMAs against ln 2 split into a double and its remainder), e^r from
+// p(r) = 1 + r + c2 r^2 + ... + c11 r^11, a fit in relative error (7e-18;
+// exp_fit.py beside this file derives it), then scaled by 2^n: within an
+// ulp or two.  a is first raised to -708 (a NaN stays NaN), which keeps 2^n
+// normal in place of the library's underflow tests (e^-708 = 3e-308 stands
+// for 0); past 709.78 the scaling overflows to inf as the library's does.
+template <>
+__device__ __forceinline__ double exp_k(double a) {
+  a = a < -708.0 ? -708.0 : a;
+  const double n = rint(a * 1.4426950408889634);
+  double r = fma(-n, 0.6931471805599453, a);
+  r = fma(-n, 2.3190468138462996e-17, r);
+  double p = 2.499930047834758e-08;
+  p = fma(p, r, 2.763033850976749e-07);
+  p = fma(p, r, 2.755758895260481e-06);
+  p = fma(p, r, 2.4801492811609905e-05);
+  p = fma(p, r, 0.00019841269503400455);
+  p = fma(p, r, 0.0013888888943951836);
+  p = fma(p, r, 0.008333333333496135);
+  p = fma(p, r, 0.04166666666652869);
+  p = fma(p, r, 0.1666666666666641);
+  p = fma(p, r, 0.5000000000000011);
+  p = fma(p, r, 1.0);
+  p = fma(p, r, 1.0);
+  return ldexp(p, (int)n);
 }
 
 // a / b without qdiv's last correction: the reciprocal after its two Newton
