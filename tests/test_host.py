@@ -422,3 +422,42 @@ def test_erf_from_exponential_constants():
         e = -e if s < 0 else e
         worst = max(worst, abs(e - math.erf(s)))
     assert worst <= 6e-16, worst
+
+
+def test_exp_reduction_constants():
+    # physics.hpp's double exp_k, a = n ln2 + r and 2^n (1 + r + c2 r^2 + ...
+    # + c11 r^11), evaluated here in double (math.fma needs Python 3.13, so
+    # the FMAs are emulated exactly with fractions) with the constants read
+    # from the source and compared with the C library's exp: within 3 ulps
+    # relative on [-700, 700], and the clamp below -708
+    import math
+    import re
+    from fractions import Fraction
+    src = open(os.path.join(REPO, "c-_serial_amd", "csrc", "physics.hpp")).read()
+    body = src[src.index("__device__ __forceinline__ double exp_k(double a)"):]
+    body = body[:body.index("\n}\n")]
+    c = [float(x) for x in re.findall(r"p = fma\(p, r, ([0-9.e+-]+)\)", body)]
+    c0 = float(re.search(r"double p = ([0-9.e+-]+);", body).group(1))
+    inv_ln2 = float(re.search(r"rint\(a \* ([0-9.e+-]+)\)", body).group(1))
+    hi, lo = (float(x) for x in re.findall(r"r = fma\(-n, ([0-9.e+-]+), [ar]\)", body))
+    assert len(c) == 11 and c[-2:] == [1.0, 1.0]
+    assert abs(hi + lo - math.log(2)) < 1e-30 + 2 ** -60 and abs(inv_ln2 * math.log(2) - 1) < 2e-16
+
+    def fma(a, b, d):
+        return float(Fraction(a) * Fraction(b) + Fraction(d))
+
+    def exp_k(a):
+        a = -708.0 if a < -708.0 else a
+        n = float(round(a * inv_ln2))  # round half to even, as rint
+        r = fma(-n, lo, fma(-n, hi, a))
+        p = c0
+        for k in c:
+            p = fma(p, r, k)
+        return math.ldexp(p, int(n))
+
+    worst = 0.0
+    for a in np.concatenate([np.linspace(-700.0, 700.0, 4001), np.linspace(-2.0, 2.0, 4001)]):
+        a = float(a)
+        worst = max(worst, abs(exp_k(a) / math.exp(a) - 1.0))
+    assert worst <= 3 * 2.0 ** -53, worst
+    assert exp_k(-800.0) == exp_k(-708.0) > 0.0
