@@ -86,7 +86,9 @@ __device__ __forceinline__ double exp_k(double a) {
   p = fma(p, r, 0.5000000000000011);
   p = fma(p, r, 1.0);
   p = fma(p, r, 1.0);
-  return ldexp(p, (int)n);
+  // n capped so that its conversion to int stays defined (2^1100 overflows
+  // to inf as well; a NaN n leaves p NaN)
+  return ldexp(p, (int)fmin(n, 1100.0));
 }
 
 // a / b without qdiv's last correction: the reciprocal after its two Newton
