@@ -752,13 +752,28 @@ __global__ __launch_bounds__(256) void k_reduce(const int32_t *__restrict__ seg,
   const int b0 = seg ? seg[c] : 0, b1 = seg ? seg[c + 1] : nb;
   double s = 0.0, mx = -1.0;
   int arg = 0x7fffffff;
-  for (int b = b0 + threadIdx.x; b < b1; b += blockDim.x) {
-    s = s + psum[b];
-    const double v = pmax[b];
-    const int a = parg[b];
-    if (v > mx || (v == mx && a < arg)) {
-      mx = v;
-      arg = a;
+  // eight partials a thread loaded at once (indices clamped to the last),
+  // then taken in the same order as one at a time
+  constexpr int U = 8;
+  const int stride = (int)blockDim.x;
+  for (int b = b0 + threadIdx.x; b < b1; b += U * stride) {
+    double ps[U], pm[U];
+    int pa[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int bu = min(b + u * stride, b1 - 1);
+      ps[u] = psum[bu];
+      pm[u] = pmax[bu];
+      pa[u] = parg[bu];
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      if (b + u * stride >= b1) break;
+      s = s + ps[u];
+      if (pm[u] > mx || (pm[u] == mx && pa[u] < arg)) {
+        mx = pm[u];
+        arg = pa[u];
+      }
     }
   }
   block_reduce3(s, mx, arg);
