@@ -390,7 +390,7 @@ def main():
                     help="shape workload: iterations the adjoint differentiates")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU only; the q-LSKUM iteration would shard "
+        print("bench.py: the library runs on one GPU only; the q-LSKUM iteration would shard "
               "by point partition with a one-ring halo, which is not built. Use --gpus 1.",
               file=sys.stderr)
         sys.exit(2)
