@@ -49,4 +49,14 @@ void rcm_order(const lskum_cloud &c, std::vector<int32_t> &perm);
 // block of consecutive points reaches a small set of neighbours.
 void hilbert_order(const lskum_cloud &c, std::vector<int32_t> &perm);
 
+// The reverse of the sub-stencil coefficients for the shape gradient: per
+// point d, from the adjoints gc[8d..8d+7] of its coefficients (a, b) =
+// (syy, -sxy)/det (x+, x-) or (sxx, -sxy)/det (y+, y-), the adjoints of its
+// pairs' offsets (added to ge[2e], ge[2e+1]) and of its normal (added to
+// gn[2d], gn[2d+1]).  conn entries are index | mask << 28.  Points are
+// independent, so the loop runs on every host thread.
+void coef_adjoint(int32_t n, const double *x, const double *y, const double *nx,
+                  const double *ny, const int32_t *off, const uint32_t *conn,
+                  const double *gc, double *ge, double *gn);
+
 }  // namespace lskum

@@ -346,55 +346,8 @@ int lskum_shape_vjp(lskum_ctx *ctx, int32_t n_iter, const double w[4], double *J
   for (size_t j = 0; j < 2 * (size_t)n; j++) gn[j] += nb[j];
   // sub-stencil coefficients (coef = (syy, -sxy)/det or (sxx, -sxy)/det of
   // sums over the members) -> pairs and normals
-  for (int32_t d = 0; d < n; d++) {
-    const double nx = L.nx[d], ny = L.ny[d], tx = ny, ty = -nx;
-    double sxx[4] = {0, 0, 0, 0}, syy[4] = {0, 0, 0, 0}, sxy[4] = {0, 0, 0, 0};
-    for (int32_t e = L.off[d]; e < L.off[d + 1]; e++) {
-      const int32_t k = (int32_t)(L.conn[e] & kIndexMask);
-      const uint32_t m = L.conn[e] >> kMaskShift;
-      const double delx = L.x[k] - L.x[d], dely = L.y[k] - L.y[d];
-      const double dels = delx * tx + dely * ty, deln = delx * nx + dely * ny;
-      for (int sb = 0; sb < 4; sb++) {
-        if (!(m & (1u << sb))) continue;
-        sxx[sb] += dels * dels;
-        syy[sb] += deln * deln;
-        sxy[sb] += dels * deln;
-      }
-    }
-    double bxx[4], byy[4], bxy[4];
-    for (int sb = 0; sb < 4; sb++) {
-      bxx[sb] = byy[sb] = bxy[sb] = 0.0;
-      const double det = sxx[sb] * syy[sb] - sxy[sb] * sxy[sb];
-      if (det == 0.0) continue;  // coefficients 0 whatever the geometry
-      const double inv = 1.0 / det, A = sb < 2 ? syy[sb] : sxx[sb];
-      const double c0 = gc[(size_t)8 * d + 2 * sb], c1 = gc[(size_t)8 * d + 2 * sb + 1];
-      const double invb = c0 * A - c1 * sxy[sb];
-      const double detb = -invb * inv * inv;
-      bxx[sb] = detb * syy[sb];
-      byy[sb] = detb * sxx[sb];
-      bxy[sb] = -2.0 * detb * sxy[sb] - c1 * inv;
-      if (sb < 2)
-        byy[sb] += c0 * inv;
-      else
-        bxx[sb] += c0 * inv;
-    }
-    for (int32_t e = L.off[d]; e < L.off[d + 1]; e++) {
-      const int32_t k = (int32_t)(L.conn[e] & kIndexMask);
-      const uint32_t m = L.conn[e] >> kMaskShift;
-      const double delx = L.x[k] - L.x[d], dely = L.y[k] - L.y[d];
-      const double dels = delx * tx + dely * ty, deln = delx * nx + dely * ny;
-      double bs = 0.0, bn = 0.0;
-      for (int sb = 0; sb < 4; sb++) {
-        if (!(m & (1u << sb))) continue;
-        bs += 2.0 * bxx[sb] * dels + bxy[sb] * deln;
-        bn += 2.0 * byy[sb] * deln + bxy[sb] * dels;
-      }
-      ge[2 * (size_t)e] += bs * ny + bn * nx;
-      ge[2 * (size_t)e + 1] += -bs * nx + bn * ny;
-      gn[2 * (size_t)d] += -bs * dely + bn * delx;
-      gn[2 * (size_t)d + 1] += bs * delx + bn * dely;
-    }
-  }
+  coef_adjoint(n, L.x.data(), L.y.data(), L.nx.data(), L.ny.data(), L.off.data(),
+               L.conn.data(), gc.data(), ge.data(), gn.data());
   // pairs -> their points
   for (int32_t d = 0; d < n; d++)
     for (int32_t e = L.off[d]; e < L.off[d + 1]; e++) {
