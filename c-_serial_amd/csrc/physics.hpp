@@ -65,12 +65,15 @@ __device__ __forceinline__ float exp_k(float a) {
 // FMAs against ln 2 split into a double and its remainder), e^r from
 // p(r) = 1 + r + c2 r^2 + ... + c11 r^11, a fit in relative error (7e-18;
 // exp_fit.py beside this file derives it), then scaled by 2^n: within an
-// ulp or two.  a is first raised to -708 (a NaN stays NaN), which keeps 2^n
-// normal in place of the library's underflow tests (e^-708 = 3e-308 stands
-// for 0); past 709.78 the scaling overflows to inf as the library's does.
+// ulp or two.  a is first raised to -708, which keeps 2^n normal in place
+// of the library's underflow tests (e^-708 = 3e-308 stands for 0), and
+// lowered to 710, so that +inf does not turn n ln2 - a into inf - inf;
+// past 709.78 the scaling overflows to inf as the library's does.  Both
+// clamps compare, so a NaN stays NaN.
 template <>
 __device__ __forceinline__ double exp_k(double a) {
   a = a < -708.0 ? -708.0 : a;
+  a = a > 710.0 ? 710.0 : a;
   const double n = rint(a * 1.4426950408889634);
   double r = fma(-n, 0.6931471805599453, a);
   r = fma(-n, 2.3190468138462996e-17, r);

@@ -26,12 +26,29 @@ def golden(name):
 
 def normwise(a, b):
     """max|a-b| / max|b| per leading component (the parity measure: pointwise
-    relative error is ill-posed where a component crosses zero)."""
+    relative error is ill-posed where a component crosses zero), taken over
+    every component (all axes but the last; a 1-D array is one component).
+
+    Strict about non-finite values: positions where a and b are equal
+    (equal infinities included) count as no error; a NaN on either side, or
+    an infinity the other side does not have, makes the error inf.  The
+    denominator is the largest finite |b| of the component."""
     import numpy as np
     a = np.asarray(a, dtype=np.float64)
     b = np.asarray(b, dtype=np.float64)
-    if a.ndim == 1:
-        return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
-    a2 = a.reshape(-1, a.shape[-1])
-    b2 = b.reshape(-1, b.shape[-1])
-    return max(float(np.abs(x - y).max() / max(np.abs(y).max(), 1e-300)) for x, y in zip(a2, b2))
+    if a.shape != b.shape:
+        raise ValueError(f"normwise: shapes differ, {a.shape} vs {b.shape}")
+    if a.size == 0:
+        return 0.0
+    a2 = a.reshape(1, -1) if a.ndim <= 1 else a.reshape(-1, a.shape[-1])
+    b2 = b.reshape(1, -1) if b.ndim <= 1 else b.reshape(-1, b.shape[-1])
+    worst = 0.0
+    for x, y in zip(a2, b2):
+        differ = ~(x == y)
+        if np.any(differ & ~(np.isfinite(x) & np.isfinite(y))):
+            return float("inf")
+        fy = np.abs(y[np.isfinite(y)])
+        scale = max(float(fy.max()) if fy.size else 0.0, 1e-300)
+        err = float(np.abs(np.where(differ, x - y, 0.0)).max()) / scale
+        worst = max(worst, err)
+    return worst

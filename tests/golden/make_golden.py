@@ -15,8 +15,8 @@ come from the reference's own functions run by oracle/ref_driver.cpp.
                      20 iterations: history, final prim and flux_res
   naca0012_9760_m063_1000it.npz
                      naca0012_9760_k20 with the same numerics, 1000
-                     iterations: history, final prim and SHA-256 digests of
-                     every final array
+                     iterations: history, final prim and flux_res and
+                     SHA-256 digests of every final array
   diag_small.npz     naca0012_1480_k20, the same numerics, 40 iterations:
                      final prim and the reference's diagnostics of it
                      (entropy, vorticity, its square and total enstrophy;
@@ -122,9 +122,87 @@ def long_9760():
         os.path.join(HERE, "naca0012_9760_m063_1000it.npz"), grid_sha=gsha,
         params=np.array([0.63, 2.0, 0.2, 5.0, 0.0, 0]),
         history=load("history").reshape(-1, 5), prim=final["prim"].reshape(4, n),
+        flux_res=final["flux_res"].reshape(4, n),
         **{nm + "_sha": sha(v) for nm, v in final.items()},
         sub_counts_sha=sha(load("substencil_counts").astype(np.int32)),
         sub_members_sha=sha(load("substencil_members").astype(np.int32)))
+
+
+C2_PARAMS = dict(mach=0.85, aoa=1.25, cfl=0.2, vl_const=5.0, power=0.0, inner_iterations=2)
+C2_ITERS = 500
+C2_FORCE_EVERY = 50
+
+
+def cloud_sha(cloud):
+    h = hashlib.sha256()
+    for a in (cloud.x, cloud.y, cloud.flag_1, cloud.flag_2, cloud.left, cloud.right,
+              cloud.min_dist, cloud.conn_off, cloud.conn):
+        h.update(np.ascontiguousarray(a).tobytes())
+    return h.hexdigest()
+
+
+def c2_sample(cloud, n_random=4000, seed=2502):
+    """Every wall point and n_random others drawn without replacement."""
+    wall = np.flatnonzero(cloud.flag_1 == 0)
+    rest = np.flatnonzero(cloud.flag_1 != 0)
+    pick = np.random.default_rng(seed).choice(rest, size=n_random, replace=False)
+    return np.sort(np.concatenate([wall, pick])).astype(np.int32)
+
+
+def c2_500(threads=8, checkpoint_dir=None):
+    """naca0012_2p5m_k24 (2,502,000 points), M 0.85, AoA 1.25, CFL 0.2, K 5,
+    2 inner passes, 500 iterations, run by the oracle (the reference at this
+    size takes about a day on one core; the oracle reproduces it bit for bit
+    on the fixtures above).  Stored: the residual history, Cl/Cd/Cm every 50
+    iterations, per component max|.| and the L2 norm of the final prim and
+    flux_res, and both arrays at every wall point plus 4,000 others.
+    checkpoint_dir keeps the state every 25 iterations, so an interrupted run
+    resumes exactly (the state is all an iteration reads: q, the derivatives
+    and delta are recomputed from prim)."""
+    sys.path.insert(0, REPO)
+    from oracle.oracle import Oracle
+    cloud = lskum.workload("naca0012_2p5m_k24")
+    n = cloud.n
+    o = Oracle(cloud, threads=threads, **C2_PARAMS)
+    hist = np.zeros((C2_ITERS, 2))
+    forces = []
+    done = 0
+    ck = os.path.join(checkpoint_dir, "c2_checkpoint.npz") if checkpoint_dir else None
+    if ck and os.path.isfile(ck):
+        z = np.load(ck)
+        done = int(z["it"])
+        o.set("prim", z["prim"])
+        o.set_iteration(done, float(z["res_old"]), 1)
+        hist[:done] = z["hist"][:done]
+        forces = [tuple(r) for r in z["forces"]]
+        print("resumed at iteration", done, flush=True)
+    while done < C2_ITERS:
+        step = min(25, C2_ITERS - done)
+        rn, rs = o.q_lskum(step)
+        hist[done:done + step, 0] = rn
+        hist[done:done + step, 1] = rs
+        done += step
+        if done % C2_FORCE_EVERY == 0:
+            o.compute_cl_cd_cm()
+            sc = o.scalars()
+            forces.append((done, sc["cl"], sc["cd"], sc["cm"]))
+        print(f"iteration {done} res_new {hist[done - 1, 0]:.6e}", flush=True)
+        if ck:
+            tmp = ck + ".tmp.npz"
+            np.savez(tmp, it=done, prim=o.get("prim"), res_old=o.scalars()["res_old"],
+                     hist=hist, forces=np.array(forces).reshape(-1, 4))
+            os.replace(tmp, ck)
+    prim = o.get("prim")
+    fres = o.get("flux_res")
+    sample = c2_sample(cloud)
+    np.savez_compressed(
+        os.path.join(HERE, "naca0012_2p5m_m085_500it.npz"), cloud_sha=cloud_sha(cloud),
+        params=np.array([0.85, 1.25, 0.2, 5.0, 0.0, 2]), history=hist,
+        forces=np.array(forces), n_points=n, sample=sample,
+        prim_sample=prim[:, sample], flux_res_sample=fres[:, sample],
+        prim_absmax=np.abs(prim).max(axis=1), prim_l2=np.sqrt((prim * prim).sum(axis=1)),
+        flux_res_absmax=np.abs(fres).max(axis=1),
+        flux_res_l2=np.sqrt((fres * fres).sum(axis=1)))
 
 
 def diag():
@@ -147,10 +225,13 @@ if __name__ == "__main__":
     ap.add_argument("--long", action="store_true",
                     help="also the 1000-iteration run on the 9,760-point cloud")
     ap.add_argument("--only", default="")
+    ap.add_argument("--threads", type=int, default=8, help="oracle threads (c2 job)")
+    ap.add_argument("--checkpoint-dir", default=None, help="resumable state (c2 job)")
     a = ap.parse_args()
-    if not os.path.isfile(DRIVER):
+    if not os.path.isfile(DRIVER) and a.only != "c2":
         sys.exit("build the reference harness first: make -C oracle ref")
-    jobs = {"kat": kat, "small": small_m063, "k24": k24, "long": long_9760, "diag": diag}
+    jobs = {"kat": kat, "small": small_m063, "k24": k24, "long": long_9760, "diag": diag,
+            "c2": lambda: c2_500(a.threads, a.checkpoint_dir)}
     todo = (a.only.split(",") if a.only else
             ["kat", "small", "k24", "diag"] + (["long"] if a.long else []))
     for j in todo:

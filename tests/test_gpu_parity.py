@@ -47,7 +47,18 @@ def test_device_fluxes_match_reference_known_answers():
     for f in range(8):
         assert normwise(got[:, f, :].T, ref[:, f, :].T) <= 1e-14, f
     qt = np.stack([rows[:, 0], rows[:, 1], rows[:, 2], -rows[:, 3]], axis=1)
-    assert normwise(kat_qtilde(qt).T, g["qtilde_prim"].T) <= 1e-14
+    got = kat_qtilde(qt)
+    ref = g["qtilde_prim"]
+    # the reference's exp overflows in rho (and so p) on three rows: the
+    # device must overflow on exactly those, to +inf, and agree elsewhere
+    inf_rows = np.flatnonzero(~np.isfinite(ref).all(axis=1))
+    assert inf_rows.tolist() == [3, 450, 788]
+    np.testing.assert_array_equal(np.isposinf(got), np.isposinf(ref))
+    assert np.isfinite(got[~np.isinf(ref)]).all()
+    keep = np.setdiff1d(np.arange(ref.shape[0]), inf_rows)
+    for comp in range(4):  # u1, u2, rho, p, each on every finite row
+        assert normwise(got[keep, comp], ref[keep, comp]) <= 1e-14, comp
+    assert normwise(got.T, ref.T) <= 1e-14
 
 
 def test_stages_match_oracle_from_a_developed_state():
@@ -106,8 +117,11 @@ def test_iterations_match_reference_small(reorder):
     np.testing.assert_allclose(rs[2:], h[2:, 1], rtol=0, atol=1e-9)
     cl, cd, cm = s.compute_cl_cd_cm()
     np.testing.assert_allclose([cl, cd, cm], h[-1, 2:], rtol=ITER_TOL, atol=1e-13)
-    assert normwise(s.prim, g["prim"]) <= ITER_TOL
-    assert normwise(s.flux_res, g["flux_res"]) <= 1e-9
+    e_prim, e_flux = normwise(s.prim, g["prim"]), normwise(s.flux_res, g["flux_res"])
+    print(f"case: 40 iterations reorder {reorder}: prim {e_prim:.2e} flux_res {e_flux:.2e}",
+          flush=True)
+    assert e_prim <= ITER_TOL
+    assert e_flux <= ITER_TOL
 
 
 def test_point_orders_give_bit_identical_states():
@@ -141,7 +155,10 @@ def test_iterations_match_reference_k24_inner_passes():
     assert normwise(rn, h[:, 0]) <= ITER_TOL
     cl, cd, cm = s.compute_cl_cd_cm()
     np.testing.assert_allclose([cl, cd, cm], h[-1, 2:], rtol=ITER_TOL, atol=1e-13)
-    assert normwise(s.prim, g["prim"]) <= ITER_TOL
+    e_prim, e_flux = normwise(s.prim, g["prim"]), normwise(s.flux_res, g["flux_res"])
+    print(f"case: k24 20 iterations: prim {e_prim:.2e} flux_res {e_flux:.2e}", flush=True)
+    assert e_prim <= ITER_TOL
+    assert e_flux <= ITER_TOL
 
 
 def test_9760_points_1000_iterations_match_reference():
@@ -159,8 +176,11 @@ def test_9760_points_1000_iterations_match_reference():
         np.testing.assert_allclose([cl, cd], ref[2:4], rtol=ITER_TOL, atol=0)
         assert abs(cm - ref[4]) <= ITER_TOL * max(abs(ref[4]), abs(ref[2]))
     rn_all = np.concatenate(rn_all)
-    assert np.abs(rn_all - h[:, 0]).max() <= ITER_TOL * np.abs(h[:, 0]).max()
-    assert normwise(s.prim, g["prim"]) <= ITER_TOL
+    assert normwise(rn_all, h[:, 0]) <= ITER_TOL
+    e_prim, e_flux = normwise(s.prim, g["prim"]), normwise(s.flux_res, g["flux_res"])
+    print(f"case: 1000 iterations: prim {e_prim:.2e} flux_res {e_flux:.2e}", flush=True)
+    assert e_prim <= ITER_TOL
+    assert e_flux <= ITER_TOL
 
 
 def test_runs_are_deterministic():

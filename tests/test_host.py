@@ -447,13 +447,19 @@ def test_exp_reduction_constants():
         return float(Fraction(a) * Fraction(b) + Fraction(d))
 
     def exp_k(a):
+        if a != a:
+            return a  # both clamps compare, so a NaN passes through
         a = -708.0 if a < -708.0 else a
+        a = 710.0 if a > 710.0 else a
         n = float(round(a * inv_ln2))  # round half to even, as rint
         r = fma(-n, lo, fma(-n, hi, a))
         p = c0
         for k in c:
             p = fma(p, r, k)
-        return math.ldexp(p, int(min(n, 1100.0)))
+        try:
+            return math.ldexp(p, int(min(n, 1100.0)))
+        except OverflowError:  # the device's ldexp overflows to inf
+            return math.inf
 
     worst = 0.0
     for a in np.concatenate([np.linspace(-700.0, 700.0, 4001), np.linspace(-2.0, 2.0, 4001)]):
@@ -462,3 +468,42 @@ def test_exp_reduction_constants():
     assert worst <= 3 * 2.0 ** -53, worst
     assert exp_k(-800.0) == exp_k(-708.0) > 0.0
     assert re.search(r"ldexp\(p, \(int\)fmin\(n, 1100\.0\)\)", body)
+    # the upper end: finite up to the largest double's logarithm, inf past
+    # it and for +inf (not inf - inf = NaN in the reduction)
+    assert re.search(r"a = a < -708\.0 \? -708\.0 : a;", body)
+    assert re.search(r"a = a > 710\.0 \? 710\.0 : a;", body)
+    assert abs(exp_k(709.78) / math.exp(709.78) - 1.0) <= 3 * 2.0 ** -53
+    assert exp_k(709.79) == math.inf and exp_k(710.0) == math.inf
+    assert exp_k(math.inf) == math.inf and exp_k(1e300) == math.inf
+    assert math.isnan(exp_k(math.nan))
+
+
+def test_normwise_is_strict_about_non_finite_values():
+    # the parity measure fails on any NaN and on an infinity the other side
+    # lacks, in every component, and masks equal infinities
+    from conftest import normwise
+    ref = np.arange(1.0, 13.0).reshape(3, 4)
+    assert normwise(ref, ref) == 0.0
+    for r in range(3):
+        bad = ref.copy()
+        bad[r, 2] = np.nan
+        assert normwise(bad, ref) == np.inf, r
+        assert normwise(ref, bad) == np.inf, r
+        one_sided = ref.copy()
+        one_sided[r, 1] = np.inf
+        assert normwise(one_sided, ref) == np.inf, r
+        assert normwise(ref, one_sided) == np.inf, r
+        both = one_sided.copy()
+        assert normwise(both, one_sided) == 0.0, r
+        off = ref.copy()
+        off[r, 3] += 1e-3 * np.abs(ref[r]).max()
+        assert abs(normwise(off, ref) - 1e-3) < 1e-12, r
+    # a later component's error counts as much as the first's
+    shifted = ref.copy()
+    shifted[2] *= 2.0
+    assert normwise(shifted, ref) == 1.0
+    # signs of infinity matter
+    a = np.array([np.inf, 1.0])
+    assert normwise(a, np.array([-np.inf, 1.0])) == np.inf
+    with pytest.raises(ValueError):
+        normwise(np.zeros(3), np.zeros(4))
