@@ -65,16 +65,17 @@ __device__ __forceinline__ float exp_k(float a) {
 // FMAs against ln 2 split into a double and its remainder), e^r from
 // p(r) = 1 + r + c2 r^2 + ... + c11 r^11, a fit in relative error (7e-18;
 // exp_fit.py beside this file derives it), then scaled by 2^n: within an
-// ulp or two.  a is first raised to -708, which keeps 2^n normal in place
-// of the library's underflow tests (e^-708 = 3e-308 stands for 0), and
-// lowered to 710, so that +inf does not turn n ln2 - a into inf - inf;
-// past 709.78 the scaling overflows to inf as the library's does.  Both
-// clamps compare, so a NaN stays NaN.
+// ulp or two.  a is first raised to -708 (a NaN stays NaN: the clamp
+// compares), which keeps 2^n normal in place of the library's underflow
+// tests (e^-708 = 3e-308 stands for 0).  n is capped at 1100, which keeps
+// its conversion to int defined and, for a past 762, leaves r large: every
+// coefficient is positive, so p(r) and then 2^n p overflow to inf, as the
+// library's exp does past 709.78 -- +inf included, where an uncapped n
+// would make n ln2 - a = inf - inf.  A NaN a gives a NaN r and result.
 template <>
 __device__ __forceinline__ double exp_k(double a) {
   a = a < -708.0 ? -708.0 : a;
-  a = a > 710.0 ? 710.0 : a;
-  const double n = rint(a * 1.4426950408889634);
+  const double n = fmin(rint(a * 1.4426950408889634), 1100.0);
   double r = fma(-n, 0.6931471805599453, a);
   r = fma(-n, 2.3190468138462996e-17, r);
   double p = 2.499930047834758e-08;
@@ -89,9 +90,7 @@ __device__ __forceinline__ double exp_k(double a) {
   p = fma(p, r, 0.5000000000000011);
   p = fma(p, r, 1.0);
   p = fma(p, r, 1.0);
-  // n capped so that its conversion to int stays defined (2^1100 overflows
-  // to inf as well; a NaN n leaves p NaN)
-  return ldexp(p, (int)fmin(n, 1100.0));
+  return ldexp(p, (int)n);
 }
 
 // a / b without qdiv's last correction: the reciprocal after its two Newton

@@ -12,6 +12,8 @@ import numpy as np
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "liblskum.so")
 
 LSKUM_OK = 0
+LSKUM_EINVAL, LSKUM_EHIP, LSKUM_ENOMEM, LSKUM_EIO, LSKUM_ENONFINITE, LSKUM_ESTENCIL = (
+    -1, -2, -3, -4, -5, -6)
 ERRORS = {
     -1: "EINVAL", -2: "EHIP", -3: "ENOMEM", -4: "EIO", -5: "ENONFINITE", -6: "ESTENCIL",
 }

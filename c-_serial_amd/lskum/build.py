@@ -16,6 +16,7 @@ CSRC = os.path.join(ROOT, "csrc")
 REPO = os.path.dirname(ROOT)
 INCLUDE = os.path.join(REPO, "include")
 LIB_PATH = os.path.join(PKG_DIR, "liblskum.so")
+FPI_PATH = os.path.join(PKG_DIR, "lskum_fpi")
 BUILD_DIR = os.path.join(ROOT, "_build")
 
 HIP_SOURCES = ["kernels.hip", "kernels_mixed.hip", "adjoint.hip", "trajectory.hip", "api.hip",
@@ -61,8 +62,14 @@ def build(verbose=False):
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-o", tmp, *objs, "-lgomp",
           "-Wl,-rpath,/opt/rocm/lib"])
     os.replace(tmp, LIB_PATH)
+    # the reference's main() as a program over the C ABI (per-stage loop of
+    # include/lskum_fpi.hpp), linked against the library beside it
+    tmp = FPI_PATH + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-I" + INCLUDE, os.path.join(CSRC, "fpi_main.cpp"),
+          "-o", tmp, "-L" + PKG_DIR, "-llskum", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, FPI_PATH)
     if verbose:
-        print("built", LIB_PATH)
+        print("built", LIB_PATH, "and", FPI_PATH)
     return LIB_PATH
 
 
@@ -74,6 +81,8 @@ def build_oracle(with_reference=True):
     ref = os.environ.get("LSKUM_REFERENCE", "/root/reference")
     if with_reference and os.path.isfile(os.path.join(ref, "fpi_solver_mod.h")):
         _run(["make", "-B", "-C", oracle_dir, "ref", "REF=" + ref])
+        if os.path.isfile(LIB_PATH):
+            _run(["make", "-B", "-C", oracle_dir, "dropin", "REF=" + ref])
 
 
 if __name__ == "__main__":

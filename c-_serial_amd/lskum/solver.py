@@ -311,10 +311,7 @@ class Solver:
         """restart.dat of the current state and iteration counter (the
         layout initial_conditions_mod.h:7-21 reads)."""
         it, res_old = self.iteration()
-        view = self.cloud.view()
-        prim = self.prim
-        L.check(L.lib().lskum_restart_write(str(path).encode(), C.byref(view), it, res_old,
-                                            L.dptr(prim)))
+        restart_write(path, self.cloud, it, res_old, self.prim)
 
     def write_solution(self, path):
         """output.dat of the current state with its entropy
@@ -334,12 +331,9 @@ class Solver:
     def read_restart(self, path):
         """Resume from a restart file: its state, iteration counter and
         res_old, with the res_old latch kept (the reference's restart = 1)."""
-        it, ro = C.c_int32(), C.c_double()
-        prim = np.zeros((4, self.n))
-        L.check(L.lib().lskum_restart_read(str(path).encode(), self.n, C.byref(it),
-                                           C.byref(ro), L.dptr(prim)))
+        it, res_old, prim = restart_read(path, self.n)
         self.set_state(prim)
-        self.set_iteration(it.value, ro.value, restart=True)
+        self.set_iteration(it, res_old, restart=True)
 
     def layout(self):
         """(mask of kernels using LDS tiles, tile_max, order) from lskum_create."""
@@ -381,3 +375,21 @@ def kat_qtilde(qt):
     out = np.zeros((n, 4))
     L.check(L.lib().lskum_kat_qtilde(n, L.dptr(qt), L.dptr(out)))
     return out
+
+
+def restart_write(path, cloud, it, res_old, prim):
+    """Host-only writer of a restart file in the layout restart_sol reads
+    (initial_conditions_mod.h:7-21): prim [4][n] in the cloud's order."""
+    view = cloud.view()
+    prim = np.ascontiguousarray(prim, dtype=np.float64).reshape(4, cloud.n)
+    L.check(L.lib().lskum_restart_write(str(path).encode(), C.byref(view), int(it),
+                                        float(res_old), L.dptr(prim)))
+
+
+def restart_read(path, n):
+    """(it, res_old, prim [4][n]) of a restart file."""
+    it, ro = C.c_int32(), C.c_double()
+    prim = np.zeros((4, n))
+    L.check(L.lib().lskum_restart_read(str(path).encode(), n, C.byref(it), C.byref(ro),
+                                       L.dptr(prim)))
+    return it.value, ro.value, prim

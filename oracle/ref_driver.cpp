@@ -19,7 +19,13 @@
 // Usage:
 //   ref_driver run GRID OUTDIR --iters N [--cfl C] [--vl K] [--inner I]
 //              [--mach M --aoa A] [--stages it1,it2,...] [--dump it1,...]
-//              [--diag 1]
+//              [--diag 1] [--print 1] [--restart FILE]
+// --print keeps the reference's own per-iteration line on stdout
+// (fpi_solver_mod.h:150).  --restart copies FILE to ./restart.dat and runs
+// with the reference's restart = 1: initial_conditions() reads it with
+// restart_sol() (initial_conditions_mod.h:7-21) and the loop continues from
+// its iteration count, q_lskum_mod.h:29-37 (--mach/--aoa then set only the
+// free stream, not the state read from the file).
 // --diag runs the reference's diagnostics on the final state: entropy and
 // enstrophy are dumped; stagnation_pressure() prints its own line (its last
 // two fields are ints printed with %f, so only the first four mean anything).
@@ -36,6 +42,7 @@
 #include "compute_entropy_mod.h"
 #include "compute_enstrophy_mod.h"
 #include "stagnation_values_mod.h"
+#include "ref_grid.hpp"
 
 #include <chrono>
 #include <cstdio>
@@ -100,57 +107,12 @@ static std::set<int> parse_list(const char *s) {
   return out;
 }
 
-// partGrid reader with the field order of main.cpp:30-58.
-static int read_grid(const char *path) {
-  std::ifstream fin(path);
-  if (!fin) return -1;
-  int n = 0;
-  fin >> n;
-  if (!fin || n <= 0) return -1;
-  max_points = n;
-  std::vector<std::vector<int>> conn(n + 1);
-  point.x.assign(n + 1, 0.0);
-  point.y.assign(n + 1, 0.0);
-  point.left.assign(n + 1, 0);
-  point.right.assign(n + 1, 0);
-  point.flag_1.assign(n + 1, 0);
-  point.flag_2.assign(n + 1, 0);
-  point.nbhs.assign(n + 1, 0);
-  point.min_dist.assign(n + 1, 0.0);
-  point.nx.assign(n + 1, 0.0);
-  point.ny.assign(n + 1, 0.0);
-  int width = 20;
-  for (int i = 1; i <= n; i++) {
-    fin >> point.x[i] >> point.y[i] >> point.left[i] >> point.right[i] >>
-        point.flag_1[i] >> point.flag_2[i] >> point.min_dist[i] >>
-        point.nbhs[i];
-    conn[i].assign(point.nbhs[i] + 1, 0);
-    for (int r = 1; r <= point.nbhs[i]; r++) fin >> conn[i][r];
-    if (!fin) return -1;
-    if (point.nbhs[i] > width) width = point.nbhs[i];
-    if (point.flag_1[i] == 0) {
-      wall_points++;
-      wall_points_index.push_back(i);
-    } else if (point.flag_1[i] == 1) {
-      interior_points++;
-      interior_points_index.push_back(i);
-    } else if (point.flag_1[i] == 2) {
-      outer_points++;
-      outer_points_index.push_back(i);
-    }
-    if (point.flag_2[i] > 0) shape_points++;
-  }
-  point.conn.assign(n + 1, std::vector<int>(width + 1, 0));
-  for (int i = 1; i <= n; i++)
-    for (int r = 1; r <= point.nbhs[i]; r++) point.conn[i][r] = conn[i][r];
-  return width;
-}
-
 static int cmd_run(int argc, char **argv) {
   if (argc < 4) return 2;
   const char *grid = argv[2];
   g_out = argv[3];
-  int iters = 1, diag = 0;
+  int iters = 1, diag = 0, print = 0;
+  std::string restart_file;
   double mach_o = -1, aoa_o = 0;
   std::set<int> stages, dumps;
   for (int a = 4; a < argc; a++) {
@@ -166,21 +128,24 @@ static int cmd_run(int argc, char **argv) {
     else if (k == "--stages") stages = parse_list(v);
     else if (k == "--dump") dumps = parse_list(v);
     else if (k == "--diag") diag = std::atoi(v);
+    else if (k == "--print") print = std::atoi(v);
+    else if (k == "--restart") restart_file = v;
     else return 2;
   }
   int width = read_grid(grid);
   if (width < 0) { std::fprintf(stderr, "cannot read %s\n", grid); return 3; }
   // main.cpp:59-65
-  allocate_soln();
-  // allocate_soln() sizes the sub-stencil lists for 20 neighbours
-  // (data_structure_mod.h:141-144); widen them for wider clouds.
-  point.xpos_conn.assign(max_points + 1, std::vector<int>(width + 1, 0));
-  point.xneg_conn.assign(max_points + 1, std::vector<int>(width + 1, 0));
-  point.ypos_conn.assign(max_points + 1, std::vector<int>(width + 1, 0));
-  point.yneg_conn.assign(max_points + 1, std::vector<int>(width + 1, 0));
-  local_points = max_points;
+  allocate_soln_wide(width);
   compute_normals();
   generate_connectivity();
+  if (!restart_file.empty()) {
+    std::ifstream src(restart_file, std::ios::binary);
+    std::ofstream dst("restart.dat", std::ios::binary);
+    if (!src || !dst) { std::fprintf(stderr, "cannot copy %s\n", restart_file.c_str()); return 3; }
+    dst << src.rdbuf();
+    dst.close();
+    restart = 1;
+  }
   initial_conditions();
   double force_scale = 1.0;
   if (mach_o > 0) {
@@ -189,8 +154,9 @@ static int cmd_run(int argc, char **argv) {
     q_init[2] = q_inf[2] = mach_o * cos(theta);
     q_init[3] = q_inf[3] = mach_o * sin(theta);
     q_init[4] = q_inf[4] = pr_inf;
-    for (int k = 1; k <= max_points; k++)
-      for (int r = 1; r <= 4; r++) point.prim[r][k] = q_init[r];
+    if (restart == 0)
+      for (int k = 1; k <= max_points; k++)
+        for (int r = 1; r <= 4; r++) point.prim[r][k] = q_init[r];
     force_scale = (0.5 * rho_inf * mach * mach) / (0.5 * rho_inf * mach_o * mach_o);
   }
   {  // sub-stencil lists, 0-based, as counts then members
@@ -212,13 +178,14 @@ static int cmd_run(int argc, char **argv) {
     nrm.insert(nrm.end(), point.ny.begin() + 1, point.ny.end());
     dump_raw("normals", nrm);
   }
-  // q_lskum_mod.h:28-37 with restart == 0
+  // q_lskum_mod.h:28-37
+  if (restart == 0) itr = 0;
   std::vector<double> hist;  // per iteration: res_new, residue, Cl, Cd, Cm
   std::streambuf *saved = std::cout.rdbuf();
   std::ofstream sink("/dev/null");
-  std::cout.rdbuf(sink.rdbuf());
+  if (!print) std::cout.rdbuf(sink.rdbuf());
   const auto t0 = std::chrono::steady_clock::now();
-  for (it = 1; it <= iters; it++) {
+  for (it = itr + 1; it <= itr + iters; it++) {
     if (stages.count(it)) {
       char tag[64];
       // fpi_solver_mod.h:19-61, stage by stage

@@ -52,3 +52,14 @@ def normwise(a, b):
         err = float(np.abs(np.where(differ, x - y, 0.0)).max()) / scale
         worst = max(worst, err)
     return worst
+
+
+def cloud_sha(cloud):
+    """SHA-256 over a cloud's arrays (the fixtures' cloud_sha)."""
+    import hashlib
+    import numpy as np
+    h = hashlib.sha256()
+    for a in (cloud.x, cloud.y, cloud.flag_1, cloud.flag_2, cloud.left, cloud.right,
+              cloud.min_dist, cloud.conn_off, cloud.conn):
+        h.update(np.ascontiguousarray(a).tobytes())
+    return h.hexdigest()

@@ -10,7 +10,8 @@ come from the reference's own functions run by oracle/ref_driver.cpp.
                      row, the eight flux functions and qtilde_to_primitive
   small_m063.npz     naca0012_1480_k20, M 0.63, AoA 2, CFL 0.2, K 5, no inner
                      passes, 40 iterations: history, final arrays,
-                     stage-by-stage flux_res/prim at iteration 30
+                     stage-by-stage flux_res/prim at iteration 30, and
+                     the printed lines of an uninterrupted run
   k24_m085.npz       naca0012_3620_k24, M 0.85, AoA 1.25, 2 inner passes,
                      20 iterations: history, final prim and flux_res
   naca0012_9760_m063_1000it.npz
@@ -21,6 +22,17 @@ come from the reference's own functions run by oracle/ref_driver.cpp.
                      final prim and the reference's diagnostics of it
                      (entropy, vorticity, its square and total enstrophy;
                      the first four values stagnation_pressure() prints)
+  restart_k24_m085.npz
+                     naca0012_3620_k24 as k24_m085: the state after 10
+                     iterations written by the library's restart writer,
+                     and the reference's own restart run from that file
+                     (iterations 11..20: history, final prim and flux_res,
+                     the printed lines)
+  naca0012_2p5m_m085_500it.npz  (--only c2; hours on all host threads)
+                     naca0012_2p5m_k24, M 0.85, AoA 1.25, 2 inner passes,
+                     500 iterations by the oracle (the reference itself
+                     would take about a day on one core): history, forces
+                     every 50 iterations, final-array norms and a sample
 """
 import argparse
 import hashlib
@@ -73,6 +85,10 @@ def kat():
                         fluxes=res[:, :32].reshape(1000, 8, 4), qtilde_prim=res[:, 32:])
 
 
+def printed_lines(text):
+    return np.array([ln for ln in text.splitlines() if ln.startswith("iterations ")])
+
+
 def small_m063():
     cloud = lskum.workload("naca0012_1480_k20")
     n = cloud.n
@@ -87,6 +103,11 @@ def small_m063():
     stage["rk1_dq"] = load("it30_rk1_dq").reshape(2, 4, n)
     stage["rk1_qm"] = load("it30_rk1_qm").reshape(2, 4, n)
     stage["delta30"] = load("it30_delta")
+    # the reference's own per-iteration lines (fpi_solver_mod.h:150) of an
+    # uninterrupted run (the stage-by-stage iteration above prints none)
+    out = []
+    run_ref(cloud, 40, 0.2, 5.0, 0, ("--print", "1"), stdout=out)
+    stage["printed"] = printed_lines(out[0])
     np.savez_compressed(
         os.path.join(HERE, "small_m063.npz"), grid_sha=gsha,
         params=np.array([0.63, 2.0, 0.2, 5.0, 0.0, 0]),
@@ -113,6 +134,33 @@ def k24():
         sub_members_sha=sha(load("substencil_members").astype(np.int32)))
 
 
+def restart_k24():
+    """naca0012_3620_k24 as k24_m085 (M 0.85, 2 inner passes): the reference
+    runs 10 iterations; the library's writer puts that state in a restart
+    file; the reference's own restart path (initial_conditions_mod.h:7-21,
+    q_lskum_mod.h:29-37) reads it and runs iterations 11..20."""
+    from lskum.solver import restart_write
+    cloud = lskum.workload("naca0012_3620_k24")
+    n = cloud.n
+    flow = ("--mach", "0.85", "--aoa", "1.25")
+    d, gsha, load = run_ref(cloud, 10, 0.2, 5.0, 2, flow)
+    h10 = load("history").reshape(-1, 5)
+    prim10 = load("it10_prim").reshape(4, n)
+    res_old = h10[1, 0]  # latched at iteration 2 (fpi_solver_mod.h:139-143)
+    path = os.path.join(d, "restart_written_by_library.dat")
+    restart_write(path, cloud, 10, res_old, prim10)
+    out = []
+    d2, _, load2 = run_ref(cloud, 10, 0.2, 5.0, 2,
+                           flow + ("--restart", path, "--dump", "20", "--print", "1"),
+                           stdout=out)
+    np.savez_compressed(
+        os.path.join(HERE, "restart_k24_m085.npz"), grid_sha=gsha,
+        params=np.array([0.85, 1.25, 0.2, 5.0, 0.0, 2]), it=10, res_old=res_old,
+        prim10=prim10, history=load2("history").reshape(-1, 5),
+        prim=load2("it20_prim").reshape(4, n), flux_res=load2("it20_flux_res").reshape(4, n),
+        printed=printed_lines(out[0]))
+
+
 def long_9760():
     cloud = lskum.workload("naca0012_9760_k20")
     n = cloud.n
@@ -131,14 +179,6 @@ def long_9760():
 C2_PARAMS = dict(mach=0.85, aoa=1.25, cfl=0.2, vl_const=5.0, power=0.0, inner_iterations=2)
 C2_ITERS = 500
 C2_FORCE_EVERY = 50
-
-
-def cloud_sha(cloud):
-    h = hashlib.sha256()
-    for a in (cloud.x, cloud.y, cloud.flag_1, cloud.flag_2, cloud.left, cloud.right,
-              cloud.min_dist, cloud.conn_off, cloud.conn):
-        h.update(np.ascontiguousarray(a).tobytes())
-    return h.hexdigest()
 
 
 def c2_sample(cloud, n_random=4000, seed=2502):
@@ -160,6 +200,8 @@ def c2_500(threads=8, checkpoint_dir=None):
     resumes exactly (the state is all an iteration reads: q, the derivatives
     and delta are recomputed from prim)."""
     sys.path.insert(0, REPO)
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    from conftest import cloud_sha
     from oracle.oracle import Oracle
     cloud = lskum.workload("naca0012_2p5m_k24")
     n = cloud.n
@@ -231,9 +273,10 @@ if __name__ == "__main__":
     if not os.path.isfile(DRIVER) and a.only != "c2":
         sys.exit("build the reference harness first: make -C oracle ref")
     jobs = {"kat": kat, "small": small_m063, "k24": k24, "long": long_9760, "diag": diag,
+            "restart": restart_k24,
             "c2": lambda: c2_500(a.threads, a.checkpoint_dir)}
     todo = (a.only.split(",") if a.only else
-            ["kat", "small", "k24", "diag"] + (["long"] if a.long else []))
+            ["kat", "small", "k24", "diag", "restart"] + (["long"] if a.long else []))
     for j in todo:
         jobs[j]()
         print("wrote", j, flush=True)

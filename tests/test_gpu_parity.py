@@ -17,7 +17,7 @@ import pytest
 import lskum
 from lskum import _lib as L
 from lskum.solver import Solver, kat_fluxes, kat_qtilde
-from conftest import golden, normwise
+from conftest import cloud_sha, golden, normwise
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -56,9 +56,24 @@ def test_device_fluxes_match_reference_known_answers():
     np.testing.assert_array_equal(np.isposinf(got), np.isposinf(ref))
     assert np.isfinite(got[~np.isinf(ref)]).all()
     keep = np.setdiff1d(np.arange(ref.shape[0]), inf_rows)
-    for comp in range(4):  # u1, u2, rho, p, each on every finite row
+    for comp in range(2):  # u1, u2 on every finite row
         assert normwise(got[keep, comp], ref[keep, comp]) <= 1e-14, comp
-    assert normwise(got.T, ref.T) <= 1e-14
+    # rho = exp(arg), arg = q~1 + beta |u|^2 - 2.5 ln beta (q_variables_mod.h:
+    # 245-251), and p = rho / (2 beta): a last-bit change of the argument
+    # moves exp by |arg| ulps in relative terms (the device forms
+    # exp(q~1 + beta |u|^2) beta^-2.5 instead, the same value in real
+    # arithmetic), and the data.dat rows reach arg = 186 (rho = 9e80).  So
+    # rho and p are held, row by row, to 4 (1 + |arg|) ulps of relative error.
+    beta = -qt[:, 3] * 0.5
+    u1, u2 = qt[:, 1] * (0.5 / beta), qt[:, 2] * (0.5 / beta)
+    arg = qt[:, 0] + beta * (u1 * u1 + u2 * u2) - np.log(beta) * 2.5
+    bound = 4.0 * (1.0 + np.abs(arg[keep])) * np.finfo(np.float64).eps
+    for comp in (2, 3):
+        rel = np.abs(got[keep, comp] / ref[keep, comp] - 1.0)
+        worst = float((rel / bound).max())
+        print(f"case: kat qtilde comp {comp}: worst relative error / bound {worst:.3f}",
+              flush=True)
+        assert worst <= 1.0, comp
 
 
 def test_stages_match_oracle_from_a_developed_state():
@@ -237,6 +252,48 @@ def test_full_size_iterations_match_oracle():
     o.compute_cl_cd_cm()
     sc = o.scalars()
     np.testing.assert_allclose([s.Cl, s.Cd], [sc["cl"], sc["cd"]], rtol=1e-11)
+
+
+def test_full_size_500_iterations_match_oracle():
+    # the configuration's own length: 500 iterations at 2.5M points, k = 24,
+    # two inner passes, M 0.85, against the oracle's run of the same cloud
+    # (tests/golden/make_golden.py --only c2): the residual history, Cl, Cd
+    # and Cm every 50 iterations, and the final prim and flux_res -- their
+    # per-component max and L2 norms over all points and their values at
+    # every wall point plus 4,000 others, normwise against the whole array
+    g = golden("naca0012_2p5m_m085_500it.npz")
+    cloud = lskum.workload("naca0012_2p5m_k24")
+    assert cloud_sha(cloud) == str(g["cloud_sha"]), "generator changed"
+    print("case: 500 iterations naca0012_2p5m_k24 vs the oracle's run", flush=True)
+    s = _solver(cloud, g["params"])
+    h = g["history"]
+    rn_all, rs_all = [], []
+    for chunk, (it, cl_o, cd_o, cm_o) in enumerate(g["forces"]):
+        rn, rs = s.q_lskum(50)
+        rn_all.append(rn)
+        rs_all.append(rs)
+        assert s.iteration()[0] == int(it)
+        cl, cd, cm = s.compute_cl_cd_cm()
+        print(f"case: it {int(it)} dCl {abs(cl - cl_o) / abs(cl_o):.2e} "
+              f"dCd {abs(cd - cd_o) / abs(cd_o):.2e}", flush=True)
+        np.testing.assert_allclose([cl, cd], [cl_o, cd_o], rtol=ITER_TOL, atol=0)
+        assert abs(cm - cm_o) <= ITER_TOL * max(abs(cm_o), abs(cl_o))
+    rn_all, rs_all = np.concatenate(rn_all), np.concatenate(rs_all)
+    assert normwise(rn_all, h[:, 0]) <= ITER_TOL
+    np.testing.assert_allclose(rs_all[2:], h[2:, 1], rtol=0, atol=1e-9)
+    prim, fres = s.prim, s.flux_res
+    smp = g["sample"]
+    for name, a, ref, amax, l2 in (("prim", prim, g["prim_sample"], g["prim_absmax"],
+                                    g["prim_l2"]),
+                                   ("flux_res", fres, g["flux_res_sample"],
+                                    g["flux_res_absmax"], g["flux_res_l2"])):
+        assert np.isfinite(a).all(), name
+        e_pts = max(float(np.abs(a[r, smp] - ref[r]).max() / amax[r]) for r in range(4))
+        e_max = float(np.abs(np.abs(a).max(axis=1) - amax).max() / amax.min())
+        e_l2 = float(np.abs(np.sqrt((a * a).sum(axis=1)) / l2 - 1.0).max())
+        print(f"case: 500 iterations {name}: sample {e_pts:.2e} max {e_max:.2e} "
+              f"l2 {e_l2:.2e}", flush=True)
+        assert e_pts <= ITER_TOL and e_max <= ITER_TOL and e_l2 <= ITER_TOL, name
 
 
 def test_restart_resumes_bit_for_bit(tmp_path):

@@ -444,20 +444,23 @@ def test_exp_reduction_constants():
     assert abs(hi + lo - math.log(2)) < 1e-30 + 2 ** -60 and abs(inv_ln2 * math.log(2) - 1) < 2e-16
 
     def fma(a, b, d):
-        return float(Fraction(a) * Fraction(b) + Fraction(d))
+        if not all(math.isfinite(x) for x in (a, b, d)):
+            return a * b + d  # IEEE rules for inf and NaN operands, no rounding involved
+        v = Fraction(a) * Fraction(b) + Fraction(d)
+        try:
+            return float(v)
+        except OverflowError:  # rounds past the largest double: +-inf
+            return math.inf if v > 0 else -math.inf
 
     def exp_k(a):
-        if a != a:
-            return a  # both clamps compare, so a NaN passes through
         a = -708.0 if a < -708.0 else a
-        a = 710.0 if a > 710.0 else a
-        n = float(round(a * inv_ln2))  # round half to even, as rint
+        n = min(float(round(a * inv_ln2)), 1100.0)  # round half to even, as rint
         r = fma(-n, lo, fma(-n, hi, a))
         p = c0
         for k in c:
             p = fma(p, r, k)
         try:
-            return math.ldexp(p, int(min(n, 1100.0)))
+            return math.ldexp(p, int(n))
         except OverflowError:  # the device's ldexp overflows to inf
             return math.inf
 
@@ -467,15 +470,20 @@ def test_exp_reduction_constants():
         worst = max(worst, abs(exp_k(a) / math.exp(a) - 1.0))
     assert worst <= 3 * 2.0 ** -53, worst
     assert exp_k(-800.0) == exp_k(-708.0) > 0.0
-    assert re.search(r"ldexp\(p, \(int\)fmin\(n, 1100\.0\)\)", body)
-    # the upper end: finite up to the largest double's logarithm, inf past
-    # it and for +inf (not inf - inf = NaN in the reduction)
+    # the upper end: n is capped (keeping its conversion to int defined) and
+    # the reduction uses the capped n, so past a = 762 r grows with a;
+    # every coefficient being positive, p(r) then grows without bound and
+    # 2^n p overflows: finite up to the largest double's logarithm, inf past
+    # it, for huge finite a and for +inf (where r = inf - 1100 ln2 = inf,
+    # not the inf - inf of an uncapped n); a NaN a makes r, p and the result
+    # NaN (the lower clamp compares, so it keeps the NaN)
     assert re.search(r"a = a < -708\.0 \? -708\.0 : a;", body)
-    assert re.search(r"a = a > 710\.0 \? 710\.0 : a;", body)
+    assert re.search(r"const double n = fmin\(rint\(a \* [0-9.e+-]+\), 1100\.0\);", body)
+    assert re.search(r"ldexp\(p, \(int\)n\)", body)
+    assert c0 > 0 and all(k > 0 for k in c)
     assert abs(exp_k(709.78) / math.exp(709.78) - 1.0) <= 3 * 2.0 ** -53
-    assert exp_k(709.79) == math.inf and exp_k(710.0) == math.inf
-    assert exp_k(math.inf) == math.inf and exp_k(1e300) == math.inf
-    assert math.isnan(exp_k(math.nan))
+    for a in (709.79, 710.0, 762.0, 763.0, 1e4, 1e300):
+        assert exp_k(a) == math.inf, a
 
 
 def test_normwise_is_strict_about_non_finite_values():
@@ -507,3 +515,38 @@ def test_normwise_is_strict_about_non_finite_values():
     assert normwise(a, np.array([-np.inf, 1.0])) == np.inf
     with pytest.raises(ValueError):
         normwise(np.zeros(3), np.zeros(4))
+
+
+FPI = os.path.join(REPO, "c-_serial_amd", "lskum", "lskum_fpi")
+
+
+def test_fpi_program_usage_and_input_errors(tmp_path):
+    # the per-stage program fails before touching a device on bad usage or
+    # a missing grid
+    assert subprocess.run([FPI], capture_output=True).returncode == 2
+    assert subprocess.run([FPI, "grid", "--bogus", "1"], capture_output=True).returncode == 2
+    proc = subprocess.run([FPI, str(tmp_path / "missing.txt")], capture_output=True, text=True)
+    assert proc.returncode == 3 and "cannot read" in proc.stderr
+
+
+def test_fpi_header_compiles_on_its_own(tmp_path):
+    src = tmp_path / "tu.cpp"
+    src.write_text('#include "lskum_fpi.hpp"\n'
+                   'int run(lskum_ctx *c) { lskum_fpi::Globals g;\n'
+                   '  return lskum_fpi::q_lskum(c, 3, 2, 100, g, stdout); }\n')
+    subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-fsyntax-only",
+                    "-I" + os.path.join(REPO, "include"), str(src)], check=True)
+
+
+def test_reference_binding_builds_against_reference_headers():
+    # include/lskum_fpi_reference.hpp on the reference's own globals: the
+    # reference's main() with the drop-in, compiled from the reference's
+    # headers where they lie and linked against the library
+    ref = os.environ.get("LSKUM_REFERENCE", "/root/reference")
+    if not os.path.isfile(os.path.join(ref, "data_structure_mod.h")):
+        pytest.skip("reference sources absent")
+    subprocess.run(["make", "-B", "-C", os.path.join(REPO, "oracle"), "dropin", "REF=" + ref],
+                   check=True, capture_output=True)
+    exe = os.path.join(REPO, "oracle", "_ref", "fpi_dropin")
+    proc = subprocess.run([exe], capture_output=True, text=True)
+    assert proc.returncode == 2 and "usage" in proc.stderr

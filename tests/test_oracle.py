@@ -7,6 +7,7 @@ properties the mathematics fixes, independently of both.
 """
 import hashlib
 import math
+import os
 
 import numpy as np
 import pytest
@@ -424,3 +425,37 @@ def test_entropy_and_vorticity_closed_forms():
     o.set("prim", np.stack([1 + 0 * x, x, -y, 0.7 + 0 * x]))
     v, v2, tot = o.compute_enstrophy()
     assert np.abs(v).max() < 1e-11
+
+
+def test_reference_restart_from_a_library_written_file_is_exact():
+    # the reference's restart_sol() (initial_conditions_mod.h:7-21) read the
+    # file lskum_restart_write made from its own state after 10 iterations,
+    # and its q_lskum (q_lskum_mod.h:29-37) ran iterations 11..20: bit for
+    # bit its uninterrupted run, so the writer's layout is the one the
+    # reference reads (17 significant digits round-trip exactly)
+    r = golden("restart_k24_m085.npz")
+    k = golden("k24_m085.npz")
+    assert str(r["grid_sha"]) == str(k["grid_sha"])
+    np.testing.assert_array_equal(r["history"], k["history"][10:])
+    np.testing.assert_array_equal(r["prim"], k["prim"])
+    np.testing.assert_array_equal(r["flux_res"], k["flux_res"])
+    assert [ln.split()[1] for ln in r["printed"]] == [str(i) for i in range(11, 21)]
+    # the oracle resumes from the same file, read by the library's reader
+    import lskum
+    from lskum.solver import restart_read, restart_write
+    import tempfile
+    cloud = lskum.workload("naca0012_3620_k24")
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "restart.dat")
+        restart_write(path, cloud, int(r["it"]), float(r["res_old"]), r["prim10"])
+        it, res_old, prim = restart_read(path, cloud.n)
+    assert it == 10 and res_old == float(r["res_old"])
+    np.testing.assert_array_equal(prim, r["prim10"])
+    o = O.Oracle(cloud, mach=0.85, aoa=1.25, cfl=0.2, vl_const=5.0, power=0.0,
+                 inner_iterations=2, threads=4)
+    o.set("prim", prim)
+    o.set_iteration(it, res_old, 1)
+    rn, rs = o.q_lskum(10)
+    np.testing.assert_array_equal(rn, r["history"][:, 0])
+    np.testing.assert_array_equal(rs, r["history"][:, 1])
+    np.testing.assert_array_equal(o.get("prim"), r["prim"])
