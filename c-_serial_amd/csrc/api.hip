@@ -19,6 +19,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -193,8 +194,8 @@ int do_state_update(lskum_ctx *c, int rk, bool with_hist) {
     KTimer t(c, LSKUM_K_STATE);
     launch_state_update(c->dc(), rk, c->prim, c->prim_old, c->flux_res, c->fcs, c->block_case,
                         c->part, c->base, c->mixed ? c->q32 : nullptr, rk == 4, c->stream);
-    launch_reduce(c->part, c->n_cases(), c->seg, c->red, with_hist ? c->hist : nullptr,
-                  c->hist_cap, 0, c->counter, c->stream);
+    launch_reduce(c->part, c->n_cases(), c->seg, c->red, with_hist ? c->hist_dst : nullptr,
+                  c->hist_stride, 0, c->counter, c->stream);
   }
   c->q_fresh = true;
   c->speed_fresh = rk == 4;
@@ -637,6 +638,13 @@ int create_impl(const lskum_cloud *clouds, int32_t n_clouds, const lskum_case *c
 #define ALLOC(ptr, count)                                                              \
   if ((e = dalloc(&(ptr), (count))) != hipSuccess)                                     \
     return fail(LSKUM_ENOMEM, std::string("hipMalloc " #ptr ": ") + hipGetErrorString(e));
+  if ((e = hipGetDevice(&c->device)) != hipSuccess)
+    return fail(LSKUM_EHIP, std::string("hipGetDevice: ") + hipGetErrorString(e));
+  {
+    std::string serr;
+    const int src = noalloc_scratch_init(c->device, serr);
+    if (src) return fail(src, serr);
+  }
   if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess)
     return fail(LSKUM_EHIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
   ALLOC(c->base, (size_t)B_STRIDE * n);
@@ -850,6 +858,107 @@ int download_caller(lskum_ctx *c, const double *dev, double *dst) {
 
 }  // namespace lskum
 
+namespace lskum {
+
+// The scratch array of lskum_iterate_noalloc: the per-iteration residual
+// sums of the running call, [n_cases][n_iter].  Each device holds its own
+// copy; host code reaches a device's copy only through the address
+// hipGetSymbolAddress gave for it (the array's name is a host-side stand-in).
+static __device__ __align__(256) double scratch[LSKUM_NOALLOC_HIST_MAX];
+
+constexpr int kMaxDevices = 64;
+static double *g_scratch_ptr[kMaxDevices] = {};
+static std::mutex g_scratch_mu;
+
+// Looks up the current device's copy once per device (lskum_create calls it
+// with the context's device current).
+int noalloc_scratch_init(int device, std::string &err) {
+  if (device < 0 || device >= kMaxDevices) {
+    err = "device index " + std::to_string(device) + " beyond the scratch table";
+    return LSKUM_EINVAL;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device]) return LSKUM_OK;
+  void *p = nullptr;
+  hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) {
+    err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+    return LSKUM_EHIP;
+  }
+  g_scratch_ptr[device] = (double *)p;
+  return LSKUM_OK;
+}
+
+double *noalloc_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  return g_scratch_ptr[device];
+}
+
+// n_iter whole iterations from the current state; stage-4 sums go to
+// hist_dst (stride hist_stride).  Graph replays only when allowed (the
+// first replay of a new graph instantiates it).
+int launch_iterations(lskum_ctx *ctx, int n_iter, bool allow_graph) {
+  int rc = LSKUM_OK;
+  // q and the speed of the current state: from the last stage-4 update, else
+  // computed here
+  if (!ctx->speed_fresh) rc = do_speed(ctx);
+  if (!rc && !ctx->q_fresh) rc = do_q_variables(ctx);
+  if (rc) return rc;
+  launch_set_int(ctx->counter, 0, ctx->stream);
+  int done = 0;
+  const int K = ctx->graph_iters;
+  if (allow_graph && K > 0 && n_iter >= K && !ctx->timing) {
+    if (!ctx->gexec || ctx->gexec_iters != K || ctx->gexec_hist != ctx->hist)
+      rc = capture_graph(ctx, K);
+    for (; !rc && done + K <= n_iter; done += K) {
+      hipError_t e = hipGraphLaunch(ctx->gexec, ctx->stream);
+      if (e != hipSuccess) rc = set_err(ctx, LSKUM_EHIP, std::string("hipGraphLaunch: ") + hipGetErrorString(e));
+    }
+  }
+  for (; !rc && done < n_iter; done++) rc = run_iteration(ctx);
+  return rc;
+}
+
+// Residual bookkeeping of n_iter iterations (fpi_solver_mod.h:138-147) from
+// their stage-4 sums, sums[case * n_iter + t]; res_new may be sums itself
+// (each entry is read before it is overwritten).
+int finish_iterations(lskum_ctx *ctx, int n_iter, const double *sums, double *res_new,
+                      double *residue) {
+  const int nc = ctx->n_cases();
+  int bad = -1;
+  for (int t = 0; t < n_iter; t++) {
+    for (int j = 0; j < nc; j++)
+      if (!std::isfinite(sums[(size_t)j * n_iter + t]) && !ctx->cases[j].failed_at)
+        ctx->cases[j].failed_at = ctx->it + 1 + t;
+    const int it = ctx->it + 1;
+    for (int j = 0; j < nc; j++) {
+      CaseInfo &cs = ctx->cases[j];
+      const double s = sums[(size_t)j * n_iter + t];
+      const double rn = std::sqrt(s) / cs.n;
+      double rs;
+      if (it <= 2 && ctx->restart == 0) {
+        cs.res_old = rn;
+        rs = 0;
+      } else {
+        rs = std::log10(rn / cs.res_old);
+      }
+      if (res_new) res_new[(size_t)j * n_iter + t] = rn;
+      if (residue) residue[(size_t)j * n_iter + t] = rs;
+      if (!std::isfinite(s) && bad < 0) bad = j;
+    }
+    ctx->it = it;
+  }
+  // A batch's cases share no points, stencils or reductions, so a case that
+  // diverges cannot disturb the others: the batch goes on and
+  // lskum_case_status names the failed cases.  A single case fails the call.
+  if (bad >= 0 && nc == 1)
+    return set_err(ctx, LSKUM_ENONFINITE, "residual became non-finite");
+  return LSKUM_OK;
+}
+
+}  // namespace lskum
+
 extern "C" {
 
 const char *lskum_last_error(const lskum_ctx *ctx) {
@@ -1011,24 +1120,9 @@ int lskum_iterate(lskum_ctx *ctx, int32_t n_iter, double *res_new, double *resid
     HIP_TRY(ctx, dalloc(&ctx->hist, (size_t)n_iter * nc));
     ctx->hist_cap = n_iter;
   }
-  int rc = LSKUM_OK;
-  // q and the speed of the current state: from the last stage-4 update, else
-  // computed here
-  if (!ctx->speed_fresh) rc = do_speed(ctx);
-  if (!rc && !ctx->q_fresh) rc = do_q_variables(ctx);
-  if (rc) return rc;
-  launch_set_int(ctx->counter, 0, ctx->stream);
-  int done = 0;
-  const int K = ctx->graph_iters;
-  if (K > 0 && n_iter >= K && !ctx->timing) {
-    if (!ctx->gexec || ctx->gexec_iters != K || ctx->gexec_hist != ctx->hist)
-      rc = capture_graph(ctx, K);
-    for (; !rc && done + K <= n_iter; done += K) {
-      hipError_t e = hipGraphLaunch(ctx->gexec, ctx->stream);
-      if (e != hipSuccess) rc = set_err(ctx, LSKUM_EHIP, std::string("hipGraphLaunch: ") + hipGetErrorString(e));
-    }
-  }
-  for (; !rc && done < n_iter; done++) rc = run_iteration(ctx);
+  ctx->hist_dst = ctx->hist;
+  ctx->hist_stride = ctx->hist_cap;
+  int rc = launch_iterations(ctx, n_iter, true);
   if (rc) return rc;
   std::vector<double> sums((size_t)nc * n_iter);
   HIP_TRY(ctx, hipMemcpy2DAsync(sums.data(), sizeof(double) * n_iter, ctx->hist,
@@ -1037,36 +1131,37 @@ int lskum_iterate(lskum_ctx *ctx, int32_t n_iter, double *res_new, double *resid
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   rc = collect_timing(ctx);
   if (rc) return rc;
-  int bad = -1;
-  for (int t = 0; t < n_iter; t++) {
-    for (int j = 0; j < nc; j++)
-      if (!std::isfinite(sums[(size_t)j * n_iter + t]) && !ctx->cases[j].failed_at)
-        ctx->cases[j].failed_at = ctx->it + 1 + t;
-    // fpi_solver_mod.h:138-147
-    const int it = ctx->it + 1;
-    for (int j = 0; j < nc; j++) {
-      CaseInfo &cs = ctx->cases[j];
-      const double s = sums[(size_t)j * n_iter + t];
-      const double rn = std::sqrt(s) / cs.n;
-      double rs;
-      if (it <= 2 && ctx->restart == 0) {
-        cs.res_old = rn;
-        rs = 0;
-      } else {
-        rs = std::log10(rn / cs.res_old);
-      }
-      if (res_new) res_new[(size_t)j * n_iter + t] = rn;
-      if (residue) residue[(size_t)j * n_iter + t] = rs;
-      if (!std::isfinite(s) && bad < 0) bad = j;
-    }
-    ctx->it = it;
-  }
-  // A batch's cases share no points, stencils or reductions, so a case that
-  // diverges cannot disturb the others: the batch goes on and
-  // lskum_case_status names the failed cases.  A single case fails the call.
-  if (bad >= 0 && nc == 1)
-    return set_err(ctx, LSKUM_ENONFINITE, "residual became non-finite");
-  return LSKUM_OK;
+  return finish_iterations(ctx, n_iter, sums.data(), res_new, residue);
+}
+
+int lskum_iterate_noalloc(lskum_ctx *ctx, int32_t n_iter, double *res_new, double *residue) {
+  CTX_OK(ctx);
+  if (n_iter < 0) return set_err(ctx, LSKUM_EINVAL, "n_iter < 0");
+  if (n_iter == 0) return LSKUM_OK;
+  if (!res_new)
+    return set_err(ctx, LSKUM_EINVAL, "lskum_iterate_noalloc: res_new is required");
+  if (ctx->timing)
+    return set_err(ctx, LSKUM_EINVAL,
+                   "lskum_iterate_noalloc: kernel timing creates events; disable it first");
+  const size_t need = (size_t)n_iter * ctx->n_cases();
+  if (need > (size_t)LSKUM_NOALLOC_HIST_MAX)
+    return set_err(ctx, LSKUM_EINVAL,
+                   "lskum_iterate_noalloc: n_iter x n_cases = " + std::to_string(need) +
+                       " exceeds the scratch array's " +
+                       std::to_string(LSKUM_NOALLOC_HIST_MAX) + " entries");
+  double *scratch = noalloc_scratch(ctx->device);
+  if (!scratch)
+    return set_err(ctx, LSKUM_EHIP, "lskum_iterate_noalloc: scratch array not initialised");
+  ctx->hist_dst = scratch;
+  ctx->hist_stride = n_iter;
+  // every slot [case][t] is written by iteration t's stage-4 reduction
+  // before it is read, so nothing needs clearing
+  int rc = launch_iterations(ctx, n_iter, false);
+  if (rc) return rc;
+  HIP_TRY(ctx, hipMemcpyAsync(res_new, scratch, sizeof(double) * need, hipMemcpyDeviceToHost,
+                              ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return finish_iterations(ctx, n_iter, res_new, res_new, residue);
 }
 
 int lskum_case_status(lskum_ctx *ctx, int32_t *failed_at) {

@@ -87,6 +87,12 @@ struct lskum_ctx {
   double *hist = nullptr;     // [n_cases][hist_cap] per-iteration sum_res_sqr
   int hist_cap = 0;
   int32_t *counter = nullptr; // iteration slot of hist within an lskum_iterate call
+  // where the stage-4 reductions of the running call record their sums:
+  // hist with stride hist_cap (lskum_iterate) or the library's static
+  // scratch array (lskum_iterate_noalloc)
+  double *hist_dst = nullptr;
+  int hist_stride = 0;
+  int device = 0;             // the device current at create (owns stream and buffers)
   double *red_h = nullptr;    // pinned host mirror of red / forces
   int32_t *f_m = nullptr, *f_l = nullptr, *f_r = nullptr, *f_s = nullptr;
   double *forces = nullptr;   // [n_cases][3] H, V, pitch moment
@@ -189,6 +195,13 @@ void compute_coef(const CloudLayout &L, const double *x, const double *y, const 
 int prepare_adjoint(lskum_ctx *c);
 int recompute_stages(lskum_ctx *c);
 int upload_caller(lskum_ctx *c, const double *src, double *dev);
+// lskum_iterate_noalloc's static scratch array: the address of the
+// device's copy, looked up once per device (init) and then only read
+int noalloc_scratch_init(int device, std::string &err);
+double *noalloc_scratch(int device);
+int launch_iterations(lskum_ctx *ctx, int n_iter, bool allow_graph);
+int finish_iterations(lskum_ctx *ctx, int n_iter, const double *sums, double *res_new,
+                      double *residue);
 int download_caller(lskum_ctx *c, const double *dev, double *dst);
 
 }  // namespace lskum

@@ -105,6 +105,7 @@ _SIGS = [
     ("lskum_state_update", C.c_int, [C.c_void_p, C.c_int32, _dp, _dp, _ip]),
     ("lskum_forces", C.c_int, [C.c_void_p, _dp, _dp, _dp]),
     ("lskum_iterate", C.c_int, [C.c_void_p, C.c_int32, _dp, _dp]),
+    ("lskum_iterate_noalloc", C.c_int, [C.c_void_p, C.c_int32, _dp, _dp]),
     ("lskum_get_iteration", C.c_int, [C.c_void_p, _ip, _dp]),
     ("lskum_set_iteration", C.c_int, [C.c_void_p, C.c_int32, C.c_double, C.c_int32]),
     ("lskum_restart_write", C.c_int, [C.c_char_p, C.POINTER(CloudView), C.c_int32, C.c_double,

@@ -243,6 +243,28 @@ int lskum_forces(lskum_ctx *ctx, double *cl, double *cd, double *cm);
 int lskum_iterate(lskum_ctx *ctx, int32_t n_iter, double *res_new,
                   double *residue);
 /*
+ * lskum_iterate without any allocation (no reference counterpart: it is for
+ * callers that cannot allocate device memory -- tight serving loops,
+ * harnesses that forbid hipMalloc, code around graph capture).  The same
+ * kernels as lskum_iterate, launched directly (no graph is instantiated);
+ * the per-iteration residual sums go to a static device array of the
+ * library, LSKUM_NOALLOC_HIST_MAX doubles (1 MiB) per device, held until
+ * the process ends, instead of a buffer the context allocates.  Limits and
+ * rules:
+ *  - n_iter x n_cases <= LSKUM_NOALLOC_HIST_MAX, else LSKUM_EINVAL and
+ *    nothing runs;
+ *  - res_new is required (the sums are copied into it and turned into the
+ *    residuals in place); residue may be NULL;
+ *  - kernel timing (lskum_set_timing) must be off, else LSKUM_EINVAL;
+ *  - every context on one device shares that device's array: calls that
+ *    overlap in time on one device (two host threads, or two streams) race
+ *    and give wrong residuals; the state itself is per context.
+ * Results equal lskum_iterate's bit for bit.
+ */
+#define LSKUM_NOALLOC_HIST_MAX 131072
+int lskum_iterate_noalloc(lskum_ctx *ctx, int32_t n_iter, double *res_new,
+                          double *residue);
+/*
  * Run whole iterations as captured HIP graphs of iters_per_graph iterations
  * each (0: launch every kernel from the host, the default).  Graphs are
  * captured on first use and replayed; kernel timing (lskum_set_timing)
