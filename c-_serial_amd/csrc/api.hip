@@ -54,7 +54,8 @@ void release(lskum_ctx *c) {
                   c->flag, c->conn_off, c->perm_d, c->conn, c->part.sum, c->part.max,
                   c->part.arg, c->red, c->hist, c->f_m, c->f_l, c->f_r, c->f_s, c->forces,
                   c->xy, c->q32, c->dq32[0], c->dq32[1], c->qm32, c->eps32, c->fcs,
-                  c->block_case, c->seg, c->soff, c->qdyn, c->counter, c->qsum};
+                  c->block_case, c->seg, c->soff, c->qdyn, c->counter, c->qsum,
+                  c->slice_off, c->connS, c->conn16S};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   void *aptrs[] = {c->adj.own_q, c->adj.own_dq, c->adj.own_qm, c->adj.edge, c->adj.lq,
@@ -583,6 +584,47 @@ int create_impl(const lskum_cloud *clouds, int32_t n_clouds, const lskum_case *c
       for (int32_t k : extra) loc[k] = -1;
     }
   }
+  // The lists in slices for the forward stencil kernels (entry e of the
+  // lane t of stencil block blk at slice_off[blk] + e * TB + t), each list
+  // padded with its last entry to its block's longest; a point without
+  // neighbours (batch padding) points at itself.
+  std::vector<int32_t> slice_h(nblk, 0);
+  std::vector<uint32_t> connS_h;
+  std::vector<uint16_t> conn16S_h;
+  {
+    int64_t total_s = 0;
+    for (int blk = 0; blk < nblk; blk++) {
+      const int32_t p0 = blk * TB, p1 = std::min(n, p0 + TB);
+      int32_t width = 0;
+      for (int32_t d = p0; d < p1; d++) width = std::max(width, off_h[d + 1] - off_h[d]);
+      if (total_s > (int64_t)INT32_MAX) break;
+      slice_h[blk] = (int32_t)total_s;
+      total_s += (int64_t)width * TB;
+    }
+    if (total_s > (int64_t)INT32_MAX)
+      return fail(LSKUM_EINVAL, "neighbour slices exceed 31-bit entry indices");
+    connS_h.assign((size_t)total_s, 0u);
+    conn16S_h.assign((size_t)total_s, 0);
+    for (int blk = 0; blk < nblk; blk++) {
+      const int32_t p0 = blk * TB, p1 = std::min(n, p0 + TB);
+      const int64_t width = (blk + 1 < nblk ? slice_h[blk + 1] : total_s) - slice_h[blk];
+      const int32_t w = (int32_t)(width / TB);
+      for (int32_t d = p0; d < p1; d++) {
+        const int32_t cnt = off_h[d + 1] - off_h[d];
+        for (int32_t e = 0; e < w; e++) {
+          const size_t at = (size_t)slice_h[blk] + (size_t)e * TB + (d - p0);
+          if (cnt == 0) {
+            connS_h[at] = (uint32_t)d;
+            conn16S_h[at] = (uint16_t)(d - p0);
+            continue;
+          }
+          const int32_t src = off_h[d] + std::min(e, cnt - 1);
+          connS_h[at] = conn_d[src];
+          conn16S_h[at] = tiles_ok ? conn16_h[src] : (uint16_t)0;
+        }
+      }
+    }
+  }
   // The gather kernels stage neighbours in LDS.  The flux stages part of
   // each neighbour record (FP64: dq, qmax/qmin; mixed: all but the
   // coordinates), at most flux_cap entries of a tile: as many as let a CU
@@ -672,6 +714,9 @@ int create_impl(const lskum_cloud *clouds, int32_t n_clouds, const lskum_case *c
   ALLOC(c->conn_off, (size_t)n + 1);
   ALLOC(c->perm_d, (size_t)n);
   ALLOC(c->conn, conn_d.size());
+  ALLOC(c->slice_off, slice_h.size());
+  ALLOC(c->connS, connS_h.size());
+  ALLOC(c->conn16S, conn16S_h.size());
   if (c->tiles) {
     ALLOC(c->conn16, conn16_h.size());
     ALLOC(c->tile_off, toff.size());
@@ -707,6 +752,9 @@ int create_impl(const lskum_cloud *clouds, int32_t n_clouds, const lskum_case *c
                          {c->conn_off, off_h.data(), (n + 1) * sizeof(int32_t)},
                          {c->perm_d, perm_h.data(), n * sizeof(int32_t)},
                          {c->conn, conn_d.data(), conn_d.size() * sizeof(uint32_t)},
+                         {c->slice_off, slice_h.data(), slice_h.size() * sizeof(int32_t)},
+                         {c->connS, connS_h.data(), connS_h.size() * sizeof(uint32_t)},
+                         {c->conn16S, conn16S_h.data(), conn16S_h.size() * sizeof(uint16_t)},
                          {c->f_m, fm.data(), fm.size() * sizeof(int32_t)},
                          {c->f_l, fl.data(), fl.size() * sizeof(int32_t)},
                          {c->f_r, fr.data(), fr.size() * sizeof(int32_t)},

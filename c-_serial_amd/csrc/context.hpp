@@ -71,6 +71,9 @@ struct lskum_ctx {
   int32_t *conn_off = nullptr, *perm_d = nullptr;
   uint32_t *conn = nullptr;
   uint16_t *conn16 = nullptr;
+  int32_t *slice_off = nullptr;  // the lists in slices (DevCloud)
+  uint32_t *connS = nullptr;
+  uint16_t *conn16S = nullptr;
   int32_t *tile_off = nullptr, *tile_idx = nullptr;
   int tile_max = 0;
   unsigned tiles = 0;
@@ -125,6 +128,9 @@ struct lskum_ctx {
     c.conn_off = conn_off;
     c.conn = conn;
     c.conn16 = conn16;
+    c.slice_off = slice_off;
+    c.connS = connS;
+    c.conn16S = conn16S;
     c.tile_off = tile_off;
     c.tile_idx = tile_idx;
     c.tile_max = tile_max;

@@ -98,6 +98,35 @@ __device__ __forceinline__ void load_tile(double *lds, const int32_t *__restrict
   }
 }
 
+// A lane's neighbour list in its block's slice.  The forward stencil
+// kernels read the lists from slices: the host lays out the lists of a
+// stencil block's points entry by entry, entry e of the block's lane t at
+// slice_off[block] + e * tile_block + t, so that the 64 lanes of a wave
+// read 64 consecutive entries (one or two cache lines) per neighbour step
+// where the CSR lists, 24 entries apart per lane, touched a cache line per
+// lane or two.  Lists shorter than their block's longest are padded with
+// their last entry.  Global form (device index | mask) and tiled form
+// (local index | mask) have the same layout.
+struct Slice {
+  const uint32_t *c32;
+  const uint16_t *c16;
+  int stride;
+  __device__ __forceinline__ uint32_t w32(int e) const { return c32[(size_t)e * stride]; }
+  __device__ __forceinline__ uint32_t w16(int e) const {
+    return (uint32_t)c16[(size_t)e * stride];
+  }
+  __device__ __forceinline__ uint32_t raw(int e, bool tiled) const {
+    return tiled ? w16(e) : w32(e);
+  }
+};
+
+// The slice of the calling lane of stencil block b.
+__device__ __forceinline__ Slice slice_of(const int32_t *__restrict__ slice_off,
+                                          const uint32_t *conn, const uint16_t *conn16, int b) {
+  const size_t s = (size_t)slice_off[b] + threadIdx.x;
+  return {conn + s, conn16 + s, (int)blockDim.x};
+}
+
 // The raw conn entry e (tiled: the 16-bit local form) and its parts.  The
 // stencil kernels load the entry of neighbour e + 2 while neighbour e is
 // processed and split it only when it is used, so no instruction waits on
@@ -132,24 +161,22 @@ __device__ __forceinline__ uint32_t conn_mask(uint32_t w, bool tiled) {
 template <int DEPTH>
 struct ConnAhead {
   uint32_t w[DEPTH] = {};
-  __device__ __forceinline__ static uint32_t load(const uint32_t *conn, const uint16_t *conn16,
-                                                  bool tiled, int e) {
-    return tiled ? (uint32_t)__hip_atomic_load(conn16 + e, __ATOMIC_RELAXED,
+  __device__ __forceinline__ static uint32_t load(const Slice &l, bool tiled, int e) {
+    return tiled ? (uint32_t)__hip_atomic_load(l.c16 + (size_t)e * l.stride, __ATOMIC_RELAXED,
                                                __HIP_MEMORY_SCOPE_WAVEFRONT)
-                 : __hip_atomic_load(conn + e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+                 : __hip_atomic_load(l.c32 + (size_t)e * l.stride, __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_WAVEFRONT);
   }
   // entries first .. first + DEPTH - 1; first - 1 <= last
-  __device__ __forceinline__ void start(const uint32_t *conn, const uint16_t *conn16,
-                                        bool tiled, int first, int last) {
+  __device__ __forceinline__ void start(const Slice &l, bool tiled, int first, int last) {
 #pragma unroll
-    for (int d = 0; d < DEPTH; d++) w[d] = load(conn, conn16, tiled, min(first + d, last));
+    for (int d = 0; d < DEPTH; d++) w[d] = load(l, tiled, min(first + d, last));
   }
-  __device__ __forceinline__ uint32_t pop(const uint32_t *conn, const uint16_t *conn16,
-                                          bool tiled, int e_load, int last) {
+  __device__ __forceinline__ uint32_t pop(const Slice &l, bool tiled, int e_load, int last) {
     const uint32_t head = w[0];
 #pragma unroll
     for (int d = 0; d + 1 < DEPTH; d++) w[d] = w[d + 1];
-    w[DEPTH - 1] = load(conn, conn16, tiled, min(e_load, last));
+    w[DEPTH - 1] = load(l, tiled, min(e_load, last));
     return head;
   }
 };

@@ -51,6 +51,7 @@ __global__ __launch_bounds__(256) void k_speed(int n, const double *__restrict__
 // state_update_mod.h:448-496: min over the neighbours' states only.
 template <bool TILED>
 __global__ __launch_bounds__(256) void k_func_delta(int n, const int32_t *__restrict__ off,
+                                                    const int32_t *__restrict__ slice_off,
                                                     const uint32_t *__restrict__ conn,
                                                     const uint16_t *__restrict__ conn16,
                                                     const int32_t *__restrict__ tile_off,
@@ -81,19 +82,20 @@ __global__ __launch_bounds__(256) void k_func_delta(int n, const int32_t *__rest
     xyk = reinterpret_cast<const double2 *>(hk)[0];
     speed = TILED ? hk[2] : hk[B_SPEED];
   };
-  const int e0 = off[i], last = off[i + 1] - 1;
+  const Slice L = slice_of(slice_off, conn, conn16, b);
+  const int e0 = 0, last = off[i + 1] - off[i] - 1;
   double2 xyk = {0.0, 0.0};
   double speed = 1.0;
   uint32_t w_n = 0;
   if (e0 <= last) {
-    fetch(conn_raw(conn, conn16, e0, TILED), xyk, speed);
-    w_n = conn_raw(conn, conn16, min(e0 + 1, last), TILED);
+    fetch(L.raw(e0, TILED), xyk, speed);
+    w_n = L.raw(min(e0 + 1, last), TILED);
   }
   for (int e = e0; e <= last; e++) {
     double2 xy_n;
     double speed_n;
     fetch(w_n, xy_n, speed_n);
-    w_n = conn_raw(conn, conn16, min(e + 2, last), TILED);
+    w_n = L.raw(min(e + 2, last), TILED);
     double dist = (xyk.x - xy.x) * (xyk.x - xy.x) + (xyk.y - xy.y) * (xyk.y - xy.y);
     dist = sqrt(dist);
     double delta_t = dist / speed;
@@ -117,7 +119,8 @@ __global__ __launch_bounds__(256) void k_func_delta(int n, const int32_t *__rest
 // the least-squares reverse needs.
 template <bool TILED, bool POW, bool ARG = false>
 __global__ __launch_bounds__(256) void k_q_derivatives(
-    int n, const int32_t *__restrict__ off, const uint32_t *__restrict__ conn,
+    int n, const int32_t *__restrict__ off, const int32_t *__restrict__ slice_off,
+    const uint32_t *__restrict__ conn,
     const uint16_t *__restrict__ conn16, const int32_t *__restrict__ tile_off,
     const int32_t *__restrict__ tile_idx, const double *__restrict__ base,
     double *__restrict__ dq_out, double *__restrict__ qm_out, double *__restrict__ qs_out,
@@ -148,21 +151,22 @@ __global__ __launch_bounds__(256) void k_q_derivatives(
     xyk = reinterpret_cast<const double2 *>(hk)[0];
     load4(hk + B_Q, qk);
   };
-  const int e0 = off[i], last = off[i + 1] - 1;
+  const Slice L = slice_of(slice_off, conn, conn16, b);
+  const int e0 = 0, last = off[i + 1] - off[i] - 1;
   double2 xyk = {0.0, 0.0};
   double qk[4] = {0.0, 0.0, 0.0, 0.0};
   ConnAhead<kAheadQDeriv> ahead;
   uint32_t w_c = 0;
   if (e0 <= last) {
-    w_c = conn_raw(conn, conn16, e0, TILED);
+    w_c = L.raw(e0, TILED);
     fetch(w_c, xyk, qk);
-    ahead.start(conn, conn16, TILED, e0 + 1, last);
+    ahead.start(L, TILED, e0 + 1, last);
   }
 #pragma unroll kAheadQDeriv
   for (int e = e0; e <= last; e++) {
     double2 xy_n;
     double q_n[4];
-    const uint32_t w_n = ahead.pop(conn, conn16, TILED, e + 1 + kAheadQDeriv, last);
+    const uint32_t w_n = ahead.pop(L, TILED, e + 1 + kAheadQDeriv, last);
     fetch(w_n, xy_n, q_n);
     const uint32_t w_k = w_c;
     w_c = w_n;
@@ -241,7 +245,8 @@ __global__ __launch_bounds__(256) void k_q_derivatives(
 // 112 B with q, which lets three blocks share a CU instead of two).
 template <bool TILED, bool POW>
 __global__ __launch_bounds__(256) void k_q_inner(
-    int n, const int32_t *__restrict__ off, const uint32_t *__restrict__ conn,
+    int n, const int32_t *__restrict__ off, const int32_t *__restrict__ slice_off,
+    const uint32_t *__restrict__ conn,
     const uint16_t *__restrict__ conn16, const int32_t *__restrict__ tile_off,
     const int32_t *__restrict__ tile_idx, const double *__restrict__ xy,
     const double *__restrict__ qs, const double *__restrict__ src, double *__restrict__ dst,
@@ -277,19 +282,20 @@ __global__ __launch_bounds__(256) void k_q_inner(
     load4(dk, dxk);
     load4(dk + 4, dyk);
   };
-  const int e0 = off[i], last = off[i + 1] - 1;
+  const Slice L = slice_of(slice_off, conn, conn16, b);
+  const int e0 = 0, last = off[i + 1] - off[i] - 1;
   double2 xyk = {0.0, 0.0};
   double dxk[4] = {}, dyk[4] = {};
   ConnAhead<kAheadQInner> ahead;
   if (e0 <= last) {
-    fetch(conn_raw(conn, conn16, e0, TILED), xyk, dxk, dyk);
-    ahead.start(conn, conn16, TILED, e0 + 1, last);
+    fetch(L.raw(e0, TILED), xyk, dxk, dyk);
+    ahead.start(L, TILED, e0 + 1, last);
   }
 #pragma unroll kAheadQInner
   for (int e = e0; e <= last; e++) {
     double2 xy_n;
     double dx_n[4], dy_n[4];
-    fetch(ahead.pop(conn, conn16, TILED, e + 1 + kAheadQInner, last), xy_n, dx_n, dy_n);
+    fetch(ahead.pop(L, TILED, e + 1 + kAheadQInner, last), xy_n, dx_n, dy_n);
     const double delx = xyk.x - x_i, dely = xyk.y - y_i;
     const double weights = POW ? pow(sqrt(delx * delx + dely * dely), power) : 1.0;
     if (POW) {
@@ -419,7 +425,8 @@ __device__ __forceinline__ void flux_tile_entry(const double2 *lds, int r, doubl
 
 template <bool TILED, bool POW>
 __global__ __launch_bounds__(256) void k_flux(
-    int n, const int32_t *__restrict__ off, const uint32_t *__restrict__ conn,
+    int n, const int32_t *__restrict__ off, const int32_t *__restrict__ slice_off,
+    const uint32_t *__restrict__ conn,
     const uint16_t *__restrict__ conn16, const int32_t *__restrict__ tile_off,
     const int32_t *__restrict__ tile_idx, int cap, const double *__restrict__ base,
     const double *__restrict__ dq, const double *__restrict__ qm,
@@ -469,24 +476,25 @@ __global__ __launch_bounds__(256) void k_flux(
   // computed, the base record of e + 1 and the conn entries of e + 2 are in
   // flight (indices clamped to the last entry), so their L2 latency hides
   // behind a whole neighbour's arithmetic.
-  const int e0 = off[i], last = off[i + 1] - 1;
+  const Slice L = slice_of(slice_off, conn, conn16, b);
+  const int e0 = 0, last = off[i + 1] - off[i] - 1;
   uint32_t w_c = 0, w_n = 0;
   uint16_t l_c = 0, l_n = 0;
   BaseRec r_c = {};
   if (e0 <= last) {
-    w_c = conn[e0];
-    w_n = conn[min(e0 + 1, last)];
+    w_c = L.w32(e0);
+    w_n = L.w32(min(e0 + 1, last));
     if (TILED) {
-      l_c = conn16[e0];
-      l_n = conn16[min(e0 + 1, last)];
+      l_c = L.w16(e0);
+      l_n = L.w16(min(e0 + 1, last));
     }
     r_c = load_base(base, w_c & kIndexMask);
   }
   for (int e = e0; e <= last; e++) {
     const BaseRec r_n = load_base(base, w_n & kIndexMask);
     const int enn = min(e + 2, last);
-    const uint32_t w_nn = conn[enn];
-    const uint16_t l_nn = TILED ? conn16[enn] : 0;
+    const uint32_t w_nn = L.w32(enn);
+    const uint16_t l_nn = TILED ? L.w16(enn) : 0;
     const uint32_t m = w_c >> kMaskShift;
     const int k = (int)(w_c & kIndexMask);
     const double2 xyk = r_c.xy;
@@ -935,7 +943,7 @@ unsigned prepare_tiled(unsigned tiles, int tile_max, int flux_cap, size_t lds_li
   return ok;
 }
 
-#define TILE_ARGS c.conn_off, c.conn, c.conn16, c.tile_off, c.tile_idx
+#define TILE_ARGS c.conn_off, c.slice_off, c.connS, c.conn16S, c.tile_off, c.tile_idx
 
 void launch_q_variables(const DevCloud &c, const double *prim, double *base, hipStream_t s) {
   hipLaunchKernelGGL(k_q_variables, dim3(blocks_for(c.n, 256)), dim3(256), 0, s, c.n, prim,

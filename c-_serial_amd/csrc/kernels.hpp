@@ -36,6 +36,13 @@ struct DevCloud {
   const int32_t *conn_off;
   const uint32_t *conn;      // global form
   const uint16_t *conn16;    // tiled form (same order)
+  // the same lists in slices, which the forward stencil kernels read: the
+  // lists of a stencil block's points entry by entry, entry e of lane t of
+  // block b at slice_off[b] + e * tile_block + t (lists padded with their
+  // last entry to the block's longest)
+  const int32_t *slice_off;  // [stencil blocks]
+  const uint32_t *connS;     // global form
+  const uint16_t *conn16S;   // tiled form
   const int32_t *tile_off;   // [nblocks + 1]
   const int32_t *tile_idx;   // own points first, then other neighbours
   int tile_max;              // largest tile, entries

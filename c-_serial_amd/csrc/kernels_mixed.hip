@@ -127,7 +127,8 @@ __global__ __launch_bounds__(256) void k_q_variables_m(int n, const double *__re
 // they are the rounded FP64 extrema).
 template <bool TILED, bool POW>
 __global__ __launch_bounds__(256) void k_q_derivatives_m(
-    int n, const int32_t *__restrict__ off, const uint32_t *__restrict__ conn,
+    int n, const int32_t *__restrict__ off, const int32_t *__restrict__ slice_off,
+    const uint32_t *__restrict__ conn,
     const uint16_t *__restrict__ conn16, const int32_t *__restrict__ tile_off,
     const int32_t *__restrict__ tile_idx, int tile_max, const double2 *__restrict__ xy,
     const float4 *__restrict__ q32, float4 *__restrict__ dq_out, float4 *__restrict__ qm_out,
@@ -155,21 +156,22 @@ __global__ __launch_bounds__(256) void k_q_derivatives_m(
   double sdx[4] = {0.0, 0.0, 0.0, 0.0}, sdy[4] = {0.0, 0.0, 0.0, 0.0};
   for (int r = 0; r < 4; r++) qmax[r] = qmin[r] = qi[r];
   // neighbour e + 1's record and e + 2's conn entry load while e is used
-  const int e0 = off[i], last = off[i + 1] - 1;
+  const Slice L = slice_of(slice_off, conn, conn16, b);
+  const int e0 = 0, last = off[i + 1] - off[i] - 1;
   double2 pk = {0.0, 0.0};
   float4 qk4 = {0.0f, 0.0f, 0.0f, 0.0f};
   uint32_t w_n = 0;
   if (e0 <= last) {
-    const int k0 = conn_index(conn_raw(conn, conn16, e0, TILED), TILED);
+    const int k0 = conn_index(L.raw(e0, TILED), TILED);
     pk = gxy[k0];
     qk4 = gq[k0];
-    w_n = conn_raw(conn, conn16, min(e0 + 1, last), TILED);
+    w_n = L.raw(min(e0 + 1, last), TILED);
   }
   for (int e = e0; e <= last; e++) {
     const int kn = conn_index(w_n, TILED);
     const double2 pk_n = gxy[kn];
     const float4 qk4_n = gq[kn];
-    w_n = conn_raw(conn, conn16, min(e + 2, last), TILED);
+    w_n = L.raw(min(e + 2, last), TILED);
     float qk[4];
     unpack4(qk4, qk);
     for (int r = 0; r < 4; r++) {
@@ -210,7 +212,8 @@ __global__ __launch_bounds__(256) void k_q_derivatives_m(
 // corrected differences, FP64 sums and solve.
 template <bool TILED, bool POW>
 __global__ __launch_bounds__(256) void k_q_inner_m(
-    int n, const int32_t *__restrict__ off, const uint32_t *__restrict__ conn,
+    int n, const int32_t *__restrict__ off, const int32_t *__restrict__ slice_off,
+    const uint32_t *__restrict__ conn,
     const uint16_t *__restrict__ conn16, const int32_t *__restrict__ tile_off,
     const int32_t *__restrict__ tile_idx, int tile_max, const double2 *__restrict__ xy,
     const float4 *__restrict__ q32, const float4 *__restrict__ src, float4 *__restrict__ dst,
@@ -241,21 +244,22 @@ __global__ __launch_bounds__(256) void k_q_inner_m(
   double sum_delx_sqr = 0., sum_dely_sqr = 0., sum_delx_dely = 0.;
   double sx[4] = {0.0, 0.0, 0.0, 0.0}, sy[4] = {0.0, 0.0, 0.0, 0.0};
   // neighbour e + 1's record loads while e is used (conn entries further ahead)
-  const int e0 = off[i], last = off[i + 1] - 1;
+  const Slice L = slice_of(slice_off, conn, conn16, b);
+  const int e0 = 0, last = off[i + 1] - off[i] - 1;
   double2 pk = {0.0, 0.0};
   float4 qk4 = {}, dxk4 = {}, dyk4 = {};
   ConnAhead<kAheadQInner> ahead;
   if (e0 <= last) {
-    const int k0 = conn_index(conn_raw(conn, conn16, e0, TILED), TILED);
+    const int k0 = conn_index(L.raw(e0, TILED), TILED);
     pk = gxy[k0];
     qk4 = gq[k0];
     dxk4 = gd[2 * k0];
     dyk4 = gd[2 * k0 + 1];
-    ahead.start(conn, conn16, TILED, e0 + 1, last);
+    ahead.start(L, TILED, e0 + 1, last);
   }
 #pragma unroll kAheadQInner
   for (int e = e0; e <= last; e++) {
-    const int kn = conn_index(ahead.pop(conn, conn16, TILED, e + 1 + kAheadQInner, last), TILED);
+    const int kn = conn_index(ahead.pop(L, TILED, e + 1 + kAheadQInner, last), TILED);
     const double2 pk_n = gxy[kn];
     const float4 qk4_n = gq[kn], dxk4_n = gd[2 * kn], dyk4_n = gd[2 * kn + 1];
     float qk[4], dxk[4], dyk[4];
@@ -303,7 +307,8 @@ constexpr int COEF_STRIDE = 8;
 
 template <bool TILED, bool POW>
 __global__ __launch_bounds__(256) void k_flux_m(
-    int n, const int32_t *__restrict__ off, const uint32_t *__restrict__ conn,
+    int n, const int32_t *__restrict__ off, const int32_t *__restrict__ slice_off,
+    const uint32_t *__restrict__ conn,
     const uint16_t *__restrict__ conn16, const int32_t *__restrict__ tile_off,
     const int32_t *__restrict__ tile_idx, int cap, const double2 *__restrict__ xy,
     const float4 *__restrict__ q32, const float4 *__restrict__ dq32,
@@ -384,20 +389,21 @@ __global__ __launch_bounds__(256) void k_flux_m(
     return {xy[kg], q32[kg], dq32[2 * kg], dq32[2 * kg + 1], qm32[2 * kg], qm32[2 * kg + 1],
             eps32[kg]};
   };
-  const int e0 = off[i], last = off[i + 1] - 1;
+  const Slice L = slice_of(slice_off, conn, conn16, b);
+  const int e0 = 0, last = off[i + 1] - off[i] - 1;
   Rec rc = {};
   uint32_t w_c = 0, w_n = 0, l_n = 0;
   if (e0 <= last) {
-    w_c = conn[e0];
-    rc = fetch(w_c, TILED ? conn16[e0] : 0u);
-    w_n = conn[min(e0 + 1, last)];
-    l_n = TILED ? conn16[min(e0 + 1, last)] : 0u;
+    w_c = L.w32(e0);
+    rc = fetch(w_c, TILED ? L.w16(e0) : 0u);
+    w_n = L.w32(min(e0 + 1, last));
+    l_n = TILED ? L.w16(min(e0 + 1, last)) : 0u;
   }
   for (int e = e0; e <= last; e++) {
     const Rec rn = fetch(w_n, l_n);
     const int enn = min(e + 2, last);
-    const uint32_t w_nn = conn[enn];
-    const uint32_t l_nn = TILED ? conn16[enn] : 0u;
+    const uint32_t w_nn = L.w32(enn);
+    const uint32_t l_nn = TILED ? L.w16(enn) : 0u;
     const uint32_t m = conn_mask(w_c, false);
     const double2 pk = rc.p;
     float qk[4], dxk[4], dyk[4], dmaxk[4], dmink[4];
@@ -541,7 +547,7 @@ unsigned prepare_tiled_m(unsigned tiles, int tile_max, int flux_cap, size_t lds_
   return ok;
 }
 
-#define TILE_ARGS_M c.conn_off, c.conn, c.conn16, c.tile_off, c.tile_idx, c.tile_max
+#define TILE_ARGS_M c.conn_off, c.slice_off, c.connS, c.conn16S, c.tile_off, c.tile_idx, c.tile_max
 
 void launch_q_variables_m(const DevCloud &c, const double *prim, float *q32, hipStream_t s) {
   hipLaunchKernelGGL(k_q_variables_m, dim3(blocks_for(c.n, 256)), dim3(256), 0, s, c.n, prim,
@@ -595,7 +601,7 @@ void launch_flux_m(const DevCloud &c, const MixedState &m, const float *dq, cons
   const size_t L = t ? lds_bytes_flux_m(c.flux_cap) : 0;
   const int cap = t ? c.flux_cap : 0;
 #define FLUX_ARGS                                                                          \
-  c.n, c.conn_off, c.conn, c.conn16, c.tile_off, c.tile_idx, cap,                          \
+  c.n, c.conn_off, c.slice_off, c.connS, c.conn16S, c.tile_off, c.tile_idx, cap,          \
       reinterpret_cast<const double2 *>(m.xy),                                             \
       reinterpret_cast<const float4 *>(m.q32), reinterpret_cast<const float4 *>(dq),       \
       reinterpret_cast<const float4 *>(m.qm32), m.eps32, c.coef, c.nx, c.ny, c.flag, delta, \
