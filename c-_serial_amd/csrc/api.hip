@@ -588,7 +588,7 @@ int create_impl(const lskum_cloud *clouds, int32_t n_clouds, const lskum_case *c
   // lane t of stencil block blk at slice_off[blk] + e * TB + t), each list
   // padded with its last entry to its block's longest; a point without
   // neighbours (batch padding) points at itself.
-  std::vector<int32_t> slice_h(nblk, 0);
+  std::vector<int32_t> slice_h(nblk + 1, 0);
   std::vector<uint32_t> connS_h;
   std::vector<uint16_t> conn16S_h;
   {
@@ -597,17 +597,19 @@ int create_impl(const lskum_cloud *clouds, int32_t n_clouds, const lskum_case *c
       const int32_t p0 = blk * TB, p1 = std::min(n, p0 + TB);
       int32_t width = 0;
       for (int32_t d = p0; d < p1; d++) width = std::max(width, off_h[d + 1] - off_h[d]);
+      c->slice_width = std::max(c->slice_width, width);
       if (total_s > (int64_t)INT32_MAX) break;
       slice_h[blk] = (int32_t)total_s;
       total_s += (int64_t)width * TB;
     }
     if (total_s > (int64_t)INT32_MAX)
       return fail(LSKUM_EINVAL, "neighbour slices exceed 31-bit entry indices");
+    slice_h[nblk] = (int32_t)total_s;
     connS_h.assign((size_t)total_s, 0u);
     conn16S_h.assign((size_t)total_s, 0);
     for (int blk = 0; blk < nblk; blk++) {
       const int32_t p0 = blk * TB, p1 = std::min(n, p0 + TB);
-      const int64_t width = (blk + 1 < nblk ? slice_h[blk + 1] : total_s) - slice_h[blk];
+      const int64_t width = (int64_t)slice_h[blk + 1] - slice_h[blk];
       const int32_t w = (int32_t)(width / TB);
       for (int32_t d = p0; d < p1; d++) {
         const int32_t cnt = off_h[d + 1] - off_h[d];
@@ -653,6 +655,13 @@ int create_impl(const lskum_cloud *clouds, int32_t n_clouds, const lskum_case *c
                                  (size_t)lds_limit);
     else
       c->tiles = prepare_tiled(want, c->tile_max, c->flux_cap, (size_t)lds_limit);
+  }
+  // the pipelined inner pass, where the tiles are in use for it
+  if ((c->tiles & LSKUM_TILE_QINNER) && !c->mixed && p.power == 0.0 && dev_ok) {
+    int n_cu = 0;
+    if (hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess)
+      c->qi_chunks = prepare_qinner_pf(TB, c->tile_max, c->slice_width, nblk, n_cu,
+                                       (size_t)lds_limit, &c->qi_grid);
   }
   // per-case constants and reduction ranges
   c->part.nblocks = blocks_for(n, 256);

@@ -79,6 +79,8 @@ struct lskum_ctx {
   unsigned tiles = 0;
   int tile_block = kTileBlockDefault;
   int flux_cap = 0;  // tile entries the flux stages (0: it reads through L2)
+  int slice_width = 0;
+  int qi_chunks = 0, qi_grid = 0;  // pipelined inner pass (0: not used)
   // cases: flow constants, 256-block -> case map, partial and force ranges
   FlowConsts *fcs = nullptr;
   uint16_t *block_case = nullptr;  // NULL for one case
@@ -137,6 +139,9 @@ struct lskum_ctx {
     c.tiles = tiles;
     c.tile_block = tile_block;
     c.flux_cap = flux_cap;
+    c.slice_width = slice_width;
+    c.qi_chunks = qi_chunks;
+    c.qi_grid = qi_grid;
     c.xy = xy;
     c.nx = nx;
     c.ny = ny;

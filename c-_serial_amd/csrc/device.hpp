@@ -15,11 +15,11 @@ constexpr uint32_t XPOS = 1u, XNEG = 2u, YPOS = 4u, YNEG = 8u;
 // blockIdx -> block of points such that each of the 8 XCDs (which receive
 // workgroups round-robin) works on one contiguous range of points and keeps
 // its neighbours' records in its own L2.
-__device__ __forceinline__ int xcd_block() {
-  const int nb = gridDim.x, b = blockIdx.x;
-  const int q = nb >> 3, r = nb & 7, xcd = b & 7, local = b >> 3;
+__device__ __forceinline__ int xcd_map(int v, int nb) {
+  const int q = nb >> 3, r = nb & 7, xcd = v & 7, local = v >> 3;
   return xcd < r ? xcd * (q + 1) + local : r * (q + 1) + (xcd - r) * q + local;
 }
+__device__ __forceinline__ int xcd_block() { return xcd_map(blockIdx.x, gridDim.x); }
 
 __device__ __forceinline__ void load4(const double *p, double v[4]) {
   const double2 a = reinterpret_cast<const double2 *>(p)[0];

@@ -1,12 +1,12 @@
 """Derives the polynomial physics.hpp's exp_r uses for e^r on
-|r| <= ln(2)/2: p(r) = 1 + r + c2 r^2 + ... + c11 r^11, fitted in relative
+|r| <= ln(2)/2: p(r) = 1 + r + c2 r^2 + ... + c10 r^10, fitted in relative
 error by least squares on Chebyshev nodes with weights raised where the error
 peaks (the usual route towards the minimax fit), in 50-digit arithmetic.
 Prints the relative error of p with its coefficients rounded to double, and
 the coefficients c2 .. c11.
 
-    python exp_fit.py          # degree 11
-    python exp_fit.py 12       # another degree
+    python exp_fit.py          # degree 10
+    python exp_fit.py 11       # another degree
 """
 import sys
 
@@ -41,7 +41,7 @@ def fit(deg, nodes=300, sweeps=6):
 
 
 if __name__ == "__main__":
-    deg = int(sys.argv[1]) if len(sys.argv) > 1 else 11
+    deg = int(sys.argv[1]) if len(sys.argv) > 1 else 10
     c, err = fit(deg)
     print(f"degree {deg} on [-ln2/2, ln2/2]: max relative error {mp.nstr(err, 4)}")
     print("c2..:", [float(v) for v in c])

@@ -236,6 +236,64 @@ __global__ __launch_bounds__(256) void k_q_derivatives(
 }
 
 // ------------------------------------------------------------ inner passes
+// One neighbour's terms of an inner pass (q_variables_mod.h:150-195): the
+// least-squares geometry sums and w dx (-0.5 dx.(dq_k - dq_i)).
+template <bool POW>
+__device__ __forceinline__ void q_inner_term(double x_i, double y_i, const double dxi[4],
+                                             const double dyi[4], double2 xyk,
+                                             const double dxk[4], const double dyk[4],
+                                             double power, double &sum_delx_sqr,
+                                             double &sum_dely_sqr, double &sum_delx_dely,
+                                             double sx[4], double sy[4]) {
+  const double delx = xyk.x - x_i, dely = xyk.y - y_i;
+  const double weights = POW ? pow(sqrt(delx * delx + dely * dely), power) : 1.0;
+  if (POW) {
+    sum_delx_sqr = sum_delx_sqr + delx * delx * weights;
+    sum_dely_sqr = sum_dely_sqr + dely * dely * weights;
+    sum_delx_dely = sum_delx_dely + delx * dely * weights;
+  } else {
+    sum_delx_sqr = sum_delx_sqr + delx * delx;
+    sum_dely_sqr = sum_dely_sqr + dely * dely;
+    sum_delx_dely = sum_delx_dely + delx * dely;
+  }
+  for (int r = 0; r < 4; r++) {
+    // the sum of two products with one of them fused, written out: left to
+    // the compiler, which product it fuses differed between the kernels
+    // that inline this (a last-bit difference between layouts)
+    const double c = -0.5 * fma(delx, dxk[r] - dxi[r], dely * (dyk[r] - dyi[r]));
+    if (POW) {
+      sx[r] = sx[r] + (weights * delx * c);
+      sy[r] = sy[r] + (weights * dely * c);
+    } else {
+      sx[r] = sx[r] + (delx * c);
+      sy[r] = sy[r] + (dely * c);
+    }
+  }
+}
+
+// The pass's result (q_variables_mod.h:197-213): q_derivatives' q sums
+// (qx, qy) added, the 2x2 solve, the point's new derivatives stored at d.
+__device__ __forceinline__ void q_inner_finish(const double qx[4], const double qy[4],
+                                               double sum_delx_sqr, double sum_dely_sqr,
+                                               double sum_delx_dely, double sx[4],
+                                               double sy[4], double *d) {
+  for (int r = 0; r < 4; r++) {
+    sx[r] = qx[r] + sx[r];
+    sy[r] = qy[r] + sy[r];
+  }
+  const double det = fma(sum_delx_sqr, sum_dely_sqr, -(sum_delx_dely * sum_delx_dely));
+  // det > 0 for every validated point; 0 only for a batch's padding points,
+  // which have no neighbours and must not produce NaNs
+  const double one_by_det = det != 0.0 ? 1.0 / det : 0.0;
+  double ox[4], oy[4];
+  for (int r = 0; r < 4; r++) {
+    ox[r] = one_by_det * fma(sx[r], sum_dely_sqr, -(sy[r] * sum_delx_dely));
+    oy[r] = one_by_det * fma(sy[r], sum_delx_sqr, -(sx[r] * sum_delx_dely));
+  }
+  store4(d, ox);
+  store4(d + 4, oy);
+}
+
 // q_variables_mod.h:120-228: one defect-corrected LS pass, Jacobi (reads
 // src, writes dst: the reference's temp array + copy).  Per pair the pass
 // sums w dx ((q_k - q_i) - 0.5 dx.(dq_k - dq_i)) (dx = (delx, dely)); the
@@ -296,54 +354,187 @@ __global__ __launch_bounds__(256) void k_q_inner(
     double2 xy_n;
     double dx_n[4], dy_n[4];
     fetch(ahead.pop(L, TILED, e + 1 + kAheadQInner, last), xy_n, dx_n, dy_n);
-    const double delx = xyk.x - x_i, dely = xyk.y - y_i;
-    const double weights = POW ? pow(sqrt(delx * delx + dely * dely), power) : 1.0;
-    if (POW) {
-      sum_delx_sqr = sum_delx_sqr + delx * delx * weights;
-      sum_dely_sqr = sum_dely_sqr + dely * dely * weights;
-      sum_delx_dely = sum_delx_dely + delx * dely * weights;
-    } else {
-      sum_delx_sqr = sum_delx_sqr + delx * delx;
-      sum_dely_sqr = sum_dely_sqr + dely * dely;
-      sum_delx_dely = sum_delx_dely + delx * dely;
-    }
-    for (int r = 0; r < 4; r++) {
-      const double c = -0.5 * (delx * (dxk[r] - dxi[r]) + dely * (dyk[r] - dyi[r]));
-      if (POW) {
-        sx[r] = sx[r] + (weights * delx * c);
-        sy[r] = sy[r] + (weights * dely * c);
-      } else {
-        sx[r] = sx[r] + (delx * c);
-        sy[r] = sy[r] + (dely * c);
-      }
-    }
+    q_inner_term<POW>(x_i, y_i, dxi, dyi, xyk, dxk, dyk, power, sum_delx_sqr, sum_dely_sqr,
+                      sum_delx_dely, sx, sy);
     xyk = xy_n;
     for (int r = 0; r < 4; r++) {
       dxk[r] = dx_n[r];
       dyk[r] = dy_n[r];
     }
   }
-  {
+  double qx[4], qy[4];
+  load4(qs + 8 * (size_t)i, qx);
+  load4(qs + 8 * (size_t)i + 4, qy);
+  q_inner_finish(qx, qy, sum_delx_sqr, sum_dely_sqr, sum_delx_dely, sx, sy,
+                 dst + (size_t)DQ_STRIDE * i);
+}
+
+// The inner pass again, pipelined.  A workgroup of tile_block threads walks
+// a sequence of stencil blocks (persistent: a few workgroups per CU, each
+// block of an XCD's range taken by a workgroup of that XCD).  While it
+// computes block j from LDS, block j+1's gather -- its tile entries'
+// coordinates and derivatives and its neighbour-list slice, as 16-byte
+// chunks -- is in flight to registers, and block j+2's tile indices before
+// it; the arithmetic reads its lists from LDS, so none of it waits on a
+// global load.  Every thread loads exactly P chunks of a block (past the
+// end it reloads the last one), so the compiler's memory-operation counts
+// stay exact.  The per-point arithmetic is k_q_inner's (q_inner_term,
+// q_inner_finish), so results are bit-identical.
+struct QiBlock {
+  int t0, tn, nconn;  // first tile entry, tile entries, slice chunks
+  int s0;             // first slice entry
+  __device__ __forceinline__ int chunks() const { return 5 * tn + nconn; }
+};
+
+__device__ __forceinline__ QiBlock qi_block(const int32_t *__restrict__ tile_off,
+                                            const int32_t *__restrict__ slice_off, int b) {
+  QiBlock q;
+  q.t0 = tile_off[b];
+  q.tn = tile_off[b + 1] - q.t0;
+  q.s0 = slice_off[b];
+  q.nconn = (slice_off[b + 1] - q.s0) >> 3;
+  return q;
+}
+
+// Chunk u of this thread (t + u * tile_block, the last one past the end)
+// and where it comes from and goes: the coordinates of entry c (from xy),
+// chunk (c - tn) & 3 of entry (c - tn) / 4's derivatives (from src), or
+// chunk c - 5 tn of the list slice.  Addresses and positions are selected
+// without branches, so the loads stay in flight in registers.
+// A 16-byte chunk in registers: a native vector type, which the compiler
+// keeps in registers where an array of HIP's double2 struct, copied from
+// global memory, went to scratch.
+typedef double qi_chunk __attribute__((ext_vector_type(2)));
+
+template <int P>
+__device__ __forceinline__ void qi_load_idx(int (&idx)[P], const QiBlock &q, int t, int TB,
+                                            const int32_t *__restrict__ tile_idx) {
+  const int last = q.chunks() - 1;
+#pragma unroll
+  for (int u = 0; u < P; u++) {
+    const int c = min(t + u * TB, last);
+    const int e = c < q.tn ? c : min((c - q.tn) >> 2, q.tn - 1);
+    idx[u] = tile_idx[q.t0 + e];
+  }
+}
+
+template <int P>
+__device__ __forceinline__ void qi_load_data(qi_chunk (&v)[P], const int (&idx)[P],
+                                             const QiBlock &q, int t, int TB,
+                                             const double *__restrict__ xy,
+                                             const double *__restrict__ src,
+                                             const uint16_t *__restrict__ conn16) {
+  const int last = q.chunks() - 1;
+  const qi_chunk *xy2 = reinterpret_cast<const qi_chunk *>(xy);
+  const qi_chunk *src2 = reinterpret_cast<const qi_chunk *>(src);
+  const qi_chunk *cn2 = reinterpret_cast<const qi_chunk *>(conn16 + q.s0);
+#pragma unroll
+  for (int u = 0; u < P; u++) {
+    const int c = min(t + u * TB, last);
+    const size_t g = (size_t)idx[u];
+    const qi_chunk *a_xy = xy2 + g;
+    const qi_chunk *a_dq = src2 + (DQ_STRIDE / 2) * g + ((c - q.tn) & 3);
+    const qi_chunk *a_cn = cn2 + (c - 5 * q.tn);
+    const qi_chunk *a = c < q.tn ? a_xy : (c < 5 * q.tn ? a_dq : a_cn);
+    v[u] = *a;
+  }
+}
+
+template <int P>
+__device__ __forceinline__ void qi_store_lds(qi_chunk *l2, const qi_chunk (&v)[P], const QiBlock &q,
+                                             int t, int TB, int lds_conn2) {
+  const int last = q.chunks() - 1;
+#pragma unroll
+  for (int u = 0; u < P; u++) {
+    const int c = min(t + u * TB, last);
+    const int at_xy = 5 * c;
+    const int at_dq = 5 * ((c - q.tn) >> 2) + 1 + ((c - q.tn) & 3);
+    const int at_cn = lds_conn2 + (c - 5 * q.tn);
+    l2[c < q.tn ? at_xy : (c < 5 * q.tn ? at_dq : at_cn)] = v[u];
+  }
+}
+
+template <int P>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_q_inner_pf(
+    int n, int nblk, int lds_conn, const int32_t *__restrict__ off,
+    const int32_t *__restrict__ slice_off, const uint16_t *__restrict__ conn16,
+    const int32_t *__restrict__ tile_off, const int32_t *__restrict__ tile_idx,
+    const double *__restrict__ xy, const double *__restrict__ qs,
+    const double *__restrict__ src, double *__restrict__ dst) {
+  extern __shared__ double lds[];
+  constexpr int S = 10;  // x, y, dqx[4], dqy[4]
+  const uint16_t *cl = reinterpret_cast<const uint16_t *>(lds + lds_conn);
+  qi_chunk *l2 = reinterpret_cast<qi_chunk *>(lds);
+  const int t = threadIdx.x, TB = blockDim.x;
+  const int G = gridDim.x;
+  int idx[P];
+  qi_chunk v[P];
+  int vb = blockIdx.x;
+  if (vb >= nblk) return;  // uniform per workgroup: no barrier is reached
+  int b = xcd_map(vb, nblk);
+  QiBlock q = qi_block(tile_off, slice_off, b);
+  qi_load_idx<P>(idx, q, t, TB, tile_idx);
+  qi_load_data<P>(v, idx, q, t, TB, xy, src, conn16);
+  // past the end of the sequence the next block is the current one again:
+  // its loads are redundant but keep every pass through the loop alike
+  int vn = vb + G;
+  QiBlock qn = vn < nblk ? qi_block(tile_off, slice_off, xcd_map(vn, nblk)) : q;
+  qi_load_idx<P>(idx, qn, t, TB, tile_idx);
+  for (;;) {
+    __syncthreads();  // the previous block's arithmetic is done with the LDS
+    qi_store_lds<P>(l2, v, q, t, TB, lds_conn / 2);
+    const int i = b * TB + t;
+    const int ii = min(i, n - 1);
+    const int e_off = off[ii], deg = off[ii + 1] - e_off;
     double qx[4], qy[4];
-    load4(qs + 8 * (size_t)i, qx);
-    load4(qs + 8 * (size_t)i + 4, qy);
-    for (int r = 0; r < 4; r++) {
-      sx[r] = qx[r] + sx[r];
-      sy[r] = qy[r] + sy[r];
+    load4(qs + 8 * (size_t)ii, qx);
+    load4(qs + 8 * (size_t)ii + 4, qy);
+    const int vnn = vn + G;
+    const QiBlock qnn = vnn < nblk ? qi_block(tile_off, slice_off, xcd_map(vnn, nblk)) : qn;
+    qi_load_data<P>(v, idx, qn, t, TB, xy, src, conn16);
+    qi_load_idx<P>(idx, qnn, t, TB, tile_idx);
+    __syncthreads();  // this block's tile and lists are in the LDS
+    if (i < n) {
+      const double *hi = lds + S * t;
+      const double2 pi = reinterpret_cast<const double2 *>(hi)[0];
+      const double x_i = pi.x, y_i = pi.y;
+      double dxi[4], dyi[4];
+      load4(hi + 2, dxi);
+      load4(hi + 6, dyi);
+      double sum_delx_sqr = 0., sum_dely_sqr = 0., sum_delx_dely = 0.;
+      double sx[4] = {0.0, 0.0, 0.0, 0.0}, sy[4] = {0.0, 0.0, 0.0, 0.0};
+      auto fetch = [&](int e, double2 &xyk, double dxk[4], double dyk[4]) {
+        const double *hk = lds + S * (int)(cl[e * TB + t] & kLocalMask);
+        xyk = reinterpret_cast<const double2 *>(hk)[0];
+        load4(hk + 2, dxk);
+        load4(hk + 6, dyk);
+      };
+      const int last = deg - 1;
+      double2 xyk = {0.0, 0.0};
+      double dxk[4] = {}, dyk[4] = {};
+      if (last >= 0) fetch(0, xyk, dxk, dyk);
+      for (int e = 0; e <= last; e++) {
+        double2 xy_n;
+        double dx_n[4], dy_n[4];
+        fetch(min(e + 1, last), xy_n, dx_n, dy_n);
+        q_inner_term<false>(x_i, y_i, dxi, dyi, xyk, dxk, dyk, 0.0, sum_delx_sqr,
+                            sum_dely_sqr, sum_delx_dely, sx, sy);
+        xyk = xy_n;
+        for (int r = 0; r < 4; r++) {
+          dxk[r] = dx_n[r];
+          dyk[r] = dy_n[r];
+        }
+      }
+      q_inner_finish(qx, qy, sum_delx_sqr, sum_dely_sqr, sum_delx_dely, sx, sy,
+                     dst + (size_t)DQ_STRIDE * i);
     }
+    if (vn >= nblk) break;
+    vb = vn;
+    vn = vnn;
+    b = xcd_map(vb, nblk);
+    q = qn;
+    qn = qnn;
   }
-  const double det = sum_delx_sqr * sum_dely_sqr - sum_delx_dely * sum_delx_dely;
-  // det > 0 for every validated point; 0 only for a batch's padding points,
-  // which have no neighbours and must not produce NaNs
-  const double one_by_det = det != 0.0 ? 1.0 / det : 0.0;
-  double ox[4], oy[4];
-  for (int r = 0; r < 4; r++) {
-    ox[r] = one_by_det * (sx[r] * sum_dely_sqr - sy[r] * sum_delx_dely);
-    oy[r] = one_by_det * (sy[r] * sum_delx_sqr - sx[r] * sum_delx_dely);
-  }
-  double *d = dst + (size_t)DQ_STRIDE * i;
-  store4(d, ox);
-  store4(d + 4, oy);
 }
 
 // ------------------------------------------------------------ flux residual
@@ -368,9 +559,13 @@ constexpr int COEF_STRIDE = 8;
 // (x, y, q, eps2) is read through L2.  The tile then takes 128 bytes per
 // entry, so two blocks of a 605-entry tile share a CU's LDS and the kernel
 // keeps the two waves per SIMD its registers allow, while the bytes it
-// gathers through L2 per neighbour fall from 192 to 64.  Chunk c of entry r
-// is stored at position c ^ (r & 7): lanes reading the same chunk of
-// different entries then fall on different LDS banks.
+// gathers through L2 per neighbour fall from 192 to 64.  The eight chunks
+// of an entry are consecutive, so the reads of one entry are eight
+// ds_read_b128 at immediate offsets from one address; an XOR swizzle of
+// the chunk positions, which spread the lanes' reads over more LDS banks,
+// cost more in address arithmetic (16 instructions per neighbour) than the
+// bank conflicts it saved (flux 1.093 -> 1.110 ms at 2.5M points): the
+// flux keeps its LDS busy less than a fifth of the time.
 constexpr int FLUX_TILE_CHUNKS = 8;
 
 __device__ __forceinline__ void load_flux_tile(double2 *lds, const int32_t *__restrict__ tile_off,
@@ -395,7 +590,7 @@ __device__ __forceinline__ void load_flux_tile(double2 *lds, const int32_t *__re
       const int t = base + u * blockDim.x;
       if (t < total) {
         const int r = t >> 3, ch = t & 7;
-        lds[FLUX_TILE_CHUNKS * r + (ch ^ (r & 7))] = v[u];
+        lds[FLUX_TILE_CHUNKS * r + ch] = v[u];
       }
     }
   }
@@ -416,7 +611,7 @@ __device__ __forceinline__ void flux_tile_entry(const double2 *lds, int r, doubl
                                                 double dy[4], double qmx[4], double qmn[4]) {
   double2 c[FLUX_TILE_CHUNKS];
 #pragma unroll
-  for (int j = 0; j < FLUX_TILE_CHUNKS; j++) c[j] = lds[FLUX_TILE_CHUNKS * r + (j ^ (r & 7))];
+  for (int j = 0; j < FLUX_TILE_CHUNKS; j++) c[j] = lds[FLUX_TILE_CHUNKS * r + j];
   dx[0] = c[0].x; dx[1] = c[0].y; dx[2] = c[1].x; dx[3] = c[1].y;
   dy[0] = c[2].x; dy[1] = c[2].y; dy[2] = c[3].x; dy[3] = c[3].y;
   qmx[0] = c[4].x; qmx[1] = c[4].y; qmx[2] = c[5].x; qmx[3] = c[5].y;
@@ -943,6 +1138,45 @@ unsigned prepare_tiled(unsigned tiles, int tile_max, int flux_cap, size_t lds_li
   return ok;
 }
 
+// The pipelined inner pass's LDS: the tile, then the block's list slice
+// (double offset lds_conn).
+static size_t qi_pf_lds_conn(int tile_max) { return (size_t)10 * tile_max; }
+static size_t qi_pf_lds_bytes(int tile_block, int tile_max, int slice_width) {
+  return sizeof(double) * qi_pf_lds_conn(tile_max) +
+         sizeof(uint16_t) * (size_t)slice_width * tile_block;
+}
+
+template <int P>
+static int qi_pf_try(int tile_block, int tile_max, int slice_width, int nblk, int n_cu,
+                     size_t lds_limit, int *grid) {
+  const size_t bytes = qi_pf_lds_bytes(tile_block, tile_max, slice_width);
+  const int chunks = 5 * tile_max + slice_width * tile_block / 8;
+  if (chunks > P * tile_block || bytes > lds_limit) return 0;
+  const void *f = (const void *)k_q_inner_pf<P>;
+  if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) !=
+      hipSuccess)
+    return 0;
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_q_inner_pf<P>, tile_block,
+                                                   bytes) != hipSuccess ||
+      per_cu <= 0)
+    return 0;
+  int g = per_cu * n_cu;
+  g = g >= 8 ? g / 8 * 8 : g;
+  *grid = std::min(g, nblk);
+  return P;
+}
+
+int prepare_qinner_pf(int tile_block, int tile_max, int slice_width, int nblk, int n_cu,
+                      size_t lds_limit, int *grid) {
+  *grid = 0;
+  if (tile_block % 8 || n_cu <= 0 || nblk <= 0) return 0;
+  int p = qi_pf_try<12>(tile_block, tile_max, slice_width, nblk, n_cu, lds_limit, grid);
+  if (!p) p = qi_pf_try<16>(tile_block, tile_max, slice_width, nblk, n_cu, lds_limit, grid);
+  if (!p) p = qi_pf_try<20>(tile_block, tile_max, slice_width, nblk, n_cu, lds_limit, grid);
+  return p;
+}
+
 #define TILE_ARGS c.conn_off, c.slice_off, c.connS, c.conn16S, c.tile_off, c.tile_idx
 
 void launch_q_variables(const DevCloud &c, const double *prim, double *base, hipStream_t s) {
@@ -998,7 +1232,20 @@ void launch_q_inner(const DevCloud &c, const double *qs, const double *dq_src, d
   const bool t = (c.tiles & TILE_QINNER) != 0;
   const size_t L = t ? lds_bytes_qinner(c.tile_max) : 0;
 #define QI_ARGS c.n, TILE_ARGS, c.xy, qs, dq_src, dq_dst, power
-  if (t && power == 0.0)
+  if (t && power == 0.0 && c.qi_chunks) {
+    const int nblk = blocks_for(c.n, c.tile_block);
+    const size_t Lp = qi_pf_lds_bytes(c.tile_block, c.tile_max, c.slice_width);
+    const int lc = (int)qi_pf_lds_conn(c.tile_max);
+#define PF_ARGS c.n, nblk, lc, c.conn_off, c.slice_off, c.conn16S, c.tile_off, c.tile_idx, \
+                c.xy, qs, dq_src, dq_dst
+    if (c.qi_chunks == 12)
+      hipLaunchKernelGGL((k_q_inner_pf<12>), dim3(c.qi_grid), b, Lp, s, PF_ARGS);
+    else if (c.qi_chunks == 16)
+      hipLaunchKernelGGL((k_q_inner_pf<16>), dim3(c.qi_grid), b, Lp, s, PF_ARGS);
+    else
+      hipLaunchKernelGGL((k_q_inner_pf<20>), dim3(c.qi_grid), b, Lp, s, PF_ARGS);
+#undef PF_ARGS
+  } else if (t && power == 0.0)
     hipLaunchKernelGGL((k_q_inner<true, false>), g, b, L, s, QI_ARGS);
   else if (t)
     hipLaunchKernelGGL((k_q_inner<true, true>), g, b, L, s, QI_ARGS);

@@ -40,7 +40,7 @@ struct DevCloud {
   // lists of a stencil block's points entry by entry, entry e of lane t of
   // block b at slice_off[b] + e * tile_block + t (lists padded with their
   // last entry to the block's longest)
-  const int32_t *slice_off;  // [stencil blocks]
+  const int32_t *slice_off;  // [stencil blocks + 1]
   const uint32_t *connS;     // global form
   const uint16_t *conn16S;   // tiled form
   const int32_t *tile_off;   // [nblocks + 1]
@@ -49,6 +49,9 @@ struct DevCloud {
   unsigned tiles;            // LSKUM_TILE_* bits of the kernels using them
   int tile_block;            // points per block of the stencil kernels
   int flux_cap;              // entries of a tile the flux stages (<= tile_max)
+  int slice_width;           // longest list of any point (entries of a slice row)
+  int qi_chunks;             // pipelined inner pass: chunks per thread (0: not used)
+  int qi_grid;               // its workgroups
   const double *xy;   // [n][2] coordinates (also in the base records)
   const double *nx, *ny;
   const double *coef;  // per point: least-squares coefficients of the
@@ -85,6 +88,13 @@ int flux_tile_cap(int block, int tile_max, size_t lds_limit, size_t lds_per_cu);
 // raises the dynamic-LDS limits of the kernels in `tiles`; returns the
 // subset whose tiles fit in lds_limit bytes
 unsigned prepare_tiled(unsigned tiles, int tile_max, int flux_cap, size_t lds_limit);
+// The pipelined inner pass (tiled, power 0) for a layout whose tiles hold at
+// most tile_max entries and whose lists at most slice_width: the chunks
+// each thread gathers per block (0 when the gather does not fit the
+// registers or the tile and list slice do not fit lds_limit), and, through
+// *grid, the workgroups that fill n_cu CUs (a multiple of 8, at most nblk).
+int prepare_qinner_pf(int tile_block, int tile_max, int slice_width, int nblk, int n_cu,
+                      size_t lds_limit, int *grid);
 
 void launch_q_variables(const DevCloud &c, const double *prim, double *base, hipStream_t s);
 void launch_speed(const DevCloud &c, const double *prim, double *base, hipStream_t s);

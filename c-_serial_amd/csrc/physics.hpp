@@ -63,34 +63,49 @@ __device__ __forceinline__ float exp_k(float a) {
 }
 // Doubles: a = n ln2 + r with n = rint(a / ln2) and |r| <= ln2/2 (r by two
 // FMAs against ln 2 split into a double and its remainder), e^r from
-// p(r) = 1 + r + c2 r^2 + ... + c11 r^11, a fit in relative error (7e-18;
-// exp_fit.py beside this file derives it), then scaled by 2^n: within an
-// ulp or two.  a is first raised to -708 (a NaN stays NaN: the clamp
-// compares), which keeps 2^n normal in place of the library's underflow
-// tests (e^-708 = 3e-308 stands for 0).  n is capped at 1100, which keeps
-// its conversion to int defined and, for a past 762, leaves r large: every
-// coefficient is positive, so p(r) and then 2^n p overflow to inf, as the
-// library's exp does past 709.78 -- +inf included, where an uncapped n
-// would make n ln2 - a = inf - inf.  A NaN a gives a NaN r and result.
-template <>
-__device__ __forceinline__ double exp_k(double a) {
-  a = a < -708.0 ? -708.0 : a;
+// p(r) = 1 + r + c2 r^2 + ... + c10 r^10, a fit in relative error (3.3e-16;
+// exp_fit.py beside this file derives it), then scaled by 2^n: within two
+// or three ulps.  n is capped at 1100, which keeps its conversion to int
+// defined and, for a past 762, leaves r large: every coefficient is
+// positive, so p(r) and then 2^n p overflow to inf, as the library's exp
+// does past 709.78 -- +inf included, where an uncapped n would make
+// n ln2 - a = inf - inf.  A NaN a gives a NaN r and result.  The callers
+// first raise a to -708, which keeps 2^n normal in place of the library's
+// underflow tests (e^-708 = 3e-308 stands for 0).
+__device__ __forceinline__ double exp_reduced(double a) {
   const double n = fmin(rint(a * 1.4426950408889634), 1100.0);
   double r = fma(-n, 0.6931471805599453, a);
   r = fma(-n, 2.3190468138462996e-17, r);
-  double p = 2.499930047834758e-08;
-  p = fma(p, r, 2.763033850976749e-07);
-  p = fma(p, r, 2.755758895260481e-06);
-  p = fma(p, r, 2.4801492811609905e-05);
-  p = fma(p, r, 0.00019841269503400455);
-  p = fma(p, r, 0.0013888888943951836);
-  p = fma(p, r, 0.008333333333496135);
-  p = fma(p, r, 0.04166666666652869);
-  p = fma(p, r, 0.1666666666666641);
-  p = fma(p, r, 0.5000000000000011);
+  double p = 2.743938836864164e-07;
+  p = fma(p, r, 2.7635062228304872e-06);
+  p = fma(p, r, 2.4802048439065005e-05);
+  p = fma(p, r, 0.00019841185049797322);
+  p = fma(p, r, 0.0013888888387934915);
+  p = fma(p, r, 0.008333333371024567);
+  p = fma(p, r, 0.04166666666871237);
+  p = fma(p, r, 0.16666666666611155);
+  p = fma(p, r, 0.49999999999997463);
   p = fma(p, r, 1.0);
   p = fma(p, r, 1.0);
   return ldexp(p, (int)n);
+}
+
+// The clamp compares, so a NaN a stays NaN.
+template <>
+__device__ __forceinline__ double exp_k(double a) {
+  return exp_reduced(a < -708.0 ? -708.0 : a);
+}
+
+// e^a for the kinetic terms' a = -S^2 <= 0.  The clamp is one fmax, which
+// maps a NaN a to -708; a NaN S still reaches every flux through the
+// velocity component or the sqrt(beta) it is the product of.
+template <typename T>
+__device__ __forceinline__ T exp_neg_sq(T a) {
+  return exp_k(a);
+}
+template <>
+__device__ __forceinline__ double exp_neg_sq(double a) {
+  return exp_reduced(fmax(a, -708.0));
 }
 
 // a / b without qdiv's last correction: the reciprocal after its two Newton
@@ -232,7 +247,7 @@ __device__ __forceinline__ EndTerms<T> end_terms_sb(T nx, T ny, T u1, T u2, T rh
   const T S1 = e.ut * sb;
   const T S2 = e.un * sb;
   const T h = T(kInvSqrtPi) * sb * pr_by_rho;
-  const T X1 = exp_k(-S1 * S1), X2 = exp_k(-S2 * S2);
+  const T X1 = exp_neg_sq(-S1 * S1), X2 = exp_neg_sq(-S2 * S2);
   e.B1 = X1 * h;
   e.B2 = X2 * h;
   e.e1 = erf_x(S1, X1);

@@ -425,22 +425,23 @@ def test_erf_from_exponential_constants():
 
 
 def test_exp_reduction_constants():
-    # physics.hpp's double exp_k, a = n ln2 + r and 2^n (1 + r + c2 r^2 + ...
-    # + c11 r^11), evaluated here in double (math.fma needs Python 3.13, so
-    # the FMAs are emulated exactly with fractions) with the constants read
-    # from the source and compared with the C library's exp: within 3 ulps
-    # relative on [-700, 700], and the clamp below -708
+    # physics.hpp's double exponential, a = n ln2 + r and 2^n (1 + r + c2 r^2
+    # + ... + c10 r^10), evaluated here in double (math.fma needs Python
+    # 3.13, so the FMAs are emulated exactly with fractions) with the
+    # constants read from the source and compared with the C library's exp:
+    # within 4 ulps relative on [-700, 700] (the fit's own error is 3.3e-16),
+    # and the clamps of its two entry points at -708
     import math
     import re
     from fractions import Fraction
     src = open(os.path.join(REPO, "c-_serial_amd", "csrc", "physics.hpp")).read()
-    body = src[src.index("__device__ __forceinline__ double exp_k(double a)"):]
+    body = src[src.index("__device__ __forceinline__ double exp_reduced(double a)"):]
     body = body[:body.index("\n}\n")]
     c = [float(x) for x in re.findall(r"p = fma\(p, r, ([0-9.e+-]+)\)", body)]
     c0 = float(re.search(r"double p = ([0-9.e+-]+);", body).group(1))
     inv_ln2 = float(re.search(r"rint\(a \* ([0-9.e+-]+)\)", body).group(1))
     hi, lo = (float(x) for x in re.findall(r"r = fma\(-n, ([0-9.e+-]+), [ar]\)", body))
-    assert len(c) == 11 and c[-2:] == [1.0, 1.0]
+    assert len(c) == 10 and c[-2:] == [1.0, 1.0]
     assert abs(hi + lo - math.log(2)) < 1e-30 + 2 ** -60 and abs(inv_ln2 * math.log(2) - 1) < 2e-16
 
     def fma(a, b, d):
@@ -452,8 +453,7 @@ def test_exp_reduction_constants():
         except OverflowError:  # rounds past the largest double: +-inf
             return math.inf if v > 0 else -math.inf
 
-    def exp_k(a):
-        a = -708.0 if a < -708.0 else a
+    def exp_reduced(a):
         n = min(float(round(a * inv_ln2)), 1100.0)  # round half to even, as rint
         r = fma(-n, lo, fma(-n, hi, a))
         p = c0
@@ -464,24 +464,34 @@ def test_exp_reduction_constants():
         except OverflowError:  # the device's ldexp overflows to inf
             return math.inf
 
+    def exp_k(a):
+        return exp_reduced(-708.0 if a < -708.0 else a)
+
     worst = 0.0
     for a in np.concatenate([np.linspace(-700.0, 700.0, 4001), np.linspace(-2.0, 2.0, 4001)]):
         a = float(a)
         worst = max(worst, abs(exp_k(a) / math.exp(a) - 1.0))
-    assert worst <= 3 * 2.0 ** -53, worst
+    assert worst <= 4 * 2.0 ** -53, worst
     assert exp_k(-800.0) == exp_k(-708.0) > 0.0
+    # the entry points: exp_k clamps by a comparison (a NaN passes through
+    # it), the kinetic terms' exp_neg_sq by one fmax
+    ek = src[src.index("__device__ __forceinline__ double exp_k(double a)"):]
+    ek = ek[:ek.index("\n}\n")]
+    assert re.search(r"return exp_reduced\(a < -708\.0 \? -708\.0 : a\);", ek)
+    en = src[src.index("__device__ __forceinline__ double exp_neg_sq(double a)"):]
+    en = en[:en.index("\n}\n")]
+    assert re.search(r"return exp_reduced\(fmax\(a, -708\.0\)\);", en)
     # the upper end: n is capped (keeping its conversion to int defined) and
     # the reduction uses the capped n, so past a = 762 r grows with a;
     # every coefficient being positive, p(r) then grows without bound and
     # 2^n p overflows: finite up to the largest double's logarithm, inf past
     # it, for huge finite a and for +inf (where r = inf - 1100 ln2 = inf,
     # not the inf - inf of an uncapped n); a NaN a makes r, p and the result
-    # NaN (the lower clamp compares, so it keeps the NaN)
-    assert re.search(r"a = a < -708\.0 \? -708\.0 : a;", body)
+    # NaN
     assert re.search(r"const double n = fmin\(rint\(a \* [0-9.e+-]+\), 1100\.0\);", body)
     assert re.search(r"ldexp\(p, \(int\)n\)", body)
     assert c0 > 0 and all(k > 0 for k in c)
-    assert abs(exp_k(709.78) / math.exp(709.78) - 1.0) <= 3 * 2.0 ** -53
+    assert abs(exp_k(709.78) / math.exp(709.78) - 1.0) <= 4 * 2.0 ** -53
     for a in (709.79, 710.0, 762.0, 763.0, 1e4, 1e300):
         assert exp_k(a) == math.inf, a
 
