@@ -657,11 +657,16 @@ int create_impl(const lskum_cloud *clouds, int32_t n_clouds, const lskum_case *c
       c->tiles = prepare_tiled(want, c->tile_max, c->flux_cap, (size_t)lds_limit);
   }
   // the pipelined inner pass, where the tiles are in use for it
-  if ((c->tiles & LSKUM_TILE_QINNER) && !c->mixed && p.power == 0.0 && dev_ok) {
-    int n_cu = 0;
-    if (hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess)
+  // and q_derivatives likewise
+  int n_cu = 0;
+  if (!c->mixed && p.power == 0.0 && dev_ok &&
+      hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess) {
+    if (c->tiles & LSKUM_TILE_QINNER)
       c->qi_chunks = prepare_qinner_pf(TB, c->tile_max, c->slice_width, nblk, n_cu,
                                        (size_t)lds_limit, &c->qi_grid);
+    if (c->tiles & LSKUM_TILE_QDERIV)
+      c->qd_chunks = prepare_qderiv_pf(TB, c->tile_max, c->slice_width, nblk, n_cu,
+                                       (size_t)lds_limit, &c->qd_grid);
   }
   // per-case constants and reduction ranges
   c->part.nblocks = blocks_for(n, 256);

@@ -81,6 +81,7 @@ struct lskum_ctx {
   int flux_cap = 0;  // tile entries the flux stages (0: it reads through L2)
   int slice_width = 0;
   int qi_chunks = 0, qi_grid = 0;  // pipelined inner pass (0: not used)
+  int qd_chunks = 0, qd_grid = 0;  // pipelined q_derivatives (0: not used)
   // cases: flow constants, 256-block -> case map, partial and force ranges
   FlowConsts *fcs = nullptr;
   uint16_t *block_case = nullptr;  // NULL for one case
@@ -142,6 +143,8 @@ struct lskum_ctx {
     c.slice_width = slice_width;
     c.qi_chunks = qi_chunks;
     c.qi_grid = qi_grid;
+    c.qd_chunks = qd_chunks;
+    c.qd_grid = qd_grid;
     c.xy = xy;
     c.nx = nx;
     c.ny = ny;

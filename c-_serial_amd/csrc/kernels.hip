@@ -117,6 +117,50 @@ __global__ __launch_bounds__(256) void k_func_delta(int n, const int32_t *__rest
 // smaller value, in conn order; device index), which the reverse sweep
 // routes the extrema's adjoints to, and the geometry sums (sxx, syy, sxy)
 // the least-squares reverse needs.
+// One neighbour's terms of the unweighted least squares
+// (q_variables_mod.h:83-101, weights = pow(dist, 0) = 1 exactly).
+__device__ __forceinline__ void q_deriv_term(double x_i, double y_i, const double qi[4],
+                                             double2 xyk, const double qk[4],
+                                             double &sum_delx_sqr, double &sum_dely_sqr,
+                                             double &sum_delx_dely, double sdx[4],
+                                             double sdy[4]) {
+  const double delx = xyk.x - x_i, dely = xyk.y - y_i;
+  sum_delx_sqr = sum_delx_sqr + delx * delx;
+  sum_dely_sqr = sum_dely_sqr + dely * dely;
+  sum_delx_dely = sum_delx_dely + delx * dely;
+  for (int r = 0; r < 4; r++) {
+    sdx[r] = sdx[r] + delx * (qk[r] - qi[r]);
+    sdy[r] = sdy[r] + dely * (qk[r] - qi[r]);
+  }
+}
+
+// The 2x2 solve (q_variables_mod.h:103-115), its one fused product written
+// out (the kernels that inline this then agree to the last bit), and the
+// stores of dq, qmax/qmin and, when qs is set, the q sums.
+__device__ __forceinline__ void q_deriv_finish(double sum_delx_sqr, double sum_dely_sqr,
+                                               double sum_delx_dely, const double sdx[4],
+                                               const double sdy[4], const double qmax[4],
+                                               const double qmin[4], double *dq, double *qm,
+                                               double *qs) {
+  const double det = fma(sum_delx_sqr, sum_dely_sqr, -(sum_delx_dely * sum_delx_dely));
+  // det > 0 for every validated point; 0 only for a batch's padding points,
+  // which have no neighbours and must not produce NaNs
+  const double one_by_det = det != 0.0 ? 1.0 / det : 0.0;
+  double dqx[4], dqy[4];
+  for (int r = 0; r < 4; r++) {
+    dqx[r] = fma(sdx[r], sum_dely_sqr, -(sdy[r] * sum_delx_dely)) * one_by_det;
+    dqy[r] = fma(sdy[r], sum_delx_sqr, -(sdx[r] * sum_delx_dely)) * one_by_det;
+  }
+  store4(dq, dqx);
+  store4(dq + 4, dqy);
+  store4(qm, qmax);
+  store4(qm + 4, qmin);
+  if (qs) {  // the q sums the inner passes reuse
+    store4(qs, sdx);
+    store4(qs + 4, sdy);
+  }
+}
+
 template <bool TILED, bool POW, bool ARG = false>
 __global__ __launch_bounds__(256) void k_q_derivatives(
     int n, const int32_t *__restrict__ off, const int32_t *__restrict__ slice_off,
@@ -178,9 +222,13 @@ __global__ __launch_bounds__(256) void k_q_derivatives(
         if (qk[r] < qmin[r]) { qmin[r] = qk[r]; an[r] = kg; }
       }
     } else {
+      // one max / min instruction each in place of a compare and two
+      // selects: the same value as the reference's strict comparisons for
+      // every finite q (on equal values, +0 and -0 included, either is the
+      // result); only a NaN q, which no finite state has, would differ
       for (int r = 0; r < 4; r++) {
-        if (qk[r] > qmax[r]) qmax[r] = qk[r];
-        if (qk[r] < qmin[r]) qmin[r] = qk[r];
+        qmax[r] = fmax(qmax[r], qk[r]);
+        qmin[r] = fmin(qmin[r], qk[r]);
       }
     }
     const double delx = xyk.x - x_i, dely = xyk.y - y_i;
@@ -194,36 +242,14 @@ __global__ __launch_bounds__(256) void k_q_derivatives(
         sdy[r] = sdy[r] + weights * dely * (qk[r] - qi[r]);
       }
     } else {  // weights == 1.0 exactly: x*1.0 == x
-      sum_delx_sqr = sum_delx_sqr + delx * delx;
-      sum_dely_sqr = sum_dely_sqr + dely * dely;
-      sum_delx_dely = sum_delx_dely + delx * dely;
-      for (int r = 0; r < 4; r++) {
-        sdx[r] = sdx[r] + delx * (qk[r] - qi[r]);
-        sdy[r] = sdy[r] + dely * (qk[r] - qi[r]);
-      }
+      q_deriv_term(x_i, y_i, qi, xyk, qk, sum_delx_sqr, sum_dely_sqr, sum_delx_dely, sdx, sdy);
     }
     xyk = xy_n;
     for (int r = 0; r < 4; r++) qk[r] = q_n[r];
   }
-  const double det = sum_delx_sqr * sum_dely_sqr - sum_delx_dely * sum_delx_dely;
-  // det > 0 for every validated point; 0 only for a batch's padding points,
-  // which have no neighbours and must not produce NaNs
-  const double one_by_det = det != 0.0 ? 1.0 / det : 0.0;
-  double dqx[4], dqy[4];
-  for (int r = 0; r < 4; r++) {
-    dqx[r] = (sdx[r] * sum_dely_sqr - sdy[r] * sum_delx_dely) * one_by_det;
-    dqy[r] = (sdy[r] * sum_delx_sqr - sdx[r] * sum_delx_dely) * one_by_det;
-  }
-  double *w = dq_out + (size_t)DQ_STRIDE * i;
-  store4(w, dqx);
-  store4(w + 4, dqy);
-  double *qmw = qm_out + (size_t)QM_STRIDE * i;
-  store4(qmw, qmax);
-  store4(qmw + 4, qmin);
-  if (qs_out) {  // the q sums the inner passes reuse
-    store4(qs_out + 8 * (size_t)i, sdx);
-    store4(qs_out + 8 * (size_t)i + 4, sdy);
-  }
+  q_deriv_finish(sum_delx_sqr, sum_dely_sqr, sum_delx_dely, sdx, sdy, qmax, qmin,
+                 dq_out + (size_t)DQ_STRIDE * i, qm_out + (size_t)QM_STRIDE * i,
+                 qs_out ? qs_out + 8 * (size_t)i : nullptr);
   if (ARG) {
     for (int r = 0; r < 4; r++) {
       arg[8 * (size_t)i + r] = am[r];
@@ -527,6 +553,125 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
       }
       q_inner_finish(qx, qy, sum_delx_sqr, sum_dely_sqr, sum_delx_dely, sx, sy,
                      dst + (size_t)DQ_STRIDE * i);
+    }
+    if (vn >= nblk) break;
+    vb = vn;
+    vn = vnn;
+    b = xcd_map(vb, nblk);
+    q = qn;
+    qn = qnn;
+  }
+}
+
+// q_derivatives pipelined the same way (power 0, without the extremum
+// origins): tile entries of three chunks, x, y and q from the base records.
+template <int P>
+__device__ __forceinline__ void qd_load_idx(int (&idx)[P], const QiBlock &q, int t, int TB,
+                                            const int32_t *__restrict__ tile_idx) {
+  const int last = 3 * q.tn + q.nconn - 1;
+#pragma unroll
+  for (int u = 0; u < P; u++) {
+    const int c = min(t + u * TB, last);
+    idx[u] = tile_idx[q.t0 + min(c / 3, q.tn - 1)];
+  }
+}
+
+template <int P>
+__device__ __forceinline__ void qd_load_data(qi_chunk (&v)[P], const int (&idx)[P],
+                                             const QiBlock &q, int t, int TB,
+                                             const double *__restrict__ base,
+                                             const uint16_t *__restrict__ conn16) {
+  const int last = 3 * q.tn + q.nconn - 1;
+  const qi_chunk *b2 = reinterpret_cast<const qi_chunk *>(base);
+  const qi_chunk *cn2 = reinterpret_cast<const qi_chunk *>(conn16 + q.s0);
+#pragma unroll
+  for (int u = 0; u < P; u++) {
+    const int c = min(t + u * TB, last);
+    const qi_chunk *a_b = b2 + (B_STRIDE / 2) * (size_t)idx[u] + (c - 3 * (c / 3));
+    const qi_chunk *a_cn = cn2 + (c - 3 * q.tn);
+    v[u] = *(c < 3 * q.tn ? a_b : a_cn);
+  }
+}
+
+template <int P>
+__device__ __forceinline__ void qd_store_lds(qi_chunk *l2, const qi_chunk (&v)[P],
+                                             const QiBlock &q, int t, int TB, int lds_conn2) {
+  const int last = 3 * q.tn + q.nconn - 1;
+#pragma unroll
+  for (int u = 0; u < P; u++) {
+    const int c = min(t + u * TB, last);
+    l2[c < 3 * q.tn ? c : lds_conn2 + (c - 3 * q.tn)] = v[u];
+  }
+}
+
+template <int P>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_q_derivatives_pf(
+    int n, int nblk, int lds_conn, const int32_t *__restrict__ off,
+    const int32_t *__restrict__ slice_off, const uint16_t *__restrict__ conn16,
+    const int32_t *__restrict__ tile_off, const int32_t *__restrict__ tile_idx,
+    const double *__restrict__ base, double *__restrict__ dq_out, double *__restrict__ qm_out,
+    double *__restrict__ qs_out) {
+  extern __shared__ double lds[];
+  constexpr int S = 6;  // x, y, q[4]
+  const uint16_t *cl = reinterpret_cast<const uint16_t *>(lds + lds_conn);
+  qi_chunk *l2 = reinterpret_cast<qi_chunk *>(lds);
+  const int t = threadIdx.x, TB = blockDim.x;
+  const int G = gridDim.x;
+  int idx[P];
+  qi_chunk v[P];
+  int vb = blockIdx.x;
+  if (vb >= nblk) return;  // uniform per workgroup: no barrier is reached
+  int b = xcd_map(vb, nblk);
+  QiBlock q = qi_block(tile_off, slice_off, b);
+  qd_load_idx<P>(idx, q, t, TB, tile_idx);
+  qd_load_data<P>(v, idx, q, t, TB, base, conn16);
+  int vn = vb + G;
+  QiBlock qn = vn < nblk ? qi_block(tile_off, slice_off, xcd_map(vn, nblk)) : q;
+  qd_load_idx<P>(idx, qn, t, TB, tile_idx);
+  for (;;) {
+    __syncthreads();  // the previous block's arithmetic is done with the LDS
+    qd_store_lds<P>(l2, v, q, t, TB, lds_conn / 2);
+    const int i = b * TB + t;
+    const int ii = min(i, n - 1);
+    const int deg = off[ii + 1] - off[ii];
+    const int vnn = vn + G;
+    const QiBlock qnn = vnn < nblk ? qi_block(tile_off, slice_off, xcd_map(vnn, nblk)) : qn;
+    qd_load_data<P>(v, idx, qn, t, TB, base, conn16);
+    qd_load_idx<P>(idx, qnn, t, TB, tile_idx);
+    __syncthreads();  // this block's tile and lists are in the LDS
+    if (i < n) {
+      const double *hi = lds + S * t;
+      const double2 xy = reinterpret_cast<const double2 *>(hi)[0];
+      double qi[4], qmax[4], qmin[4];
+      load4(hi + 2, qi);
+      double sum_delx_sqr = 0., sum_dely_sqr = 0., sum_delx_dely = 0.;
+      double sdx[4] = {0.0, 0.0, 0.0, 0.0}, sdy[4] = {0.0, 0.0, 0.0, 0.0};
+      for (int r = 0; r < 4; r++) qmax[r] = qmin[r] = qi[r];
+      auto fetch = [&](int e, double2 &xyk, double qk[4]) {
+        const double *hk = lds + S * (int)(cl[e * TB + t] & kLocalMask);
+        xyk = reinterpret_cast<const double2 *>(hk)[0];
+        load4(hk + 2, qk);
+      };
+      const int last = deg - 1;
+      double2 xyk = {0.0, 0.0};
+      double qk[4] = {0.0, 0.0, 0.0, 0.0};
+      if (last >= 0) fetch(0, xyk, qk);
+      for (int e = 0; e <= last; e++) {
+        double2 xy_n;
+        double q_n[4];
+        fetch(min(e + 1, last), xy_n, q_n);
+        for (int r = 0; r < 4; r++) {
+          qmax[r] = fmax(qmax[r], qk[r]);
+          qmin[r] = fmin(qmin[r], qk[r]);
+        }
+        q_deriv_term(xy.x, xy.y, qi, xyk, qk, sum_delx_sqr, sum_dely_sqr, sum_delx_dely, sdx,
+                     sdy);
+        xyk = xy_n;
+        for (int r = 0; r < 4; r++) qk[r] = q_n[r];
+      }
+      q_deriv_finish(sum_delx_sqr, sum_dely_sqr, sum_delx_dely, sdx, sdy, qmax, qmin,
+                     dq_out + (size_t)DQ_STRIDE * i, qm_out + (size_t)QM_STRIDE * i,
+                     qs_out ? qs_out + 8 * (size_t)i : nullptr);
     }
     if (vn >= nblk) break;
     vb = vn;
@@ -1167,6 +1312,43 @@ static int qi_pf_try(int tile_block, int tile_max, int slice_width, int nblk, in
   return P;
 }
 
+static size_t qd_pf_lds_conn(int tile_max) { return (size_t)6 * tile_max; }
+static size_t qd_pf_lds_bytes(int tile_block, int tile_max, int slice_width) {
+  return sizeof(double) * qd_pf_lds_conn(tile_max) +
+         sizeof(uint16_t) * (size_t)slice_width * tile_block;
+}
+
+template <int P>
+static int qd_pf_try(int tile_block, int tile_max, int slice_width, int nblk, int n_cu,
+                     size_t lds_limit, int *grid) {
+  const size_t bytes = qd_pf_lds_bytes(tile_block, tile_max, slice_width);
+  const int chunks = 3 * tile_max + slice_width * tile_block / 8;
+  if (chunks > P * tile_block || bytes > lds_limit) return 0;
+  const void *f = (const void *)k_q_derivatives_pf<P>;
+  if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) !=
+      hipSuccess)
+    return 0;
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_q_derivatives_pf<P>, tile_block,
+                                                   bytes) != hipSuccess ||
+      per_cu <= 0)
+    return 0;
+  int g = per_cu * n_cu;
+  g = g >= 8 ? g / 8 * 8 : g;
+  *grid = std::min(g, nblk);
+  return P;
+}
+
+int prepare_qderiv_pf(int tile_block, int tile_max, int slice_width, int nblk, int n_cu,
+                      size_t lds_limit, int *grid) {
+  *grid = 0;
+  if (tile_block % 8 || n_cu <= 0 || nblk <= 0) return 0;
+  int p = qd_pf_try<8>(tile_block, tile_max, slice_width, nblk, n_cu, lds_limit, grid);
+  if (!p) p = qd_pf_try<12>(tile_block, tile_max, slice_width, nblk, n_cu, lds_limit, grid);
+  if (!p) p = qd_pf_try<16>(tile_block, tile_max, slice_width, nblk, n_cu, lds_limit, grid);
+  return p;
+}
+
 int prepare_qinner_pf(int tile_block, int tile_max, int slice_width, int nblk, int n_cu,
                       size_t lds_limit, int *grid) {
   *grid = 0;
@@ -1204,7 +1386,20 @@ void launch_q_derivatives(const DevCloud &c, const double *base, double *dq, dou
   const bool t = (c.tiles & TILE_QDERIV) != 0;
   const size_t L = t ? lds_bytes_qderiv(c.tile_max) : 0;
 #define QD_ARGS c.n, TILE_ARGS, base, dq, qm, qs, power
-  if (t && power == 0.0)
+  if (t && power == 0.0 && c.qd_chunks) {
+    const int nblk = blocks_for(c.n, c.tile_block);
+    const size_t Lp = qd_pf_lds_bytes(c.tile_block, c.tile_max, c.slice_width);
+    const int lc = (int)qd_pf_lds_conn(c.tile_max);
+#define PF_ARGS c.n, nblk, lc, c.conn_off, c.slice_off, c.conn16S, c.tile_off, c.tile_idx, \
+                base, dq, qm, qs
+    if (c.qd_chunks == 8)
+      hipLaunchKernelGGL((k_q_derivatives_pf<8>), dim3(c.qd_grid), b, Lp, s, PF_ARGS);
+    else if (c.qd_chunks == 12)
+      hipLaunchKernelGGL((k_q_derivatives_pf<12>), dim3(c.qd_grid), b, Lp, s, PF_ARGS);
+    else
+      hipLaunchKernelGGL((k_q_derivatives_pf<16>), dim3(c.qd_grid), b, Lp, s, PF_ARGS);
+#undef PF_ARGS
+  } else if (t && power == 0.0)
     hipLaunchKernelGGL((k_q_derivatives<true, false>), g, b, L, s, QD_ARGS);
   else if (t)
     hipLaunchKernelGGL((k_q_derivatives<true, true>), g, b, L, s, QD_ARGS);

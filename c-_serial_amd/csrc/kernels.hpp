@@ -52,6 +52,8 @@ struct DevCloud {
   int slice_width;           // longest list of any point (entries of a slice row)
   int qi_chunks;             // pipelined inner pass: chunks per thread (0: not used)
   int qi_grid;               // its workgroups
+  int qd_chunks;             // pipelined q_derivatives: chunks per thread (0: not used)
+  int qd_grid;
   const double *xy;   // [n][2] coordinates (also in the base records)
   const double *nx, *ny;
   const double *coef;  // per point: least-squares coefficients of the
@@ -94,6 +96,9 @@ unsigned prepare_tiled(unsigned tiles, int tile_max, int flux_cap, size_t lds_li
 // registers or the tile and list slice do not fit lds_limit), and, through
 // *grid, the workgroups that fill n_cu CUs (a multiple of 8, at most nblk).
 int prepare_qinner_pf(int tile_block, int tile_max, int slice_width, int nblk, int n_cu,
+                      size_t lds_limit, int *grid);
+// The same for the pipelined q_derivatives (tiled, power 0).
+int prepare_qderiv_pf(int tile_block, int tile_max, int slice_width, int nblk, int n_cu,
                       size_t lds_limit, int *grid);
 
 void launch_q_variables(const DevCloud &c, const double *prim, double *base, hipStream_t s);

@@ -175,8 +175,8 @@ __global__ __launch_bounds__(256) void k_q_derivatives_m(
     float qk[4];
     unpack4(qk4, qk);
     for (int r = 0; r < 4; r++) {
-      qmax[r] = qk[r] > qmax[r] ? qk[r] : qmax[r];
-      qmin[r] = qk[r] < qmin[r] ? qk[r] : qmin[r];
+      qmax[r] = fmaxf(qmax[r], qk[r]);  // the reference's strict comparisons for finite q
+      qmin[r] = fminf(qmin[r], qk[r]);
     }
     const double delx = pk.x - pi.x, dely = pk.y - pi.y;
     const double weights = POW ? pow(sqrt(delx * delx + dely * dely), power) : 1.0;

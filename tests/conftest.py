@@ -49,7 +49,8 @@ def normwise(a, b):
             return float("inf")
         fy = np.abs(y[np.isfinite(y)])
         scale = max(float(fy.max()) if fy.size else 0.0, 1e-300)
-        err = float(np.abs(np.where(differ, x - y, 0.0)).max()) / scale
+        # only finite pairs differ here: equal infinities are masked out
+        err = float(np.abs(x[differ] - y[differ]).max(initial=0.0)) / scale
         worst = max(worst, err)
     return worst
 
