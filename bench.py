@@ -288,39 +288,46 @@ def run_mixed10m(args):
     }
 
 
-def run_shape(args):
+SHAPE_FLOW = dict(mach=0.8, aoa=1.25, cfl=0.2, vl_const=150.0, power=0.0, inner_iterations=0)
+FD_VARS = (1, 3, 5, 7, 9, 10, 12, 14, 16, 18)  # five bumps on each surface
+
+
+def shape_measurement(args, n_fd=len(FD_VARS)):
     """Shape gradients (the adjoint workload): the 1,000,940-point NACA0012
-    cloud (wall 1,500, 806 x 1,240 O-grid, k = 24) at M 0.8, AoA 1.25 deg,
-    the reference's limiter constant K = 150, no inner passes, CFL 0.1 (at
-    0.2 the trailing-edge point, whose nearest wall neighbour lies 4.4e-6
-    chords away, runs away after about 4,000 iterations), advanced
-    `--converge` iterations from free stream;
-    20 Hicks-Henne bumps (10 per surface, peaks over x/c 0.05..0.95, width
+    cloud naca0012_1m_k24 (wall 1,500 with blended spacing, 806 x 1,240
+    O-grid, k = 24) at M 0.8, AoA 1.25 deg, the reference's limiter constant
+    K = 150, no inner passes, CFL 0.2, advanced args.converge iterations from
+    free stream as captured graphs (the state to linearise about); 20
+    Hicks-Henne bumps (10 per surface, peaks over x/c 0.05..0.95, width
     exponent 3, inverse-distance propagation within 0.5 chord of the wall).
-    Timed: one adjoint sweep over `--adjoint-iters` iterations per objective
-    (Cl, Cd), each giving all 20 derivatives.  value = gradients
-    (objectives x design variables) per second of adjoint time.  Checked on
-    sample variables: the tangent-linear run and central differences
-    (h = 1e-6) of the plain iterations; the spread of the one-sided
-    differences is reported beside them (h J'' even where J is smooth)."""
+    Timed: one adjoint sweep over args.adjoint_iters iterations per
+    objective (Cl, Cd), each giving all 20 derivatives; value = gradients
+    (objectives x design variables) per second of sweep time.  Checked:
+    central and one-sided differences (h = 1e-8) of the plain iterations
+    along n_fd design fields, five on each surface, for both objectives from
+    the same runs.  A limiter or extremum switch within h of the state is a
+    kink of J: the one-sided difference on the side free of it is the slope
+    the adjoint takes, the central one is not; the one-sided spread says
+    where that happened.  Also the tangent-linear run along one field per
+    surface."""
     from lskum import shape
     t = time.perf_counter()
     cloud = lskum.workload("naca0012_1m_k24")
     log(f"shape: cloud {cloud.n} points, {time.perf_counter() - t:.1f}s")
-    s = Solver(cloud, mach=0.8, aoa=1.25, cfl=0.1, vl_const=150.0, power=0.0,
-               inner_iterations=0)
+    s = Solver(cloud, **SHAPE_FLOW)
+    s.set_graph(50)
     t = time.perf_counter()
-    done = 0
+    done, forces = 0, []
     while done < args.converge:
-        step = min(500, args.converge - done)
+        step = min(1000, args.converge - done)
         rn, rs = s.q_lskum(step)
         done += step
-        log(f"shape: converge {done}/{args.converge}: res_new {rn[-1]:.3e} residue {rs[-1]:.2f}")
+        forces.append(s.compute_cl_cd_cm())
+        log(f"shape: converge {done}/{args.converge}: res_new {rn[-1]:.3e} residue "
+            f"{rs[-1]:.2f} cl {forces[-1][0]:.8f} cd {forces[-1][1]:.8f}")
     conv_s = time.perf_counter() - t
-    t = time.perf_counter()
     fields = shape.hicks_henne_fields(cloud, n_per_side=10, p_lo=0.05, p_hi=0.95,
                                       width_exponent=3, decay_radius=0.5)
-    log(f"shape: fields {fields.shape} in {time.perf_counter() - t:.1f}s")
     K = args.adjoint_iters
     objectives = {"cl": (1.0, 0.0, 0.0), "cd": (0.0, 1.0, 0.0)}
     grads, Js, sweep_s = {}, {}, {}
@@ -330,44 +337,71 @@ def run_shape(args):
         sweep_s[name] = time.perf_counter() - t
         log(f"shape: adjoint sweep {name} over {K} iterations in {sweep_s[name]:.1f}s")
     t = time.perf_counter()
-    s.objective_after(K)
+    s.forces_after(K)
     fwd_s = time.perf_counter() - t
-    checks = {}
-    for name, j in (("cl", 3), ("cd", 3), ("cl", 16)):
-        w = objectives[name]
+    fd, h_fd = {}, 1e-8
+    for j in FD_VARS[:n_fd]:
+        c, f_, b_ = shape.design_fd_forces(s, fields[j], K, h=h_fd)
+        for k, name in enumerate(("cl", "cd")):
+            g = float(grads[name][j])
+            rel = [abs(float(d[k]) - g) / abs(g) for d in (c, f_, b_)]
+            fd[f"{name}_var{j}"] = {"adjoint": g, "fd_central": float(c[k]),
+                                    "rel_err": rel[0], "rel_err_best": min(rel),
+                                    "one_sided_spread_rel": abs(float(f_[k] - b_[k])) /
+                                    abs(float(c[k]))}
+        log(f"shape: fd var {j}: " + ", ".join(
+            f"{n} {fd[f'{n}_var{j}']['rel_err']:.2e} (best side "
+            f"{fd[f'{n}_var{j}']['rel_err_best']:.2e})" for n in ("cl", "cd")))
+    tan = {}
+    for j in (3, 14):
         t = time.perf_counter()
-        tan = shape.design_tangent(s, fields[j], K, w)
-        tan_s = time.perf_counter() - t
-        fd, fwd, bwd = shape.design_fd(s, fields[j], K, w, h=1e-6, one_sided=True)
-        g = float(grads[name][j])
-        checks[f"{name}_var{j}"] = {
-            "adjoint": g, "tangent": tan, "fd_central": fd, "fd_forward": fwd,
-            "fd_backward": bwd, "rel_err_tangent": abs(tan - g) / abs(g),
-            "rel_err_fd": abs(fd - g) / abs(g),
-            "one_sided_spread_rel": abs(fwd - bwd) / abs(fd), "tangent_s": tan_s}
-        log(f"shape: check {name} var {j}: {checks[f'{name}_var{j}']}")
+        d = shape.design_tangent(s, fields[j], K, objectives["cl"])
+        g = float(grads["cl"][j])
+        tan[f"cl_var{j}"] = {"adjoint": g, "tangent": d, "rel_err": abs(d - g) / abs(g),
+                             "seconds": time.perf_counter() - t}
+    s.close()
     total = sum(sweep_s.values())
     n_grads = len(objectives) * fields.shape[0]
+    last = np.array(forces[-2:]) if len(forces) >= 2 else np.zeros((2, 3))
     return {
-        "metric": METRIC, "value": n_grads / total, "unit": "gradients/s",
-        "n_gpus": 1, "steps": K, "warmup": 0, "ms_per_step": 1e3 * total / (len(objectives) * K),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
-        "data": "synthetic",
-        "config": {"workload": "shape_naca0012_1m_k24", "points": cloud.n, "k": 24,
-                   "inner_iterations": 0, "mach": 0.8, "aoa_deg": 1.25, "cfl": 0.1,
-                   "vl_const": 150.0, "converge_iterations": args.converge,
-                   "adjoint_iterations": K, "design_variables": int(fields.shape[0]),
-                   "bumps": "10 upper + 10 lower, x/c 0.05..0.95, exponent 3",
-                   "propagation": "inverse distance |d|^-3, faded to 0 at 0.5 chord"},
-        "state": {"res_new": float(s.res_new), "residue": float(s.residue),
-                  "converge_s": conv_s},
-        "objectives": Js,
-        "adjoint_sweep_s": sweep_s,
+        "value": n_grads / total, "unit": "gradients/s",
+        "workload": "shape_naca0012_1m_k24", "points": cloud.n,
+        "objectives": list(objectives), "design_variables": int(fields.shape[0]),
+        "adjoint_iterations": K, "adjoint_sweep_s": sweep_s,
         "adjoint_point_iterations_per_s": cloud.n * K * len(objectives) / total,
         "forward_s_same_iterations": fwd_s,
         "adjoint_cost_vs_forward": total / len(objectives) / fwd_s,
-        "gradients": {k: [float(v) for v in g] for k, g in grads.items()},
-        "checks": checks,
+        "state": {"iterations": done, "res_new": float(s.res_new), "residue": float(s.residue),
+                  "abs_dCl_last_1000": float(abs(last[1][0] - last[0][0])),
+                  "abs_dCd_last_1000": float(abs(last[1][1] - last[0][1])),
+                  "converge_s": conv_s},
+        "J": Js,
+        "fd_checks": {"h": h_fd, "count": len(fd),
+                      "worst_rel_err": max(v["rel_err"] for v in fd.values()),
+                      "worst_rel_err_best": max(v["rel_err_best"] for v in fd.values()),
+                      "median_rel_err": float(np.median([v["rel_err"] for v in fd.values()])),
+                      "per_variable": fd},
+        "tangent_checks": {"worst_rel_err": max(v["rel_err"] for v in tan.values()),
+                           "per_variable": tan},
+    }
+
+
+def run_shape(args):
+    """The shape-gradient workload alone, as its own JSON line."""
+    r = shape_measurement(args)
+    return {
+        "metric": METRIC, "value": r["value"], "unit": "gradients/s", "n_gpus": 1,
+        "steps": args.adjoint_iters, "warmup": 0,
+        "ms_per_step": 1e3 * sum(r["adjoint_sweep_s"].values()) / (2 * args.adjoint_iters),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "shape_naca0012_1m_k24", "points": r["points"], "k": 24,
+                   **{k: v for k, v in SHAPE_FLOW.items()},
+                   "converge_iterations": args.converge, "adjoint_iterations": args.adjoint_iters,
+                   "design_variables": r["design_variables"],
+                   "bumps": "10 upper + 10 lower, x/c 0.05..0.95, exponent 3",
+                   "propagation": "inverse distance |d|^-3, faded to 0 at 0.5 chord"},
+        "adjoint": r,
     }
 
 
@@ -384,7 +418,9 @@ def main():
     ap.add_argument("--tile-block", type=int, default=0, choices=[0, 64, 128, 256])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
-    ap.add_argument("--converge", type=int, default=8000,
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="leave the shape-gradient (adjoint) measurement out of the line")
+    ap.add_argument("--converge", type=int, default=20000,
                     help="shape workload: iterations before the adjoint")
     ap.add_argument("--adjoint-iters", type=int, default=300,
                     help="shape workload: iterations the adjoint differentiates")
@@ -462,6 +498,8 @@ def main():
                               "res_new_normwise": float(np.abs(m["res_new"] - rn64).max()
                                                         / np.abs(rn64).max())},
         }
+    if not args.no_adjoint:
+        out["adjoint"] = shape_measurement(args)
     if not args.no_cpu_baseline:
         log("cpu baseline: reference serial fpi_solver on a bounded sample ...")
         out["cpu_baseline"] = cpu_baseline(args)

@@ -110,6 +110,14 @@ struct ArcSurface {
     }
     total = cum[nfine];
   }
+  // the arc-length fraction of the point at parameter t in [0, 2pi)
+  double fraction(double t) const {
+    const double f = t / (2.0 * kPi) * (double)(th.size() - 1);
+    size_t lo = (size_t)f;
+    if (lo >= th.size() - 1) lo = th.size() - 2;
+    const double w = f - (double)lo;
+    return (cum[lo] + w * (cum[lo + 1] - cum[lo])) / total;
+  }
   void at(double s, double *X, double *Y) const {
     s -= std::floor(s);
     double target = s * total;
@@ -309,7 +317,8 @@ int lskum_cloud_generate(const lskum_gen_params *p, lskum_cloud_h **out) {
   *out = nullptr;
   if (p->n_wall < 8 || p->n_radial < 2 || p->n_circ < 8 || p->k < 3 ||
       p->k > 255 || p->r_far <= 1.0 || p->jitter < 0.0 || p->jitter >= 0.5 ||
-      p->radius_factor <= 0.0 || p->naca_t <= 0 || p->naca_t >= 40)
+      p->radius_factor <= 0.0 || p->naca_t <= 0 || p->naca_t >= 40 ||
+      !(p->wall_blend >= 0.0 && p->wall_blend <= 1.0))
     return LSKUM_EINVAL;
   const int64_t n64 = (int64_t)p->n_wall + (int64_t)p->n_radial * p->n_circ;
   if (n64 >= (1 << 28)) return LSKUM_EINVAL;
@@ -356,9 +365,16 @@ int lskum_cloud_generate(const lskum_gen_params *p, lskum_cloud_h **out) {
     *Y = (1.0 - gg) * sy + gg * fy;
   };
 
-  // wall: cosine chord spacing, clockwise from the trailing edge
+  // wall: cosine chord spacing, clockwise from the trailing edge, or its
+  // arc-length positions blended toward uniform arc length
+  const double wb = p->wall_blend;
   for (int32_t i = 0; i < nw; i++) {
-    af.surf(2.0 * kPi * i / nw, &c->x[i], &c->y[i]);
+    if (wb > 0.0) {
+      const double s = (1.0 - wb) * arc.fraction(2.0 * kPi * i / nw) + wb * ((double)i / nw);
+      arc.at(s, &c->x[i], &c->y[i]);
+    } else {
+      af.surf(2.0 * kPi * i / nw, &c->x[i], &c->y[i]);
+    }
     c->flag1[i] = 0;
     c->flag2[i] = 1;
     c->left[i] = (i + nw - 1) % nw;

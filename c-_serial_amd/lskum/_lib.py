@@ -52,6 +52,7 @@ class GenParams(C.Structure):
         ("n_wall", C.c_int32), ("n_radial", C.c_int32), ("n_circ", C.c_int32),
         ("r_far", C.c_double), ("stretch", C.c_double), ("jitter", C.c_double),
         ("k", C.c_int32), ("radius_factor", C.c_double), ("seed", C.c_uint64),
+        ("wall_blend", C.c_double),
     ]
 
 

@@ -123,13 +123,13 @@ class Cloud:
 
     @classmethod
     def generate(cls, naca="0012", n_wall=160, n_radial=64, n_circ=150, r_far=20.0,
-                 stretch=1.08, jitter=0.1, k=20, radius_factor=3.0, seed=1):
+                 stretch=1.08, jitter=0.1, k=20, radius_factor=3.0, seed=1, wall_blend=0.0):
         """Synthetic NACA 4-digit cloud (see lskum_gen_params in lskum.h)."""
         digits = str(naca)
         if len(digits) != 4 or not digits.isdigit():
             raise ValueError("naca must be a 4-digit string such as '0012'")
         p = L.GenParams(int(digits[0]), int(digits[1]), int(digits[2:]), n_wall, n_radial,
-                        n_circ, r_far, stretch, jitter, k, radius_factor, seed)
+                        n_circ, r_far, stretch, jitter, k, radius_factor, seed, wall_blend)
         h = C.c_void_p()
         L.check(L.lib().lskum_cloud_generate(C.byref(p), C.byref(h)))
         try:
@@ -149,9 +149,14 @@ WORKLOADS = {
     # NACA0012, 1,500 wall, 806 x 1,240 near-isotropic O-grid, k = 24 (the
     # shape-gradient cloud; a 266 x 3,750 layout of the same size, far
     # cells 14 times longer radially than around, loses stability within
-    # a few hundred iterations at M 0.8)
+    # a few hundred iterations at M 0.8).  The wall points' spacing is
+    # blended half way from cosine to uniform arc length: cosine spacing
+    # alone puts the trailing-edge points 4.4e-6 chords apart against
+    # 1.6e-3 to the first ring, and the iteration then ran away at the
+    # trailing edge after about 9,000 iterations even at CFL 0.1; blended
+    # (wall spacing 6.8e-4 to 1.7e-3) it ran 120,000 iterations at CFL 0.2
     "naca0012_1m_k24": dict(naca="0012", n_wall=1500, n_radial=806, n_circ=1240,
-                            stretch=0.0, k=24, seed=5),
+                            stretch=0.0, k=24, seed=5, wall_blend=0.5),
     # NACA0012, 4,000 wall, 1,000 x 10,000, k = 24
     "naca0012_10m_k24": dict(naca="0012", n_wall=4000, n_radial=1000, n_circ=10000,
                              stretch=0.0, k=24, seed=3),

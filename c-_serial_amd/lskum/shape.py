@@ -48,3 +48,17 @@ def design_fd(solver, field, n_iter, w=(1.0, 0.0, 0.0), h=1e-6, one_sided=False)
     if not one_sided:
         return central
     return central, (J[1] - J[0]) / h, (J[0] - J[-1]) / h
+
+
+def design_fd_forces(solver, field, n_iter, h=1e-6):
+    """Central, forward and backward differences of (Cl, Cd, Cm) along one
+    design field from the same three runs (points moved by +h, -h and 0,
+    then put back): three arrays of 3."""
+    x0 = np.ascontiguousarray(solver.cloud.x, dtype=np.float64)
+    y0 = np.ascontiguousarray(solver.cloud.y, dtype=np.float64)
+    F = {}
+    for sgn in (1, -1, 0):
+        solver.set_coordinates(x0, y0 + sgn * h * field)
+        F[sgn] = solver.forces_after(n_iter)
+    solver.set_coordinates(x0, y0)
+    return (F[1] - F[-1]) / (2 * h), (F[1] - F[0]) / h, (F[0] - F[-1]) / h

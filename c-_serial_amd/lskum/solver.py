@@ -277,6 +277,15 @@ class Solver:
         self.set_state(p0)
         return float(np.dot(w, (cl, cd, cm, jp0)))
 
+    def forces_after(self, n_iter):
+        """(Cl, Cd, Cm) after n_iter plain iterations from the current state,
+        which is restored afterwards (the iteration counter is not)."""
+        p0 = self.prim
+        self.q_lskum(n_iter)
+        out = self.compute_cl_cd_cm()
+        self.set_state(p0)
+        return np.array(out)
+
     def residual_of(self, prim):
         """flux_res at the state prim with delta held at its last value: the
         map lskum_flux_vjp / lskum_flux_jvp differentiate."""

@@ -94,6 +94,10 @@ typedef struct lskum_gen_params {
   int32_t k;        /* neighbours per point                             */
   double radius_factor; /* search radius in local spacings              */
   uint64_t seed;
+  double wall_blend; /* 0: cosine chord spacing of the wall points; w in  */
+                     /* (0, 1]: their arc-length positions blended, w to */
+                     /* 1 - w, between uniform arc length and cosine     */
+                     /* spacing (bounds the clustering at the edges)     */
 } lskum_gen_params;
 
 int lskum_cloud_generate(const lskum_gen_params *p, lskum_cloud_h **out);
