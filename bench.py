@@ -288,7 +288,10 @@ def run_mixed10m(args):
     }
 
 
-SHAPE_FLOW = dict(mach=0.8, aoa=1.25, cfl=0.2, vl_const=150.0, power=0.0, inner_iterations=0)
+# CFL 1: the iteration reaches its plateau (residue oscillating between
+# about -1.8 and -2.5, Cl 0.31-0.34) within 30,000 iterations, where CFL
+# 0.2 is still drifting after 120,000
+SHAPE_FLOW = dict(mach=0.8, aoa=1.25, cfl=1.0, vl_const=150.0, power=0.0, inner_iterations=0)
 FD_VARS = (1, 3, 5, 7, 9, 10, 12, 14, 16, 18)  # five bumps on each surface
 
 
@@ -296,7 +299,7 @@ def shape_measurement(args, n_fd=len(FD_VARS)):
     """Shape gradients (the adjoint workload): the 1,000,940-point NACA0012
     cloud naca0012_1m_k24 (wall 1,500 with blended spacing, 806 x 1,240
     O-grid, k = 24) at M 0.8, AoA 1.25 deg, the reference's limiter constant
-    K = 150, no inner passes, CFL 0.2, advanced args.converge iterations from
+    K = 150, no inner passes, CFL 1, advanced args.converge iterations from
     free stream as captured graphs (the state to linearise about); 20
     Hicks-Henne bumps (10 per surface, peaks over x/c 0.05..0.95, width
     exponent 3, inverse-distance propagation within 0.5 chord of the wall).
@@ -305,7 +308,8 @@ def shape_measurement(args, n_fd=len(FD_VARS)):
     (objectives x design variables) per second of sweep time.  Checked:
     central and one-sided differences (h = 1e-8) of the plain iterations
     along n_fd design fields, five on each surface, for both objectives from
-    the same runs.  A limiter or extremum switch within h of the state is a
+    the same runs (and central differences with h = 1e-6, for comparison).
+    A limiter or extremum switch within h of the state is a
     kink of J: the one-sided difference on the side free of it is the slope
     the adjoint takes, the central one is not; the one-sided spread says
     where that happened.  Also the tangent-linear run along one field per
@@ -342,11 +346,13 @@ def shape_measurement(args, n_fd=len(FD_VARS)):
     fd, h_fd = {}, 1e-8
     for j in FD_VARS[:n_fd]:
         c, f_, b_ = shape.design_fd_forces(s, fields[j], K, h=h_fd)
+        c6 = shape.design_fd_forces(s, fields[j], K, h=1e-6)[0]
         for k, name in enumerate(("cl", "cd")):
             g = float(grads[name][j])
             rel = [abs(float(d[k]) - g) / abs(g) for d in (c, f_, b_)]
             fd[f"{name}_var{j}"] = {"adjoint": g, "fd_central": float(c[k]),
                                     "rel_err": rel[0], "rel_err_best": min(rel),
+                                    "rel_err_central_h1e-6": abs(float(c6[k]) - g) / abs(g),
                                     "one_sided_spread_rel": abs(float(f_[k] - b_[k])) /
                                     abs(float(c[k]))}
         log(f"shape: fd var {j}: " + ", ".join(
@@ -379,6 +385,8 @@ def shape_measurement(args, n_fd=len(FD_VARS)):
         "fd_checks": {"h": h_fd, "count": len(fd),
                       "worst_rel_err": max(v["rel_err"] for v in fd.values()),
                       "worst_rel_err_best": max(v["rel_err_best"] for v in fd.values()),
+                      "worst_rel_err_central_h1e-6": max(v["rel_err_central_h1e-6"]
+                                                         for v in fd.values()),
                       "median_rel_err": float(np.median([v["rel_err"] for v in fd.values()])),
                       "per_variable": fd},
         "tangent_checks": {"worst_rel_err": max(v["rel_err"] for v in tan.values()),
@@ -420,7 +428,7 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="leave the shape-gradient (adjoint) measurement out of the line")
-    ap.add_argument("--converge", type=int, default=20000,
+    ap.add_argument("--converge", type=int, default=30000,
                     help="shape workload: iterations before the adjoint")
     ap.add_argument("--adjoint-iters", type=int, default=300,
                     help="shape workload: iterations the adjoint differentiates")

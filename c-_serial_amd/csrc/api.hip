@@ -146,11 +146,14 @@ int do_q_variables(lskum_ctx *c) {
 }
 
 // derivatives into the buffer from which n_later Jacobi passes end in buffer 0
-int do_q_derivatives(lskum_ctx *c, int n_later, bool keep_sums) {
+int do_q_derivatives(lskum_ctx *c, int n_later, bool keep_sums, bool with_delta) {
   c->cur = n_later % 2;
   {
     KTimer t(c, LSKUM_K_QDERIV);
-    if (c->mixed)
+    if (with_delta)
+      launch_q_derivatives_delta(c->dc(), c->base, c->dqb[c->cur], c->qm,
+                                 keep_sums ? c->qsum : nullptr, c->prm.cfl, c->delta, c->stream);
+    else if (c->mixed)
       launch_q_derivatives_m(c->dc(), c->ms(), c->dq32[c->cur], c->prm.power, c->stream);
     else
       launch_q_derivatives(c->dc(), c->base, c->dqb[c->cur], c->qm, keep_sums ? c->qsum : nullptr,
@@ -204,13 +207,16 @@ int do_state_update(lskum_ctx *c, int rk, bool with_hist) {
 }
 
 // One fpi_solver iteration (fpi_solver_mod.h:14-150) with q and the speed
-// already fresh; advances the hist slot counter.
+// already fresh; advances the hist slot counter.  func_delta reads the
+// iteration's starting state, as stage 1's q_derivatives does, so where the
+// pipelined kernel allows it the two share one pass.
 int run_iteration(lskum_ctx *c) {
   const int inner = c->prm.inner_iterations;
+  const bool fused_delta = c->qdd_chunks > 0;
   int rc = do_snapshot(c);
-  if (!rc) rc = do_func_delta(c);
+  if (!rc && !fused_delta) rc = do_func_delta(c);
   for (int rk = 1; rk <= 4 && !rc; rk++) {
-    rc = do_q_derivatives(c, inner, inner > 0);
+    rc = do_q_derivatives(c, inner, inner > 0, rk == 1 && fused_delta);
     if (!rc) rc = do_q_inner(c, inner);
     if (!rc) rc = do_flux(c);
     if (!rc) rc = do_state_update(c, rk, rk == 4);
@@ -664,9 +670,13 @@ int create_impl(const lskum_cloud *clouds, int32_t n_clouds, const lskum_case *c
     if (c->tiles & LSKUM_TILE_QINNER)
       c->qi_chunks = prepare_qinner_pf(TB, c->tile_max, c->slice_width, nblk, n_cu,
                                        (size_t)lds_limit, &c->qi_grid);
-    if (c->tiles & LSKUM_TILE_QDERIV)
+    if (c->tiles & LSKUM_TILE_QDERIV) {
       c->qd_chunks = prepare_qderiv_pf(TB, c->tile_max, c->slice_width, nblk, n_cu,
-                                       (size_t)lds_limit, &c->qd_grid);
+                                       (size_t)lds_limit, false, &c->qd_grid);
+      if (c->qd_chunks)
+        c->qdd_chunks = prepare_qderiv_pf(TB, c->tile_max, c->slice_width, nblk, n_cu,
+                                          (size_t)lds_limit, true, &c->qdd_grid);
+    }
   }
   // per-case constants and reduction ranges
   c->part.nblocks = blocks_for(n, 256);

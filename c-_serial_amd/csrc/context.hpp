@@ -82,6 +82,7 @@ struct lskum_ctx {
   int slice_width = 0;
   int qi_chunks = 0, qi_grid = 0;  // pipelined inner pass (0: not used)
   int qd_chunks = 0, qd_grid = 0;  // pipelined q_derivatives (0: not used)
+  int qdd_chunks = 0, qdd_grid = 0;  // the same with func_delta (0: not used)
   // cases: flow constants, 256-block -> case map, partial and force ranges
   FlowConsts *fcs = nullptr;
   uint16_t *block_case = nullptr;  // NULL for one case
@@ -145,6 +146,8 @@ struct lskum_ctx {
     c.qi_grid = qi_grid;
     c.qd_chunks = qd_chunks;
     c.qd_grid = qd_grid;
+    c.qdd_chunks = qdd_chunks;
+    c.qdd_grid = qdd_grid;
     c.xy = xy;
     c.nx = nx;
     c.ny = ny;
@@ -192,8 +195,9 @@ int do_speed(lskum_ctx *c);
 int do_func_delta(lskum_ctx *c);
 int do_q_variables(lskum_ctx *c);
 // keep_sums (FP64 form): also write qsum, which an inner pass that follows
-// reads (the single-stage entry point cannot know, so it always writes them)
-int do_q_derivatives(lskum_ctx *c, int n_later, bool keep_sums);
+// reads (the single-stage entry point cannot know, so it always writes them);
+// with_delta: func_delta in the same pass (only where qdd_chunks is set)
+int do_q_derivatives(lskum_ctx *c, int n_later, bool keep_sums, bool with_delta = false);
 int do_q_inner(lskum_ctx *c, int n_inner);
 int do_flux(lskum_ctx *c);
 int do_state_update(lskum_ctx *c, int rk, bool with_hist);

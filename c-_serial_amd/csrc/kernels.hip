@@ -48,6 +48,15 @@ __global__ __launch_bounds__(256) void k_speed(int n, const double *__restrict__
 }
 
 // ------------------------------------------------------------ local dt
+// One neighbour's step, state_update_mod.h:478-488 (speed: the neighbour's
+// |u| + 3 sqrt(p / rho)), its one fused product written out so that every
+// kernel inlining it agrees to the last bit.
+__device__ __forceinline__ double delta_step(double2 xy, double2 xyk, double speed, double cfl) {
+  const double dx = xyk.x - xy.x, dy = xyk.y - xy.y;
+  const double dist = sqrt(fma(dx, dx, dy * dy));
+  return cfl * (dist / speed);
+}
+
 // state_update_mod.h:448-496: min over the neighbours' states only.
 template <bool TILED>
 __global__ __launch_bounds__(256) void k_func_delta(int n, const int32_t *__restrict__ off,
@@ -96,10 +105,7 @@ __global__ __launch_bounds__(256) void k_func_delta(int n, const int32_t *__rest
     double speed_n;
     fetch(w_n, xy_n, speed_n);
     w_n = L.raw(min(e + 2, last), TILED);
-    double dist = (xyk.x - xy.x) * (xyk.x - xy.x) + (xyk.y - xy.y) * (xyk.y - xy.y);
-    dist = sqrt(dist);
-    double delta_t = dist / speed;
-    delta_t = cfl * delta_t;
+    const double delta_t = delta_step(xy, xyk, speed, cfl);
     if (min_delt > delta_t) min_delt = delta_t;
     xyk = xy_n;
     speed = speed_n;
@@ -564,55 +570,58 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 }
 
 // q_derivatives pipelined the same way (power 0, without the extremum
-// origins): tile entries of three chunks, x, y and q from the base records.
-template <int P>
+// origins): tile entries of C chunks from the base records, x, y and q
+// (C = 3), or the whole record with eps^2 and the speed (C = 4) where the
+// kernel also computes func_delta, which walks the same lists.
+template <int P, int C>
 __device__ __forceinline__ void qd_load_idx(int (&idx)[P], const QiBlock &q, int t, int TB,
                                             const int32_t *__restrict__ tile_idx) {
-  const int last = 3 * q.tn + q.nconn - 1;
+  const int last = C * q.tn + q.nconn - 1;
 #pragma unroll
   for (int u = 0; u < P; u++) {
     const int c = min(t + u * TB, last);
-    idx[u] = tile_idx[q.t0 + min(c / 3, q.tn - 1)];
+    idx[u] = tile_idx[q.t0 + min(c / C, q.tn - 1)];
   }
 }
 
-template <int P>
+template <int P, int C>
 __device__ __forceinline__ void qd_load_data(qi_chunk (&v)[P], const int (&idx)[P],
                                              const QiBlock &q, int t, int TB,
                                              const double *__restrict__ base,
                                              const uint16_t *__restrict__ conn16) {
-  const int last = 3 * q.tn + q.nconn - 1;
+  const int last = C * q.tn + q.nconn - 1;
   const qi_chunk *b2 = reinterpret_cast<const qi_chunk *>(base);
   const qi_chunk *cn2 = reinterpret_cast<const qi_chunk *>(conn16 + q.s0);
 #pragma unroll
   for (int u = 0; u < P; u++) {
     const int c = min(t + u * TB, last);
-    const qi_chunk *a_b = b2 + (B_STRIDE / 2) * (size_t)idx[u] + (c - 3 * (c / 3));
-    const qi_chunk *a_cn = cn2 + (c - 3 * q.tn);
-    v[u] = *(c < 3 * q.tn ? a_b : a_cn);
+    const qi_chunk *a_b = b2 + (B_STRIDE / 2) * (size_t)idx[u] + (c - C * (c / C));
+    const qi_chunk *a_cn = cn2 + (c - C * q.tn);
+    v[u] = *(c < C * q.tn ? a_b : a_cn);
   }
 }
 
-template <int P>
+template <int P, int C>
 __device__ __forceinline__ void qd_store_lds(qi_chunk *l2, const qi_chunk (&v)[P],
                                              const QiBlock &q, int t, int TB, int lds_conn2) {
-  const int last = 3 * q.tn + q.nconn - 1;
+  const int last = C * q.tn + q.nconn - 1;
 #pragma unroll
   for (int u = 0; u < P; u++) {
     const int c = min(t + u * TB, last);
-    l2[c < 3 * q.tn ? c : lds_conn2 + (c - 3 * q.tn)] = v[u];
+    l2[c < C * q.tn ? c : lds_conn2 + (c - C * q.tn)] = v[u];
   }
 }
 
-template <int P>
+template <int P, bool DELTA>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_q_derivatives_pf(
     int n, int nblk, int lds_conn, const int32_t *__restrict__ off,
     const int32_t *__restrict__ slice_off, const uint16_t *__restrict__ conn16,
     const int32_t *__restrict__ tile_off, const int32_t *__restrict__ tile_idx,
     const double *__restrict__ base, double *__restrict__ dq_out, double *__restrict__ qm_out,
-    double *__restrict__ qs_out) {
+    double *__restrict__ qs_out, double cfl, double *__restrict__ delta) {
   extern __shared__ double lds[];
-  constexpr int S = 6;  // x, y, q[4]
+  constexpr int C = DELTA ? 4 : 3;
+  constexpr int S = 2 * C;  // x, y, q[4] (, eps^2, speed)
   const uint16_t *cl = reinterpret_cast<const uint16_t *>(lds + lds_conn);
   qi_chunk *l2 = reinterpret_cast<qi_chunk *>(lds);
   const int t = threadIdx.x, TB = blockDim.x;
@@ -623,21 +632,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
   if (vb >= nblk) return;  // uniform per workgroup: no barrier is reached
   int b = xcd_map(vb, nblk);
   QiBlock q = qi_block(tile_off, slice_off, b);
-  qd_load_idx<P>(idx, q, t, TB, tile_idx);
-  qd_load_data<P>(v, idx, q, t, TB, base, conn16);
+  qd_load_idx<P, C>(idx, q, t, TB, tile_idx);
+  qd_load_data<P, C>(v, idx, q, t, TB, base, conn16);
   int vn = vb + G;
   QiBlock qn = vn < nblk ? qi_block(tile_off, slice_off, xcd_map(vn, nblk)) : q;
-  qd_load_idx<P>(idx, qn, t, TB, tile_idx);
+  qd_load_idx<P, C>(idx, qn, t, TB, tile_idx);
   for (;;) {
     __syncthreads();  // the previous block's arithmetic is done with the LDS
-    qd_store_lds<P>(l2, v, q, t, TB, lds_conn / 2);
+    qd_store_lds<P, C>(l2, v, q, t, TB, lds_conn / 2);
     const int i = b * TB + t;
     const int ii = min(i, n - 1);
     const int deg = off[ii + 1] - off[ii];
     const int vnn = vn + G;
     const QiBlock qnn = vnn < nblk ? qi_block(tile_off, slice_off, xcd_map(vnn, nblk)) : qn;
-    qd_load_data<P>(v, idx, qn, t, TB, base, conn16);
-    qd_load_idx<P>(idx, qnn, t, TB, tile_idx);
+    qd_load_data<P, C>(v, idx, qn, t, TB, base, conn16);
+    qd_load_idx<P, C>(idx, qnn, t, TB, tile_idx);
     __syncthreads();  // this block's tile and lists are in the LDS
     if (i < n) {
       const double *hi = lds + S * t;
@@ -646,32 +655,41 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
       load4(hi + 2, qi);
       double sum_delx_sqr = 0., sum_dely_sqr = 0., sum_delx_dely = 0.;
       double sdx[4] = {0.0, 0.0, 0.0, 0.0}, sdy[4] = {0.0, 0.0, 0.0, 0.0};
+      double min_delt = 1.0;
       for (int r = 0; r < 4; r++) qmax[r] = qmin[r] = qi[r];
-      auto fetch = [&](int e, double2 &xyk, double qk[4]) {
+      auto fetch = [&](int e, double2 &xyk, double qk[4], double &speed) {
         const double *hk = lds + S * (int)(cl[e * TB + t] & kLocalMask);
         xyk = reinterpret_cast<const double2 *>(hk)[0];
         load4(hk + 2, qk);
+        speed = DELTA ? hk[7] : 1.0;
       };
       const int last = deg - 1;
       double2 xyk = {0.0, 0.0};
       double qk[4] = {0.0, 0.0, 0.0, 0.0};
-      if (last >= 0) fetch(0, xyk, qk);
+      double speed = 1.0;
+      if (last >= 0) fetch(0, xyk, qk, speed);
       for (int e = 0; e <= last; e++) {
         double2 xy_n;
-        double q_n[4];
-        fetch(min(e + 1, last), xy_n, q_n);
+        double q_n[4], speed_n;
+        fetch(min(e + 1, last), xy_n, q_n, speed_n);
         for (int r = 0; r < 4; r++) {
           qmax[r] = fmax(qmax[r], qk[r]);
           qmin[r] = fmin(qmin[r], qk[r]);
         }
         q_deriv_term(xy.x, xy.y, qi, xyk, qk, sum_delx_sqr, sum_dely_sqr, sum_delx_dely, sdx,
                      sdy);
+        if (DELTA) {
+          const double delta_t = delta_step(xy, xyk, speed, cfl);
+          if (min_delt > delta_t) min_delt = delta_t;
+        }
         xyk = xy_n;
         for (int r = 0; r < 4; r++) qk[r] = q_n[r];
+        speed = speed_n;
       }
       q_deriv_finish(sum_delx_sqr, sum_dely_sqr, sum_delx_dely, sdx, sdy, qmax, qmin,
                      dq_out + (size_t)DQ_STRIDE * i, qm_out + (size_t)QM_STRIDE * i,
                      qs_out ? qs_out + 8 * (size_t)i : nullptr);
+      if (DELTA) delta[i] = min_delt;
     }
     if (vn >= nblk) break;
     vb = vn;
@@ -1312,25 +1330,27 @@ static int qi_pf_try(int tile_block, int tile_max, int slice_width, int nblk, in
   return P;
 }
 
-static size_t qd_pf_lds_conn(int tile_max) { return (size_t)6 * tile_max; }
-static size_t qd_pf_lds_bytes(int tile_block, int tile_max, int slice_width) {
-  return sizeof(double) * qd_pf_lds_conn(tile_max) +
+// C chunks per tile entry: 3, or 4 with func_delta
+static size_t qd_pf_lds_conn(int tile_max, int C) { return (size_t)2 * C * tile_max; }
+static size_t qd_pf_lds_bytes(int tile_block, int tile_max, int slice_width, int C) {
+  return sizeof(double) * qd_pf_lds_conn(tile_max, C) +
          sizeof(uint16_t) * (size_t)slice_width * tile_block;
 }
 
-template <int P>
+template <int P, bool DELTA>
 static int qd_pf_try(int tile_block, int tile_max, int slice_width, int nblk, int n_cu,
                      size_t lds_limit, int *grid) {
-  const size_t bytes = qd_pf_lds_bytes(tile_block, tile_max, slice_width);
-  const int chunks = 3 * tile_max + slice_width * tile_block / 8;
+  constexpr int C = DELTA ? 4 : 3;
+  const size_t bytes = qd_pf_lds_bytes(tile_block, tile_max, slice_width, C);
+  const int chunks = C * tile_max + slice_width * tile_block / 8;
   if (chunks > P * tile_block || bytes > lds_limit) return 0;
-  const void *f = (const void *)k_q_derivatives_pf<P>;
+  const void *f = (const void *)k_q_derivatives_pf<P, DELTA>;
   if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) !=
       hipSuccess)
     return 0;
   int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_q_derivatives_pf<P>, tile_block,
-                                                   bytes) != hipSuccess ||
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_q_derivatives_pf<P, DELTA>,
+                                                   tile_block, bytes) != hipSuccess ||
       per_cu <= 0)
     return 0;
   int g = per_cu * n_cu;
@@ -1340,12 +1360,18 @@ static int qd_pf_try(int tile_block, int tile_max, int slice_width, int nblk, in
 }
 
 int prepare_qderiv_pf(int tile_block, int tile_max, int slice_width, int nblk, int n_cu,
-                      size_t lds_limit, int *grid) {
+                      size_t lds_limit, bool with_delta, int *grid) {
   *grid = 0;
   if (tile_block % 8 || n_cu <= 0 || nblk <= 0) return 0;
-  int p = qd_pf_try<8>(tile_block, tile_max, slice_width, nblk, n_cu, lds_limit, grid);
-  if (!p) p = qd_pf_try<12>(tile_block, tile_max, slice_width, nblk, n_cu, lds_limit, grid);
-  if (!p) p = qd_pf_try<16>(tile_block, tile_max, slice_width, nblk, n_cu, lds_limit, grid);
+  int p = 0;
+  if (with_delta) {  // 20 chunks would spill: func_delta then runs on its own
+    p = qd_pf_try<12, true>(tile_block, tile_max, slice_width, nblk, n_cu, lds_limit, grid);
+    if (!p) p = qd_pf_try<16, true>(tile_block, tile_max, slice_width, nblk, n_cu, lds_limit, grid);
+    return p;
+  }
+  p = qd_pf_try<8, false>(tile_block, tile_max, slice_width, nblk, n_cu, lds_limit, grid);
+  if (!p) p = qd_pf_try<12, false>(tile_block, tile_max, slice_width, nblk, n_cu, lds_limit, grid);
+  if (!p) p = qd_pf_try<16, false>(tile_block, tile_max, slice_width, nblk, n_cu, lds_limit, grid);
   return p;
 }
 
@@ -1388,16 +1414,16 @@ void launch_q_derivatives(const DevCloud &c, const double *base, double *dq, dou
 #define QD_ARGS c.n, TILE_ARGS, base, dq, qm, qs, power
   if (t && power == 0.0 && c.qd_chunks) {
     const int nblk = blocks_for(c.n, c.tile_block);
-    const size_t Lp = qd_pf_lds_bytes(c.tile_block, c.tile_max, c.slice_width);
-    const int lc = (int)qd_pf_lds_conn(c.tile_max);
+    const size_t Lp = qd_pf_lds_bytes(c.tile_block, c.tile_max, c.slice_width, 3);
+    const int lc = (int)qd_pf_lds_conn(c.tile_max, 3);
 #define PF_ARGS c.n, nblk, lc, c.conn_off, c.slice_off, c.conn16S, c.tile_off, c.tile_idx, \
-                base, dq, qm, qs
+                base, dq, qm, qs, 0.0, nullptr
     if (c.qd_chunks == 8)
-      hipLaunchKernelGGL((k_q_derivatives_pf<8>), dim3(c.qd_grid), b, Lp, s, PF_ARGS);
+      hipLaunchKernelGGL((k_q_derivatives_pf<8, false>), dim3(c.qd_grid), b, Lp, s, PF_ARGS);
     else if (c.qd_chunks == 12)
-      hipLaunchKernelGGL((k_q_derivatives_pf<12>), dim3(c.qd_grid), b, Lp, s, PF_ARGS);
+      hipLaunchKernelGGL((k_q_derivatives_pf<12, false>), dim3(c.qd_grid), b, Lp, s, PF_ARGS);
     else
-      hipLaunchKernelGGL((k_q_derivatives_pf<16>), dim3(c.qd_grid), b, Lp, s, PF_ARGS);
+      hipLaunchKernelGGL((k_q_derivatives_pf<16, false>), dim3(c.qd_grid), b, Lp, s, PF_ARGS);
 #undef PF_ARGS
   } else if (t && power == 0.0)
     hipLaunchKernelGGL((k_q_derivatives<true, false>), g, b, L, s, QD_ARGS);
@@ -1408,6 +1434,21 @@ void launch_q_derivatives(const DevCloud &c, const double *base, double *dq, dou
   else
     hipLaunchKernelGGL((k_q_derivatives<false, true>), g, b, 0, s, QD_ARGS);
 #undef QD_ARGS
+}
+
+void launch_q_derivatives_delta(const DevCloud &c, const double *base, double *dq, double *qm,
+                                double *qs, double cfl, double *delta, hipStream_t s) {
+  const int nblk = blocks_for(c.n, c.tile_block);
+  const size_t Lp = qd_pf_lds_bytes(c.tile_block, c.tile_max, c.slice_width, 4);
+  const int lc = (int)qd_pf_lds_conn(c.tile_max, 4);
+  const dim3 g(c.qdd_grid), b(c.tile_block);
+#define PF_ARGS c.n, nblk, lc, c.conn_off, c.slice_off, c.conn16S, c.tile_off, c.tile_idx, \
+                base, dq, qm, qs, cfl, delta
+  if (c.qdd_chunks == 12)
+    hipLaunchKernelGGL((k_q_derivatives_pf<12, true>), g, b, Lp, s, PF_ARGS);
+  else
+    hipLaunchKernelGGL((k_q_derivatives_pf<16, true>), g, b, Lp, s, PF_ARGS);
+#undef PF_ARGS
 }
 
 void launch_q_derivatives_arg(const DevCloud &c, const double *base, double *dq, double *qm,

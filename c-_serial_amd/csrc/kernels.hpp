@@ -54,6 +54,8 @@ struct DevCloud {
   int qi_grid;               // its workgroups
   int qd_chunks;             // pipelined q_derivatives: chunks per thread (0: not used)
   int qd_grid;
+  int qdd_chunks;            // the same with func_delta (0: not used)
+  int qdd_grid;
   const double *xy;   // [n][2] coordinates (also in the base records)
   const double *nx, *ny;
   const double *coef;  // per point: least-squares coefficients of the
@@ -97,9 +99,10 @@ unsigned prepare_tiled(unsigned tiles, int tile_max, int flux_cap, size_t lds_li
 // *grid, the workgroups that fill n_cu CUs (a multiple of 8, at most nblk).
 int prepare_qinner_pf(int tile_block, int tile_max, int slice_width, int nblk, int n_cu,
                       size_t lds_limit, int *grid);
-// The same for the pipelined q_derivatives (tiled, power 0).
+// The same for the pipelined q_derivatives (tiled, power 0), with_delta:
+// the variant that also computes func_delta.
 int prepare_qderiv_pf(int tile_block, int tile_max, int slice_width, int nblk, int n_cu,
-                      size_t lds_limit, int *grid);
+                      size_t lds_limit, bool with_delta, int *grid);
 
 void launch_q_variables(const DevCloud &c, const double *prim, double *base, hipStream_t s);
 void launch_speed(const DevCloud &c, const double *prim, double *base, hipStream_t s);
@@ -109,6 +112,10 @@ void launch_func_delta(const DevCloud &c, const double *base, double cfl, double
 // (q_k - q_i), which the inner passes reuse (qs may be null when none run)
 void launch_q_derivatives(const DevCloud &c, const double *base, double *dq, double *qm,
                           double *qs, double power, hipStream_t s);
+// launch_q_derivatives (power 0) and launch_func_delta from the same base
+// records in one pipelined pass; only where c.qdd_chunks is set
+void launch_q_derivatives_delta(const DevCloud &c, const double *base, double *dq, double *qm,
+                                double *qs, double cfl, double *delta, hipStream_t s);
 void launch_q_inner(const DevCloud &c, const double *qs, const double *dq_src, double *dq_dst,
                     double power, hipStream_t s);
 // q_derivatives (power 0) that also writes arg [n][8], per qmax/qmin
