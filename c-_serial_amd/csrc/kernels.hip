@@ -753,7 +753,9 @@ __device__ __forceinline__ void load_flux_tile(double2 *lds, const int32_t *__re
       const int t = base + u * blockDim.x;
       if (t < total) {
         const int r = t >> 3, ch = t & 7;
-        lds[FLUX_TILE_CHUNKS * r + ch] = v[u];
+        // dq halved (exact): the flux only uses d/2 = (dx . dq)/2
+        const double sc = ch < 4 ? 0.5 : 1.0;
+        lds[FLUX_TILE_CHUNKS * r + ch] = {v[u].x * sc, v[u].y * sc};
       }
     }
   }
@@ -769,7 +771,7 @@ __device__ __forceinline__ BaseRec load_base(const double *__restrict__ base, ui
   return {p[0], p[1], p[2], p[3]};
 }
 
-// entry r's dq/dx, dq/dy, qmax, qmin from the flux tile
+// entry r's dq/dx / 2, dq/dy / 2, qmax, qmin from the flux tile
 __device__ __forceinline__ void flux_tile_entry(const double2 *lds, int r, double dx[4],
                                                 double dy[4], double qmx[4], double qmn[4]) {
   double2 c[FLUX_TILE_CHUNKS];
@@ -809,6 +811,7 @@ __global__ __launch_bounds__(256) void k_flux(
   const double *hi = base + (size_t)B_STRIDE * i;
   const double2 xy = reinterpret_cast<const double2 *>(hi)[0];
   const double x_i = xy.x, y_i = xy.y;
+  // dxi, dyi, dxk, dyk: the derivatives halved (exact; the tile holds them so)
   double qi[4], dxi[4], dyi[4], qmxi[4], qmni[4], cf[8];
   load4(hi + B_Q, qi);
   if (TILED) {
@@ -820,6 +823,10 @@ __global__ __launch_bounds__(256) void k_flux(
     load4(di + 4, dyi);
     load4(mi, qmxi);
     load4(mi + 4, qmni);
+    for (int r = 0; r < 4; r++) {
+      dxi[r] *= 0.5;
+      dyi[r] *= 0.5;
+    }
   }
   load4(coef + (size_t)COEF_STRIDE * i, cf);
   load4(coef + (size_t)COEF_STRIDE * i + 4, cf + 4);
@@ -867,6 +874,10 @@ __global__ __launch_bounds__(256) void k_flux(
       load4(dk + 4, dyk);
       load4(mk, qmxk);
       load4(mk + 4, qmnk);
+      for (int r = 0; r < 4; r++) {
+        dxk[r] *= 0.5;
+        dyk[r] *= 0.5;
+      }
     }
     const double eps2_k = r_c.eps_speed.x;
     const double delx = xyk.x - x_i, dely = xyk.y - y_i;
@@ -878,15 +889,17 @@ __global__ __launch_bounds__(256) void k_flux(
       dsw = dels * weights;
       dnw = deln * weights;
     }
-    // limited q~ at both ends (interior_fluxes_mod.h:72-83)
+    // limited q~ at both ends (interior_fluxes_mod.h:72-83) from the halved
+    // derivatives: d/2 = dx dq/dx/2 + dy dq/dy/2, and q - 0.5 phi d is
+    // q - phi (d/2)
     double qti[4], qtk[4];
     for (int r = 0; r < 4; r++) {
-      const double di = delx * dxi[r] + dely * dyi[r];
-      const double dk = delx * dxk[r] + dely * dyk[r];
-      const double phi_i = venkat_phi(qi[r] - 0.5 * di, qi[r], qmxi[r], qmni[r], eps2_i);
-      const double phi_k = venkat_phi(qk[r] - 0.5 * dk, qk[r], qmxk[r], qmnk[r], eps2_k);
-      qti[r] = qi[r] - 0.5 * phi_i * di;
-      qtk[r] = qk[r] - 0.5 * phi_k * dk;
+      const double hi_r = fma(delx, dxi[r], dely * dyi[r]);
+      const double hk_r = fma(delx, dxk[r], dely * dyk[r]);
+      const double phi_i = venkat_phi(qi[r] - hi_r, qi[r], qmxi[r], qmni[r], eps2_i);
+      const double phi_k = venkat_phi(qk[r] - hk_r, qk[r], qmxk[r], qmnk[r], eps2_k);
+      qti[r] = qi[r] - phi_i * hi_r;
+      qtk[r] = qk[r] - phi_k * hk_r;
     }
     double u1, u2, rho, sb, temp;
     qtilde_state(qti, u1, u2, rho, sb, temp);

@@ -25,20 +25,29 @@ constexpr double kGammaM1 = 1.4 - 1.0;          // gamma_new - 1 as evaluated
 constexpr double kInvGammaM1 = 1.0 / kGammaM1;
 constexpr double kInvSqrtPi = 0.56418958354775628695;  // 1/sqrt(pi)
 
-// a / b from the hardware reciprocal: two Newton steps and a final residual
-// correction (the refinement the correctly rounded division also performs,
-// without its scaling for operands near the ends of the exponent range);
-// within an ulp or two of a / b for the normal-range operands it gets here.
-// Other types (the dual numbers of the derivative kernels) divide plainly.
+// 1/b for normal b from the hardware estimate r0 (relative error e0 well
+// under 2^-22): with e = 1 - b r0, 1/b = r0 (1 + e + e^2 + e^3 + ...), and one
+// third-order step r0 + r0 (e + e^2) leaves e0^3 (2^-66 and below) plus its
+// own rounding -- the accuracy of two Newton steps in three FMAs instead of
+// four.
+__device__ __forceinline__ double rcp_refined(double b) {
+  const double r = __builtin_amdgcn_rcp(b);
+  const double e = fma(-b, r, 1.0);
+  return fma(r, fma(e, e, e), r);
+}
+
+// a / b from the refined reciprocal and a final residual correction (the
+// refinement the correctly rounded division also performs, without its
+// scaling for operands near the ends of the exponent range); within an ulp
+// or two of a / b for the normal-range operands it gets here.  Other types
+// (the dual numbers of the derivative kernels) divide plainly.
 template <typename T>
 __device__ __forceinline__ T qdiv(T a, T b) {
   return a / b;
 }
 template <>
 __device__ __forceinline__ double qdiv(double a, double b) {
-  double r = __builtin_amdgcn_rcp(b);
-  r = fma(fma(-b, r, 1.0), r, r);
-  r = fma(fma(-b, r, 1.0), r, r);
+  const double r = rcp_refined(b);
   const double q = a * r;
   return fma(fma(-b, q, a), r, q);
 }
@@ -72,10 +81,8 @@ __device__ __forceinline__ float exp_k(float a) {
 // n ln2 - a = inf - inf.  A NaN a gives a NaN r and result.  The callers
 // first raise a to -708, which keeps 2^n normal in place of the library's
 // underflow tests (e^-708 = 3e-308 stands for 0).
-__device__ __forceinline__ double exp_reduced(double a) {
-  const double n = fmin(rint(a * 1.4426950408889634), 1100.0);
-  double r = fma(-n, 0.6931471805599453, a);
-  r = fma(-n, 2.3190468138462996e-17, r);
+// e^r on |r| <= ln2/2 (the fit above)
+__device__ __forceinline__ double exp_poly(double r) {
   double p = 2.743938836864164e-07;
   p = fma(p, r, 2.7635062228304872e-06);
   p = fma(p, r, 2.4802048439065005e-05);
@@ -87,7 +94,29 @@ __device__ __forceinline__ double exp_reduced(double a) {
   p = fma(p, r, 0.49999999999997463);
   p = fma(p, r, 1.0);
   p = fma(p, r, 1.0);
-  return ldexp(p, (int)n);
+  return p;
+}
+
+__device__ __forceinline__ double exp_reduced(double a) {
+  const double n = fmin(rint(a * 1.4426950408889634), 1100.0);
+  double r = fma(-n, 0.6931471805599453, a);
+  r = fma(-n, 2.3190468138462996e-17, r);
+  return ldexp(exp_poly(r), (int)n);
+}
+
+// The same for a in [-708, 0] (or NaN), where no cap is needed: n =
+// rint(a / ln2) by the shifter 1.5 * 2^52, whose sum with a / ln2 lies in
+// [2^52, 2^53) and so is rounded to an integer, whose low 32 bits are n
+// itself (2^51 = 0 mod 2^32).  Two instructions fewer than rint, the cap
+// and the conversion.  n may differ from rint(a * (1/ln2)) by one at a
+// tie of the rounded product, which leaves |r| at ln2/2 to within rounding.
+__device__ __forceinline__ double exp_reduced_nonpos(double a) {
+  constexpr double kShift = 6755399441055744.0;  // 1.5 * 2^52
+  const double nb = fma(a, 1.4426950408889634, kShift);
+  const double n = nb - kShift;
+  double r = fma(-n, 0.6931471805599453, a);
+  r = fma(-n, 2.3190468138462996e-17, r);
+  return ldexp(exp_poly(r), (int)(uint32_t)__double_as_longlong(nb));
 }
 
 // The clamp compares, so a NaN a stays NaN.
@@ -105,37 +134,35 @@ __device__ __forceinline__ T exp_neg_sq(T a) {
 }
 template <>
 __device__ __forceinline__ double exp_neg_sq(double a) {
-  return exp_reduced(fmax(a, -708.0));
+  return exp_reduced_nonpos(fmax(a, -708.0));
 }
 
-// a / b without qdiv's last correction: the reciprocal after its two Newton
-// steps times a, within two ulps.  For quotients whose last bits do not
-// matter downstream (the limiter ratio, erf's rational).
+// a / b without qdiv's last correction: the refined reciprocal times a,
+// within two ulps.  For quotients whose last bits do not matter downstream
+// (the limiter ratio, erf's rational).
 template <typename T>
 __device__ __forceinline__ T qdiv_loose(T a, T b) {
   return qdiv(a, b);
 }
 template <>
 __device__ __forceinline__ double qdiv_loose(double a, double b) {
-  double r = __builtin_amdgcn_rcp(b);
-  r = fma(fma(-b, r, 1.0), r, r);
-  r = fma(fma(-b, r, 1.0), r, r);
-  return a * r;
+  return a * rcp_refined(b);
 }
 template <>
 __device__ __forceinline__ float qdiv_loose(float a, float b) {
   return a * __builtin_amdgcn_rcpf(b);
 }
 
-// 1/sqrt(b) for normal b > 0: the hardware estimate and two Newton steps
-// y <- y + y (1/2 - (b/2) y^2), each of which squares the relative error of
-// the last; within an ulp or two.
+// 1/sqrt(b) for normal b > 0 from the hardware estimate y0: with
+// e = 1 - b y0^2, 1/sqrt(b) = y0 (1 - e)^(-1/2) = y0 (1 + e/2 + 3e^2/8 + ...),
+// and the step y0 + y0 e (1/2 + 3e/8) leaves 5e^3/16 (the estimate is good
+// to 2^-24 here, so about 2^-70) plus its own rounding: within an ulp or
+// two, the accuracy of two Newton steps in five instructions instead of
+// seven.
 __device__ __forceinline__ double rsqrt_nr(double b) {
-  double y = __builtin_amdgcn_rsq(b);
-  const double hb = 0.5 * b;
-  y = fma(y, fma(-hb * y, y, 0.5), y);
-  y = fma(y, fma(-hb * y, y, 0.5), y);
-  return y;
+  const double y = __builtin_amdgcn_rsq(b);
+  const double e = fma(-b * y, y, 1.0);
+  return fma(y, e * fma(0.375, e, 0.5), y);
 }
 // In FP32 the estimate is within an ulp or so; one step rounds it off.
 __device__ __forceinline__ float rsqrt_nr(float b) {
@@ -264,57 +291,66 @@ __device__ __forceinline__ EndTerms<T> end_terms(T nx, T ny, T u1, T u2, T rho, 
   return end_terms_sb(nx, ny, u1, u2, rho, sqrt(T(0.5) * rho / pr), pr / rho);
 }
 
-// split_fluxes_mod.h:6-43 (s = +1, Gxp) and :45-84 (s = -1, Gxn)
+// split_fluxes_mod.h:6-43 (s = +1, Gxp) and :45-84 (s = -1, Gxn).
+// Regrouped, equal up to rounding: G[2] = rho (ut un A1 + s un B1) is
+// un G[0], formed as such.  The rest is the same numbers by fewer
+// operations, since products by s = +-1 and by powers of two are exact:
+//  - the sign s is applied once, to B1 (s (x B1) and x (s B1) are equal);
+//  - A1 = (1 + s erf)/2 is fma(s/2, erf, 1/2);
+//  - 0.5 ut (7 p/rho + |u|^2) A1 and 0.5 (6 p/rho + |u|^2) s B1 take the
+//    halves inside: (3.5 p/rho + |u|^2/2) and (3 p/rho + |u|^2/2), which
+//    the x and y fluxes of one end share.
+// split_y likewise with the roles of ut and un exchanged.
 template <typename T>
 __device__ __forceinline__ void split_x(const EndTerms<T> &e, T s, T G[4]) {
-  const T A1 = T(0.5) * (T(1.0) + s * e.e1);
-  G[0] = e.rho * (e.ut * A1 + s * e.B1);
-  T temp1 = e.pr_by_rho + e.ut * e.ut;
-  T temp2 = temp1 * A1 + s * (e.ut * e.B1);
+  const T sB1 = s * e.B1;
+  const T A1 = (T(0.5) * s) * e.e1 + T(0.5);
+  G[0] = e.rho * (e.ut * A1 + sB1);
+  const T temp1 = e.pr_by_rho + e.ut * e.ut;
+  const T temp2 = temp1 * A1 + e.ut * sB1;
   G[1] = e.rho * temp2;
-  temp1 = e.ut * e.un * A1 + s * (e.un * e.B1);
-  G[2] = e.rho * temp1;
-  temp1 = (T(7.0) * e.pr_by_rho) + e.u_sqr;
-  temp2 = T(0.5) * e.ut * temp1 * A1;
-  temp1 = (T(6.0) * e.pr_by_rho) + e.u_sqr;
-  G[3] = e.rho * (temp2 + s * (T(0.5) * temp1 * e.B1));
+  G[2] = e.un * G[0];
+  const T hu = T(0.5) * e.u_sqr;
+  const T t7h = T(3.5) * e.pr_by_rho + hu, t6h = T(3.0) * e.pr_by_rho + hu;
+  G[3] = e.rho * (e.ut * t7h * A1 + t6h * sB1);
 }
 
 // split_fluxes_mod.h:86-125 (s = +1, Gyp) and :127-164 (s = -1, Gyn)
 template <typename T>
 __device__ __forceinline__ void split_y(const EndTerms<T> &e, T s, T G[4]) {
-  const T A2 = T(0.5) * (T(1.0) + s * e.e2);
-  G[0] = e.rho * (e.un * A2 + s * e.B2);
-  T temp1 = e.pr_by_rho + e.un * e.un;
-  T temp2 = temp1 * A2 + s * (e.un * e.B2);
+  const T sB2 = s * e.B2;
+  const T A2 = (T(0.5) * s) * e.e2 + T(0.5);
+  G[0] = e.rho * (e.un * A2 + sB2);
+  const T temp1 = e.pr_by_rho + e.un * e.un;
+  const T temp2 = temp1 * A2 + e.un * sB2;
   G[2] = e.rho * temp2;
-  temp1 = e.ut * e.un * A2 + s * (e.ut * e.B2);
-  G[1] = e.rho * temp1;
-  temp1 = (T(7.0) * e.pr_by_rho) + e.u_sqr;
-  temp2 = T(0.5) * e.un * temp1 * A2;
-  temp1 = (T(6.0) * e.pr_by_rho) + e.u_sqr;
-  G[3] = e.rho * (temp2 + s * (T(0.5) * temp1 * e.B2));
+  G[1] = e.ut * G[0];
+  const T hu = T(0.5) * e.u_sqr;
+  const T t7h = T(3.5) * e.pr_by_rho + hu, t6h = T(3.0) * e.pr_by_rho + hu;
+  G[3] = e.rho * (e.un * t7h * A2 + t6h * sB2);
 }
 
 // quadrant_fluxes_mod.h: GxI :6 (s1 -, s2 -), GxII :57 (+, -),
-// GxIII :108 (+, +), GxIV :161 (-, +)
+// GxIII :108 (+, +), GxIV :161 (-, +); the signs and halves placed as in
+// split_x (the same numbers as the reference's grouping).
 template <typename T>
 __device__ __forceinline__ void quad_x(const EndTerms<T> &e, T s1, T s2, T G[4]) {
-  const T A1 = T(0.5) * (T(1.0) + s1 * e.e1);
-  const T A2 = T(0.5) * (T(1.0) + s2 * e.e2);
-  const T a1b1 = e.ut * A1 + s1 * e.B1;
+  const T sB1 = s1 * e.B1, sB2 = s2 * e.B2;
+  const T A1 = (T(0.5) * s1) * e.e1 + T(0.5);
+  const T A2 = (T(0.5) * s2) * e.e2 + T(0.5);
+  const T a1b1 = e.ut * A1 + sB1;
   G[0] = e.rho * A2 * a1b1;
-  T temp1 = e.pr_by_rho + e.ut * e.ut;
-  T temp2 = temp1 * A1 + s1 * (e.ut * e.B1);
+  const T temp1 = e.pr_by_rho + e.ut * e.ut;
+  T temp2 = temp1 * A1 + e.ut * sB1;
   G[1] = e.rho * A2 * temp2;
-  temp2 = e.un * A2 + s2 * e.B2;
+  temp2 = e.un * A2 + sB2;
   G[2] = e.rho * a1b1 * temp2;
-  temp1 = (T(7.0) * e.pr_by_rho) + e.u_sqr;
-  temp2 = T(0.5) * e.ut * temp1 * A1;
-  temp1 = (T(6.0) * e.pr_by_rho) + e.u_sqr;
-  const T temp3 = T(0.5) * e.B1 * temp1;
-  const T temp4 = T(0.5) * e.rho * e.un * e.B2 * a1b1;
-  G[3] = e.rho * A2 * (temp2 + s1 * temp3) + s2 * temp4;
+  const T hu = T(0.5) * e.u_sqr;
+  const T t7h = T(3.5) * e.pr_by_rho + hu, t6h = T(3.0) * e.pr_by_rho + hu;
+  temp2 = e.ut * t7h * A1;
+  const T temp3 = sB1 * t6h;
+  const T temp4 = T(0.5) * e.rho * e.un * sB2 * a1b1;
+  G[3] = e.rho * A2 * (temp2 + temp3) + temp4;
 }
 
 // The Venkatakrishnan ratio of limiters_mod.h:85-99.  There num and den
@@ -338,7 +374,7 @@ template <typename T>
 __device__ __forceinline__ T venkat_phi(T qtilde, T q, T qmax, T qmin, T eps2) {
   const T del_neg = qtilde - q;
   if (fabs(del_neg) <= T(10e-6)) return T(1.0);
-  return venkat_ratio(del_neg, del_neg > T(0) ? qmax - q : qmin - q, eps2);
+  return venkat_ratio(del_neg, (del_neg > T(0) ? qmax : qmin) - q, eps2);
 }
 
 // The same limiter from the increments themselves: del_neg = q~ - q =

@@ -430,19 +430,33 @@ def test_exp_reduction_constants():
     # 3.13, so the FMAs are emulated exactly with fractions) with the
     # constants read from the source and compared with the C library's exp:
     # within 4 ulps relative on [-700, 700] (the fit's own error is 3.3e-16),
-    # and the clamps of its two entry points at -708
+    # the clamps of its two entry points at -708, and the kinetic terms'
+    # variant for a <= 0, whose n comes from the 1.5 * 2^52 shifter
     import math
     import re
+    import struct
     from fractions import Fraction
     src = open(os.path.join(REPO, "c-_serial_amd", "csrc", "physics.hpp")).read()
-    body = src[src.index("__device__ __forceinline__ double exp_reduced(double a)"):]
-    body = body[:body.index("\n}\n")]
-    c = [float(x) for x in re.findall(r"p = fma\(p, r, ([0-9.e+-]+)\)", body)]
-    c0 = float(re.search(r"double p = ([0-9.e+-]+);", body).group(1))
+
+    def fn(sig):
+        b = src[src.index(sig):]
+        return b[:b.index("\n}\n")]
+
+    poly = fn("__device__ __forceinline__ double exp_poly(double r)")
+    body = fn("__device__ __forceinline__ double exp_reduced(double a)")
+    nonpos = fn("__device__ __forceinline__ double exp_reduced_nonpos(double a)")
+    c = [float(x) for x in re.findall(r"p = fma\(p, r, ([0-9.e+-]+)\)", poly)]
+    c0 = float(re.search(r"double p = ([0-9.e+-]+);", poly).group(1))
     inv_ln2 = float(re.search(r"rint\(a \* ([0-9.e+-]+)\)", body).group(1))
     hi, lo = (float(x) for x in re.findall(r"r = fma\(-n, ([0-9.e+-]+), [ar]\)", body))
     assert len(c) == 10 and c[-2:] == [1.0, 1.0]
     assert abs(hi + lo - math.log(2)) < 1e-30 + 2 ** -60 and abs(inv_ln2 * math.log(2) - 1) < 2e-16
+    # the variant reduces with the same constants
+    assert re.findall(r"r = fma\(-n, ([0-9.e+-]+), [ar]\)", nonpos) == \
+        re.findall(r"r = fma\(-n, ([0-9.e+-]+), [ar]\)", body)
+    shift = float(re.search(r"kShift = ([0-9.]+);", nonpos).group(1))
+    assert shift == 1.5 * 2.0 ** 52
+    assert re.search(r"fma\(a, " + re.escape(repr(inv_ln2)) + r", kShift\)", nonpos)
 
     def fma(a, b, d):
         if not all(math.isfinite(x) for x in (a, b, d)):
@@ -453,34 +467,54 @@ def test_exp_reduction_constants():
         except OverflowError:  # rounds past the largest double: +-inf
             return math.inf if v > 0 else -math.inf
 
-    def exp_reduced(a):
-        n = min(float(round(a * inv_ln2)), 1100.0)  # round half to even, as rint
+    def tail(n, a, n_int):
         r = fma(-n, lo, fma(-n, hi, a))
         p = c0
         for k in c:
             p = fma(p, r, k)
         try:
-            return math.ldexp(p, int(n))
+            return math.ldexp(p, n_int)
         except OverflowError:  # the device's ldexp overflows to inf
             return math.inf
 
+    def exp_reduced(a):
+        n = min(float(round(a * inv_ln2)), 1100.0)  # round half to even, as rint
+        return tail(n, a, int(n))
+
+    def exp_reduced_nonpos(a):
+        nb = fma(a, inv_ln2, shift)
+        n = nb - shift
+        low = struct.unpack("<q", struct.pack("<d", nb))[0] & 0xFFFFFFFF
+        n_int = low - (1 << 32) if low >= 1 << 31 else low
+        assert n_int == n  # the shifter's low word is n itself
+        return tail(n, a, n_int)
+
     def exp_k(a):
         return exp_reduced(-708.0 if a < -708.0 else a)
+
+    def exp_neg_sq(a):
+        return exp_reduced_nonpos(max(a, -708.0))
 
     worst = 0.0
     for a in np.concatenate([np.linspace(-700.0, 700.0, 4001), np.linspace(-2.0, 2.0, 4001)]):
         a = float(a)
         worst = max(worst, abs(exp_k(a) / math.exp(a) - 1.0))
+        if a <= 0.0:
+            worst = max(worst, abs(exp_neg_sq(a) / math.exp(a) - 1.0))
     assert worst <= 4 * 2.0 ** -53, worst
+    # ties of a / ln2 at half-integers, where the two roundings of n may differ
+    for k in range(-1020, 1, 7):
+        a = (k + 0.5) * math.log(2)
+        assert abs(exp_neg_sq(a) / math.exp(a) - 1.0) <= 4 * 2.0 ** -53
     assert exp_k(-800.0) == exp_k(-708.0) > 0.0
+    assert exp_neg_sq(-800.0) == exp_neg_sq(-708.0) > 0.0
+    assert exp_neg_sq(0.0) == 1.0
     # the entry points: exp_k clamps by a comparison (a NaN passes through
     # it), the kinetic terms' exp_neg_sq by one fmax
-    ek = src[src.index("__device__ __forceinline__ double exp_k(double a)"):]
-    ek = ek[:ek.index("\n}\n")]
+    ek = fn("__device__ __forceinline__ double exp_k(double a)")
     assert re.search(r"return exp_reduced\(a < -708\.0 \? -708\.0 : a\);", ek)
-    en = src[src.index("__device__ __forceinline__ double exp_neg_sq(double a)"):]
-    en = en[:en.index("\n}\n")]
-    assert re.search(r"return exp_reduced\(fmax\(a, -708\.0\)\);", en)
+    en = fn("__device__ __forceinline__ double exp_neg_sq(double a)")
+    assert re.search(r"return exp_reduced_nonpos\(fmax\(a, -708\.0\)\);", en)
     # the upper end: n is capped (keeping its conversion to int defined) and
     # the reduction uses the capped n, so past a = 762 r grows with a;
     # every coefficient being positive, p(r) then grows without bound and
@@ -489,11 +523,53 @@ def test_exp_reduction_constants():
     # not the inf - inf of an uncapped n); a NaN a makes r, p and the result
     # NaN
     assert re.search(r"const double n = fmin\(rint\(a \* [0-9.e+-]+\), 1100\.0\);", body)
-    assert re.search(r"ldexp\(p, \(int\)n\)", body)
+    assert re.search(r"ldexp\(exp_poly\(r\), \(int\)n\)", body)
     assert c0 > 0 and all(k > 0 for k in c)
     assert abs(exp_k(709.78) / math.exp(709.78) - 1.0) <= 4 * 2.0 ** -53
     for a in (709.79, 710.0, 762.0, 763.0, 1e4, 1e300):
         assert exp_k(a) == math.inf, a
+
+
+def test_reciprocal_and_rsqrt_refinements():
+    # physics.hpp's rcp_refined and rsqrt_nr refine the hardware estimate
+    # (relative error well under 2^-22) in one higher-order step.  Evaluated
+    # here with exact FMAs from estimates perturbed by up to 2^-22 either
+    # way, the results must be within 1.5 ulps of 1/b and 1/sqrt(b); the
+    # steps' constants are read from the source
+    import math
+    import re
+    from fractions import Fraction
+    src = open(os.path.join(REPO, "c-_serial_amd", "csrc", "physics.hpp")).read()
+    rc = src[src.index("__device__ __forceinline__ double rcp_refined(double b)"):]
+    rc = rc[:rc.index("\n}\n")]
+    rs = src[src.index("__device__ __forceinline__ double rsqrt_nr(double b)"):]
+    rs = rs[:rs.index("\n}\n")]
+    assert "const double e = fma(-b, r, 1.0);" in rc and "return fma(r, fma(e, e, e), r);" in rc
+    m = re.search(r"return fma\(y, e \* fma\(([0-9.]+), e, ([0-9.]+)\), y\);", rs)
+    assert m and "const double e = fma(-b * y, y, 1.0);" in rs
+    c1, c0 = float(m.group(1)), float(m.group(2))
+
+    def fma(a, b, d):
+        return float(Fraction(a) * Fraction(b) + Fraction(d))
+
+    def ulps(x, exact):
+        return abs(Fraction(x) - exact) / Fraction(math.ulp(float(exact)))
+
+    worst_r = worst_s = 0
+    rng = np.random.default_rng(3)
+    bs = np.concatenate([rng.uniform(1e-3, 1e3, 300), 10.0 ** rng.uniform(-30, 30, 300)])
+    for b in bs.tolist():
+        for d in (-2.0 ** -22, -2.0 ** -24, 0.0, 2.0 ** -24, 2.0 ** -22):
+            r0 = (1.0 / b) * (1.0 + d)
+            e = fma(-b, r0, 1.0)
+            r = fma(r0, fma(e, e, e), r0)
+            worst_r = max(worst_r, ulps(r, 1 / Fraction(b)))
+            y0 = (1.0 / math.sqrt(b)) * (1.0 + d)
+            e = fma(-(b * y0), y0, 1.0)
+            y = fma(y0, e * fma(c1, e, c0), y0)
+            exact = Fraction(1.0 / math.sqrt(b))  # within half an ulp of 1/sqrt(b)
+            worst_s = max(worst_s, ulps(y, exact) - Fraction(1, 2))
+    assert worst_r <= 1.5 and worst_s <= 1.5, (float(worst_r), float(worst_s))
 
 
 def test_normwise_is_strict_about_non_finite_values():
