@@ -269,7 +269,9 @@ __global__ __launch_bounds__(256) void k_q_derivatives(
 
 // ------------------------------------------------------------ inner passes
 // One neighbour's terms of an inner pass (q_variables_mod.h:150-195): the
-// least-squares geometry sums and w dx (-0.5 dx.(dq_k - dq_i)).
+// least-squares geometry sums and w dx (dx.(dq_k - dq_i)).  The reference's
+// factor -0.5 of each term is applied to the sums once, in q_inner_finish:
+// scaling by -0.5 is exact, so the sums are the same numbers.
 template <bool POW>
 __device__ __forceinline__ void q_inner_term(double x_i, double y_i, const double dxi[4],
                                              const double dyi[4], double2 xyk,
@@ -292,7 +294,7 @@ __device__ __forceinline__ void q_inner_term(double x_i, double y_i, const doubl
     // the sum of two products with one of them fused, written out: left to
     // the compiler, which product it fuses differed between the kernels
     // that inline this (a last-bit difference between layouts)
-    const double c = -0.5 * fma(delx, dxk[r] - dxi[r], dely * (dyk[r] - dyi[r]));
+    const double c = fma(delx, dxk[r] - dxi[r], dely * (dyk[r] - dyi[r]));
     if (POW) {
       sx[r] = sx[r] + (weights * delx * c);
       sy[r] = sy[r] + (weights * dely * c);
@@ -303,15 +305,16 @@ __device__ __forceinline__ void q_inner_term(double x_i, double y_i, const doubl
   }
 }
 
-// The pass's result (q_variables_mod.h:197-213): q_derivatives' q sums
-// (qx, qy) added, the 2x2 solve, the point's new derivatives stored at d.
+// The pass's result (q_variables_mod.h:197-213): the sums scaled by -0.5
+// and q_derivatives' q sums (qx, qy) added, the 2x2 solve, the point's new
+// derivatives stored at d.
 __device__ __forceinline__ void q_inner_finish(const double qx[4], const double qy[4],
                                                double sum_delx_sqr, double sum_dely_sqr,
                                                double sum_delx_dely, double sx[4],
                                                double sy[4], double *d) {
   for (int r = 0; r < 4; r++) {
-    sx[r] = qx[r] + sx[r];
-    sy[r] = qy[r] + sy[r];
+    sx[r] = qx[r] + (-0.5 * sx[r]);
+    sy[r] = qy[r] + (-0.5 * sy[r]);
   }
   const double det = fma(sum_delx_sqr, sum_dely_sqr, -(sum_delx_dely * sum_delx_dely));
   // det > 0 for every validated point; 0 only for a batch's padding points,
@@ -408,14 +411,17 @@ __global__ __launch_bounds__(256) void k_q_inner(
 // coordinates and derivatives and its neighbour-list slice, as 16-byte
 // chunks -- is in flight to registers, and block j+2's tile indices before
 // it; the arithmetic reads its lists from LDS, so none of it waits on a
-// global load.  Every thread loads exactly P chunks of a block (past the
-// end it reloads the last one), so the compiler's memory-operation counts
-// stay exact.  The per-point arithmetic is k_q_inner's (q_inner_term,
-// q_inner_finish), so results are bit-identical.
+// global load.  Each thread holds E entry slots (tile entry t + u TB) with
+// their chunks and NC slots of the list slice (chunk t + u TB); a slot past
+// the end takes the last entry or chunk again and writes the same bytes to
+// the same place, so every thread issues the same loads and the compiler's
+// memory-operation counts stay exact.  The kind of every slot is fixed at
+// compile time, so addresses take a few integer operations.  The per-point
+// arithmetic is k_q_inner's (q_inner_term, q_inner_finish), so results are
+// bit-identical.
 struct QiBlock {
   int t0, tn, nconn;  // first tile entry, tile entries, slice chunks
   int s0;             // first slice entry
-  __device__ __forceinline__ int chunks() const { return 5 * tn + nconn; }
 };
 
 __device__ __forceinline__ QiBlock qi_block(const int32_t *__restrict__ tile_off,
@@ -428,65 +434,75 @@ __device__ __forceinline__ QiBlock qi_block(const int32_t *__restrict__ tile_off
   return q;
 }
 
-// Chunk u of this thread (t + u * tile_block, the last one past the end)
-// and where it comes from and goes: the coordinates of entry c (from xy),
-// chunk (c - tn) & 3 of entry (c - tn) / 4's derivatives (from src), or
-// chunk c - 5 tn of the list slice.  Addresses and positions are selected
-// without branches, so the loads stay in flight in registers.
 // A 16-byte chunk in registers: a native vector type, which the compiler
 // keeps in registers where an array of HIP's double2 struct, copied from
 // global memory, went to scratch.
 typedef double qi_chunk __attribute__((ext_vector_type(2)));
 
-template <int P>
-__device__ __forceinline__ void qi_load_idx(int (&idx)[P], const QiBlock &q, int t, int TB,
+// the tile indices of this thread's entry slots (every block has tn >= 1:
+// its own points)
+template <int E>
+__device__ __forceinline__ void pf_load_idx(int (&idx)[E], const QiBlock &q, int t, int TB,
                                             const int32_t *__restrict__ tile_idx) {
-  const int last = q.chunks() - 1;
 #pragma unroll
-  for (int u = 0; u < P; u++) {
-    const int c = min(t + u * TB, last);
-    const int e = c < q.tn ? c : min((c - q.tn) >> 2, q.tn - 1);
-    idx[u] = tile_idx[q.t0 + e];
-  }
+  for (int u = 0; u < E; u++) idx[u] = tile_idx[q.t0 + min(t + u * TB, q.tn - 1)];
 }
 
-template <int P>
-__device__ __forceinline__ void qi_load_data(qi_chunk (&v)[P], const int (&idx)[P],
+// the list-slice chunks into v[at .. at + NC); a block whose points have no
+// neighbours (nconn = 0) reads its first tile entry's coordinates instead
+// and, in pf_store_conn, writes nothing
+template <int NC, int NV>
+__device__ __forceinline__ void pf_load_conn(qi_chunk (&v)[NV], int at, const QiBlock &q, int t,
+                                             int TB, const uint16_t *__restrict__ conn16,
+                                             const qi_chunk *fallback) {
+  const qi_chunk *cn2 = q.nconn > 0 ? reinterpret_cast<const qi_chunk *>(conn16 + q.s0)
+                                    : fallback;
+  const int last = max(q.nconn - 1, 0);
+#pragma unroll
+  for (int u = 0; u < NC; u++) v[at + u] = cn2[min(t + u * TB, last)];
+}
+
+template <int NC, int NV>
+__device__ __forceinline__ void pf_store_conn(qi_chunk *l2, const qi_chunk (&v)[NV], int at,
+                                              const QiBlock &q, int t, int TB, int lds_conn2) {
+  if (q.nconn <= 0) return;  // uniform per block
+#pragma unroll
+  for (int u = 0; u < NC; u++) l2[lds_conn2 + min(t + u * TB, q.nconn - 1)] = v[at + u];
+}
+
+// q_inner: entry slot u holds the coordinates (from xy) and the four
+// derivative chunks (from src) of its tile entry, stored at LDS chunk 5 e
+template <int E, int NC>
+__device__ __forceinline__ void qi_load_data(qi_chunk (&v)[5 * E + NC], const int (&idx)[E],
                                              const QiBlock &q, int t, int TB,
                                              const double *__restrict__ xy,
                                              const double *__restrict__ src,
                                              const uint16_t *__restrict__ conn16) {
-  const int last = q.chunks() - 1;
   const qi_chunk *xy2 = reinterpret_cast<const qi_chunk *>(xy);
   const qi_chunk *src2 = reinterpret_cast<const qi_chunk *>(src);
-  const qi_chunk *cn2 = reinterpret_cast<const qi_chunk *>(conn16 + q.s0);
 #pragma unroll
-  for (int u = 0; u < P; u++) {
-    const int c = min(t + u * TB, last);
+  for (int u = 0; u < E; u++) {
     const size_t g = (size_t)idx[u];
-    const qi_chunk *a_xy = xy2 + g;
-    const qi_chunk *a_dq = src2 + (DQ_STRIDE / 2) * g + ((c - q.tn) & 3);
-    const qi_chunk *a_cn = cn2 + (c - 5 * q.tn);
-    const qi_chunk *a = c < q.tn ? a_xy : (c < 5 * q.tn ? a_dq : a_cn);
-    v[u] = *a;
-  }
-}
-
-template <int P>
-__device__ __forceinline__ void qi_store_lds(qi_chunk *l2, const qi_chunk (&v)[P], const QiBlock &q,
-                                             int t, int TB, int lds_conn2) {
-  const int last = q.chunks() - 1;
+    v[5 * u] = xy2[g];
 #pragma unroll
-  for (int u = 0; u < P; u++) {
-    const int c = min(t + u * TB, last);
-    const int at_xy = 5 * c;
-    const int at_dq = 5 * ((c - q.tn) >> 2) + 1 + ((c - q.tn) & 3);
-    const int at_cn = lds_conn2 + (c - 5 * q.tn);
-    l2[c < q.tn ? at_xy : (c < 5 * q.tn ? at_dq : at_cn)] = v[u];
+    for (int j = 0; j < 4; j++) v[5 * u + 1 + j] = src2[(DQ_STRIDE / 2) * g + j];
   }
+  pf_load_conn<NC>(v, 5 * E, q, t, TB, conn16, xy2 + idx[0]);
 }
 
-template <int P>
+template <int E, int NC>
+__device__ __forceinline__ void qi_store_lds(qi_chunk *l2, const qi_chunk (&v)[5 * E + NC],
+                                             const QiBlock &q, int t, int TB, int lds_conn2) {
+#pragma unroll
+  for (int u = 0; u < E; u++) {
+    const int e = min(t + u * TB, q.tn - 1);
+#pragma unroll
+    for (int j = 0; j < 5; j++) l2[5 * e + j] = v[5 * u + j];
+  }
+  pf_store_conn<NC>(l2, v, 5 * E, q, t, TB, lds_conn2);
+}
+
+template <int E, int NC>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_q_inner_pf(
     int n, int nblk, int lds_conn, const int32_t *__restrict__ off,
     const int32_t *__restrict__ slice_off, const uint16_t *__restrict__ conn16,
@@ -499,22 +515,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
   qi_chunk *l2 = reinterpret_cast<qi_chunk *>(lds);
   const int t = threadIdx.x, TB = blockDim.x;
   const int G = gridDim.x;
-  int idx[P];
-  qi_chunk v[P];
+  int idx[E];
+  qi_chunk v[5 * E + NC];
   int vb = blockIdx.x;
   if (vb >= nblk) return;  // uniform per workgroup: no barrier is reached
   int b = xcd_map(vb, nblk);
   QiBlock q = qi_block(tile_off, slice_off, b);
-  qi_load_idx<P>(idx, q, t, TB, tile_idx);
-  qi_load_data<P>(v, idx, q, t, TB, xy, src, conn16);
+  pf_load_idx<E>(idx, q, t, TB, tile_idx);
+  qi_load_data<E, NC>(v, idx, q, t, TB, xy, src, conn16);
   // past the end of the sequence the next block is the current one again:
   // its loads are redundant but keep every pass through the loop alike
   int vn = vb + G;
   QiBlock qn = vn < nblk ? qi_block(tile_off, slice_off, xcd_map(vn, nblk)) : q;
-  qi_load_idx<P>(idx, qn, t, TB, tile_idx);
+  pf_load_idx<E>(idx, qn, t, TB, tile_idx);
   for (;;) {
     __syncthreads();  // the previous block's arithmetic is done with the LDS
-    qi_store_lds<P>(l2, v, q, t, TB, lds_conn / 2);
+    qi_store_lds<E, NC>(l2, v, q, t, TB, lds_conn / 2);
     const int i = b * TB + t;
     const int ii = min(i, n - 1);
     const int e_off = off[ii], deg = off[ii + 1] - e_off;
@@ -523,8 +539,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     load4(qs + 8 * (size_t)ii + 4, qy);
     const int vnn = vn + G;
     const QiBlock qnn = vnn < nblk ? qi_block(tile_off, slice_off, xcd_map(vnn, nblk)) : qn;
-    qi_load_data<P>(v, idx, qn, t, TB, xy, src, conn16);
-    qi_load_idx<P>(idx, qnn, t, TB, tile_idx);
+    qi_load_data<E, NC>(v, idx, qn, t, TB, xy, src, conn16);
+    pf_load_idx<E>(idx, qnn, t, TB, tile_idx);
     __syncthreads();  // this block's tile and lists are in the LDS
     if (i < n) {
       const double *hi = lds + S * t;
@@ -570,49 +586,37 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 }
 
 // q_derivatives pipelined the same way (power 0, without the extremum
-// origins): tile entries of C chunks from the base records, x, y and q
-// (C = 3), or the whole record with eps^2 and the speed (C = 4) where the
-// kernel also computes func_delta, which walks the same lists.
-template <int P, int C>
-__device__ __forceinline__ void qd_load_idx(int (&idx)[P], const QiBlock &q, int t, int TB,
-                                            const int32_t *__restrict__ tile_idx) {
-  const int last = C * q.tn + q.nconn - 1;
-#pragma unroll
-  for (int u = 0; u < P; u++) {
-    const int c = min(t + u * TB, last);
-    idx[u] = tile_idx[q.t0 + min(c / C, q.tn - 1)];
-  }
-}
-
-template <int P, int C>
-__device__ __forceinline__ void qd_load_data(qi_chunk (&v)[P], const int (&idx)[P],
+// origins): entry slot u holds C chunks of its tile entry's base record,
+// x, y and q (C = 3), or the whole record with eps^2 and the speed (C = 4)
+// where the kernel also computes func_delta, which walks the same lists.
+template <int E, int C, int NC>
+__device__ __forceinline__ void qd_load_data(qi_chunk (&v)[C * E + NC], const int (&idx)[E],
                                              const QiBlock &q, int t, int TB,
                                              const double *__restrict__ base,
                                              const uint16_t *__restrict__ conn16) {
-  const int last = C * q.tn + q.nconn - 1;
   const qi_chunk *b2 = reinterpret_cast<const qi_chunk *>(base);
-  const qi_chunk *cn2 = reinterpret_cast<const qi_chunk *>(conn16 + q.s0);
 #pragma unroll
-  for (int u = 0; u < P; u++) {
-    const int c = min(t + u * TB, last);
-    const qi_chunk *a_b = b2 + (B_STRIDE / 2) * (size_t)idx[u] + (c - C * (c / C));
-    const qi_chunk *a_cn = cn2 + (c - C * q.tn);
-    v[u] = *(c < C * q.tn ? a_b : a_cn);
+  for (int u = 0; u < E; u++) {
+    const size_t g = (size_t)idx[u];
+#pragma unroll
+    for (int j = 0; j < C; j++) v[C * u + j] = b2[(B_STRIDE / 2) * g + j];
   }
+  pf_load_conn<NC>(v, C * E, q, t, TB, conn16, b2 + (B_STRIDE / 2) * (size_t)idx[0]);
 }
 
-template <int P, int C>
-__device__ __forceinline__ void qd_store_lds(qi_chunk *l2, const qi_chunk (&v)[P],
+template <int E, int C, int NC>
+__device__ __forceinline__ void qd_store_lds(qi_chunk *l2, const qi_chunk (&v)[C * E + NC],
                                              const QiBlock &q, int t, int TB, int lds_conn2) {
-  const int last = C * q.tn + q.nconn - 1;
 #pragma unroll
-  for (int u = 0; u < P; u++) {
-    const int c = min(t + u * TB, last);
-    l2[c < C * q.tn ? c : lds_conn2 + (c - C * q.tn)] = v[u];
+  for (int u = 0; u < E; u++) {
+    const int e = min(t + u * TB, q.tn - 1);
+#pragma unroll
+    for (int j = 0; j < C; j++) l2[C * e + j] = v[C * u + j];
   }
+  pf_store_conn<NC>(l2, v, C * E, q, t, TB, lds_conn2);
 }
 
-template <int P, bool DELTA>
+template <int E, int NC, bool DELTA>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_q_derivatives_pf(
     int n, int nblk, int lds_conn, const int32_t *__restrict__ off,
     const int32_t *__restrict__ slice_off, const uint16_t *__restrict__ conn16,
@@ -626,27 +630,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
   qi_chunk *l2 = reinterpret_cast<qi_chunk *>(lds);
   const int t = threadIdx.x, TB = blockDim.x;
   const int G = gridDim.x;
-  int idx[P];
-  qi_chunk v[P];
+  int idx[E];
+  qi_chunk v[C * E + NC];
   int vb = blockIdx.x;
   if (vb >= nblk) return;  // uniform per workgroup: no barrier is reached
   int b = xcd_map(vb, nblk);
   QiBlock q = qi_block(tile_off, slice_off, b);
-  qd_load_idx<P, C>(idx, q, t, TB, tile_idx);
-  qd_load_data<P, C>(v, idx, q, t, TB, base, conn16);
+  pf_load_idx<E>(idx, q, t, TB, tile_idx);
+  qd_load_data<E, C, NC>(v, idx, q, t, TB, base, conn16);
   int vn = vb + G;
   QiBlock qn = vn < nblk ? qi_block(tile_off, slice_off, xcd_map(vn, nblk)) : q;
-  qd_load_idx<P, C>(idx, qn, t, TB, tile_idx);
+  pf_load_idx<E>(idx, qn, t, TB, tile_idx);
   for (;;) {
     __syncthreads();  // the previous block's arithmetic is done with the LDS
-    qd_store_lds<P, C>(l2, v, q, t, TB, lds_conn / 2);
+    qd_store_lds<E, C, NC>(l2, v, q, t, TB, lds_conn / 2);
     const int i = b * TB + t;
     const int ii = min(i, n - 1);
     const int deg = off[ii + 1] - off[ii];
     const int vnn = vn + G;
     const QiBlock qnn = vnn < nblk ? qi_block(tile_off, slice_off, xcd_map(vnn, nblk)) : qn;
-    qd_load_data<P, C>(v, idx, qn, t, TB, base, conn16);
-    qd_load_idx<P, C>(idx, qnn, t, TB, tile_idx);
+    qd_load_data<E, C, NC>(v, idx, qn, t, TB, base, conn16);
+    pf_load_idx<E>(idx, qnn, t, TB, tile_idx);
     __syncthreads();  // this block's tile and lists are in the LDS
     if (i < n) {
       const double *hi = lds + S * t;
@@ -1316,86 +1320,72 @@ unsigned prepare_tiled(unsigned tiles, int tile_max, int flux_cap, size_t lds_li
 
 // The pipelined inner pass's LDS: the tile, then the block's list slice
 // (double offset lds_conn).
-static size_t qi_pf_lds_conn(int tile_max) { return (size_t)10 * tile_max; }
-static size_t qi_pf_lds_bytes(int tile_block, int tile_max, int slice_width) {
-  return sizeof(double) * qi_pf_lds_conn(tile_max) +
-         sizeof(uint16_t) * (size_t)slice_width * tile_block;
+// The pipelined kernels' layouts: E entry slots and NC list-slice chunks per
+// thread, E tile_block >= tile_max and 8 NC >= slice_width (a block's slice
+// is slice_width entries of 2 bytes per point).  A layout code is 10 E + NC.
+#define PF_LAYOUTS(X) X(2, 3) X(2, 4) X(3, 3) X(3, 4) X(4, 3) X(4, 4)
+
+static int pf_layout(int tile_block, int tile_max, int slice_width) {
+  const int E = (tile_max + tile_block - 1) / tile_block, NC = (slice_width + 7) / 8;
+  const int code = 10 * E + std::max(NC, 3);
+#define PF_HAS(e, nc) if (code == 10 * e + nc) return code;
+  PF_LAYOUTS(PF_HAS)
+#undef PF_HAS
+  return 0;
 }
 
-template <int P>
-static int qi_pf_try(int tile_block, int tile_max, int slice_width, int nblk, int n_cu,
-                     size_t lds_limit, int *grid) {
-  const size_t bytes = qi_pf_lds_bytes(tile_block, tile_max, slice_width);
-  const int chunks = 5 * tile_max + slice_width * tile_block / 8;
-  if (chunks > P * tile_block || bytes > lds_limit) return 0;
-  const void *f = (const void *)k_q_inner_pf<P>;
-  if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) !=
-      hipSuccess)
+// lds_conn: the double offset of the list slice, after S doubles per tile entry
+static size_t pf_lds_bytes(int tile_block, int tile_max, int slice_width, int S) {
+  return sizeof(double) * (size_t)S * tile_max + sizeof(uint16_t) * (size_t)slice_width * tile_block;
+}
+
+// the workgroups that fill n_cu CUs with kernel f (a multiple of 8, at most
+// nblk), or 0 where it cannot run with these LDS bytes
+static int pf_grid(const void *f, int tile_block, size_t bytes, size_t lds_limit, int nblk,
+                   int n_cu) {
+  if (bytes > lds_limit ||
+      hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) != hipSuccess)
     return 0;
   int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_q_inner_pf<P>, tile_block,
-                                                   bytes) != hipSuccess ||
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, tile_block, bytes) != hipSuccess ||
       per_cu <= 0)
     return 0;
   int g = per_cu * n_cu;
   g = g >= 8 ? g / 8 * 8 : g;
-  *grid = std::min(g, nblk);
-  return P;
-}
-
-// C chunks per tile entry: 3, or 4 with func_delta
-static size_t qd_pf_lds_conn(int tile_max, int C) { return (size_t)2 * C * tile_max; }
-static size_t qd_pf_lds_bytes(int tile_block, int tile_max, int slice_width, int C) {
-  return sizeof(double) * qd_pf_lds_conn(tile_max, C) +
-         sizeof(uint16_t) * (size_t)slice_width * tile_block;
-}
-
-template <int P, bool DELTA>
-static int qd_pf_try(int tile_block, int tile_max, int slice_width, int nblk, int n_cu,
-                     size_t lds_limit, int *grid) {
-  constexpr int C = DELTA ? 4 : 3;
-  const size_t bytes = qd_pf_lds_bytes(tile_block, tile_max, slice_width, C);
-  const int chunks = C * tile_max + slice_width * tile_block / 8;
-  if (chunks > P * tile_block || bytes > lds_limit) return 0;
-  const void *f = (const void *)k_q_derivatives_pf<P, DELTA>;
-  if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) !=
-      hipSuccess)
-    return 0;
-  int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_q_derivatives_pf<P, DELTA>,
-                                                   tile_block, bytes) != hipSuccess ||
-      per_cu <= 0)
-    return 0;
-  int g = per_cu * n_cu;
-  g = g >= 8 ? g / 8 * 8 : g;
-  *grid = std::min(g, nblk);
-  return P;
-}
-
-int prepare_qderiv_pf(int tile_block, int tile_max, int slice_width, int nblk, int n_cu,
-                      size_t lds_limit, bool with_delta, int *grid) {
-  *grid = 0;
-  if (tile_block % 8 || n_cu <= 0 || nblk <= 0) return 0;
-  int p = 0;
-  if (with_delta) {  // 20 chunks would spill: func_delta then runs on its own
-    p = qd_pf_try<12, true>(tile_block, tile_max, slice_width, nblk, n_cu, lds_limit, grid);
-    if (!p) p = qd_pf_try<16, true>(tile_block, tile_max, slice_width, nblk, n_cu, lds_limit, grid);
-    return p;
-  }
-  p = qd_pf_try<8, false>(tile_block, tile_max, slice_width, nblk, n_cu, lds_limit, grid);
-  if (!p) p = qd_pf_try<12, false>(tile_block, tile_max, slice_width, nblk, n_cu, lds_limit, grid);
-  if (!p) p = qd_pf_try<16, false>(tile_block, tile_max, slice_width, nblk, n_cu, lds_limit, grid);
-  return p;
+  return std::min(g, nblk);
 }
 
 int prepare_qinner_pf(int tile_block, int tile_max, int slice_width, int nblk, int n_cu,
                       size_t lds_limit, int *grid) {
   *grid = 0;
   if (tile_block % 8 || n_cu <= 0 || nblk <= 0) return 0;
-  int p = qi_pf_try<12>(tile_block, tile_max, slice_width, nblk, n_cu, lds_limit, grid);
-  if (!p) p = qi_pf_try<16>(tile_block, tile_max, slice_width, nblk, n_cu, lds_limit, grid);
-  if (!p) p = qi_pf_try<20>(tile_block, tile_max, slice_width, nblk, n_cu, lds_limit, grid);
-  return p;
+  const int code = pf_layout(tile_block, tile_max, slice_width);
+  const size_t bytes = pf_lds_bytes(tile_block, tile_max, slice_width, 10);
+  const void *f = nullptr;
+#define PF_F(e, nc) if (code == 10 * e + nc) f = (const void *)k_q_inner_pf<e, nc>;
+  PF_LAYOUTS(PF_F)
+#undef PF_F
+  if (!f) return 0;
+  *grid = pf_grid(f, tile_block, bytes, lds_limit, nblk, n_cu);
+  return *grid ? code : 0;
+}
+
+int prepare_qderiv_pf(int tile_block, int tile_max, int slice_width, int nblk, int n_cu,
+                      size_t lds_limit, bool with_delta, int *grid) {
+  *grid = 0;
+  if (tile_block % 8 || n_cu <= 0 || nblk <= 0) return 0;
+  const int code = pf_layout(tile_block, tile_max, slice_width);
+  const size_t bytes = pf_lds_bytes(tile_block, tile_max, slice_width, with_delta ? 8 : 6);
+  const void *f = nullptr;
+#define PF_F(e, nc)                                                                   \
+  if (code == 10 * e + nc)                                                            \
+    f = with_delta ? (const void *)k_q_derivatives_pf<e, nc, true>                    \
+                   : (const void *)k_q_derivatives_pf<e, nc, false>;
+  PF_LAYOUTS(PF_F)
+#undef PF_F
+  if (!f) return 0;
+  *grid = pf_grid(f, tile_block, bytes, lds_limit, nblk, n_cu);
+  return *grid ? code : 0;
 }
 
 #define TILE_ARGS c.conn_off, c.slice_off, c.connS, c.conn16S, c.tile_off, c.tile_idx
@@ -1427,17 +1417,15 @@ void launch_q_derivatives(const DevCloud &c, const double *base, double *dq, dou
 #define QD_ARGS c.n, TILE_ARGS, base, dq, qm, qs, power
   if (t && power == 0.0 && c.qd_chunks) {
     const int nblk = blocks_for(c.n, c.tile_block);
-    const size_t Lp = qd_pf_lds_bytes(c.tile_block, c.tile_max, c.slice_width, 3);
-    const int lc = (int)qd_pf_lds_conn(c.tile_max, 3);
-#define PF_ARGS c.n, nblk, lc, c.conn_off, c.slice_off, c.conn16S, c.tile_off, c.tile_idx, \
-                base, dq, qm, qs, 0.0, nullptr
-    if (c.qd_chunks == 8)
-      hipLaunchKernelGGL((k_q_derivatives_pf<8, false>), dim3(c.qd_grid), b, Lp, s, PF_ARGS);
-    else if (c.qd_chunks == 12)
-      hipLaunchKernelGGL((k_q_derivatives_pf<12, false>), dim3(c.qd_grid), b, Lp, s, PF_ARGS);
-    else
-      hipLaunchKernelGGL((k_q_derivatives_pf<16, false>), dim3(c.qd_grid), b, Lp, s, PF_ARGS);
-#undef PF_ARGS
+    const size_t Lp = pf_lds_bytes(c.tile_block, c.tile_max, c.slice_width, 6);
+    const int lc = 6 * c.tile_max;
+#define PF_L(e, nc)                                                                        \
+  if (c.qd_chunks == 10 * e + nc)                                                          \
+    hipLaunchKernelGGL((k_q_derivatives_pf<e, nc, false>), dim3(c.qd_grid), b, Lp, s, c.n, \
+                       nblk, lc, c.conn_off, c.slice_off, c.conn16S, c.tile_off, c.tile_idx, \
+                       base, dq, qm, qs, 0.0, nullptr);
+    PF_LAYOUTS(PF_L)
+#undef PF_L
   } else if (t && power == 0.0)
     hipLaunchKernelGGL((k_q_derivatives<true, false>), g, b, L, s, QD_ARGS);
   else if (t)
@@ -1452,16 +1440,16 @@ void launch_q_derivatives(const DevCloud &c, const double *base, double *dq, dou
 void launch_q_derivatives_delta(const DevCloud &c, const double *base, double *dq, double *qm,
                                 double *qs, double cfl, double *delta, hipStream_t s) {
   const int nblk = blocks_for(c.n, c.tile_block);
-  const size_t Lp = qd_pf_lds_bytes(c.tile_block, c.tile_max, c.slice_width, 4);
-  const int lc = (int)qd_pf_lds_conn(c.tile_max, 4);
+  const size_t Lp = pf_lds_bytes(c.tile_block, c.tile_max, c.slice_width, 8);
+  const int lc = 8 * c.tile_max;
   const dim3 g(c.qdd_grid), b(c.tile_block);
-#define PF_ARGS c.n, nblk, lc, c.conn_off, c.slice_off, c.conn16S, c.tile_off, c.tile_idx, \
-                base, dq, qm, qs, cfl, delta
-  if (c.qdd_chunks == 12)
-    hipLaunchKernelGGL((k_q_derivatives_pf<12, true>), g, b, Lp, s, PF_ARGS);
-  else
-    hipLaunchKernelGGL((k_q_derivatives_pf<16, true>), g, b, Lp, s, PF_ARGS);
-#undef PF_ARGS
+#define PF_L(e, nc)                                                                        \
+  if (c.qdd_chunks == 10 * e + nc)                                                         \
+    hipLaunchKernelGGL((k_q_derivatives_pf<e, nc, true>), g, b, Lp, s, c.n, nblk, lc,      \
+                       c.conn_off, c.slice_off, c.conn16S, c.tile_off, c.tile_idx, base, dq, \
+                       qm, qs, cfl, delta);
+  PF_LAYOUTS(PF_L)
+#undef PF_L
 }
 
 void launch_q_derivatives_arg(const DevCloud &c, const double *base, double *dq, double *qm,
@@ -1483,17 +1471,15 @@ void launch_q_inner(const DevCloud &c, const double *qs, const double *dq_src, d
 #define QI_ARGS c.n, TILE_ARGS, c.xy, qs, dq_src, dq_dst, power
   if (t && power == 0.0 && c.qi_chunks) {
     const int nblk = blocks_for(c.n, c.tile_block);
-    const size_t Lp = qi_pf_lds_bytes(c.tile_block, c.tile_max, c.slice_width);
-    const int lc = (int)qi_pf_lds_conn(c.tile_max);
-#define PF_ARGS c.n, nblk, lc, c.conn_off, c.slice_off, c.conn16S, c.tile_off, c.tile_idx, \
-                c.xy, qs, dq_src, dq_dst
-    if (c.qi_chunks == 12)
-      hipLaunchKernelGGL((k_q_inner_pf<12>), dim3(c.qi_grid), b, Lp, s, PF_ARGS);
-    else if (c.qi_chunks == 16)
-      hipLaunchKernelGGL((k_q_inner_pf<16>), dim3(c.qi_grid), b, Lp, s, PF_ARGS);
-    else
-      hipLaunchKernelGGL((k_q_inner_pf<20>), dim3(c.qi_grid), b, Lp, s, PF_ARGS);
-#undef PF_ARGS
+    const size_t Lp = pf_lds_bytes(c.tile_block, c.tile_max, c.slice_width, 10);
+    const int lc = 10 * c.tile_max;
+#define PF_L(e, nc)                                                                          \
+  if (c.qi_chunks == 10 * e + nc)                                                            \
+    hipLaunchKernelGGL((k_q_inner_pf<e, nc>), dim3(c.qi_grid), b, Lp, s, c.n, nblk, lc,      \
+                       c.conn_off, c.slice_off, c.conn16S, c.tile_off, c.tile_idx, c.xy, qs, \
+                       dq_src, dq_dst);
+    PF_LAYOUTS(PF_L)
+#undef PF_L
   } else if (t && power == 0.0)
     hipLaunchKernelGGL((k_q_inner<true, false>), g, b, L, s, QI_ARGS);
   else if (t)

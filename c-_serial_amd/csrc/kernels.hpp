@@ -50,9 +50,9 @@ struct DevCloud {
   int tile_block;            // points per block of the stencil kernels
   int flux_cap;              // entries of a tile the flux stages (<= tile_max)
   int slice_width;           // longest list of any point (entries of a slice row)
-  int qi_chunks;             // pipelined inner pass: chunks per thread (0: not used)
+  int qi_chunks;             // pipelined inner pass: its layout code (0: not used)
   int qi_grid;               // its workgroups
-  int qd_chunks;             // pipelined q_derivatives: chunks per thread (0: not used)
+  int qd_chunks;             // pipelined q_derivatives: its layout code (0: not used)
   int qd_grid;
   int qdd_chunks;            // the same with func_delta (0: not used)
   int qdd_grid;
@@ -93,10 +93,10 @@ int flux_tile_cap(int block, int tile_max, size_t lds_limit, size_t lds_per_cu);
 // subset whose tiles fit in lds_limit bytes
 unsigned prepare_tiled(unsigned tiles, int tile_max, int flux_cap, size_t lds_limit);
 // The pipelined inner pass (tiled, power 0) for a layout whose tiles hold at
-// most tile_max entries and whose lists at most slice_width: the chunks
-// each thread gathers per block (0 when the gather does not fit the
-// registers or the tile and list slice do not fit lds_limit), and, through
-// *grid, the workgroups that fill n_cu CUs (a multiple of 8, at most nblk).
+// most tile_max entries and whose lists at most slice_width: the layout
+// code of its per-thread gather slots (0 when no built layout covers the
+// tile and lists, or they do not fit lds_limit), and, through *grid, the
+// workgroups that fill n_cu CUs (a multiple of 8, at most nblk).
 int prepare_qinner_pf(int tile_block, int tile_max, int slice_width, int nblk, int n_cu,
                       size_t lds_limit, int *grid);
 // The same for the pipelined q_derivatives (tiled, power 0), with_delta:
