@@ -181,12 +181,15 @@ def valu_bound(name, avg_ms):
 def run_batch(args):
     """The many-case workload: 64 flow conditions over 8 NACA 4-digit clouds
     of 50k-800k points (k = 20, inner passes 0, CFL 0.2, K 5), all cases
-    advanced together, each iteration one captured graph replay per 10
-    iterations, with the layout chosen by lskum_autotune on one case per
+    advanced together, one captured graph replay per args.graph_iters
+    iterations (--steps 1000 --graph-iters 1000: all iterations in one
+    launch), with the layout chosen by lskum_autotune on one case per
     cloud.  value = sum of case points x iterations / s.  Two cases are
     rerun alone and must match their batch results bit for bit; cases whose
     residual turned non-finite are listed (they run on as NaN without
-    touching the others)."""
+    touching the others).  With args.graph_sweep, also the time per
+    iteration with host launches (0) and with graphs of 10, 100 and 1,000
+    iterations, after the run."""
     from lskum.batch import Batch, autotune, batch_workload
     num = dict(cfl=0.2, vl_const=5.0, power=0.0, inner_iterations=0)
     t = time.perf_counter()
@@ -197,7 +200,7 @@ def run_batch(args):
     best, trials = autotune(clouds, [cases[j] for j in reps], iters=5, **num)
     log(f"autotune: best {best}")
     b = Batch(clouds, cases, tile_block=best["tile_block"], lds_tiles=best["lds_tiles"],
-              graph_iters=10, **num)
+              graph_iters=args.graph_iters, **num)
     if args.warmup:
         b.q_lskum(args.warmup)
     t0 = time.perf_counter()
@@ -207,6 +210,17 @@ def run_batch(args):
     prims = b.prim()
     failed = {int(j): int(it) for j, it in enumerate(b.failed_at()) if it}
     pts = int(b.points.sum())
+    sweep = {}
+    if args.graph_sweep:
+        for K, iters in ((0, 100), (10, 200), (100, 200), (1000, 1000)):
+            b.set_graph(K)
+            b.q_lskum(K if K else 10)  # capture and first replay, untimed
+            t = time.perf_counter()
+            b.q_lskum(iters)
+            sweep[str(K)] = {"iterations": iters,
+                             "ms_per_iter": 1e3 * (time.perf_counter() - t) / iters}
+            log(f"batch: graphs of {K} iterations: {sweep[str(K)]['ms_per_iter']:.3f} ms "
+                f"per iteration")
     b.close()
     ok = np.isfinite(forces).all(axis=1)
     same = {}
@@ -226,10 +240,12 @@ def run_batch(args):
         "ms_per_step": 1e3 * elapsed / args.steps, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "batch_64cases_8clouds_50k-800k_k20", "cases": len(cases),
-                   "clouds": [c.n for c in clouds], "case_points": pts, "graph_iters": 10,
+                   "clouds": [c.n for c in clouds], "case_points": pts,
+                   "graph_iters": args.graph_iters,
                    "tile_block": best["tile_block"], "lds_tile_mask": best["lds_tiles"]},
         "autotune": {"best": best, "trials": trials},
         "cases_bit_identical_to_single_runs": same,
+        "graph_length_sweep": sweep,
         "cases_failed": failed,
         "cl_cd": {"min_cl": float(forces[ok, 0].min()), "max_cl": float(forces[ok, 0].max()),
                   "min_cd": float(forces[ok, 1].min()), "max_cd": float(forces[ok, 1].max())},
@@ -428,6 +444,10 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="leave the shape-gradient (adjoint) measurement out of the line")
+    ap.add_argument("--graph-iters", type=int, default=10,
+                    help="many-case workload: iterations per captured graph")
+    ap.add_argument("--graph-sweep", action="store_true",
+                    help="many-case workload: also time graphs of 0, 10, 100, 1000 iterations")
     ap.add_argument("--converge", type=int, default=30000,
                     help="shape workload: iterations before the adjoint")
     ap.add_argument("--adjoint-iters", type=int, default=300,

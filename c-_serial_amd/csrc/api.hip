@@ -662,20 +662,25 @@ int create_impl(const lskum_cloud *clouds, int32_t n_clouds, const lskum_case *c
     else
       c->tiles = prepare_tiled(want, c->tile_max, c->flux_cap, (size_t)lds_limit);
   }
-  // the pipelined inner pass, where the tiles are in use for it
-  // and q_derivatives likewise
+  // the pipelined inner pass and q_derivatives (with func_delta in stage
+  // 1's), where the tiles are in use for them
   int n_cu = 0;
-  if (!c->mixed && p.power == 0.0 && dev_ok &&
+  if (p.power == 0.0 && dev_ok &&
       hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess) {
-    if (c->tiles & LSKUM_TILE_QINNER)
-      c->qi_chunks = prepare_qinner_pf(TB, c->tile_max, c->slice_width, nblk, n_cu,
-                                       (size_t)lds_limit, &c->qi_grid);
-    if (c->tiles & LSKUM_TILE_QDERIV) {
-      c->qd_chunks = prepare_qderiv_pf(TB, c->tile_max, c->slice_width, nblk, n_cu,
-                                       (size_t)lds_limit, false, &c->qd_grid);
-      if (c->qd_chunks)
-        c->qdd_chunks = prepare_qderiv_pf(TB, c->tile_max, c->slice_width, nblk, n_cu,
-                                          (size_t)lds_limit, true, &c->qdd_grid);
+    const size_t lim = (size_t)lds_limit;
+    const int tm = c->tile_max, sw = c->slice_width;
+    // the mixed form's tiles (FP32 records) let more blocks share a CU
+    // than the pipelined kernels' registers and list slices do: pipelined
+    // mixed kernels, and func_delta folded into its q_derivatives, were
+    // slower, so it keeps its own kernels
+    if (!c->mixed) {
+      if (c->tiles & LSKUM_TILE_QINNER)
+        c->qi_chunks = prepare_qinner_pf(TB, tm, sw, nblk, n_cu, lim, &c->qi_grid);
+      if (c->tiles & LSKUM_TILE_QDERIV) {
+        c->qd_chunks = prepare_qderiv_pf(TB, tm, sw, nblk, n_cu, lim, false, &c->qd_grid);
+        if (c->qd_chunks)
+          c->qdd_chunks = prepare_qderiv_pf(TB, tm, sw, nblk, n_cu, lim, true, &c->qdd_grid);
+      }
     }
   }
   // per-case constants and reduction ranges

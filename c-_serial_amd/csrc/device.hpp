@@ -186,4 +186,13 @@ struct ConnAhead {
 constexpr int kAheadQDeriv = 3;
 constexpr int kAheadQInner = 4;
 
+// One neighbour's step, state_update_mod.h:478-488 (speed: the neighbour's
+// |u| + 3 sqrt(p / rho)), its one fused product written out so that every
+// kernel inlining it agrees to the last bit.
+__device__ __forceinline__ double delta_step(double2 xy, double2 xyk, double speed, double cfl) {
+  const double dx = xyk.x - xy.x, dy = xyk.y - xy.y;
+  const double dist = sqrt(fma(dx, dx, dy * dy));
+  return cfl * (dist / speed);
+}
+
 }  // namespace lskum

@@ -13,6 +13,7 @@
 
 #include "device.hpp"
 #include "kernels.hpp"
+#include "pipeline.hpp"
 #include "physics.hpp"
 
 namespace lskum {
@@ -48,15 +49,6 @@ __global__ __launch_bounds__(256) void k_speed(int n, const double *__restrict__
 }
 
 // ------------------------------------------------------------ local dt
-// One neighbour's step, state_update_mod.h:478-488 (speed: the neighbour's
-// |u| + 3 sqrt(p / rho)), its one fused product written out so that every
-// kernel inlining it agrees to the last bit.
-__device__ __forceinline__ double delta_step(double2 xy, double2 xyk, double speed, double cfl) {
-  const double dx = xyk.x - xy.x, dy = xyk.y - xy.y;
-  const double dist = sqrt(fma(dx, dx, dy * dy));
-  return cfl * (dist / speed);
-}
-
 // state_update_mod.h:448-496: min over the neighbours' states only.
 template <bool TILED>
 __global__ __launch_bounds__(256) void k_func_delta(int n, const int32_t *__restrict__ off,
@@ -419,57 +411,6 @@ __global__ __launch_bounds__(256) void k_q_inner(
 // compile time, so addresses take a few integer operations.  The per-point
 // arithmetic is k_q_inner's (q_inner_term, q_inner_finish), so results are
 // bit-identical.
-struct QiBlock {
-  int t0, tn, nconn;  // first tile entry, tile entries, slice chunks
-  int s0;             // first slice entry
-};
-
-__device__ __forceinline__ QiBlock qi_block(const int32_t *__restrict__ tile_off,
-                                            const int32_t *__restrict__ slice_off, int b) {
-  QiBlock q;
-  q.t0 = tile_off[b];
-  q.tn = tile_off[b + 1] - q.t0;
-  q.s0 = slice_off[b];
-  q.nconn = (slice_off[b + 1] - q.s0) >> 3;
-  return q;
-}
-
-// A 16-byte chunk in registers: a native vector type, which the compiler
-// keeps in registers where an array of HIP's double2 struct, copied from
-// global memory, went to scratch.
-typedef double qi_chunk __attribute__((ext_vector_type(2)));
-
-// the tile indices of this thread's entry slots (every block has tn >= 1:
-// its own points)
-template <int E>
-__device__ __forceinline__ void pf_load_idx(int (&idx)[E], const QiBlock &q, int t, int TB,
-                                            const int32_t *__restrict__ tile_idx) {
-#pragma unroll
-  for (int u = 0; u < E; u++) idx[u] = tile_idx[q.t0 + min(t + u * TB, q.tn - 1)];
-}
-
-// the list-slice chunks into v[at .. at + NC); a block whose points have no
-// neighbours (nconn = 0) reads its first tile entry's coordinates instead
-// and, in pf_store_conn, writes nothing
-template <int NC, int NV>
-__device__ __forceinline__ void pf_load_conn(qi_chunk (&v)[NV], int at, const QiBlock &q, int t,
-                                             int TB, const uint16_t *__restrict__ conn16,
-                                             const qi_chunk *fallback) {
-  const qi_chunk *cn2 = q.nconn > 0 ? reinterpret_cast<const qi_chunk *>(conn16 + q.s0)
-                                    : fallback;
-  const int last = max(q.nconn - 1, 0);
-#pragma unroll
-  for (int u = 0; u < NC; u++) v[at + u] = cn2[min(t + u * TB, last)];
-}
-
-template <int NC, int NV>
-__device__ __forceinline__ void pf_store_conn(qi_chunk *l2, const qi_chunk (&v)[NV], int at,
-                                              const QiBlock &q, int t, int TB, int lds_conn2) {
-  if (q.nconn <= 0) return;  // uniform per block
-#pragma unroll
-  for (int u = 0; u < NC; u++) l2[lds_conn2 + min(t + u * TB, q.nconn - 1)] = v[at + u];
-}
-
 // q_inner: entry slot u holds the coordinates (from xy) and the four
 // derivative chunks (from src) of its tile entry, stored at LDS chunk 5 e
 template <int E, int NC>
@@ -514,6 +455,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
   const uint16_t *cl = reinterpret_cast<const uint16_t *>(lds + lds_conn);
   qi_chunk *l2 = reinterpret_cast<qi_chunk *>(lds);
   const int t = threadIdx.x, TB = blockDim.x;
+  const int lg_tb = __ffs(TB) - 1;  // tile_block is 64, 128 or 256: row e of the slice at e << lg_tb
   const int G = gridDim.x;
   int idx[E];
   qi_chunk v[5 * E + NC];
@@ -552,7 +494,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
       double sum_delx_sqr = 0., sum_dely_sqr = 0., sum_delx_dely = 0.;
       double sx[4] = {0.0, 0.0, 0.0, 0.0}, sy[4] = {0.0, 0.0, 0.0, 0.0};
       auto fetch = [&](int e, double2 &xyk, double dxk[4], double dyk[4]) {
-        const double *hk = lds + S * (int)(cl[e * TB + t] & kLocalMask);
+        const double *hk = lds + S * (int)(cl[(e << lg_tb) + t] & kLocalMask);
         xyk = reinterpret_cast<const double2 *>(hk)[0];
         load4(hk + 2, dxk);
         load4(hk + 6, dyk);
@@ -629,6 +571,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
   const uint16_t *cl = reinterpret_cast<const uint16_t *>(lds + lds_conn);
   qi_chunk *l2 = reinterpret_cast<qi_chunk *>(lds);
   const int t = threadIdx.x, TB = blockDim.x;
+  const int lg_tb = __ffs(TB) - 1;  // tile_block is 64, 128 or 256: row e of the slice at e << lg_tb
   const int G = gridDim.x;
   int idx[E];
   qi_chunk v[C * E + NC];
@@ -662,7 +605,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
       double min_delt = 1.0;
       for (int r = 0; r < 4; r++) qmax[r] = qmin[r] = qi[r];
       auto fetch = [&](int e, double2 &xyk, double qk[4], double &speed) {
-        const double *hk = lds + S * (int)(cl[e * TB + t] & kLocalMask);
+        const double *hk = lds + S * (int)(cl[(e << lg_tb) + t] & kLocalMask);
         xyk = reinterpret_cast<const double2 *>(hk)[0];
         load4(hk + 2, qk);
         speed = DELTA ? hk[7] : 1.0;
@@ -1320,45 +1263,10 @@ unsigned prepare_tiled(unsigned tiles, int tile_max, int flux_cap, size_t lds_li
 
 // The pipelined inner pass's LDS: the tile, then the block's list slice
 // (double offset lds_conn).
-// The pipelined kernels' layouts: E entry slots and NC list-slice chunks per
-// thread, E tile_block >= tile_max and 8 NC >= slice_width (a block's slice
-// is slice_width entries of 2 bytes per point).  A layout code is 10 E + NC.
-#define PF_LAYOUTS(X) X(2, 3) X(2, 4) X(3, 3) X(3, 4) X(4, 3) X(4, 4)
-
-static int pf_layout(int tile_block, int tile_max, int slice_width) {
-  const int E = (tile_max + tile_block - 1) / tile_block, NC = (slice_width + 7) / 8;
-  const int code = 10 * E + std::max(NC, 3);
-#define PF_HAS(e, nc) if (code == 10 * e + nc) return code;
-  PF_LAYOUTS(PF_HAS)
-#undef PF_HAS
-  return 0;
-}
-
-// lds_conn: the double offset of the list slice, after S doubles per tile entry
-static size_t pf_lds_bytes(int tile_block, int tile_max, int slice_width, int S) {
-  return sizeof(double) * (size_t)S * tile_max + sizeof(uint16_t) * (size_t)slice_width * tile_block;
-}
-
-// the workgroups that fill n_cu CUs with kernel f (a multiple of 8, at most
-// nblk), or 0 where it cannot run with these LDS bytes
-static int pf_grid(const void *f, int tile_block, size_t bytes, size_t lds_limit, int nblk,
-                   int n_cu) {
-  if (bytes > lds_limit ||
-      hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) != hipSuccess)
-    return 0;
-  int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, tile_block, bytes) != hipSuccess ||
-      per_cu <= 0)
-    return 0;
-  int g = per_cu * n_cu;
-  g = g >= 8 ? g / 8 * 8 : g;
-  return std::min(g, nblk);
-}
-
 int prepare_qinner_pf(int tile_block, int tile_max, int slice_width, int nblk, int n_cu,
                       size_t lds_limit, int *grid) {
   *grid = 0;
-  if (tile_block % 8 || n_cu <= 0 || nblk <= 0) return 0;
+  if (tile_block < 64 || (tile_block & (tile_block - 1)) || n_cu <= 0 || nblk <= 0) return 0;
   const int code = pf_layout(tile_block, tile_max, slice_width);
   const size_t bytes = pf_lds_bytes(tile_block, tile_max, slice_width, 10);
   const void *f = nullptr;
@@ -1373,7 +1281,7 @@ int prepare_qinner_pf(int tile_block, int tile_max, int slice_width, int nblk, i
 int prepare_qderiv_pf(int tile_block, int tile_max, int slice_width, int nblk, int n_cu,
                       size_t lds_limit, bool with_delta, int *grid) {
   *grid = 0;
-  if (tile_block % 8 || n_cu <= 0 || nblk <= 0) return 0;
+  if (tile_block < 64 || (tile_block & (tile_block - 1)) || n_cu <= 0 || nblk <= 0) return 0;
   const int code = pf_layout(tile_block, tile_max, slice_width);
   const size_t bytes = pf_lds_bytes(tile_block, tile_max, slice_width, with_delta ? 8 : 6);
   const void *f = nullptr;

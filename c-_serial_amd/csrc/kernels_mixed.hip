@@ -125,6 +125,52 @@ __global__ __launch_bounds__(256) void k_q_variables_m(int n, const double *__re
 // q_variables_mod.h:36-118: FP32 gathers of q, FP64 geometry sums, solve
 // and accumulation; qmax/qmin over FP32 values (rounding is monotone, so
 // they are the rounded FP64 extrema).
+// One neighbour's terms of the mixed q_derivatives: qmax/qmin in FP32 and
+// the least-squares sums in FP64, every fused product written out (the
+// kernels that inline this then agree to the last bit).
+template <bool POW>
+__device__ __forceinline__ void qdm_term(double2 pi, const float qi[4], double2 pk,
+                                         const float qk[4], double power, float qmax[4],
+                                         float qmin[4], double &sum_delx_sqr,
+                                         double &sum_dely_sqr, double &sum_delx_dely,
+                                         double sdx[4], double sdy[4]) {
+  for (int r = 0; r < 4; r++) {
+    qmax[r] = fmaxf(qmax[r], qk[r]);  // the reference's strict comparisons for finite q
+    qmin[r] = fminf(qmin[r], qk[r]);
+  }
+  const double delx = pk.x - pi.x, dely = pk.y - pi.y;
+  const double weights = POW ? pow(sqrt(fma(delx, delx, dely * dely)), power) : 1.0;
+  const double wx = POW ? weights * delx : delx, wy = POW ? weights * dely : dely;
+  sum_delx_sqr = fma(wx, delx, sum_delx_sqr);
+  sum_dely_sqr = fma(wy, dely, sum_dely_sqr);
+  sum_delx_dely = fma(wx, dely, sum_delx_dely);
+  for (int r = 0; r < 4; r++) {
+    const double d = (double)(qk[r] - qi[r]);
+    sdx[r] = fma(wx, d, sdx[r]);
+    sdy[r] = fma(wy, d, sdy[r]);
+  }
+}
+
+// The 2x2 solve and the stores of point i's dq and qmax/qmin.
+__device__ __forceinline__ void qdm_finish(double sum_delx_sqr, double sum_dely_sqr,
+                                           double sum_delx_dely, const double sdx[4],
+                                           const double sdy[4], const float qmax[4],
+                                           const float qmin[4], float4 *dq_i, float4 *qm_i) {
+  const double det = fma(sum_delx_sqr, sum_dely_sqr, -(sum_delx_dely * sum_delx_dely));
+  // det > 0 for every validated point; 0 only for a batch's padding points,
+  // which have no neighbours and must not produce NaNs
+  const double one_by_det = det != 0.0 ? 1.0 / det : 0.0;
+  float dqx[4], dqy[4];
+  for (int r = 0; r < 4; r++) {
+    dqx[r] = (float)(fma(sdx[r], sum_dely_sqr, -(sdy[r] * sum_delx_dely)) * one_by_det);
+    dqy[r] = (float)(fma(sdy[r], sum_delx_sqr, -(sdx[r] * sum_delx_dely)) * one_by_det);
+  }
+  dq_i[0] = make_float4(dqx[0], dqx[1], dqx[2], dqx[3]);
+  dq_i[1] = make_float4(dqy[0], dqy[1], dqy[2], dqy[3]);
+  qm_i[0] = make_float4(qmax[0], qmax[1], qmax[2], qmax[3]);
+  qm_i[1] = make_float4(qmin[0], qmin[1], qmin[2], qmin[3]);
+}
+
 template <bool TILED, bool POW>
 __global__ __launch_bounds__(256) void k_q_derivatives_m(
     int n, const int32_t *__restrict__ off, const int32_t *__restrict__ slice_off,
@@ -174,42 +220,57 @@ __global__ __launch_bounds__(256) void k_q_derivatives_m(
     w_n = L.raw(min(e + 2, last), TILED);
     float qk[4];
     unpack4(qk4, qk);
-    for (int r = 0; r < 4; r++) {
-      qmax[r] = fmaxf(qmax[r], qk[r]);  // the reference's strict comparisons for finite q
-      qmin[r] = fminf(qmin[r], qk[r]);
-    }
-    const double delx = pk.x - pi.x, dely = pk.y - pi.y;
-    const double weights = POW ? pow(sqrt(delx * delx + dely * dely), power) : 1.0;
-    const double wx = POW ? weights * delx : delx, wy = POW ? weights * dely : dely;
-    sum_delx_sqr = sum_delx_sqr + wx * delx;
-    sum_dely_sqr = sum_dely_sqr + wy * dely;
-    sum_delx_dely = sum_delx_dely + wx * dely;
-    for (int r = 0; r < 4; r++) {
-      const double d = (double)(qk[r] - qi[r]);
-      sdx[r] = sdx[r] + wx * d;
-      sdy[r] = sdy[r] + wy * d;
-    }
+    qdm_term<POW>(pi, qi, pk, qk, power, qmax, qmin, sum_delx_sqr, sum_dely_sqr,
+                  sum_delx_dely, sdx, sdy);
     pk = pk_n;
     qk4 = qk4_n;
   }
-  const double det = sum_delx_sqr * sum_dely_sqr - sum_delx_dely * sum_delx_dely;
-  // det > 0 for every validated point; 0 only for a batch's padding points,
-  // which have no neighbours and must not produce NaNs
-  const double one_by_det = det != 0.0 ? 1.0 / det : 0.0;
-  float dqx[4], dqy[4];
-  for (int r = 0; r < 4; r++) {
-    dqx[r] = (float)((sdx[r] * sum_dely_sqr - sdy[r] * sum_delx_dely) * one_by_det);
-    dqy[r] = (float)((sdy[r] * sum_delx_sqr - sdx[r] * sum_delx_dely) * one_by_det);
-  }
-  dq_out[2 * (size_t)i] = make_float4(dqx[0], dqx[1], dqx[2], dqx[3]);
-  dq_out[2 * (size_t)i + 1] = make_float4(dqy[0], dqy[1], dqy[2], dqy[3]);
-  qm_out[2 * (size_t)i] = make_float4(qmax[0], qmax[1], qmax[2], qmax[3]);
-  qm_out[2 * (size_t)i + 1] = make_float4(qmin[0], qmin[1], qmin[2], qmin[3]);
+  qdm_finish(sum_delx_sqr, sum_dely_sqr, sum_delx_dely, sdx, sdy, qmax, qmin,
+             dq_out + 2 * (size_t)i, qm_out + 2 * (size_t)i);
 }
 
 // ------------------------------------------------------------ inner passes
 // q_variables_mod.h:120-228, one Jacobi pass (src -> dst), FP32 gathers and
 // corrected differences, FP64 sums and solve.
+// One neighbour's terms, every fused product written out (the kernels that
+// inline this then agree to the last bit).
+template <bool POW>
+__device__ __forceinline__ void qim_term(double2 pi, const float qi[4], const float dxi[4],
+                                         const float dyi[4], double2 pk, const float qk[4],
+                                         const float dxk[4], const float dyk[4], double power,
+                                         double &sum_delx_sqr, double &sum_dely_sqr,
+                                         double &sum_delx_dely, double sx[4], double sy[4]) {
+  const double delx = pk.x - pi.x, dely = pk.y - pi.y;
+  const float fx = (float)delx, fy = (float)dely;
+  const double weights = POW ? pow(sqrt(fma(delx, delx, dely * dely)), power) : 1.0;
+  const double wx = POW ? weights * delx : delx, wy = POW ? weights * dely : dely;
+  sum_delx_sqr = fma(wx, delx, sum_delx_sqr);
+  sum_dely_sqr = fma(wy, dely, sum_dely_sqr);
+  sum_delx_dely = fma(wx, dely, sum_delx_dely);
+  for (int r = 0; r < 4; r++) {
+    const float t1 = fmaf(fx, dxk[r] - dxi[r], fy * (dyk[r] - dyi[r]));
+    const float d = fmaf(-0.5f, t1, qk[r] - qi[r]);
+    sx[r] = fma(wx, (double)d, sx[r]);
+    sy[r] = fma(wy, (double)d, sy[r]);
+  }
+}
+
+__device__ __forceinline__ void qim_finish(double sum_delx_sqr, double sum_dely_sqr,
+                                           double sum_delx_dely, const double sx[4],
+                                           const double sy[4], float4 *d_i) {
+  const double det = fma(sum_delx_sqr, sum_dely_sqr, -(sum_delx_dely * sum_delx_dely));
+  // det > 0 for every validated point; 0 only for a batch's padding points,
+  // which have no neighbours and must not produce NaNs
+  const double one_by_det = det != 0.0 ? 1.0 / det : 0.0;
+  float ox[4], oy[4];
+  for (int r = 0; r < 4; r++) {
+    ox[r] = (float)(one_by_det * fma(sx[r], sum_dely_sqr, -(sy[r] * sum_delx_dely)));
+    oy[r] = (float)(one_by_det * fma(sy[r], sum_delx_sqr, -(sx[r] * sum_delx_dely)));
+  }
+  d_i[0] = make_float4(ox[0], ox[1], ox[2], ox[3]);
+  d_i[1] = make_float4(oy[0], oy[1], oy[2], oy[3]);
+}
+
 template <bool TILED, bool POW>
 __global__ __launch_bounds__(256) void k_q_inner_m(
     int n, const int32_t *__restrict__ off, const int32_t *__restrict__ slice_off,
@@ -266,35 +327,14 @@ __global__ __launch_bounds__(256) void k_q_inner_m(
     unpack4(qk4, qk);
     unpack4(dxk4, dxk);
     unpack4(dyk4, dyk);
-    const double delx = pk.x - pi.x, dely = pk.y - pi.y;
-    const float fx = (float)delx, fy = (float)dely;
-    const double weights = POW ? pow(sqrt(delx * delx + dely * dely), power) : 1.0;
-    const double wx = POW ? weights * delx : delx, wy = POW ? weights * dely : dely;
-    sum_delx_sqr = sum_delx_sqr + wx * delx;
-    sum_dely_sqr = sum_dely_sqr + wy * dely;
-    sum_delx_dely = sum_delx_dely + wx * dely;
-    for (int r = 0; r < 4; r++) {
-      const float d =
-          (qk[r] - qi[r]) - 0.5f * (fx * (dxk[r] - dxi[r]) + fy * (dyk[r] - dyi[r]));
-      sx[r] = sx[r] + wx * (double)d;
-      sy[r] = sy[r] + wy * (double)d;
-    }
+    qim_term<POW>(pi, qi, dxi, dyi, pk, qk, dxk, dyk, power, sum_delx_sqr, sum_dely_sqr,
+                  sum_delx_dely, sx, sy);
     pk = pk_n;
     qk4 = qk4_n;
     dxk4 = dxk4_n;
     dyk4 = dyk4_n;
   }
-  const double det = sum_delx_sqr * sum_dely_sqr - sum_delx_dely * sum_delx_dely;
-  // det > 0 for every validated point; 0 only for a batch's padding points,
-  // which have no neighbours and must not produce NaNs
-  const double one_by_det = det != 0.0 ? 1.0 / det : 0.0;
-  float ox[4], oy[4];
-  for (int r = 0; r < 4; r++) {
-    ox[r] = (float)(one_by_det * (sx[r] * sum_dely_sqr - sy[r] * sum_delx_dely));
-    oy[r] = (float)(one_by_det * (sy[r] * sum_delx_sqr - sx[r] * sum_delx_dely));
-  }
-  dst[2 * (size_t)i] = make_float4(ox[0], ox[1], ox[2], ox[3]);
-  dst[2 * (size_t)i + 1] = make_float4(oy[0], oy[1], oy[2], oy[3]);
+  qim_finish(sum_delx_sqr, sum_dely_sqr, sum_delx_dely, sx, sy, dst + 2 * (size_t)i);
 }
 
 // ------------------------------------------------------------ flux residual
@@ -507,6 +547,7 @@ __global__ void k_kat_fluxes_m(int n, const double *__restrict__ in, double *__r
 }  // namespace
 
 size_t lds_bytes_qderiv_m(int tile_max) { return 32 * (size_t)tile_max; }
+
 size_t lds_bytes_qinner_m(int tile_max) { return 64 * (size_t)tile_max; }
 size_t lds_bytes_flux_m(int tile_max) { return 84 * (size_t)tile_max; }
 
