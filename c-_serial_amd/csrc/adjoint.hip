@@ -312,9 +312,11 @@ __device__ __forceinline__ void load8(const double *p, double a[4], double b[4])
 // Per point i, over its neighbours: the adjoint of q~ at both ends of each
 // pair, pushed through the reconstruction.  i's own share accumulates in
 // registers (own_q, own_dq, own_qm); the neighbour's goes to the edge's
-// record: c_q[4], c_d[4] (per unit of d, times dx/dy at the gather),
-// c_max[4], c_min[4], stored at the edge's position in the transpose lists
-// so that the gather reads each point's records contiguously.
+// record: c_q[4], c_d[4] (per unit of d, times dx/dy at the gather) and
+// c_m[4], the adjoint of the extremum each limiter read, with a byte whose
+// bit r says whether that was qmax or qmin, both stored at the edge's
+// position in the transpose lists so that the gather reads each point's
+// records contiguously (96 + 1 bytes a pair).
 //
 // GEO adds the adjoint of the geometry the flux reads, accumulated (+=)
 // into per-edge and per-point arrays owned by i alone:
@@ -339,9 +341,9 @@ __global__ __launch_bounds__(256) void k_flux_vjp(
     const double *__restrict__ nxa, const double *__restrict__ nya,
     const uint8_t *__restrict__ flag, const double *__restrict__ delta,
     const double *__restrict__ lam, double *__restrict__ own_q, double *__restrict__ own_dq,
-    double *__restrict__ own_qm, double *__restrict__ edge, const int32_t *__restrict__ tr_pos,
-    double *__restrict__ geo_edge, double *__restrict__ geo_n, double *__restrict__ geo_cf,
-    double *__restrict__ geo_delta) {
+    double *__restrict__ own_qm, double *__restrict__ edge, uint8_t *__restrict__ edge_mx,
+    const int32_t *__restrict__ tr_pos, double *__restrict__ geo_edge,
+    double *__restrict__ geo_n, double *__restrict__ geo_cf, double *__restrict__ geo_delta) {
   extern __shared__ double lds[];
   constexpr int S = 24;  // base (8), dq (8), qm (8)
   const int b = xcd_block();
@@ -435,7 +437,9 @@ __global__ __launch_bounds__(256) void k_flux_vjp(
       geo_edge[2 * (size_t)e] += gx;
       geo_edge[2 * (size_t)e + 1] += gy;
     }
-    double *rec = edge + 16 * (size_t)tr_pos[e];
+    const size_t tp = (size_t)tr_pos[e];
+    double *rec = edge + EDGE_REC * tp;
+    uint32_t mx = 0;
     for (int r = 0; r < 4; r++) {
       aq[r] += a_i[r] * Ri[r].dq;
       const double ad = a_i[r] * Ri[r].dd;
@@ -447,9 +451,10 @@ __global__ __launch_bounds__(256) void k_flux_vjp(
         amn[r] += a_i[r] * Ri[r].dm;
       rec[r] = a_k[r] * Rk[r].dq;
       rec[4 + r] = a_k[r] * Rk[r].dd;
-      rec[8 + r] = Rk[r].use_max ? a_k[r] * Rk[r].dm : 0.0;
-      rec[12 + r] = Rk[r].use_max ? 0.0 : a_k[r] * Rk[r].dm;
+      rec[8 + r] = a_k[r] * Rk[r].dm;
+      mx |= (Rk[r].use_max ? 1u : 0u) << r;
     }
+    edge_mx[tp] = (uint8_t)mx;
   }
   store4(own_q + 4 * (size_t)i, aq);
   store4(own_dq + 8 * (size_t)i, adx);
@@ -468,7 +473,8 @@ __global__ __launch_bounds__(256) void k_flux_vjp(
 // it is the neighbour (transpose CSR, owners ascending).
 __global__ __launch_bounds__(256) void k_flux_vjp_gather(
     int n, const int32_t *__restrict__ tr_off, const int32_t *__restrict__ tr_owner,
-    const double2 *__restrict__ xy, const double *__restrict__ edge, const double *__restrict__ own_q,
+    const double2 *__restrict__ xy, const double *__restrict__ edge,
+    const uint8_t *__restrict__ edge_mx, const double *__restrict__ own_q,
     const double *__restrict__ own_dq, const double *__restrict__ own_qm,
     double *__restrict__ lq, double *__restrict__ ldq, double *__restrict__ lqm) {
   const int j = xcd_block() * blockDim.x + threadIdx.x;
@@ -481,23 +487,26 @@ __global__ __launch_bounds__(256) void k_flux_vjp_gather(
   // record t + 1 with its owner's coordinates, and owner t + 2's index, load
   // while record t is added
   const int t0 = tr_off[j], tlast = tr_off[j + 1] - 1;
-  auto fetch = [&](int t, int i, double2 &p, double r16[16]) {
+  auto fetch = [&](int t, int i, double2 &p, double r12[EDGE_REC], uint32_t &mx) {
     p = xy[i];
-    const double *rec = edge + 16 * (size_t)t;
-    load8(rec, r16, r16 + 4);
-    load8(rec + 8, r16 + 8, r16 + 12);
+    const double *rec = edge + EDGE_REC * (size_t)t;
+    load8(rec, r12, r12 + 4);
+    load4(rec + 8, r12 + 8);
+    mx = edge_mx[t];
   };
   double2 po = {0.0, 0.0};
-  double rc[16] = {};
+  double rc[EDGE_REC] = {};
+  uint32_t mc = 0;
   int i_n = 0;
   if (t0 <= tlast) {
-    fetch(t0, tr_owner[t0], po, rc);
+    fetch(t0, tr_owner[t0], po, rc, mc);
     i_n = tr_owner[min(t0 + 1, tlast)];
   }
   for (int t = t0; t <= tlast; t++) {
     double2 po_n;
-    double rn[16];
-    fetch(min(t + 1, tlast), i_n, po_n, rn);
+    double rn[EDGE_REC];
+    uint32_t mn;
+    fetch(min(t + 1, tlast), i_n, po_n, rn, mn);
     i_n = tr_owner[min(t + 2, tlast)];
     const double delx = x_j - po.x;
     const double dely = y_j - po.y;
@@ -505,11 +514,14 @@ __global__ __launch_bounds__(256) void k_flux_vjp_gather(
       aq[r] += rc[r];
       adx[r] += rc[4 + r] * delx;
       ady[r] += rc[4 + r] * dely;
-      amx[r] += rc[8 + r];
-      amn[r] += rc[12 + r];
+      // the other extremum gains an exact 0, as it did from a record of both
+      const bool use_max = (mc >> r) & 1u;
+      amx[r] += use_max ? rc[8 + r] : 0.0;
+      amn[r] += use_max ? 0.0 : rc[8 + r];
     }
     po = po_n;
-    for (int r = 0; r < 16; r++) rc[r] = rn[r];
+    for (int r = 0; r < EDGE_REC; r++) rc[r] = rn[r];
+    mc = mn;
   }
   store4(lq + 4 * (size_t)j, aq);
   store4(ldq + 8 * (size_t)j, adx);
@@ -856,7 +868,7 @@ void launch_flux_vjp(const DevCloud &c, const AdjointBuffers &a, const double *b
                      const GeoAdjoint *geo, hipStream_t s) {
 #define VJP_ARGS(G)                                                                       \
   c.n, c.conn_off, c.conn, c.conn16, c.tile_off, c.tile_idx, base, dq, qm, c.coef, c.nx,  \
-      c.ny, c.flag, delta, lam, a.own_q, a.own_dq, a.own_qm, a.edge, a.tr_pos,            \
+      c.ny, c.flag, delta, lam, a.own_q, a.own_dq, a.own_qm, a.edge, a.edge_mx, a.tr_pos, \
       G ? geo->edge : nullptr, G ? geo->nrm : nullptr, G ? geo->coef : nullptr,           \
       G ? geo->delta : nullptr
   if (a.vjp_tiled) {
@@ -875,7 +887,7 @@ void launch_flux_vjp(const DevCloud &c, const AdjointBuffers &a, const double *b
   }
 #undef VJP_ARGS
   hipLaunchKernelGGL(k_flux_vjp_gather, dim3(blocks_of(c.n)), dim3(256), 0, s, c.n, a.tr_off,
-                     a.tr_owner, reinterpret_cast<const double2 *>(c.xy), a.edge, a.own_q,
+                     a.tr_owner, reinterpret_cast<const double2 *>(c.xy), a.edge, a.edge_mx, a.own_q,
                      a.own_dq, a.own_qm, a.lq, a.ldq[0], a.lqm);
 }
 

@@ -58,7 +58,7 @@ void release(lskum_ctx *c) {
                   c->slice_off, c->connS, c->conn16S};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
-  void *aptrs[] = {c->adj.own_q, c->adj.own_dq, c->adj.own_qm, c->adj.edge, c->adj.lq,
+  void *aptrs[] = {c->adj.own_q, c->adj.own_dq, c->adj.own_qm, c->adj.edge, c->adj.edge_mx, c->adj.lq,
                    c->adj.ldq[0], c->adj.ldq[1], c->adj.lqm, c->adj.ls, c->adj.arg,
                    c->adj.tr_off, c->adj.tr_edge, c->adj.tr_owner, c->adj.tr_pos, c->adj.tbase,
                    c->adj.tdq[0],
@@ -866,7 +866,8 @@ int prepare_adjoint(lskum_ctx *c) {
   AALLOC(a.own_q, (size_t)4 * n);
   AALLOC(a.own_dq, (size_t)8 * n);
   AALLOC(a.own_qm, (size_t)8 * n);
-  AALLOC(a.edge, 16 * E);
+  AALLOC(a.edge, EDGE_REC * E);
+  AALLOC(a.edge_mx, E);
   AALLOC(a.lq, (size_t)4 * n);
   AALLOC(a.ldq[0], (size_t)8 * n);
   AALLOC(a.ldq[1], (size_t)8 * n);

@@ -23,6 +23,9 @@ enum BaseSlot : int {
 };
 constexpr int DQ_STRIDE = 8;
 constexpr int QM_STRIDE = 8;
+// the flux reverse's record of one (point, neighbour) pair: the adjoints of
+// the neighbour's q[4], of its increment d[4] and of its extremum [4]
+constexpr int EDGE_REC = 12;
 // conn entries: neighbour index in the low 28 bits, sub-stencil mask above.
 constexpr uint32_t kIndexMask = 0x0FFFFFFFu;
 constexpr int kMaskShift = 28;
@@ -188,7 +191,10 @@ void launch_kat_fluxes_m(int n, const double *in6, double *out, hipStream_t s);
 // tangents (tbase a copy of the base records with tangent q); vec_in /
 // vec_out [4][n] staging.
 struct AdjointBuffers {
+  // edge: the flux reverse's per-pair records, EDGE_REC doubles each;
+  // edge_mx: per pair, bit r set where component r's extremum is qmax
   double *own_q, *own_dq, *own_qm, *edge, *lq, *ldq[2], *lqm, *ls;
+  uint8_t *edge_mx;
   int32_t *arg, *tr_off, *tr_edge, *tr_owner, *tr_pos;
   double *tbase, *tdq[2], *tqm, *vec_in, *vec_out;
   double *tqs;     // tangent of q_derivatives' q sums

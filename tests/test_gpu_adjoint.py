@@ -10,7 +10,9 @@ limiter is active).  Checks:
     h = 1e-7 on a relative perturbation of the developed region near the
     body (1e-5 of the largest entry), wherever the one-sided differences
     agree: the limiter's switches are kinks, and a state within h of one
-    has two one-sided slopes;
+    has two one-sided slopes; the share of residuals set aside is bounded by
+    its measured value, and at most of them one one-sided difference must
+    equal the tangent;
   - the same at free stream, where the residual is smooth (every limiter
     stays at phi = 1), with no exclusions (1e-6);
   - torch.autograd.gradcheck of the torch.autograd.Function wrapper on a
@@ -28,6 +30,15 @@ from lskum.solver import Solver
 from conftest import normwise
 
 pytestmark = pytest.mark.gpu
+
+# Residuals with a limiter switch within +-h along the test's direction, as
+# a share of all residuals: measured 0.033 (no inner passes) and 0.158 (two
+# inner passes, where a residual reads about 200 limiter evaluations) on
+# these seeded states and directions; the bounds add a small margin.
+KINK_FRACTION_MAX = {0: 0.04, 2: 0.18}
+# Share of those residuals at which one of the one-sided differences equals
+# the tangent: measured 0.653 and 0.442; the bounds take a small margin off.
+ONE_SIDED_MIN = {0: 0.60, 2: 0.40}
 
 
 def _developed(name, mach, aoa, inner, iters):
@@ -78,12 +89,17 @@ def test_tangent_matches_central_differences(name, inner):
     scale = np.abs(jv).max()
     kink = (np.abs(fwd - bwd) > 1e-5 * scale).any(axis=0)
     err = np.abs(fd - jv)[:, ~kink].max() / scale
-    print(f"case: FD {name}: {kink.sum()} of {cloud.n} residuals straddle a switch; "
-          f"elsewhere max error {err:.2e}", flush=True)
-    # with two inner passes a residual reads ~200 limiter evaluations, so
-    # many more straddle a switch; the bound only guards against a broken
-    # comparison
-    assert kink.mean() <= (0.05 if inner == 0 else 0.25)
+    # At a residual with one switch within +-h, one of the two one-sided
+    # differences stays on the state's own branch and equals the tangent;
+    # only residuals whose stencils hold switches on both sides (or two
+    # switches of one sign) match neither.
+    side = np.minimum(np.abs(fwd - jv), np.abs(bwd - jv))[:, kink].max(axis=0) / scale
+    one_side = float((side <= 1e-5).mean()) if kink.any() else 1.0
+    print(f"case: FD {name}: {kink.sum()} of {cloud.n} residuals straddle a switch "
+          f"({kink.mean():.4f}), {one_side:.3f} of them on one side; elsewhere max error "
+          f"{err:.2e}", flush=True)
+    assert kink.mean() <= KINK_FRACTION_MAX[inner]
+    assert one_side >= ONE_SIDED_MIN[inner]
     assert err <= 1e-5
 
 
