@@ -64,7 +64,7 @@ def run_form(cloud, precision, args, k_mean):
                "none": L.REORDER_NONE}[args.reorder]
     t = time.perf_counter()
     solver = Solver(cloud, reorder=reorder, lds_tiles=args.lds_tiles, tile_block=args.tile_block,
-                    precision=precision, **CASE)
+                    points_per_thread=args.points_per_thread, precision=precision, **CASE)
     tiles, tile_max, _ = solver.layout()
     log(f"{'mixed' if precision else 'fp64'} context created in "
         f"{time.perf_counter() - t:.1f}s; LDS tile mask {tiles}, largest tile {tile_max} points")
@@ -181,10 +181,11 @@ def valu_bound(name, avg_ms):
 def run_batch(args):
     """The many-case workload: 64 flow conditions over 8 NACA 4-digit clouds
     of 50k-800k points (k = 20, inner passes 0, CFL 0.2, K 5), all cases
-    advanced together, one captured graph replay per args.graph_iters
-    iterations (--steps 1000 --graph-iters 1000: all iterations in one
-    launch), with the layout chosen by lskum_autotune on one case per
-    cloud.  value = sum of case points x iterations / s.  Two cases are
+    advanced together as one captured graph of all --steps iterations,
+    replayed once (--graph-iters K: one replay per K iterations), with the
+    block shape, flux tiles and state-update points per thread chosen by
+    lskum_autotune on one case per cloud.  value = sum of case points x
+    iterations / s.  Two cases are
     rerun alone and must match their batch results bit for bit; cases whose
     residual turned non-finite are listed (they run on as NaN without
     touching the others).  With args.graph_sweep, also the time per
@@ -199,8 +200,9 @@ def run_batch(args):
     reps = [next(j for j, c in enumerate(cases) if c[0] == ci) for ci in range(len(clouds))]
     best, trials = autotune(clouds, [cases[j] for j in reps], iters=5, **num)
     log(f"autotune: best {best}")
+    graph_iters = args.graph_iters or args.steps
     b = Batch(clouds, cases, tile_block=best["tile_block"], lds_tiles=best["lds_tiles"],
-              graph_iters=args.graph_iters, **num)
+              points_per_thread=best["points_per_thread"], graph_iters=graph_iters, **num)
     if args.warmup:
         b.q_lskum(args.warmup)
     t0 = time.perf_counter()
@@ -234,7 +236,8 @@ def run_batch(args):
     for j in (0, len(cases) - 1):
         ci, mach, aoa = cases[j]
         s = Solver(clouds[ci], mach=mach, aoa=aoa, tile_block=best["tile_block"],
-                   lds_tiles=best["lds_tiles"], **num)
+                   lds_tiles=best["lds_tiles"], points_per_thread=best["points_per_thread"],
+                   **num)
         s.q_lskum(args.warmup)
         srn, _ = s.q_lskum(args.steps)
         same[j] = bool(np.array_equal(srn, rn[j]) and
@@ -248,8 +251,9 @@ def run_batch(args):
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "batch_64cases_8clouds_50k-800k_k20", "cases": len(cases),
                    "clouds": [c.n for c in clouds], "case_points": pts,
-                   "graph_iters": args.graph_iters,
-                   "tile_block": best["tile_block"], "lds_tile_mask": best["lds_tiles"]},
+                   "graph_iters": graph_iters,
+                   "tile_block": best["tile_block"], "lds_tile_mask": best["lds_tiles"],
+                   "points_per_thread": best["points_per_thread"]},
         "autotune": {"best": best, "trials": trials},
         "cases_bit_identical_to_single_runs": same,
         "graph_length_sweep": sweep,
@@ -447,12 +451,15 @@ def main():
                     help="-1 library default, else mask: 1 q_derivatives, 2 q_inner, "
                          "4 func_delta, 8 flux")
     ap.add_argument("--tile-block", type=int, default=0, choices=[0, 64, 128, 256])
+    ap.add_argument("--points-per-thread", type=int, default=0, choices=[0, 1, 2, 4],
+                    help="state update points per thread (0: library default)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="leave the shape-gradient (adjoint) measurement out of the line")
-    ap.add_argument("--graph-iters", type=int, default=10,
-                    help="many-case workload: iterations per captured graph")
+    ap.add_argument("--graph-iters", type=int, default=0,
+                    help="many-case workload: iterations per captured graph (0: all --steps "
+                         "iterations in one graph, one launch)")
     ap.add_argument("--graph-sweep", action="store_true",
                     help="many-case workload: also time graphs of 0, 10, 100, 1000 iterations")
     ap.add_argument("--converge", type=int, default=30000,
@@ -501,7 +508,8 @@ def main():
                    "inner_iterations": 2, "mach": 0.85, "aoa_deg": 1.25, "cfl": 0.2,
                    "vl_const": 5.0, "precision": "fp64", "order": args.reorder,
                    "lds_tile_mask": f["tiles"], "tile_max": f["tile_max"],
-                   "tile_block": args.tile_block or 256},
+                   "tile_block": args.tile_block or 256,
+                   "points_per_thread": args.points_per_thread or 1},
         "roofline": {"bound": "hbm", "kernel": "k_flux", "achieved": achieved,
                      "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS,
                      "traffic": load_traffic("flux_naca0012_2p5m_k24_traffic.json"),

@@ -319,7 +319,9 @@ bool params_ok(const lskum_params &p) {
          (p.reorder >= LSKUM_REORDER_NONE && p.reorder <= LSKUM_REORDER_HILBERT) &&
          (p.lds_tiles == LSKUM_TILES_AUTO || (p.lds_tiles >= 0 && p.lds_tiles <= 15)) &&
          (p.tile_block == 0 || p.tile_block == 64 || p.tile_block == 128 ||
-          p.tile_block == 256);
+          p.tile_block == 256) &&
+         (p.points_per_thread == 0 || p.points_per_thread == 1 || p.points_per_thread == 2 ||
+          p.points_per_thread == 4);
 }
 
 // Least-squares coefficients of each sub-stencil (geometry only): the sums
@@ -468,6 +470,7 @@ int create_impl(const lskum_cloud *clouds, int32_t n_clouds, const lskum_case *c
   c->prm = p;
   c->mixed = p.precision == LSKUM_MIXED;
   c->tile_block = p.tile_block ? p.tile_block : kTileBlockDefault;
+  c->su_ppt = p.points_per_thread ? p.points_per_thread : 1;
   c->clouds.resize(n_clouds);
   for (int32_t ci = 0; ci < n_clouds; ci++) {
     std::string err;
@@ -1453,6 +1456,7 @@ int lskum_autotune(const lskum_cloud *clouds, int32_t n_clouds, const lskum_case
   const int32_t masks[2] = {LSKUM_TILE_QDERIV | LSKUM_TILE_QINNER | LSKUM_TILE_DELTA,
                             LSKUM_TILE_QDERIV | LSKUM_TILE_QINNER | LSKUM_TILE_DELTA |
                                 LSKUM_TILE_FLUX};
+  const int32_t ppts[2] = {2, 4};
   int32_t count = 0;
   best->ms_per_iter = -1.0;
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1460,40 +1464,49 @@ int lskum_autotune(const lskum_cloud *clouds, int32_t n_clouds, const lskum_case
     if (e0) (void)hipEventDestroy(e0);
     return LSKUM_EHIP;
   }
-  int rc = LSKUM_OK;
-  for (int32_t tb : blocks) {
-    for (int32_t mask : masks) {
-      lskum_params p = *params;
-      p.tile_block = tb;
-      p.lds_tiles = mask;
-      lskum_ctx *ctx = nullptr;
-      rc = create_impl(clouds, n_clouds, cases, n_cases, &p, &ctx);
-      if (rc) {
-        lskum_destroy(ctx);
-        break;
-      }
-      lskum_tune_result r;
-      r.tile_block = tb;
-      r.lds_tiles = (int32_t)ctx->tiles;  // what fits: a tile too big for LDS is dropped
-      // one graph of `iters` iterations, after one warm-up replay of it
-      rc = lskum_set_graph(ctx, iters);
-      if (!rc) rc = lskum_iterate(ctx, iters, nullptr, nullptr);
-      if (!rc && hipEventRecord(e0, ctx->stream) != hipSuccess) rc = LSKUM_EHIP;
-      if (!rc) rc = lskum_iterate(ctx, iters, nullptr, nullptr);
-      if (!rc && hipEventRecord(e1, ctx->stream) != hipSuccess) rc = LSKUM_EHIP;
-      float ms = 0.f;
-      if (!rc && (hipEventSynchronize(e1) != hipSuccess ||
-                  hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
-        rc = LSKUM_EHIP;
+  // one candidate: built once, timed over one graph of `iters` iterations
+  // after one warm-up replay of it
+  auto trial = [&](int32_t tb, int32_t mask, int32_t ppt) -> int {
+    lskum_params p = *params;
+    p.tile_block = tb;
+    p.lds_tiles = mask;
+    p.points_per_thread = ppt;
+    lskum_ctx *ctx = nullptr;
+    int rc = create_impl(clouds, n_clouds, cases, n_cases, &p, &ctx);
+    if (rc) {
       lskum_destroy(ctx);
-      if (rc) break;
-      r.ms_per_iter = ms / iters;
-      if (trials && count < max_trials) trials[count] = r;
-      count++;
-      if (best->ms_per_iter < 0.0 || r.ms_per_iter < best->ms_per_iter) *best = r;
+      return rc;
     }
-    if (rc) break;
-  }
+    lskum_tune_result r;
+    r.tile_block = tb;
+    r.lds_tiles = (int32_t)ctx->tiles;  // what fits: a tile too big for LDS is dropped
+    r.points_per_thread = ctx->su_ppt;
+    rc = lskum_set_graph(ctx, iters);
+    if (!rc) rc = lskum_iterate(ctx, iters, nullptr, nullptr);
+    if (!rc && hipEventRecord(e0, ctx->stream) != hipSuccess) rc = LSKUM_EHIP;
+    if (!rc) rc = lskum_iterate(ctx, iters, nullptr, nullptr);
+    if (!rc && hipEventRecord(e1, ctx->stream) != hipSuccess) rc = LSKUM_EHIP;
+    float ms = 0.f;
+    if (!rc && (hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+      rc = LSKUM_EHIP;
+    lskum_destroy(ctx);
+    if (rc) return rc;
+    r.ms_per_iter = ms / iters;
+    if (trials && count < max_trials) trials[count] = r;
+    count++;
+    if (best->ms_per_iter < 0.0 || r.ms_per_iter < best->ms_per_iter) *best = r;
+    return LSKUM_OK;
+  };
+  int rc = LSKUM_OK;
+  // block shape and tiles over a grid, then the state update's points per
+  // thread on the best of them (its kernel is independent of the others')
+  for (int32_t tb : blocks)
+    for (int32_t mask : masks)
+      if (!rc) rc = trial(tb, mask, 1);
+  const lskum_tune_result grid_best = *best;
+  for (int32_t ppt : ppts)
+    if (!rc) rc = trial(grid_best.tile_block, grid_best.lds_tiles, ppt);
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (n_trials) *n_trials = count;

@@ -78,6 +78,7 @@ struct lskum_ctx {
   int tile_max = 0;
   unsigned tiles = 0;
   int tile_block = kTileBlockDefault;
+  int su_ppt = 1;  // points per thread of the state update
   int flux_cap = 0;  // tile entries the flux stages (0: it reads through L2)
   int slice_width = 0;
   int qi_chunks = 0, qi_grid = 0;  // pipelined inner pass (0: not used)
@@ -140,6 +141,7 @@ struct lskum_ctx {
     c.tile_max = tile_max;
     c.tiles = tiles;
     c.tile_block = tile_block;
+    c.su_ppt = su_ppt;
     c.flux_cap = flux_cap;
     c.slice_width = slice_width;
     c.qi_chunks = qi_chunks;

@@ -902,9 +902,13 @@ __global__ __launch_bounds__(256) void k_flux(
 }
 
 // ------------------------------------------------------------ state update
+// The state update's arithmetic is not contracted into FMAs: left to the
+// compiler, which products it fuses could differ between the kernel's
+// points-per-thread instances, and their results must agree to the bit.
 // state_update_mod.h:18-33
 __device__ __forceinline__ void primitive_to_conserved(const double p[4], double nx, double ny,
                                                        double U[4]) {
+#pragma clang fp contract(off)
   const double rho = p[0];
   U[0] = rho;
   const double temp1 = rho * p[1];
@@ -917,6 +921,7 @@ __device__ __forceinline__ void primitive_to_conserved(const double p[4], double
 // state_update_mod.h:35-90
 __device__ __forceinline__ void conserved_vector_Ubar(const double p[4], double nx, double ny,
                                                       const FlowConsts &fc, double Ubar[4]) {
+#pragma clang fp contract(off)
   const double rho_inf = fc.rho_inf, pr_inf = fc.pr_inf, gamma_new = 1.4;
   const double tx = ny, ty = -nx;
   const double u1_inf_rot = fc.u1_inf * tx + fc.u2_inf * ty;
@@ -973,11 +978,103 @@ __device__ void block_reduce3(double &s, double &mx, int &arg) {
   arg = sa[0];
 }
 
-// state_update_mod.h:92-283.  It also writes, from the new state, the
-// q-variables of the next stage (q_variables_mod.h:9-34) and, when `speed`
-// is set, the denominator func_delta divides by
+// One point's inputs to the state update, loaded before any point of the
+// thread is computed (so that a thread's PPT points have their loads in
+// flight together).
+struct StateIn {
+  double p1[4], p2[4], fr[4], nx, ny;
+  int type;
+};
+
+__device__ __forceinline__ void state_load(int n, int i, const double *__restrict__ prim,
+                                           const double *__restrict__ prim_old,
+                                           const double *__restrict__ fres,
+                                           const double *__restrict__ nxa,
+                                           const double *__restrict__ nya,
+                                           const uint8_t *__restrict__ flag, StateIn &in) {
+  for (int r = 0; r < 4; r++) {
+    in.p1[r] = prim[(size_t)r * n + i];
+    in.p2[r] = prim_old[(size_t)r * n + i];
+    in.fr[r] = fres[(size_t)r * n + i];
+  }
+  in.nx = nxa[i];
+  in.ny = nya[i];
+  in.type = flag[i];
+}
+
+// state_update_mod.h:92-283 for point i.  It also writes, from the new
+// state, the q-variables of the next stage (q_variables_mod.h:9-34) and,
+// when `speed` is set, the denominator func_delta divides by
 // (state_update_mod.h:483-485): both are pointwise functions of the state,
 // so computing them here gives the values the separate kernels would.
+// Returns the point's residual term (0 for outer points) and sets arg to
+// its caller index (outer points: left unchanged).
+__device__ __forceinline__ double state_point(int n, int i, int rk, const StateIn &in,
+                                              const FlowConsts &fc, double *__restrict__ prim,
+                                              const int32_t *__restrict__ perm,
+                                              double *__restrict__ base,
+                                              float4 *__restrict__ q32, int speed, int &arg) {
+#pragma clang fp contract(off)
+  const double obt = 1.0 / 3.0, tbt = 2.0 / 3.0;
+  const double nx = in.nx, ny = in.ny;
+  const int type = in.type;
+  double U[4], U_old[4];
+  double res_sqr = 0.0;
+  if (type != 2) {
+    primitive_to_conserved(in.p1, nx, ny, U);
+    primitive_to_conserved(in.p2, nx, ny, U_old);
+  } else {
+    conserved_vector_Ubar(in.p1, nx, ny, fc, U);
+    conserved_vector_Ubar(in.p2, nx, ny, fc, U_old);
+  }
+  const double temp0 = U[0];
+  if (rk != 3) {
+    for (int j = 0; j < 4; j++) U[j] = U[j] - (0.5 * in.fr[j]);
+  } else {
+    for (int j = 0; j < 4; j++) U[j] = tbt * U_old[j] + obt * (U[j] - 0.5 * in.fr[j]);
+  }
+  if (type == 0) U[2] = 0.;
+  const double U2_rot = U[1], U3_rot = U[2];
+  U[1] = U2_rot * ny + U3_rot * nx;
+  U[2] = U3_rot * ny - U2_rot * nx;
+  if (type != 2) {
+    res_sqr = (U[0] - temp0) * (U[0] - temp0);
+    arg = perm[i];
+  }
+  const double temp = 1.0 / U[0];
+  const double rho = U[0], u1 = U[1] * temp, u2 = U[2] * temp;
+  const double pr = 0.4 * U[3] - (0.2 * temp) * (U[1] * U[1] + U[2] * U[2]);
+  prim[i] = rho;
+  prim[(size_t)n + i] = u1;
+  prim[(size_t)2 * n + i] = u2;
+  prim[(size_t)3 * n + i] = pr;
+  if (base || q32) {
+    const double beta = 0.5 * rho / pr;
+    double q[4];
+    q[0] = log(rho) + (log(beta) * 2.5) - beta * (u1 * u1 + u2 * u2);
+    const double two_times_beta = 2.0 * beta;
+    q[1] = two_times_beta * u1;
+    q[2] = two_times_beta * u2;
+    q[3] = -two_times_beta;
+    if (q32)  // mixed form: q rounded once, from the FP64 state
+      q32[i] = make_float4((float)q[0], (float)q[1], (float)q[2], (float)q[3]);
+    else
+      store4(base + (size_t)B_STRIDE * i + B_Q, q);
+  }
+  if (base && speed) {
+    const double mod_u = sqrt(u1 * u1 + u2 * u2);
+    base[(size_t)B_STRIDE * i + B_SPEED] = mod_u + 3.0 * sqrt(pr / rho);
+  }
+  return res_sqr;
+}
+
+// The state update over 256-point blocks, PPT points per thread (256 / PPT
+// threads a block; thread t takes points t, t + 256/PPT, ...).  Each point's
+// residual term goes to its slot in the block and the slots are reduced in
+// one fixed tree, so the partial sums do not depend on PPT: every choice
+// gives the same bits.  Ties of the maximum go to the smaller caller index,
+// as the reference's strict '>' over points in order.
+template <int PPT>
 __global__ __launch_bounds__(256) void k_state_update(int n, int rk, double *__restrict__ prim,
                                                       const double *__restrict__ prim_old,
                                                       const double *__restrict__ fres,
@@ -992,74 +1089,46 @@ __global__ __launch_bounds__(256) void k_state_update(int n, int rk, double *__r
                                                       int32_t *__restrict__ parg,
                                                       double *__restrict__ base,
                                                       float4 *__restrict__ q32, int speed) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  constexpr int T = 256 / PPT;
+  __shared__ double ss[256], sm[256];
+  __shared__ int sa[256];
+  const int t = threadIdx.x;
+  const int i0 = blockIdx.x * 256 + t;
   // a batch pads every case to whole 256-point blocks: one case per block
   const FlowConsts fc = fcs[block_case ? block_case[blockIdx.x] : 0];
-  double res_sqr = 0.0;
-  int arg = 0x7fffffff;
-  if (i < n) {
-    const double obt = 1.0 / 3.0, tbt = 2.0 / 3.0;
-    const double nx = nxa[i], ny = nya[i];
-    const int type = flag[i];
-    double p1[4], p2[4], U[4], U_old[4];
-    for (int r = 0; r < 4; r++) {
-      p1[r] = prim[(size_t)r * n + i];
-      p2[r] = prim_old[(size_t)r * n + i];
-    }
-    if (type != 2) {
-      primitive_to_conserved(p1, nx, ny, U);
-      primitive_to_conserved(p2, nx, ny, U_old);
-    } else {
-      conserved_vector_Ubar(p1, nx, ny, fc, U);
-      conserved_vector_Ubar(p2, nx, ny, fc, U_old);
-    }
-    const double temp0 = U[0];
-    if (rk != 3) {
-      for (int j = 0; j < 4; j++) U[j] = U[j] - (0.5 * fres[(size_t)j * n + i]);
-    } else {
-      for (int j = 0; j < 4; j++)
-        U[j] = tbt * U_old[j] + obt * (U[j] - 0.5 * fres[(size_t)j * n + i]);
-    }
-    if (type == 0) U[2] = 0.;
-    const double U2_rot = U[1], U3_rot = U[2];
-    U[1] = U2_rot * ny + U3_rot * nx;
-    U[2] = U3_rot * ny - U2_rot * nx;
-    if (type != 2) {
-      res_sqr = (U[0] - temp0) * (U[0] - temp0);
-      arg = perm[i];
-    }
-    const double temp = 1.0 / U[0];
-    const double rho = U[0], u1 = U[1] * temp, u2 = U[2] * temp;
-    const double pr = 0.4 * U[3] - (0.2 * temp) * (U[1] * U[1] + U[2] * U[2]);
-    prim[i] = rho;
-    prim[(size_t)n + i] = u1;
-    prim[(size_t)2 * n + i] = u2;
-    prim[(size_t)3 * n + i] = pr;
-    if (base || q32) {
-      const double beta = 0.5 * rho / pr;
-      double q[4];
-      q[0] = log(rho) + (log(beta) * 2.5) - beta * (u1 * u1 + u2 * u2);
-      const double two_times_beta = 2.0 * beta;
-      q[1] = two_times_beta * u1;
-      q[2] = two_times_beta * u2;
-      q[3] = -two_times_beta;
-      if (q32)  // mixed form: q rounded once, from the FP64 state
-        q32[i] = make_float4((float)q[0], (float)q[1], (float)q[2], (float)q[3]);
-      else
-        store4(base + (size_t)B_STRIDE * i + B_Q, q);
-    }
-    if (base && speed) {
-      const double mod_u = sqrt(u1 * u1 + u2 * u2);
-      base[(size_t)B_STRIDE * i + B_SPEED] = mod_u + 3.0 * sqrt(pr / rho);
-    }
+  StateIn in[PPT];
+#pragma unroll
+  for (int j = 0; j < PPT; j++) {
+    const int i = i0 + j * T;
+    if (i < n) state_load(n, i, prim, prim_old, fres, nxa, nya, flag, in[j]);
   }
-  double s = res_sqr, mx = res_sqr;
-  if (arg == 0x7fffffff) mx = -1.0;
-  block_reduce3(s, mx, arg);
-  if (threadIdx.x == 0) {
-    psum[blockIdx.x] = s;
-    pmax[blockIdx.x] = mx;
-    parg[blockIdx.x] = arg;
+#pragma unroll
+  for (int j = 0; j < PPT; j++) {
+    const int i = i0 + j * T;
+    double res_sqr = 0.0;
+    int arg = 0x7fffffff;
+    if (i < n) res_sqr = state_point(n, i, rk, in[j], fc, prim, perm, base, q32, speed, arg);
+    ss[t + j * T] = res_sqr;
+    sm[t + j * T] = arg == 0x7fffffff ? -1.0 : res_sqr;
+    sa[t + j * T] = arg;
+  }
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    for (int u = t; u < w; u += T) {
+      ss[u] = ss[u] + ss[u + w];
+      const double a = sm[u], b = sm[u + w];
+      const int ia = sa[u], ib = sa[u + w];
+      if (b > a || (b == a && ib < ia)) {
+        sm[u] = b;
+        sa[u] = ib;
+      }
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    psum[blockIdx.x] = ss[0];
+    pmax[blockIdx.x] = sm[0];
+    parg[blockIdx.x] = sa[0];
   }
 }
 
@@ -1426,10 +1495,18 @@ void launch_state_update(const DevCloud &c, int rk, double *prim, const double *
                          const double *flux_res, const FlowConsts *fcs,
                          const uint16_t *block_case, const Partials &p, double *base,
                          float *q32, bool speed, hipStream_t s) {
-  // fixed 256-thread blocks: the partial layout must not depend on tuning
-  hipLaunchKernelGGL(k_state_update, dim3(p.nblocks), dim3(256), 0, s, c.n, rk, prim, prim_old,
-                     flux_res, c.nx, c.ny, c.flag, c.perm, fcs, block_case, p.sum, p.max, p.arg,
-                     base, reinterpret_cast<float4 *>(q32), speed ? 1 : 0);
+  // fixed 256-point blocks, whatever the points per thread: the partial
+  // layout must not depend on tuning
+#define SU_ARGS                                                                               \
+  c.n, rk, prim, prim_old, flux_res, c.nx, c.ny, c.flag, c.perm, fcs, block_case, p.sum, p.max, \
+      p.arg, base, reinterpret_cast<float4 *>(q32), speed ? 1 : 0
+  if (c.su_ppt == 4)
+    hipLaunchKernelGGL(k_state_update<4>, dim3(p.nblocks), dim3(64), 0, s, SU_ARGS);
+  else if (c.su_ppt == 2)
+    hipLaunchKernelGGL(k_state_update<2>, dim3(p.nblocks), dim3(128), 0, s, SU_ARGS);
+  else
+    hipLaunchKernelGGL(k_state_update<1>, dim3(p.nblocks), dim3(256), 0, s, SU_ARGS);
+#undef SU_ARGS
 }
 
 void launch_reduce(const Partials &p, int n_cases, const int32_t *seg, double *out,

@@ -51,6 +51,7 @@ struct DevCloud {
   int tile_max;              // largest tile, entries
   unsigned tiles;            // LSKUM_TILE_* bits of the kernels using them
   int tile_block;            // points per block of the stencil kernels
+  int su_ppt;                // points per thread of the state update: 1, 2, 4
   int flux_cap;              // entries of a tile the flux stages (<= tile_max)
   int slice_width;           // longest list of any point (entries of a slice row)
   int qi_chunks;             // pipelined inner pass: its layout code (0: not used)

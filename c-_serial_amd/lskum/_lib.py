@@ -62,6 +62,7 @@ class Params(C.Structure):
         ("vl_const", C.c_double), ("power", C.c_double),
         ("inner_iterations", C.c_int32), ("precision", C.c_int32),
         ("reorder", C.c_int32), ("lds_tiles", C.c_int32), ("tile_block", C.c_int32),
+        ("points_per_thread", C.c_int32),
     ]
 
 
@@ -70,7 +71,8 @@ class Case(C.Structure):
 
 
 class TuneResult(C.Structure):
-    _fields_ = [("tile_block", C.c_int32), ("lds_tiles", C.c_int32), ("ms_per_iter", C.c_double)]
+    _fields_ = [("tile_block", C.c_int32), ("lds_tiles", C.c_int32),
+                ("points_per_thread", C.c_int32), ("ms_per_iter", C.c_double)]
 
 
 # (name, restype, argtypes)

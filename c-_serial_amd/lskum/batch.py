@@ -18,12 +18,13 @@ from .cloud import Cloud
 class Batch:
     def __init__(self, clouds, cases, cfl=0.2, vl_const=5.0, power=0.0, inner_iterations=0,
                  precision=L.FP64, reorder=L.REORDER_HILBERT, lds_tiles=L.TILES_AUTO,
-                 tile_block=0, graph_iters=0):
+                 tile_block=0, points_per_thread=0, graph_iters=0):
         """clouds: list of Cloud; cases: list of (cloud index, mach, aoa_deg)."""
         self.clouds = list(clouds)
         self.cases = [(int(c), float(m), float(a)) for c, m, a in cases]
         self.params = L.Params(self.cases[0][1], self.cases[0][2], cfl, vl_const, power,
-                               inner_iterations, precision, reorder, lds_tiles, tile_block)
+                               inner_iterations, precision, reorder, lds_tiles, tile_block,
+                               points_per_thread)
         self._views = [c.view() for c in self.clouds]
         views = (L.CloudView * len(self._views))(*self._views)
         cs = (L.Case * len(self.cases))(*[L.Case(c, m, a) for c, m, a in self.cases])
@@ -128,19 +129,20 @@ class Batch:
 def autotune(clouds, cases, iters=5, cfl=0.2, vl_const=5.0, power=0.0, inner_iterations=0,
              precision=L.FP64, reorder=L.REORDER_HILBERT):
     """lskum_autotune over the given cases: returns (best, trials) as dicts
-    with tile_block, lds_tiles and ms_per_iter."""
+    with tile_block, lds_tiles, points_per_thread and ms_per_iter."""
     cases = [(int(c), float(m), float(a)) for c, m, a in cases]
     views = [c.view() for c in clouds]
     va = (L.CloudView * len(views))(*views)
     cs = (L.Case * len(cases))(*[L.Case(c, m, a) for c, m, a in cases])
     p = L.Params(cases[0][1], cases[0][2], cfl, vl_const, power, inner_iterations, precision,
-                 reorder, L.TILES_AUTO, 0)
+                 reorder, L.TILES_AUTO, 0, 0)
     best = L.TuneResult()
-    trials = (L.TuneResult * 8)()
+    trials = (L.TuneResult * 16)()
     n = C.c_int32()
     L.check(L.lib().lskum_autotune(va, len(views), cs, len(cases), C.byref(p), iters,
-                                   C.byref(best), trials, 8, C.byref(n)))
+                                   C.byref(best), trials, 16, C.byref(n)))
     as_dict = lambda r: {"tile_block": r.tile_block, "lds_tiles": r.lds_tiles,
+                         "points_per_thread": r.points_per_thread,
                          "ms_per_iter": r.ms_per_iter}
     return as_dict(best), [as_dict(trials[i]) for i in range(n.value)]
 

@@ -30,11 +30,11 @@ _SHAPES = {L.A_PRIM: (4,), L.A_PRIM_OLD: (4,), L.A_Q: (4,), L.A_DQ: (2, 4),
 class Solver:
     def __init__(self, cloud: Cloud, mach=0.63, aoa=2.0, cfl=0.1, vl_const=150.0, power=0.0,
                  inner_iterations=3, precision=L.FP64, reorder=L.REORDER_HILBERT,
-                 lds_tiles=L.TILES_AUTO, tile_block=0):
+                 lds_tiles=L.TILES_AUTO, tile_block=0, points_per_thread=0):
         self.cloud = cloud
         self.n = cloud.n
         self.params = L.Params(mach, aoa, cfl, vl_const, power, inner_iterations, precision,
-                               reorder, lds_tiles, tile_block)
+                               reorder, lds_tiles, tile_block, points_per_thread)
         self.inner_iterations = inner_iterations
         view = cloud.view()
         ctx = C.c_void_p()

@@ -159,6 +159,10 @@ typedef struct lskum_params {
                             /* of LSKUM_TILE_* (used when the tiles fit)   */
   int32_t tile_block;       /* points per block of the stencil kernels (=  */
                             /* per LDS tile): 64, 128, 256; 0 = default    */
+  int32_t points_per_thread; /* points per thread of the pointwise state   */
+                            /* update (its blocks stay 256 points): 1, 2, */
+                            /* 4; 0 = default (1).  Results do not depend */
+                            /* on it, bit for bit                          */
 } lskum_params;
 
 #define LSKUM_TILES_AUTO (-1)
@@ -445,16 +449,19 @@ int lskum_hicks_henne_fields(const lskum_cloud *cloud, int32_t n_per_side, doubl
  * candidate layout -- points per block of the stencil kernels 64, 128 or
  * 256 (the block shape; one point per thread, whose neighbour list the
  * thread walks), and the flux's neighbour data staged in LDS or read
- * through L2 -- times one captured graph of `iters` whole iterations after
- * a warm-up replay, and returns the fastest in *best.  Per-point arithmetic
- * does not depend on the layout, so the choice changes speed only.
- * trials (up to max_trials entries) receives every candidate; lds_tiles is
- * the mask that fitted.  Pass a representative subset of a large batch (for
- * example one case per cloud): the set-up runs once per candidate.
+ * through L2; then, on the fastest of those six, 2 and 4 points per thread
+ * of the pointwise state update -- times one captured graph of `iters`
+ * whole iterations after a warm-up replay, and returns the fastest in
+ * *best.  Per-point arithmetic does not depend on the layout, so the choice
+ * changes speed only.  trials (up to max_trials entries, 8 candidates)
+ * receives every candidate; lds_tiles is the mask that fitted.  Pass a
+ * representative subset of a large batch (for example one case per cloud):
+ * the set-up runs once per candidate.
  */
 typedef struct lskum_tune_result {
   int32_t tile_block;
   int32_t lds_tiles;
+  int32_t points_per_thread;
   double ms_per_iter;
 } lskum_tune_result;
 int lskum_autotune(const lskum_cloud *clouds, int32_t n_clouds,

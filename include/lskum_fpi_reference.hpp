@@ -93,6 +93,7 @@ inline lskum_params params_from_reference(int32_t precision = LSKUM_FP64) {
   p.reorder = LSKUM_REORDER_HILBERT;
   p.lds_tiles = LSKUM_TILES_AUTO;
   p.tile_block = 0;
+  p.points_per_thread = 0;
   return p;
 }
 

@@ -103,3 +103,32 @@ def test_a_failing_case_leaves_the_others_untouched():
     np.testing.assert_array_equal(np.delete(rn, 2, axis=0), np.delete(rr, 2, axis=0))
     f, fr = b.compute_cl_cd_cm(), ref.compute_cl_cd_cm()
     np.testing.assert_array_equal(np.delete(f, 2, axis=0), np.delete(fr, 2, axis=0))
+
+
+@pytest.mark.parametrize("precision", [L.FP64, L.MIXED])
+def test_points_per_thread_leave_results_unchanged(precision):
+    # the state update at 1, 2 and 4 points per thread (256-point blocks in
+    # all three): the same per-point arithmetic and the same reduction tree
+    # over each block's slots, so the same bits -- on a batch whose cases
+    # pad their last blocks, and on a single case whose point count is not
+    # a multiple of 256
+    clouds, cases = _small_batch()
+    cloud = lskum.workload("naca0012_1480_k20")
+    print("case: points per thread 1/2/4, precision", precision, flush=True)
+    runs = []
+    for ppt in (1, 2, 4):
+        b = Batch(clouds, cases, precision=precision, points_per_thread=ppt, **NUMERICS)
+        rn, rs = b.q_lskum(12)
+        s = Solver(cloud, precision=precision, points_per_thread=ppt, **NUMERICS)
+        srn, _ = s.q_lskum(12)
+        runs.append((rn, rs, b.compute_cl_cd_cm(), b.prim(), srn, s.prim))
+        b.close()
+        s.close()
+    for other in runs[1:]:
+        np.testing.assert_array_equal(runs[0][0], other[0])
+        np.testing.assert_array_equal(runs[0][1], other[1])
+        np.testing.assert_array_equal(runs[0][2], other[2])
+        for pa, pb in zip(runs[0][3], other[3]):
+            np.testing.assert_array_equal(pa, pb)
+        np.testing.assert_array_equal(runs[0][4], other[4])
+        np.testing.assert_array_equal(runs[0][5], other[5])

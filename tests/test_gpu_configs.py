@@ -81,21 +81,27 @@ def test_many_case_workload_autotuned_graphs_equal_single_runs():
     reps = [next(j for j, c in enumerate(cases) if c[0] == ci) for ci in range(len(clouds))]
     best, trials = autotune(clouds, [cases[j] for j in reps], iters=5, **num)
     print("case: autotune best", best, "of", len(trials), "trials", flush=True)
-    assert len(trials) >= 4
+    # six block-shape/tile candidates, then 2 and 4 points per thread
+    assert len(trials) == 8
+    assert {t["points_per_thread"] for t in trials} == {1, 2, 4}
     assert best["tile_block"] in (64, 128, 256)
     assert best["ms_per_iter"] == min(t["ms_per_iter"] for t in trials)
+    # all 50 iterations as one captured graph: one launch
     b = Batch(clouds, cases, tile_block=best["tile_block"], lds_tiles=best["lds_tiles"],
-              graph_iters=10, **num)
+              points_per_thread=best["points_per_thread"], graph_iters=50, **num)
     rn, rs = b.q_lskum(50)
     forces = b.compute_cl_cd_cm()
     prims = b.prim()
     assert not any(b.failed_at())
     b.close()
     assert np.isfinite(rn).all() and np.isfinite(forces).all()
+    # the single runs take another points-per-thread choice: results do
+    # not depend on it
+    other_ppt = 1 if best["points_per_thread"] != 1 else 4
     for j in (0, 27, 63):
         ci, mach, aoa = cases[j]
         s = Solver(clouds[ci], mach=mach, aoa=aoa, tile_block=best["tile_block"],
-                   lds_tiles=best["lds_tiles"], **num)
+                   lds_tiles=best["lds_tiles"], points_per_thread=other_ppt, **num)
         srn, srs = s.q_lskum(50)
         print(f"case: case {j} (cloud {ci}, M {mach:.3f}, AoA {aoa:.2f}) alone", flush=True)
         np.testing.assert_array_equal(rn[j], srn)

@@ -168,11 +168,11 @@ def test_rcm_is_a_permutation_and_narrows_a_shuffled_cloud():
 
 def _params(**kw):
     p = dict(mach=0.63, aoa=2.0, cfl=0.2, vl_const=5.0, power=0.0, inner_iterations=0,
-             precision=0, reorder=0, lds_tiles=-1, tile_block=0)
+             precision=0, reorder=0, lds_tiles=-1, tile_block=0, points_per_thread=0)
     p.update(kw)
     return L.Params(p["mach"], p["aoa"], p["cfl"], p["vl_const"], p["power"],
                     p["inner_iterations"], p["precision"], p["reorder"], p["lds_tiles"],
-                    p["tile_block"])
+                    p["tile_block"], p["points_per_thread"])
 
 
 def _create(cloud, params):
@@ -189,7 +189,8 @@ def test_create_rejects_bad_parameters_before_touching_the_device():
     c = lskum.workload("naca0012_1480_k20")
     for bad in [dict(mach=-1.0), dict(cfl=0.0), dict(inner_iterations=-1), dict(precision=7),
                 dict(reorder=5), dict(lds_tiles=16), dict(lds_tiles=-2), dict(tile_block=100),
-                dict(tile_block=512)]:
+                dict(tile_block=512), dict(points_per_thread=3), dict(points_per_thread=8),
+                dict(points_per_thread=-1)]:
         rc, msg = _create(c, _params(**bad))
         assert rc == -1, bad
         assert msg
