@@ -186,6 +186,28 @@ struct ConnAhead {
 constexpr int kAheadQDeriv = 3;
 constexpr int kAheadQInner = 4;
 
+// q_variables_mod.h:9-34 for one primitive state, and the denominator
+// func_delta divides by (state_update_mod.h:483-485), with no products fused
+// into FMAs: every kernel that forms them from a state (the q-variables and
+// speed passes, the state update at any points per thread) gives the same
+// bits, whatever the compiler would have fused in each.
+__device__ __forceinline__ void q_of_state(double rho, double u1, double u2, double pr,
+                                           double q[4]) {
+#pragma clang fp contract(off)
+  const double beta = 0.5 * rho / pr;
+  q[0] = log(rho) + (log(beta) * 2.5) - beta * (u1 * u1 + u2 * u2);
+  const double two_times_beta = 2.0 * beta;
+  q[1] = two_times_beta * u1;
+  q[2] = two_times_beta * u2;
+  q[3] = -two_times_beta;
+}
+
+__device__ __forceinline__ double speed_of_state(double rho, double u1, double u2, double pr) {
+#pragma clang fp contract(off)
+  const double mod_u = sqrt(u1 * u1 + u2 * u2);
+  return mod_u + 3.0 * sqrt(pr / rho);
+}
+
 // One neighbour's step, state_update_mod.h:478-488 (speed: the neighbour's
 // |u| + 3 sqrt(p / rho)), its one fused product written out so that every
 // kernel inlining it agrees to the last bit.

@@ -26,14 +26,8 @@ __global__ __launch_bounds__(256) void k_q_variables(int n, const double *__rest
                                                      double *__restrict__ base) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
-  const double rho = prim[k], u1 = prim[n + k], u2 = prim[2 * n + k], pr = prim[3 * n + k];
-  const double beta = 0.5 * rho / pr;
   double q[4];
-  q[0] = log(rho) + (log(beta) * 2.5) - beta * (u1 * u1 + u2 * u2);
-  const double two_times_beta = 2.0 * beta;
-  q[1] = two_times_beta * u1;
-  q[2] = two_times_beta * u2;
-  q[3] = -two_times_beta;
+  q_of_state(prim[k], prim[n + k], prim[2 * n + k], prim[3 * n + k], q);
   store4(base + (size_t)B_STRIDE * k + B_Q, q);
 }
 
@@ -43,9 +37,8 @@ __global__ __launch_bounds__(256) void k_speed(int n, const double *__restrict__
                                                double *__restrict__ base) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
-  const double rho = prim[k], u1 = prim[n + k], u2 = prim[2 * n + k], pr = prim[3 * n + k];
-  const double mod_u = sqrt(u1 * u1 + u2 * u2);
-  base[(size_t)B_STRIDE * k + B_SPEED] = mod_u + 3.0 * sqrt(pr / rho);
+  base[(size_t)B_STRIDE * k + B_SPEED] =
+      speed_of_state(prim[k], prim[n + k], prim[2 * n + k], prim[3 * n + k]);
 }
 
 // ------------------------------------------------------------ local dt
@@ -559,7 +552,7 @@ __device__ __forceinline__ void qd_store_lds(qi_chunk *l2, const qi_chunk (&v)[C
 }
 
 template <int E, int NC, bool DELTA>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_q_derivatives_pf(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DELTA ? 2 : 3, DELTA ? 2 : 3))) void k_q_derivatives_pf(
     int n, int nblk, int lds_conn, const int32_t *__restrict__ off,
     const int32_t *__restrict__ slice_off, const uint16_t *__restrict__ conn16,
     const int32_t *__restrict__ tile_off, const int32_t *__restrict__ tile_idx,
@@ -1049,22 +1042,14 @@ __device__ __forceinline__ double state_point(int n, int i, int rk, const StateI
   prim[(size_t)2 * n + i] = u2;
   prim[(size_t)3 * n + i] = pr;
   if (base || q32) {
-    const double beta = 0.5 * rho / pr;
     double q[4];
-    q[0] = log(rho) + (log(beta) * 2.5) - beta * (u1 * u1 + u2 * u2);
-    const double two_times_beta = 2.0 * beta;
-    q[1] = two_times_beta * u1;
-    q[2] = two_times_beta * u2;
-    q[3] = -two_times_beta;
+    q_of_state(rho, u1, u2, pr, q);
     if (q32)  // mixed form: q rounded once, from the FP64 state
       q32[i] = make_float4((float)q[0], (float)q[1], (float)q[2], (float)q[3]);
     else
       store4(base + (size_t)B_STRIDE * i + B_Q, q);
   }
-  if (base && speed) {
-    const double mod_u = sqrt(u1 * u1 + u2 * u2);
-    base[(size_t)B_STRIDE * i + B_SPEED] = mod_u + 3.0 * sqrt(pr / rho);
-  }
+  if (base && speed) base[(size_t)B_STRIDE * i + B_SPEED] = speed_of_state(rho, u1, u2, pr);
   return res_sqr;
 }
 

@@ -114,12 +114,9 @@ __global__ __launch_bounds__(256) void k_q_variables_m(int n, const double *__re
                                                        float4 *__restrict__ q32) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
-  const double rho = prim[k], u1 = prim[n + k], u2 = prim[2 * n + k], pr = prim[3 * n + k];
-  const double beta = 0.5 * rho / pr;
-  const double q0 = log(rho) + (log(beta) * 2.5) - beta * (u1 * u1 + u2 * u2);
-  const double two_times_beta = 2.0 * beta;
-  q32[k] = make_float4((float)q0, (float)(two_times_beta * u1), (float)(two_times_beta * u2),
-                       (float)(-two_times_beta));
+  double q[4];
+  q_of_state(prim[k], prim[n + k], prim[2 * n + k], prim[3 * n + k], q);
+  q32[k] = make_float4((float)q[0], (float)q[1], (float)q[2], (float)q[3]);
 }
 
 // ------------------------------------------------------------ q derivatives
