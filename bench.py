@@ -346,7 +346,8 @@ def shape_measurement(args, n_fd=len(FD_VARS)):
     cloud = lskum.workload("naca0012_1m_k24")
     log(f"shape: cloud {cloud.n} points, {time.perf_counter() - t:.1f}s")
     s = Solver(cloud, **SHAPE_FLOW)
-    s.set_graph(50)
+    if args.shape_graph:
+        s.set_graph(args.shape_graph)
     t = time.perf_counter()
     done, forces = 0, []
     while done < args.converge:
@@ -433,6 +434,7 @@ def run_shape(args):
         "config": {"workload": "shape_naca0012_1m_k24", "points": r["points"], "k": 24,
                    **{k: v for k, v in SHAPE_FLOW.items()},
                    "converge_iterations": args.converge, "adjoint_iterations": args.adjoint_iters,
+                   "converge_graph_iters": args.shape_graph,
                    "design_variables": r["design_variables"],
                    "bumps": "10 upper + 10 lower, x/c 0.05..0.95, exponent 3",
                    "propagation": "inverse distance |d|^-3, faded to 0 at 0.5 chord"},
@@ -466,6 +468,9 @@ def main():
                     help="shape workload: iterations before the adjoint")
     ap.add_argument("--adjoint-iters", type=int, default=300,
                     help="shape workload: iterations the adjoint differentiates")
+    ap.add_argument("--shape-graph", type=int, default=50,
+                    help="shape workload: iterations per captured graph while converging "
+                         "(0: host launches, as a kernel trace under rocprofv3 needs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: the library runs on one GPU only; the q-LSKUM iteration would shard "
