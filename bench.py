@@ -123,7 +123,7 @@ def cpu_baseline(args):
     return {"value": cloud.n * iters / secs, "unit": "point-iterations/s", "cores": 1,
             "kind": kind, "sample": desc, "seconds": round(secs, 3),
             "size_check": "the reference's rate per point-iteration on one core was the "
-                          "same within 5 % on this sample and on the full 2,502,000-point "
+                          "same within about 5 % on this sample and on the full 2,502,000-point "
                           "cloud (one iteration), measured side by side on one host",
             "cpu": platform.processor() or platform.machine(), "cpu_model": cpu_model(),
             "host_cpus": os.cpu_count(),
