@@ -751,21 +751,30 @@ __global__ __launch_bounds__(256) void k_flux(
   const double *hi = base + (size_t)B_STRIDE * i;
   const double2 xy = reinterpret_cast<const double2 *>(hi)[0];
   const double x_i = xy.x, y_i = xy.y;
-  // dxi, dyi, dxk, dyk: the derivatives halved (exact; the tile holds them so)
-  double qi[4], dxi[4], dyi[4], qmxi[4], qmni[4], cf[8];
+  // dxi, dyi, dxk, dyk: the derivatives halved (exact; the tile holds them
+  // so); dmxi, dmni: the point's qmax - q and qmin - q, the candidates for
+  // its limiters' del_pos, formed once (the reference forms the same
+  // differences per pair, limiters_mod.h:72-78)
+  double qi[4], dxi[4], dyi[4], dmxi[4], dmni[4], cf[8];
   load4(hi + B_Q, qi);
   if (TILED) {
-    flux_tile_entry(lds2, threadIdx.x, dxi, dyi, qmxi, qmni);
+    flux_tile_entry(lds2, threadIdx.x, dxi, dyi, dmxi, dmni);
+    for (int r = 0; r < 4; r++) {
+      dmxi[r] -= qi[r];
+      dmni[r] -= qi[r];
+    }
   } else {
     const double *di = dq + (size_t)DQ_STRIDE * i;
     const double *mi = qm + (size_t)QM_STRIDE * i;
     load4(di, dxi);
     load4(di + 4, dyi);
-    load4(mi, qmxi);
-    load4(mi + 4, qmni);
+    load4(mi, dmxi);
+    load4(mi + 4, dmni);
     for (int r = 0; r < 4; r++) {
       dxi[r] *= 0.5;
       dyi[r] *= 0.5;
+      dmxi[r] -= qi[r];
+      dmni[r] -= qi[r];
     }
   }
   load4(coef + (size_t)COEF_STRIDE * i, cf);
@@ -836,16 +845,14 @@ __global__ __launch_bounds__(256) void k_flux(
     for (int r = 0; r < 4; r++) {
       const double hi_r = fma(delx, dxi[r], dely * dyi[r]);
       const double hk_r = fma(delx, dxk[r], dely * dyk[r]);
-      const double phi_i = venkat_phi(qi[r] - hi_r, qi[r], qmxi[r], qmni[r], eps2_i);
+      // del_neg = q~ - q as the reference rounds it: (q - d/2) - q
+      const double phi_i = venkat_phi_delta((qi[r] - hi_r) - qi[r], dmxi[r], dmni[r], eps2_i);
       const double phi_k = venkat_phi(qk[r] - hk_r, qk[r], qmxk[r], qmnk[r], eps2_k);
       qti[r] = qi[r] - phi_i * hi_r;
       qtk[r] = qk[r] - phi_k * hk_r;
     }
-    double u1, u2, rho, sb, temp;
-    qtilde_state(qti, u1, u2, rho, sb, temp);
-    const EndTerms<double> ei = end_terms_sb(nx, ny, u1, u2, rho, sb, temp);
-    qtilde_state(qtk, u1, u2, rho, sb, temp);
-    const EndTerms<double> ek = end_terms_sb(nx, ny, u1, u2, rho, sb, temp);
+    const EndTerms<double> ei = qtilde_end_terms(qti, nx, ny);
+    const EndTerms<double> ek = qtilde_end_terms(qtk, nx, ny);
     double Gi[4], Gk[4];
     // x direction: x+ (dels <= 0) or x- (dels >= 0)
     const bool xp = (m & XPOS) != 0;

@@ -248,8 +248,10 @@ __device__ __forceinline__ double erf_x(double s, double x) {
   q = fma(q, t, 3.5495262609381824);
   q = fma(q, t, 2.797342582242985);
   q = fma(q, t, 1.0);
+  // e >= 0 (x R(t) <= R(0) < 1), so the sign of s is copied onto it in one
+  // instruction; erf(-0) comes out as -e(0), i.e. -3e-16, as erf(+0) is +3e-16
   const double e = fma(-x, qdiv_loose(p, q), 1.0);
-  return s < 0.0 ? -e : e;
+  return copysign(e, s);
 }
 
 // Terms shared by all split and quadrant fluxes of one state in the frame
@@ -281,6 +283,38 @@ __device__ __forceinline__ EndTerms<T> end_terms_sb(T nx, T ny, T u1, T u2, T rh
   e.e2 = erf_x(S2, X2);
   e.pr_by_rho = pr_by_rho;
   e.u_sqr = e.ut * e.ut + e.un * e.un;
+  return e;
+}
+
+// q~ -> primitive and the end terms in one, for the FP64 flux: the numbers
+// of qtilde_state followed by end_terms_sb, up to rounding, in fewer
+// operations -- |u|^2 is the u1^2 + u2^2 that q~ -> primitive forms (the
+// frame's ut^2 + un^2 in end_terms_sb; a rotation leaves it unchanged), and
+// h = sqrt(beta) (p/rho) / sqrt(pi) = y / (2 sqrt(pi)) with y = beta^-1/2,
+// since beta y^2 = 1.
+__device__ __forceinline__ EndTerms<double> qtilde_end_terms(const double qt[4], double nx,
+                                                             double ny) {
+  const double beta = -qt[3] * 0.5;
+  const double y = rsqrt_nr(beta);
+  const double y2 = y * y;
+  const double temp = 0.5 * y2;
+  const double u1 = qt[1] * temp, u2 = qt[2] * temp;
+  const double usq = u1 * u1 + u2 * u2;
+  const double sb = beta * y;
+  EndTerms<double> e;
+  e.rho = exp_k(qt[0] + beta * usq) * (y2 * y2 * y);
+  const double tx = ny, ty = -nx;
+  e.ut = u1 * tx + u2 * ty;
+  e.un = u1 * nx + u2 * ny;
+  const double S1 = e.ut * sb, S2 = e.un * sb;
+  const double h = (0.5 * kInvSqrtPi) * y;
+  const double X1 = exp_neg_sq(-S1 * S1), X2 = exp_neg_sq(-S2 * S2);
+  e.B1 = X1 * h;
+  e.B2 = X2 * h;
+  e.e1 = erf_x(S1, X1);
+  e.e2 = erf_x(S2, X2);
+  e.pr_by_rho = temp;
+  e.u_sqr = usq;
   return e;
 }
 
