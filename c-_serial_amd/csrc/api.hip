@@ -128,7 +128,10 @@ int do_speed(lskum_ctx *c) {
 int do_func_delta(lskum_ctx *c) {
   {
     KTimer t(c, LSKUM_K_DELTA);
-    launch_func_delta(c->dc(), c->base, c->prm.cfl, c->delta, c->stream);
+    if (c->mixed)
+      launch_func_delta_m(c->dc(), c->base, c->prm.cfl, c->delta, c->stream);
+    else
+      launch_func_delta(c->dc(), c->base, c->prm.cfl, c->delta, c->stream);
   }
   return check_launch(c, "func_delta");
 }
@@ -658,10 +661,8 @@ int create_impl(const lskum_cloud *clouds, int32_t n_clouds, const lskum_case *c
   unsigned want = p.lds_tiles == LSKUM_TILES_AUTO ? auto_mask : (unsigned)p.lds_tiles;
   if (c->flux_cap <= 0) want &= ~(unsigned)LSKUM_TILE_FLUX;
   if (tiles_ok && want && dev_ok) {
-    if (c->mixed)  // func_delta is FP64 in both forms
-      c->tiles = prepare_tiled(want & TILE_DELTA, c->tile_max, c->flux_cap, (size_t)lds_limit) |
-                 prepare_tiled_m(want & ~TILE_DELTA, c->tile_max, c->flux_cap,
-                                 (size_t)lds_limit);
+    if (c->mixed)
+      c->tiles = prepare_tiled_m(want, c->tile_max, c->flux_cap, (size_t)lds_limit);
     else
       c->tiles = prepare_tiled(want, c->tile_max, c->flux_cap, (size_t)lds_limit);
   }

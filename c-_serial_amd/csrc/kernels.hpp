@@ -173,6 +173,10 @@ size_t lds_bytes_flux_m(int tile_max);
 int flux_tile_cap_m(int block, int tile_max, size_t lds_limit, size_t lds_per_cu);
 unsigned prepare_tiled_m(unsigned tiles, int tile_max, int flux_cap, size_t lds_limit);
 void launch_q_variables_m(const DevCloud &c, const double *prim, float *q32, hipStream_t s);
+// func_delta with each neighbour's step in FP32 (records and the minimum's
+// storage as launch_func_delta's)
+void launch_func_delta_m(const DevCloud &c, const double *base, double cfl, double *delta,
+                         hipStream_t s);
 void launch_q_derivatives_m(const DevCloud &c, const MixedState &m, float *dq, double power,
                             hipStream_t s);
 void launch_q_inner_m(const DevCloud &c, const MixedState &m, const float *src, float *dst,

@@ -6,12 +6,13 @@
 // before it is rounded to FP32: at a cosine-clustered leading edge |dx| is
 // ~1e-6 chord, and an FP32 difference of absolute coordinates would keep two
 // digits of it.  The limiter, q~ -> primitive and the split/quadrant fluxes
-// run in FP32, and so do the flux's sub-stencil weights and its per-point
-// sums of the FP32 flux differences.  FP64 is kept for
+// run in FP32, and so do the flux's sub-stencil weights, its per-point sums
+// of the FP32 flux differences, the least-squares right-hand sides of
+// q_derivatives and the inner passes (sums of w dx dq over FP32 neighbour
+// data) and each neighbour's local-step term.  FP64 is kept for
 //  - the least-squares geometry sums, determinants and 2x2 solves (here in
 //    q_derivatives and the inner passes; the flux applies its per-point
 //    coefficients, solved in FP64 once on the host and rounded once),
-//  - the accumulation of every per-point derivative sum,
 //  - the state, the SSPRK update and the residual and force reductions
 //    (kernels.hip, shared with the FP64 form).
 // Differences are formed directly where FP32 would otherwise cancel: the
@@ -119,19 +120,80 @@ __global__ __launch_bounds__(256) void k_q_variables_m(int n, const double *__re
   q32[k] = make_float4((float)q[0], (float)q[1], (float)q[2], (float)q[3]);
 }
 
+// ------------------------------------------------------------ local dt
+// state_update_mod.h:448-496 for the mixed form: the neighbours' coordinates
+// and speeds gathered as k_func_delta does (FP64 records; the offsets
+// x_k - x_i formed in FP64 before they are rounded), the step
+// cfl |x_k - x_i| / speed_k of each neighbour in FP32 -- a neighbour term,
+// like the fluxes it scales -- and the minimum over the neighbours, with the
+// reference's strict comparison from 1, stored as a double.
+template <bool TILED>
+__global__ __launch_bounds__(256) void k_func_delta_m(
+    int n, const int32_t *__restrict__ off, const int32_t *__restrict__ slice_off,
+    const uint32_t *__restrict__ conn, const uint16_t *__restrict__ conn16,
+    const int32_t *__restrict__ tile_off, const int32_t *__restrict__ tile_idx,
+    const double *__restrict__ base, float cfl, double *__restrict__ delta) {
+  extern __shared__ double lds[];
+  constexpr int S = 4;  // x, y, speed, pad
+  const int b = xcd_block();
+  if (TILED) {
+    const int t0 = tile_off[b], tn = tile_off[b + 1] - t0;
+    for (int t = threadIdx.x; t < tn; t += blockDim.x) {
+      const double *h = base + (size_t)B_STRIDE * tile_idx[t0 + t];
+      reinterpret_cast<double2 *>(lds + S * t)[0] = reinterpret_cast<const double2 *>(h)[0];
+      lds[S * t + 2] = h[B_SPEED];
+    }
+    __syncthreads();
+  }
+  const int i = b * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double *hi = TILED ? lds + S * threadIdx.x : base + (size_t)B_STRIDE * i;
+  const double2 xy = reinterpret_cast<const double2 *>(hi)[0];
+  float min_delt = 1.0f;
+  // neighbour e + 1's record and e + 2's conn entry load while e is used
+  auto fetch = [&](uint32_t w, double2 &xyk, double &speed) {
+    const int k = conn_index(w, TILED);
+    const double *hk = TILED ? lds + S * k : base + (size_t)B_STRIDE * k;
+    xyk = reinterpret_cast<const double2 *>(hk)[0];
+    speed = TILED ? hk[2] : hk[B_SPEED];
+  };
+  const Slice L = slice_of(slice_off, conn, conn16, b);
+  const int e0 = 0, last = off[i + 1] - off[i] - 1;
+  double2 xyk = {0.0, 0.0};
+  double speed = 1.0;
+  uint32_t w_n = 0;
+  if (e0 <= last) {
+    fetch(L.raw(e0, TILED), xyk, speed);
+    w_n = L.raw(min(e0 + 1, last), TILED);
+  }
+  for (int e = e0; e <= last; e++) {
+    double2 xy_n;
+    double speed_n;
+    fetch(w_n, xy_n, speed_n);
+    w_n = L.raw(min(e + 2, last), TILED);
+    const float fx = (float)(xyk.x - xy.x), fy = (float)(xyk.y - xy.y);
+    const float dist = sqrtf(fmaf(fx, fx, fy * fy));
+    const float delta_t = cfl * (dist * __builtin_amdgcn_rcpf((float)speed));
+    if (min_delt > delta_t) min_delt = delta_t;
+    xyk = xy_n;
+    speed = speed_n;
+  }
+  delta[i] = (double)min_delt;
+}
+
 // ------------------------------------------------------------ q derivatives
-// q_variables_mod.h:36-118: FP32 gathers of q, FP64 geometry sums, solve
-// and accumulation; qmax/qmin over FP32 values (rounding is monotone, so
-// they are the rounded FP64 extrema).
-// One neighbour's terms of the mixed q_derivatives: qmax/qmin in FP32 and
-// the least-squares sums in FP64, every fused product written out (the
-// kernels that inline this then agree to the last bit).
+// q_variables_mod.h:36-118: FP32 gathers of q, FP64 geometry sums and
+// solve, FP32 right-hand sums w dx (q_k - q_i); qmax/qmin over FP32 values
+// (rounding is monotone, so they are the rounded FP64 extrema).
+// One neighbour's terms of the mixed q_derivatives: qmax/qmin and the
+// right-hand sums in FP32, the geometry sums in FP64, every fused product
+// written out (the kernels that inline this then agree to the last bit).
 template <bool POW>
 __device__ __forceinline__ void qdm_term(double2 pi, const float qi[4], double2 pk,
                                          const float qk[4], double power, float qmax[4],
                                          float qmin[4], double &sum_delx_sqr,
                                          double &sum_dely_sqr, double &sum_delx_dely,
-                                         double sdx[4], double sdy[4]) {
+                                         float sdx[4], float sdy[4]) {
   for (int r = 0; r < 4; r++) {
     qmax[r] = fmaxf(qmax[r], qk[r]);  // the reference's strict comparisons for finite q
     qmin[r] = fminf(qmin[r], qk[r]);
@@ -142,17 +204,18 @@ __device__ __forceinline__ void qdm_term(double2 pi, const float qi[4], double2 
   sum_delx_sqr = fma(wx, delx, sum_delx_sqr);
   sum_dely_sqr = fma(wy, dely, sum_dely_sqr);
   sum_delx_dely = fma(wx, dely, sum_delx_dely);
+  const float fwx = (float)wx, fwy = (float)wy;
   for (int r = 0; r < 4; r++) {
-    const double d = (double)(qk[r] - qi[r]);
-    sdx[r] = fma(wx, d, sdx[r]);
-    sdy[r] = fma(wy, d, sdy[r]);
+    const float d = qk[r] - qi[r];
+    sdx[r] = fmaf(fwx, d, sdx[r]);
+    sdy[r] = fmaf(fwy, d, sdy[r]);
   }
 }
 
 // The 2x2 solve and the stores of point i's dq and qmax/qmin.
 __device__ __forceinline__ void qdm_finish(double sum_delx_sqr, double sum_dely_sqr,
-                                           double sum_delx_dely, const double sdx[4],
-                                           const double sdy[4], const float qmax[4],
+                                           double sum_delx_dely, const float sdx[4],
+                                           const float sdy[4], const float qmax[4],
                                            const float qmin[4], float4 *dq_i, float4 *qm_i) {
   const double det = fma(sum_delx_sqr, sum_dely_sqr, -(sum_delx_dely * sum_delx_dely));
   // det > 0 for every validated point; 0 only for a batch's padding points,
@@ -160,8 +223,9 @@ __device__ __forceinline__ void qdm_finish(double sum_delx_sqr, double sum_dely_
   const double one_by_det = det != 0.0 ? 1.0 / det : 0.0;
   float dqx[4], dqy[4];
   for (int r = 0; r < 4; r++) {
-    dqx[r] = (float)(fma(sdx[r], sum_dely_sqr, -(sdy[r] * sum_delx_dely)) * one_by_det);
-    dqy[r] = (float)(fma(sdy[r], sum_delx_sqr, -(sdx[r] * sum_delx_dely)) * one_by_det);
+    const double sx = sdx[r], sy = sdy[r];
+    dqx[r] = (float)(fma(sx, sum_dely_sqr, -(sy * sum_delx_dely)) * one_by_det);
+    dqy[r] = (float)(fma(sy, sum_delx_sqr, -(sx * sum_delx_dely)) * one_by_det);
   }
   dq_i[0] = make_float4(dqx[0], dqx[1], dqx[2], dqx[3]);
   dq_i[1] = make_float4(dqy[0], dqy[1], dqy[2], dqy[3]);
@@ -197,7 +261,7 @@ __global__ __launch_bounds__(256) void k_q_derivatives_m(
   float qi[4], qmax[4], qmin[4];
   unpack4(gq[li], qi);
   double sum_delx_sqr = 0., sum_dely_sqr = 0., sum_delx_dely = 0.;
-  double sdx[4] = {0.0, 0.0, 0.0, 0.0}, sdy[4] = {0.0, 0.0, 0.0, 0.0};
+  float sdx[4] = {0.0f, 0.0f, 0.0f, 0.0f}, sdy[4] = {0.0f, 0.0f, 0.0f, 0.0f};
   for (int r = 0; r < 4; r++) qmax[r] = qmin[r] = qi[r];
   // neighbour e + 1's record and e + 2's conn entry load while e is used
   const Slice L = slice_of(slice_off, conn, conn16, b);
@@ -228,8 +292,8 @@ __global__ __launch_bounds__(256) void k_q_derivatives_m(
 }
 
 // ------------------------------------------------------------ inner passes
-// q_variables_mod.h:120-228, one Jacobi pass (src -> dst), FP32 gathers and
-// corrected differences, FP64 sums and solve.
+// q_variables_mod.h:120-228, one Jacobi pass (src -> dst), FP32 gathers,
+// corrected differences and right-hand sums, FP64 geometry sums and solve.
 // One neighbour's terms, every fused product written out (the kernels that
 // inline this then agree to the last bit).
 template <bool POW>
@@ -237,7 +301,7 @@ __device__ __forceinline__ void qim_term(double2 pi, const float qi[4], const fl
                                          const float dyi[4], double2 pk, const float qk[4],
                                          const float dxk[4], const float dyk[4], double power,
                                          double &sum_delx_sqr, double &sum_dely_sqr,
-                                         double &sum_delx_dely, double sx[4], double sy[4]) {
+                                         double &sum_delx_dely, float sx[4], float sy[4]) {
   const double delx = pk.x - pi.x, dely = pk.y - pi.y;
   const float fx = (float)delx, fy = (float)dely;
   const double weights = POW ? pow(sqrt(fma(delx, delx, dely * dely)), power) : 1.0;
@@ -245,25 +309,27 @@ __device__ __forceinline__ void qim_term(double2 pi, const float qi[4], const fl
   sum_delx_sqr = fma(wx, delx, sum_delx_sqr);
   sum_dely_sqr = fma(wy, dely, sum_dely_sqr);
   sum_delx_dely = fma(wx, dely, sum_delx_dely);
+  const float fwx = POW ? (float)wx : fx, fwy = POW ? (float)wy : fy;
   for (int r = 0; r < 4; r++) {
     const float t1 = fmaf(fx, dxk[r] - dxi[r], fy * (dyk[r] - dyi[r]));
     const float d = fmaf(-0.5f, t1, qk[r] - qi[r]);
-    sx[r] = fma(wx, (double)d, sx[r]);
-    sy[r] = fma(wy, (double)d, sy[r]);
+    sx[r] = fmaf(fwx, d, sx[r]);
+    sy[r] = fmaf(fwy, d, sy[r]);
   }
 }
 
 __device__ __forceinline__ void qim_finish(double sum_delx_sqr, double sum_dely_sqr,
-                                           double sum_delx_dely, const double sx[4],
-                                           const double sy[4], float4 *d_i) {
+                                           double sum_delx_dely, const float sx[4],
+                                           const float sy[4], float4 *d_i) {
   const double det = fma(sum_delx_sqr, sum_dely_sqr, -(sum_delx_dely * sum_delx_dely));
   // det > 0 for every validated point; 0 only for a batch's padding points,
   // which have no neighbours and must not produce NaNs
   const double one_by_det = det != 0.0 ? 1.0 / det : 0.0;
   float ox[4], oy[4];
   for (int r = 0; r < 4; r++) {
-    ox[r] = (float)(one_by_det * fma(sx[r], sum_dely_sqr, -(sy[r] * sum_delx_dely)));
-    oy[r] = (float)(one_by_det * fma(sy[r], sum_delx_sqr, -(sx[r] * sum_delx_dely)));
+    const double a = sx[r], c = sy[r];
+    ox[r] = (float)(one_by_det * fma(a, sum_dely_sqr, -(c * sum_delx_dely)));
+    oy[r] = (float)(one_by_det * fma(c, sum_delx_sqr, -(a * sum_delx_dely)));
   }
   d_i[0] = make_float4(ox[0], ox[1], ox[2], ox[3]);
   d_i[1] = make_float4(oy[0], oy[1], oy[2], oy[3]);
@@ -301,7 +367,7 @@ __global__ __launch_bounds__(256) void k_q_inner_m(
   unpack4(gd[2 * li], dxi);
   unpack4(gd[2 * li + 1], dyi);
   double sum_delx_sqr = 0., sum_dely_sqr = 0., sum_delx_dely = 0.;
-  double sx[4] = {0.0, 0.0, 0.0, 0.0}, sy[4] = {0.0, 0.0, 0.0, 0.0};
+  float sx[4] = {0.0f, 0.0f, 0.0f, 0.0f}, sy[4] = {0.0f, 0.0f, 0.0f, 0.0f};
   // neighbour e + 1's record loads while e is used (conn entries further ahead)
   const Slice L = slice_of(slice_off, conn, conn16, b);
   const int e0 = 0, last = off[i + 1] - off[i] - 1;
@@ -486,11 +552,8 @@ __global__ __launch_bounds__(256) void k_flux_m(
       qti[r] = qi[r] - 0.5f * phi_i * di;
       qtk[r] = qk[r] - 0.5f * phi_k * dk;
     }
-    float u1, u2, rho, sb, temp;
-    qtilde_state(qti, u1, u2, rho, sb, temp);
-    const EndTerms<float> ei = end_terms_sb(nxf, nyf, u1, u2, rho, sb, temp);
-    qtilde_state(qtk, u1, u2, rho, sb, temp);
-    const EndTerms<float> ek = end_terms_sb(nxf, nyf, u1, u2, rho, sb, temp);
+    const EndTerms<float> ei = qtilde_end_terms(qti, nxf, nyf);
+    const EndTerms<float> ek = qtilde_end_terms(qtk, nxf, nyf);
     float Gi[4], Gk[4];
     const bool xp = (m & XPOS) != 0;
     const float sx = xp ? 1.0f : -1.0f;
@@ -583,6 +646,7 @@ int flux_tile_cap_m(int block, int tile_max, size_t lds_limit, size_t lds_per_cu
 unsigned prepare_tiled_m(unsigned tiles, int tile_max, int flux_cap, size_t lds_limit) {
   struct K { unsigned bit; const void *f; size_t bytes; };
   const K ks[] = {
+      {TILE_DELTA, (const void *)k_func_delta_m<true>, lds_bytes_delta(tile_max)},
       {TILE_QDERIV, (const void *)k_q_derivatives_m<true, false>, lds_bytes_qderiv_m(tile_max)},
       {TILE_QDERIV, (const void *)k_q_derivatives_m<true, true>, lds_bytes_qderiv_m(tile_max)},
       {TILE_QINNER, (const void *)k_q_inner_m<true, false>, lds_bytes_qinner_m(tile_max)},
@@ -605,6 +669,18 @@ unsigned prepare_tiled_m(unsigned tiles, int tile_max, int flux_cap, size_t lds_
 void launch_q_variables_m(const DevCloud &c, const double *prim, float *q32, hipStream_t s) {
   hipLaunchKernelGGL(k_q_variables_m, dim3(blocks_for(c.n, 256)), dim3(256), 0, s, c.n, prim,
                      reinterpret_cast<float4 *>(q32));
+}
+
+void launch_func_delta_m(const DevCloud &c, const double *base, double cfl, double *delta,
+                         hipStream_t s) {
+  dim3 g(blocks_for(c.n, c.tile_block)), b(c.tile_block);
+  if (c.tiles & TILE_DELTA)
+    hipLaunchKernelGGL(k_func_delta_m<true>, g, b, lds_bytes_delta(c.tile_max), s, c.n,
+                       c.conn_off, c.slice_off, c.connS, c.conn16S, c.tile_off, c.tile_idx, base,
+                       (float)cfl, delta);
+  else
+    hipLaunchKernelGGL(k_func_delta_m<false>, g, b, 0, s, c.n, c.conn_off, c.slice_off, c.connS,
+                       c.conn16S, c.tile_off, c.tile_idx, base, (float)cfl, delta);
 }
 
 void launch_q_derivatives_m(const DevCloud &c, const MixedState &m, float *dq, double power,

@@ -286,29 +286,29 @@ __device__ __forceinline__ EndTerms<T> end_terms_sb(T nx, T ny, T u1, T u2, T rh
   return e;
 }
 
-// q~ -> primitive and the end terms in one, for the FP64 flux: the numbers
-// of qtilde_state followed by end_terms_sb, up to rounding, in fewer
-// operations -- |u|^2 is the u1^2 + u2^2 that q~ -> primitive forms (the
-// frame's ut^2 + un^2 in end_terms_sb; a rotation leaves it unchanged), and
-// h = sqrt(beta) (p/rho) / sqrt(pi) = y / (2 sqrt(pi)) with y = beta^-1/2,
-// since beta y^2 = 1.
-__device__ __forceinline__ EndTerms<double> qtilde_end_terms(const double qt[4], double nx,
-                                                             double ny) {
-  const double beta = -qt[3] * 0.5;
-  const double y = rsqrt_nr(beta);
-  const double y2 = y * y;
-  const double temp = 0.5 * y2;
-  const double u1 = qt[1] * temp, u2 = qt[2] * temp;
-  const double usq = u1 * u1 + u2 * u2;
-  const double sb = beta * y;
-  EndTerms<double> e;
+// q~ -> primitive and the end terms in one, for the FP64 and FP32 fluxes:
+// the numbers of qtilde_state followed by end_terms_sb, up to rounding, in
+// fewer operations -- |u|^2 is the u1^2 + u2^2 that q~ -> primitive forms
+// (the frame's ut^2 + un^2 in end_terms_sb; a rotation leaves it
+// unchanged), and h = sqrt(beta) (p/rho) / sqrt(pi) = y / (2 sqrt(pi)) with
+// y = beta^-1/2, since beta y^2 = 1.
+template <typename T>
+__device__ __forceinline__ EndTerms<T> qtilde_end_terms(const T qt[4], T nx, T ny) {
+  const T beta = -qt[3] * T(0.5);
+  const T y = rsqrt_nr(beta);
+  const T y2 = y * y;
+  const T temp = T(0.5) * y2;
+  const T u1 = qt[1] * temp, u2 = qt[2] * temp;
+  const T usq = u1 * u1 + u2 * u2;
+  const T sb = beta * y;
+  EndTerms<T> e;
   e.rho = exp_k(qt[0] + beta * usq) * (y2 * y2 * y);
-  const double tx = ny, ty = -nx;
+  const T tx = ny, ty = -nx;
   e.ut = u1 * tx + u2 * ty;
   e.un = u1 * nx + u2 * ny;
-  const double S1 = e.ut * sb, S2 = e.un * sb;
-  const double h = (0.5 * kInvSqrtPi) * y;
-  const double X1 = exp_neg_sq(-S1 * S1), X2 = exp_neg_sq(-S2 * S2);
+  const T S1 = e.ut * sb, S2 = e.un * sb;
+  const T h = T(0.5 * kInvSqrtPi) * y;
+  const T X1 = exp_neg_sq(-S1 * S1), X2 = exp_neg_sq(-S2 * S2);
   e.B1 = X1 * h;
   e.B2 = X2 * h;
   e.e1 = erf_x(S1, X1);

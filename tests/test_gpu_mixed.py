@@ -2,11 +2,13 @@
 C ABI.
 
 The mixed form gathers neighbour data in FP32 and evaluates the limiter,
-q~ -> primitive and the fluxes in FP32, keeping FP64 for coordinates,
-least-squares solves, sums, the state update and the reductions.  Its bars
+q~ -> primitive, the fluxes and each neighbour's local-step term in FP32,
+keeping FP64 for coordinates, least-squares solves, sums, the state update
+and the reductions.  Its bars
 are therefore set by FP32 rounding (unit roundoff 6e-8) and by how the
 stages amplify it, not by the FP64 parity bar:
   - q: one rounding of the FP64 value                          <= 1e-7
+  - delta: FP32 offset, sqrtf and reciprocal per neighbour     <= 1e-6
   - dq: differences of neighbouring FP32 q, ~1e-3 apart in
     relative terms, so ~6e-8/1e-3 relative per term            <= 2e-5
   - flux residual: the same cancellation in G_k - G_i          <= 5e-5
@@ -60,7 +62,9 @@ def test_mixed_stages_against_oracle():
     s.snapshot()
     s.func_delta()
     o.func_delta()
-    assert normwise(s.delta, o.get("delta")) <= 1e-12  # func_delta stays FP64
+    # each neighbour's step cfl |x_k - x_i| / speed_k in FP32: a rounded
+    # offset, sqrtf and a hardware reciprocal, a few FP32 ulps in all
+    assert normwise(s.delta, o.get("delta")) <= 1e-6
     for rk in range(1, 5):
         s.eval_q_variables()
         o.eval_q_variables()
