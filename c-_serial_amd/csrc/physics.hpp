@@ -74,13 +74,14 @@ __device__ __forceinline__ float exp_k(float a) {
 // FMAs against ln 2 split into a double and its remainder), e^r from
 // p(r) = 1 + r + c2 r^2 + ... + c10 r^10, a fit in relative error (3.3e-16;
 // exp_fit.py beside this file derives it), then scaled by 2^n: within two
-// or three ulps.  n is capped at 1100, which keeps its conversion to int
-// defined and, for a past 762, leaves r large: every coefficient is
-// positive, so p(r) and then 2^n p overflow to inf, as the library's exp
-// does past 709.78 -- +inf included, where an uncapped n would make
-// n ln2 - a = inf - inf.  A NaN a gives a NaN r and result.  The callers
-// first raise a to -708, which keeps 2^n normal in place of the library's
-// underflow tests (e^-708 = 3e-308 stands for 0).
+// or three ulps.  n comes from the shifter 1.5 * 2^52: nb = a / ln2 +
+// 1.5 * 2^52, for |a / ln2| < 2^51, lies in [2^52, 2^53) and is therefore
+// rounded to an integer, n = nb - 1.5 * 2^52 exactly, and the low 32 bits
+// of nb are n itself (2^51 = 0 mod 2^32) -- n and its integer without
+// rint or a conversion.  n may differ from rint(a * (1/ln2)) by one at a
+// tie of the rounded product, which leaves |r| at ln2/2 to within
+// rounding.  The callers first raise a to -708, which keeps 2^n normal in
+// place of the library's underflow tests (e^-708 = 3e-308 stands for 0).
 // e^r on |r| <= ln2/2 (the fit above)
 __device__ __forceinline__ double exp_poly(double r) {
   double p = 2.743938836864164e-07;
@@ -97,26 +98,28 @@ __device__ __forceinline__ double exp_poly(double r) {
   return p;
 }
 
-__device__ __forceinline__ double exp_reduced(double a) {
-  const double n = fmin(rint(a * 1.4426950408889634), 1100.0);
-  double r = fma(-n, 0.6931471805599453, a);
-  r = fma(-n, 2.3190468138462996e-17, r);
-  return ldexp(exp_poly(r), (int)n);
-}
+constexpr double kExpShift = 6755399441055744.0;  // 1.5 * 2^52
 
-// The same for a in [-708, 0] (or NaN), where no cap is needed: n =
-// rint(a / ln2) by the shifter 1.5 * 2^52, whose sum with a / ln2 lies in
-// [2^52, 2^53) and so is rounded to an integer, whose low 32 bits are n
-// itself (2^51 = 0 mod 2^32).  Two instructions fewer than rint, the cap
-// and the conversion.  n may differ from rint(a * (1/ln2)) by one at a
-// tie of the rounded product, which leaves |r| at ln2/2 to within rounding.
-__device__ __forceinline__ double exp_reduced_nonpos(double a) {
-  constexpr double kShift = 6755399441055744.0;  // 1.5 * 2^52
-  const double nb = fma(a, 1.4426950408889634, kShift);
-  const double n = nb - kShift;
+// e^a from nb = 1.5 * 2^52 + n
+__device__ __forceinline__ double exp_shifted(double a, double nb) {
+  const double n = nb - kExpShift;
   double r = fma(-n, 0.6931471805599453, a);
   r = fma(-n, 2.3190468138462996e-17, r);
   return ldexp(exp_poly(r), (int)(uint32_t)__double_as_longlong(nb));
+}
+
+// Any a >= -708.  nb is capped at n = 1100 (the fmin, which also takes a
+// NaN nb there): for a past 762 the capped n leaves r large, and every
+// coefficient being positive, p(r) and then 2^n p overflow to inf, as the
+// library's exp does past 709.78 -- for huge finite a and for +inf
+// (r = inf - 1100 ln2 = inf) too.  A NaN a gives a NaN r and result.
+__device__ __forceinline__ double exp_reduced(double a) {
+  return exp_shifted(a, fmin(fma(a, 1.4426950408889634, kExpShift), kExpShift + 1100.0));
+}
+
+// The same for a in [-708, 0] (or NaN), where nb needs no cap.
+__device__ __forceinline__ double exp_reduced_nonpos(double a) {
+  return exp_shifted(a, fma(a, 1.4426950408889634, kExpShift));
 }
 
 // The clamp compares, so a NaN a stays NaN.

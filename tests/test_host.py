@@ -431,8 +431,8 @@ def test_exp_reduction_constants():
     # 3.13, so the FMAs are emulated exactly with fractions) with the
     # constants read from the source and compared with the C library's exp:
     # within 4 ulps relative on [-700, 700] (the fit's own error is 3.3e-16),
-    # the clamps of its two entry points at -708, and the kinetic terms'
-    # variant for a <= 0, whose n comes from the 1.5 * 2^52 shifter
+    # the clamps of its two entry points at -708, the cap of n at 1100, and
+    # n from the 1.5 * 2^52 shifter in both variants
     import math
     import re
     import struct
@@ -444,20 +444,27 @@ def test_exp_reduction_constants():
         return b[:b.index("\n}\n")]
 
     poly = fn("__device__ __forceinline__ double exp_poly(double r)")
+    shifted = fn("__device__ __forceinline__ double exp_shifted(double a, double nb)")
     body = fn("__device__ __forceinline__ double exp_reduced(double a)")
     nonpos = fn("__device__ __forceinline__ double exp_reduced_nonpos(double a)")
     c = [float(x) for x in re.findall(r"p = fma\(p, r, ([0-9.e+-]+)\)", poly)]
     c0 = float(re.search(r"double p = ([0-9.e+-]+);", poly).group(1))
-    inv_ln2 = float(re.search(r"rint\(a \* ([0-9.e+-]+)\)", body).group(1))
-    hi, lo = (float(x) for x in re.findall(r"r = fma\(-n, ([0-9.e+-]+), [ar]\)", body))
+    hi, lo = (float(x) for x in re.findall(r"r = fma\(-n, ([0-9.e+-]+), [ar]\)", shifted))
     assert len(c) == 10 and c[-2:] == [1.0, 1.0]
-    assert abs(hi + lo - math.log(2)) < 1e-30 + 2 ** -60 and abs(inv_ln2 * math.log(2) - 1) < 2e-16
-    # the variant reduces with the same constants
-    assert re.findall(r"r = fma\(-n, ([0-9.e+-]+), [ar]\)", nonpos) == \
-        re.findall(r"r = fma\(-n, ([0-9.e+-]+), [ar]\)", body)
-    shift = float(re.search(r"kShift = ([0-9.]+);", nonpos).group(1))
+    assert abs(hi + lo - math.log(2)) < 1e-30 + 2 ** -60
+    shift = float(re.search(r"constexpr double kExpShift = ([0-9.]+);", src).group(1))
     assert shift == 1.5 * 2.0 ** 52
-    assert re.search(r"fma\(a, " + re.escape(repr(inv_ln2)) + r", kShift\)", nonpos)
+    assert re.search(r"const double n = nb - kExpShift;", shifted)
+    assert re.search(r"ldexp\(exp_poly\(r\), \(int\)\(uint32_t\)__double_as_longlong\(nb\)\)",
+                     shifted)
+    # both variants shift a / ln2 by the same constant; exp_reduced caps nb
+    pat = r"fma\(a, ([0-9.e+-]+), kExpShift\)"
+    inv_ln2 = float(re.search(pat, body).group(1))
+    assert float(re.search(pat, nonpos).group(1)) == inv_ln2
+    assert abs(inv_ln2 * math.log(2) - 1) < 2e-16
+    assert re.search(r"exp_shifted\(a, fmin\(fma\(a, [0-9.e+-]+, kExpShift\), "
+                     r"kExpShift \+ 1100\.0\)\)", body)
+    assert re.search(r"exp_shifted\(a, fma\(a, [0-9.e+-]+, kExpShift\)\)", nonpos)
 
     def fma(a, b, d):
         if not all(math.isfinite(x) for x in (a, b, d)):
@@ -478,17 +485,21 @@ def test_exp_reduction_constants():
         except OverflowError:  # the device's ldexp overflows to inf
             return math.inf
 
-    def exp_reduced(a):
-        n = min(float(round(a * inv_ln2)), 1100.0)  # round half to even, as rint
-        return tail(n, a, int(n))
-
-    def exp_reduced_nonpos(a):
-        nb = fma(a, inv_ln2, shift)
+    def shifted(a, nb):
         n = nb - shift
         low = struct.unpack("<q", struct.pack("<d", nb))[0] & 0xFFFFFFFF
         n_int = low - (1 << 32) if low >= 1 << 31 else low
         assert n_int == n  # the shifter's low word is n itself
         return tail(n, a, n_int)
+
+    def fmin(x, y):  # IEEE minNum, as the device's: a NaN operand gives the other
+        return y if math.isnan(x) else min(x, y)
+
+    def exp_reduced(a):
+        return shifted(a, fmin(fma(a, inv_ln2, shift), shift + 1100.0))
+
+    def exp_reduced_nonpos(a):
+        return shifted(a, fma(a, inv_ln2, shift))
 
     def exp_k(a):
         return exp_reduced(-708.0 if a < -708.0 else a)
@@ -503,10 +514,13 @@ def test_exp_reduction_constants():
         if a <= 0.0:
             worst = max(worst, abs(exp_neg_sq(a) / math.exp(a) - 1.0))
     assert worst <= 4 * 2.0 ** -53, worst
-    # ties of a / ln2 at half-integers, where the two roundings of n may differ
-    for k in range(-1020, 1, 7):
+    # ties of a / ln2 at half-integers, where the shifter's n may differ
+    # from rint's by one
+    for k in range(-1020, 1020, 7):
         a = (k + 0.5) * math.log(2)
-        assert abs(exp_neg_sq(a) / math.exp(a) - 1.0) <= 4 * 2.0 ** -53
+        assert abs(exp_k(a) / math.exp(a) - 1.0) <= 4 * 2.0 ** -53
+        if a <= 0.0:
+            assert abs(exp_neg_sq(a) / math.exp(a) - 1.0) <= 4 * 2.0 ** -53
     assert exp_k(-800.0) == exp_k(-708.0) > 0.0
     assert exp_neg_sq(-800.0) == exp_neg_sq(-708.0) > 0.0
     assert exp_neg_sq(0.0) == 1.0
@@ -516,19 +530,18 @@ def test_exp_reduction_constants():
     assert re.search(r"return exp_reduced\(a < -708\.0 \? -708\.0 : a\);", ek)
     en = fn("__device__ __forceinline__ double exp_neg_sq(double a)")
     assert re.search(r"return exp_reduced_nonpos\(fmax\(a, -708\.0\)\);", en)
-    # the upper end: n is capped (keeping its conversion to int defined) and
-    # the reduction uses the capped n, so past a = 762 r grows with a;
-    # every coefficient being positive, p(r) then grows without bound and
-    # 2^n p overflows: finite up to the largest double's logarithm, inf past
-    # it, for huge finite a and for +inf (where r = inf - 1100 ln2 = inf,
-    # not the inf - inf of an uncapped n); a NaN a makes r, p and the result
-    # NaN
-    assert re.search(r"const double n = fmin\(rint\(a \* [0-9.e+-]+\), 1100\.0\);", body)
-    assert re.search(r"ldexp\(exp_poly\(r\), \(int\)n\)", body)
+    # the upper end: nb is capped at n = 1100 (keeping n's low word n
+    # itself) and the reduction uses the capped n, so past a = 762 r grows
+    # with a; every coefficient being positive, p(r) then grows without
+    # bound and 2^n p overflows: finite up to the largest double's
+    # logarithm, inf past it, for huge finite a and for +inf (where
+    # r = inf - 1100 ln2 = inf, not the inf - inf of an uncapped n); a NaN
+    # a makes r, p and the result NaN
     assert c0 > 0 and all(k > 0 for k in c)
     assert abs(exp_k(709.78) / math.exp(709.78) - 1.0) <= 4 * 2.0 ** -53
-    for a in (709.79, 710.0, 762.0, 763.0, 1e4, 1e300):
+    for a in (709.79, 710.0, 762.0, 763.0, 1e4, 1e15, 1e16, 1e300, math.inf):
         assert exp_k(a) == math.inf, a
+    assert math.isnan(exp_k(math.nan))
 
 
 def test_reciprocal_and_rsqrt_refinements():
