@@ -53,7 +53,7 @@ void release(lskum_ctx *c) {
                   c->tile_idx, c->prim, c->prim_old, c->flux_res, c->delta, c->nx, c->ny,
                   c->flag, c->conn_off, c->perm_d, c->conn, c->part.sum, c->part.max,
                   c->part.arg, c->red, c->hist, c->f_m, c->f_l, c->f_r, c->f_s, c->forces,
-                  c->xy, c->q32, c->dq32[0], c->dq32[1], c->qm32, c->eps32, c->fcs,
+                  c->xy, c->q32, c->dq32[0], c->dq32[1], c->qm32, c->eps32, c->ls32, c->fcs,
                   c->block_case, c->seg, c->soff, c->qdyn, c->counter, c->qsum,
                   c->slice_off, c->connS, c->conn16S};
   for (void *p : ptrs)
@@ -364,6 +364,33 @@ void compute_coef(const CloudLayout &L, const double *x, const double *y, const 
   }
 }
 
+// The mixed form's least-squares coefficients of the full stencil (geometry
+// only): the sums of q_variables_mod.h:66-81 over the neighbours in conn
+// order, in FP64, then (syy, -sxy, sxx)/det rounded to FP32 (padding: 0).
+void compute_ls_coef(const CloudLayout &L, const double *x, const double *y, double power,
+                     std::vector<float> &ls) {
+  const int32_t n = L.n;
+  ls.assign((size_t)4 * n, 0.0f);
+  for (int32_t d = 0; d < n; d++) {
+    const int32_t v = L.perm[d];
+    double sxx = 0.0, syy = 0.0, sxy = 0.0;
+    for (int32_t t = L.conn_off_h[v]; t < L.conn_off_h[v + 1]; t++) {
+      const int32_t k = L.conn_h[t];
+      const double delx = x[k] - x[v], dely = y[k] - y[v];
+      const double w =
+          power != 0.0 ? std::pow(std::sqrt(delx * delx + dely * dely), power) : 1.0;
+      sxx = sxx + delx * delx * w;
+      syy = syy + dely * dely * w;
+      sxy = sxy + delx * dely * w;
+    }
+    const double det = sxx * syy - sxy * sxy;
+    const double inv = det != 0.0 ? 1.0 / det : 0.0;
+    ls[(size_t)4 * d] = (float)(syy * inv);
+    ls[(size_t)4 * d + 1] = (float)(-sxy * inv);
+    ls[(size_t)4 * d + 2] = (float)(sxx * inv);
+  }
+}
+
 // Host set-up of one cloud: normals and sub-stencils in the caller's order,
 // stencil validation, device order, device-order images, least-squares
 // coefficients, force panels.
@@ -436,6 +463,7 @@ int build_layout(const lskum_cloud &cloud, const lskum_params &p, CloudLayout &L
   L.flag2_h.assign(cloud.flag_2, cloud.flag_2 + n);
   L.min_dist_h.assign(cloud.min_dist, cloud.min_dist + n);
   compute_coef(L, cloud.x, cloud.y, L.nx_h.data(), L.ny_h.data(), p.power, L.coef);
+  if (p.precision == LSKUM_MIXED) compute_ls_coef(L, cloud.x, cloud.y, p.power, L.ls);
   // force panels: the first shape_points points (compute_force_coeffs_mod.h:
   // 25-27 takes m = j)
   int shape_points = 0;
@@ -510,6 +538,7 @@ int create_impl(const lskum_cloud *clouds, int32_t n_clouds, const lskum_case *c
   c->n = n;
   std::vector<double> base_h((size_t)B_STRIDE * n, 0.0), nxd(n, 0.0), nyd(n, 1.0),
       coef_h((size_t)8 * n, 0.0), prim0((size_t)4 * n);
+  std::vector<float> ls_h(c->mixed ? (size_t)4 * n : 0, 0.0f);
   std::vector<uint8_t> flag_h(n, 1);
   std::vector<int32_t> off_h(n + 1, 0), perm_h(n);
   std::vector<uint32_t> conn_d((size_t)entries);
@@ -535,6 +564,9 @@ int create_impl(const lskum_cloud *clouds, int32_t n_clouds, const lskum_case *c
       perm_h[g] = L.perm[d];
       std::copy(L.coef.begin() + (size_t)8 * d, L.coef.begin() + (size_t)8 * (d + 1),
                 coef_h.begin() + (size_t)8 * g);
+      if (c->mixed)
+        std::copy(L.ls.begin() + (size_t)4 * d, L.ls.begin() + (size_t)4 * (d + 1),
+                  ls_h.begin() + (size_t)4 * g);
       const int32_t cnt = L.off[d + 1] - L.off[d];
       off_h[g + 1] = off_h[g] + cnt;
       for (int32_t t = 0; t < cnt; t++) {
@@ -730,6 +762,7 @@ int create_impl(const lskum_cloud *clouds, int32_t n_clouds, const lskum_case *c
     ALLOC(c->dq32[1], (size_t)8 * n);
     ALLOC(c->qm32, (size_t)8 * n);
     ALLOC(c->eps32, (size_t)n);
+    ALLOC(c->ls32, (size_t)4 * n);
   } else {
     ALLOC(c->dqb[0], (size_t)DQ_STRIDE * n);
     ALLOC(c->dqb[1], (size_t)DQ_STRIDE * n);
@@ -812,6 +845,7 @@ int create_impl(const lskum_cloud *clouds, int32_t n_clouds, const lskum_case *c
     eps_h.resize(n);
     for (int32_t d = 0; d < n; d++) eps_h[d] = (float)base_h[(size_t)B_STRIDE * d + B_EPS2];
     ups.push_back({c->eps32, eps_h.data(), eps_h.size() * sizeof(float)});
+    ups.push_back({c->ls32, ls_h.data(), ls_h.size() * sizeof(float)});
   }
   if (c->tiles) {
     ups.push_back({c->conn16, conn16_h.data(), conn16_h.size() * sizeof(uint16_t)});

@@ -24,6 +24,7 @@ struct CloudLayout {
   std::vector<double> min_dist_h;
   // device order
   std::vector<double> x, y, eps2, nx, ny, coef;
+  std::vector<float> ls;  // mixed form: [n][4] full-stencil solve coefficients
   std::vector<uint8_t> flag;
   std::vector<int32_t> off;
   std::vector<uint32_t> conn;               // local index | mask << kMaskShift
@@ -64,6 +65,7 @@ struct lskum_ctx {
   bool mixed = false;
   double *xy = nullptr;
   float *q32 = nullptr, *dq32[2] = {nullptr, nullptr}, *qm32 = nullptr, *eps32 = nullptr;
+  float *ls32 = nullptr;  // [n][4] the full stencil's (syy, -sxy, sxx, 0)/det
   double *prim = nullptr, *prim_old = nullptr;
   double *flux_res = nullptr, *delta = nullptr, *nx = nullptr, *ny = nullptr;
   double *coef = nullptr;
@@ -164,6 +166,7 @@ struct lskum_ctx {
     m.q32 = q32;
     m.qm32 = qm32;
     m.eps32 = eps32;
+    m.ls32 = ls32;
     return m;
   }
 };

@@ -166,6 +166,7 @@ struct MixedState {
   float *q32;        // [n][4]
   float *qm32;       // [n][8]
   const float *eps32;  // [n], (pow(K*min_dist,3))^2 rounded to FP32
+  const float *ls32;   // [n][4], the full stencil's (syy, -sxy, sxx)/det rounded
 };
 size_t lds_bytes_qderiv_m(int tile_max);
 size_t lds_bytes_qinner_m(int tile_max);

@@ -10,9 +10,10 @@
 // of the FP32 flux differences, the least-squares right-hand sides of
 // q_derivatives and the inner passes (sums of w dx dq over FP32 neighbour
 // data) and each neighbour's local-step term.  FP64 is kept for
-//  - the least-squares geometry sums, determinants and 2x2 solves (here in
-//    q_derivatives and the inner passes; the flux applies its per-point
-//    coefficients, solved in FP64 once on the host and rounded once),
+//  - the least-squares geometry: every kernel applies per-point
+//    coefficients of its 2x2 solves, formed in FP64 once on the host from
+//    the geometry sums and rounded once (the full stencil's for
+//    q_derivatives and the inner passes, the sub-stencils' for the flux),
 //  - the state, the SSPRK update and the residual and force reductions
 //    (kernels.hip, shared with the FP64 form).
 // Differences are formed directly where FP32 would otherwise cancel: the
@@ -182,29 +183,47 @@ __global__ __launch_bounds__(256) void k_func_delta_m(
 }
 
 // ------------------------------------------------------------ q derivatives
-// q_variables_mod.h:36-118: FP32 gathers of q, FP64 geometry sums and
-// solve, FP32 right-hand sums w dx (q_k - q_i); qmax/qmin over FP32 values
-// (rounding is monotone, so they are the rounded FP64 extrema).
+// q_variables_mod.h:36-118: FP32 gathers of q, FP32 right-hand sums
+// w dx (q_k - q_i) with the offsets formed in FP64, the solve from the
+// point's coefficients; qmax/qmin over FP32 values (rounding is monotone,
+// so they are the rounded FP64 extrema).
+// A neighbour's weighted offset (w dx, w dy) in FP32: the difference of the
+// FP64 coordinates, rounded once.
+template <bool POW>
+__device__ __forceinline__ void ls_offset(double2 pi, double2 pk, double power, float &fx,
+                                          float &fy) {
+  const double delx = pk.x - pi.x, dely = pk.y - pi.y;
+  if (POW) {
+    const double weights = pow(sqrt(fma(delx, delx, dely * dely)), power);
+    fx = (float)(weights * delx);
+    fy = (float)(weights * dely);
+  } else {
+    fx = (float)delx;
+    fy = (float)dely;
+  }
+}
+
+// The 2x2 solve of q_variables_mod.h:98-104 from the point's coefficients
+// c = (syy, -sxy, sxx) / det of its geometry sums.  A batch's padding points
+// have c = 0, so their derivatives are 0.
+__device__ __forceinline__ void ls_solve(float4 c, float sx, float sy, float &ox, float &oy) {
+  ox = fmaf(c.x, sx, c.y * sy);
+  oy = fmaf(c.z, sy, c.y * sx);
+}
+
 // One neighbour's terms of the mixed q_derivatives: qmax/qmin and the
-// right-hand sums in FP32, the geometry sums in FP64, every fused product
-// written out (the kernels that inline this then agree to the last bit).
+// right-hand sums, every fused product written out (the kernels that
+// inline this then agree to the last bit).
 template <bool POW>
 __device__ __forceinline__ void qdm_term(double2 pi, const float qi[4], double2 pk,
                                          const float qk[4], double power, float qmax[4],
-                                         float qmin[4], double &sum_delx_sqr,
-                                         double &sum_dely_sqr, double &sum_delx_dely,
-                                         float sdx[4], float sdy[4]) {
+                                         float qmin[4], float sdx[4], float sdy[4]) {
   for (int r = 0; r < 4; r++) {
     qmax[r] = fmaxf(qmax[r], qk[r]);  // the reference's strict comparisons for finite q
     qmin[r] = fminf(qmin[r], qk[r]);
   }
-  const double delx = pk.x - pi.x, dely = pk.y - pi.y;
-  const double weights = POW ? pow(sqrt(fma(delx, delx, dely * dely)), power) : 1.0;
-  const double wx = POW ? weights * delx : delx, wy = POW ? weights * dely : dely;
-  sum_delx_sqr = fma(wx, delx, sum_delx_sqr);
-  sum_dely_sqr = fma(wy, dely, sum_dely_sqr);
-  sum_delx_dely = fma(wx, dely, sum_delx_dely);
-  const float fwx = (float)wx, fwy = (float)wy;
+  float fwx, fwy;
+  ls_offset<POW>(pi, pk, power, fwx, fwy);
   for (int r = 0; r < 4; r++) {
     const float d = qk[r] - qi[r];
     sdx[r] = fmaf(fwx, d, sdx[r]);
@@ -213,20 +232,11 @@ __device__ __forceinline__ void qdm_term(double2 pi, const float qi[4], double2 
 }
 
 // The 2x2 solve and the stores of point i's dq and qmax/qmin.
-__device__ __forceinline__ void qdm_finish(double sum_delx_sqr, double sum_dely_sqr,
-                                           double sum_delx_dely, const float sdx[4],
-                                           const float sdy[4], const float qmax[4],
-                                           const float qmin[4], float4 *dq_i, float4 *qm_i) {
-  const double det = fma(sum_delx_sqr, sum_dely_sqr, -(sum_delx_dely * sum_delx_dely));
-  // det > 0 for every validated point; 0 only for a batch's padding points,
-  // which have no neighbours and must not produce NaNs
-  const double one_by_det = det != 0.0 ? 1.0 / det : 0.0;
+__device__ __forceinline__ void qdm_finish(float4 ls, const float sdx[4], const float sdy[4],
+                                           const float qmax[4], const float qmin[4],
+                                           float4 *dq_i, float4 *qm_i) {
   float dqx[4], dqy[4];
-  for (int r = 0; r < 4; r++) {
-    const double sx = sdx[r], sy = sdy[r];
-    dqx[r] = (float)(fma(sx, sum_dely_sqr, -(sy * sum_delx_dely)) * one_by_det);
-    dqy[r] = (float)(fma(sy, sum_delx_sqr, -(sx * sum_delx_dely)) * one_by_det);
-  }
+  for (int r = 0; r < 4; r++) ls_solve(ls, sdx[r], sdy[r], dqx[r], dqy[r]);
   dq_i[0] = make_float4(dqx[0], dqx[1], dqx[2], dqx[3]);
   dq_i[1] = make_float4(dqy[0], dqy[1], dqy[2], dqy[3]);
   qm_i[0] = make_float4(qmax[0], qmax[1], qmax[2], qmax[3]);
@@ -239,8 +249,8 @@ __global__ __launch_bounds__(256) void k_q_derivatives_m(
     const uint32_t *__restrict__ conn,
     const uint16_t *__restrict__ conn16, const int32_t *__restrict__ tile_off,
     const int32_t *__restrict__ tile_idx, int tile_max, const double2 *__restrict__ xy,
-    const float4 *__restrict__ q32, float4 *__restrict__ dq_out, float4 *__restrict__ qm_out,
-    double power) {
+    const float4 *__restrict__ q32, const float4 *__restrict__ ls, float4 *__restrict__ dq_out,
+    float4 *__restrict__ qm_out, double power) {
   extern __shared__ uint4 lds4[];
   double2 *lxy = reinterpret_cast<double2 *>(lds4);
   float4 *lq = reinterpret_cast<float4 *>(lds4 + tile_max);
@@ -260,7 +270,7 @@ __global__ __launch_bounds__(256) void k_q_derivatives_m(
   const double2 pi = gxy[li];
   float qi[4], qmax[4], qmin[4];
   unpack4(gq[li], qi);
-  double sum_delx_sqr = 0., sum_dely_sqr = 0., sum_delx_dely = 0.;
+  const float4 lsi = ls[i];
   float sdx[4] = {0.0f, 0.0f, 0.0f, 0.0f}, sdy[4] = {0.0f, 0.0f, 0.0f, 0.0f};
   for (int r = 0; r < 4; r++) qmax[r] = qmin[r] = qi[r];
   // neighbour e + 1's record and e + 2's conn entry load while e is used
@@ -282,34 +292,30 @@ __global__ __launch_bounds__(256) void k_q_derivatives_m(
     w_n = L.raw(min(e + 2, last), TILED);
     float qk[4];
     unpack4(qk4, qk);
-    qdm_term<POW>(pi, qi, pk, qk, power, qmax, qmin, sum_delx_sqr, sum_dely_sqr,
-                  sum_delx_dely, sdx, sdy);
+    qdm_term<POW>(pi, qi, pk, qk, power, qmax, qmin, sdx, sdy);
     pk = pk_n;
     qk4 = qk4_n;
   }
-  qdm_finish(sum_delx_sqr, sum_dely_sqr, sum_delx_dely, sdx, sdy, qmax, qmin,
-             dq_out + 2 * (size_t)i, qm_out + 2 * (size_t)i);
+  qdm_finish(lsi, sdx, sdy, qmax, qmin, dq_out + 2 * (size_t)i, qm_out + 2 * (size_t)i);
 }
 
 // ------------------------------------------------------------ inner passes
 // q_variables_mod.h:120-228, one Jacobi pass (src -> dst), FP32 gathers,
-// corrected differences and right-hand sums, FP64 geometry sums and solve.
+// corrected differences and right-hand sums, the solve from the point's
+// coefficients.
 // One neighbour's terms, every fused product written out (the kernels that
 // inline this then agree to the last bit).
 template <bool POW>
 __device__ __forceinline__ void qim_term(double2 pi, const float qi[4], const float dxi[4],
                                          const float dyi[4], double2 pk, const float qk[4],
                                          const float dxk[4], const float dyk[4], double power,
-                                         double &sum_delx_sqr, double &sum_dely_sqr,
-                                         double &sum_delx_dely, float sx[4], float sy[4]) {
-  const double delx = pk.x - pi.x, dely = pk.y - pi.y;
-  const float fx = (float)delx, fy = (float)dely;
-  const double weights = POW ? pow(sqrt(fma(delx, delx, dely * dely)), power) : 1.0;
-  const double wx = POW ? weights * delx : delx, wy = POW ? weights * dely : dely;
-  sum_delx_sqr = fma(wx, delx, sum_delx_sqr);
-  sum_dely_sqr = fma(wy, dely, sum_dely_sqr);
-  sum_delx_dely = fma(wx, dely, sum_delx_dely);
-  const float fwx = POW ? (float)wx : fx, fwy = POW ? (float)wy : fy;
+                                         float sx[4], float sy[4]) {
+  float fx, fy, fwx, fwy;
+  ls_offset<false>(pi, pk, 0.0, fx, fy);
+  if (POW)
+    ls_offset<true>(pi, pk, power, fwx, fwy);
+  else
+    fwx = fx, fwy = fy;
   for (int r = 0; r < 4; r++) {
     const float t1 = fmaf(fx, dxk[r] - dxi[r], fy * (dyk[r] - dyi[r]));
     const float d = fmaf(-0.5f, t1, qk[r] - qi[r]);
@@ -318,19 +324,10 @@ __device__ __forceinline__ void qim_term(double2 pi, const float qi[4], const fl
   }
 }
 
-__device__ __forceinline__ void qim_finish(double sum_delx_sqr, double sum_dely_sqr,
-                                           double sum_delx_dely, const float sx[4],
-                                           const float sy[4], float4 *d_i) {
-  const double det = fma(sum_delx_sqr, sum_dely_sqr, -(sum_delx_dely * sum_delx_dely));
-  // det > 0 for every validated point; 0 only for a batch's padding points,
-  // which have no neighbours and must not produce NaNs
-  const double one_by_det = det != 0.0 ? 1.0 / det : 0.0;
+__device__ __forceinline__ void qim_finish(float4 ls, const float sx[4], const float sy[4],
+                                           float4 *d_i) {
   float ox[4], oy[4];
-  for (int r = 0; r < 4; r++) {
-    const double a = sx[r], c = sy[r];
-    ox[r] = (float)(one_by_det * fma(a, sum_dely_sqr, -(c * sum_delx_dely)));
-    oy[r] = (float)(one_by_det * fma(c, sum_delx_sqr, -(a * sum_delx_dely)));
-  }
+  for (int r = 0; r < 4; r++) ls_solve(ls, sx[r], sy[r], ox[r], oy[r]);
   d_i[0] = make_float4(ox[0], ox[1], ox[2], ox[3]);
   d_i[1] = make_float4(oy[0], oy[1], oy[2], oy[3]);
 }
@@ -341,8 +338,8 @@ __global__ __launch_bounds__(256) void k_q_inner_m(
     const uint32_t *__restrict__ conn,
     const uint16_t *__restrict__ conn16, const int32_t *__restrict__ tile_off,
     const int32_t *__restrict__ tile_idx, int tile_max, const double2 *__restrict__ xy,
-    const float4 *__restrict__ q32, const float4 *__restrict__ src, float4 *__restrict__ dst,
-    double power) {
+    const float4 *__restrict__ q32, const float4 *__restrict__ ls,
+    const float4 *__restrict__ src, float4 *__restrict__ dst, double power) {
   extern __shared__ uint4 lds4[];
   double2 *lxy = reinterpret_cast<double2 *>(lds4);
   float4 *lq = reinterpret_cast<float4 *>(lds4 + tile_max);
@@ -366,7 +363,7 @@ __global__ __launch_bounds__(256) void k_q_inner_m(
   unpack4(gq[li], qi);
   unpack4(gd[2 * li], dxi);
   unpack4(gd[2 * li + 1], dyi);
-  double sum_delx_sqr = 0., sum_dely_sqr = 0., sum_delx_dely = 0.;
+  const float4 lsi = ls[i];
   float sx[4] = {0.0f, 0.0f, 0.0f, 0.0f}, sy[4] = {0.0f, 0.0f, 0.0f, 0.0f};
   // neighbour e + 1's record loads while e is used (conn entries further ahead)
   const Slice L = slice_of(slice_off, conn, conn16, b);
@@ -391,14 +388,13 @@ __global__ __launch_bounds__(256) void k_q_inner_m(
     unpack4(qk4, qk);
     unpack4(dxk4, dxk);
     unpack4(dyk4, dyk);
-    qim_term<POW>(pi, qi, dxi, dyi, pk, qk, dxk, dyk, power, sum_delx_sqr, sum_dely_sqr,
-                  sum_delx_dely, sx, sy);
+    qim_term<POW>(pi, qi, dxi, dyi, pk, qk, dxk, dyk, power, sx, sy);
     pk = pk_n;
     qk4 = qk4_n;
     dxk4 = dxk4_n;
     dyk4 = dyk4_n;
   }
-  qim_finish(sum_delx_sqr, sum_dely_sqr, sum_delx_dely, sx, sy, dst + 2 * (size_t)i);
+  qim_finish(lsi, sx, sy, dst + 2 * (size_t)i);
 }
 
 // ------------------------------------------------------------ flux residual
@@ -690,8 +686,8 @@ void launch_q_derivatives_m(const DevCloud &c, const MixedState &m, float *dq, d
   const size_t L = t ? lds_bytes_qderiv_m(c.tile_max) : 0;
 #define QD_ARGS                                                                           \
   c.n, TILE_ARGS_M, reinterpret_cast<const double2 *>(m.xy),                              \
-      reinterpret_cast<const float4 *>(m.q32), reinterpret_cast<float4 *>(dq),            \
-      reinterpret_cast<float4 *>(m.qm32), power
+      reinterpret_cast<const float4 *>(m.q32), reinterpret_cast<const float4 *>(m.ls32),   \
+      reinterpret_cast<float4 *>(dq), reinterpret_cast<float4 *>(m.qm32), power
   if (t && power == 0.0)
     hipLaunchKernelGGL((k_q_derivatives_m<true, false>), g, b, L, s, QD_ARGS);
   else if (t)
@@ -710,8 +706,8 @@ void launch_q_inner_m(const DevCloud &c, const MixedState &m, const float *src, 
   const size_t L = t ? lds_bytes_qinner_m(c.tile_max) : 0;
 #define QI_ARGS                                                                           \
   c.n, TILE_ARGS_M, reinterpret_cast<const double2 *>(m.xy),                              \
-      reinterpret_cast<const float4 *>(m.q32), reinterpret_cast<const float4 *>(src),     \
-      reinterpret_cast<float4 *>(dst), power
+      reinterpret_cast<const float4 *>(m.q32), reinterpret_cast<const float4 *>(m.ls32),   \
+      reinterpret_cast<const float4 *>(src), reinterpret_cast<float4 *>(dst), power
   if (t && power == 0.0)
     hipLaunchKernelGGL((k_q_inner_m<true, false>), g, b, L, s, QI_ARGS);
   else if (t)
