@@ -534,7 +534,11 @@ __global__ __launch_bounds__(256) void k_flux_vjp_gather(
 // ls_i = M_i^-T lo_i: the adjoint of the sums (sx, sy) a least-squares pass
 // solves, from the adjoint of its output (q_variables_mod.h:94-112 and
 // :176-210 solve with the same 2x2 matrix of the point's geometry, whose
-// sums lsum the recomputed q_derivatives stored).
+// sums lsum the recomputed q_derivatives stored).  The pairs (i, k) of i's
+// own list then give i its share t = dx.ls_i per component: -t to its q
+// adjoint (lq, accumulated) and, for an inner pass (with_dq), +0.5 dx t to
+// the adjoint of the derivatives the pass read (ldq_in, written); the
+// gather adds the shares of the pairs in which i is the neighbour.
 //
 // GEO adds, per pair e = (i, k) with rho = (delx, dely), the derivative of
 // sum_r lo_r . out_r with respect to rho (accumulated into geo_edge[e]):
@@ -553,7 +557,9 @@ __global__ __launch_bounds__(256) void k_ls_adj(int n, const int32_t *__restrict
                                                 double *__restrict__ ls,
                                                 const double *__restrict__ out,
                                                 const double *__restrict__ dq_in,
-                                                double *__restrict__ geo_edge) {
+                                                double *__restrict__ geo_edge,
+                                                double *__restrict__ lq, int with_dq,
+                                                double *__restrict__ ldq_in) {
   const int i = xcd_block() * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double x_i = base[(size_t)B_STRIDE * i + B_X], y_i = base[(size_t)B_STRIDE * i + B_Y];
@@ -571,7 +577,35 @@ __global__ __launch_bounds__(256) void k_ls_adj(int n, const int32_t *__restrict
   }
   store4(ls + 8 * (size_t)i, sx);
   store4(ls + 8 * (size_t)i + 4, sy);
-  if (GEO) {
+  double aq[4], adx[4] = {0, 0, 0, 0}, ady[4] = {0, 0, 0, 0};
+  load4(lq + 4 * (size_t)i, aq);
+  // one pair's share to i (the same t as the gather's)
+  auto own_share = [&](double delx, double dely) {
+    for (int r = 0; r < 4; r++) {
+      const double t = delx * sx[r] + dely * sy[r];
+      aq[r] -= t;
+      adx[r] += 0.5 * delx * t;
+      ady[r] += 0.5 * dely * t;
+    }
+  };
+  if (!GEO) {
+    // neighbour e + 1's coordinates and e + 2's conn entry load while e is used
+    auto xy_of = [&](uint32_t w) {
+      return reinterpret_cast<const double2 *>(base + (size_t)B_STRIDE * (w & kIndexMask))[0];
+    };
+    double2 pk = {0.0, 0.0};
+    uint32_t w_n = 0;
+    if (e0 <= last) {
+      pk = xy_of(conn[e0]);
+      w_n = conn[min(e0 + 1, last)];
+    }
+    for (int e = e0; e <= last; e++) {
+      const double2 pk_n = xy_of(w_n);
+      w_n = conn[min(e + 2, last)];
+      own_share(pk.x - x_i, pk.y - y_i);
+      pk = pk_n;
+    }
+  } else {
     double qi[4], px[4], py[4], hxi[4] = {0, 0, 0, 0}, hyi[4] = {0, 0, 0, 0};
     load4(base + (size_t)B_STRIDE * i + B_Q, qi);
     load8(out + (size_t)DQ_STRIDE * i, px, py);
@@ -612,55 +646,39 @@ __global__ __launch_bounds__(256) void k_ls_adj(int n, const int32_t *__restrict
       }
       geo_edge[2 * (size_t)e] += gx;
       geo_edge[2 * (size_t)e + 1] += gy;
+      own_share(delx, dely);
       pk = pk_n;
       for (int r = 0; r < 4; r++) qk[r] = qk_n[r];
       w_c = w_n;
       w_n = w_nn;
     }
   }
+  store4(lq + 4 * (size_t)i, aq);
+  if (with_dq) {
+    store4(ldq_in + 8 * (size_t)i, adx);
+    store4(ldq_in + 8 * (size_t)i + 4, ady);
+  }
 }
 
 // Per point j, from the sums' adjoints ls of every point: the adjoint of
 // q (added to lq) and, for an inner pass (with_dq), of the derivatives the
-// pass read (written to ldq_in).  Each pair (i, k) contributes
-// t = dx.ls_i per component to q_k and -t to q_i, and for an inner pass
-// -0.5 dx t to dq_k and +0.5 dx t to dq_i.  For q_derivatives (with_ext)
-// the adjoints of qmax/qmin are routed to the point each came from.
+// pass read (added to ldq_in).  Each pair (i, k) contributes t = dx.ls_i per
+// component to q_k and -t to q_i, and for an inner pass -0.5 dx t to dq_k
+// and +0.5 dx t to dq_i; k_ls_adj has added the shares of j's own pairs,
+// this kernel those of the pairs in which j is the neighbour.  For
+// q_derivatives (with_ext) the adjoints of qmax/qmin are routed to the
+// point each came from.
 __global__ __launch_bounds__(256) void k_ls_adj_gather(
-    int n, const int32_t *__restrict__ off, const uint32_t *__restrict__ conn,
-    const int32_t *__restrict__ tr_off, const int32_t *__restrict__ tr_owner,
+    int n, const int32_t *__restrict__ tr_off, const int32_t *__restrict__ tr_owner,
     const double2 *__restrict__ xy, const double *__restrict__ ls, int with_dq,
     double *__restrict__ ldq_in, const int32_t *__restrict__ arg,
     const double *__restrict__ lqm, double *__restrict__ lq) {
   const int j = xcd_block() * blockDim.x + threadIdx.x;
   if (j >= n) return;
   const double x_j = xy[j].x, y_j = xy[j].y;
-  double sx[4], sy[4], aq[4], adx[4] = {0, 0, 0, 0}, ady[4] = {0, 0, 0, 0};
-  load8(ls + 8 * (size_t)j, sx, sy);
+  double aq[4], adx[4] = {0, 0, 0, 0}, ady[4] = {0, 0, 0, 0};
   load4(lq + 4 * (size_t)j, aq);
-  // pairs (j, k) of j's own list; neighbour e + 1's coordinates and e + 2's
-  // conn entry load while e is used
-  const int e0 = off[j], last = off[j + 1] - 1;
-  auto xy_of = [&](uint32_t w) { return xy[w & kIndexMask]; };
-  double2 pk = {0.0, 0.0};
-  uint32_t w_n = 0;
-  if (e0 <= last) {
-    pk = xy_of(conn[e0]);
-    w_n = conn[min(e0 + 1, last)];
-  }
-  for (int e = e0; e <= last; e++) {
-    const double2 pk_n = xy_of(w_n);
-    w_n = conn[min(e + 2, last)];
-    const double delx = pk.x - x_j;
-    const double dely = pk.y - y_j;
-    pk = pk_n;
-    for (int r = 0; r < 4; r++) {
-      const double t = delx * sx[r] + dely * sy[r];
-      aq[r] -= t;
-      adx[r] += 0.5 * delx * t;
-      ady[r] += 0.5 * dely * t;
-    }
-  }
+  if (with_dq) load8(ldq_in + 8 * (size_t)j, adx, ady);
   // pairs (i, j) in which j is the neighbour; owner t + 1's coordinates and
   // sums' adjoints and t + 2's index load while t is used
   const int t0 = tr_off[j], tlast = tr_off[j + 1] - 1;
@@ -896,13 +914,15 @@ void launch_ls_vjp(const DevCloud &c, const AdjointBuffers &a, const double *bas
                    const double *dq_in, double *geo_edge, hipStream_t s) {
   if (geo_edge)
     hipLaunchKernelGGL(k_ls_adj<true>, dim3(blocks_of(c.n)), dim3(256), 0, s, c.n, c.conn_off,
-                       c.conn, base, a.lsum, lo, a.ls, out, dq_in, geo_edge);
+                       c.conn, base, a.lsum, lo, a.ls, out, dq_in, geo_edge, a.lq,
+                       extrema ? 0 : 1, ldq_in);
   else
     hipLaunchKernelGGL(k_ls_adj<false>, dim3(blocks_of(c.n)), dim3(256), 0, s, c.n, c.conn_off,
-                       c.conn, base, a.lsum, lo, a.ls, nullptr, nullptr, nullptr);
-  hipLaunchKernelGGL(k_ls_adj_gather, dim3(blocks_of(c.n)), dim3(256), 0, s, c.n, c.conn_off,
-                     c.conn, a.tr_off, a.tr_owner, reinterpret_cast<const double2 *>(c.xy), a.ls,
-                     extrema ? 0 : 1, ldq_in, extrema ? a.arg : nullptr, a.lqm, a.lq);
+                       c.conn, base, a.lsum, lo, a.ls, nullptr, nullptr, nullptr, a.lq,
+                       extrema ? 0 : 1, ldq_in);
+  hipLaunchKernelGGL(k_ls_adj_gather, dim3(blocks_of(c.n)), dim3(256), 0, s, c.n, a.tr_off,
+                     a.tr_owner, reinterpret_cast<const double2 *>(c.xy), a.ls, extrema ? 0 : 1,
+                     ldq_in, extrema ? a.arg : nullptr, a.lqm, a.lq);
 }
 
 void launch_qvar_vjp(const DevCloud &c, const AdjointBuffers &a, const double *prim,
