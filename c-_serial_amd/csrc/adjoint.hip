@@ -470,10 +470,12 @@ __global__ __launch_bounds__(256) void k_flux_vjp(
 }
 
 // Per point j: its own share plus the edge records of every pair in which
-// it is the neighbour (transpose CSR, owners ascending).
+// it is the neighbour (owners ascending), its r-th at slot
+// sbase[j / 256] + 256 r + j % 256.
 __global__ __launch_bounds__(256) void k_flux_vjp_gather(
-    int n, const int32_t *__restrict__ tr_off, const int32_t *__restrict__ tr_owner,
-    const double2 *__restrict__ xy, const double *__restrict__ edge,
+    int n, const int32_t *__restrict__ tr_off, const int32_t *__restrict__ sbase,
+    const int32_t *__restrict__ owner_s, const double2 *__restrict__ xy,
+    const double *__restrict__ edge,
     const uint8_t *__restrict__ edge_mx, const double *__restrict__ own_q,
     const double *__restrict__ own_dq, const double *__restrict__ own_qm,
     double *__restrict__ lq, double *__restrict__ ldq, double *__restrict__ lqm) {
@@ -484,30 +486,31 @@ __global__ __launch_bounds__(256) void k_flux_vjp_gather(
   load8(own_dq + 8 * (size_t)j, adx, ady);
   load8(own_qm + 8 * (size_t)j, amx, amn);
   const double x_j = xy[j].x, y_j = xy[j].y;
-  // record t + 1 with its owner's coordinates, and owner t + 2's index, load
-  // while record t is added
-  const int t0 = tr_off[j], tlast = tr_off[j + 1] - 1;
-  auto fetch = [&](int t, int i, double2 &p, double r12[EDGE_REC], uint32_t &mx) {
+  // record r + 1 with its owner's coordinates, and owner r + 2's index, load
+  // while record r is added
+  const int s0 = sbase[j >> 8] + (j & 255), rlast = tr_off[j + 1] - tr_off[j] - 1;
+  auto slot = [&](int r) { return (size_t)s0 + 256 * (size_t)r; };
+  auto fetch = [&](int r, int i, double2 &p, double r12[EDGE_REC], uint32_t &mx) {
     p = xy[i];
-    const double *rec = edge + EDGE_REC * (size_t)t;
+    const double *rec = edge + EDGE_REC * slot(r);
     load8(rec, r12, r12 + 4);
     load4(rec + 8, r12 + 8);
-    mx = edge_mx[t];
+    mx = edge_mx[slot(r)];
   };
   double2 po = {0.0, 0.0};
   double rc[EDGE_REC] = {};
   uint32_t mc = 0;
   int i_n = 0;
-  if (t0 <= tlast) {
-    fetch(t0, tr_owner[t0], po, rc, mc);
-    i_n = tr_owner[min(t0 + 1, tlast)];
+  if (rlast >= 0) {
+    fetch(0, owner_s[slot(0)], po, rc, mc);
+    i_n = owner_s[slot(min(1, rlast))];
   }
-  for (int t = t0; t <= tlast; t++) {
+  for (int t = 0; t <= rlast; t++) {
     double2 po_n;
     double rn[EDGE_REC];
     uint32_t mn;
-    fetch(min(t + 1, tlast), i_n, po_n, rn, mn);
-    i_n = tr_owner[min(t + 2, tlast)];
+    fetch(min(t + 1, rlast), i_n, po_n, rn, mn);
+    i_n = owner_s[slot(min(t + 2, rlast))];
     const double delx = x_j - po.x;
     const double dely = y_j - po.y;
     for (int r = 0; r < 4; r++) {
@@ -905,7 +908,8 @@ void launch_flux_vjp(const DevCloud &c, const AdjointBuffers &a, const double *b
   }
 #undef VJP_ARGS
   hipLaunchKernelGGL(k_flux_vjp_gather, dim3(blocks_of(c.n)), dim3(256), 0, s, c.n, a.tr_off,
-                     a.tr_owner, reinterpret_cast<const double2 *>(c.xy), a.edge, a.edge_mx, a.own_q,
+                     a.tr_sbase, a.tr_owner_s, reinterpret_cast<const double2 *>(c.xy), a.edge,
+                     a.edge_mx, a.own_q,
                      a.own_dq, a.own_qm, a.lq, a.ldq[0], a.lqm);
 }
 

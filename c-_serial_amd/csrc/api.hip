@@ -60,7 +60,8 @@ void release(lskum_ctx *c) {
     if (p) (void)hipFree(p);
   void *aptrs[] = {c->adj.own_q, c->adj.own_dq, c->adj.own_qm, c->adj.edge, c->adj.edge_mx, c->adj.lq,
                    c->adj.ldq[0], c->adj.ldq[1], c->adj.lqm, c->adj.ls, c->adj.arg,
-                   c->adj.tr_off, c->adj.tr_edge, c->adj.tr_owner, c->adj.tr_pos, c->adj.tbase,
+                   c->adj.tr_off, c->adj.tr_edge, c->adj.tr_owner, c->adj.tr_pos,
+                   c->adj.tr_sbase, c->adj.tr_owner_s, c->adj.tbase,
                    c->adj.tdq[0],
                    c->adj.tdq[1], c->adj.tqm, c->adj.tqs, c->adj.lsum, c->adj.vec_in,
                    c->adj.vec_out};
@@ -893,8 +894,29 @@ int prepare_adjoint(lskum_ctx *c) {
     for (int32_t e = L.off[i]; e < L.off[i + 1]; e++) {
       const int32_t j = (int32_t)(L.conn[e] & kIndexMask);
       tr_edge[pos[j]] = e;
-      tr_pos[e] = pos[j];
       tr_owner[pos[j]++] = i;
+    }
+  // record slots: per block of 256 points, its points' r-th records side by
+  // side, as many rows as its largest transposed degree
+  const int32_t nb = (n + 255) / 256;
+  std::vector<int32_t> sbase(nb);
+  int64_t slots = 0;
+  for (int32_t B = 0; B < nb; B++) {
+    int32_t dmax = 0;
+    for (int32_t j = 256 * B; j < std::min(n, 256 * B + 256); j++)
+      dmax = std::max(dmax, tr_off[j + 1] - tr_off[j]);
+    sbase[B] = (int32_t)slots;
+    slots += (int64_t)256 * dmax;
+    if (slots > (int64_t)INT32_MAX)
+      return set_err(c, LSKUM_EINVAL, "too many pairs for the adjoint's 31-bit record slots");
+  }
+  const size_t S = (size_t)slots;
+  std::vector<int32_t> owner_s(S, 0);
+  for (int32_t j = 0; j < n; j++)
+    for (int32_t t = tr_off[j]; t < tr_off[j + 1]; t++) {
+      const int32_t slot = sbase[j / 256] + 256 * (t - tr_off[j]) + j % 256;
+      tr_pos[tr_edge[t]] = slot;
+      owner_s[slot] = tr_owner[t];
     }
   AdjointBuffers &a = c->adj;
   hipError_t e;
@@ -904,8 +926,8 @@ int prepare_adjoint(lskum_ctx *c) {
   AALLOC(a.own_q, (size_t)4 * n);
   AALLOC(a.own_dq, (size_t)8 * n);
   AALLOC(a.own_qm, (size_t)8 * n);
-  AALLOC(a.edge, EDGE_REC * E);
-  AALLOC(a.edge_mx, E);
+  AALLOC(a.edge, EDGE_REC * S);
+  AALLOC(a.edge_mx, S);
   AALLOC(a.lq, (size_t)4 * n);
   AALLOC(a.ldq[0], (size_t)8 * n);
   AALLOC(a.ldq[1], (size_t)8 * n);
@@ -916,6 +938,8 @@ int prepare_adjoint(lskum_ctx *c) {
   AALLOC(a.tr_edge, E);
   AALLOC(a.tr_owner, E);
   AALLOC(a.tr_pos, E);
+  AALLOC(a.tr_sbase, (size_t)nb);
+  AALLOC(a.tr_owner_s, S);
   AALLOC(a.tbase, (size_t)B_STRIDE * n);
   AALLOC(a.tdq[0], (size_t)DQ_STRIDE * n);
   AALLOC(a.tdq[1], (size_t)DQ_STRIDE * n);
@@ -930,6 +954,9 @@ int prepare_adjoint(lskum_ctx *c) {
   HIP_TRY(c, hipMemcpy(a.tr_edge, tr_edge.data(), E * sizeof(int32_t), hipMemcpyHostToDevice));
   HIP_TRY(c, hipMemcpy(a.tr_owner, tr_owner.data(), E * sizeof(int32_t), hipMemcpyHostToDevice));
   HIP_TRY(c, hipMemcpy(a.tr_pos, tr_pos.data(), E * sizeof(int32_t), hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(a.tr_sbase, sbase.data(), nb * sizeof(int32_t), hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(a.tr_owner_s, owner_s.data(), S * sizeof(int32_t),
+                       hipMemcpyHostToDevice));
   int dev = 0, lds_limit = 0;
   a.vjp_tiled =
       hipGetDevice(&dev) == hipSuccess &&

@@ -192,8 +192,11 @@ void launch_kat_fluxes_m(int n, const double *in6, double *out, hipStream_t s);
 // ldq [n][8] (two, for the inner passes), lqm [n][8], ls [n][8] are
 // adjoints of q, dq, qmax/qmin and of the least-squares sums; arg [n][8]
 // the point each qmax/qmin came from; tr_* the transpose of conn (for each
-// point, the edges that name it, owners ascending; tr_pos[e] the position of
-// edge e in those lists, where its record in `edge` is kept); tbase, tdq, tqm the
+// point, the edges that name it, owners ascending); tr_pos[e] the slot of
+// edge e's record in `edge`: point j's r-th transposed edge is kept at
+// tr_sbase[j / 256] + 256 r + j % 256 (the 256 points of a block in
+// adjacent slots, so that the gather's lanes read adjacent records), with
+// tr_owner_s the owner of each slot; tbase, tdq, tqm the
 // tangents (tbase a copy of the base records with tangent q); vec_in /
 // vec_out [4][n] staging.
 struct AdjointBuffers {
@@ -201,7 +204,7 @@ struct AdjointBuffers {
   // edge_mx: per pair, bit r set where component r's extremum is qmax
   double *own_q, *own_dq, *own_qm, *edge, *lq, *ldq[2], *lqm, *ls;
   uint8_t *edge_mx;
-  int32_t *arg, *tr_off, *tr_edge, *tr_owner, *tr_pos;
+  int32_t *arg, *tr_off, *tr_edge, *tr_owner, *tr_pos, *tr_sbase, *tr_owner_s;
   double *tbase, *tdq[2], *tqm, *vec_in, *vec_out;
   double *tqs;     // tangent of q_derivatives' q sums
   double *lsum;    // per point sxx, syy, sxy of the least-squares solve
