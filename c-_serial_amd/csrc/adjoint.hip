@@ -550,10 +550,15 @@ __global__ __launch_bounds__(256) void k_flux_vjp_gather(
 // where dq_r is the pair's difference the pass sums (q_k - q_i, less
 // 0.5 rho . h_r for an inner pass) and h_r = dq_in_k,r - dq_in_i,r the
 // difference of the derivatives an inner pass reads (0 for q_derivatives,
-// dq_in = NULL).
-template <bool GEO>
+// dq_in = NULL).  TILED (GEO with dq_in = NULL only) first stages the
+// coordinates and q of the block's tile in LDS (64 B per entry) and reads
+// the neighbours' from there.
+template <bool GEO, bool TILED = false>
 __global__ __launch_bounds__(256) void k_ls_adj(int n, const int32_t *__restrict__ off,
                                                 const uint32_t *__restrict__ conn,
+                                                const uint16_t *__restrict__ conn16,
+                                                const int32_t *__restrict__ tile_off,
+                                                const int32_t *__restrict__ tile_idx,
                                                 const double *__restrict__ base,
                                                 const double *__restrict__ lsum,
                                                 const double *__restrict__ lo,
@@ -563,7 +568,14 @@ __global__ __launch_bounds__(256) void k_ls_adj(int n, const int32_t *__restrict
                                                 double *__restrict__ geo_edge,
                                                 double *__restrict__ lq, int with_dq,
                                                 double *__restrict__ ldq_in) {
-  const int i = xcd_block() * blockDim.x + threadIdx.x;
+  extern __shared__ double lds[];
+  constexpr int S = 8;  // x, y, q[4], pad
+  const int b = xcd_block();
+  if (TILED) {
+    load_tile<S>(lds, tile_off, tile_idx, b, base, B_STRIDE, 0, 3, 0);
+    __syncthreads();
+  }
+  const int i = b * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double x_i = base[(size_t)B_STRIDE * i + B_X], y_i = base[(size_t)B_STRIDE * i + B_Y];
   const int e0 = off[i], last = off[i + 1] - 1;
@@ -616,7 +628,8 @@ __global__ __launch_bounds__(256) void k_ls_adj(int n, const int32_t *__restrict
     // neighbour e + 1's base record and e + 2's conn entry load while e is
     // used (its dq_in is read when it is used)
     auto rec_of = [&](uint32_t w, double2 &p, double q[4]) {
-      const double *h = base + (size_t)B_STRIDE * (w & kIndexMask);
+      const double *h = TILED ? lds + S * conn_index(w, true)
+                              : base + (size_t)B_STRIDE * conn_index(w, false);
       p = reinterpret_cast<const double2 *>(h)[0];
       load4(h + B_Q, q);
     };
@@ -624,20 +637,19 @@ __global__ __launch_bounds__(256) void k_ls_adj(int n, const int32_t *__restrict
     double qk[4] = {0, 0, 0, 0};
     uint32_t w_c = 0, w_n = 0;
     if (e0 <= last) {
-      w_c = conn[e0];
+      w_c = conn_raw(conn, conn16, e0, TILED);
       rec_of(w_c, pk, qk);
-      w_n = conn[min(e0 + 1, last)];
+      w_n = conn_raw(conn, conn16, min(e0 + 1, last), TILED);
     }
     for (int e = e0; e <= last; e++) {
       double2 pk_n;
       double qk_n[4];
       rec_of(w_n, pk_n, qk_n);
-      const uint32_t w_nn = conn[min(e + 2, last)];
-      const int k = (int)(w_c & kIndexMask);
+      const uint32_t w_nn = conn_raw(conn, conn16, min(e + 2, last), TILED);
       const double delx = pk.x - x_i;
       const double dely = pk.y - y_i;
       double hxk[4] = {0, 0, 0, 0}, hyk[4] = {0, 0, 0, 0};
-      if (dq_in) load8(dq_in + (size_t)DQ_STRIDE * k, hxk, hyk);
+      if (!TILED && dq_in) load8(dq_in + (size_t)DQ_STRIDE * conn_index(w_c, false), hxk, hyk);
       double gx = 0.0, gy = 0.0;
       for (int r = 0; r < 4; r++) {
         const double hx = hxk[r] - hxi[r], hy = hyk[r] - hyi[r];
@@ -913,17 +925,32 @@ void launch_flux_vjp(const DevCloud &c, const AdjointBuffers &a, const double *b
                      a.own_dq, a.own_qm, a.lq, a.ldq[0], a.lqm);
 }
 
+size_t lds_bytes_ls_adj(int tile_max) { return (size_t)tile_max * 8 * sizeof(double); }
+
+bool prepare_ls_adj_tiles(const DevCloud &c, size_t lds_limit) {
+  if (!c.tiles || c.tile_max <= 0) return false;
+  const size_t L = lds_bytes_ls_adj(c.tile_max);
+  return L <= lds_limit &&
+         hipFuncSetAttribute((const void *)k_ls_adj<true, true>,
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)L) == hipSuccess;
+}
+
 void launch_ls_vjp(const DevCloud &c, const AdjointBuffers &a, const double *base,
                    const double *lo, double *ldq_in, bool extrema, const double *out,
                    const double *dq_in, double *geo_edge, hipStream_t s) {
-  if (geo_edge)
-    hipLaunchKernelGGL(k_ls_adj<true>, dim3(blocks_of(c.n)), dim3(256), 0, s, c.n, c.conn_off,
-                       c.conn, base, a.lsum, lo, a.ls, out, dq_in, geo_edge, a.lq,
-                       extrema ? 0 : 1, ldq_in);
+  if (geo_edge && a.ls_tiled && !dq_in)
+    hipLaunchKernelGGL((k_ls_adj<true, true>), dim3(blocks_for(c.n, c.tile_block)),
+                       dim3(c.tile_block), lds_bytes_ls_adj(c.tile_max), s, c.n, c.conn_off,
+                       c.conn, c.conn16, c.tile_off, c.tile_idx, base, a.lsum, lo, a.ls, out,
+                       nullptr, geo_edge, a.lq, extrema ? 0 : 1, ldq_in);
+  else if (geo_edge)
+    hipLaunchKernelGGL((k_ls_adj<true, false>), dim3(blocks_of(c.n)), dim3(256), 0, s, c.n,
+                       c.conn_off, c.conn, nullptr, nullptr, nullptr, base, a.lsum, lo, a.ls,
+                       out, dq_in, geo_edge, a.lq, extrema ? 0 : 1, ldq_in);
   else
-    hipLaunchKernelGGL(k_ls_adj<false>, dim3(blocks_of(c.n)), dim3(256), 0, s, c.n, c.conn_off,
-                       c.conn, base, a.lsum, lo, a.ls, nullptr, nullptr, nullptr, a.lq,
-                       extrema ? 0 : 1, ldq_in);
+    hipLaunchKernelGGL((k_ls_adj<false, false>), dim3(blocks_of(c.n)), dim3(256), 0, s, c.n,
+                       c.conn_off, c.conn, nullptr, nullptr, nullptr, base, a.lsum, lo, a.ls,
+                       nullptr, nullptr, nullptr, a.lq, extrema ? 0 : 1, ldq_in);
   hipLaunchKernelGGL(k_ls_adj_gather, dim3(blocks_of(c.n)), dim3(256), 0, s, c.n, a.tr_off,
                      a.tr_owner, reinterpret_cast<const double2 *>(c.xy), a.ls, extrema ? 0 : 1,
                      ldq_in, extrema ? a.arg : nullptr, a.lqm, a.lq);

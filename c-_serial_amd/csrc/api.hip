@@ -963,6 +963,7 @@ int prepare_adjoint(lskum_ctx *c) {
       hipDeviceGetAttribute(&lds_limit, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) ==
           hipSuccess &&
       prepare_flux_vjp_tiles(c->dc(), (size_t)lds_limit);
+  a.ls_tiled = lds_limit > 0 && prepare_ls_adj_tiles(c->dc(), (size_t)lds_limit);
   c->adj_ready = true;
   return LSKUM_OK;
 }

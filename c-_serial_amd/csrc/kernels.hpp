@@ -209,11 +209,13 @@ struct AdjointBuffers {
   double *tqs;     // tangent of q_derivatives' q sums
   double *lsum;    // per point sxx, syy, sxy of the least-squares solve
   bool vjp_tiled;  // the flux reverse stages neighbour records in LDS
+  bool ls_tiled;   // so does the least-squares reverse with geometry terms
 };
 // LDS bytes of the flux reverse's tile, and whether it can use the cloud's
 // tiles (they exist and fit in lds_limit; sets the kernels' LDS attribute)
 size_t lds_bytes_flux_vjp(int tile_max);
 bool prepare_flux_vjp_tiles(const DevCloud &c, size_t lds_limit);
+bool prepare_ls_adj_tiles(const DevCloud &c, size_t lds_limit);
 // Geometry adjoints the reverse kernels accumulate (+=), device order:
 // edge [n_edges][2] d/d(delx, dely) of each (point, neighbour) pair, nrm
 // [n][2] d/d(nx, ny), coef [n][8] d/d of the sub-stencil coefficients,
