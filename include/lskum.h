@@ -128,10 +128,13 @@ void lskum_cloud_free(lskum_cloud_h *c);
 /* ------------------------------------------------------------ parameters */
 #define LSKUM_FP64 0  /* every operation in FP64, in the reference's order  */
 /* Mixed: neighbour q, dq, qmax/qmin and eps^2 stored and gathered in FP32;
- * limiter, q~ -> primitive and split/quadrant fluxes in FP32; coordinates,
- * least-squares sums/determinants/solves, flux accumulation, the state,
- * SSPRK update, local time step and all reductions in FP64.  q, dq and qm
- * read back through lskum_get_array are the FP32 values widened. */
+ * the limiter, q~ -> primitive, the split/quadrant fluxes and their
+ * per-point sums, the least-squares right-hand sums and each neighbour's
+ * local-step term in FP32; coordinates, the least-squares geometry (per-point
+ * solve coefficients formed once in FP64 on the host, then rounded), the
+ * state, the SSPRK update, the stored local time step and all reductions in
+ * FP64.  q, dq and qm read back through lskum_get_array are the FP32 values
+ * widened. */
 #define LSKUM_MIXED 1
 
 #define LSKUM_REORDER_NONE 0
