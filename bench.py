@@ -302,7 +302,7 @@ def run_mixed10m(args):
         "n_gpus": 1, "steps": args.steps, "warmup": 0,
         "ms_per_step": 1e3 * m["elapsed"] / args.steps, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32 gathers/fluxes/limiter, f64 solves/state/reductions", "data": "synthetic",
+        "dtype": "f32 gathers/fluxes/limiter/least-squares sums, f64 coordinates/least-squares geometry/state/reductions", "data": "synthetic",
         "config": {"workload": "naca0012_10m_k24", "points": n, "k": 24, "inner_iterations": 0,
                    "mach": 0.63, "aoa_deg": 2.0, "cfl": 0.2, "vl_const": 5.0,
                    "graph_iters": 50},
@@ -536,7 +536,7 @@ def main():
         out["mixed"] = {
             "value": m["value"], "ms_per_step": m["ms_per_step"],
             "speedup_vs_fp64": m["value"] / f["value"],
-            "dtype": "f32 gathers/fluxes/limiter, f64 solves/state/reductions",
+            "dtype": "f32 gathers/fluxes/limiter/least-squares sums, f64 coordinates/least-squares geometry/state/reductions",
             "lds_tile_mask": m["tiles"],
             "flux_roofline": {"achieved": ach_m, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                               "frac": ach_m / HBM_PEAK_GBPS, "avg_ms": m["flux_ms"],
