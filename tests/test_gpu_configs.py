@@ -2,8 +2,9 @@
 
   - 10M points (naca0012_10m_k24: wall 4,000, 1,000 x 10,000, k = 24), M 0.63,
     AoA 2 deg, no inner passes: the flux residual of the free stream is
-    exactly zero in both forms, and the mixed form (FP32 gathers, fluxes and limiter; FP64
-    solves, state and reductions) stays within 1e-6 of the FP64 form in Cl
+    exactly zero in both forms, and the mixed form (FP32 gathers, fluxes,
+    limiter and least-squares sums; FP64 coordinates, least-squares
+    geometry, state and reductions) stays within 1e-6 of the FP64 form in Cl
     and Cd over 100 iterations from the same start;
   - the many-case workload (64 flow conditions over 8 NACA 4-digit clouds of
     50k-800k points, k = 20): the layout lskum_autotune picks, 50 iterations
