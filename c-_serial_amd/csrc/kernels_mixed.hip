@@ -73,37 +73,36 @@ __device__ __forceinline__ void tile_gather(uint4 *d0, const uint4 *__restrict__
   }
 }
 
-// The flux's tile: per entry q, dq/dx, dq/dy, qmax, qmin as five 16-byte
-// chunks (dst[5r + ch]) and eps2 (eps[r]), in one round of loads.
+// The flux's tile: per entry q, dq/dx, dq/dy as three 16-byte chunks
+// (dst[3r + ch]) and eps2 (eps[r]), in one round of loads.
 __device__ __forceinline__ void tile_copy_flux(uint4 *dst, float *eps, const uint4 *__restrict__ q,
                                                const uint4 *__restrict__ dq,
-                                               const uint4 *__restrict__ qm,
                                                const float *__restrict__ eps32,
                                                const int32_t *__restrict__ idx, int count) {
-  constexpr int U = 12;  // a 605-entry tile's 3,630 loads in one round
-  const int total = count * 6;
+  constexpr int U = 10;  // a 605-entry tile's 2,420 loads in one round
+  const int total = count * 4;
   for (int base = threadIdx.x; base < total; base += U * blockDim.x) {
     uint4 v[U];
 #pragma unroll
     for (int u = 0; u < U; u++) {
       int t = base + u * blockDim.x;
       t = t < total ? t : total - 1;
-      const int r = t / 6, ch = t - 6 * r;
+      const int r = t >> 2, ch = t & 3;
       const size_t g = (size_t)idx[r];
-      if (ch == 5)
+      if (ch == 3)
         v[u] = make_uint4(__float_as_uint(eps32[g]), 0u, 0u, 0u);
       else
-        v[u] = ch == 0 ? q[g] : ch < 3 ? dq[2 * g + (ch - 1)] : qm[2 * g + (ch - 3)];
+        v[u] = ch == 0 ? q[g] : dq[2 * g + (ch - 1)];
     }
 #pragma unroll
     for (int u = 0; u < U; u++) {
       const int t = base + u * blockDim.x;
       if (t < total) {
-        const int r = t / 6, ch = t - 6 * r;
-        if (ch == 5)
+        const int r = t >> 2, ch = t & 3;
+        if (ch == 3)
           eps[r] = __uint_as_float(v[u].x);
         else
-          dst[5 * r + ch] = v[u];
+          dst[3 * r + ch] = v[u];
       }
     }
   }
@@ -407,7 +406,7 @@ __global__ __launch_bounds__(256) void k_q_inner_m(
 constexpr int COEF_STRIDE = 8;
 
 template <bool TILED, bool POW>
-__global__ __launch_bounds__(256) void k_flux_m(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(POW ? 1 : 4))) void k_flux_m(
     int n, const int32_t *__restrict__ off, const int32_t *__restrict__ slice_off,
     const uint32_t *__restrict__ conn,
     const uint16_t *__restrict__ conn16, const int32_t *__restrict__ tile_off,
@@ -417,15 +416,15 @@ __global__ __launch_bounds__(256) void k_flux_m(
     const double *__restrict__ coef, const double *__restrict__ nxa,
     const double *__restrict__ nya, const uint8_t *__restrict__ flag,
     const double *__restrict__ delta, double power, double *__restrict__ fres) {
-  // LDS tile: q, dq, qm (80 B) and eps2 (4 B) per entry; the FP64
-  // coordinates (16 B) are read through L2.  cap entries (LDS for cap
-  // entries per block) let a CU hold the three blocks the kernel's
+  // LDS tile: q, dq (48 B) and eps2 (4 B) per entry; the FP64 coordinates
+  // (16 B) and qmax/qmin (32 B) are read through L2.  cap entries (LDS for
+  // cap entries per block) let a CU hold the four blocks the kernel's
   // registers allow.  Tiled, a launch takes the blocks whose tile has at
   // most cap entries; untiled with cap > 0, the others; untiled with cap =
   // 0, every block.
   extern __shared__ uint4 lds4[];
   const float4 *lt = reinterpret_cast<const float4 *>(lds4);
-  float *le = reinterpret_cast<float *>(lds4 + 5 * cap);
+  float *le = reinterpret_cast<float *>(lds4 + 3 * cap);
   const int b = xcd_block();
   if (TILED || cap > 0) {
     const int tn = tile_off[b + 1] - tile_off[b];
@@ -435,8 +434,7 @@ __global__ __launch_bounds__(256) void k_flux_m(
     const int t0 = tile_off[b], tn = tile_off[b + 1] - t0;
     const int32_t *idx = tile_idx + t0;
     tile_copy_flux(lds4, le, reinterpret_cast<const uint4 *>(q32),
-                   reinterpret_cast<const uint4 *>(dq32), reinterpret_cast<const uint4 *>(qm32),
-                   eps32, idx, tn);
+                   reinterpret_cast<const uint4 *>(dq32), eps32, idx, tn);
     __syncthreads();
   }
   const int i = b * blockDim.x + threadIdx.x;
@@ -445,12 +443,12 @@ __global__ __launch_bounds__(256) void k_flux_m(
   float qi[4], dxi[4], dyi[4], dmaxi[4], dmini[4];
   float eps2_i;
   if (TILED) {
-    const int li = 5 * (int)threadIdx.x;
+    const int li = 3 * (int)threadIdx.x;
     unpack4(lt[li], qi);
     unpack4(lt[li + 1], dxi);
     unpack4(lt[li + 2], dyi);
-    unpack4(lt[li + 3], dmaxi);
-    unpack4(lt[li + 4], dmini);
+    unpack4(qm32[2 * i], dmaxi);
+    unpack4(qm32[2 * i + 1], dmini);
     eps2_i = le[threadIdx.x];
   } else {
     unpack4(q32[i], qi);
@@ -494,8 +492,8 @@ __global__ __launch_bounds__(256) void k_flux_m(
     const int kg = conn_index(wg, false);
     if (TILED) {
       const int kl = conn_index(wl, true);
-      const float4 *t = lt + 5 * kl;
-      return {xy[kg], t[0], t[1], t[2], t[3], t[4], le[kl]};
+      const float4 *t = lt + 3 * kl;
+      return {xy[kg], t[0], t[1], t[2], qm32[2 * kg], qm32[2 * kg + 1], le[kl]};
     }
     return {xy[kg], q32[kg], dq32[2 * kg], dq32[2 * kg + 1], qm32[2 * kg], qm32[2 * kg + 1],
             eps32[kg]};
@@ -620,7 +618,7 @@ __global__ void k_kat_fluxes_m(int n, const double *__restrict__ in, double *__r
 size_t lds_bytes_qderiv_m(int tile_max) { return 32 * (size_t)tile_max; }
 
 size_t lds_bytes_qinner_m(int tile_max) { return 64 * (size_t)tile_max; }
-size_t lds_bytes_flux_m(int tile_max) { return 84 * (size_t)tile_max; }
+size_t lds_bytes_flux_m(int tile_max) { return 52 * (size_t)tile_max; }
 
 int flux_tile_cap_m(int block, int tile_max, size_t lds_limit, size_t lds_per_cu) {
   int nb_regs = 0;
