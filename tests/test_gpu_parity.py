@@ -320,20 +320,30 @@ def test_restart_resumes_bit_for_bit(tmp_path):
 
 @pytest.mark.parametrize("precision", [L.FP64, L.MIXED])
 def test_flux_tile_cap_reads_the_rest_through_l2(precision):
-    # with 64 neighbours per point the largest tiles (700 entries) exceed the
-    # entries the flux stages per block; the entries beyond the cap are read
-    # through L2, and the states equal those of the untiled layout bit for bit
-    cloud = lskum.Cloud.generate(naca="0012", n_wall=160, n_radial=64, n_circ=150,
-                                 stretch=1.08, k=64, radius_factor=6.0, seed=1)
-    runs = []
-    for tiles in (L.TILES_AUTO, 0):
-        s = Solver(cloud, mach=0.63, aoa=2.0, cfl=0.2, vl_const=5.0, inner_iterations=1,
-                   precision=precision, lds_tiles=tiles)
-        print("case: flux tile cap, precision", precision, "layout", s.layout(),
-              "flux tile", s.flux_tile(), flush=True)
-        runs.append((s.layout()[1], s.flux_tile()))
-        s.q_lskum(10)
-        runs[-1] += (s.prim, s.flux_res)
+    # with enough neighbours per point the largest tiles exceed the entries
+    # the flux stages per block (64 neighbours: 700-entry tiles, beyond the
+    # FP64 flux's cap; the mixed flux's smaller records need more); the
+    # blocks beyond the cap read their neighbours through L2, and the states
+    # equal those of the untiled layout bit for bit
+    for k in (64, 96, 128):
+        cloud = lskum.Cloud.generate(naca="0012", n_wall=160, n_radial=64, n_circ=150,
+                                     stretch=1.08, k=k, radius_factor=6.0, seed=1)
+        runs = []
+        for tiles in (L.TILES_AUTO, 0):
+            s = Solver(cloud, mach=0.63, aoa=2.0, cfl=0.2, vl_const=5.0, inner_iterations=1,
+                       precision=precision, lds_tiles=tiles)
+            print("case: flux tile cap, precision", precision, "k", k, "layout", s.layout(),
+                  "flux tile", s.flux_tile(), flush=True)
+            runs.append((s.layout()[1], s.flux_tile()))
+            if tiles == L.TILES_AUTO and not 0 < runs[0][1] < runs[0][0]:
+                s.close()
+                break  # every tile fits: more neighbours
+            s.q_lskum(10)
+            runs[-1] += (s.prim, s.flux_res)
+            s.close()
+        if len(runs) == 2:
+            break
+    assert len(runs) == 2, "no neighbour count made a tile exceed the flux's cap"
     (tmax, cap), (_, cap0) = runs[0][:2], runs[1][:2]
     assert 0 < cap < tmax and cap0 == 0
     for a, b in zip(runs[0][2:], runs[1][2:]):
