@@ -154,11 +154,12 @@ int do_q_derivatives(lskum_ctx *c, int n_later, bool keep_sums, bool with_delta)
   c->cur = n_later % 2;
   {
     KTimer t(c, LSKUM_K_QDERIV);
-    if (with_delta)
+    if (c->mixed)
+      launch_q_derivatives_m(c->dc(), c->ms(), c->dq32[c->cur], c->prm.power, c->stream,
+                             c->base, c->prm.cfl, with_delta ? c->delta : nullptr);
+    else if (with_delta)
       launch_q_derivatives_delta(c->dc(), c->base, c->dqb[c->cur], c->qm,
                                  keep_sums ? c->qsum : nullptr, c->prm.cfl, c->delta, c->stream);
-    else if (c->mixed)
-      launch_q_derivatives_m(c->dc(), c->ms(), c->dq32[c->cur], c->prm.power, c->stream);
     else
       launch_q_derivatives(c->dc(), c->base, c->dqb[c->cur], c->qm, keep_sums ? c->qsum : nullptr,
                            c->prm.power, c->stream);
@@ -216,7 +217,7 @@ int do_state_update(lskum_ctx *c, int rk, bool with_hist) {
 // pipelined kernel allows it the two share one pass.
 int run_iteration(lskum_ctx *c) {
   const int inner = c->prm.inner_iterations;
-  const bool fused_delta = c->qdd_chunks > 0;
+  const bool fused_delta = c->mixed ? c->prm.power == 0.0 : c->qdd_chunks > 0;
   int rc = do_snapshot(c);
   if (!rc && !fused_delta) rc = do_func_delta(c);
   for (int rk = 1; rk <= 4 && !rc; rk++) {

@@ -178,8 +178,11 @@ void launch_q_variables_m(const DevCloud &c, const double *prim, float *q32, hip
 // storage as launch_func_delta's)
 void launch_func_delta_m(const DevCloud &c, const double *base, double cfl, double *delta,
                          hipStream_t s);
+// with delta set (power 0 only): func_delta's minimum in the same pass,
+// the same bits as launch_func_delta_m
 void launch_q_derivatives_m(const DevCloud &c, const MixedState &m, float *dq, double power,
-                            hipStream_t s);
+                            hipStream_t s, const double *base = nullptr, double cfl = 0.0,
+                            double *delta = nullptr);
 void launch_q_inner_m(const DevCloud &c, const MixedState &m, const float *src, float *dst,
                       double power, hipStream_t s);
 void launch_flux_m(const DevCloud &c, const MixedState &m, const float *dq, const double *delta,

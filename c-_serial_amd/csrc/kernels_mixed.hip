@@ -121,6 +121,18 @@ __global__ __launch_bounds__(256) void k_q_variables_m(int n, const double *__re
 }
 
 // ------------------------------------------------------------ local dt
+// One neighbour's step cfl |x_k - x_i| / speed_k in FP32 (the offset formed
+// in FP64, then rounded) and the running minimum with the reference's
+// strict comparison; k_func_delta_m and stage 1's q_derivatives share it,
+// so the two give the same bits.
+__device__ __forceinline__ void delta_step_m(double2 xy, double2 xyk, float speed, float cfl,
+                                             float &min_delt) {
+  const float fx = (float)(xyk.x - xy.x), fy = (float)(xyk.y - xy.y);
+  const float dist = sqrtf(fmaf(fx, fx, fy * fy));
+  const float delta_t = cfl * (dist * __builtin_amdgcn_rcpf(speed));
+  if (min_delt > delta_t) min_delt = delta_t;
+}
+
 // state_update_mod.h:448-496 for the mixed form: the neighbours' coordinates
 // and speeds gathered as k_func_delta does (FP64 records; the offsets
 // x_k - x_i formed in FP64 before they are rounded), the step
@@ -171,10 +183,7 @@ __global__ __launch_bounds__(256) void k_func_delta_m(
     double speed_n;
     fetch(w_n, xy_n, speed_n);
     w_n = L.raw(min(e + 2, last), TILED);
-    const float fx = (float)(xyk.x - xy.x), fy = (float)(xyk.y - xy.y);
-    const float dist = sqrtf(fmaf(fx, fx, fy * fy));
-    const float delta_t = cfl * (dist * __builtin_amdgcn_rcpf((float)speed));
-    if (min_delt > delta_t) min_delt = delta_t;
+    delta_step_m(xy, xyk, (float)speed, cfl, min_delt);
     xyk = xy_n;
     speed = speed_n;
   }
@@ -242,23 +251,31 @@ __device__ __forceinline__ void qdm_finish(float4 ls, const float sdx[4], const 
   qm_i[1] = make_float4(qmin[0], qmin[1], qmin[2], qmin[3]);
 }
 
-template <bool TILED, bool POW>
+// DELTA (stage 1 of an iteration) also takes func_delta's minimum over the
+// same neighbours, from the speeds in the base records (tiled: staged as
+// floats after the tile's q).
+template <bool TILED, bool POW, bool DELTA = false>
 __global__ __launch_bounds__(256) void k_q_derivatives_m(
     int n, const int32_t *__restrict__ off, const int32_t *__restrict__ slice_off,
     const uint32_t *__restrict__ conn,
     const uint16_t *__restrict__ conn16, const int32_t *__restrict__ tile_off,
     const int32_t *__restrict__ tile_idx, int tile_max, const double2 *__restrict__ xy,
     const float4 *__restrict__ q32, const float4 *__restrict__ ls, float4 *__restrict__ dq_out,
-    float4 *__restrict__ qm_out, double power) {
+    float4 *__restrict__ qm_out, double power, const double *__restrict__ base, float cfl,
+    double *__restrict__ delta) {
   extern __shared__ uint4 lds4[];
   double2 *lxy = reinterpret_cast<double2 *>(lds4);
   float4 *lq = reinterpret_cast<float4 *>(lds4 + tile_max);
+  float *lsp = reinterpret_cast<float *>(lds4 + 2 * tile_max);
   const int b = xcd_block();
   if (TILED) {
     const int t0 = tile_off[b], tn = tile_off[b + 1] - t0;
     tile_gather<1, 1, 0>(lds4, reinterpret_cast<const uint4 *>(xy), lds4 + tile_max,
                          reinterpret_cast<const uint4 *>(q32), nullptr, nullptr,
                          tile_idx + t0, tn);
+    if (DELTA)
+      for (int t = threadIdx.x; t < tn; t += blockDim.x)
+        lsp[t] = (float)base[(size_t)B_STRIDE * tile_idx[t0 + t] + B_SPEED];
     __syncthreads();
   }
   const int i = b * blockDim.x + threadIdx.x;
@@ -275,27 +292,36 @@ __global__ __launch_bounds__(256) void k_q_derivatives_m(
   // neighbour e + 1's record and e + 2's conn entry load while e is used
   const Slice L = slice_of(slice_off, conn, conn16, b);
   const int e0 = 0, last = off[i + 1] - off[i] - 1;
+  auto speed_of = [&](int k) {
+    return TILED ? lsp[k] : (float)base[(size_t)B_STRIDE * k + B_SPEED];
+  };
   double2 pk = {0.0, 0.0};
   float4 qk4 = {0.0f, 0.0f, 0.0f, 0.0f};
+  float spk = 1.0f, min_delt = 1.0f;
   uint32_t w_n = 0;
   if (e0 <= last) {
     const int k0 = conn_index(L.raw(e0, TILED), TILED);
     pk = gxy[k0];
     qk4 = gq[k0];
+    if (DELTA) spk = speed_of(k0);
     w_n = L.raw(min(e0 + 1, last), TILED);
   }
   for (int e = e0; e <= last; e++) {
     const int kn = conn_index(w_n, TILED);
     const double2 pk_n = gxy[kn];
     const float4 qk4_n = gq[kn];
+    const float spk_n = DELTA ? speed_of(kn) : 1.0f;
     w_n = L.raw(min(e + 2, last), TILED);
     float qk[4];
     unpack4(qk4, qk);
     qdm_term<POW>(pi, qi, pk, qk, power, qmax, qmin, sdx, sdy);
+    if (DELTA) delta_step_m(pi, pk, spk, cfl, min_delt);
     pk = pk_n;
     qk4 = qk4_n;
+    spk = spk_n;
   }
   qdm_finish(lsi, sdx, sdy, qmax, qmin, dq_out + 2 * (size_t)i, qm_out + 2 * (size_t)i);
+  if (DELTA) delta[i] = (double)min_delt;
 }
 
 // ------------------------------------------------------------ inner passes
@@ -616,6 +642,8 @@ __global__ void k_kat_fluxes_m(int n, const double *__restrict__ in, double *__r
 }  // namespace
 
 size_t lds_bytes_qderiv_m(int tile_max) { return 32 * (size_t)tile_max; }
+// with func_delta: the tile's speeds after its coordinates and q
+size_t lds_bytes_qderiv_delta_m(int tile_max) { return 36 * (size_t)tile_max; }
 
 size_t lds_bytes_qinner_m(int tile_max) { return 64 * (size_t)tile_max; }
 size_t lds_bytes_flux_m(int tile_max) { return 52 * (size_t)tile_max; }
@@ -643,6 +671,8 @@ unsigned prepare_tiled_m(unsigned tiles, int tile_max, int flux_cap, size_t lds_
       {TILE_DELTA, (const void *)k_func_delta_m<true>, lds_bytes_delta(tile_max)},
       {TILE_QDERIV, (const void *)k_q_derivatives_m<true, false>, lds_bytes_qderiv_m(tile_max)},
       {TILE_QDERIV, (const void *)k_q_derivatives_m<true, true>, lds_bytes_qderiv_m(tile_max)},
+      {TILE_QDERIV, (const void *)k_q_derivatives_m<true, false, true>,
+       lds_bytes_qderiv_delta_m(tile_max)},
       {TILE_QINNER, (const void *)k_q_inner_m<true, false>, lds_bytes_qinner_m(tile_max)},
       {TILE_QINNER, (const void *)k_q_inner_m<true, true>, lds_bytes_qinner_m(tile_max)},
       {TILE_FLUX, (const void *)k_flux_m<true, false>, lds_bytes_flux_m(flux_cap)},
@@ -678,15 +708,21 @@ void launch_func_delta_m(const DevCloud &c, const double *base, double cfl, doub
 }
 
 void launch_q_derivatives_m(const DevCloud &c, const MixedState &m, float *dq, double power,
-                            hipStream_t s) {
+                            hipStream_t s, const double *base, double cfl, double *delta) {
   dim3 g(blocks_for(c.n, c.tile_block)), b(c.tile_block);
   const bool t = (c.tiles & TILE_QDERIV) != 0;
   const size_t L = t ? lds_bytes_qderiv_m(c.tile_max) : 0;
 #define QD_ARGS                                                                           \
   c.n, TILE_ARGS_M, reinterpret_cast<const double2 *>(m.xy),                              \
       reinterpret_cast<const float4 *>(m.q32), reinterpret_cast<const float4 *>(m.ls32),   \
-      reinterpret_cast<float4 *>(dq), reinterpret_cast<float4 *>(m.qm32), power
-  if (t && power == 0.0)
+      reinterpret_cast<float4 *>(dq), reinterpret_cast<float4 *>(m.qm32), power, base,     \
+      (float)cfl, delta
+  if (delta && t)  // power 0 (the caller's condition for fusing)
+    hipLaunchKernelGGL((k_q_derivatives_m<true, false, true>), g, b,
+                       lds_bytes_qderiv_delta_m(c.tile_max), s, QD_ARGS);
+  else if (delta)
+    hipLaunchKernelGGL((k_q_derivatives_m<false, false, true>), g, b, 0, s, QD_ARGS);
+  else if (t && power == 0.0)
     hipLaunchKernelGGL((k_q_derivatives_m<true, false>), g, b, L, s, QD_ARGS);
   else if (t)
     hipLaunchKernelGGL((k_q_derivatives_m<true, true>), g, b, L, s, QD_ARGS);

@@ -319,6 +319,29 @@ def test_restart_resumes_bit_for_bit(tmp_path):
 
 
 @pytest.mark.parametrize("precision", [L.FP64, L.MIXED])
+def test_iteration_delta_equals_the_stand_alone_pass(precision):
+    # lskum_iterate takes func_delta's minimum inside stage 1's
+    # q_derivatives (where it can); the delta an iteration leaves is the
+    # stand-alone pass's on the iteration's starting state, bit for bit
+    g = golden("small_m063.npz")
+    cloud = lskum.workload("naca0012_1480_k20")
+    params = (0.63, 2.0, 0.2, 5.0, 0.0, 2)
+    prim = g["rk1_prim_in"]
+    print("case: iteration delta vs func_delta, precision", precision, flush=True)
+    a = _solver(cloud, params, precision=precision)
+    a.set_state(prim)
+    a.q_lskum(1)
+    da = a.delta
+    a.close()
+    b = _solver(cloud, params, precision=precision)
+    b.set_state(prim)
+    b.func_delta()
+    db = b.delta
+    b.close()
+    np.testing.assert_array_equal(da, db)
+
+
+@pytest.mark.parametrize("precision", [L.FP64, L.MIXED])
 def test_flux_tile_cap_reads_the_rest_through_l2(precision):
     # with enough neighbours per point the largest tiles exceed the entries
     # the flux stages per block (64 neighbours: 700-entry tiles, beyond the
