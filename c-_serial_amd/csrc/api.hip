@@ -709,8 +709,8 @@ int create_impl(const lskum_cloud *clouds, int32_t n_clouds, const lskum_case *c
     const int tm = c->tile_max, sw = c->slice_width;
     // the mixed form's tiles (FP32 records) let more blocks share a CU
     // than the pipelined kernels' registers and list slices do: pipelined
-    // mixed kernels, and func_delta folded into its q_derivatives, were
-    // slower, so it keeps its own kernels
+    // mixed kernels were slower, so it keeps its own (whose stage-1
+    // q_derivatives takes func_delta's minimum itself)
     if (!c->mixed) {
       if (c->tiles & LSKUM_TILE_QINNER)
         c->qi_chunks = prepare_qinner_pf(TB, tm, sw, nblk, n_cu, lim, &c->qi_grid);
