@@ -551,8 +551,11 @@ __device__ __forceinline__ void qd_store_lds(qi_chunk *l2, const qi_chunk (&v)[C
   pf_store_conn<NC>(l2, v, C * E, q, t, TB, lds_conn2);
 }
 
+// Three waves per SIMD (168 VGPRs) fit without spills except for the
+// layouts with func_delta and more than three entry or list slots a thread.
 template <int E, int NC, bool DELTA>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DELTA ? 2 : 3, DELTA ? 2 : 3))) void k_q_derivatives_pf(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
+    DELTA && (E > 3 || NC > 3) ? 2 : 3, DELTA && (E > 3 || NC > 3) ? 2 : 3))) void k_q_derivatives_pf(
     int n, int nblk, int lds_conn, const int32_t *__restrict__ off,
     const int32_t *__restrict__ slice_off, const uint16_t *__restrict__ conn16,
     const int32_t *__restrict__ tile_off, const int32_t *__restrict__ tile_idx,
